@@ -1,0 +1,495 @@
+"""CPU oracle for MGmol's orbital-iteration hot path.
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker: tests/, the
+``smoke()`` entry of ``__graft_entry__`` and the ``cpu_baseline`` leg of
+``bench.py`` may import it; the product package ``mgmol_amd`` never does.
+
+The arithmetic kernels live in ``oracle.cpp`` (ctypes, ``liboracle.so``),
+each citing the reference file:line it restates.  Dense FP64 linear algebra
+(Cholesky, inverse, theta, traces, Ritz) uses numpy/scipy LAPACK, which is the
+same BLAS/LAPACK family the reference calls (``DSYRK``/``DGEMM``/``DPOTRF``/
+``DPOTRI``, ``src/linear_algebra/mputils.cc:294-305``, ``src/GramMatrix.cc``).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+F32, F64 = np.float32, np.float64
+
+
+class Grid(C.Structure):
+    _fields_ = [("dim", C.c_int * 3), ("ghosts", C.c_int), ("h", C.c_double * 3)]
+
+    @staticmethod
+    def make(dim, ghosts, h):
+        g = Grid()
+        for i in range(3):
+            g.dim[i] = int(dim[i])
+            g.h[i] = float(h[i] if np.ndim(h) else h)
+        g.ghosts = int(ghosts)
+        return g
+
+    @property
+    def shape_g(self):
+        ng = self.ghosts
+        return tuple(self.dim[i] + 2 * ng for i in range(3))
+
+    @property
+    def sizeg(self):
+        a, b, c = self.shape_g
+        return a * b * c
+
+    @property
+    def size(self):
+        return self.dim[0] * self.dim[1] * self.dim[2]
+
+    def coarse(self):
+        return Grid.make([d // 2 for d in self.dim], self.ghosts, [2 * x for x in self.h])
+
+
+class Boxes(C.Structure):
+    _fields_ = [
+        ("nbox", C.c_int),
+        ("gdim", C.c_int * 3),
+        ("h", C.c_double * 3),
+        ("origin", C.POINTER(C.c_int)),
+        ("edge", C.POINTER(C.c_int)),
+    ]
+
+    @staticmethod
+    def make(origin, edge, gdim, h):
+        b = Boxes()
+        b._origin = np.ascontiguousarray(np.asarray(origin, dtype=np.int32).reshape(-1, 3))
+        b._edge = np.ascontiguousarray(np.asarray(edge, dtype=np.int32).reshape(-1))
+        b.nbox = len(b._edge)
+        for i in range(3):
+            b.gdim[i] = int(gdim[i])
+            b.h[i] = float(h[i] if np.ndim(h) else h)
+        b.origin = b._origin.ctypes.data_as(C.POINTER(C.c_int))
+        b.edge = b._edge.ctypes.data_as(C.POINTER(C.c_int))
+        return b
+
+    def size(self, level=0):
+        return int(lib().or_b_size(C.byref(self), level))
+
+    def offsets(self, level=0):
+        e = self._edge.astype(np.int64) >> level
+        off = np.zeros(len(e) + 1, dtype=np.int64)
+        off[1:] = np.cumsum(e ** 3)
+        return off
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        I, Lg, D = C.c_int, C.c_long, C.c_double
+        GP, BP = C.POINTER(Grid), C.POINTER(Boxes)
+        IP = C.POINTER(C.c_int)
+        sig = {
+            "or_g_trade_%s": (GP, P, Lg, IP),
+            "or_g_lap_%s": (GP, I, P, P, Lg),
+            "or_g_rhs_mehr_%s": (GP, P, P, I, Lg),
+            "or_g_lap4_withpot_%s": (GP, P, P, P, Lg, Lg),
+            "or_g_pointwise_%s": (GP, P, P, P, Lg),
+            "or_axpy_%s": (Lg, D, P, P),
+            "or_g_jacobi_%s": (GP, I, P, P, P, D, Lg, IP),
+            "or_g_restrict_%s": (GP, P, GP, P, Lg),
+            "or_g_extend_%s": (GP, P, GP, P, Lg),
+            "or_g_mask_%s": (GP, P, P, Lg),
+            "or_g_mg_%s": (GP, I, I, P, P, Lg, IP, P),
+            "or_g_precond_%s": (GP, I, I, D, P, P, Lg, Lg, IP, P),
+            "or_b_lap_%s": (BP, I, I, P, P),
+            "or_b_hloc_%s": (BP, P, P, P),
+            "or_b_jacobi_%s": (BP, I, I, P, P, P, D),
+            "or_b_restrict_%s": (BP, I, P, P),
+            "or_b_prolong_%s": (BP, I, P, P),
+            "or_b_mg_%s": (BP, I, I, P, P),
+            "or_b_precond_%s": (BP, I, I, D, P, P),
+        }
+        for name, args in sig.items():
+            for suf in ("f32", "f64"):
+                fn = getattr(L, name % suf)
+                fn.argtypes = list(args)
+                fn.restype = I
+        L.or_g_hloc_f64.argtypes = [GP, I, P, P, P, Lg, Lg, IP]
+        L.or_g_hloc_f64.restype = I
+        L.or_diag_el.argtypes = [I, C.POINTER(D)]
+        L.or_diag_el.restype = D
+        L.or_jacobi_factor.argtypes = [I, C.POINTER(D)]
+        L.or_jacobi_factor.restype = D
+        L.or_gamma.argtypes = [C.POINTER(D), I, D, D]
+        L.or_gamma.restype = D
+        L.or_local_gram.argtypes = [P, Lg, Lg, I, I, D, P]
+        L.or_local_product.argtypes = [P, P, Lg, Lg, I, I, D, P]
+        L.or_local_to_global.argtypes = [P, P, I, I, I, D, P]
+        L.or_residual.argtypes = [P, P, P, Lg, Lg, I, P]
+        L.or_b_size.argtypes = [BP, I]
+        L.or_b_size.restype = Lg
+        L.or_b_offset.argtypes = [BP, I, I]
+        L.or_b_offset.restype = Lg
+        L.or_b_pairs.argtypes = [BP, Lg, P, P]
+        L.or_b_pairs.restype = Lg
+        L.or_b_gram_h.argtypes = [BP, P, P, P, P]
+        L.or_b_residual.argtypes = [BP, P, P, P, P]
+        L.or_b_diag_dots.argtypes = [BP, P, P]
+        L.or_b_axpy.argtypes = [Lg, D, P, P]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _suf(a):
+    if a.dtype == F32:
+        return "f32"
+    if a.dtype == F64:
+        return "f64"
+    raise TypeError(a.dtype)
+
+
+def _bc(bc):
+    return (C.c_int * 3)(*[int(x) for x in bc])
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise ValueError("oracle %s failed (rc=%d)" % (what, rc))
+
+
+def _masks(masks):
+    if masks is None:
+        return None, []
+    keep = [None if m is None else np.ascontiguousarray(m, dtype=np.uint8) for m in masks]
+    arr = (C.c_void_p * len(keep))(*[None if m is None else m.ctypes.data for m in keep])
+    return arr, keep
+
+
+# ----------------------------------------------------------------- grid family
+def empty_block(g, nfunc, dtype=F64):
+    return np.zeros((nfunc,) + g.shape_g, dtype=dtype)
+
+
+def embed(g, interior, dtype=None):
+    """interior (nfunc, dx, dy, dz) -> ghosted block with zero ghosts."""
+    interior = np.asarray(interior)
+    dtype = dtype or interior.dtype
+    nf = interior.shape[0]
+    out = empty_block(g, nf, dtype)
+    ng = g.ghosts
+    out[:, ng:ng + g.dim[0], ng:ng + g.dim[1], ng:ng + g.dim[2]] = interior
+    return out
+
+
+def interior(g, block):
+    ng = g.ghosts
+    return block[..., ng:ng + g.dim[0], ng:ng + g.dim[1], ng:ng + g.dim[2]]
+
+
+def trade(g, u, bc=(1, 1, 1)):
+    nf = u.shape[0]
+    _chk(getattr(lib(), "or_g_trade_" + _suf(u))(C.byref(g), _p(u), nf, _bc(bc)), "trade")
+    return u
+
+
+def lap(g, lap_type, u):
+    out = np.zeros_like(u)
+    _chk(getattr(lib(), "or_g_lap_" + _suf(u))(C.byref(g), lap_type, _p(u), _p(out), u.shape[0]), "lap")
+    return out
+
+
+def rhs_mehr(g, u, rhs_ghosts=None):
+    rg = g.ghosts if rhs_ghosts is None else rhs_ghosts
+    sh = tuple(d + 2 * rg for d in g.dim)
+    out = np.zeros((u.shape[0],) + sh, dtype=u.dtype)
+    _chk(getattr(lib(), "or_g_rhs_mehr_" + _suf(u))(C.byref(g), _p(u), _p(out), rg, u.shape[0]), "rhs")
+    return out
+
+
+def lap4_withpot(g, u, pot):
+    pot = np.ascontiguousarray(pot, dtype=F64).reshape(-1)
+    nf = u.shape[0]
+    out = np.zeros((nf, g.size), dtype=u.dtype)
+    _chk(getattr(lib(), "or_g_lap4_withpot_" + _suf(u))(C.byref(g), _p(u), _p(pot), _p(out), g.size, nf),
+         "withpot")
+    return out
+
+
+def pointwise(g, a, b):
+    b = np.ascontiguousarray(b, dtype=F64)
+    out = np.zeros_like(a)
+    _chk(getattr(lib(), "or_g_pointwise_" + _suf(a))(C.byref(g), _p(a), _p(b), _p(out), a.shape[0]), "pw")
+    return out
+
+
+def hloc(g, lap_type, phi, vtot, bc=(1, 1, 1)):
+    phi = np.ascontiguousarray(phi, dtype=F64)
+    vtot = np.ascontiguousarray(vtot, dtype=F64).reshape(-1)
+    nf = phi.shape[0]
+    out = np.zeros((nf, g.size), dtype=F64)
+    _chk(lib().or_g_hloc_f64(C.byref(g), lap_type, _p(phi), _p(vtot), _p(out), g.size, nf, _bc(bc)), "hloc")
+    return out
+
+
+def axpy(alpha, x, y):
+    _chk(getattr(lib(), "or_axpy_" + _suf(y))(y.size, alpha, _p(x), _p(y)), "axpy")
+    return y
+
+
+def jacobi(g, lap_type, f, x, w, jf, bc=(1, 1, 1)):
+    _chk(getattr(lib(), "or_g_jacobi_" + _suf(x))(C.byref(g), lap_type, _p(f), _p(x), _p(w), jf,
+                                                    x.shape[0], _bc(bc)), "jacobi")
+
+
+def restrict(gf, uf, gc):
+    uc = empty_block(gc, uf.shape[0], uf.dtype)
+    _chk(getattr(lib(), "or_g_restrict_" + _suf(uf))(C.byref(gf), _p(uf), C.byref(gc), _p(uc), uf.shape[0]),
+         "restrict")
+    return uc
+
+
+def extend(gc, uc, gf, uf=None):
+    if uf is None:
+        uf = empty_block(gf, uc.shape[0], uc.dtype)
+    _chk(getattr(lib(), "or_g_extend_" + _suf(uc))(C.byref(gc), _p(uc), C.byref(gf), _p(uf), uc.shape[0]),
+         "extend")
+    return uf
+
+
+def mask(g, u, m):
+    m = np.ascontiguousarray(m, dtype=np.uint8)
+    _chk(getattr(lib(), "or_g_mask_" + _suf(u))(C.byref(g), _p(u), _p(m), u.shape[0]), "mask")
+    return u
+
+
+def diag_el(lap_type, h):
+    return lib().or_diag_el(lap_type, (C.c_double * 3)(*h))
+
+
+def jacobi_factor(lap_type, h):
+    return lib().or_jacobi_factor(lap_type, (C.c_double * 3)(*h))
+
+
+def gamma(h, mg_levels, vmax, small_eig):
+    return lib().or_gamma((C.c_double * 3)(*h), mg_levels, vmax, small_eig)
+
+
+def mg(g, lap_type, max_levels, v, f, bc=(1, 1, 1), masks=None):
+    marr, _keep = _masks(masks)
+    _chk(getattr(lib(), "or_g_mg_" + _suf(v))(C.byref(g), lap_type, max_levels, _p(v), _p(f), v.shape[0],
+                                                _bc(bc), marr), "mg")
+    return v
+
+
+def precond(g, lap_type, max_levels, gamma_, res, dtype=F32, bc=(1, 1, 1), masks=None):
+    """res: (nfunc, numpt) FP64 packed -> preconditioned (nfunc, numpt) FP64."""
+    res = np.ascontiguousarray(res, dtype=F64)
+    nf = res.shape[0]
+    out = np.zeros_like(res)
+    marr, _keep = _masks(masks)
+    fn = lib().or_g_precond_f32 if dtype == F32 else lib().or_g_precond_f64
+    _chk(fn(C.byref(g), lap_type, max_levels, gamma_, _p(res), _p(out), res.shape[1], nf, _bc(bc), marr),
+         "precond")
+    return out
+
+
+def local_gram(psi, loc_numpt, subdivx, vel):
+    """psi: (n, ld) rows = functions (column-major K x n)."""
+    psi = np.ascontiguousarray(psi, dtype=F64)
+    n, ld = psi.shape
+    ss = np.zeros((subdivx, n, n), dtype=F64)
+    lib().or_local_gram(_p(psi), ld, loc_numpt, n, subdivx, vel, _p(ss))
+    return ss
+
+
+def local_product(a, b, loc_numpt, subdivx, vel):
+    a = np.ascontiguousarray(a, dtype=F64)
+    b = np.ascontiguousarray(b, dtype=F64)
+    n, ld = a.shape
+    ss = np.zeros((subdivx, n, n), dtype=F64)
+    lib().or_local_product(_p(a), _p(b), ld, loc_numpt, n, subdivx, vel, _p(ss))
+    return ss
+
+
+def local_to_global(ss, gids, numst, tol=1e-14):
+    ss = np.ascontiguousarray(ss, dtype=F64)
+    gids = np.ascontiguousarray(gids, dtype=np.int32)
+    subdivx, m, _ = ss.shape
+    glob = np.zeros((numst, numst), dtype=F64)
+    lib().or_local_to_global(_p(ss), _p(gids), subdivx, m, numst, tol, _p(glob))
+    return glob
+
+
+def residual(psi, hphi, theta):
+    """psi, hphi: (n, ld); theta (n, n) as a numpy matrix theta[i, j]."""
+    psi = np.ascontiguousarray(psi, dtype=F64)
+    hphi = np.ascontiguousarray(hphi, dtype=F64)
+    th = np.asfortranarray(theta, dtype=F64)  # column-major theta[i + j*n]
+    n, ld = psi.shape
+    res = np.zeros_like(psi)
+    lib().or_residual(_p(psi), _p(hphi), th.ctypes.data_as(C.c_void_p), ld, ld, n, _p(res))
+    return res
+
+
+# -------------------------------------------------------- dense FP64 (LAPACK)
+def chol_inv(S):
+    """GramMatrix::updateLS + computeInverse (src/GramMatrix.cc:165-192, 82-93):
+    L = potrf('l', S), S^-1 = potri(L) symmetrised.  Raises on non-SPD."""
+    import scipy.linalg as sl
+    L = np.linalg.cholesky(np.asarray(S, dtype=F64))
+    Linv = sl.solve_triangular(L, np.eye(L.shape[0]), lower=True)
+    Sinv = Linv.T @ Linv
+    return L, 0.5 * (Sinv + Sinv.T)
+
+
+def theta(Sinv, H):
+    """ProjectedMatrices::updateTheta, src/ProjectedMatrices.h:218-225."""
+    return Sinv @ H
+
+
+def eigsum(H, Sinv):
+    """ProjectedMatrices::getEigSum, src/ProjectedMatrices.cc:550-569: tr(H S^-1) [Ry]."""
+    return float(np.sum(H * Sinv.T))
+
+
+def ritz(H, S):
+    """ProjectedMatrices::solveGenEigenProblem, src/ProjectedMatrices.cc:277-295."""
+    import scipy.linalg as sl
+    return sl.eigh(H, S)
+
+
+def residual_norm(d, Sinv):
+    """MGmol::computeConstraintResidual with the diagonal dot product,
+    src/MGmol.cc:1296-1332, src/GramMatrix.cc:302-310: 0.5*sqrt(sum d_i Sinv_ii)."""
+    return 0.5 * float(np.sqrt(np.sum(np.asarray(d) * np.diag(Sinv))))
+
+
+# ------------------------------------------------------------------ box family
+def b_call(name, b, *args):
+    return getattr(lib(), name)(C.byref(b), *args)
+
+
+def b_lap(b, level, lap_type, u):
+    out = np.zeros_like(u)
+    _chk(b_call("or_b_lap_" + _suf(u), b, level, lap_type, _p(u), _p(out)), "b_lap")
+    return out
+
+
+def b_hloc(b, vglob, u):
+    v = np.ascontiguousarray(vglob, dtype=F64).reshape(-1)
+    out = np.zeros_like(u)
+    _chk(b_call("or_b_hloc_" + _suf(u), b, _p(v), _p(u), _p(out)), "b_hloc")
+    return out
+
+
+def b_jacobi(b, level, lap_type, f, x, w, jf):
+    _chk(b_call("or_b_jacobi_" + _suf(x), b, level, lap_type, _p(f), _p(x), _p(w), jf), "b_jacobi")
+
+
+def b_restrict(b, level, fine):
+    out = np.zeros(b.size(level + 1), dtype=fine.dtype)
+    _chk(b_call("or_b_restrict_" + _suf(fine), b, level, _p(fine), _p(out)), "b_restrict")
+    return out
+
+
+def b_prolong(b, level, coarse):
+    out = np.zeros(b.size(level), dtype=coarse.dtype)
+    _chk(b_call("or_b_prolong_" + _suf(coarse), b, level, _p(coarse), _p(out)), "b_prolong")
+    return out
+
+
+def b_mg(b, lap_type, max_levels, v, f):
+    _chk(b_call("or_b_mg_" + _suf(v), b, lap_type, max_levels, _p(v), _p(f)), "b_mg")
+    return v
+
+
+def b_precond(b, lap_type, max_levels, gamma_, res, dtype=F32):
+    res = np.ascontiguousarray(res, dtype=F64)
+    out = np.zeros_like(res)
+    name = "or_b_precond_f32" if dtype == F32 else "or_b_precond_f64"
+    _chk(b_call(name, b, lap_type, max_levels, gamma_, _p(res), _p(out)), "b_precond")
+    return out
+
+
+def b_pairs(b):
+    n = int(lib().or_b_pairs(C.byref(b), 0, None, None))
+    pi = np.zeros(n, dtype=np.int32)
+    pj = np.zeros(n, dtype=np.int32)
+    lib().or_b_pairs(C.byref(b), n, _p(pi), _p(pj))
+    return pi, pj
+
+
+def b_gram_h(b, psi, hpsi):
+    n = b.nbox
+    S = np.zeros((n, n), dtype=F64)
+    H = np.zeros((n, n), dtype=F64)
+    _chk(b_call("or_b_gram_h", b, _p(psi), _p(hpsi), _p(S), _p(H)), "b_gram_h")
+    return S, H
+
+
+def b_residual(b, psi, hpsi, th):
+    th = np.asfortranarray(th, dtype=F64)
+    res = np.zeros_like(psi)
+    _chk(b_call("or_b_residual", b, _p(psi), _p(hpsi), th.ctypes.data_as(C.c_void_p), _p(res)), "b_res")
+    return res
+
+
+def b_diag_dots(b, r):
+    d = np.zeros(b.nbox, dtype=F64)
+    _chk(b_call("or_b_diag_dots", b, _p(r), _p(d)), "b_dots")
+    return d
+
+
+def b_embed(b, data, level=0):
+    """Packed boxes -> (nbox, gx, gy, gz) periodic full grids (zero outside)."""
+    gd = [b.gdim[i] >> level for i in range(3)]
+    off = b.offsets(level)
+    out = np.zeros((b.nbox,) + tuple(gd), dtype=data.dtype)
+    for j in range(b.nbox):
+        e = int(b._edge[j]) >> level
+        o = [int(b._origin[j, d]) >> level for d in range(3)]
+        blk = data[off[j]:off[j + 1]].reshape(e, e, e)
+        ix = (np.arange(e) + o[0]) % gd[0]
+        iy = (np.arange(e) + o[1]) % gd[1]
+        iz = (np.arange(e) + o[2]) % gd[2]
+        out[j][np.ix_(ix, iy, iz)] = blk
+    return out
+
+
+def b_masks(b, level=0):
+    """0/1 cube masks (nbox, size_level) for the grid family."""
+    gd = [b.gdim[i] >> level for i in range(3)]
+    m = np.zeros((b.nbox,) + tuple(gd), dtype=np.uint8)
+    for j in range(b.nbox):
+        e = int(b._edge[j]) >> level
+        o = [int(b._origin[j, d]) >> level for d in range(3)]
+        ix = (np.arange(e) + o[0]) % gd[0]
+        iy = (np.arange(e) + o[1]) % gd[1]
+        iz = (np.arange(e) + o[2]) % gd[2]
+        m[j][np.ix_(ix, iy, iz)] = 1
+    return m.reshape(b.nbox, -1)
+
+
+def b_extract(b, full, level=0):
+    """(nbox, gx, gy, gz) -> packed boxes."""
+    gd = [b.gdim[i] >> level for i in range(3)]
+    off = b.offsets(level)
+    out = np.zeros(int(off[-1]), dtype=full.dtype)
+    for j in range(b.nbox):
+        e = int(b._edge[j]) >> level
+        o = [int(b._origin[j, d]) >> level for d in range(3)]
+        ix = (np.arange(e) + o[0]) % gd[0]
+        iy = (np.arange(e) + o[1]) % gd[1]
+        iz = (np.arange(e) + o[2]) % gd[2]
+        out[off[j]:off[j + 1]] = full[j][np.ix_(ix, iy, iz)].reshape(-1)
+    return out

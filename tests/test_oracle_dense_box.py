@@ -1,0 +1,180 @@
+"""Pins for the oracle's dense products and its box family (CPU only).
+
+Dense: per-slab Gram / local product against brute-force numpy, the
+LocalMatrices2DistMatrix drop tolerance, tests/testGramMatrix.cc identities
+restated (n=213, L L^T = S to 1e-14), tr(S^-1 H) = sum of generalized
+eigenvalues.  Box family: identical to the grid family applied to the boxes
+embedded in the periodic grid with 0/1 cube masks (the reference's masked
+formulation), for every operator and for the full V-cycle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mgmol_amd import synth
+
+
+def test_local_gram_and_product_bruteforce():
+    rng = np.random.default_rng(0)
+    n, ld, lnp, sub = 5, 64, 30, 2
+    psi = rng.standard_normal((n, ld))
+    hp = rng.standard_normal((n, ld))
+    ss = O.local_gram(psi, lnp, sub, 0.125)
+    pp = O.local_product(psi, hp, lnp, sub, 0.125)
+    for s in range(sub):
+        a = psi[:, s * lnp:(s + 1) * lnp]
+        b = hp[:, s * lnp:(s + 1) * lnp]
+        assert np.allclose(ss[s], 0.125 * a @ a.T, rtol=1e-14, atol=1e-14)
+        assert np.allclose(pp[s], 0.125 * b @ a.T, rtol=1e-14, atol=1e-14) or np.allclose(
+            pp[s].T, 0.125 * b @ a.T, rtol=1e-14, atol=1e-14)
+    # column-major ss[s][j, i] stores <psi_i, hp_j>: row index = j
+    assert np.allclose(pp[0][1, 0], 0.125 * psi[0, :lnp] @ hp[1, :lnp])
+
+
+def test_local_to_global_tolerance():
+    ss = np.zeros((2, 2, 2))
+    ss[0] = [[1.0, 1e-15], [1e-15, 2.0]]
+    ss[1] = [[3.0, 0.5], [0.5, 4.0]]
+    gids = np.array([[0, 1], [0, -1]], dtype=np.int32)
+    G = O.local_to_global(ss, gids, 2)
+    assert G[0, 0] == 4.0 and G[1, 1] == 2.0 and G[0, 1] == 0.0 and G[1, 0] == 0.0
+
+
+def test_gram_matrix_reference_identities():
+    """tests/testGramMatrix.cc:46-91: B = sym(U(-1/n,1/n)), diag 2; L L^T = B."""
+    n = 213
+    rng = np.random.default_rng(11)
+    A = rng.uniform(-1.0 / n, 1.0 / n, (n, n))
+    B = 0.5 * (A + A.T)
+    np.fill_diagonal(B, 2.0)
+    L, Binv = O.chol_inv(B)
+    assert np.max(np.abs(np.tril(L) @ np.tril(L).T - B)) < 1e-14
+    assert np.max(np.abs(Binv @ B - np.eye(n))) < 1e-13
+    H = rng.standard_normal((n, n))
+    H = H + H.T
+    w, _ = O.ritz(H, B)
+    assert O.eigsum(H, Binv) == pytest.approx(np.sum(w), rel=1e-12)
+    with pytest.raises(np.linalg.LinAlgError):
+        O.chol_inv(-B)
+
+
+def test_residual_bruteforce():
+    rng = np.random.default_rng(2)
+    n, ld = 4, 50
+    psi = rng.standard_normal((n, ld))
+    hp = rng.standard_normal((n, ld))
+    th = rng.standard_normal((n, n))
+    r = O.residual(psi, hp, th)
+    assert np.allclose(r, th.T @ psi - hp, rtol=1e-13, atol=1e-13)
+
+
+def _boxes(seed=3, nbox=6, gdim=(32, 32, 32), edges=(8, 12, 16), h=0.4):
+    rng = np.random.default_rng(seed)
+    edge = rng.choice(edges, nbox)
+    origin = 4 * rng.integers(0, gdim[0] // 4, (nbox, 3))
+    return O.Boxes.make(origin, edge, gdim, h)
+
+
+def _rand_boxes(b, rng, dt=np.float64):
+    return rng.standard_normal(b.size(0)).astype(dt)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_box_lap_equals_masked_grid(kind, dt):
+    rng = np.random.default_rng(4)
+    b = _boxes()
+    u = _rand_boxes(b, rng, dt)
+    out = O.b_lap(b, 0, kind, u)
+    g = O.Grid.make(b.gdim, 2, b.h)
+    blk = O.trade(g, O.embed(g, O.b_embed(b, u)))
+    ref = O.interior(g, O.lap(g, kind, blk))
+    assert np.array_equal(out, O.b_extract(b, np.ascontiguousarray(ref)))
+
+
+def test_box_hloc_equals_grid_withpot():
+    rng = np.random.default_rng(5)
+    b = _boxes()
+    u = _rand_boxes(b, rng)
+    v = rng.standard_normal(tuple(b.gdim))
+    out = O.b_hloc(b, v, u)
+    g = O.Grid.make(b.gdim, 2, b.h)
+    blk = O.trade(g, O.embed(g, O.b_embed(b, u)))
+    ref = O.lap4_withpot(g, blk, v).reshape((b.nbox,) + tuple(b.gdim))
+    assert np.array_equal(out, O.b_extract(b, ref))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_box_transfer_equals_masked_grid(dt):
+    rng = np.random.default_rng(6)
+    b = _boxes()
+    g = O.Grid.make(b.gdim, 2, b.h)
+    gc = g.coarse()
+    u = _rand_boxes(b, rng, dt)
+    c = O.b_restrict(b, 0, u)
+    blk = O.trade(g, O.embed(g, O.b_embed(b, u)))
+    cref = O.interior(gc, O.restrict(g, blk, gc))
+    cref = O.b_extract(b, np.ascontiguousarray(cref), level=1)
+    assert np.array_equal(c, cref)
+    cu = rng.standard_normal(b.size(1)).astype(dt)
+    f = O.b_prolong(b, 0, cu)
+    cb = O.trade(gc, O.embed(gc, O.b_embed(b, cu, level=1)))
+    fref = O.interior(g, O.extend(gc, cb, g))
+    assert np.array_equal(f, O.b_extract(b, np.ascontiguousarray(fref)))
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_box_vcycle_equals_masked_grid(dt):
+    rng = np.random.default_rng(8)
+    b = _boxes()
+    g = O.Grid.make(b.gdim, 2, b.h)
+    f = _rand_boxes(b, rng, dt)
+    v = np.zeros_like(f)
+    O.axpy(0.01, f, v)
+    O.b_mg(b, 2, 2, v, f)
+    fb = O.embed(g, O.b_embed(b, f))
+    vb = np.zeros_like(fb)
+    O.axpy(0.01, fb, vb)
+    masks = [O.b_masks(b, l) for l in range(3)]
+    O.mg(g, 2, 2, vb, fb, masks=masks)
+    ref = O.b_extract(b, np.ascontiguousarray(O.interior(g, vb)))
+    if dt == np.float64:
+        assert np.max(np.abs(v - ref)) <= 1e-13 * np.max(np.abs(ref))
+    else:
+        assert np.max(np.abs(v - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_box_gram_h_residual_equal_dense():
+    rng = np.random.default_rng(9)
+    b = _boxes()
+    psi = _rand_boxes(b, rng)
+    hp = _rand_boxes(b, rng)
+    S, H = O.b_gram_h(b, psi, hp)
+    E = O.b_embed(b, psi).reshape(b.nbox, -1)
+    EH = O.b_embed(b, hp).reshape(b.nbox, -1)
+    vel = np.prod(list(b.h))
+    assert np.allclose(S, vel * E @ E.T, rtol=1e-12, atol=1e-12)
+    Hd = 0.5 * vel * (E @ EH.T + EH @ E.T)
+    assert np.allclose(H, Hd, rtol=1e-12, atol=1e-12)
+    th = rng.standard_normal((b.nbox, b.nbox))
+    r = O.b_residual(b, psi, hp, th)
+    m = O.b_masks(b)
+    rd = (th.T @ E - EH) * m
+    assert np.allclose(O.b_embed(b, r).reshape(b.nbox, -1), rd, rtol=1e-12, atol=1e-12)
+    d = O.b_diag_dots(b, r)
+    assert np.allclose(d, vel * np.sum(rd * rd, axis=1), rtol=1e-12)
+    pi, pj = O.b_pairs(b)
+    nz = {(i, j) for i in range(b.nbox) for j in range(i, b.nbox) if np.any((E[i] != 0) & (E[j] != 0))}
+    assert nz <= set(zip(pi.tolist(), pj.tolist()))
+
+
+def test_synth_boxes_valid():
+    cfg = synth.box_config(nbox=16, gdim=64, h=0.25, edges=(8, 12, 16), nwells=4, seed=1, levels=3)
+    b = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    assert b.size(0) == cfg.psi.size
+    assert np.all(cfg.origin % 4 == 0)
+    # orbitals normalised: h^3 sum psi^2 = 1 on each box
+    off = b.offsets(0)
+    vel = cfg.h ** 3
+    for j in range(b.nbox):
+        assert vel * np.sum(cfg.psi[off[j]:off[j + 1]] ** 2) == pytest.approx(1.0, rel=1e-12)
