@@ -1,0 +1,279 @@
+/*
+ * mgx.h -- C ABI of the MI355X engine for MGmol's orbital-iteration hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (MGX_OK or a negative MGX_E* code; mgx_last_error() gives the
+ * text) and never throws.  Field and matrix pointers are DEVICE pointers
+ * (HIP, gfx950) unless a parameter says "host".  Calls are ordered on the
+ * context's stream and return after the work completes (synchronous), so a
+ * caller may read results immediately.  A context is not re-entrant: use one
+ * per host thread.
+ *
+ * Layout (src/pb/Grid.cc:80-82): a grid function has x slowest, z fastest;
+ * a ghosted block stores nfunc functions of sizeg = prod(dim+2*ghosts)
+ * points each; a packed block stores nfunc functions of numpt = prod(dim)
+ * points each with leading dimension ld (LocGridOrbitals / BlockVector
+ * storage, src/BlockVector.h:59-67).  Small matrices are column-major FP64.
+ *
+ * Each declaration names the reference interface it replaces (file:line).
+ */
+#ifndef MGX_H
+#define MGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGX_OK 0
+#define MGX_EINVAL -1   /* bad argument (reference: assert / exit(2)) */
+#define MGX_EHIP -2     /* HIP runtime or library failure */
+#define MGX_ENOTSPD -3  /* Cholesky failed: S not SPD (reference: mmpi.abort) */
+#define MGX_ENUMERIC -4 /* NaN/Inf detected */
+#define MGX_ENOMEM -5   /* device allocation failed */
+
+#define MGX_F32 0
+#define MGX_F64 1
+
+/* lap_type values of LapFactory<T>::createLap, src/LapFactory.h:26-56 */
+#define MGX_LAP_MEHRSTELLEN 0
+#define MGX_LAP_2ND 1
+#define MGX_LAP_4TH 2
+
+/* precision of the S / H contractions */
+#define MGX_PREC_F64 0   /* FP64 MFMA, FP64 accumulation */
+#define MGX_PREC_SPLIT 1 /* FP32 MFMA on hi/lo split, FP32 chunk sums, FP64 total */
+
+typedef struct mgx_ctx mgx_ctx;
+typedef struct mgx_boxset mgx_boxset;
+typedef struct mgx_solver mgx_solver;
+
+/* pb::Grid, src/pb/Grid.h:24-121 (one task owns the whole grid) */
+typedef struct
+{
+    int dim[3];  /* interior points */
+    int ghosts;  /* ghost width */
+    double h[3]; /* spacing */
+    int bc[3];   /* 1 = periodic, 0 = zero (Dirichlet) ghosts */
+} mgx_grid_t;
+
+/* pb::GridFuncVector<T> storage, src/pb/GridFuncVector.h:30-493 */
+typedef struct
+{
+    void* data; /* device, nfunc*sizeg elements of dtype */
+    int dtype;  /* MGX_F32 | MGX_F64 */
+    int nfunc;
+    mgx_grid_t grid;
+} mgx_block_t;
+
+/* per-orbital cubic boxes on a periodic grid (0/1 cube masks of
+ * Map2Masks, src/Map2Masks.cc:26-58).  origin/edge are HOST arrays. */
+typedef struct
+{
+    int nbox;
+    int gdim[3];       /* global grid at level 0 */
+    double h[3];       /* spacing at level 0 */
+    const int* origin; /* host, 3*nbox, multiples of 2^(levels-1) */
+    const int* edge;   /* host, nbox, multiples of 2^(levels-1), 2*edge+4 <= gdim */
+} mgx_boxes_t;
+
+/* ------------------------------------------------------------ context ---- */
+/* Device selection and stream: replaces MagmaSingleton (src/magma_singleton.cc) */
+int mgx_ctx_create(int device, mgx_ctx** ctx);
+int mgx_ctx_destroy(mgx_ctx* ctx);
+const char* mgx_last_error(const mgx_ctx* ctx);
+/* hipStream_t of the context (for callers that interleave their own work) */
+void* mgx_ctx_stream(mgx_ctx* ctx);
+
+/* device memory: MemorySpace::Memory<T, Device>::allocate/free and
+ * MemorySpace::copy_to_dev / copy_to_host, src/tools/memory_space.h */
+int mgx_malloc(mgx_ctx* ctx, size_t bytes, void** ptr);
+int mgx_free(mgx_ctx* ctx, void* ptr);
+int mgx_memcpy_htod(mgx_ctx* ctx, void* dst_device, const void* src_host, size_t bytes);
+int mgx_memcpy_dtoh(mgx_ctx* ctx, void* dst_host, const void* src_device, size_t bytes);
+
+/* -------------------------------------------- grid operators (ghosted) ---- */
+/* GridFuncVector::trade_boundaries, src/pb/GridFuncVector.cc:1544-1622 */
+int mgx_halo(mgx_ctx* ctx, mgx_block_t* b);
+/* FDkernelDel2_4th_Mehr / _2nd / _4th via GridFuncVector::applyLap,
+ * src/pb/GridFuncVector.cc:2370-2397; src/pb/FDkernels.cc:19-162,448-520.
+ * Reads the ghosts of `in` as stored (trade first), writes the interior. */
+int mgx_lap_apply(mgx_ctx* ctx, int lap_type, const mgx_block_t* in, mgx_block_t* out);
+/* FDkernelRHS_4th_Mehr1, src/pb/FDkernels.cc:523-584 (out ghost width free) */
+int mgx_rhs_apply(mgx_ctx* ctx, const mgx_block_t* in, mgx_block_t* out);
+/* GridFuncVector::pointwiseProduct, src/pb/GridFuncVector.cc:90-136;
+ * v: device FP64 ghosted single function */
+int mgx_pointwise_product(mgx_ctx* ctx, const mgx_block_t* a, const double* v, mgx_block_t* out);
+/* LinearAlgebraUtils::MPaxpy, src/linear_algebra/mputils.cc:221-244:
+ * y += (T)(alpha*(double)x), n elements of dtype */
+int mgx_axpy(mgx_ctx* ctx, int dtype, int64_t n, double alpha, const void* x, void* y);
+/* GridFuncVector::jacobi, src/pb/GridFuncVector.cc:2416-2425 (trades x):
+ * w = L x; w -= f; x += -jacobi_factor * w */
+int mgx_jacobi(mgx_ctx* ctx, int lap_type, double jacobi_factor, const mgx_block_t* f,
+    mgx_block_t* x, mgx_block_t* w);
+/* GridFuncVector::restrict3D -> MGkernelRestrict3D, src/pb/MGkernels.cc:204-283
+ * (fine ghosts must be traded) */
+int mgx_restrict(mgx_ctx* ctx, const mgx_block_t* fine, mgx_block_t* coarse);
+/* GridFuncVector::extend3D -> MGkernelExtend3D, src/pb/MGkernels.cc:29-201
+ * (coarse ghosts must be traded) */
+int mgx_prolong(mgx_ctx* ctx, const mgx_block_t* coarse, mgx_block_t* fine);
+/* GridFuncVector::app_mask -> Map2Masks::apply, src/pb/GridFuncVector.cc:2428-2438;
+ * mask: device uint8, nfunc*numpt (0 zeroes the interior point) */
+int mgx_mask(mgx_ctx* ctx, mgx_block_t* u, const uint8_t* mask);
+/* Preconditioning<T>::mg, src/Preconditioning.cc:155-216: V-cycle on v for
+ * right-hand side f (same dtype); masks: NULL or host array of max_levels+1
+ * device pointers (each NULL or nfunc*numpt_level bytes) */
+int mgx_mg(mgx_ctx* ctx, int lap_type, int max_levels, mgx_block_t* v, const mgx_block_t* f,
+    const uint8_t* const* masks);
+
+/* ------------------------------------ Hamiltonian / preconditioner (packed) */
+/* Hamiltonian<T>::applyLocal(ncolors, phi, hphi), src/Hamiltonian.cc:86-159
+ * (4th order: FDoper::del2_4th_withPot, src/pb/FDoper.cc:322-399; Mehrstellen:
+ * -Lap_M psi + B(V psi)).  psi/hpsi packed (numpt per function, lds), halo
+ * from grid->bc; vtot device FP64 packed numpt (Rydberg). */
+int mgx_hloc_apply(mgx_ctx* ctx, int lap_type, const mgx_grid_t* grid, int dtype, const void* psi,
+    int64_t ld, int nfunc, const double* vtot, void* hpsi, int64_t ld_out);
+/* OrbitalsPreconditioning<T>::precond_mg, src/OrbitalsPreconditioning.cc:87-117:
+ * out = mg(x0 = gamma*res, f = res) in prec_dtype (MGX_F32 = MGPRECONDTYPE of
+ * the reference, src/global.h:33); res/out FP64 packed.  gamma > 0 as in the
+ * reference; the box variant below also accepts gamma = 0 (start from 0). */
+int mgx_precond_mg(mgx_ctx* ctx, int lap_type, int max_levels, double gamma, int prec_dtype,
+    const mgx_grid_t* grid, const double* res, int64_t ld, int nfunc, const uint8_t* const* masks,
+    double* out, int64_t ld_out);
+/* OrbitalsPreconditioning<T>::setGamma, src/OrbitalsPreconditioning.cc:120-145
+ * (host computation): gamma = 1/(2/(invDiag(Laph4)*4^(mg_levels+1)) + |vmax - small_eig|) */
+double mgx_gamma(const double h[3], int mg_levels, double vmax, double small_eig);
+
+/* ------------------------------------------------------ dense FP64 layer */
+/* LocGridOrbitals::getLocalOverlap -> LocalMatrices::syrk -> MPsyrk,
+ * src/LocGridOrbitals.cc:1504-1530, src/local_matrices/LocalMatrices.cc:210-247:
+ * ss[s] = vel * psi_s^T psi_s (full symmetric, n x n col-major) for slabs
+ * s < subdivx of loc_numpt rows; psi column-major (ld). */
+int mgx_local_gram(mgx_ctx* ctx, int prec, const double* psi, int64_t ld, int64_t loc_numpt, int n,
+    int subdivx, double vel, double* ss);
+/* LocGridOrbitals::computeLocalProduct -> LocalMatrices::gemm -> MPgemm('t','n'),
+ * src/LocGridOrbitals.cc:1554-1604: ss[s](i,j) = vel * <a_i, b_j>_s */
+int mgx_local_product(mgx_ctx* ctx, int prec, const double* a, const double* b, int64_t ld,
+    int64_t loc_numpt, int n, int subdivx, double vel, double* ss);
+/* LocalMatrices2DistMatrix::convert/accumulate, src/LocalMatrices2DistMatrix.cc:21-107:
+ * global += slab entries (|x| > tol) at gid pairs; gids device int, subdivx*m */
+int mgx_local_to_global(mgx_ctx* ctx, const double* ss, const int* gids, int subdivx, int m,
+    int numst, double tol, double* global);
+/* GramMatrix::updateLS + computeInverse, src/GramMatrix.cc:165-192, 82-93:
+ * L = potrf('l', S) (lower, upper zeroed), Sinv = S^-1 (symmetric).
+ * MGX_ENOTSPD if S is not positive definite. */
+int mgx_chol_inv(mgx_ctx* ctx, int n, const double* S, double* L, double* Sinv);
+/* ProjectedMatrices::updateTheta, src/ProjectedMatrices.h:218-225: theta = Sinv*H */
+int mgx_theta(mgx_ctx* ctx, int n, const double* Sinv, const double* H, double* theta);
+/* ProjectedMatrices::getEigSum, src/ProjectedMatrices.cc:550-569: tr(H Sinv) [Ry], host out */
+int mgx_eigsum(mgx_ctx* ctx, int n, const double* H, const double* Sinv, double* trace);
+/* ProjectedMatrices::solveGenEigenProblem, src/ProjectedMatrices.cc:277-295:
+ * H z = lambda S z; eigvals ascending (device n), Z device n x n (may be NULL) */
+int mgx_ritz(mgx_ctx* ctx, int n, const double* H, const double* S, double* eigvals, double* Z);
+/* MGmol::computeResidualUsingHPhi (non-Mehrstellen), src/MGmol.cc:1227-1291:
+ * res = psi*theta - hpsi (packed, ld, numpt rows, n functions) */
+int mgx_residual(mgx_ctx* ctx, const double* psi, const double* hpsi, const double* theta,
+    int64_t ld, int64_t numpt, int n, double* res);
+/* MGmol::computeConstraintResidual with LocGridOrbitals::dotProductDiagonal,
+ * src/MGmol.cc:1296-1332, src/LocGridOrbitals.cc:1606-1630:
+ * 0.5*sqrt(sum_i Sinv_ii * vel*|r_i|^2) [Ha], host out */
+int mgx_residual_norm(mgx_ctx* ctx, const double* res, int64_t ld, int64_t numpt, int n,
+    double vel, const double* Sinv, double* norm);
+
+/* --------------------------------------------------------- box family ---- */
+int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* boxes, int levels, mgx_boxset** bs);
+int mgx_boxset_destroy(mgx_boxset* bs);
+/* total points of all boxes at a level (host out) and offset of box j */
+int mgx_boxset_size(const mgx_boxset* bs, int level, int64_t* npoints);
+int mgx_boxset_offset(const mgx_boxset* bs, int level, int box, int64_t* offset);
+/* number of overlapping box pairs (i <= j) */
+int mgx_boxset_npairs(const mgx_boxset* bs, int64_t* npairs);
+
+/* Same operators as above on packed boxes, zero outside each box. */
+int mgx_box_lap(mgx_ctx* ctx, const mgx_boxset* bs, int level, int lap_type, int dtype,
+    const void* in, void* out);
+int mgx_box_hloc(mgx_ctx* ctx, const mgx_boxset* bs, int dtype, const void* psi,
+    const double* vtot, void* hpsi);
+int mgx_box_jacobi(mgx_ctx* ctx, const mgx_boxset* bs, int level, int lap_type, int dtype,
+    double jacobi_factor, const void* f, void* x, void* w);
+int mgx_box_restrict(mgx_ctx* ctx, const mgx_boxset* bs, int level, int dtype, const void* fine,
+    void* coarse);
+int mgx_box_prolong(mgx_ctx* ctx, const mgx_boxset* bs, int level, int dtype, const void* coarse,
+    void* fine);
+/* gamma = 0 starts the fine level from zero, as every coarse level of
+ * Preconditioning::mg already does (the gamma*res start makes the V-cycle
+ * indefinite on rough residuals) */
+int mgx_box_precond_mg(mgx_ctx* ctx, const mgx_boxset* bs, int lap_type, int max_levels,
+    double gamma, int prec_dtype, const double* res, double* out);
+/* S_ij = vel*<psi_i, psi_j>; H_ij = vel*<psi_i, H psi_j> (Galerkin: hpsi on the
+ * overlap plus the stencil's reach 2 points outside each box, averaged over
+ * both sides so H is symmetric); dense n x n FP64, zero for pairs that do not
+ * interact.  vtot is not needed: outside a box psi is zero. */
+int mgx_box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi,
+    const double* hpsi, double* S, double* H);
+/* res_j = sum_k psi_k theta_kj - hpsi_j on box j */
+int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
+    const double* theta, double* res);
+/* d_j = vel*|r_j|^2 (device n); norm = 0.5*sqrt(sum d_j Sinv_jj) host out */
+int mgx_box_residual_norm(mgx_ctx* ctx, const mgx_boxset* bs, const double* res,
+    const double* Sinv, double* norm);
+
+/* ------------------------------------------------------------- solver ---- */
+/* ABPG<T>::updateWF / update_states with DFTsolver's per-step normalisation,
+ * src/ABPG.cc:41-140, src/DFTsolver.cc:262-401, on a box set. */
+typedef struct
+{
+    int lap_type;        /* MGX_LAP_4TH */
+    int mg_levels;       /* preconditioner levels - 1 (Control::getMGlevels) */
+    int precond_dtype;   /* MGX_F64 (plain) or MGX_F32 */
+    int gram_prec;       /* MGX_PREC_F64 or MGX_PREC_SPLIT */
+    double step_length;  /* Quench.step_length (1.5 for 4th order) */
+    int ritz_every;      /* recompute lowest Ritz value every k steps (0 = first only) */
+    int precond_start;   /* MGX_START_GAMMA (reference) or MGX_START_ZERO */
+    int profile;         /* 1: time kernel groups with hipEvents (mgx_solver_profile) */
+} mgx_solver_cfg_t;
+
+#define MGX_START_GAMMA 0
+#define MGX_START_ZERO 1
+
+/* kernel groups timed when cfg.profile = 1 */
+#define MGX_PROF_HLOC 0      /* H psi */
+#define MGX_PROF_GRAM 1      /* S, H pair contractions */
+#define MGX_PROF_DENSE 2     /* Cholesky, inverse, theta, trace, Ritz */
+#define MGX_PROF_RESID 3     /* residual + norm */
+#define MGX_PROF_JACOBI0 4   /* fine-level Jacobi sweeps (one launch each) */
+#define MGX_PROF_PRECOND 5   /* whole V-cycle incl. update */
+#define MGX_PROF_NORMALIZE 6 /* orbital normalisation */
+#define MGX_PROF_N 7
+
+typedef struct
+{
+    int iterations;
+    int converged;
+    double residual;     /* Ha */
+    double band_energy;  /* tr(S^-1 H), Ha */
+    double seconds;
+    double lowest_eig;   /* Ry */
+} mgx_solver_stats_t;
+
+int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t* cfg,
+    const double* vtot_host, const double* psi0_host, mgx_solver** s);
+int mgx_solver_destroy(mgx_solver* s);
+/* one wave-function step; returns residual / energy of the step's input psi */
+int mgx_solver_step(mgx_solver* s, mgx_solver_stats_t* st);
+int mgx_solver_run(mgx_solver* s, double tol, int maxit, mgx_solver_stats_t* st);
+/* restart from the initial orbitals (kept on the device) */
+int mgx_solver_reset(mgx_solver* s);
+/* accumulated milliseconds and launch counts per MGX_PROF_* group (host
+ * arrays of MGX_PROF_N); reset=1 clears the accumulators afterwards */
+int mgx_solver_profile(mgx_solver* s, double* ms, int64_t* launches, int reset);
+int mgx_solver_get_psi(mgx_solver* s, double* psi_host);
+/* device pointers of the solver's psi / hpsi / S / H (for inspection) */
+int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1102 @@
+// box_ops.hip -- per-orbital cubic boxes on a periodic grid.
+//
+// A box is the reference's localisation mask (Map2Masks / GridMaskMult,
+// src/Map2Masks.cc:26-58) specialised to a 0/1 cube: data outside the box is
+// zero, so stencils read zeros there and every result is stored on the box
+// only (the mask applied after each operation).  Storage: box j at level l is
+// (edge_j >> l)^3 values, x slowest, packed one box after another.
+//
+// Work decomposition: tiles of 256 consecutive points of one box (one point
+// per lane), blocks remapped so that consecutive tiles of a box run on the
+// same XCD and share its L2.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+struct mgx_boxset
+{
+    mgx_ctx* ctx = nullptr;
+    int nbox = 0, levels = 0;
+    int gdim[3];
+    double h[3];
+    std::vector<int> origin, edge;
+    std::vector<std::vector<int64_t>> off;  // [level][nbox+1]
+    int* d_origin = nullptr;
+    int* d_edge   = nullptr;
+    std::vector<int64_t*> d_off;
+    std::vector<int> ntiles;
+    std::vector<int*> d_tile;  // per level: 2*ntiles ints (box, start)
+    // interacting pairs i <= j (overlap or within the stencil reach): 11 ints
+    // (i, j, a[3], b[3], sh[3]); [a,b) = overlap in i-local (may be empty),
+    // sh maps i-local -> j-local
+    int64_t npairs = 0;
+    int* d_pairs   = nullptr;
+    // neighbours of each box (k ascending, incl. itself): 10 ints
+    // (k, a[3], b[3], sh[3]) in j-local coordinates, sh maps j-local -> k-local
+    int* d_nbr_ptr = nullptr;
+    int* d_nbr     = nullptr;
+    int64_t nnbr   = 0;
+    int maxedge    = 0;
+};
+
+namespace mgx
+{
+
+constexpr int kTile = 256;
+
+namespace
+{
+
+constexpr double kInv12 = 1. / 12.;
+constexpr double kInv64 = 1. / 64.;
+
+int wrap_h(int i, int n)
+{
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// unique periodic shift (i-local -> j-local) under which box i meets box j
+// extended by the stencil reach (2 points); false if they do not interact
+bool ext_shift1d(int oi, int ei, int oj, int ej, int G, int* sh)
+{
+    const int s = wrap_h(oj - oi, G);
+    for (int img = -1; img <= 1; img++)
+    {
+        const int shift = -s + img * G;
+        const int lo = shift, hi = ei - 1 + shift;
+        if (hi >= -2 && lo <= ej + 1)
+        {
+            *sh = shift;
+            return true;
+        }
+    }
+    return false;
+}
+
+struct BoxView
+{
+    const int64_t* off;
+    const int* edge;
+    const int* origin;
+    const int* tile;
+    int level;
+    int ntiles;
+    int g0, g1, g2;  // global dims at this level
+};
+
+BoxView view(const mgx_boxset* bs, int level)
+{
+    BoxView v;
+    v.off    = bs->d_off[level];
+    v.edge   = bs->d_edge;
+    v.origin = bs->d_origin;
+    v.tile   = bs->d_tile[level];
+    v.level  = level;
+    v.ntiles = bs->ntiles[level];
+    v.g0     = bs->gdim[0] >> level;
+    v.g1     = bs->gdim[1] >> level;
+    v.g2     = bs->gdim[2] >> level;
+    return v;
+}
+
+unsigned tile_grid(int ntiles) { return (unsigned)(((ntiles + 7) / 8) * 8); }
+
+// resolve the tile of this block; false if the block is padding
+__device__ __forceinline__ bool tile_point(const BoxView& v, int* j, int* e, int* x, int* y, int* z, int64_t* base,
+    int* p)
+{
+    const int64_t t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= v.ntiles) return false;
+    *j          = v.tile[2 * t];
+    const int s = v.tile[2 * t + 1];
+    *e          = v.edge[*j] >> v.level;
+    const int n = (*e) * (*e) * (*e);
+    *p          = s + threadIdx.x;
+    if (*p >= n) return false;
+    *base  = v.off[*j];
+    int r  = *p;
+    *x     = r / ((*e) * (*e));
+    r -= (*x) * (*e) * (*e);
+    *y = r / (*e);
+    *z = r - (*y) * (*e);
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ T bv(const T* a, int e, int x, int y, int z)
+{
+    if ((unsigned)x >= (unsigned)e || (unsigned)y >= (unsigned)e || (unsigned)z >= (unsigned)e) return (T)0;
+    return a[((int64_t)x * e + y) * e + z];
+}
+
+// stencils on a box (zero outside) -- same arithmetic as the ghosted kernels
+template <typename T, int TYPE>
+__device__ __forceinline__ double box_stencil(const T* a, int e, int x, int y, int z, const Coef4& c4,
+    const Coef2& c2, const CoefM& cm, double diag4)
+{
+#define A(dx, dy, dz) bv(a, e, x + (dx), y + (dy), z + (dz))
+    if (TYPE == MGX_LAP_4TH)
+        return diag4 * (double)A(0, 0, 0) + c4.c1x * ((double)A(-1, 0, 0) + (double)A(1, 0, 0))
+               + c4.c1y * ((double)A(0, -1, 0) + (double)A(0, 1, 0))
+               + c4.c1z * ((double)A(0, 0, -1) + (double)A(0, 0, 1))
+               + c4.c2x * ((double)A(-2, 0, 0) + (double)A(2, 0, 0))
+               + c4.c2y * ((double)A(0, -2, 0) + (double)A(0, 2, 0))
+               + c4.c2z * ((double)A(0, 0, -2) + (double)A(0, 0, 2));
+    else if (TYPE == MGX_LAP_2ND)
+        return c2.c0 * A(0, 0, 0) + c2.c1x * (T)(A(-1, 0, 0) + A(1, 0, 0)) + c2.c1y * (T)(A(0, -1, 0) + A(0, 1, 0))
+               + c2.c1z * (T)(A(0, 0, -1) + A(0, 0, 1));
+    else
+        return cm.c0 * (double)A(0, 0, 0) + cm.cz * (double)(T)(A(0, 0, -1) + A(0, 0, 1))
+               + cm.cy * (double)(T)(A(0, -1, 0) + A(0, 1, 0)) + cm.cx * (double)(T)(A(-1, 0, 0) + A(1, 0, 0))
+               + cm.cxz * (double)(T)(A(-1, 0, -1) + A(-1, 0, 1) + A(1, 0, -1) + A(1, 0, 1))
+               + cm.cyz * (double)(T)(A(0, -1, -1) + A(0, -1, 1) + A(0, 1, -1) + A(0, 1, 1))
+               + cm.cxy * (double)(T)(A(-1, -1, 0) + A(1, -1, 0) + A(-1, 1, 0) + A(1, 1, 0));
+#undef A
+}
+
+struct Coefs
+{
+    Coef4 c4;
+    Coef2 c2;
+    CoefM cm;
+};
+
+Coefs coefs_at(const mgx_boxset* bs, int level)
+{
+    double h[3] = { bs->h[0] * (1 << level), bs->h[1] * (1 << level), bs->h[2] * (1 << level) };
+    Coefs c;
+    c.c4 = coef4(h);
+    c.c2 = coef2(h);
+    c.cm = coefm(h);
+    return c;
+}
+
+template <typename T, int TYPE>
+__global__ void __launch_bounds__(256) k_box_lap(const T* in, T* out, BoxView v, Coefs c)
+{
+    int j, e, x, y, z, p;
+    int64_t base;
+    if (!tile_point(v, &j, &e, &x, &y, &z, &base, &p)) return;
+    out[base + p] = (T)box_stencil<T, TYPE>(in + base, e, x, y, z, c.c4, c.c2, c.cm, c.c4.c0);
+}
+
+// FDoper::del2_4th_withPot on a box, V gathered from the periodic grid
+template <typename T>
+__global__ void __launch_bounds__(256) k_box_hloc(const T* in, const double* vtot, T* out, BoxView v, Coef4 c4)
+{
+    int j, e, x, y, z, p;
+    int64_t base;
+    if (!tile_point(v, &j, &e, &x, &y, &z, &base, &p)) return;
+    int gx = v.origin[3 * j] + x, gy = v.origin[3 * j + 1] + y, gz = v.origin[3 * j + 2] + z;
+    if (gx >= v.g0) gx -= v.g0;
+    if (gy >= v.g1) gy -= v.g1;
+    if (gz >= v.g2) gz -= v.g2;
+    const double pot = vtot[((int64_t)gx * v.g1 + gy) * v.g2 + gz];
+    Coef2 c2{};
+    CoefM cm{};
+    out[base + p] = (T)box_stencil<T, MGX_LAP_4TH>(in + base, e, x, y, z, c4, c2, cm, c4.c0 + pot);
+}
+
+// One GridFuncVector::jacobi sweep + mask, out of place:
+// w = (T)L x; w -= f; x_out = x + (T)(-jf*w)
+template <typename T, int TYPE>
+__global__ void __launch_bounds__(256) k_box_jacobi(const T* x, const T* f, T* xo, T* w, BoxView v, Coefs c,
+    double mjf)
+{
+    int j, e, xx, y, z, p;
+    int64_t base;
+    if (!tile_point(v, &j, &e, &xx, &y, &z, &base, &p)) return;
+    const int64_t i = base + p;
+    T wv            = (T)box_stencil<T, TYPE>(x + base, e, xx, y, z, c.c4, c.c2, c.cm, c.c4.c0);
+    wv += static_cast<T>(-1. * static_cast<double>(f[i]));
+    if (w) w[i] = wv;
+    T xv = x[i];
+    xv += static_cast<T>(mjf * static_cast<double>(wv));
+    xo[i] = xv;
+}
+
+// MGkernelRestrict3D on a box (fine values outside the box are zero)
+template <typename T>
+__global__ void __launch_bounds__(256) k_box_restrict2(const T* fine, const int64_t* off_f, T* coarse, BoxView vc)
+{
+    int j, ec, x, y, z, p;
+    int64_t cbase;
+    if (!tile_point(vc, &j, &ec, &x, &y, &z, &cbase, &p)) return;
+    const int ef = 2 * ec;
+    const T* u   = fine + off_f[j];
+    const int X = 2 * x, Y = 2 * y, Z = 2 * z;
+#define F(dx, dy, dz) (double)bv(u, ef, X + (dx), Y + (dy), Z + (dz))
+    double face   = F(1, 0, 0) + F(-1, 0, 0) + F(0, 1, 0) + F(0, -1, 0) + F(0, 0, -1) + F(0, 0, 1);
+    double corner = F(1, 1, -1) + F(1, 1, 1) + F(1, -1, -1) + F(1, -1, 1) + F(-1, 1, -1) + F(-1, 1, 1)
+                    + F(-1, -1, -1) + F(-1, -1, 1);
+    double edge = F(0, 1, -1) + F(0, 1, 1) + F(0, -1, -1) + F(0, -1, 1) + F(1, 0, -1) + F(1, 0, 1) + F(-1, 0, -1)
+                  + F(-1, 0, 1) + F(-1, -1, 0) + F(1, -1, 0) + F(-1, 1, 0) + F(1, 1, 0);
+    coarse[cbase + p] = (T)(kInv64 * (8. * F(0, 0, 0) + 4. * face + 2. * edge + corner));
+#undef F
+}
+
+// MGkernelExtend3D value at a fine box point (injected coarse values, zero
+// outside the coarse box)
+template <typename T>
+__device__ __forceinline__ T box_prolong_val(const T* uc, int ec, int x, int y, int z)
+{
+    const int ox = x & 1, oy = y & 1, oz = z & 1;
+#define I(dx, dy, dz) bv(uc, ec, (x + (dx)) >> 1, (y + (dy)) >> 1, (z + (dz)) >> 1)
+    if (!ox && !oy && !oz) return I(0, 0, 0);
+    if (!ox && !oy && oz) return (T)(0.5 * (T)(I(0, 0, -1) + I(0, 0, 1)));
+    if (!ox && oy && !oz) return (T)(0.5 * (T)(I(0, 1, 0) + I(0, -1, 0)));
+    if (!ox && oy && oz) return (T)(0.25 * (T)(I(0, 1, 1) + I(0, -1, 1) + I(0, 1, -1) + I(0, -1, -1)));
+    if (ox && !oy && oz) return (T)(0.25 * (T)(I(1, 0, 1) + I(1, 0, -1) + I(-1, 0, 1) + I(-1, 0, -1)));
+    if (ox && !oy && !oz) return (T)(0.5 * (T)(I(1, 0, 0) + I(-1, 0, 0)));
+    if (ox && oy && oz)
+        return (T)(0.125
+                   * (T)(I(1, 1, 1) + I(1, 1, -1) + I(1, -1, 1) + I(1, -1, -1) + I(-1, 1, 1) + I(-1, 1, -1)
+                         + I(-1, -1, 1) + I(-1, -1, -1)));
+    return (T)(0.25 * (T)(I(1, 1, 0) + I(1, -1, 0) + I(-1, 1, 0) + I(-1, -1, 0)));
+#undef I
+}
+
+// prolongation; with SUB: v -= P(coarse) (extend3D, app_mask, operator-=)
+template <typename T, bool SUB>
+__global__ void __launch_bounds__(256) k_box_prolong(const T* coarse, const int64_t* off_c, T* fine, BoxView vf)
+{
+    int j, ef, x, y, z, p;
+    int64_t fbase;
+    if (!tile_point(vf, &j, &ef, &x, &y, &z, &fbase, &p)) return;
+    const T pv = box_prolong_val(coarse + off_c[j], ef >> 1, x, y, z);
+    if (SUB)
+    {
+        T v = fine[fbase + p];
+        v += static_cast<T>(-1. * static_cast<double>(pv));
+        fine[fbase + p] = v;
+    }
+    else
+        fine[fbase + p] = pv;
+}
+
+template <typename T>
+__global__ void k_fill_b(T* u, int64_t n, T v)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        u[t] = v;
+}
+
+// precond_mg entry: f = (T)res; x = 0 + (T)(gamma*(double)f)
+template <typename T>
+__global__ void k_prep(const double* res, T* f, T* x, int64_t n, double gamma)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const T fv = (T)res[t];
+        f[t]       = fv;
+        T xv       = (T)0;
+        xv += static_cast<T>(gamma * static_cast<double>(fv));
+        x[t] = xv;
+    }
+}
+
+template <typename T>
+__global__ void k_to_f64(const T* x, double* out, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = (double)x[t];
+}
+
+// psi += alpha * (double)x  (ABPG::update_states, src/ABPG.cc:110,129-130)
+template <typename T>
+__global__ void k_update(double* psi, const T* x, int64_t n, double alpha)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        psi[t] += alpha * (double)x[t];
+}
+
+// -------------------------------------------------------- reductions -----
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+    {
+        v[k] = wave_sum(v[k]);
+        if (lane == 0) lds[wid * NV + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+        {
+            double s = 0.;
+            for (int w = 0; w < nw; w++)
+                s += lds[w * NV + k];
+            v[k] = s;
+        }
+    }
+}
+
+// Stencil part of (H psi_B)(X) at a point X outside box B along axis d only
+// (layer 1 or 2 beyond face `side`), from the legs that land inside B.
+__device__ __forceinline__ double skin_hv(const double* pB, int eB, int d, int side, int layer, int X0, int X1,
+    int X2, double c1, double c2)
+{
+    // X0..X2: coordinates, the one along d is outside
+    int q[3] = { X0, X1, X2 };
+    auto at  = [&](int v) {
+        q[d] = v;
+        return pB[((int64_t)q[0] * eB + q[1]) * eB + q[2]];
+    };
+    if (side == 0)
+        return layer == 1 ? c1 * at(0) + c2 * at(1) : c2 * at(0);
+    return layer == 1 ? c1 * at(eB - 1) + c2 * at(eB - 2) : c2 * at(eB - 1);
+}
+
+// sum over r in box A, r outside box B (within reach 2), psi_A(r) (H psi_B)(r);
+// shift maps A-local -> B-local coordinates.  Threads of the block stride.
+__device__ double skin_part(const double* pA, int eA, const double* pB, int eB, const int sh[3], const Coef4& c)
+{
+    double acc = 0.;
+    const double c1[3] = { c.c1x, c.c1y, c.c1z }, c2[3] = { c.c2x, c.c2y, c.c2z };
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++)
+    {
+        lo[k] = max(0, sh[k]);
+        hi[k] = min(eB, sh[k] + eA);
+    }
+    for (int slab = 0; slab < 12; slab++)
+    {
+        const int d = slab >> 2, side = (slab >> 1) & 1, layer = (slab & 1) + 1;
+        const int Xd = side == 0 ? -layer : eB - 1 + layer;
+        const int xd = Xd - sh[d];
+        if (xd < 0 || xd >= eA) continue;
+        const int u = (d + 1) % 3, w = (d + 2) % 3;
+        const int nu = hi[u] - lo[u], nw = hi[w] - lo[w];
+        if (nu <= 0 || nw <= 0) continue;
+        for (int t = threadIdx.x; t < nu * nw; t += blockDim.x)
+        {
+            const int a = t / nw, b = t - (t / nw) * nw;
+            int X[3];
+            X[d] = Xd;
+            X[u] = lo[u] + a;
+            X[w] = lo[w] + b;
+            const double hv = skin_hv(pB, eB, d, side, layer, X[0], X[1], X[2], c1[d], c2[d]);
+            const int x0 = X[0] - sh[0], x1 = X[1] - sh[1], x2 = X[2] - sh[2];
+            acc += pA[((int64_t)x0 * eA + x1) * eA + x2] * hv;
+        }
+    }
+    return acc;
+}
+
+// S / H partial sums of one interacting pair (one block per pair):
+// overlap term from the stored (masked) H psi, plus the stencil reach
+// outside each box, so that H_ij = <psi_i, H psi_j> (Galerkin).
+template <int PREC>
+__global__ void __launch_bounds__(256) k_pair_gram(const double* psi, const double* hpsi, const int64_t* off,
+    const int* edge, const int* pairs, int64_t npairs, Coef4 c4, double* out)
+{
+    __shared__ double lds[3 * 4];
+    const int64_t pid = blockIdx.x;
+    if (pid >= npairs) return;
+    const int* P = pairs + 11 * pid;
+    const int i = P[0], j = P[1];
+    const int a0 = P[2], a1 = P[3], a2 = P[4], b0 = P[5], b1 = P[6], b2 = P[7];
+    const int s0 = P[8], s1 = P[9], s2 = P[10];
+    const int ei = edge[i], ej = edge[j];
+    const int n1 = max(b1 - a1, 0), n2 = max(b2 - a2, 0);
+    const int64_t npt = (int64_t)max(b0 - a0, 0) * n1 * n2;
+    const double* pi  = psi + off[i];
+    const double* pj  = psi + off[j];
+    const double* hi  = hpsi + off[i];
+    const double* hj  = hpsi + off[j];
+    double acc[3] = { 0., 0., 0. };
+    float fs = 0.f, fh1 = 0.f, fh2 = 0.f, fx = 0.f, fx1 = 0.f, fx2 = 0.f;
+    int cnt = 0;
+    for (int64_t q = threadIdx.x; q < npt; q += blockDim.x)
+    {
+        int r       = (int)q;
+        const int x = r / (n1 * n2);
+        r -= x * n1 * n2;
+        const int y = r / n2;
+        const int z = r - y * n2;
+        const int X = a0 + x, Y = a1 + y, Z = a2 + z;
+        const int64_t ii = ((int64_t)X * ei + Y) * ei + Z;
+        const int64_t jj = ((int64_t)(X + s0) * ej + (Y + s1)) * ej + (Z + s2);
+        const double vi = pi[ii], vj = pj[jj], wi = hi[ii], wj = hj[jj];
+        if (PREC == MGX_PREC_F64)
+        {
+            acc[0] += vi * vj;
+            acc[1] += vi * wj;
+            acc[2] += wi * vj;
+        }
+        else
+        {
+            // hi/lo split: x = hi + lo, hi = fl32(x), lo = fl32(x - hi);
+            // products hi*hi + (hi*lo + lo*hi) in FP32, chunks summed in FP64
+            const float vih = (float)vi, vil = (float)(vi - (double)vih);
+            const float vjh = (float)vj, vjl = (float)(vj - (double)vjh);
+            const float wih = (float)wi, wil = (float)(wi - (double)wih);
+            const float wjh = (float)wj, wjl = (float)(wj - (double)wjh);
+            fs += vih * vjh;
+            fx += vih * vjl + vil * vjh;
+            fh1 += vih * wjh;
+            fx1 += vih * wjl + vil * wjh;
+            fh2 += wih * vjh;
+            fx2 += wih * vjl + wil * vjh;
+            if (++cnt == 16)
+            {
+                acc[0] += (double)fs + (double)fx;
+                acc[1] += (double)fh1 + (double)fx1;
+                acc[2] += (double)fh2 + (double)fx2;
+                fs = fh1 = fh2 = fx = fx1 = fx2 = 0.f;
+                cnt = 0;
+            }
+        }
+    }
+    if (PREC != MGX_PREC_F64)
+    {
+        acc[0] += (double)fs + (double)fx;
+        acc[1] += (double)fh1 + (double)fx1;
+        acc[2] += (double)fh2 + (double)fx2;
+    }
+    const int shij[3] = { s0, s1, s2 }, shji[3] = { -s0, -s1, -s2 };
+    acc[1] += skin_part(pi, ei, pj, ej, shij, c4);
+    acc[2] += skin_part(pj, ej, pi, ei, shji, c4);
+    block_sum<3>(acc, lds);
+    if (threadIdx.x == 0)
+    {
+        out[3 * pid]     = acc[0];
+        out[3 * pid + 1] = acc[1];
+        out[3 * pid + 2] = acc[2];
+    }
+}
+
+__global__ void k_pair_scatter(const double* part, const int* pairs, int64_t npairs, int n, double vel, double* S,
+    double* H)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int i = pairs[11 * t], j = pairs[11 * t + 1];
+        const double s = part[3 * t] * vel;
+        const double h = 0.5 * (part[3 * t + 1] + part[3 * t + 2]) * vel;
+        S[(int64_t)j * n + i] = s;
+        S[(int64_t)i * n + j] = s;
+        H[(int64_t)j * n + i] = h;
+        H[(int64_t)i * n + j] = h;
+    }
+}
+
+// res_j = sum_k psi_k theta_kj - hpsi_j on box j (multiplyByMatrix + axpy),
+// neighbours k in ascending order as the oracle accumulates them
+__global__ void __launch_bounds__(256) k_box_residual(const double* psi, const double* hpsi, const double* theta,
+    int n, const int* nbr_ptr, const int* nbr, BoxView v, double* res)
+{
+    int j, e, x, y, z, p;
+    int64_t base;
+    if (!tile_point(v, &j, &e, &x, &y, &z, &base, &p)) return;
+    double acc = 0.;
+    const int k0 = nbr_ptr[j], k1 = nbr_ptr[j + 1];
+    for (int q = k0; q < k1; q++)
+    {
+        const int* N = nbr + 10 * q;
+        if (x < N[1] || x >= N[4] || y < N[2] || y >= N[5] || z < N[3] || z >= N[6]) continue;
+        const int k   = N[0];
+        const int ek  = v.edge[k];
+        const double t = theta[(int64_t)j * n + k];
+        acc += psi[v.off[k] + ((int64_t)(x + N[7]) * ek + (y + N[8])) * ek + (z + N[9])] * t;
+    }
+    acc += -1. * hpsi[base + p];
+    res[base + p] = acc;
+}
+
+// d_j = vel * sum r_j^2 : one block per box
+__global__ void __launch_bounds__(256) k_box_sumsq(const double* r, const int64_t* off, int nbox, double vel,
+    double* d)
+{
+    __shared__ double lds[4];
+    const int j = blockIdx.x;
+    if (j >= nbox) return;
+    double acc[1] = { 0. };
+    for (int64_t q = off[j] + threadIdx.x; q < off[j + 1]; q += blockDim.x)
+        acc[0] += r[q] * r[q];
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) d[j] = acc[0] * vel;
+}
+
+// 0.5*sqrt(sum_j d_j Sinv_jj): single block
+__global__ void __launch_bounds__(256) k_norm(const double* d, const double* Sinv, int n, double* out)
+{
+    __shared__ double lds[4];
+    double acc[1] = { 0. };
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+        acc[0] += d[j] * Sinv[(int64_t)j * n + j];
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) out[0] = 0.5 * sqrt(acc[0]);
+}
+
+// LocGridOrbitals::normalize, src/LocGridOrbitals.cc:2230-2290:
+// psi_j *= 1/sqrt(vel*|psi_j|^2)
+__global__ void k_box_scale(double* psi, const int64_t* off, const int* tile, int ntiles, const int* edge,
+    const double* d)
+{
+    const int64_t t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= ntiles) return;
+    const int j = tile[2 * t];
+    const int p = tile[2 * t + 1] + threadIdx.x;
+    const int e = edge[j];
+    if (p >= e * e * e) return;
+    psi[off[j] + p] *= 1. / sqrt(d[j]);
+}
+
+}  // namespace
+
+// ----------------------------------------------------------- internals ---
+int box_tiles_launch_check(const mgx_boxset* bs, int level)
+{
+    return (level >= 0 && level < bs->levels) ? MGX_OK : MGX_EINVAL;
+}
+
+template <typename T>
+int box_lap_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, const T* in, T* out)
+{
+    BoxView v = view(bs, level);
+    Coefs c   = coefs_at(bs, level);
+    dim3 g(tile_grid(v.ntiles)), b(kTile);
+    switch (type)
+    {
+        case MGX_LAP_4TH: hipLaunchKernelGGL((k_box_lap<T, MGX_LAP_4TH>), g, b, 0, ctx->stream, in, out, v, c); break;
+        case MGX_LAP_2ND: hipLaunchKernelGGL((k_box_lap<T, MGX_LAP_2ND>), g, b, 0, ctx->stream, in, out, v, c); break;
+        case MGX_LAP_MEHRSTELLEN:
+            hipLaunchKernelGGL((k_box_lap<T, MGX_LAP_MEHRSTELLEN>), g, b, 0, ctx->stream, in, out, v, c);
+            break;
+        default: return set_err(ctx, MGX_EINVAL, "LapFactory::createLap() --- option invalid");
+    }
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vtot, T* out)
+{
+    BoxView v = view(bs, 0);
+    Coef4 c4  = coef4(bs->h);
+    hipLaunchKernelGGL(k_box_hloc<T>, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, in, vtot, out, v, c4);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int box_jacobi_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, double jf, const T* x, const T* f, T* xo,
+    T* w)
+{
+    BoxView v = view(bs, level);
+    Coefs c   = coefs_at(bs, level);
+    dim3 g(tile_grid(v.ntiles)), b(kTile);
+    const double mjf = -1. * jf;
+    switch (type)
+    {
+        case MGX_LAP_4TH:
+            hipLaunchKernelGGL((k_box_jacobi<T, MGX_LAP_4TH>), g, b, 0, ctx->stream, x, f, xo, w, v, c, mjf);
+            break;
+        case MGX_LAP_2ND:
+            hipLaunchKernelGGL((k_box_jacobi<T, MGX_LAP_2ND>), g, b, 0, ctx->stream, x, f, xo, w, v, c, mjf);
+            break;
+        case MGX_LAP_MEHRSTELLEN:
+            hipLaunchKernelGGL((k_box_jacobi<T, MGX_LAP_MEHRSTELLEN>), g, b, 0, ctx->stream, x, f, xo, w, v, c, mjf);
+            break;
+        default: return set_err(ctx, MGX_EINVAL, "LapFactory::createLap() --- option invalid");
+    }
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int box_restrict_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, const T* fine, T* coarse)
+{
+    BoxView vc = view(bs, level + 1);
+    hipLaunchKernelGGL(k_box_restrict2<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile), 0, ctx->stream, fine,
+        (const int64_t*)bs->d_off[level], coarse, vc);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T, bool SUB>
+int box_prolong_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, const T* coarse, T* fine)
+{
+    BoxView vf = view(bs, level);
+    hipLaunchKernelGGL((k_box_prolong<T, SUB>), dim3(tile_grid(vf.ntiles)), dim3(kTile), 0, ctx->stream, coarse,
+        (const int64_t*)bs->d_off[level + 1], fine, vf);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+struct BoxMG
+{
+    std::vector<T*> tmp, w, rc, nv;
+};
+
+template <typename T>
+int box_mg_setup(mgx_ctx* ctx, const mgx_boxset* bs, int max_levels, BoxMG<T>& B, int slot)
+{
+    MGX_REQUIRE(ctx, max_levels >= 0 && max_levels < bs->levels, "box set has too few levels");
+    size_t total = 0;
+    for (int l = 0; l <= max_levels; l++)
+    {
+        total += 2 * (size_t)bs->off[l][bs->nbox];
+        if (l > 0) total += 2 * (size_t)bs->off[l][bs->nbox];
+    }
+    T* p = (T*)workspace(ctx, slot, total * sizeof(T));
+    if (!p) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    for (int l = 0; l <= max_levels; l++)
+    {
+        const size_t n = bs->off[l][bs->nbox];
+        B.tmp.push_back(p);
+        p += n;
+        B.w.push_back(p);
+        p += n;
+    }
+    for (int l = 1; l <= max_levels; l++)
+    {
+        const size_t n = bs->off[l][bs->nbox];
+        B.rc.push_back(p);
+        p += n;
+        B.nv.push_back(p);
+        p += n;
+    }
+    return MGX_OK;
+}
+
+// Preconditioning<T>::mg, src/Preconditioning.cc:155-216, masks implicit.
+// Sweeps ping-pong between v and tmp; every sweep count is even, so the
+// result is back in v.
+template <typename T>
+int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, T* v, const T* f, int type,
+    int level)
+{
+    const int ncycl = (level == max_levels) ? 4 : 2;
+    double h[3]     = { bs->h[0] * (1 << level), bs->h[1] * (1 << level), bs->h[2] * (1 << level) };
+    const double jf = jacobi_factor(type, h);
+    T* a = v;
+    T* b = B.tmp[level];
+    int rc;
+    for (int it = 0; it < ncycl; it++)
+    {
+        T* w          = (it == ncycl - 1 && level < max_levels) ? B.w[level] : nullptr;
+        const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0) : -1;
+        if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, w))) return rc;
+        prof_end(ctx, tok);
+        std::swap(a, b);
+    }
+    if (level == max_levels) return MGX_OK;
+    T* rcoarse = B.rc[level];
+    T* nv      = B.nv[level];
+    if ((rc = box_restrict_t(ctx, bs, level, (const T*)B.w[level], rcoarse))) return rc;
+    const int64_t nc = bs->off[level + 1][bs->nbox];
+    hipLaunchKernelGGL(k_fill_b<T>, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, nv, nc, (T)0);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = box_mg_rec(ctx, bs, B, max_levels, nv, (const T*)rcoarse, MGX_LAP_2ND, level + 1))) return rc;
+    if ((rc = box_prolong_t<T, true>(ctx, bs, level, (const T*)nv, v))) return rc;
+    a = v;
+    b = B.tmp[level];
+    for (int it = 0; it < 2; it++)
+    {
+        const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0) : -1;
+        if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr))) return rc;
+        prof_end(ctx, tok);
+        std::swap(a, b);
+    }
+    return MGX_OK;
+}
+
+// OrbitalsPreconditioning::precond_mg on boxes; result left in x (type T);
+// out (FP64) written if non-null, else psi += alpha*x if psi non-null.
+template <typename T>
+int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, const double* res,
+    double* out, double* psi, double alpha)
+{
+    MGX_REQUIRE(ctx, type >= 0 && type <= 2, "LapFactory::createLap() --- option invalid");
+    MGX_REQUIRE(ctx, gamma >= 0., "gamma must be non-negative");
+    const int64_t n = bs->off[0][bs->nbox];
+    T* xf           = (T*)workspace(ctx, 2, 2 * n * sizeof(T));
+    if (!xf) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    T* x = xf;
+    T* f = xf + n;
+    hipLaunchKernelGGL(k_prep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, x, n, gamma);
+    MGX_CHECK_LAUNCH(ctx);
+    BoxMG<T> B;
+    int rc = box_mg_setup(ctx, bs, max_levels, B, 3);
+    if (rc) return rc;
+    if ((rc = box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0))) return rc;
+    if (out)
+        hipLaunchKernelGGL(k_to_f64<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, (const T*)x, out, n);
+    else if (psi)
+        hipLaunchKernelGGL(k_update<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, psi, (const T*)x, n, alpha);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_precond(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
+    const double* res, double* out, double* psi, double alpha)
+{
+    return prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, out, psi, alpha)
+                           : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, out, psi, alpha);
+}
+
+int box_hloc(mgx_ctx* ctx, const mgx_boxset* bs, int dtype, const void* psi, const double* vtot, void* hpsi)
+{
+    return dtype == MGX_F32 ? box_hloc_t(ctx, bs, (const float*)psi, vtot, (float*)hpsi)
+                            : box_hloc_t(ctx, bs, (const double*)psi, vtot, (double*)hpsi);
+}
+
+int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, const double* hpsi, double* S,
+    double* H)
+{
+    const int n = bs->nbox;
+    double* part = (double*)workspace(ctx, 4, 3 * std::max<int64_t>(bs->npairs, 1) * sizeof(double));
+    if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    const int64_t nn = (int64_t)n * n;
+    hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, S, nn, 0.);
+    hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, H, nn, 0.);
+    if (bs->npairs > 0)
+    {
+        if (prec == MGX_PREC_SPLIT)
+            hipLaunchKernelGGL(k_pair_gram<MGX_PREC_SPLIT>, dim3((unsigned)bs->npairs), dim3(256), 0, ctx->stream,
+                psi, hpsi, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_pairs,
+                bs->npairs, coef4(bs->h), part);
+        else
+            hipLaunchKernelGGL(k_pair_gram<MGX_PREC_F64>, dim3((unsigned)bs->npairs), dim3(256), 0, ctx->stream,
+                psi, hpsi, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_pairs,
+                bs->npairs, coef4(bs->h), part);
+        MGX_CHECK_LAUNCH(ctx);
+        const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+        hipLaunchKernelGGL(k_pair_scatter, dim3(nblocks(bs->npairs)), dim3(kBlock), 0, ctx->stream,
+            (const double*)part, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H);
+    }
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
+    double* res)
+{
+    BoxView v = view(bs, 0);
+    hipLaunchKernelGGL(k_box_residual, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi, hpsi, theta,
+        bs->nbox, (const int*)bs->d_nbr_ptr, (const int*)bs->d_nbr, v, res);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_sumsq(mgx_ctx* ctx, const mgx_boxset* bs, const double* r, double* d)
+{
+    const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+    hipLaunchKernelGGL(k_box_sumsq, dim3(bs->nbox), dim3(256), 0, ctx->stream, r, (const int64_t*)bs->d_off[0],
+        bs->nbox, vel, d);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_norm(mgx_ctx* ctx, const mgx_boxset* bs, const double* d, const double* Sinv, double* dev_out)
+{
+    hipLaunchKernelGGL(k_norm, dim3(1), dim3(256), 0, ctx->stream, d, Sinv, bs->nbox, dev_out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_normalize(mgx_ctx* ctx, const mgx_boxset* bs, double* psi, double* d)
+{
+    int rc = box_sumsq(ctx, bs, psi, d);
+    if (rc) return rc;
+    BoxView v = view(bs, 0);
+    hipLaunchKernelGGL(k_box_scale, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi,
+        (const int64_t*)bs->d_off[0], (const int*)bs->d_tile[0], bs->ntiles[0], (const int*)bs->d_edge,
+        (const double*)d);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int64_t box_size(const mgx_boxset* bs, int level) { return bs->off[level][bs->nbox]; }
+
+void boxset_geom(const mgx_boxset* bs, int gdim[3], double h[3], int* nbox)
+{
+    for (int d = 0; d < 3; d++)
+    {
+        gdim[d] = bs->gdim[d];
+        h[d]    = bs->h[d];
+    }
+    *nbox = bs->nbox;
+}
+
+}  // namespace mgx
+
+using namespace mgx;
+
+extern "C" {
+
+int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset** out)
+{
+    MGX_REQUIRE(ctx, b && out && b->origin && b->edge && b->nbox > 0, "bad box table");
+    MGX_REQUIRE(ctx, levels >= 1 && levels <= 8, "levels must be in [1,8]");
+    const int m = 1 << (levels - 1);
+    for (int d = 0; d < 3; d++)
+        MGX_REQUIRE(ctx, b->gdim[d] > 0 && b->gdim[d] % m == 0 && b->h[d] > 0., "bad global grid");
+    for (int j = 0; j < b->nbox; j++)
+    {
+        MGX_REQUIRE(ctx, b->edge[j] >= m && b->edge[j] % m == 0 && b->edge[j] <= 1024, "box edge not aligned");
+        for (int d = 0; d < 3; d++)
+        {
+            MGX_REQUIRE(ctx, b->origin[3 * j + d] % m == 0, "box origin not aligned");
+            MGX_REQUIRE(ctx, b->origin[3 * j + d] >= 0 && b->origin[3 * j + d] < b->gdim[d], "box origin outside grid");
+            MGX_REQUIRE(ctx, 2 * b->edge[j] + 4 <= b->gdim[d], "box edge larger than half the grid minus 2");
+        }
+    }
+    mgx_boxset* s = new mgx_boxset();
+    s->ctx        = ctx;
+    s->nbox       = b->nbox;
+    s->levels     = levels;
+    for (int d = 0; d < 3; d++)
+    {
+        s->gdim[d] = b->gdim[d];
+        s->h[d]    = b->h[d];
+    }
+    s->origin.assign(b->origin, b->origin + 3 * b->nbox);
+    s->edge.assign(b->edge, b->edge + b->nbox);
+    s->maxedge = *std::max_element(s->edge.begin(), s->edge.end());
+    auto fail  = [&](const char* what) {
+        mgx_boxset_destroy(s);
+        return set_err(ctx, MGX_EHIP, what);
+    };
+    s->off.resize(levels);
+    s->d_off.assign(levels, nullptr);
+    s->ntiles.assign(levels, 0);
+    s->d_tile.assign(levels, nullptr);
+    for (int l = 0; l < levels; l++)
+    {
+        auto& o = s->off[l];
+        o.resize(s->nbox + 1);
+        int64_t acc = 0;
+        std::vector<int> tiles;
+        for (int j = 0; j < s->nbox; j++)
+        {
+            o[j]        = acc;
+            const int e = s->edge[j] >> l;
+            const int n = e * e * e;
+            acc += n;
+            for (int st = 0; st < n; st += kTile)
+            {
+                tiles.push_back(j);
+                tiles.push_back(st);
+            }
+        }
+        o[s->nbox]   = acc;
+        s->ntiles[l] = (int)(tiles.size() / 2);
+        if (hipMalloc(&s->d_off[l], sizeof(int64_t) * (s->nbox + 1)) != hipSuccess) return fail("hipMalloc offsets");
+        if (hipMalloc(&s->d_tile[l], sizeof(int) * tiles.size()) != hipSuccess) return fail("hipMalloc tiles");
+        hipMemcpy(s->d_off[l], o.data(), sizeof(int64_t) * (s->nbox + 1), hipMemcpyHostToDevice);
+        hipMemcpy(s->d_tile[l], tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice);
+    }
+    if (hipMalloc(&s->d_origin, sizeof(int) * 3 * s->nbox) != hipSuccess) return fail("hipMalloc origin");
+    if (hipMalloc(&s->d_edge, sizeof(int) * s->nbox) != hipSuccess) return fail("hipMalloc edge");
+    hipMemcpy(s->d_origin, s->origin.data(), sizeof(int) * 3 * s->nbox, hipMemcpyHostToDevice);
+    hipMemcpy(s->d_edge, s->edge.data(), sizeof(int) * s->nbox, hipMemcpyHostToDevice);
+
+    // overlap tables (level 0), sorted by (i, j)
+    std::vector<int> pairs;
+    std::vector<std::vector<int>> nb(s->nbox);
+    for (int i = 0; i < s->nbox; i++)
+        for (int j = i; j < s->nbox; j++)
+        {
+            int a[3], bb[3], sh[3];
+            bool inter = true;
+            for (int d = 0; d < 3 && inter; d++)
+                inter = ext_shift1d(s->origin[3 * i + d], s->edge[i], s->origin[3 * j + d], s->edge[j], s->gdim[d],
+                    &sh[d]);
+            if (!inter) continue;
+            bool ok = true;
+            for (int d = 0; d < 3; d++)
+            {
+                // overlap in i-local coordinates: x in [0,ei) with x+sh in [0,ej)
+                a[d]  = std::max(0, -sh[d]);
+                bb[d] = std::min(s->edge[i], s->edge[j] - sh[d]);
+                if (a[d] >= bb[d]) ok = false;
+            }
+            pairs.insert(pairs.end(), { i, j, a[0], a[1], a[2], bb[0], bb[1], bb[2], sh[0], sh[1], sh[2] });
+            if (!ok) continue;
+            // neighbour records in j-local coordinates for box j (k = i) and
+            // in i-local coordinates for box i (k = j)
+            nb[i].insert(nb[i].end(), { j, a[0], a[1], a[2], bb[0], bb[1], bb[2], sh[0], sh[1], sh[2] });
+            if (i != j)
+                nb[j].insert(nb[j].end(), { i, a[0] + sh[0], a[1] + sh[1], a[2] + sh[2], bb[0] + sh[0],
+                                              bb[1] + sh[1], bb[2] + sh[2], -sh[0], -sh[1], -sh[2] });
+        }
+    s->npairs = (int64_t)(pairs.size() / 11);
+    std::vector<int> ptr(s->nbox + 1, 0), flat;
+    for (int j = 0; j < s->nbox; j++)
+    {
+        // sort records of box j by k
+        const int cnt = (int)(nb[j].size() / 10);
+        std::vector<int> idx(cnt);
+        for (int q = 0; q < cnt; q++)
+            idx[q] = q;
+        std::sort(idx.begin(), idx.end(), [&](int u, int w) { return nb[j][10 * u] < nb[j][10 * w]; });
+        ptr[j] = (int)(flat.size() / 10);
+        for (int q : idx)
+            flat.insert(flat.end(), nb[j].begin() + 10 * q, nb[j].begin() + 10 * q + 10);
+    }
+    ptr[s->nbox] = (int)(flat.size() / 10);
+    s->nnbr      = ptr[s->nbox];
+    if (hipMalloc(&s->d_pairs, sizeof(int) * std::max<size_t>(pairs.size(), 1)) != hipSuccess)
+        return fail("hipMalloc pairs");
+    if (hipMalloc(&s->d_nbr_ptr, sizeof(int) * ptr.size()) != hipSuccess) return fail("hipMalloc nbr ptr");
+    if (hipMalloc(&s->d_nbr, sizeof(int) * std::max<size_t>(flat.size(), 1)) != hipSuccess) return fail("hipMalloc nbr");
+    if (!pairs.empty()) hipMemcpy(s->d_pairs, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice);
+    hipMemcpy(s->d_nbr_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice);
+    if (!flat.empty()) hipMemcpy(s->d_nbr, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
+    if (hipDeviceSynchronize() != hipSuccess) return fail("box table upload");
+    *out = s;
+    return MGX_OK;
+}
+
+int mgx_boxset_destroy(mgx_boxset* s)
+{
+    if (!s) return MGX_OK;
+    for (auto p : s->d_off)
+        if (p) hipFree(p);
+    for (auto p : s->d_tile)
+        if (p) hipFree(p);
+    if (s->d_origin) hipFree(s->d_origin);
+    if (s->d_edge) hipFree(s->d_edge);
+    if (s->d_pairs) hipFree(s->d_pairs);
+    if (s->d_nbr_ptr) hipFree(s->d_nbr_ptr);
+    if (s->d_nbr) hipFree(s->d_nbr);
+    delete s;
+    return MGX_OK;
+}
+
+int mgx_boxset_size(const mgx_boxset* bs, int level, int64_t* n)
+{
+    if (!bs || !n || level < 0 || level >= bs->levels) return MGX_EINVAL;
+    *n = bs->off[level][bs->nbox];
+    return MGX_OK;
+}
+
+int mgx_boxset_offset(const mgx_boxset* bs, int level, int j, int64_t* o)
+{
+    if (!bs || !o || level < 0 || level >= bs->levels || j < 0 || j > bs->nbox) return MGX_EINVAL;
+    *o = bs->off[level][j];
+    return MGX_OK;
+}
+
+int mgx_boxset_npairs(const mgx_boxset* bs, int64_t* n)
+{
+    if (!bs || !n) return MGX_EINVAL;
+    *n = bs->npairs;
+    return MGX_OK;
+}
+
+#define BOX_ARGS_OK(ctx, bs, level)                                                           \
+    MGX_REQUIRE(ctx, bs != nullptr, "null box set");                                         \
+    MGX_REQUIRE(ctx, (level) >= 0 && (level) < bs->levels, "level outside the box set")
+
+int mgx_box_lap(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, int dtype, const void* in, void* out)
+{
+    BOX_ARGS_OK(ctx, bs, level);
+    MGX_REQUIRE(ctx, in && out, "null data");
+    int rc = dtype == MGX_F32 ? box_lap_t(ctx, bs, level, type, (const float*)in, (float*)out)
+                              : box_lap_t(ctx, bs, level, type, (const double*)in, (double*)out);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_hloc(mgx_ctx* ctx, const mgx_boxset* bs, int dtype, const void* psi, const double* vtot, void* hpsi)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, psi && vtot && hpsi, "null data");
+    int rc = box_hloc(ctx, bs, dtype, psi, vtot, hpsi);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_jacobi(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, int dtype, double jf, const void* f,
+    void* x, void* w)
+{
+    BOX_ARGS_OK(ctx, bs, level);
+    MGX_REQUIRE(ctx, f && x && w, "null data");
+    const int64_t n = bs->off[level][bs->nbox];
+    void* xo        = workspace(ctx, 5, n * esize(dtype));
+    if (!xo) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    int rc = dtype == MGX_F32
+                 ? box_jacobi_t(ctx, bs, level, type, jf, (const float*)x, (const float*)f, (float*)xo, (float*)w)
+                 : box_jacobi_t(ctx, bs, level, type, jf, (const double*)x, (const double*)f, (double*)xo, (double*)w);
+    if (rc) return rc;
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(x, xo, n * esize(dtype), hipMemcpyDeviceToDevice, ctx->stream));
+    return sync(ctx);
+}
+
+int mgx_box_restrict(mgx_ctx* ctx, const mgx_boxset* bs, int level, int dtype, const void* fine, void* coarse)
+{
+    BOX_ARGS_OK(ctx, bs, level + 1);
+    MGX_REQUIRE(ctx, fine && coarse, "null data");
+    int rc = dtype == MGX_F32 ? box_restrict_t(ctx, bs, level, (const float*)fine, (float*)coarse)
+                              : box_restrict_t(ctx, bs, level, (const double*)fine, (double*)coarse);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_prolong(mgx_ctx* ctx, const mgx_boxset* bs, int level, int dtype, const void* coarse, void* fine)
+{
+    BOX_ARGS_OK(ctx, bs, level + 1);
+    MGX_REQUIRE(ctx, fine && coarse, "null data");
+    int rc = dtype == MGX_F32 ? box_prolong_t<float, false>(ctx, bs, level, (const float*)coarse, (float*)fine)
+                              : box_prolong_t<double, false>(ctx, bs, level, (const double*)coarse, (double*)fine);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_precond_mg(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
+    const double* res, double* out)
+{
+    BOX_ARGS_OK(ctx, bs, max_levels);
+    MGX_REQUIRE(ctx, res && out, "null data");
+    MGX_REQUIRE(ctx, prec == MGX_F32 || prec == MGX_F64, "bad precision");
+    int rc = box_precond(ctx, bs, type, max_levels, gamma, prec, res, out, nullptr, 0.);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, const double* hpsi, double* S,
+    double* H)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, psi && hpsi && S && H, "null data");
+    MGX_REQUIRE(ctx, prec == MGX_PREC_F64 || prec == MGX_PREC_SPLIT, "bad precision mode");
+    int rc = box_gram_h(ctx, bs, prec, psi, hpsi, S, H);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
+    double* res)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, psi && hpsi && theta && res, "null data");
+    int rc = box_residual(ctx, bs, psi, hpsi, theta, res);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_residual_norm(mgx_ctx* ctx, const mgx_boxset* bs, const double* res, const double* Sinv, double* norm)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, res && Sinv && norm, "null data");
+    double* d = (double*)workspace(ctx, 6, (bs->nbox + 8) * sizeof(double));
+    if (!d) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    int rc = box_sumsq(ctx, bs, res, d);
+    if (!rc) rc = box_norm(ctx, bs, d, Sinv, d + bs->nbox);
+    if (rc) return rc;
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(norm, d + bs->nbox, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+}  // extern "C"

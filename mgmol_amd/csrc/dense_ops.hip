@@ -1,0 +1,430 @@
+// dense_ops.hip -- tall-skinny contractions on MFMA and the FP64 dense layer.
+//
+// ss(i,j) = vel * sum_k A(k,i) B(k,j) over one slab of rows (the reference's
+// per-subdomain LocalMatrices, src/LocGridOrbitals.cc:1504-1604).  Each wave
+// owns one 16x16 output tile and a chunk of rows:
+//   PREC_F64  : v_mfma_f64_16x16x4_f64, FP64 accumulation;
+//   PREC_SPLIT: x = hi + lo (hi = fl32(x), lo = fl32(x - hi)),
+//               v_mfma_f32_16x16x4_f32 on hi*hi and hi*lo + lo*hi, FP32
+//               accumulation over 64-row sub-chunks, FP64 sum of sub-chunks.
+// Chunk partials are reduced in a fixed order (deterministic) and scaled.
+#include "common.hpp"
+
+#include <rocsolver/rocsolver.h>
+
+namespace mgx
+{
+namespace
+{
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;  // waves per block, each on its own row sub-chunk
+
+// MFMA 16x16x4 operand layout (CDNA3/CDNA4): lane l supplies A[m=l%16][k=l/16]
+// and B[k=l/16][n=l%16]; accumulator register v of lane l is D[4*(l/16)+v][l%16].
+template <int PREC>
+__global__ void __launch_bounds__(64 * kWaves) k_tsgemm(const double* A, const double* B, int64_t ld,
+    int64_t loc_numpt, int n, int ntile, int nchunk, int64_t rows_per_chunk, int lower_only, double* part)
+{
+    const int tile  = blockIdx.x;  // tile index over (ti, tj)
+    const int chunk = blockIdx.y;
+    const int slab  = blockIdx.z;
+    const int ti = tile / ntile, tj = tile - (tile / ntile) * ntile;
+    if (lower_only && tj > ti) return;
+    const int lane = threadIdx.x & 63;
+    const int wid  = threadIdx.x >> 6;
+    const int c = lane & 15, r = lane >> 4;
+    const int i0 = ti * 16, j0 = tj * 16;
+    const int64_t k_begin = (int64_t)slab * loc_numpt + (int64_t)chunk * rows_per_chunk;
+    int64_t k_end         = k_begin + rows_per_chunk;
+    const int64_t slab_end = (int64_t)(slab + 1) * loc_numpt;
+    if (k_end > slab_end) k_end = slab_end;
+    const int ia = i0 + c, jb = j0 + c;
+    const bool va = ia < n, vb = jb < n;
+    const double* pa = A + (int64_t)(va ? ia : 0) * ld;
+    const double* pb = B + (int64_t)(vb ? jb : 0) * ld;
+    // the wave's rows: interleave waves in blocks of 64 rows
+    double acc[4] = { 0., 0., 0., 0. };
+    if (PREC == MGX_PREC_F64)
+    {
+        d4 d = { 0., 0., 0., 0. };
+        for (int64_t k0 = k_begin + 64 * wid; k0 < k_end; k0 += 64 * kWaves)
+        {
+#pragma unroll 4
+            for (int s = 0; s < 16; s++)
+            {
+                const int64_t k = k0 + 4 * s + r;
+                const bool in   = k < k_end;
+                const double a  = (va && in) ? pa[k] : 0.;
+                const double b  = (vb && in) ? pb[k] : 0.;
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+            }
+        }
+        for (int v = 0; v < 4; v++)
+            acc[v] = d[v];
+    }
+    else
+    {
+        for (int64_t k0 = k_begin + 64 * wid; k0 < k_end; k0 += 64 * kWaves)
+        {
+            f4 hh = { 0.f, 0.f, 0.f, 0.f }, hx = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll 4
+            for (int s = 0; s < 16; s++)
+            {
+                const int64_t k = k0 + 4 * s + r;
+                const bool in   = k < k_end;
+                const double a  = (va && in) ? pa[k] : 0.;
+                const double b  = (vb && in) ? pb[k] : 0.;
+                const float ah = (float)a, al = (float)(a - (double)ah);
+                const float bh = (float)b, bl = (float)(b - (double)bh);
+                hh = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, hh, 0, 0, 0);
+                hx = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, hx, 0, 0, 0);
+                hx = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, hx, 0, 0, 0);
+            }
+            for (int v = 0; v < 4; v++)
+                acc[v] += (double)hh[v] + (double)hx[v];
+        }
+    }
+    // combine the waves' partial tiles in LDS (fixed order)
+    __shared__ double red[kWaves][64][4];
+    for (int v = 0; v < 4; v++)
+        red[wid][lane][v] = acc[v];
+    __syncthreads();
+    if (wid == 0)
+    {
+        for (int v = 0; v < 4; v++)
+        {
+            double s = red[0][lane][v];
+            for (int w = 1; w < kWaves; w++)
+                s += red[w][lane][v];
+            const int row = i0 + 4 * r + v, col = j0 + c;
+            if (row < n && col < n)
+                part[(((int64_t)slab * nchunk + chunk) * n + col) * n + row] = s;
+        }
+    }
+}
+
+// ss[slab](i,j) = vel * sum_chunks part; lower_only fills the upper triangle
+// from the lower one (SquareLocalMatrices::fillUpperWithLower)
+__global__ void k_chunk_reduce(const double* part, int n, int nchunk, int subdivx, double vel, int lower_only,
+    double* ss)
+{
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn * subdivx;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t slab = t / nn;
+        const int64_t e    = t - slab * nn;
+        int col = (int)(e / n), row = (int)(e - (e / n) * n);
+        int64_t src = e;
+        if (lower_only && col > row) src = (int64_t)row * n + col;  // (col,row) element in col-major
+        double s = 0.;
+        for (int c = 0; c < nchunk; c++)
+            s += part[(slab * nchunk + c) * nn + src];
+        ss[slab * nn + e] = s * vel;
+    }
+}
+
+__global__ void k_l2g(const double* ss, const int* gids, int subdivx, int m, int numst, double tol, double* glob)
+{
+    // LocalMatrices2DistMatrix::convert, src/LocalMatrices2DistMatrix.cc:21-78;
+    // one thread per (icolor, jcolor), slabs in order, consecutive same-pair
+    // entries summed before the push.
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)m * m;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int ic = (int)(t / m), jc = (int)(t - (t / m) * m);
+        int pst_old = -1;
+        double val  = 0.;
+        int st1o = -1, st2o = -1;
+        for (int s = 0; s < subdivx; s++)
+        {
+            const int st1 = gids[s * m + ic];
+            if (st1 == -1) continue;
+            const int st2 = gids[s * m + jc];
+            if (st2 == -1) continue;
+            const int pst = st2 * numst + st1;
+            const double v = ss[((int64_t)s * m + jc) * m + ic];
+            if (fabs(v) > tol)
+            {
+                if (pst == pst_old)
+                    val += v;
+                else
+                {
+                    if (pst_old >= 0) atomicAdd(&glob[(int64_t)st2o * numst + st1o], val);
+                    val     = v;
+                    st1o    = st1;
+                    st2o    = st2;
+                    pst_old = pst;
+                }
+            }
+        }
+        if (pst_old >= 0) atomicAdd(&glob[(int64_t)st2o * numst + st1o], val);
+    }
+}
+
+__global__ void k_tri(double* L, int n, int zero_upper)
+{
+    // zero_upper: L(i,j) = 0 for j > i ; else symmetrise from the lower triangle
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)n * n;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t / n), row = (int)(t - (t / n) * n);
+        if (col <= row) continue;
+        L[t] = zero_upper ? 0. : L[(int64_t)row * n + col];
+    }
+}
+
+// single block: out = sum_{i,k} H(i,k) * Sinv(k,i)
+__global__ void __launch_bounds__(1024) k_trace_prod(const double* H, const double* Sinv, int n, double* out)
+{
+    __shared__ double red[1024];
+    double s = 0.;
+    for (int64_t t = threadIdx.x; t < (int64_t)n * n; t += blockDim.x)
+    {
+        const int k = (int)(t / n), i = (int)(t - (t / n) * n);  // H(i,k) at t
+        s += H[t] * Sinv[(int64_t)i * n + k];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ void k_sub(double* res, const double* hpsi, int64_t ld, int64_t numpt, int n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < numpt * n;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / numpt, p = t - f * numpt;
+        res[f * ld + p] += -1. * hpsi[f * ld + p];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_col_sumsq(const double* r, int64_t ld, int64_t numpt, double vel, double* d)
+{
+    __shared__ double red[256];
+    const int f = blockIdx.x;
+    double s    = 0.;
+    for (int64_t p = threadIdx.x; p < numpt; p += blockDim.x)
+        s += r[f * ld + p] * r[f * ld + p];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d[f] = red[0] * vel;
+}
+
+__global__ void __launch_bounds__(256) k_diag_dot(const double* d, const double* Sinv, int n, double* out)
+{
+    __shared__ double red[256];
+    double s = 0.;
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+        s += d[j] * Sinv[(int64_t)j * n + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = 0.5 * sqrt(red[0]);
+}
+
+}  // namespace
+
+int tsgemm(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_t ld, int64_t lnp, int n, int subdivx,
+    double vel, int lower_only, double* ss)
+{
+    MGX_REQUIRE(ctx, A && B && ss && n > 0 && subdivx > 0 && lnp > 0, "bad product arguments");
+    MGX_REQUIRE(ctx, (int64_t)subdivx * lnp <= ld, "slabs exceed the leading dimension");
+    MGX_REQUIRE(ctx, prec == MGX_PREC_F64 || prec == MGX_PREC_SPLIT, "bad precision mode");
+    const int ntile  = (n + 15) / 16;
+    const int ntiles = ntile * ntile;
+    // enough chunks to fill the chip, each at least 256 rows
+    int64_t want = (4096 + ntiles - 1) / ntiles;
+    int64_t maxc = (lnp + 255) / 256;
+    int nchunk   = (int)std::max<int64_t>(1, std::min<int64_t>(want, maxc));
+    const int64_t rows = (lnp + nchunk - 1) / nchunk;
+    const size_t pbytes = (size_t)subdivx * nchunk * n * n * sizeof(double);
+    double* part        = (double*)workspace(ctx, 7, pbytes);
+    if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    dim3 grid(ntiles, nchunk, subdivx), blk(64 * kWaves);
+    if (prec == MGX_PREC_F64)
+        hipLaunchKernelGGL(k_tsgemm<MGX_PREC_F64>, grid, blk, 0, ctx->stream, A, B, ld, lnp, n, ntile, nchunk, rows,
+            lower_only, part);
+    else
+        hipLaunchKernelGGL(k_tsgemm<MGX_PREC_SPLIT>, grid, blk, 0, ctx->stream, A, B, ld, lnp, n, ntile, nchunk,
+            rows, lower_only, part);
+    MGX_CHECK_LAUNCH(ctx);
+    const int64_t tot = (int64_t)n * n * subdivx;
+    hipLaunchKernelGGL(k_chunk_reduce, dim3(nblocks(tot)), dim3(kBlock), 0, ctx->stream, (const double*)part, n,
+        nchunk, subdivx, vel, lower_only, ss);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int chol_inv(mgx_ctx* ctx, int n, const double* S, double* L, double* Sinv)
+{
+    MGX_REQUIRE(ctx, n > 0 && S && L && Sinv, "bad chol_inv arguments");
+    int* info = (int*)workspace(ctx, 6, 64);
+    if (!info) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    const size_t bytes = (size_t)n * n * sizeof(double);
+    if (L != S) MGX_CHECK_HIP(ctx, hipMemcpyAsync(L, S, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, n, L, n, info) != rocblas_status_success)
+        return set_err(ctx, MGX_EHIP, "rocsolver_dpotrf failed");
+    int hinfo = 0;
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hinfo != 0)
+        return set_err(ctx, MGX_ENOTSPD, "ERROR in GramMatrix<MatrixType>::updateLS(): potrf info != 0");
+    hipLaunchKernelGGL(k_tri, dim3(nblocks((int64_t)n * n)), dim3(kBlock), 0, ctx->stream, L, n, 1);
+    MGX_CHECK_LAUNCH(ctx);
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(Sinv, L, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, n, Sinv, n, info) != rocblas_status_success)
+        return set_err(ctx, MGX_EHIP, "rocsolver_dpotri failed");
+    hipLaunchKernelGGL(k_tri, dim3(nblocks((int64_t)n * n)), dim3(kBlock), 0, ctx->stream, Sinv, n, 0);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int theta(mgx_ctx* ctx, int n, const double* Sinv, const double* H, double* th)
+{
+    const double one = 1., zero = 0.;
+    rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, Sinv, n, H, n, &zero,
+            th, n)
+        != rocblas_status_success)
+        return set_err(ctx, MGX_EHIP, "rocblas_dgemm failed");
+    return MGX_OK;
+}
+
+int trace_prod(mgx_ctx* ctx, int n, const double* H, const double* Sinv, double* dev_out)
+{
+    hipLaunchKernelGGL(k_trace_prod, dim3(1), dim3(1024), 0, ctx->stream, H, Sinv, n, dev_out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int ritz(mgx_ctx* ctx, int n, const double* H, const double* S, double* eig, double* Z)
+{
+    const size_t bytes = (size_t)n * n * sizeof(double);
+    double* w          = (double*)workspace(ctx, 5, 2 * bytes + (size_t)n * sizeof(double) + 256);
+    if (!w) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    double* A  = w;
+    double* B  = w + (size_t)n * n;
+    double* E  = B + (size_t)n * n;
+    int* info  = (int*)workspace(ctx, 6, 64);
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(A, H, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(B, S, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (rocsolver_dsygvd(ctx->blas, rocblas_eform_ax, Z ? rocblas_evect_original : rocblas_evect_none,
+            rocblas_fill_lower, n, A, n, B, n, eig, E, info)
+        != rocblas_status_success)
+        return set_err(ctx, MGX_EHIP, "rocsolver_dsygvd failed");
+    int hinfo = 0;
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hinfo != 0) return set_err(ctx, hinfo > n ? MGX_ENOTSPD : MGX_ENUMERIC, "sygvd did not converge / S not SPD");
+    if (Z) MGX_CHECK_HIP(ctx, hipMemcpyAsync(Z, A, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return MGX_OK;
+}
+
+}  // namespace mgx
+
+using namespace mgx;
+
+extern "C" {
+
+int mgx_local_gram(mgx_ctx* ctx, int prec, const double* psi, int64_t ld, int64_t lnp, int n, int subdivx,
+    double vel, double* ss)
+{
+    int rc = tsgemm(ctx, prec, psi, psi, ld, lnp, n, subdivx, vel, 1, ss);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_local_product(mgx_ctx* ctx, int prec, const double* a, const double* b, int64_t ld, int64_t lnp, int n,
+    int subdivx, double vel, double* ss)
+{
+    int rc = tsgemm(ctx, prec, a, b, ld, lnp, n, subdivx, vel, 0, ss);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_local_to_global(mgx_ctx* ctx, const double* ss, const int* gids, int subdivx, int m, int numst, double tol,
+    double* glob)
+{
+    MGX_REQUIRE(ctx, ss && gids && glob && subdivx > 0 && m > 0 && numst > 0, "bad local_to_global arguments");
+    hipLaunchKernelGGL(k_l2g, dim3(nblocks((int64_t)m * m)), dim3(kBlock), 0, ctx->stream, ss, gids, subdivx, m,
+        numst, tol, glob);
+    MGX_CHECK_LAUNCH(ctx);
+    return sync(ctx);
+}
+
+int mgx_chol_inv(mgx_ctx* ctx, int n, const double* S, double* L, double* Sinv)
+{
+    int rc = chol_inv(ctx, n, S, L, Sinv);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_theta(mgx_ctx* ctx, int n, const double* Sinv, const double* H, double* th)
+{
+    MGX_REQUIRE(ctx, n > 0 && Sinv && H && th, "bad theta arguments");
+    int rc = theta(ctx, n, Sinv, H, th);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_eigsum(mgx_ctx* ctx, int n, const double* H, const double* Sinv, double* trace)
+{
+    MGX_REQUIRE(ctx, n > 0 && H && Sinv && trace, "bad eigsum arguments");
+    double* d = (double*)workspace(ctx, 4, 256);
+    if (!d) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    int rc = trace_prod(ctx, n, H, Sinv, d);
+    if (rc) return rc;
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(trace, d, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+int mgx_ritz(mgx_ctx* ctx, int n, const double* H, const double* S, double* eig, double* Z)
+{
+    MGX_REQUIRE(ctx, n > 0 && H && S && eig, "bad ritz arguments");
+    int rc = ritz(ctx, n, H, S, eig, Z);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_residual(mgx_ctx* ctx, const double* psi, const double* hpsi, const double* th, int64_t ld, int64_t numpt,
+    int n, double* res)
+{
+    MGX_REQUIRE(ctx, psi && hpsi && th && res && n > 0 && numpt > 0 && ld >= numpt, "bad residual arguments");
+    const double one = 1., zero = 0.;
+    rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)numpt, n, n, &one, psi,
+            (rocblas_int)ld, th, n, &zero, res, (rocblas_int)ld)
+        != rocblas_status_success)
+        return set_err(ctx, MGX_EHIP, "rocblas_dgemm failed");
+    hipLaunchKernelGGL(k_sub, dim3(nblocks(numpt * n)), dim3(kBlock), 0, ctx->stream, res, hpsi, ld, numpt, n);
+    MGX_CHECK_LAUNCH(ctx);
+    return sync(ctx);
+}
+
+int mgx_residual_norm(mgx_ctx* ctx, const double* res, int64_t ld, int64_t numpt, int n, double vel,
+    const double* Sinv, double* norm)
+{
+    MGX_REQUIRE(ctx, res && Sinv && norm && n > 0, "bad residual_norm arguments");
+    double* d = (double*)workspace(ctx, 4, (n + 8) * sizeof(double));
+    if (!d) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    hipLaunchKernelGGL(k_col_sumsq, dim3(n), dim3(256), 0, ctx->stream, res, ld, numpt, vel, d);
+    hipLaunchKernelGGL(k_diag_dot, dim3(1), dim3(256), 0, ctx->stream, (const double*)d, Sinv, n, d + n);
+    MGX_CHECK_LAUNCH(ctx);
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(norm, d + n, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return sync(ctx);
+}
+
+}  // extern "C"

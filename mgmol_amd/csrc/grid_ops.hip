@@ -1,0 +1,907 @@
+// grid_ops.hip -- MGmol grid operators on ghosted blocks (drop-in for the
+// pb:: layer) and the packed Hamiltonian / preconditioner entry points.
+//
+// Arithmetic follows the reference kernels operation by operation (the
+// library is built with -ffp-contract=off), so FP64 results agree with the
+// reference formulas to the last bit wherever the summation order is fixed.
+#include "common.hpp"
+
+namespace mgx
+{
+namespace
+{
+
+constexpr double kInv12 = 1. / 12.;
+constexpr double kInv64 = 1. / 64.;
+
+__device__ __forceinline__ int wrapi(int i, int n)
+{
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+// ------------------------------------------------------------------ halo --
+// GridFuncVector::trade_boundaries single-task branches
+// (src/pb/GridFuncVector.cc:585-604, 1052-1070, 1426-1440): every ghost point
+// receives the periodic image, or zero along a Dirichlet direction.
+template <typename T>
+__global__ void k_halo(T* u, GGeom g, int64_t total, int bc0, int bc1, int bc2)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.sizeg;
+        int64_t r       = t - f * g.sizeg;
+        const int x     = (int)(r / g.incx);
+        r -= (int64_t)x * g.incx;
+        const int y = (int)(r / g.incy);
+        const int z = (int)(r - (int64_t)y * g.incy);
+        int c[3]    = { x - g.ng, y - g.ng, z - g.ng };
+        const int d[3]  = { g.d0, g.d1, g.d2 };
+        const int bc[3] = { bc0, bc1, bc2 };
+        bool ghost = false, zero = false;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+        {
+            if (c[k] < 0 || c[k] >= d[k])
+            {
+                ghost = true;
+                if (bc[k] == 1)
+                    c[k] = wrapi(c[k], d[k]);
+                else
+                    zero = true;
+            }
+        }
+        if (!ghost) continue;
+        T* uf = u + f * g.sizeg;
+        const int64_t dst = (int64_t)x * g.incx + (int64_t)y * g.incy + z;
+        const int64_t src = (int64_t)(c[0] + g.ng) * g.incx + (int64_t)(c[1] + g.ng) * g.incy + (c[2] + g.ng);
+        uf[dst]           = zero ? (T)0 : uf[src];
+    }
+}
+
+// ---------------------------------------------------------------- stencils --
+template <typename T>
+__device__ __forceinline__ double lap4_at(const T* a, int64_t i, int64_t X, int64_t Y, const Coef4& c,
+    double diag)
+{
+    // FDkernelDel2_4th body, src/pb/FDkernels.cc:140-156
+    return diag * (double)a[i] + c.c1x * ((double)a[i - X] + (double)a[i + X])
+           + c.c1y * ((double)a[i - Y] + (double)a[i + Y]) + c.c1z * ((double)a[i - 1] + (double)a[i + 1])
+           + c.c2x * ((double)a[i - 2 * X] + (double)a[i + 2 * X])
+           + c.c2y * ((double)a[i - 2 * Y] + (double)a[i + 2 * Y])
+           + c.c2z * ((double)a[i - 2] + (double)a[i + 2]);
+}
+
+template <typename T>
+__device__ __forceinline__ double lap2_at(const T* v, int64_t i, int64_t X, int64_t Y, const Coef2& c)
+{
+    // FDkernelDel2_2nd body, src/pb/FDkernels.cc:66-69 (pair sums in T)
+    return c.c0 * v[i] + c.c1x * (T)(v[i - X] + v[i + X]) + c.c1y * (T)(v[i - Y] + v[i + Y])
+           + c.c1z * (T)(v[i - 1] + v[i + 1]);
+}
+
+template <typename T>
+__device__ __forceinline__ double lapM_at(const T* v, int64_t i, int64_t X, int64_t Y, const CoefM& c)
+{
+    // FDkernelDel2_4th_Mehr body, src/pb/FDkernels.cc:498-513
+    const T* p = v + i;
+    return c.c0 * (double)p[0] + c.cz * (double)(T)(p[-1] + p[1]) + c.cy * (double)(T)(p[-Y] + p[Y])
+           + c.cx * (double)(T)(p[-X] + p[X])
+           + c.cxz * (double)(T)(p[-X - 1] + p[-X + 1] + p[X - 1] + p[X + 1])
+           + c.cyz * (double)(T)(p[-Y - 1] + p[-Y + 1] + p[Y - 1] + p[Y + 1])
+           + c.cxy * (double)(T)(p[-X - Y] + p[X - Y] + p[-X + Y] + p[X + Y]);
+}
+
+struct AllCoef
+{
+    Coef4 c4;
+    Coef2 c2;
+    CoefM cm;
+};
+
+template <typename T, int TYPE>
+__global__ void k_lap(const T* in, T* out, GGeom g, int64_t total, AllCoef c)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.size;
+        int r           = (int)(t - f * g.size);
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y     = r / g.d2;
+        const int z     = r - y * g.d2;
+        const int64_t i = f * g.sizeg + (int64_t)(x + g.ng) * g.incx + (int64_t)(y + g.ng) * g.incy + z + g.ng;
+        double v;
+        if (TYPE == MGX_LAP_4TH)
+            v = lap4_at(in, i, g.incx, g.incy, c.c4, c.c4.c0);
+        else if (TYPE == MGX_LAP_2ND)
+            v = lap2_at(in, i, g.incx, g.incy, c.c2);
+        else
+            v = lapM_at(in, i, g.incx, g.incy, c.cm);
+        out[i] = (T)v;
+    }
+}
+
+// FDkernelRHS_4th_Mehr1, src/pb/FDkernels.cc:523-584
+template <typename T>
+__global__ void k_rhs(const T* v, T* rhs, GGeom g, GGeom go, int64_t total)
+{
+    const int64_t X = g.incx, Y = g.incy;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.size;
+        int r           = (int)(t - f * g.size);
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y     = r / g.d2;
+        const int z     = r - y * g.d2;
+        const T* p      = v + f * g.sizeg + (int64_t)(x + g.ng) * X + (int64_t)(y + g.ng) * Y + z + g.ng;
+        const int64_t o = f * go.sizeg + (int64_t)(x + go.ng) * go.incx + (int64_t)(y + go.ng) * go.incy + z + go.ng;
+        rhs[o] = (T)(0.5 * (double)p[0] + kInv12 * (double)(T)(p[-X] + p[X] + p[-Y] + p[Y] + p[-1] + p[1]));
+    }
+}
+
+template <typename T>
+__global__ void k_pointwise(const T* a, const double* v, T* out, int64_t sizeg, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = (T)(a[t] * v[t % sizeg]);
+}
+
+template <typename T>
+__global__ void k_axpy(int64_t n, double alpha, const T* x, T* y)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        y[t] += static_cast<T>(alpha * static_cast<double>(x[t]));
+}
+
+// second half of GridFuncVector::jacobi over all sizeg points:
+// w -= f (MPaxpy -1); x += -jf * w (MPaxpy)
+template <typename T>
+__global__ void k_jacobi_update(int64_t n, double mjf, const T* f, T* w, T* x)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        T wv = w[t];
+        wv += static_cast<T>(-1. * static_cast<double>(f[t]));
+        w[t] = wv;
+        x[t] += static_cast<T>(mjf * static_cast<double>(wv));
+    }
+}
+
+// MGkernelRestrict3D, src/pb/MGkernels.cc:204-283
+template <typename T>
+__global__ void k_restrict(const T* uf, GGeom gf, T* uc, GGeom gc, int64_t total)
+{
+    const int64_t X = gf.incx, Y = gf.incy;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / gc.size;
+        int r           = (int)(t - f * gc.size);
+        const int x     = r / (gc.d1 * gc.d2);
+        r -= x * gc.d1 * gc.d2;
+        const int y = r / gc.d2;
+        const int z = r - y * gc.d2;
+        const T* u0 = uf + f * gf.sizeg + (int64_t)(2 * x + gf.ng) * X + (int64_t)(2 * y + gf.ng) * Y + gf.ng + 2 * z;
+        const T* umx = u0 - X;
+        const T* upx = u0 + X;
+        const T* umy = u0 - Y;
+        const T* upy = u0 + Y;
+        const T* umxpy = u0 - X + Y;
+        const T* upxpy = u0 + X + Y;
+        const T* umxmy = u0 - X - Y;
+        const T* upxmy = u0 + X - Y;
+        double face = (double)upx[0] + (double)umx[0] + (double)upy[0] + (double)umy[0] + (double)u0[-1]
+                      + (double)u0[1];
+        double corner = (double)upxpy[-1] + (double)upxpy[1] + (double)upxmy[-1] + (double)upxmy[1]
+                        + (double)umxpy[-1] + (double)umxpy[1] + (double)umxmy[-1] + (double)umxmy[1];
+        double edge = (double)upy[-1] + (double)upy[1] + (double)umy[-1] + (double)umy[1] + (double)upx[-1]
+                      + (double)upx[1] + (double)umx[-1] + (double)umx[1] + (double)umxmy[0] + (double)upxmy[0]
+                      + (double)umxpy[0] + (double)upxpy[0];
+        uc[f * gc.sizeg + (int64_t)(x + gc.ng) * gc.incx + (int64_t)(y + gc.ng) * gc.incy + z + gc.ng]
+            = (T)(kInv64 * (8. * u0[0] + 4. * face + 2. * edge + corner));
+    }
+}
+
+// MGkernelExtend3D injection, src/pb/MGkernels.cc:59-82 (includes the first
+// ghost layer on the + side)
+template <typename T>
+__global__ void k_inject(const T* uc, GGeom gc, T* uf, GGeom gf, int64_t total)
+{
+    const int a0 = gc.d0 + 1, a1 = gc.d1 + 1, a2 = gc.d2 + 1;
+    const int64_t per = (int64_t)a0 * a1 * a2;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / per;
+        int r           = (int)(t - f * per);
+        const int x     = r / (a1 * a2);
+        r -= x * a1 * a2;
+        const int y = r / a2;
+        const int z = r - y * a2;
+        uf[f * gf.sizeg + (int64_t)(2 * x + gf.ng) * gf.incx + (int64_t)(2 * y + gf.ng) * gf.incy + 2 * z + gf.ng]
+            = uc[f * gc.sizeg + (int64_t)(x + gc.ng) * gc.incx + (int64_t)(y + gc.ng) * gc.incy + z + gc.ng];
+    }
+}
+
+// MGkernelExtend3D interpolation, src/pb/MGkernels.cc:85-195: every odd
+// point reads only injected (even,even,even) points, in T arithmetic.
+template <typename T>
+__device__ __forceinline__ T prolong_point(const T* fd, int64_t i, int ox, int oy, int oz, int64_t X, int64_t Y)
+{
+    if (!ox && !oy && !oz) return fd[i];
+    if (!ox && !oy && oz) return (T)(0.5 * (T)(fd[i - 1] + fd[i + 1]));
+    if (!ox && oy && !oz) return (T)(0.5 * (T)(fd[i + Y] + fd[i - Y]));
+    if (!ox && oy && oz) return (T)(0.25 * (T)(fd[i + 1 + Y] + fd[i + 1 - Y] + fd[i - 1 + Y] + fd[i - 1 - Y]));
+    if (ox && !oy && oz) return (T)(0.25 * (T)(fd[i + X + 1] + fd[i + X - 1] + fd[i - X + 1] + fd[i - X - 1]));
+    if (ox && !oy && !oz) return (T)(0.5 * (T)(fd[i + X] + fd[i - X]));
+    if (ox && oy && oz)
+        return (T)(0.125
+                   * (T)(fd[i + X + Y + 1] + fd[i + X + Y - 1] + fd[i + X - Y + 1] + fd[i + X - Y - 1]
+                         + fd[i - X + Y + 1] + fd[i - X + Y - 1] + fd[i - X - Y + 1] + fd[i - X - Y - 1]));
+    return (T)(0.25 * (T)(fd[i + X + Y] + fd[i + X - Y] + fd[i - X + Y] + fd[i - X - Y]));
+}
+
+template <typename T>
+__global__ void k_fill_odd(T* uf, GGeom gf, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / gf.size;
+        int r           = (int)(t - f * gf.size);
+        const int x     = r / (gf.d1 * gf.d2);
+        r -= x * gf.d1 * gf.d2;
+        const int y = r / gf.d2;
+        const int z = r - y * gf.d2;
+        const int ox = x & 1, oy = y & 1, oz = z & 1;
+        if (!ox && !oy && !oz) continue;
+        T* fd           = uf + f * gf.sizeg;
+        const int64_t i = (int64_t)(x + gf.ng) * gf.incx + (int64_t)(y + gf.ng) * gf.incy + z + gf.ng;
+        fd[i]           = prolong_point(fd, i, ox, oy, oz, gf.incx, gf.incy);
+    }
+}
+
+template <typename T>
+__global__ void k_mask(T* u, const uint8_t* m, GGeom g, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        if (m[t]) continue;
+        const int64_t f = t / g.size;
+        int r           = (int)(t - f * g.size);
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y = r / g.d2;
+        const int z = r - y * g.d2;
+        u[f * g.sizeg + (int64_t)(x + g.ng) * g.incx + (int64_t)(y + g.ng) * g.incy + z + g.ng] = (T)0;
+    }
+}
+
+template <typename T>
+__global__ void k_fill(T* u, int64_t n, T v)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        u[t] = v;
+}
+
+// packed FP64 -> ghosted T (zero ghosts): BlockVector::setDataWithGhosts,
+// src/BlockVector.cc:489-517 (resetData then assign)
+template <typename T>
+__global__ void k_packed_to_ghosted(const double* src, int64_t ld, T* dst, GGeom g, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.sizeg;
+        int64_t r       = t - f * g.sizeg;
+        const int x     = (int)(r / g.incx) - g.ng;
+        r -= (int64_t)(x + g.ng) * g.incx;
+        const int y = (int)(r / g.incy) - g.ng;
+        const int z = (int)(r - (int64_t)(y + g.ng) * g.incy) - g.ng;
+        T v         = (T)0;
+        if (x >= 0 && y >= 0 && z >= 0 && x < g.d0 && y < g.d1 && z < g.d2)
+            v = (T)src[f * ld + ((int64_t)x * g.d1 + y) * g.d2 + z];
+        dst[t] = v;
+    }
+}
+
+template <typename T>
+__global__ void k_ghosted_to_packed(const T* src, GGeom g, double* dst, int64_t ld, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.size;
+        int r           = (int)(t - f * g.size);
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y = r / g.d2;
+        const int z = r - y * g.d2;
+        dst[f * ld + ((int64_t)x * g.d1 + y) * g.d2 + z]
+            = (double)src[f * g.sizeg + (int64_t)(x + g.ng) * g.incx + (int64_t)(y + g.ng) * g.incy + z + g.ng];
+    }
+}
+
+// --------------------------------------------- packed Hamiltonian apply ---
+struct PGeom
+{
+    int d0, d1, d2;
+    int bc0, bc1, bc2;
+    int64_t size;
+};
+
+__device__ __forceinline__ bool nb(int c, int d, int n, int bc, int* out)
+{
+    int v = c + d;
+    if (v < 0 || v >= n)
+    {
+        if (!bc) return false;
+        v = wrapi(v, n);
+    }
+    *out = v;
+    return true;
+}
+
+template <typename T>
+__device__ __forceinline__ double pval(const T* p, const PGeom& g, int x, int y, int z, int dx, int dy, int dz)
+{
+    int a, b, c;
+    if (!nb(x, dx, g.d0, g.bc0, &a) || !nb(y, dy, g.d1, g.bc1, &b) || !nb(z, dz, g.d2, g.bc2, &c)) return 0.;
+    return (double)p[((int64_t)a * g.d1 + b) * g.d2 + c];
+}
+
+template <typename T>
+__device__ __forceinline__ T pvalT(const T* p, const PGeom& g, int x, int y, int z, int dx, int dy, int dz)
+{
+    int a, b, c;
+    if (!nb(x, dx, g.d0, g.bc0, &a) || !nb(y, dy, g.d1, g.bc1, &b) || !nb(z, dz, g.d2, g.bc2, &c)) return (T)0;
+    return p[((int64_t)a * g.d1 + b) * g.d2 + c];
+}
+
+// FDoper<T>::del2_4th_withPot, src/pb/FDoper.cc:322-399, on packed data with
+// the halo applied by index wrapping.
+template <typename T>
+__global__ void k_hloc4_packed(const T* psi, int64_t ld, const double* vtot, T* hpsi, int64_t ldo, PGeom g,
+    Coef4 c, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.size;
+        const int64_t p = t - f * g.size;
+        int r           = (int)p;
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y  = r / g.d2;
+        const int z  = r - y * g.d2;
+        const T* a   = psi + f * ld;
+        double v     = (c.c0 + vtot[p]) * pval(a, g, x, y, z, 0, 0, 0)
+                   + c.c1x * (pval(a, g, x, y, z, -1, 0, 0) + pval(a, g, x, y, z, 1, 0, 0))
+                   + c.c1y * (pval(a, g, x, y, z, 0, -1, 0) + pval(a, g, x, y, z, 0, 1, 0))
+                   + c.c1z * (pval(a, g, x, y, z, 0, 0, -1) + pval(a, g, x, y, z, 0, 0, 1))
+                   + c.c2x * (pval(a, g, x, y, z, -2, 0, 0) + pval(a, g, x, y, z, 2, 0, 0))
+                   + c.c2y * (pval(a, g, x, y, z, 0, -2, 0) + pval(a, g, x, y, z, 0, 2, 0))
+                   + c.c2z * (pval(a, g, x, y, z, 0, 0, -2) + pval(a, g, x, y, z, 0, 0, 2));
+        hpsi[f * ldo + p] = (T)v;
+    }
+}
+
+// Hamiltonian<T>::applyLocal Mehrstellen branch, src/Hamiltonian.cc:106-131:
+// w1 = psi*V (pointwise), work1 = B w1, w1 = L_M psi, work1 += w1.
+template <typename T>
+__device__ __forceinline__ T vpsi(const T* a, const double* v, const PGeom& g, int x, int y, int z, int dx, int dy,
+    int dz)
+{
+    int aa, b, c;
+    if (!nb(x, dx, g.d0, g.bc0, &aa) || !nb(y, dy, g.d1, g.bc1, &b) || !nb(z, dz, g.d2, g.bc2, &c)) return (T)0;
+    const int64_t q = ((int64_t)aa * g.d1 + b) * g.d2 + c;
+    return (T)(a[q] * v[q]);
+}
+
+template <typename T>
+__global__ void k_hlocM_packed(const T* psi, int64_t ld, const double* vtot, T* hpsi, int64_t ldo, PGeom g,
+    CoefM c, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.size;
+        const int64_t p = t - f * g.size;
+        int r           = (int)p;
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y = r / g.d2;
+        const int z = r - y * g.d2;
+        const T* a  = psi + f * ld;
+        const T bw  = (T)(0.5 * (double)vpsi(a, vtot, g, x, y, z, 0, 0, 0)
+                         + kInv12
+                               * (double)(T)(vpsi(a, vtot, g, x, y, z, -1, 0, 0) + vpsi(a, vtot, g, x, y, z, 1, 0, 0)
+                                             + vpsi(a, vtot, g, x, y, z, 0, -1, 0) + vpsi(a, vtot, g, x, y, z, 0, 1, 0)
+                                             + vpsi(a, vtot, g, x, y, z, 0, 0, -1)
+                                             + vpsi(a, vtot, g, x, y, z, 0, 0, 1)));
+#define P(dx, dy, dz) pvalT(a, g, x, y, z, dx, dy, dz)
+        const T lm = (T)(c.c0 * (double)P(0, 0, 0) + c.cz * (double)(T)(P(0, 0, -1) + P(0, 0, 1))
+                         + c.cy * (double)(T)(P(0, -1, 0) + P(0, 1, 0)) + c.cx * (double)(T)(P(-1, 0, 0) + P(1, 0, 0))
+                         + c.cxz * (double)(T)(P(-1, 0, -1) + P(-1, 0, 1) + P(1, 0, -1) + P(1, 0, 1))
+                         + c.cyz * (double)(T)(P(0, -1, -1) + P(0, -1, 1) + P(0, 1, -1) + P(0, 1, 1))
+                         + c.cxy * (double)(T)(P(-1, -1, 0) + P(1, -1, 0) + P(-1, 1, 0) + P(1, 1, 0)));
+#undef P
+        T out = bw;
+        out += static_cast<T>(1. * static_cast<double>(lm));
+        hpsi[f * ldo + p] = out;
+    }
+}
+
+// --------------------------------------------------------------- drivers --
+int check_block(mgx_ctx* ctx, const mgx_block_t* b)
+{
+    MGX_REQUIRE(ctx, b && b->data, "null block");
+    MGX_REQUIRE(ctx, b->dtype == MGX_F32 || b->dtype == MGX_F64, "bad dtype");
+    MGX_REQUIRE(ctx, b->nfunc > 0, "nfunc must be positive");
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, b->grid.dim[i] > 0 && b->grid.dim[i] < 100000, "bad grid dim");
+    MGX_REQUIRE(ctx, b->grid.ghosts >= 0, "bad ghost width");
+    return MGX_OK;
+}
+
+bool same_shape(const mgx_block_t* a, const mgx_block_t* b)
+{
+    return a->dtype == b->dtype && a->nfunc == b->nfunc && a->grid.ghosts == b->grid.ghosts
+           && a->grid.dim[0] == b->grid.dim[0] && a->grid.dim[1] == b->grid.dim[1]
+           && a->grid.dim[2] == b->grid.dim[2];
+}
+
+template <typename T>
+int launch_halo(mgx_ctx* ctx, T* u, int nfunc, const mgx_grid_t& gr)
+{
+    GGeom g = ggeom(gr);
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, gr.ghosts <= gr.dim[i], "ghost width exceeds dimension");
+    const int64_t total = (int64_t)nfunc * g.sizeg;
+    hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, u, g, total, gr.bc[0],
+        gr.bc[1], gr.bc[2]);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int launch_lap(mgx_ctx* ctx, int type, const T* in, T* out, int nfunc, const mgx_grid_t& gr)
+{
+    GGeom g = ggeom(gr);
+    AllCoef c;
+    c.c4 = coef4(gr.h);
+    c.c2 = coef2(gr.h);
+    c.cm = coefm(gr.h);
+    const int64_t total = (int64_t)nfunc * g.size;
+    dim3 grd(nblocks(total)), blk(kBlock);
+    switch (type)
+    {
+        case MGX_LAP_4TH:
+            MGX_REQUIRE(ctx, gr.ghosts >= 2, "4th-order stencil needs 2 ghosts");
+            hipLaunchKernelGGL((k_lap<T, MGX_LAP_4TH>), grd, blk, 0, ctx->stream, in, out, g, total, c);
+            break;
+        case MGX_LAP_2ND:
+            MGX_REQUIRE(ctx, gr.ghosts >= 1, "2nd-order stencil needs 1 ghost");
+            hipLaunchKernelGGL((k_lap<T, MGX_LAP_2ND>), grd, blk, 0, ctx->stream, in, out, g, total, c);
+            break;
+        case MGX_LAP_MEHRSTELLEN:
+            MGX_REQUIRE(ctx, gr.ghosts >= 1, "Mehrstellen stencil needs 1 ghost");
+            hipLaunchKernelGGL((k_lap<T, MGX_LAP_MEHRSTELLEN>), grd, blk, 0, ctx->stream, in, out, g, total, c);
+            break;
+        default:
+            return set_err(ctx, MGX_EINVAL, "LapFactory::createLap() --- option invalid");
+    }
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int launch_axpy(mgx_ctx* ctx, int64_t n, double a, const T* x, T* y)
+{
+    hipLaunchKernelGGL(k_axpy<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, n, a, x, y);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int launch_jacobi(mgx_ctx* ctx, int type, double jf, const T* f, T* x, T* w, int nfunc, const mgx_grid_t& gr)
+{
+    int rc = launch_halo(ctx, x, nfunc, gr);
+    if (rc) return rc;
+    rc = launch_lap(ctx, type, x, w, nfunc, gr);
+    if (rc) return rc;
+    const int64_t n = (int64_t)nfunc * ggeom(gr).sizeg;
+    hipLaunchKernelGGL(k_jacobi_update<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, n, -1. * jf, f, w, x);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int launch_restrict(mgx_ctx* ctx, const T* uf, const mgx_grid_t& gf, T* uc, const mgx_grid_t& gc, int nfunc)
+{
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, gf.dim[i] >= 2 * gc.dim[i], "coarse grid too large");
+    MGX_REQUIRE(ctx, gf.ghosts >= 1, "restriction needs fine ghosts");
+    GGeom a = ggeom(gf), b = ggeom(gc);
+    const int64_t total = (int64_t)nfunc * b.size;
+    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, uf, a, uc, b, total);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int launch_extend(mgx_ctx* ctx, const T* uc, const mgx_grid_t& gc, T* uf, const mgx_grid_t& gf, int nfunc)
+{
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, gf.dim[i] >= 2 * gc.dim[i], "coarse grid too large");
+    MGX_REQUIRE(ctx, gf.ghosts >= 1 && gc.ghosts >= 1, "prolongation needs ghosts");
+    GGeom a = ggeom(gc), b = ggeom(gf);
+    const int64_t ninj = (int64_t)nfunc * (gc.dim[0] + 1) * (gc.dim[1] + 1) * (gc.dim[2] + 1);
+    hipLaunchKernelGGL(k_inject<T>, dim3(nblocks(ninj)), dim3(kBlock), 0, ctx->stream, uc, a, uf, b, ninj);
+    MGX_CHECK_LAUNCH(ctx);
+    const int64_t total = (int64_t)nfunc * b.size;
+    hipLaunchKernelGGL(k_fill_odd<T>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, uf, b, total);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int launch_mask(mgx_ctx* ctx, T* u, const uint8_t* m, int nfunc, const mgx_grid_t& gr)
+{
+    if (!m) return MGX_OK;
+    GGeom g             = ggeom(gr);
+    const int64_t total = (int64_t)nfunc * g.size;
+    hipLaunchKernelGGL(k_mask<T>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, u, m, g, total);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+template <typename T>
+int launch_fill(mgx_ctx* ctx, T* u, int64_t n, T v)
+{
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, u, n, v);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+mgx_grid_t coarse_of(const mgx_grid_t& g)
+{
+    // Grid::coarse_grid, src/pb/Grid.cc:214-231
+    mgx_grid_t c = g;
+    for (int i = 0; i < 3; i++)
+    {
+        c.dim[i] = g.dim[i] / 2;
+        c.h[i]   = 2. * g.h[i];
+    }
+    return c;
+}
+
+template <typename T>
+struct MGBufs
+{
+    std::vector<mgx_grid_t> grids;
+    std::vector<T*> w, rc, nv;
+};
+
+template <typename T>
+int mg_setup(mgx_ctx* ctx, MGBufs<T>& B, const mgx_grid_t& g0, int max_levels, int nfunc, int slot)
+{
+    MGX_REQUIRE(ctx, max_levels >= 0 && max_levels < 10, "bad number of levels");
+    B.grids.push_back(g0);
+    for (int l = 1; l <= max_levels; l++)
+    {
+        const mgx_grid_t& p = B.grids.back();
+        for (int i = 0; i < 3; i++)
+            MGX_REQUIRE(ctx, p.dim[i] % 2 == 0, "grid not divisible by 2 for coarsening");
+        B.grids.push_back(coarse_of(p));
+    }
+    size_t total = 0;
+    std::vector<size_t> sz(max_levels + 1);
+    for (int l = 0; l <= max_levels; l++)
+    {
+        sz[l] = (size_t)nfunc * ggeom(B.grids[l]).sizeg;
+        total += sz[l];
+        if (l > 0) total += 2 * sz[l];
+    }
+    T* base = (T*)workspace(ctx, slot, total * sizeof(T));
+    if (!base) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    T* p = base;
+    for (int l = 0; l <= max_levels; l++)
+    {
+        B.w.push_back(p);
+        p += sz[l];
+    }
+    for (int l = 1; l <= max_levels; l++)
+    {
+        B.rc.push_back(p);
+        p += sz[l];
+        B.nv.push_back(p);
+        p += sz[l];
+    }
+    return MGX_OK;
+}
+
+// Preconditioning<T>::mg, src/Preconditioning.cc:155-216
+template <typename T>
+int mg_rec(mgx_ctx* ctx, MGBufs<T>& B, int max_levels, T* v, const T* f, int type, int level, int nfunc,
+    const uint8_t* const* masks)
+{
+    const mgx_grid_t& g = B.grids[level];
+    const int ncycl     = (level == max_levels) ? 4 : 2;
+    const double jf     = jacobi_factor(type, g.h);
+    T* w                = B.w[level];
+    const uint8_t* mk   = masks ? masks[level] : nullptr;
+    int rc;
+    for (int it = 0; it < ncycl; it++)
+    {
+        if ((rc = launch_jacobi(ctx, type, jf, f, v, w, nfunc, g))) return rc;
+        if ((rc = launch_mask(ctx, v, mk, nfunc, g))) return rc;
+    }
+    if (level == max_levels) return MGX_OK;
+    if ((rc = launch_mask(ctx, w, mk, nfunc, g))) return rc;
+    const mgx_grid_t& gc = B.grids[level + 1];
+    T* rcoarse           = B.rc[level];
+    if ((rc = launch_halo(ctx, w, nfunc, g))) return rc;
+    if ((rc = launch_restrict(ctx, w, g, rcoarse, gc, nfunc))) return rc;
+    if ((rc = launch_mask(ctx, rcoarse, masks ? masks[level + 1] : nullptr, nfunc, gc))) return rc;
+    T* nv = B.nv[level];
+    if ((rc = launch_fill(ctx, nv, (int64_t)nfunc * ggeom(gc).sizeg, (T)0))) return rc;
+    if ((rc = mg_rec(ctx, B, max_levels, nv, rcoarse, MGX_LAP_2ND, level + 1, nfunc, masks))) return rc;
+    if ((rc = launch_halo(ctx, nv, nfunc, gc))) return rc;
+    if ((rc = launch_extend(ctx, nv, gc, w, g, nfunc))) return rc;
+    if ((rc = launch_mask(ctx, w, mk, nfunc, g))) return rc;
+    if ((rc = launch_axpy(ctx, (int64_t)nfunc * ggeom(g).sizeg, -1., (const T*)w, v))) return rc;
+    for (int it = 0; it < 2; it++)
+    {
+        if ((rc = launch_jacobi(ctx, type, jf, f, v, w, nfunc, g))) return rc;
+        if ((rc = launch_mask(ctx, v, mk, nfunc, g))) return rc;
+    }
+    if (g.bc[0] != 1 || g.bc[1] != 1 || g.bc[2] != 1) return launch_halo(ctx, v, nfunc, g);
+    return MGX_OK;
+}
+
+template <typename T>
+int mg_run(mgx_ctx* ctx, int type, int max_levels, T* v, const T* f, int nfunc, const mgx_grid_t& g0,
+    const uint8_t* const* masks, int slot)
+{
+    MGX_REQUIRE(ctx, type >= 0 && type <= 2, "LapFactory::createLap() --- option invalid");
+    MGBufs<T> B;
+    int rc = mg_setup(ctx, B, g0, max_levels, nfunc, slot);
+    if (rc) return rc;
+    return mg_rec(ctx, B, max_levels, v, f, type, 0, nfunc, masks);
+}
+
+template <typename T>
+int precond_run(mgx_ctx* ctx, int type, int max_levels, double gamma, const mgx_grid_t& g0, const double* res,
+    int64_t ld, int nfunc, const uint8_t* const* masks, double* out, int64_t ldo)
+{
+    GGeom g        = ggeom(g0);
+    const int64_t n = (int64_t)nfunc * g.sizeg;
+    T* buf         = (T*)workspace(ctx, 0, 2 * n * sizeof(T));
+    if (!buf) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    T* x = buf;
+    T* f = buf + n;
+    hipLaunchKernelGGL(k_packed_to_ghosted<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, ld, f, g, n);
+    MGX_CHECK_LAUNCH(ctx);
+    int rc = launch_fill(ctx, x, n, (T)0);
+    if (rc) return rc;
+    if ((rc = launch_axpy(ctx, n, gamma, (const T*)f, x))) return rc;
+    if ((rc = mg_run(ctx, type, max_levels, x, (const T*)f, nfunc, g0, masks, 1))) return rc;
+    const int64_t m = (int64_t)nfunc * g.size;
+    hipLaunchKernelGGL(k_ghosted_to_packed<T>, dim3(nblocks(m)), dim3(kBlock), 0, ctx->stream, x, g, out, ldo, m);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+}  // namespace
+
+int halo(mgx_ctx* ctx, int dtype, void* data, int nfunc, const mgx_grid_t& g)
+{
+    return dtype == MGX_F32 ? launch_halo(ctx, (float*)data, nfunc, g) : launch_halo(ctx, (double*)data, nfunc, g);
+}
+
+}  // namespace mgx
+
+using namespace mgx;
+
+extern "C" {
+
+int mgx_halo(mgx_ctx* ctx, mgx_block_t* b)
+{
+    int rc = check_block(ctx, b);
+    if (rc) return rc;
+    rc = halo(ctx, b->dtype, b->data, b->nfunc, b->grid);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_lap_apply(mgx_ctx* ctx, int type, const mgx_block_t* in, mgx_block_t* out)
+{
+    int rc = check_block(ctx, in);
+    if (!rc) rc = check_block(ctx, out);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, same_shape(in, out), "in/out blocks differ in shape");
+    rc = in->dtype == MGX_F32
+             ? launch_lap(ctx, type, (const float*)in->data, (float*)out->data, in->nfunc, in->grid)
+             : launch_lap(ctx, type, (const double*)in->data, (double*)out->data, in->nfunc, in->grid);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_rhs_apply(mgx_ctx* ctx, const mgx_block_t* in, mgx_block_t* out)
+{
+    int rc = check_block(ctx, in);
+    if (!rc) rc = check_block(ctx, out);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, in->grid.ghosts >= 1, "Mehrstellen rhs needs 1 ghost");
+    MGX_REQUIRE(ctx, in->dtype == out->dtype && in->nfunc == out->nfunc, "in/out mismatch");
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, in->grid.dim[i] == out->grid.dim[i], "in/out dims differ");
+    GGeom a = ggeom(in->grid), b = ggeom(out->grid);
+    const int64_t total = (int64_t)in->nfunc * a.size;
+    if (in->dtype == MGX_F32)
+        hipLaunchKernelGGL(k_rhs<float>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream,
+            (const float*)in->data, (float*)out->data, a, b, total);
+    else
+        hipLaunchKernelGGL(k_rhs<double>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream,
+            (const double*)in->data, (double*)out->data, a, b, total);
+    MGX_CHECK_LAUNCH(ctx);
+    return sync(ctx);
+}
+
+int mgx_pointwise_product(mgx_ctx* ctx, const mgx_block_t* a, const double* v, mgx_block_t* out)
+{
+    int rc = check_block(ctx, a);
+    if (!rc) rc = check_block(ctx, out);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, v != nullptr, "null potential");
+    MGX_REQUIRE(ctx, same_shape(a, out), "in/out blocks differ in shape");
+    GGeom g             = ggeom(a->grid);
+    const int64_t total = (int64_t)a->nfunc * g.sizeg;
+    if (a->dtype == MGX_F32)
+        hipLaunchKernelGGL(k_pointwise<float>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream,
+            (const float*)a->data, v, (float*)out->data, g.sizeg, total);
+    else
+        hipLaunchKernelGGL(k_pointwise<double>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream,
+            (const double*)a->data, v, (double*)out->data, g.sizeg, total);
+    MGX_CHECK_LAUNCH(ctx);
+    return sync(ctx);
+}
+
+int mgx_axpy(mgx_ctx* ctx, int dtype, int64_t n, double alpha, const void* x, void* y)
+{
+    MGX_REQUIRE(ctx, n >= 0 && (n == 0 || (x && y)), "bad axpy arguments");
+    if (n == 0) return MGX_OK;
+    int rc = dtype == MGX_F32 ? launch_axpy(ctx, n, alpha, (const float*)x, (float*)y)
+                              : launch_axpy(ctx, n, alpha, (const double*)x, (double*)y);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_jacobi(mgx_ctx* ctx, int type, double jf, const mgx_block_t* f, mgx_block_t* x, mgx_block_t* w)
+{
+    int rc = check_block(ctx, f);
+    if (!rc) rc = check_block(ctx, x);
+    if (!rc) rc = check_block(ctx, w);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, same_shape(f, x) && same_shape(x, w), "jacobi blocks differ in shape");
+    rc = x->dtype == MGX_F32 ? launch_jacobi(ctx, type, jf, (const float*)f->data, (float*)x->data,
+                                   (float*)w->data, x->nfunc, x->grid)
+                             : launch_jacobi(ctx, type, jf, (const double*)f->data, (double*)x->data,
+                                   (double*)w->data, x->nfunc, x->grid);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_restrict(mgx_ctx* ctx, const mgx_block_t* fine, mgx_block_t* coarse)
+{
+    int rc = check_block(ctx, fine);
+    if (!rc) rc = check_block(ctx, coarse);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, fine->dtype == coarse->dtype && fine->nfunc == coarse->nfunc, "fine/coarse mismatch");
+    rc = fine->dtype == MGX_F32
+             ? launch_restrict(ctx, (const float*)fine->data, fine->grid, (float*)coarse->data, coarse->grid,
+                   fine->nfunc)
+             : launch_restrict(ctx, (const double*)fine->data, fine->grid, (double*)coarse->data, coarse->grid,
+                   fine->nfunc);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_prolong(mgx_ctx* ctx, const mgx_block_t* coarse, mgx_block_t* fine)
+{
+    int rc = check_block(ctx, fine);
+    if (!rc) rc = check_block(ctx, coarse);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, fine->dtype == coarse->dtype && fine->nfunc == coarse->nfunc, "fine/coarse mismatch");
+    rc = fine->dtype == MGX_F32
+             ? launch_extend(ctx, (const float*)coarse->data, coarse->grid, (float*)fine->data, fine->grid,
+                   fine->nfunc)
+             : launch_extend(ctx, (const double*)coarse->data, coarse->grid, (double*)fine->data, fine->grid,
+                   fine->nfunc);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_mask(mgx_ctx* ctx, mgx_block_t* u, const uint8_t* mask)
+{
+    int rc = check_block(ctx, u);
+    if (rc) return rc;
+    rc = u->dtype == MGX_F32 ? launch_mask(ctx, (float*)u->data, mask, u->nfunc, u->grid)
+                             : launch_mask(ctx, (double*)u->data, mask, u->nfunc, u->grid);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_mg(mgx_ctx* ctx, int type, int max_levels, mgx_block_t* v, const mgx_block_t* f,
+    const uint8_t* const* masks)
+{
+    int rc = check_block(ctx, v);
+    if (!rc) rc = check_block(ctx, f);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, same_shape(v, f), "v/f blocks differ in shape");
+    rc = v->dtype == MGX_F32
+             ? mg_run(ctx, type, max_levels, (float*)v->data, (const float*)f->data, v->nfunc, v->grid, masks, 1)
+             : mg_run(ctx, type, max_levels, (double*)v->data, (const double*)f->data, v->nfunc, v->grid, masks,
+                   1);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_hloc_apply(mgx_ctx* ctx, int type, const mgx_grid_t* grid, int dtype, const void* psi, int64_t ld,
+    int nfunc, const double* vtot, void* hpsi, int64_t ldo)
+{
+    MGX_REQUIRE(ctx, grid && psi && vtot && hpsi && nfunc > 0, "bad hloc arguments");
+    MGX_REQUIRE(ctx, dtype == MGX_F32 || dtype == MGX_F64, "bad dtype");
+    PGeom g;
+    g.d0   = grid->dim[0];
+    g.d1   = grid->dim[1];
+    g.d2   = grid->dim[2];
+    g.bc0  = grid->bc[0] == 1;
+    g.bc1  = grid->bc[1] == 1;
+    g.bc2  = grid->bc[2] == 1;
+    g.size = (int64_t)g.d0 * g.d1 * g.d2;
+    MGX_REQUIRE(ctx, ld >= g.size && ldo >= g.size, "leading dimension smaller than numpt");
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, grid->dim[i] >= 2, "grid too small");
+    const int64_t total = (int64_t)nfunc * g.size;
+    dim3 grd(nblocks(total)), blk(kBlock);
+    if (type == MGX_LAP_4TH)
+    {
+        Coef4 c = coef4(grid->h);
+        if (dtype == MGX_F32)
+            hipLaunchKernelGGL(k_hloc4_packed<float>, grd, blk, 0, ctx->stream, (const float*)psi, ld, vtot,
+                (float*)hpsi, ldo, g, c, total);
+        else
+            hipLaunchKernelGGL(k_hloc4_packed<double>, grd, blk, 0, ctx->stream, (const double*)psi, ld, vtot,
+                (double*)hpsi, ldo, g, c, total);
+    }
+    else if (type == MGX_LAP_MEHRSTELLEN)
+    {
+        CoefM c = coefm(grid->h);
+        if (dtype == MGX_F32)
+            hipLaunchKernelGGL(k_hlocM_packed<float>, grd, blk, 0, ctx->stream, (const float*)psi, ld, vtot,
+                (float*)hpsi, ldo, g, c, total);
+        else
+            hipLaunchKernelGGL(k_hlocM_packed<double>, grd, blk, 0, ctx->stream, (const double*)psi, ld, vtot,
+                (double*)hpsi, ldo, g, c, total);
+    }
+    else
+        return set_err(ctx, MGX_EINVAL, "Hamiltonian::applyLocal supports lap types 0 and 2");
+    MGX_CHECK_LAUNCH(ctx);
+    return sync(ctx);
+}
+
+int mgx_precond_mg(mgx_ctx* ctx, int type, int max_levels, double gamma, int prec_dtype, const mgx_grid_t* grid,
+    const double* res, int64_t ld, int nfunc, const uint8_t* const* masks, double* out, int64_t ldo)
+{
+    MGX_REQUIRE(ctx, grid && res && out && nfunc > 0, "bad precond arguments");
+    MGX_REQUIRE(ctx, gamma > 0., "gamma must be positive");
+    const int64_t numpt = (int64_t)grid->dim[0] * grid->dim[1] * grid->dim[2];
+    MGX_REQUIRE(ctx, ld >= numpt && ldo >= numpt, "leading dimension smaller than numpt");
+    int rc = prec_dtype == MGX_F32
+                 ? precond_run<float>(ctx, type, max_levels, gamma, *grid, res, ld, nfunc, masks, out, ldo)
+                 : precond_run<double>(ctx, type, max_levels, gamma, *grid, res, ld, nfunc, masks, out, ldo);
+    return rc ? rc : sync(ctx);
+}
+
+}  // extern "C"

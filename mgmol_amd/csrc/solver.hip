@@ -1,0 +1,282 @@
+// solver.hip -- the orbital iteration on a box set, host C++ over the
+// device kernels.  One step follows ABPG<T>::updateWF (src/ABPG.cc:41-68)
+// inside DFTsolver::solve (src/DFTsolver.cc:262-401):
+//
+//   hpsi = H psi                        Hamiltonian::applyLocal
+//   S, H = psi^T psi, psi^T H psi       computeGram / addHlocalij
+//   L, S^-1 = chol / inverse            GramMatrix::updateLS / computeInverse
+//   theta = S^-1 H, E = tr(H S^-1)      updateTheta / getEigSum
+//   R = psi theta - hpsi, |R|           computeResidualUsingHPhi / ConstraintResidual
+//   stop if |R| < tol                   ABPG::updateWF early return
+//   psi += 1/2 step K(R)                precond_mg + update_states
+//   psi_j /= |psi_j|                    LocGridOrbitals::normalize
+//
+// S and H are formed in one pass over psi and hpsi (S of the normalised psi
+// equals the S the reference recomputes after normalize()).
+#include "common.hpp"
+
+#include <chrono>
+#include <cmath>
+
+namespace mgx
+{
+int box_hloc(mgx_ctx*, const mgx_boxset*, int, const void*, const double*, void*);
+int box_gram_h(mgx_ctx*, const mgx_boxset*, int, const double*, const double*, double*, double*);
+int box_residual(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, double*);
+int box_sumsq(mgx_ctx*, const mgx_boxset*, const double*, double*);
+int box_norm(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
+int box_normalize(mgx_ctx*, const mgx_boxset*, double*, double*);
+int box_precond(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, double*, double*, double);
+int64_t box_size(const mgx_boxset*, int);
+void boxset_geom(const mgx_boxset*, int gdim[3], double h[3], int* nbox);
+int chol_inv(mgx_ctx*, int, const double*, double*, double*);
+int theta(mgx_ctx*, int, const double*, const double*, double*);
+int trace_prod(mgx_ctx*, int, const double*, const double*, double*);
+int ritz(mgx_ctx*, int, const double*, const double*, double*, double*);
+}  // namespace mgx
+
+struct mgx_solver
+{
+    mgx_ctx* ctx          = nullptr;
+    const mgx_boxset* bs  = nullptr;
+    mgx_solver_cfg_t cfg{};
+    int n          = 0;
+    int64_t npts   = 0;
+    double h[3]    = { 0, 0, 0 };
+    double vmax    = 0.;
+    double lowest  = 0.;
+    int iter       = 0;
+    double *psi = nullptr, *psi0 = nullptr, *hpsi = nullptr, *res = nullptr, *vtot = nullptr;
+    MgxProf prof;
+    double *S = nullptr, *H = nullptr, *L = nullptr, *Sinv = nullptr, *th = nullptr;
+    double *eig = nullptr, *d = nullptr, *scal = nullptr;
+    double* host = nullptr;  // pinned: [energy, norm]
+};
+
+namespace mgx
+{
+namespace
+{
+
+void free_solver(mgx_solver* s)
+{
+    double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
+        s->scal };
+    for (double* b : bufs)
+        if (b) hipFree(b);
+    if (s->host) hipHostFree(s->host);
+    for (hipEvent_t e : s->prof.pool)
+        hipEventDestroy(e);
+    s->prof.pool.clear();
+}
+
+int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
+{
+    mgx_ctx* ctx = s->ctx;
+    int rc, tk;
+    const int n = s->n;
+    tk = prof_begin(ctx, MGX_PROF_HLOC);
+    if ((rc = box_hloc(ctx, s->bs, MGX_F64, s->psi, s->vtot, s->hpsi))) return rc;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_GRAM);
+    if ((rc = box_gram_h(ctx, s->bs, s->cfg.gram_prec, s->psi, s->hpsi, s->S, s->H))) return rc;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_DENSE);
+    if ((rc = chol_inv(ctx, n, s->S, s->L, s->Sinv))) return rc;
+    if ((rc = theta(ctx, n, s->Sinv, s->H, s->th))) return rc;
+    if ((rc = trace_prod(ctx, n, s->H, s->Sinv, s->scal))) return rc;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_RESID);
+    if ((rc = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res))) return rc;
+    if ((rc = box_sumsq(ctx, s->bs, s->res, s->d))) return rc;
+    if ((rc = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return rc;
+    prof_end(ctx, tk);
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double e_ry = s->host[0], rnorm = s->host[1];
+    if (!std::isfinite(e_ry) || !std::isfinite(rnorm))
+        return set_err(ctx, MGX_ENUMERIC, "non-finite energy or residual");
+    st->band_energy = 0.5 * e_ry;  // Ry -> Ha
+    st->residual    = rnorm;
+    *done           = (tol > 0. && rnorm < tol);
+    if (*done) return MGX_OK;
+    double gamma = 0.;
+    if (s->cfg.precond_start == MGX_START_GAMMA)
+    {
+        // lowest eigenvalue of (H, S) for gamma (ProjectedMatrices::getLowestEigenvalue)
+        if (s->iter == 0 || (s->cfg.ritz_every > 0 && s->iter % s->cfg.ritz_every == 0))
+        {
+            tk = prof_begin(ctx, MGX_PROF_DENSE);
+            if ((rc = ritz(ctx, n, s->H, s->S, s->eig, nullptr))) return rc;
+            prof_end(ctx, tk);
+            MGX_CHECK_HIP(ctx,
+                hipMemcpyAsync(s->host + 2, s->eig, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            s->lowest = s->host[2];
+        }
+        gamma = mgx_gamma(s->h, s->cfg.mg_levels, s->vmax, s->lowest);
+    }
+    st->lowest_eig     = s->lowest;
+    const double alpha = 0.5 * s->cfg.step_length;
+    tk = prof_begin(ctx, MGX_PROF_PRECOND);
+    if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, s->cfg.precond_dtype, s->res, nullptr,
+             s->psi, alpha)))
+        return rc;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
+    if ((rc = box_normalize(ctx, s->bs, s->psi, s->d))) return rc;
+    prof_end(ctx, tk);
+    s->iter++;
+    return MGX_OK;
+}
+
+int step_impl(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
+{
+    mgx_ctx* ctx = s->ctx;
+    ctx->prof    = s->cfg.profile ? &s->prof : nullptr;
+    int rc       = step_body(s, tol, st, done);
+    prof_collect(ctx);
+    ctx->prof = nullptr;
+    return rc;
+}
+
+}  // namespace
+}  // namespace mgx
+
+using namespace mgx;
+
+extern "C" {
+
+int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t* cfg, const double* vtot_host,
+    const double* psi0_host, mgx_solver** out)
+{
+    MGX_REQUIRE(ctx, bs && cfg && vtot_host && psi0_host && out, "bad solver arguments");
+    MGX_REQUIRE(ctx, cfg->lap_type == MGX_LAP_4TH, "solver supports the 4th-order operator");
+    MGX_REQUIRE(ctx, cfg->precond_dtype == MGX_F32 || cfg->precond_dtype == MGX_F64, "bad precond dtype");
+    MGX_REQUIRE(ctx, cfg->gram_prec == MGX_PREC_F64 || cfg->gram_prec == MGX_PREC_SPLIT, "bad gram precision");
+    MGX_REQUIRE(ctx, cfg->precond_start == MGX_START_GAMMA || cfg->precond_start == MGX_START_ZERO,
+        "bad precond start");
+    MGX_REQUIRE(ctx, cfg->step_length > 0., "step length must be positive");
+    int64_t dummy = 0;
+    MGX_REQUIRE(ctx, mgx_boxset_size(bs, cfg->mg_levels, &dummy) == MGX_OK, "box set has too few levels");
+    mgx_solver* s = new mgx_solver();
+    s->ctx        = ctx;
+    s->bs         = bs;
+    s->cfg        = *cfg;
+    mgx_boxset_size(bs, 0, &s->npts);
+    int g[3];
+    boxset_geom(bs, g, s->h, &s->n);
+    const int64_t gsize = (int64_t)g[0] * g[1] * g[2];
+    const size_t nn     = (size_t)s->n * s->n;
+    auto alloc = [&](double** p, size_t cnt) { return hipMalloc((void**)p, cnt * sizeof(double)) == hipSuccess; };
+    bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
+              && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
+              && alloc(&s->eig, s->n) && alloc(&s->d, s->n + 8) && alloc(&s->scal, 8)
+              && hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    if (!ok)
+    {
+        free_solver(s);
+        delete s;
+        return set_err(ctx, MGX_ENOMEM, "solver allocation failed");
+    }
+    double vmax = vtot_host[0];
+    for (int64_t i = 1; i < gsize; i++)
+        vmax = std::max(vmax, vtot_host[i]);
+    s->vmax = vmax;
+    hipMemcpy(s->vtot, vtot_host, gsize * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(s->psi0, psi0_host, s->npts * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(s->psi, s->psi0, s->npts * sizeof(double), hipMemcpyDeviceToDevice);
+    if (hipDeviceSynchronize() != hipSuccess)
+    {
+        free_solver(s);
+        delete s;
+        return set_err(ctx, MGX_EHIP, "solver upload failed");
+    }
+    *out = s;
+    return MGX_OK;
+}
+
+int mgx_solver_destroy(mgx_solver* s)
+{
+    if (!s) return MGX_OK;
+    hipStreamSynchronize(s->ctx->stream);
+    free_solver(s);
+    delete s;
+    return MGX_OK;
+}
+
+int mgx_solver_step(mgx_solver* s, mgx_solver_stats_t* st)
+{
+    if (!s || !st) return MGX_EINVAL;
+    bool done = false;
+    auto t0   = std::chrono::steady_clock::now();
+    int rc    = step_impl(s, -1., st, &done);
+    st->seconds    = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->iterations = s->iter;
+    st->converged  = 0;
+    return rc;
+}
+
+int mgx_solver_run(mgx_solver* s, double tol, int maxit, mgx_solver_stats_t* st)
+{
+    if (!s || !st || maxit < 1) return MGX_EINVAL;
+    auto t0 = std::chrono::steady_clock::now();
+    bool done = false;
+    int rc    = MGX_OK;
+    int it0   = s->iter;
+    for (int k = 0; k < maxit && !done; k++)
+    {
+        rc = step_impl(s, tol, st, &done);
+        if (rc) break;
+    }
+    st->seconds    = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    st->iterations = s->iter - it0;
+    st->converged  = done ? 1 : 0;
+    return rc;
+}
+
+int mgx_solver_reset(mgx_solver* s)
+{
+    if (!s) return MGX_EINVAL;
+    MGX_CHECK_HIP(s->ctx, hipMemcpyAsync(s->psi, s->psi0, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
+                              s->ctx->stream));
+    MGX_CHECK_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    s->iter   = 0;
+    s->lowest = 0.;
+    return MGX_OK;
+}
+
+int mgx_solver_profile(mgx_solver* s, double* ms, int64_t* launches, int reset)
+{
+    if (!s) return MGX_EINVAL;
+    for (int g = 0; g < MGX_PROF_N; g++)
+    {
+        if (ms) ms[g] = s->prof.ms[g];
+        if (launches) launches[g] = s->prof.cnt[g];
+        if (reset)
+        {
+            s->prof.ms[g]  = 0.;
+            s->prof.cnt[g] = 0;
+        }
+    }
+    return MGX_OK;
+}
+
+int mgx_solver_get_psi(mgx_solver* s, double* host)
+{
+    if (!s || !host) return MGX_EINVAL;
+    MGX_CHECK_HIP(s->ctx, hipMemcpy(host, s->psi, s->npts * sizeof(double), hipMemcpyDeviceToHost));
+    return MGX_OK;
+}
+
+int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H)
+{
+    if (!s) return MGX_EINVAL;
+    if (psi) *psi = s->psi;
+    if (hpsi) *hpsi = s->hpsi;
+    if (S) *S = s->S;
+    if (H) *H = s->H;
+    return MGX_OK;
+}
+
+}  // extern "C"

@@ -493,3 +493,28 @@ def b_extract(b, full, level=0):
         iz = (np.arange(e) + o[2]) % gd[2]
         out[off[j]:off[j + 1]] = full[j][np.ix_(ix, iy, iz)].reshape(-1)
     return out
+
+
+def b_solver_step(b, psi, vtot, mg_levels=2, prec=F64, step_length=1.5, lowest=None, zero_start=False):
+    """One orbital step on boxes, composed as ABPG::updateWF + DFTsolver
+    (src/ABPG.cc:41-140, src/DFTsolver.cc:330-363): returns
+    (new_psi, band_energy_Ha, residual_Ha, lowest_ritz_Ry).  zero_start=True
+    starts the fine-level V-cycle from 0 instead of gamma*R (the coarse levels
+    of Preconditioning::mg always start from 0)."""
+    hp = b_hloc(b, vtot, psi)
+    S, H = b_gram_h(b, psi, hp)
+    _, Sinv = chol_inv(S)
+    th = theta(Sinv, H)
+    e_ry = eigsum(H, Sinv)
+    R = b_residual(b, psi, hp, th)
+    rn = residual_norm(b_diag_dots(b, R), Sinv)
+    if lowest is None:
+        lowest = float(ritz(H, S)[0][0])
+    g = 0.0 if zero_start else gamma(list(b.h), mg_levels, float(np.max(vtot)), lowest)
+    KR = b_precond(b, 2, mg_levels, g, R, dtype=prec)
+    new = psi + 0.5 * step_length * KR
+    d = b_diag_dots(b, new)
+    off = b.offsets(0)
+    for j in range(b.nbox):
+        new[off[j]:off[j + 1]] *= 1.0 / np.sqrt(d[j])
+    return new, 0.5 * e_ry, rn, lowest

@@ -863,7 +863,7 @@ int b_check_levels(const or_boxes_t* bb, int max_levels)
         for (int d = 0; d < 3; d++)
         {
             if (bb->origin[3 * j + d] % m) return -1;
-            if (2 * bb->edge[j] > bb->gdim[d]) return -1;
+            if (2 * bb->edge[j] + 4 > bb->gdim[d]) return -1;
         }
     }
     for (int d = 0; d < 3; d++)
@@ -940,6 +940,90 @@ bool box_overlap(const or_boxes_t* bb, int i, int j, Ovl* o)
         if (!overlap1d(bb->origin[3 * i + d], bb->edge[i], bb->origin[3 * j + d], bb->edge[j],
                 bb->gdim[d], &o->a[d], &o->b[d], &o->sh[d]))
             return false;
+    return true;
+}
+
+// Stencil part of (H psi_j)(r) at a point r outside box j (j-local coords):
+// only the axis-aligned 4th-order legs that land inside the box contribute
+// (the potential term vanishes there because psi_j(r) = 0).
+double skin_value(const double* pj, int ej, const C4& c, int x, int y, int z)
+{
+    int cnt_out = (x < 0 || x >= ej) + (y < 0 || y >= ej) + (z < 0 || z >= ej);
+    if (cnt_out != 1) return 0.;
+    double v = 0.;
+    auto leg = [&](int ax, int ay, int az, double cf) {
+        if (ax < 0 || ay < 0 || az < 0 || ax >= ej || ay >= ej || az >= ej) return;
+        v += cf * pj[((long)ax * ej + ay) * ej + az];
+    };
+    leg(x - 1, y, z, c.c1x);
+    leg(x + 1, y, z, c.c1x);
+    leg(x - 2, y, z, c.c2x);
+    leg(x + 2, y, z, c.c2x);
+    leg(x, y - 1, z, c.c1y);
+    leg(x, y + 1, z, c.c1y);
+    leg(x, y - 2, z, c.c2y);
+    leg(x, y + 2, z, c.c2y);
+    leg(x, y, z - 1, c.c1z);
+    leg(x, y, z + 1, c.c1z);
+    leg(x, y, z - 2, c.c2z);
+    leg(x, y, z + 2, c.c2z);
+    return v;
+}
+
+// sum over r in box i, r outside box j, of psi_i(r) (H psi_j)(r);
+// sh maps i-local -> j-local coordinates
+double skin_sum(const double* pi, int ei, const double* pj, int ej, const int sh[3], const C4& c)
+{
+    double acc = 0.;
+    for (int x = 0; x < ei; x++)
+    {
+        const int X = x + sh[0];
+        if (X < -2 || X > ej + 1) continue;
+        for (int y = 0; y < ei; y++)
+        {
+            const int Y = y + sh[1];
+            if (Y < -2 || Y > ej + 1) continue;
+            for (int z = 0; z < ei; z++)
+            {
+                const int Z = z + sh[2];
+                if (Z < -2 || Z > ej + 1) continue;
+                const bool inside = X >= 0 && Y >= 0 && Z >= 0 && X < ej && Y < ej && Z < ej;
+                if (inside) continue;
+                const double hv = skin_value(pj, ej, c, X, Y, Z);
+                if (hv != 0.) acc += pi[((long)x * ei + y) * ei + z] * hv;
+            }
+        }
+    }
+    return acc;
+}
+
+// Periodic shift (i-local -> j-local) for boxes whose extended regions (2
+// points around box j) meet box i; false if they do not interact.
+bool ext_shift(const or_boxes_t* b, int i, int j, int sh[3])
+{
+    for (int d = 0; d < 3; d++)
+    {
+        const int G = b->gdim[d];
+        const int s = wrap(b->origin[3 * j + d] - b->origin[3 * i + d], G);
+        // j-local X = x - s (mod G); pick the image closest to the box
+        int best = 1 << 30, bsh = 0;
+        for (int img = -1; img <= 1; img++)
+        {
+            const int shift = -s + img * G;  // X = x + shift
+            const int lo = shift, hi = b->edge[i] - 1 + shift;  // range of X
+            // distance of [lo,hi] from [-2, ej+1]
+            int dist = 0;
+            if (hi < -2) dist = -2 - hi;
+            else if (lo > b->edge[j] + 1) dist = lo - (b->edge[j] + 1);
+            if (dist < best)
+            {
+                best = dist;
+                bsh  = shift;
+            }
+        }
+        if (best > 0) return false;
+        sh[d] = bsh;
+    }
     return true;
 }
 
@@ -1262,8 +1346,8 @@ long or_b_pairs(const or_boxes_t* b, long cap, int* pi, int* pj)
     for (int i = 0; i < b->nbox; i++)
         for (int j = i; j < b->nbox; j++)
         {
-            Ovl o;
-            if (!box_overlap(b, i, j, &o)) continue;
+            int s1[3], s2[3];
+            if (!ext_shift(b, i, j, s1) && !ext_shift(b, j, i, s2)) continue;
             if (cnt < cap)
             {
                 pi[cnt] = i;
@@ -1274,35 +1358,48 @@ long or_b_pairs(const or_boxes_t* b, long cap, int* pi, int* pj)
     return cnt;
 }
 
+// S_ij = h^3 <psi_i, psi_j>; H_ij = h^3 <psi_i, H psi_j> (Galerkin, as
+// LocGridOrbitals::computeLocalProduct forms it from the unmasked H psi,
+// src/LocGridOrbitals.cc:1554-1604), from the stored (masked) H psi on the
+// overlap plus the stencil's reach 2 points outside each box; the two
+// one-sided evaluations are averaged so that H is exactly symmetric.
 int or_b_gram_h(const or_boxes_t* b, const double* psi, const double* hpsi, double* S, double* H)
 {
     Boxes B(b, 1);
     const int n    = b->nbox;
     const double vel = b->h[0] * b->h[1] * b->h[2];
+    C4 c4(b->h);
     std::memset(S, 0, sizeof(double) * n * n);
     std::memset(H, 0, sizeof(double) * n * n);
 #pragma omp parallel for schedule(dynamic)
     for (int i = 0; i < n; i++)
         for (int j = i; j < n; j++)
         {
-            Ovl o;
-            if (!box_overlap(b, i, j, &o)) continue;
             const int ei = b->edge[i], ej = b->edge[j];
             const double* pi  = psi + B.off[0][i];
             const double* pj  = psi + B.off[0][j];
             const double* hi  = hpsi + B.off[0][i];
             const double* hj  = hpsi + B.off[0][j];
             double s = 0., h1 = 0., h2 = 0.;
-            for (int x = o.a[0]; x < o.b[0]; x++)
-                for (int y = o.a[1]; y < o.b[1]; y++)
-                    for (int z = o.a[2]; z < o.b[2]; z++)
-                    {
-                        long ii = ((long)x * ei + y) * ei + z;
-                        long jj = ((long)(x + o.sh[0]) * ej + (y + o.sh[1])) * ej + (z + o.sh[2]);
-                        s += pi[ii] * pj[jj];
-                        h1 += pi[ii] * hj[jj];
-                        h2 += hi[ii] * pj[jj];
-                    }
+            Ovl o;
+            const bool ov = box_overlap(b, i, j, &o);
+            if (ov)
+                for (int x = o.a[0]; x < o.b[0]; x++)
+                    for (int y = o.a[1]; y < o.b[1]; y++)
+                        for (int z = o.a[2]; z < o.b[2]; z++)
+                        {
+                            long ii = ((long)x * ei + y) * ei + z;
+                            long jj = ((long)(x + o.sh[0]) * ej + (y + o.sh[1])) * ej + (z + o.sh[2]);
+                            s += pi[ii] * pj[jj];
+                            h1 += pi[ii] * hj[jj];
+                            h2 += hi[ii] * pj[jj];
+                        }
+            int sij[3], sji[3];
+            const bool ex1 = ext_shift(b, i, j, sij);
+            const bool ex2 = ext_shift(b, j, i, sji);
+            if (!ov && !ex1 && !ex2) continue;
+            if (ex1) h1 += skin_sum(pi, ei, pj, ej, sij, c4);
+            if (ex2) h2 += skin_sum(pj, ej, pi, ei, sji, c4);
             S[(long)j * n + i] = S[(long)i * n + j] = s * vel;
             H[(long)j * n + i] = H[(long)i * n + j] = 0.5 * (h1 + h2) * vel;
         }

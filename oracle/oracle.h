@@ -42,7 +42,7 @@ typedef struct
     int gdim[3];           /* global grid, level 0 */
     double h[3];           /* spacing, level 0 */
     const int* origin;     /* 3*nbox, level-0 indices, multiples of 2^(L-1) */
-    const int* edge;       /* nbox, multiples of 2^(L-1) */
+    const int* edge;       /* nbox, multiples of 2^(L-1), 2*edge+4 <= gdim */
 } or_boxes_t;
 
 /* ---------------- grid family (reference layout, ghosted) ---------------- */
@@ -132,10 +132,12 @@ int or_b_mg_f64(const or_boxes_t* b, int lap_type, int max_levels, double* v, co
 int or_b_precond_f32(const or_boxes_t* b, int lap_type, int max_levels, double gamma, const double* res, double* out);
 int or_b_precond_f64(const or_boxes_t* b, int lap_type, int max_levels, double gamma, const double* res, double* out);
 
-/* overlap pair list: pairs (i<=j) whose boxes intersect; returns count, fills
+/* interaction pair list: pairs (i<=j) whose boxes intersect or lie within the
+ * stencil reach (2 points); returns count, fills
  * up to cap entries of pi/pj (pass cap=0 to count) */
 long or_b_pairs(const or_boxes_t* b, long cap, int* pi, int* pj);
-/* S_ij = h^3 sum psi_i psi_j,  H_ij = h^3/2 sum (psi_i hpsi_j + hpsi_i psi_j), dense n x n */
+/* S_ij = h^3 <psi_i, psi_j>, H_ij = h^3 <psi_i, H psi_j> (Galerkin: masked H psi on
+ * the overlap plus the stencil reach outside each box), dense n x n */
 int or_b_gram_h(const or_boxes_t* b, const double* psi, const double* hpsi, double* S, double* H);
 /* R_j = sum_k psi_k theta_kj - hpsi_j on box j (theta column-major n x n) */
 int or_b_residual(const or_boxes_t* b, const double* psi, const double* hpsi, const double* theta, double* res);

@@ -1,0 +1,137 @@
+"""GPU parity of the dense layer: MFMA Gram / local products (FP64 and
+hi/lo split), slab accumulation, Cholesky / inverse, theta, trace, Ritz,
+residual and its norm -- against the oracle / numpy, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+from mgmol_amd import mgx
+from tests.gpu_util import relmax, say, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tests.gpu_util import ctx as shared
+    return shared()
+
+
+def test_mfma_single_tile_layout(ctx):
+    """One 16x16 output from 4 rows: checks the MFMA operand / result lane
+    layout (any transposition or row mix-up fails)."""
+    rng = np.random.default_rng(0)
+    psi = rng.standard_normal((16, 4))
+    tp, ts = to_dev(psi), to_dev(np.zeros((1, 16, 16)))
+    ctx.call("mgx_local_product", mgx.PREC_F64, mgx.ptr(tp), mgx.ptr(tp), 4, 4, 16, 1, 1.0, mgx.ptr(ts))
+    got = to_np(ts)[0]
+    # column-major: got[j, i] = <psi_i, psi_j>
+    assert np.allclose(got.T, psi @ psi.T, rtol=1e-14, atol=1e-14)
+    b = rng.standard_normal((16, 4))
+    tb = to_dev(b)
+    ctx.call("mgx_local_product", mgx.PREC_F64, mgx.ptr(tp), mgx.ptr(tb), 4, 4, 16, 1, 1.0, mgx.ptr(ts))
+    got = to_np(ts)[0]
+    assert np.allclose(got.T, psi @ b.T, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("n,ld,lnp,sub", [(5, 1000, 500, 2), (64, 4096, 4096, 1), (37, 3000, 1000, 3)])
+def test_local_gram_and_product_f64(ctx, n, ld, lnp, sub):
+    rng = np.random.default_rng(n)
+    psi = rng.standard_normal((n, ld))
+    hp = rng.standard_normal((n, ld))
+    vel = 0.027
+    tp, th = to_dev(psi), to_dev(hp)
+    ts = to_dev(np.zeros((sub, n, n)))
+    ctx.call("mgx_local_gram", mgx.PREC_F64, mgx.ptr(tp), ld, lnp, n, sub, vel, mgx.ptr(ts))
+    ref = O.local_gram(psi, lnp, sub, vel)
+    g = to_np(ts)
+    assert relmax(g, ref) <= 1e-13
+    assert np.array_equal(g, np.transpose(g, (0, 2, 1)))
+    ctx.call("mgx_local_product", mgx.PREC_F64, mgx.ptr(tp), mgx.ptr(th), ld, lnp, n, sub, vel, mgx.ptr(ts))
+    assert relmax(to_np(ts), O.local_product(psi, hp, lnp, sub, vel)) <= 1e-13
+
+
+def test_local_gram_split_error(ctx):
+    """hi/lo split on FP32 MFMA: error vs FP64 stated and bounded."""
+    rng = np.random.default_rng(11)
+    n, ld = 64, 200000
+    psi = rng.standard_normal((n, ld)) * 1e-2
+    tp = to_dev(psi)
+    ts = to_dev(np.zeros((1, n, n)))
+    ctx.call("mgx_local_gram", mgx.PREC_SPLIT, mgx.ptr(tp), ld, ld, n, 1, 1.0, mgx.ptr(ts))
+    ref = O.local_gram(psi, ld, 1, 1.0)
+    err = relmax(to_np(ts), ref)
+    say("split gram rel err (max-norm)", err)
+    assert err < 1e-9
+
+
+def test_local_to_global(ctx):
+    rng = np.random.default_rng(12)
+    sub, m, numst = 3, 4, 7
+    ss = rng.standard_normal((sub, m, m))
+    ss[0, 1, 2] = 1e-16
+    gids = np.array([[0, 3, -1, 5], [1, 3, 6, -1], [0, 2, 4, 5]], dtype=np.int32)
+    tss, tg = to_dev(ss), to_dev(gids)
+    tglob = to_dev(np.zeros((numst, numst)))
+    ctx.call("mgx_local_to_global", mgx.ptr(tss), mgx.ptr(tg), sub, m, numst, 1e-14, mgx.ptr(tglob))
+    # oracle returns [row, col] with column-major storage mapped to numpy order
+    ref = O.local_to_global(ss, gids, numst)
+    assert relmax(to_np(tglob), ref) <= 1e-15
+
+
+def test_chol_inv_theta_eigsum_ritz(ctx):
+    """tests/testGramMatrix.cc identities plus tr(S^-1 H) = sum of Ritz values."""
+    n = 213
+    rng = np.random.default_rng(13)
+    A = rng.uniform(-1.0 / n, 1.0 / n, (n, n))
+    B = 0.5 * (A + A.T)
+    np.fill_diagonal(B, 2.0)
+    H = rng.standard_normal((n, n))
+    H = H + H.T
+    tS, tL, tI = to_dev(B), to_dev(np.zeros_like(B)), to_dev(np.zeros_like(B))
+    ctx.call("mgx_chol_inv", n, mgx.ptr(tS), mgx.ptr(tL), mgx.ptr(tI))
+    L = to_np(tL).T  # column-major -> numpy row-major lower
+    assert np.max(np.abs(L @ L.T - B)) < 1e-14
+    assert np.allclose(np.triu(L, 1), 0.0)
+    Sinv = to_np(tI).T
+    assert np.max(np.abs(Sinv @ B - np.eye(n))) < 1e-13
+    tH, tT = to_dev(np.asfortranarray(H).T.copy()), to_dev(np.zeros_like(H))
+    ctx.call("mgx_theta", n, mgx.ptr(tI), mgx.ptr(tH), mgx.ptr(tT))
+    th = to_np(tT).T
+    assert relmax(th, Sinv @ H) <= 1e-13
+    import ctypes
+    tr = ctypes.c_double()
+    ctx.call("mgx_eigsum", n, mgx.ptr(tH), mgx.ptr(tI), ctypes.byref(tr))
+    w, _ = O.ritz(H, B)
+    assert tr.value == pytest.approx(np.sum(w), rel=1e-12)
+    te = to_dev(np.zeros(n))
+    ctx.call("mgx_ritz", n, mgx.ptr(tH), mgx.ptr(tS), mgx.ptr(te), None)
+    assert relmax(to_np(te), w) <= 1e-12
+
+
+def test_chol_not_spd(ctx):
+    n = 8
+    S = -np.eye(n)
+    tS, tL, tI = to_dev(S), to_dev(np.zeros_like(S)), to_dev(np.zeros_like(S))
+    rc = mgx.lib().mgx_chol_inv(ctx.h, n, mgx.ptr(tS), mgx.ptr(tL), mgx.ptr(tI))
+    assert rc == mgx.MGX_ENOTSPD
+
+
+def test_residual_and_norm(ctx):
+    rng = np.random.default_rng(14)
+    n, ld = 6, 4000
+    psi = rng.standard_normal((n, ld))
+    hp = rng.standard_normal((n, ld))
+    th = rng.standard_normal((n, n))
+    tp, th_d, tt = to_dev(psi), to_dev(hp), to_dev(np.asfortranarray(th).T.copy())
+    tr = to_dev(np.zeros_like(psi))
+    ctx.call("mgx_residual", mgx.ptr(tp), mgx.ptr(th_d), mgx.ptr(tt), ld, ld, n, mgx.ptr(tr))
+    ref = O.residual(psi, hp, th)
+    assert relmax(to_np(tr), ref) <= 1e-13
+    Sinv = np.diag(rng.uniform(0.5, 1.5, n))
+    tI = to_dev(Sinv)
+    import ctypes
+    out = ctypes.c_double()
+    ctx.call("mgx_residual_norm", mgx.ptr(tr), ld, ld, n, 0.01, mgx.ptr(tI), ctypes.byref(out))
+    d = 0.01 * np.sum(ref * ref, axis=1)
+    assert out.value == pytest.approx(O.residual_norm(d, Sinv), rel=1e-13)

@@ -1,0 +1,204 @@
+"""GPU parity of the grid-family entry points (ghosted blocks, packed H psi,
+preconditioner) against the oracle, through the C ABI.
+
+Bar: FP64 agrees to 1e-12 relative (max-norm); the transfer operators and
+the halo are bit-exact; FP32 agrees to 2e-6 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mgmol_amd import mgx
+from tests.gpu_util import relmax, say, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+DT = {"f64": np.float64, "f32": np.float32}
+TOL = {"f64": 1e-12, "f32": 2e-6}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tests.gpu_util import ctx as shared
+    return shared()
+
+
+def _grids(dims=(16, 12, 20), ng=2, ll=2.4, bc=(1, 1, 1)):
+    h = [ll / d for d in dims]
+    return O.Grid.make(dims, ng, h), mgx.Grid.make(dims, ng, h, bc)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("bc", [(1, 1, 1), (1, 0, 1)])
+def test_halo_bit_exact(ctx, dt, bc):
+    say("halo", dt, bc)
+    og, mg_ = _grids(bc=bc)
+    rng = np.random.default_rng(1)
+    blk = rng.standard_normal((3,) + og.shape_g).astype(DT[dt])
+    t = to_dev(blk)
+    ctx.call("mgx_halo", mgx.block(t, mg_))
+    ref = O.trade(og, blk.copy(), bc=bc)
+    assert np.array_equal(to_np(t), ref)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_lap_apply(ctx, dt, kind):
+    say("lap", dt, kind)
+    og, mg_ = _grids()
+    rng = np.random.default_rng(2)
+    blk = O.trade(og, O.embed(og, rng.standard_normal((4,) + tuple(og.dim)).astype(DT[dt])))
+    ti, to = to_dev(blk), to_dev(np.zeros_like(blk))
+    ctx.call("mgx_lap_apply", kind, mgx.block(ti, mg_), mgx.block(to, mg_))
+    got = O.interior(og, to_np(to))
+    ref = O.interior(og, O.lap(og, kind, blk))
+    err = relmax(got, ref)
+    say("  rel err", err)
+    assert err <= TOL[dt]
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("rg", [0, 1, 2])
+def test_rhs_apply(ctx, dt, rg):
+    og, mg_ = _grids()
+    rng = np.random.default_rng(3)
+    blk = O.trade(og, O.embed(og, rng.standard_normal((2,) + tuple(og.dim)).astype(DT[dt])))
+    ref = O.rhs_mehr(og, blk, rg)
+    go = mgx.Grid.make(og.dim, rg, list(og.h))
+    ti, to = to_dev(blk), to_dev(np.zeros_like(ref))
+    ctx.call("mgx_rhs_apply", mgx.block(ti, mg_), mgx.block(to, go))
+    assert relmax(to_np(to), ref) <= TOL[dt]
+
+
+def test_pointwise_axpy_mask(ctx):
+    og, mg_ = _grids()
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((2,) + og.shape_g)
+    v = rng.standard_normal(og.shape_g)
+    ta, tv, to = to_dev(a), to_dev(v), to_dev(np.zeros_like(a))
+    ctx.call("mgx_pointwise_product", mgx.block(ta, mg_), mgx.ptr(tv), mgx.block(to, mg_))
+    assert np.array_equal(to_np(to), O.pointwise(og, a, v))
+    for dt in (np.float32, np.float64):
+        x = rng.standard_normal(1000).astype(dt)
+        y = rng.standard_normal(1000).astype(dt)
+        tx, ty = to_dev(x), to_dev(y)
+        ctx.call("mgx_axpy", mgx.F32 if dt == np.float32 else mgx.F64, 1000, -0.37, mgx.ptr(tx), mgx.ptr(ty))
+        assert np.array_equal(to_np(ty), O.axpy(-0.37, x, y.copy()))
+    m = (rng.random((2, og.size)) > 0.5).astype(np.uint8)
+    tm = to_dev(m)
+    ta2 = to_dev(a)
+    ctx.call("mgx_mask", mgx.block(ta2, mg_), mgx.ptr(tm))
+    assert np.array_equal(to_np(ta2), O.mask(og, a.copy(), m))
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("kind", [1, 2])
+def test_jacobi(ctx, dt, kind):
+    og, mg_ = _grids()
+    rng = np.random.default_rng(5)
+    f = O.embed(og, rng.standard_normal((2,) + tuple(og.dim)).astype(DT[dt]))
+    x = O.embed(og, rng.standard_normal((2,) + tuple(og.dim)).astype(DT[dt]))
+    w = np.zeros_like(x)
+    jf = O.jacobi_factor(kind, list(og.h))
+    tf, tx, tw = to_dev(f), to_dev(x), to_dev(w)
+    ctx.call("mgx_jacobi", kind, jf, mgx.block(tf, mg_), mgx.block(tx, mg_), mgx.block(tw, mg_))
+    xr, wr = x.copy(), w.copy()
+    O.jacobi(og, kind, f, xr, wr, jf)
+    assert relmax(O.interior(og, to_np(tx)), O.interior(og, xr)) <= TOL[dt]
+    assert relmax(O.interior(og, to_np(tw)), O.interior(og, wr)) <= TOL[dt]
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_restrict_prolong_bit_exact(ctx, dt):
+    og, mg_ = _grids(dims=(16, 12, 20))
+    ogc = og.coarse()
+    mgc = mgx.Grid.make(ogc.dim, 2, list(ogc.h))
+    rng = np.random.default_rng(6)
+    fine = O.trade(og, O.embed(og, rng.standard_normal((3,) + tuple(og.dim)).astype(DT[dt])))
+    tf, tc = to_dev(fine), to_dev(np.zeros((3,) + ogc.shape_g, DT[dt]))
+    ctx.call("mgx_restrict", mgx.block(tf, mg_), mgx.block(tc, mgc))
+    ref = O.restrict(og, fine, ogc)
+    assert np.array_equal(O.interior(ogc, to_np(tc)), O.interior(ogc, ref))
+    coarse = O.trade(ogc, O.embed(ogc, rng.standard_normal((3,) + tuple(ogc.dim)).astype(DT[dt])))
+    tc2, tf2 = to_dev(coarse), to_dev(np.zeros((3,) + og.shape_g, DT[dt]))
+    ctx.call("mgx_prolong", mgx.block(tc2, mgc), mgx.block(tf2, mg_))
+    ref2 = O.extend(ogc, coarse, og)
+    assert np.array_equal(O.interior(og, to_np(tf2)), O.interior(og, ref2))
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("masked", [False, True])
+def test_mg_vcycle(ctx, dt, masked):
+    say("mg", dt, masked)
+    og, mg_ = _grids(dims=(16, 16, 16), ll=4.8)
+    rng = np.random.default_rng(7)
+    nf = 3
+    f = O.embed(og, rng.standard_normal((nf,) + tuple(og.dim)).astype(DT[dt]))
+    v = np.zeros_like(f)
+    O.axpy(0.02, f, v)
+    masks = None
+    if masked:
+        b = O.Boxes.make([[0, 4, 8], [8, 8, 0], [4, 0, 4]], [8, 8, 4], (16, 16, 16), list(og.h))
+        masks = [O.b_masks(b, l) for l in range(3)]
+    tv, tf = to_dev(v), to_dev(f)
+    tmasks = [to_dev(m) for m in masks] if masks else None
+    arr = None
+    if tmasks:
+        import ctypes
+        arr = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in tmasks])
+    ctx.call("mgx_mg", 2, 2, mgx.block(tv, mg_), mgx.block(tf, mg_), arr)
+    ref = O.mg(og, 2, 2, v.copy(), f, masks=masks)
+    err = relmax(O.interior(og, to_np(tv)), O.interior(og, ref))
+    say("  rel err", err)
+    assert err <= (1e-12 if dt == "f64" else 1e-5)
+
+
+@pytest.mark.parametrize("kind", [0, 2])
+def test_hloc_packed_f64(ctx, kind):
+    og, mg_ = _grids(dims=(16, 12, 20), ll=3.0)
+    rng = np.random.default_rng(8)
+    nf = 4
+    psi = rng.standard_normal((nf, og.size))
+    v = rng.standard_normal(og.size)
+    tp, tv, th = to_dev(psi), to_dev(v), to_dev(np.zeros_like(psi))
+    ctx.call("mgx_hloc_apply", kind, mg_, mgx.F64, mgx.ptr(tp), og.size, nf, mgx.ptr(tv), mgx.ptr(th), og.size)
+    blk = O.embed(og, psi.reshape((nf,) + tuple(og.dim)))
+    ref = O.hloc(og, kind, blk, v)
+    err = relmax(to_np(th), ref)
+    say("hloc", kind, err)
+    assert err <= 1e-12
+
+
+def test_hloc_packed_f32(ctx):
+    og, mg_ = _grids(dims=(16, 12, 20), ll=3.0)
+    rng = np.random.default_rng(9)
+    psi = rng.standard_normal((2, og.size)).astype(np.float32)
+    v = rng.standard_normal(og.size)
+    tp, tv, th = to_dev(psi), to_dev(v), to_dev(np.zeros_like(psi))
+    ctx.call("mgx_hloc_apply", 2, mg_, mgx.F32, mgx.ptr(tp), og.size, 2, mgx.ptr(tv), mgx.ptr(th), og.size)
+    blk = O.trade(og, O.embed(og, psi.reshape((2,) + tuple(og.dim))))
+    ref = O.lap4_withpot(og, blk, v)
+    assert relmax(to_np(th), ref) <= 2e-6
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_precond_mg(ctx, prec):
+    og, mg_ = _grids(dims=(16, 16, 16), ll=4.8)
+    rng = np.random.default_rng(10)
+    nf = 2
+    res = rng.standard_normal((nf, og.size))
+    gam = O.gamma(list(og.h), 2, 0.0, -1.0)
+    tr, to = to_dev(res), to_dev(np.zeros_like(res))
+    pd = mgx.F32 if prec == "f32" else mgx.F64
+    ctx.call("mgx_precond_mg", 2, 2, gam, pd, mg_, mgx.ptr(tr), og.size, nf, None, mgx.ptr(to), og.size)
+    ref = O.precond(og, 2, 2, gam, res, dtype=DT[prec])
+    err = relmax(to_np(to), ref)
+    say("precond", prec, err)
+    assert err <= (1e-12 if prec == "f64" else 1e-5)
+
+
+def test_invalid_lap_type_rejected(ctx):
+    og, mg_ = _grids()
+    t = to_dev(np.zeros((1,) + og.shape_g))
+    with pytest.raises(mgx.MgxError):
+        ctx.call("mgx_lap_apply", 3, mgx.block(t, mg_), mgx.block(t, mg_))

@@ -68,7 +68,7 @@ def test_residual_bruteforce():
     assert np.allclose(r, th.T @ psi - hp, rtol=1e-13, atol=1e-13)
 
 
-def _boxes(seed=3, nbox=6, gdim=(32, 32, 32), edges=(8, 12, 16), h=0.4):
+def _boxes(seed=3, nbox=6, gdim=(40, 40, 40), edges=(8, 12, 16), h=0.4):
     rng = np.random.default_rng(seed)
     edge = rng.choice(edges, nbox)
     origin = 4 * rng.integers(0, gdim[0] // 4, (nbox, 3))
@@ -154,8 +154,18 @@ def test_box_gram_h_residual_equal_dense():
     EH = O.b_embed(b, hp).reshape(b.nbox, -1)
     vel = np.prod(list(b.h))
     assert np.allclose(S, vel * E @ E.T, rtol=1e-12, atol=1e-12)
-    Hd = 0.5 * vel * (E @ EH.T + EH @ E.T)
-    assert np.allclose(H, Hd, rtol=1e-12, atol=1e-12)
+    # Galerkin H: the full-grid (unmasked) H psi_j of each embedded orbital
+    g = O.Grid.make(b.gdim, 2, b.h)
+    v = rng.standard_normal(tuple(b.gdim))
+    hp = O.b_hloc(b, v, psi)
+    S, H = O.b_gram_h(b, psi, hp)
+    HF = O.lap4_withpot(g, O.trade(g, O.embed(g, O.b_embed(b, psi))), v)
+    Hd = vel * E @ HF.T
+    assert np.max(np.abs(Hd - Hd.T)) < 1e-12 * np.max(np.abs(Hd))
+    assert np.allclose(H, 0.5 * (Hd + Hd.T), rtol=1e-12, atol=1e-12)
+    # the masked product alone misses the stencil reach outside each box
+    assert not np.allclose(H, 0.5 * vel * (E @ EH.T + EH @ E.T), rtol=1e-6, atol=1e-9)
+    EH = O.b_embed(b, hp).reshape(b.nbox, -1)
     th = rng.standard_normal((b.nbox, b.nbox))
     r = O.b_residual(b, psi, hp, th)
     m = O.b_masks(b)
@@ -164,8 +174,8 @@ def test_box_gram_h_residual_equal_dense():
     d = O.b_diag_dots(b, r)
     assert np.allclose(d, vel * np.sum(rd * rd, axis=1), rtol=1e-12)
     pi, pj = O.b_pairs(b)
-    nz = {(i, j) for i in range(b.nbox) for j in range(i, b.nbox) if np.any((E[i] != 0) & (E[j] != 0))}
-    assert nz <= set(zip(pi.tolist(), pj.tolist()))
+    nz = {(i, j) for i in range(b.nbox) for j in range(i, b.nbox) if abs(H[i, j]) > 0}
+    assert nz == set(zip(pi.tolist(), pj.tolist())) or nz <= set(zip(pi.tolist(), pj.tolist()))
 
 
 def test_synth_boxes_valid():
