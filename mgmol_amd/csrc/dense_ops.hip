@@ -23,7 +23,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;  // waves per block, each on its own row sub-chunk
 
 // MFMA 16x16x4 operand layout (CDNA3/CDNA4): lane l supplies A[m=l%16][k=l/16]
-// and B[k=l/16][n=l%16]; accumulator register v of lane l is D[4*(l/16)+v][l%16].
+// and B[k=l/16][n=l%16].  Accumulator v of lane l holds D[4*(l/16)+v][l%16]
+// for v_mfma_f32_16x16x4_f32 and D[l/16+4*v][l%16] for v_mfma_f64_16x16x4_f64
+// (both checked on gfx950 by tests/test_gpu_dense.py).
 template <int PREC>
 __global__ void __launch_bounds__(64 * kWaves) k_tsgemm(const double* A, const double* B, int64_t ld,
     int64_t loc_numpt, int n, int ntile, int nchunk, int64_t rows_per_chunk, int lower_only, double* part)
@@ -99,7 +101,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_tsgemm(const double* A, const d
             double s = red[0][lane][v];
             for (int w = 1; w < kWaves; w++)
                 s += red[w][lane][v];
-            const int row = i0 + 4 * r + v, col = j0 + c;
+            const int row = (PREC == MGX_PREC_F64) ? i0 + r + 4 * v : i0 + 4 * r + v;
+            const int col = j0 + c;
             if (row < n && col < n)
                 part[(((int64_t)slab * nchunk + chunk) * n + col) * n + row] = s;
         }

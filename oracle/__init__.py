@@ -508,7 +508,7 @@ def b_solver_step(b, psi, vtot, mg_levels=2, prec=F64, step_length=1.5, lowest=N
     e_ry = eigsum(H, Sinv)
     R = b_residual(b, psi, hp, th)
     rn = residual_norm(b_diag_dots(b, R), Sinv)
-    if lowest is None:
+    if lowest is None and not zero_start:
         lowest = float(ritz(H, S)[0][0])
     g = 0.0 if zero_start else gamma(list(b.h), mg_levels, float(np.max(vtot)), lowest)
     KR = b_precond(b, 2, mg_levels, g, R, dtype=prec)

@@ -971,28 +971,34 @@ double skin_value(const double* pj, int ej, const C4& c, int x, int y, int z)
 }
 
 // sum over r in box i, r outside box j, of psi_i(r) (H psi_j)(r);
-// sh maps i-local -> j-local coordinates
+// sh maps i-local -> j-local coordinates.  Only points one axis away from box
+// j (the 12 slabs of thickness 1 at distance 1 and 2 from its faces) reach it.
 double skin_sum(const double* pi, int ei, const double* pj, int ej, const int sh[3], const C4& c)
 {
     double acc = 0.;
-    for (int x = 0; x < ei; x++)
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++)
     {
-        const int X = x + sh[0];
-        if (X < -2 || X > ej + 1) continue;
-        for (int y = 0; y < ei; y++)
-        {
-            const int Y = y + sh[1];
-            if (Y < -2 || Y > ej + 1) continue;
-            for (int z = 0; z < ei; z++)
+        lo[k] = std::max(0, sh[k]);
+        hi[k] = std::min(ej, sh[k] + ei);
+    }
+    for (int slab = 0; slab < 12; slab++)
+    {
+        const int d = slab >> 2, side = (slab >> 1) & 1, layer = (slab & 1) + 1;
+        const int Xd = side == 0 ? -layer : ej - 1 + layer;
+        const int xd = Xd - sh[d];
+        if (xd < 0 || xd >= ei) continue;
+        const int u = (d + 1) % 3, w = (d + 2) % 3;
+        for (int a = lo[u]; a < hi[u]; a++)
+            for (int b = lo[w]; b < hi[w]; b++)
             {
-                const int Z = z + sh[2];
-                if (Z < -2 || Z > ej + 1) continue;
-                const bool inside = X >= 0 && Y >= 0 && Z >= 0 && X < ej && Y < ej && Z < ej;
-                if (inside) continue;
-                const double hv = skin_value(pj, ej, c, X, Y, Z);
-                if (hv != 0.) acc += pi[((long)x * ei + y) * ei + z] * hv;
+                int X[3];
+                X[d] = Xd;
+                X[u] = a;
+                X[w] = b;
+                const double hv = skin_value(pj, ej, c, X[0], X[1], X[2]);
+                acc += pi[((long)(X[0] - sh[0]) * ei + (X[1] - sh[1])) * ei + (X[2] - sh[2])] * hv;
             }
-        }
     }
     return acc;
 }

