@@ -217,6 +217,139 @@ __global__ void __launch_bounds__(256) k_box_jacobi(const T* x, const T* f, T* x
     xo[i] = xv;
 }
 
+// ---------------------------------------------------------------------------
+// Plane-marching stencil sweeps (2nd / 4th order) on boxes.  One workgroup per
+// box walks x; each thread owns P points of the y-z plane and keeps their
+// values at x-2..x+2 in a register queue, while the current plane sits in LDS
+// with a zero ring of width 2 (the box boundary).  Each input element is read
+// from HBM once; the next plane is prefetched before the current one is
+// computed.  The arithmetic per point is the reference's, in the same order.
+//   MODE 0: out = (T) L x
+//   MODE 1: out = (T)((c0 + V) x + neighbours)   (del2_4th_withPot)
+//   MODE 2: one Jacobi sweep: w = (T)L x; w -= f; xo = x + (T)(mjf*w)
+enum
+{
+    MARCH_LAP    = 0,
+    MARCH_HLOC   = 1,
+    MARCH_JACOBI = 2
+};
+
+struct MarchArgs
+{
+    const int64_t* off;
+    const int* edge;
+    const int* origin;
+    int level;
+    int g0, g1, g2;
+    int lds_w;  // LDS row width = max edge + 4
+    double mjf;
+};
+
+template <typename T, int TYPE, int MODE, int NT, int P>
+__global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, const T* __restrict__ f,
+    const double* __restrict__ vtot, T* __restrict__ out, T* __restrict__ wout, MarchArgs a, Coef4 c4, Coef2 c2)
+{
+    extern __shared__ unsigned char lds_raw[];
+    T* pl       = reinterpret_cast<T*>(lds_raw);
+    const int j = blockIdx.x;
+    const int e = a.edge[j] >> a.level;
+    const int E2 = e * e;
+    const int W  = a.lds_w;
+    const int64_t base = a.off[j];
+    // zero the plane including its ring once; the ring is never written again
+    for (int t = threadIdx.x; t < W * W; t += NT)
+        pl[t] = (T)0;
+    int qy[P], qz[P];
+    bool act[P];
+#pragma unroll
+    for (int k = 0; k < P; k++)
+    {
+        const int q = threadIdx.x + k * NT;
+        act[k]      = q < E2;
+        qy[k]       = act[k] ? q / e : 0;
+        qz[k]       = act[k] ? q - (q / e) * e : 0;
+    }
+    T qv[P][5];
+#pragma unroll
+    for (int k = 0; k < P; k++)
+    {
+        const int q = threadIdx.x + k * NT;
+        qv[k][0]    = (T)0;
+        qv[k][1]    = (T)0;
+        qv[k][2]    = act[k] ? in[base + q] : (T)0;
+        qv[k][3]    = (act[k] && e > 1) ? in[base + E2 + q] : (T)0;
+        qv[k][4]    = (act[k] && e > 2) ? in[base + 2 * E2 + q] : (T)0;
+    }
+    __syncthreads();
+    for (int x = 0; x < e; x++)
+    {
+        // prefetch plane x+3
+        T nx[P];
+#pragma unroll
+        for (int k = 0; k < P; k++)
+        {
+            const int q = threadIdx.x + k * NT;
+            nx[k]       = (act[k] && x + 3 < e) ? in[base + (int64_t)(x + 3) * E2 + q] : (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < P; k++)
+            if (act[k]) pl[(qy[k] + 2) * W + qz[k] + 2] = qv[k][2];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < P; k++)
+        {
+            if (!act[k]) continue;
+            const int q   = threadIdx.x + k * NT;
+            const T* ctr  = pl + (qy[k] + 2) * W + qz[k] + 2;
+            const int64_t gi = base + (int64_t)x * E2 + q;
+            double r;
+            if (TYPE == MGX_LAP_4TH)
+            {
+                double diag = c4.c0;
+                if (MODE == MARCH_HLOC)
+                {
+                    int gx = a.origin[3 * j] + x, gy = a.origin[3 * j + 1] + qy[k], gz = a.origin[3 * j + 2] + qz[k];
+                    if (gx >= a.g0) gx -= a.g0;
+                    if (gy >= a.g1) gy -= a.g1;
+                    if (gz >= a.g2) gz -= a.g2;
+                    diag = c4.c0 + vtot[((int64_t)gx * a.g1 + gy) * a.g2 + gz];
+                }
+                r = diag * (double)qv[k][2] + c4.c1x * ((double)qv[k][1] + (double)qv[k][3])
+                    + c4.c1y * ((double)ctr[-W] + (double)ctr[W]) + c4.c1z * ((double)ctr[-1] + (double)ctr[1])
+                    + c4.c2x * ((double)qv[k][0] + (double)qv[k][4])
+                    + c4.c2y * ((double)ctr[-2 * W] + (double)ctr[2 * W])
+                    + c4.c2z * ((double)ctr[-2] + (double)ctr[2]);
+            }
+            else
+            {
+                r = c2.c0 * qv[k][2] + c2.c1x * (T)(qv[k][1] + qv[k][3]) + c2.c1y * (T)(ctr[-W] + ctr[W])
+                    + c2.c1z * (T)(ctr[-1] + ctr[1]);
+            }
+            if (MODE == MARCH_JACOBI)
+            {
+                T wv = (T)r;
+                wv += static_cast<T>(-1. * static_cast<double>(f[gi]));
+                if (wout) wout[gi] = wv;
+                T xv = qv[k][2];
+                xv += static_cast<T>(a.mjf * static_cast<double>(wv));
+                out[gi] = xv;
+            }
+            else
+                out[gi] = (T)r;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < P; k++)
+        {
+            qv[k][0] = qv[k][1];
+            qv[k][1] = qv[k][2];
+            qv[k][2] = qv[k][3];
+            qv[k][3] = qv[k][4];
+            qv[k][4] = nx[k];
+        }
+    }
+}
+
 // MGkernelRestrict3D on a box (fine values outside the box are zero)
 template <typename T>
 __global__ void __launch_bounds__(256) k_box_restrict2(const T* fine, const int64_t* off_f, T* coarse, BoxView vc)
@@ -564,9 +697,63 @@ int box_tiles_launch_check(const mgx_boxset* bs, int level)
     return (level >= 0 && level < bs->levels) ? MGX_OK : MGX_EINVAL;
 }
 
+template <typename T, int TYPE, int MODE, int NT, int P>
+void march_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, const T* in, const T* f, const double* v,
+    T* out, T* w, const Coefs& c)
+{
+    const size_t lds = (size_t)a.lds_w * a.lds_w * sizeof(T);
+    hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P>), dim3(bs->nbox), dim3(NT), lds, ctx->stream, in, f, v, out,
+        w, a, c.c4, c.c2);
+}
+
+// the marching sweep for 2nd/4th-order operators; returns false if the box
+// sizes at this level need another kernel
+template <typename T, int TYPE, int MODE>
+bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* in, const T* f, const double* v,
+    T* out, T* w)
+{
+    const int emax = bs->maxedge >> level;
+    const int E2   = emax * emax;
+    MarchArgs a;
+    a.off    = bs->d_off[level];
+    a.edge   = bs->d_edge;
+    a.origin = bs->d_origin;
+    a.level  = level;
+    a.g0     = bs->gdim[0] >> level;
+    a.g1     = bs->gdim[1] >> level;
+    a.g2     = bs->gdim[2] >> level;
+    a.lds_w  = emax + 4;
+    a.mjf    = mjf;
+    Coefs c  = coefs_at(bs, level);
+    if ((size_t)a.lds_w * a.lds_w * sizeof(T) > 60 * 1024) return false;
+    if (E2 <= 64)
+        march_launch<T, TYPE, MODE, 64, 1>(ctx, bs, a, in, f, v, out, w, c);
+    else if (E2 <= 256)
+        march_launch<T, TYPE, MODE, 256, 1>(ctx, bs, a, in, f, v, out, w, c);
+    else if (E2 <= 512)
+        march_launch<T, TYPE, MODE, 256, 2>(ctx, bs, a, in, f, v, out, w, c);
+    else if (E2 <= 1024)
+        march_launch<T, TYPE, MODE, 256, 4>(ctx, bs, a, in, f, v, out, w, c);
+    else if (E2 <= 2048)
+        march_launch<T, TYPE, MODE, 256, 8>(ctx, bs, a, in, f, v, out, w, c);
+    else
+        return false;
+    return true;
+}
+
 template <typename T>
 int box_lap_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, const T* in, T* out)
 {
+    if (type == MGX_LAP_4TH && march<T, MGX_LAP_4TH, MARCH_LAP>(ctx, bs, level, 0., in, nullptr, nullptr, out, nullptr))
+    {
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
+    if (type == MGX_LAP_2ND && march<T, MGX_LAP_2ND, MARCH_LAP>(ctx, bs, level, 0., in, nullptr, nullptr, out, nullptr))
+    {
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
     BoxView v = view(bs, level);
     Coefs c   = coefs_at(bs, level);
     dim3 g(tile_grid(v.ntiles)), b(kTile);
@@ -586,6 +773,11 @@ int box_lap_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, const T* 
 template <typename T>
 int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vtot, T* out)
 {
+    if (march<T, MGX_LAP_4TH, MARCH_HLOC>(ctx, bs, 0, 0., in, nullptr, vtot, out, nullptr))
+    {
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
     BoxView v = view(bs, 0);
     Coef4 c4  = coef4(bs->h);
     hipLaunchKernelGGL(k_box_hloc<T>, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, in, vtot, out, v, c4);
@@ -597,10 +789,20 @@ template <typename T>
 int box_jacobi_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, double jf, const T* x, const T* f, T* xo,
     T* w)
 {
+    const double mjf = -1. * jf;
+    if (type == MGX_LAP_4TH && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, w))
+    {
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
+    if (type == MGX_LAP_2ND && march<T, MGX_LAP_2ND, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, w))
+    {
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
     BoxView v = view(bs, level);
     Coefs c   = coefs_at(bs, level);
     dim3 g(tile_grid(v.ntiles)), b(kTile);
-    const double mjf = -1. * jf;
     switch (type)
     {
         case MGX_LAP_4TH:

@@ -200,6 +200,76 @@ __global__ void __launch_bounds__(1024) k_trace_prod(const double* H, const doub
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// single block: out = sum_i A(i,i)
+__global__ void __launch_bounds__(256) k_diag_sum(const double* A, int n, double* out)
+{
+    __shared__ double red[256];
+    double s = 0.;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        s += A[(int64_t)i * n + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// per-block max |delta_ij - T(i,j)|, then a second pass over the blocks
+__global__ void __launch_bounds__(256) k_id_err(const double* T, int n, double* blockmax)
+{
+    __shared__ double red[256];
+    double m = 0.;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)n * n;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t / n), row = (int)(t - (t / n) * n);
+        const double v = fabs(T[t] - (row == col ? 1. : 0.));
+        m = v > m ? v : m;
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blockmax[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_max_reduce(const double* v, int n, double* out)
+{
+    __shared__ double red[256];
+    double m = 0.;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        m = fmax(m, v[i]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// X = (X + X^T)/2
+__global__ void k_symmetrize(double* X, int n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)n * n;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t / n), row = (int)(t - (t / n) * n);
+        if (col >= row) continue;
+        const int64_t u = (int64_t)row * n + col;
+        const double a = 0.5 * (X[t] + X[u]);
+        X[t] = a;
+        X[u] = a;
+    }
+}
+
 __global__ void k_sub(double* res, const double* hpsi, int64_t ld, int64_t numpt, int n)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < numpt * n;
@@ -296,6 +366,65 @@ int chol_inv(mgx_ctx* ctx, int n, const double* S, double* L, double* Sinv)
     if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, n, Sinv, n, info) != rocblas_status_success)
         return set_err(ctx, MGX_EHIP, "rocsolver_dpotri failed");
     hipLaunchKernelGGL(k_tri, dim3(nblocks((int64_t)n * n)), dim3(kBlock), 0, ctx->stream, Sinv, n, 0);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int diag_sum(mgx_ctx* ctx, int n, const double* A, double* dev_out)
+{
+    hipLaunchKernelGGL(k_diag_sum, dim3(1), dim3(256), 0, ctx->stream, A, n, dev_out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+// Refresh X ~ S^-1 by Newton-Schulz steps X <- X (2I - S X) from the previous
+// inverse (S changes little between orbital steps, so the iteration starts
+// well inside its quadratic basin).  Falls back to the Cholesky inverse when
+// max|I - S X| does not start below 0.25 (first step, large change) or fails
+// to contract.  work: 2 n^2 doubles.  *steps = Newton-Schulz steps taken
+// (-1 when the Cholesky path ran).
+int inverse_update(mgx_ctx* ctx, int n, const double* S, double* L, double* X, double* work, int* steps)
+{
+    const double one = 1., zero = 0., mone = -1., two = 2.;
+    const size_t nn = (size_t)n * n;
+    double* T  = work;
+    double* X2 = work + nn;
+    double* bm = (double*)workspace(ctx, 4, 1024 * sizeof(double));
+    if (!bm) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    double prev = 1e300;
+    *steps      = 0;
+    for (int it = 0; it < 8; it++)
+    {
+        if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, S, n, X, n, &zero,
+                T, n)
+            != rocblas_status_success)
+            return set_err(ctx, MGX_EHIP, "rocblas_dgemm failed");
+        const int nb = 512;
+        hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
+        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double*)bm, nb, bm + nb);
+        MGX_CHECK_LAUNCH(ctx);
+        double delta = 0.;
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(&delta, bm + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (!(delta < 0.25) || delta > 0.5 * prev)
+        {
+            *steps = -1;
+            return chol_inv(ctx, n, S, L, X);
+        }
+        if (delta < 1e-15) break;
+        // X2 = 2 X - X T
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(X2, X, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, X, n, T, n, &two,
+                X2, n)
+            != rocblas_status_success)
+            return set_err(ctx, MGX_EHIP, "rocblas_dgemm failed");
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(X, X2, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        (*steps)++;
+        prev = delta;
+        if (delta < 3e-8) break;  // the error after this step is ~delta^2
+    }
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, X, n);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }

@@ -32,6 +32,8 @@ void boxset_geom(const mgx_boxset*, int gdim[3], double h[3], int* nbox);
 int chol_inv(mgx_ctx*, int, const double*, double*, double*);
 int theta(mgx_ctx*, int, const double*, const double*, double*);
 int trace_prod(mgx_ctx*, int, const double*, const double*, double*);
+int diag_sum(mgx_ctx*, int, const double*, double*);
+int inverse_update(mgx_ctx*, int, const double*, double*, double*, double*, int*);
 int ritz(mgx_ctx*, int, const double*, const double*, double*, double*);
 }  // namespace mgx
 
@@ -49,7 +51,9 @@ struct mgx_solver
     double *psi = nullptr, *psi0 = nullptr, *hpsi = nullptr, *res = nullptr, *vtot = nullptr;
     MgxProf prof;
     double *S = nullptr, *H = nullptr, *L = nullptr, *Sinv = nullptr, *th = nullptr;
-    double *eig = nullptr, *d = nullptr, *scal = nullptr;
+    double *eig = nullptr, *d = nullptr, *scal = nullptr, *nswork = nullptr;
+    bool have_inv = false;  // Sinv holds the inverse of a recent S
+    int64_t ns_steps = 0, chol_calls = 0;
     double* host = nullptr;  // pinned: [energy, norm]
 };
 
@@ -61,7 +65,7 @@ namespace
 void free_solver(mgx_solver* s)
 {
     double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
-        s->scal };
+        s->scal, s->nswork };
     for (double* b : bufs)
         if (b) hipFree(b);
     if (s->host) hipHostFree(s->host);
@@ -82,9 +86,26 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     if ((rc = box_gram_h(ctx, s->bs, s->cfg.gram_prec, s->psi, s->hpsi, s->S, s->H))) return rc;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_DENSE);
-    if ((rc = chol_inv(ctx, n, s->S, s->L, s->Sinv))) return rc;
+    // S^-1: Cholesky + inverse (GramMatrix::updateLS / computeInverse) on the
+    // first step, then Newton-Schulz refresh from the previous inverse
+    if (!s->have_inv)
+    {
+        if ((rc = chol_inv(ctx, n, s->S, s->L, s->Sinv))) return rc;
+        s->chol_calls++;
+        s->have_inv = true;
+    }
+    else
+    {
+        int steps = 0;
+        if ((rc = inverse_update(ctx, n, s->S, s->L, s->Sinv, s->nswork, &steps))) return rc;
+        if (steps < 0)
+            s->chol_calls++;
+        else
+            s->ns_steps += steps;
+    }
     if ((rc = theta(ctx, n, s->Sinv, s->H, s->th))) return rc;
-    if ((rc = trace_prod(ctx, n, s->H, s->Sinv, s->scal))) return rc;
+    // tr(H S^-1) = tr(S^-1 H) = tr(theta)   (ProjectedMatrices::getEigSum)
+    if ((rc = diag_sum(ctx, n, s->th, s->scal))) return rc;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_RESID);
     if ((rc = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res))) return rc;
@@ -171,7 +192,7 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     auto alloc = [&](double** p, size_t cnt) { return hipMalloc((void**)p, cnt * sizeof(double)) == hipSuccess; };
     bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
               && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
-              && alloc(&s->eig, s->n) && alloc(&s->d, s->n + 8) && alloc(&s->scal, 8)
+              && alloc(&s->eig, s->n) && alloc(&s->d, s->n + 8) && alloc(&s->scal, 8) && alloc(&s->nswork, 2 * nn)
               && hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {
@@ -241,8 +262,9 @@ int mgx_solver_reset(mgx_solver* s)
     MGX_CHECK_HIP(s->ctx, hipMemcpyAsync(s->psi, s->psi0, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
                               s->ctx->stream));
     MGX_CHECK_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    s->iter   = 0;
-    s->lowest = 0.;
+    s->iter     = 0;
+    s->lowest   = 0.;
+    s->have_inv = false;
     return MGX_OK;
 }
 
