@@ -38,6 +38,18 @@ struct mgx_boxset
     int* d_nbr     = nullptr;
     int64_t nnbr   = 0;
     int maxedge    = 0;
+    // spatial cells of 8^3 points (level 0) listing the boxes that cover
+    // them: 10 ints per entry (box, cell origin in box-local coords [3],
+    // covered cell-local range lo[3], hi[3]); boxes ascending in each cell
+    int ncell = 0, kmax = 0;
+    int* d_cell_ptr  = nullptr;
+    int* d_cell_ent  = nullptr;
+    int64_t* d_cell_slot = nullptr;  // base of the cell's k*k partial block
+    int64_t nslot    = 0;
+    // per pair: the partial slots (ab, ba) of every cell both boxes cover
+    int* d_pslot_ptr = nullptr;
+    int64_t* d_pslot = nullptr;
+    bool use_cells   = false;
 };
 
 namespace mgx
@@ -612,6 +624,245 @@ __global__ void __launch_bounds__(256) k_pair_gram(const double* psi, const doub
     }
 }
 
+constexpr int kCell  = 8;    // cell edge
+constexpr int kCellK = 32;   // max boxes per cell handled by the cell kernels
+
+typedef double d4c __attribute__((ext_vector_type(4)));
+typedef float f4c __attribute__((ext_vector_type(4)));
+
+struct CellEnt
+{
+    int64_t base;  // offset of the box's data
+    int e;
+    int b0, b1, b2;
+    int lo0, lo1, lo2, hi0, hi1, hi2;
+};
+
+__device__ __forceinline__ void load_cell(const int* ent, const int64_t* off, const int* edge, int k, CellEnt* ce)
+{
+    for (int a = threadIdx.x; a < k; a += blockDim.x)
+    {
+        const int* E = ent + 10 * a;
+        CellEnt c;
+        c.base = off[E[0]];
+        c.e    = edge[E[0]];
+        c.b0 = E[1];
+        c.b1 = E[2];
+        c.b2 = E[3];
+        c.lo0 = E[4];
+        c.lo1 = E[5];
+        c.lo2 = E[6];
+        c.hi0 = E[7];
+        c.hi1 = E[8];
+        c.hi2 = E[9];
+        ce[a] = c;
+    }
+}
+
+// index of cell point p (x = p>>6, y = (p>>3)&7, z = p&7) in the box, or -1
+__device__ __forceinline__ int64_t cell_idx(const CellEnt& c, int p)
+{
+    const int x = p >> 6, y = (p >> 3) & 7, z = p & 7;
+    if (x < c.lo0 || x >= c.hi0 || y < c.lo1 || y >= c.hi1 || z < c.lo2 || z >= c.hi2) return -1;
+    return c.base + ((int64_t)(c.b0 + x) * c.e + (c.b1 + y)) * c.e + (c.b2 + z);
+}
+
+// Per cell: G = A^T A and P = A^T B (A = psi, B = H psi, 512 x k) on MFMA.
+// Four waves split the 512 points (128 each), then combine in LDS in a fixed
+// order; partials go to part[slot + a*k + b] (G) and part[nslot + ...] (P).
+template <int PREC>
+__global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
+    const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part)
+{
+    __shared__ CellEnt ce[kCellK];
+    __shared__ double red[4][64][4];
+    const int cell = blockIdx.x;
+    const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
+    load_cell(cell_ent + 10 * e0, off, edge, k, ce);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int c = lane & 15, r = lane >> 4;
+    const int nt = (k + 15) / 16;
+    const int64_t slot = cell_slot[cell];
+    for (int ti = 0; ti < nt; ti++)
+        for (int tj = 0; tj < nt; tj++)
+        {
+            const int a = ti * 16 + c, b = tj * 16 + c;
+            const bool va = a < k, vb = b < k;
+            double accG[4] = { 0., 0., 0., 0. }, accP[4] = { 0., 0., 0., 0. };
+            if (PREC == MGX_PREC_F64)
+            {
+                d4c g = { 0., 0., 0., 0. }, h = { 0., 0., 0., 0. };
+                for (int s = 0; s < 32; s++)
+                {
+                    const int p = wid * 128 + 4 * s + r;
+                    const int64_t ia = va ? cell_idx(ce[a], p) : -1;
+                    const int64_t ib = vb ? cell_idx(ce[b], p) : -1;
+                    const double xa = ia >= 0 ? psi[ia] : 0.;
+                    const double xb = ib >= 0 ? psi[ib] : 0.;
+                    const double hb = ib >= 0 ? hpsi[ib] : 0.;
+                    g = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, xb, g, 0, 0, 0);
+                    h = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, hb, h, 0, 0, 0);
+                }
+                for (int v = 0; v < 4; v++)
+                {
+                    accG[v] = g[v];
+                    accP[v] = h[v];
+                }
+            }
+            else
+            {
+                // hi/lo split: products hi*hi + (hi*lo + lo*hi) on FP32 MFMA,
+                // FP32 sums over the wave's 128 points, FP64 afterwards
+                f4c g1 = { 0.f, 0.f, 0.f, 0.f }, g2 = g1, h1 = g1, h2 = g1;
+                for (int s = 0; s < 32; s++)
+                {
+                    const int p = wid * 128 + 4 * s + r;
+                    const int64_t ia = va ? cell_idx(ce[a], p) : -1;
+                    const int64_t ib = vb ? cell_idx(ce[b], p) : -1;
+                    const double xa = ia >= 0 ? psi[ia] : 0.;
+                    const double xb = ib >= 0 ? psi[ib] : 0.;
+                    const double hb = ib >= 0 ? hpsi[ib] : 0.;
+                    const float ah = (float)xa, al = (float)(xa - (double)ah);
+                    const float bh = (float)xb, bl = (float)(xb - (double)bh);
+                    const float hh = (float)hb, hl = (float)(hb - (double)hh);
+                    g1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, g1, 0, 0, 0);
+                    g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, g2, 0, 0, 0);
+                    g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, g2, 0, 0, 0);
+                    h1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hh, h1, 0, 0, 0);
+                    h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hl, h2, 0, 0, 0);
+                    h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, hh, h2, 0, 0, 0);
+                }
+                for (int v = 0; v < 4; v++)
+                {
+                    accG[v] = (double)g1[v] + (double)g2[v];
+                    accP[v] = (double)h1[v] + (double)h2[v];
+                }
+            }
+            for (int pass = 0; pass < 2; pass++)
+            {
+                const double* acc = pass == 0 ? accG : accP;
+                for (int v = 0; v < 4; v++)
+                    red[wid][lane][v] = acc[v];
+                __syncthreads();
+                if (wid == 0)
+                {
+                    for (int v = 0; v < 4; v++)
+                    {
+                        const double sum = red[0][lane][v] + red[1][lane][v] + red[2][lane][v] + red[3][lane][v];
+                        const int row = ti * 16 + ((PREC == MGX_PREC_F64) ? r + 4 * v : 4 * r + v);
+                        const int col = tj * 16 + c;
+                        if (row < k && col < k) part[pass * nslot + slot + row * k + col] = sum;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+}
+
+// stencil reach outside the boxes, per pair: out[2p] = sum over r in box i
+// outside box j of psi_i (H psi_j), out[2p+1] the mirrored term
+__global__ void __launch_bounds__(256) k_pair_skin(const double* psi, const int64_t* off, const int* edge,
+    const int* pairs, int64_t npairs, Coef4 c4, double* out)
+{
+    __shared__ double lds[2 * 4];
+    const int64_t pid = blockIdx.x;
+    if (pid >= npairs) return;
+    const int* P = pairs + 11 * pid;
+    const int i = P[0], j = P[1];
+    const int ei = edge[i], ej = edge[j];
+    const int shij[3] = { P[8], P[9], P[10] }, shji[3] = { -P[8], -P[9], -P[10] };
+    double acc[2];
+    acc[0] = skin_part(psi + off[i], ei, psi + off[j], ej, shij, c4);
+    acc[1] = skin_part(psi + off[j], ej, psi + off[i], ei, shji, c4);
+    block_sum<2>(acc, lds);
+    if (threadIdx.x == 0)
+    {
+        out[2 * pid]     = acc[0];
+        out[2 * pid + 1] = acc[1];
+    }
+}
+
+// S_ij, H_ij from the cell partials (slots in a fixed order) plus the skin
+__global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslot_ptr, const int64_t* pslot,
+    const double* skin, const int* pairs, int64_t npairs, int n, double vel, double* S, double* H)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        double g = 0., h1 = 0., h2 = 0.;
+        for (int q = pslot_ptr[t]; q < pslot_ptr[t + 1]; q++)
+        {
+            const int64_t ab = pslot[2 * q], ba = pslot[2 * q + 1];
+            g += part[ab];
+            h1 += part[nslot + ab];
+            h2 += part[nslot + ba];
+        }
+        h1 += skin[2 * t];
+        h2 += skin[2 * t + 1];
+        const int i = pairs[11 * t], j = pairs[11 * t + 1];
+        const double sv = g * vel, hv = 0.5 * (h1 + h2) * vel;
+        S[(int64_t)j * n + i] = sv;
+        S[(int64_t)i * n + j] = sv;
+        H[(int64_t)j * n + i] = hv;
+        H[(int64_t)i * n + j] = hv;
+    }
+}
+
+// R_b(p) = sum_a psi_a(p) theta(a,b) - hpsi_b(p) for every box b covering
+// point p of the cell, as a 512 x k by k x k product on FP64 MFMA:
+// A = psi gathered per (point, box a), B = theta sub-block from LDS.
+__global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
+    int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res)
+{
+    __shared__ CellEnt ce[kCellK];
+    __shared__ double th[kCellK * kCellK];
+    __shared__ int jid[kCellK];
+    const int cell = blockIdx.x;
+    const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
+    load_cell(cell_ent + 10 * e0, off, edge, k, ce);
+    for (int t = threadIdx.x; t < k; t += blockDim.x)
+        jid[t] = cell_ent[10 * (e0 + t)];
+    __syncthreads();
+    for (int t = threadIdx.x; t < kCellK * kCellK; t += blockDim.x)
+    {
+        const int aa = t / kCellK, bb = t - (t / kCellK) * kCellK;
+        th[t] = (aa < k && bb < k) ? theta[(int64_t)jid[bb] * n + jid[aa]] : 0.;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int c = lane & 15, r = lane >> 4;
+    const int nct = (k + 15) / 16;
+    const int ksteps = (k + 3) / 4;
+    for (int job = wid; job < 32 * nct; job += 4)
+    {
+        const int p0 = (job & 31) * 16, b0 = (job >> 5) * 16;
+        d4c acc = { 0., 0., 0., 0. };
+        for (int st = 0; st < ksteps; st++)
+        {
+            const int aa  = 4 * st + r;
+            const int pp  = p0 + c;
+            double xa     = 0.;
+            if (aa < k)
+            {
+                const int64_t ia = cell_idx(ce[aa], pp);
+                if (ia >= 0) xa = psi[ia];
+            }
+            const double tb = th[aa * kCellK + b0 + c];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, tb, acc, 0, 0, 0);
+        }
+        const int bb = b0 + c;
+        if (bb < k)
+        {
+            for (int v = 0; v < 4; v++)
+            {
+                const int pp     = p0 + r + 4 * v;
+                const int64_t ib = cell_idx(ce[bb], pp);
+                if (ib >= 0) res[ib] = acc[v] + (-1. * hpsi[ib]);
+            }
+        }
+    }
+}
+
 __global__ void k_pair_scatter(const double* part, const int* pairs, int64_t npairs, int n, double vel, double* S,
     double* H)
 {
@@ -959,9 +1210,46 @@ int box_hloc(mgx_ctx* ctx, const mgx_boxset* bs, int dtype, const void* psi, con
                             : box_hloc_t(ctx, bs, (const double*)psi, vtot, (double*)hpsi);
 }
 
+int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, const double* hpsi, double* S,
+    double* H)
+{
+    const int n      = bs->nbox;
+    const size_t pb  = 2 * (size_t)bs->nslot * sizeof(double);
+    const size_t sb  = 2 * (size_t)std::max<int64_t>(bs->npairs, 1) * sizeof(double);
+    double* part     = (double*)workspace(ctx, 4, pb + sb);
+    if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    double* skin     = part + 2 * bs->nslot;
+    const int64_t nn = (int64_t)n * n;
+    hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, S, nn, 0.);
+    hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, H, nn, 0.);
+    if (prec == MGX_PREC_SPLIT)
+        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_SPLIT>, dim3(bs->ncell), dim3(256), 0, ctx->stream, psi, hpsi,
+            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part);
+    else
+        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_F64>, dim3(bs->ncell), dim3(256), 0, ctx->stream, psi, hpsi,
+            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part);
+    MGX_CHECK_LAUNCH(ctx);
+    if (bs->npairs > 0)
+    {
+        hipLaunchKernelGGL(k_pair_skin, dim3((unsigned)bs->npairs), dim3(256), 0, ctx->stream, psi,
+            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_pairs, bs->npairs, coef4(bs->h),
+            skin);
+        MGX_CHECK_LAUNCH(ctx);
+        const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+        hipLaunchKernelGGL(k_pair_reduce, dim3(nblocks(bs->npairs)), dim3(kBlock), 0, ctx->stream,
+            (const double*)part, bs->nslot, (const int*)bs->d_pslot_ptr, (const int64_t*)bs->d_pslot,
+            (const double*)skin, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H);
+        MGX_CHECK_LAUNCH(ctx);
+    }
+    return MGX_OK;
+}
+
 int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, const double* hpsi, double* S,
     double* H)
 {
+    if (bs->use_cells) return box_gram_h_cells(ctx, bs, prec, psi, hpsi, S, H);
     const int n = bs->nbox;
     double* part = (double*)workspace(ctx, 4, 3 * std::max<int64_t>(bs->npairs, 1) * sizeof(double));
     if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
@@ -990,6 +1278,14 @@ int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, 
 int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
     double* res)
 {
+    if (bs->use_cells)
+    {
+        hipLaunchKernelGGL(k_cell_residual, dim3(bs->ncell), dim3(256), 0, ctx->stream, psi, hpsi, theta, bs->nbox,
+            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+            (const int*)bs->d_cell_ent, res);
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
     BoxView v = view(bs, 0);
     hipLaunchKernelGGL(k_box_residual, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi, hpsi, theta,
         bs->nbox, (const int*)bs->d_nbr_ptr, (const int*)bs->d_nbr, v, res);
@@ -1162,6 +1458,104 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset
     if (!pairs.empty()) hipMemcpy(s->d_pairs, pairs.data(), sizeof(int) * pairs.size(), hipMemcpyHostToDevice);
     hipMemcpy(s->d_nbr_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice);
     if (!flat.empty()) hipMemcpy(s->d_nbr, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
+    // spatial cells (level 0)
+    bool cells_ok = true;
+    for (int d = 0; d < 3; d++)
+        cells_ok = cells_ok && (s->gdim[d] % kCell == 0);
+    if (cells_ok)
+    {
+        const int C0 = s->gdim[0] / kCell, C1 = s->gdim[1] / kCell, C2 = s->gdim[2] / kCell;
+        std::vector<std::vector<int>> cl((size_t)C0 * C1 * C2);
+        for (int j = 0; j < s->nbox; j++)
+        {
+            const int e = s->edge[j];
+            int c_lo[3], c_hi[3];
+            for (int d = 0; d < 3; d++)
+            {
+                c_lo[d] = s->origin[3 * j + d] / kCell;
+                c_hi[d] = (s->origin[3 * j + d] + e - 1) / kCell;
+            }
+            for (int cx = c_lo[0]; cx <= c_hi[0]; cx++)
+                for (int cy = c_lo[1]; cy <= c_hi[1]; cy++)
+                    for (int cz = c_lo[2]; cz <= c_hi[2]; cz++)
+                    {
+                        const int cg[3] = { cx, cy, cz };
+                        int rec[10];
+                        rec[0] = j;
+                        for (int d = 0; d < 3; d++)
+                        {
+                            const int o = s->origin[3 * j + d];
+                            rec[1 + d] = kCell * cg[d] - o;
+                            rec[4 + d] = std::max(0, o - kCell * cg[d]);
+                            rec[7 + d] = std::min(kCell, o + e - kCell * cg[d]);
+                        }
+                        const size_t ci = ((size_t)(cx % C0) * C1 + (cy % C1)) * C2 + (cz % C2);
+                        cl[ci].insert(cl[ci].end(), rec, rec + 10);
+                    }
+        }
+        std::vector<int> cptr(1, 0), cent;
+        std::vector<int64_t> cslot;
+        std::vector<std::vector<int>> cells_of;  // box ids per kept cell (ascending)
+        int64_t slot = 0;
+        int kmax     = 0;
+        for (auto& c : cl)
+        {
+            if (c.empty()) continue;
+            const int k = (int)(c.size() / 10);  // boxes were appended in ascending order
+            kmax        = std::max(kmax, k);
+            cent.insert(cent.end(), c.begin(), c.end());
+            cptr.push_back((int)(cent.size() / 10));
+            cslot.push_back(slot);
+            slot += (int64_t)k * k;
+        }
+        if (kmax <= kCellK)
+        {
+            // map (i, j) -> pair index
+            std::vector<std::vector<std::pair<int, int>>> pidx(s->nbox);
+            for (int64_t q = 0; q < s->npairs; q++)
+                pidx[pairs[11 * q]].push_back({ pairs[11 * q + 1], (int)q });
+            std::vector<int> pcount(s->npairs + 1, 0);
+            std::vector<int64_t> pslot;
+            std::vector<std::vector<int64_t>> lists(s->npairs);
+            for (size_t c = 0; c + 1 < cptr.size(); c++)
+            {
+                const int k = cptr[c + 1] - cptr[c];
+                for (int a = 0; a < k; a++)
+                    for (int b = a; b < k; b++)
+                    {
+                        const int i = cent[10 * (cptr[c] + a)], j = cent[10 * (cptr[c] + b)];
+                        auto& v   = pidx[i];
+                        auto it   = std::lower_bound(v.begin(), v.end(), std::make_pair(j, -1));
+                        if (it == v.end() || it->first != j) continue;  // cannot happen: covering the same cell
+                        lists[it->second].push_back(cslot[c] + (int64_t)a * k + b);
+                        lists[it->second].push_back(cslot[c] + (int64_t)b * k + a);
+                    }
+            }
+            std::vector<int> pptr(s->npairs + 1, 0);
+            for (int64_t q = 0; q < s->npairs; q++)
+            {
+                pptr[q + 1] = pptr[q] + (int)(lists[q].size() / 2);
+                pslot.insert(pslot.end(), lists[q].begin(), lists[q].end());
+            }
+            s->ncell = (int)cslot.size();
+            s->kmax  = kmax;
+            s->nslot = slot;
+            bool ok  = hipMalloc(&s->d_cell_ptr, sizeof(int) * cptr.size()) == hipSuccess
+                      && hipMalloc(&s->d_cell_ent, sizeof(int) * std::max<size_t>(cent.size(), 1)) == hipSuccess
+                      && hipMalloc(&s->d_cell_slot, sizeof(int64_t) * std::max<size_t>(cslot.size(), 1)) == hipSuccess
+                      && hipMalloc(&s->d_pslot_ptr, sizeof(int) * pptr.size()) == hipSuccess
+                      && hipMalloc(&s->d_pslot, sizeof(int64_t) * std::max<size_t>(pslot.size(), 1)) == hipSuccess;
+            if (!ok) return fail("hipMalloc cell tables");
+            hipMemcpy(s->d_cell_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice);
+            if (!cent.empty()) hipMemcpy(s->d_cell_ent, cent.data(), sizeof(int) * cent.size(), hipMemcpyHostToDevice);
+            if (!cslot.empty())
+                hipMemcpy(s->d_cell_slot, cslot.data(), sizeof(int64_t) * cslot.size(), hipMemcpyHostToDevice);
+            hipMemcpy(s->d_pslot_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice);
+            if (!pslot.empty())
+                hipMemcpy(s->d_pslot, pslot.data(), sizeof(int64_t) * pslot.size(), hipMemcpyHostToDevice);
+            s->use_cells = true;
+        }
+    }
     if (hipDeviceSynchronize() != hipSuccess) return fail("box table upload");
     *out = s;
     return MGX_OK;
@@ -1179,6 +1573,11 @@ int mgx_boxset_destroy(mgx_boxset* s)
     if (s->d_pairs) hipFree(s->d_pairs);
     if (s->d_nbr_ptr) hipFree(s->d_nbr_ptr);
     if (s->d_nbr) hipFree(s->d_nbr);
+    if (s->d_cell_ptr) hipFree(s->d_cell_ptr);
+    if (s->d_cell_ent) hipFree(s->d_cell_ent);
+    if (s->d_cell_slot) hipFree(s->d_cell_slot);
+    if (s->d_pslot_ptr) hipFree(s->d_pslot_ptr);
+    if (s->d_pslot) hipFree(s->d_pslot);
     delete s;
     return MGX_OK;
 }

@@ -131,3 +131,25 @@ def test_boxset_rejects_misaligned(ctx):
         mgx.BoxSet(ctx, [[2, 0, 0]], [8], (32, 32, 32), 0.3, 3)
     with pytest.raises(mgx.MgxError):
         mgx.BoxSet(ctx, [[0, 0, 0]], [16], (32, 32, 32), 0.3, 3)
+
+
+@pytest.mark.parametrize("prec", [mgx.PREC_F64, mgx.PREC_SPLIT])
+def test_box_gram_residual_pair_fallback(ctx, prec):
+    """gdim = 36 is not a multiple of the 8^3 cells: the per-pair kernels run."""
+    cfg = synth.box_config(nbox=10, gdim=36, h=0.3, edges=(8, 12, 16), nwells=3, seed=8, levels=3)
+    ob = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    hp = O.b_hloc(ob, cfg.vtot, cfg.psi)
+    n = ob.nbox
+    tp, th = to_dev(cfg.psi), to_dev(hp)
+    tS, tH = to_dev(np.zeros((n, n))), to_dev(np.zeros((n, n)))
+    ctx.call("mgx_box_gram_h", bs.h, prec, mgx.ptr(tp), mgx.ptr(th), mgx.ptr(tS), mgx.ptr(tH))
+    S, H = O.b_gram_h(ob, cfg.psi, hp)
+    tol = 1e-13 if prec == mgx.PREC_F64 else 1e-7
+    assert relmax(to_np(tS), S) <= tol and relmax(to_np(tH), H) <= tol
+    th_m = np.linalg.solve(S, H)
+    tt = to_dev(np.asfortranarray(th_m).T.copy())
+    tr = to_dev(np.zeros_like(cfg.psi))
+    ctx.call("mgx_box_residual", bs.h, mgx.ptr(tp), mgx.ptr(th), mgx.ptr(tt), mgx.ptr(tr))
+    assert relmax(to_np(tr), O.b_residual(ob, cfg.psi, hp, th_m)) <= 1e-12
+    bs.close()
