@@ -50,6 +50,19 @@ struct mgx_boxset
     int* d_pslot_ptr = nullptr;
     int64_t* d_pslot = nullptr;
     bool use_cells   = false;
+    // per box j: the other boxes i it interacts with (overlap or within 2
+    // points), 4 ints (i, sh[3]) with sh mapping j-local -> i-local; and per
+    // pair the entries (i in j's list, j in i's list) for the skin terms
+    int* d_ilist_ptr = nullptr;
+    int* d_ilist     = nullptr;
+    int64_t nilist   = 0;
+    int* d_pair_ient = nullptr;  // 2 ints per pair (-1 for i == j)
+    // column tiles of the pair-sparse S / H for sparse x dense products
+    int ntile        = 0;
+    int* d_sp_ptr    = nullptr;
+    int* d_sp_col    = nullptr;
+    int* d_sp_k      = nullptr;
+    int* d_sp_e      = nullptr;
 };
 
 namespace mgx
@@ -498,7 +511,7 @@ __device__ __forceinline__ double skin_hv(const double* pB, int eB, int d, int s
     int q[3] = { X0, X1, X2 };
     auto at  = [&](int v) {
         q[d] = v;
-        return pB[((int64_t)q[0] * eB + q[1]) * eB + q[2]];
+        return v >= 0 && v < eB ? pB[((int64_t)q[0] * eB + q[1]) * eB + q[2]] : 0.;  // edge-1 boxes
     };
     if (side == 0)
         return layer == 1 ? c1 * at(0) + c2 * at(1) : c2 * at(0);
@@ -667,9 +680,18 @@ __device__ __forceinline__ int64_t cell_idx(const CellEnt& c, int p)
     return c.base + ((int64_t)(c.b0 + x) * c.e + (c.b1 + y)) * c.e + (c.b2 + z);
 }
 
+// index of the box point at cell-local (x, y, z), or -1 outside the box
+__device__ __forceinline__ int64_t cell_idx3(const CellEnt& c, int x, int y, int z)
+{
+    if (x < c.lo0 || x >= c.hi0 || y < c.lo1 || y >= c.hi1 || z < c.lo2 || z >= c.hi2) return -1;
+    return c.base + ((int64_t)(c.b0 + x) * c.e + (c.b1 + y)) * c.e + (c.b2 + z);
+}
+
 // Per cell: G = A^T A and P = A^T B (A = psi, B = H psi, 512 x k) on MFMA.
-// Four waves split the 512 points (128 each), then combine in LDS in a fixed
-// order; partials go to part[slot + a*k + b] (G) and part[nslot + ...] (P).
+// Four waves split the 512 points (two x-planes each), then combine in LDS
+// in a fixed order; partials go to part[slot + a*k + b] (G) and
+// part[nslot + slot + a*k + b] (P).  MFMA step s of wave w covers the four
+// points x = 2w + s/16, y = (s/2)%8, z = 4(s%2) + lane/16.
 template <int PREC>
 __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
     const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part)
@@ -684,20 +706,23 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
     const int c = lane & 15, r = lane >> 4;
     const int nt = (k + 15) / 16;
     const int64_t slot = cell_slot[cell];
+    CellEnt none{};
     for (int ti = 0; ti < nt; ti++)
         for (int tj = 0; tj < nt; tj++)
         {
             const int a = ti * 16 + c, b = tj * 16 + c;
-            const bool va = a < k, vb = b < k;
+            const CellEnt A = a < k ? ce[a] : none;
+            const CellEnt B = b < k ? ce[b] : none;
             double accG[4] = { 0., 0., 0., 0. }, accP[4] = { 0., 0., 0., 0. };
             if (PREC == MGX_PREC_F64)
             {
                 d4c g = { 0., 0., 0., 0. }, h = { 0., 0., 0., 0. };
+#pragma unroll 4
                 for (int s = 0; s < 32; s++)
                 {
-                    const int p = wid * 128 + 4 * s + r;
-                    const int64_t ia = va ? cell_idx(ce[a], p) : -1;
-                    const int64_t ib = vb ? cell_idx(ce[b], p) : -1;
+                    const int x = 2 * wid + (s >> 4), y = (s >> 1) & 7, z = 4 * (s & 1) + r;
+                    const int64_t ia = cell_idx3(A, x, y, z);
+                    const int64_t ib = cell_idx3(B, x, y, z);
                     const double xa = ia >= 0 ? psi[ia] : 0.;
                     const double xb = ib >= 0 ? psi[ib] : 0.;
                     const double hb = ib >= 0 ? hpsi[ib] : 0.;
@@ -712,31 +737,35 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
             }
             else
             {
-                // hi/lo split: products hi*hi + (hi*lo + lo*hi) on FP32 MFMA,
-                // FP32 sums over the wave's 128 points, FP64 afterwards
-                f4c g1 = { 0.f, 0.f, 0.f, 0.f }, g2 = g1, h1 = g1, h2 = g1;
-                for (int s = 0; s < 32; s++)
+                // hi/lo split: hi*hi + (hi*lo + lo*hi) on FP32 MFMA; FP32 sums
+                // over 8 steps (32 points), FP64 across them
+                for (int s0 = 0; s0 < 32; s0 += 8)
                 {
-                    const int p = wid * 128 + 4 * s + r;
-                    const int64_t ia = va ? cell_idx(ce[a], p) : -1;
-                    const int64_t ib = vb ? cell_idx(ce[b], p) : -1;
-                    const double xa = ia >= 0 ? psi[ia] : 0.;
-                    const double xb = ib >= 0 ? psi[ib] : 0.;
-                    const double hb = ib >= 0 ? hpsi[ib] : 0.;
-                    const float ah = (float)xa, al = (float)(xa - (double)ah);
-                    const float bh = (float)xb, bl = (float)(xb - (double)bh);
-                    const float hh = (float)hb, hl = (float)(hb - (double)hh);
-                    g1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, g1, 0, 0, 0);
-                    g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, g2, 0, 0, 0);
-                    g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, g2, 0, 0, 0);
-                    h1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hh, h1, 0, 0, 0);
-                    h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hl, h2, 0, 0, 0);
-                    h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, hh, h2, 0, 0, 0);
-                }
-                for (int v = 0; v < 4; v++)
-                {
-                    accG[v] = (double)g1[v] + (double)g2[v];
-                    accP[v] = (double)h1[v] + (double)h2[v];
+                    f4c g1 = { 0.f, 0.f, 0.f, 0.f }, g2 = g1, h1 = g1, h2 = g1;
+#pragma unroll 4
+                    for (int s = s0; s < s0 + 8; s++)
+                    {
+                        const int x = 2 * wid + (s >> 4), y = (s >> 1) & 7, z = 4 * (s & 1) + r;
+                        const int64_t ia = cell_idx3(A, x, y, z);
+                        const int64_t ib = cell_idx3(B, x, y, z);
+                        const double xa = ia >= 0 ? psi[ia] : 0.;
+                        const double xb = ib >= 0 ? psi[ib] : 0.;
+                        const double hb = ib >= 0 ? hpsi[ib] : 0.;
+                        const float ah = (float)xa, al = (float)(xa - (double)ah);
+                        const float bh = (float)xb, bl = (float)(xb - (double)bh);
+                        const float hh = (float)hb, hl = (float)(hb - (double)hh);
+                        g1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, g1, 0, 0, 0);
+                        g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, g2, 0, 0, 0);
+                        g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, g2, 0, 0, 0);
+                        h1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hh, h1, 0, 0, 0);
+                        h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hl, h2, 0, 0, 0);
+                        h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, hh, h2, 0, 0, 0);
+                    }
+                    for (int v = 0; v < 4; v++)
+                    {
+                        accG[v] += (double)g1[v] + (double)g2[v];
+                        accP[v] += (double)h1[v] + (double)h2[v];
+                    }
                 }
             }
             for (int pass = 0; pass < 2; pass++)
@@ -760,32 +789,90 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
         }
 }
 
-// stencil reach outside the boxes, per pair: out[2p] = sum over r in box i
-// outside box j of psi_i (H psi_j), out[2p+1] the mirrored term
-__global__ void __launch_bounds__(256) k_pair_skin(const double* psi, const int64_t* off, const int* edge,
-    const int* pairs, int64_t npairs, Coef4 c4, double* out)
+// Stencil reach of H psi_j outside box j, organised by slab: block (j, s)
+// with s = face*2 + layer-1 (12 slabs of one plane each, at distance 1 and 2
+// beyond each face).  The reach values hv(X) of the slab are computed once
+// into LDS; then every interacting box i contributes
+//   out[12*q + s] = sum_{X in slab, X in box i} psi_i(X) hv(X)
+// for its entry q in j's interaction list.
+__global__ void __launch_bounds__(256) k_slab_skin(const double* psi, const int64_t* off, const int* edge,
+    const int* ilist_ptr, const int* ilist, Coef4 c4, int use_lds, double* out)
 {
-    __shared__ double lds[2 * 4];
-    const int64_t pid = blockIdx.x;
-    if (pid >= npairs) return;
-    const int* P = pairs + 11 * pid;
-    const int i = P[0], j = P[1];
-    const int ei = edge[i], ej = edge[j];
-    const int shij[3] = { P[8], P[9], P[10] }, shji[3] = { -P[8], -P[9], -P[10] };
-    double acc[2];
-    acc[0] = skin_part(psi + off[i], ei, psi + off[j], ej, shij, c4);
-    acc[1] = skin_part(psi + off[j], ej, psi + off[i], ei, shji, c4);
-    block_sum<2>(acc, lds);
-    if (threadIdx.x == 0)
+    extern __shared__ double hv[];
+    const int j = blockIdx.x / 12, sl = blockIdx.x - (blockIdx.x / 12) * 12;
+    const int d = sl >> 2, side = (sl >> 1) & 1, layer = (sl & 1) + 1;
+    const int ej = edge[j];
+    const double* pj = psi + off[j];
+    const double c1 = d == 0 ? c4.c1x : (d == 1 ? c4.c1y : c4.c1z);
+    const double c2 = d == 0 ? c4.c2x : (d == 1 ? c4.c2y : c4.c2z);
+    const int u = (d + 1) % 3, w = (d + 2) % 3;
+    const int Xd = side == 0 ? -layer : ej - 1 + layer;
+    // hv over the (u, w) plane of the slab (recomputed per use when the
+    // plane does not fit in LDS)
+    for (int t = threadIdx.x; use_lds && t < ej * ej; t += blockDim.x)
     {
-        out[2 * pid]     = acc[0];
-        out[2 * pid + 1] = acc[1];
+        const int a = t / ej, b = t - (t / ej) * ej;
+        int q[3];
+        q[u] = a;
+        q[w] = b;
+        auto at = [&](int v) {
+            q[d] = v;
+            return v < ej ? pj[((int64_t)q[0] * ej + q[1]) * ej + q[2]] : 0.;  // edge-1 boxes
+        };
+        double val;
+        if (side == 0)
+            val = layer == 1 ? c1 * at(0) + c2 * at(1) : c2 * at(0);
+        else
+            val = layer == 1 ? c1 * at(ej - 1) + (ej > 1 ? c2 * at(ej - 2) : 0.) : c2 * at(ej - 1);
+        hv[t] = val;
+    }
+    __syncthreads();
+    // one wave per interacting box: no block barriers in the loop
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int qq = ilist_ptr[j] + wid; qq < ilist_ptr[j + 1]; qq += nw)
+    {
+        const int* E = ilist + 4 * qq;
+        const int i = E[0], ei = edge[i];
+        const int sh[3] = { E[1], E[2], E[3] };
+        double acc = 0.;
+        const int xd = Xd + sh[d];
+        if (xd >= 0 && xd < ei)
+        {
+            const int lo_u = max(0, -sh[u]), hi_u = min(ej, ei - sh[u]);
+            const int lo_w = max(0, -sh[w]), hi_w = min(ej, ei - sh[w]);
+            const int nu = hi_u - lo_u, nwid = hi_w - lo_w;
+            const double* pi = psi + off[i];
+            for (int t = lane; t < nu * nwid; t += 64)
+            {
+                const int a = lo_u + t / nwid, b = lo_w + (t - (t / nwid) * nwid);
+                int x[3];
+                x[d] = xd;
+                x[u] = a + sh[u];
+                x[w] = b + sh[w];
+                double h;
+                if (use_lds)
+                    h = hv[a * ej + b];
+                else
+                {
+                    int X[3];
+                    X[d] = Xd;
+                    X[u] = a;
+                    X[w] = b;
+                    h = skin_hv(pj, ej, d, side, layer, X[0], X[1], X[2], c1, c2);
+                }
+                acc += pi[((int64_t)x[0] * ei + x[1]) * ei + x[2]] * h;
+            }
+            for (int o = 32; o > 0; o >>= 1)
+                acc += __shfl_xor(acc, o, 64);
+        }
+        if (lane == 0) out[12 * (int64_t)qq + sl] = acc;
     }
 }
 
 // S_ij, H_ij from the cell partials (slots in a fixed order) plus the skin
 __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslot_ptr, const int64_t* pslot,
-    const double* skin, const int* pairs, int64_t npairs, int n, double vel, double* S, double* H)
+    const double* skin, const int* pair_ient, const int* pairs, int64_t npairs, int n, double vel, double* S, double* H,
+    double* Sv, double* Hv)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x)
     {
@@ -797,14 +884,27 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
             h1 += part[nslot + ab];
             h2 += part[nslot + ba];
         }
-        h1 += skin[2 * t];
-        h2 += skin[2 * t + 1];
         const int i = pairs[11 * t], j = pairs[11 * t + 1];
+        if (i != j)
+        {
+            // h1 = <psi_i, H psi_j>: skin of j seen by i (entry of i in j's list)
+            const int q1 = pair_ient[2 * t], q2 = pair_ient[2 * t + 1];
+            for (int sl = 0; sl < 12; sl++)
+            {
+                h1 += skin[12 * (int64_t)q1 + sl];
+                h2 += skin[12 * (int64_t)q2 + sl];
+            }
+        }
         const double sv = g * vel, hv = 0.5 * (h1 + h2) * vel;
         S[(int64_t)j * n + i] = sv;
         S[(int64_t)i * n + j] = sv;
         H[(int64_t)j * n + i] = hv;
         H[(int64_t)i * n + j] = hv;
+        if (Sv)
+        {
+            Sv[t] = sv;
+            Hv[t] = hv;
+        }
     }
 }
 
@@ -864,7 +964,7 @@ __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const 
 }
 
 __global__ void k_pair_scatter(const double* part, const int* pairs, int64_t npairs, int n, double vel, double* S,
-    double* H)
+    double* H, double* Sv, double* Hv)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x)
     {
@@ -875,6 +975,11 @@ __global__ void k_pair_scatter(const double* part, const int* pairs, int64_t npa
         S[(int64_t)i * n + j] = s;
         H[(int64_t)j * n + i] = h;
         H[(int64_t)i * n + j] = h;
+        if (Sv)
+        {
+            Sv[t] = s;
+            Hv[t] = h;
+        }
     }
 }
 
@@ -1211,11 +1316,11 @@ int box_hloc(mgx_ctx* ctx, const mgx_boxset* bs, int dtype, const void* psi, con
 }
 
 int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, const double* hpsi, double* S,
-    double* H)
+    double* H, double* Sv, double* Hv)
 {
     const int n      = bs->nbox;
     const size_t pb  = 2 * (size_t)bs->nslot * sizeof(double);
-    const size_t sb  = 2 * (size_t)std::max<int64_t>(bs->npairs, 1) * sizeof(double);
+    const size_t sb  = 12 * (size_t)std::max<int64_t>(bs->nilist, 1) * sizeof(double);
     double* part     = (double*)workspace(ctx, 4, pb + sb);
     if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     double* skin     = part + 2 * bs->nslot;
@@ -1233,23 +1338,32 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
     MGX_CHECK_LAUNCH(ctx);
     if (bs->npairs > 0)
     {
-        hipLaunchKernelGGL(k_pair_skin, dim3((unsigned)bs->npairs), dim3(256), 0, ctx->stream, psi,
-            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_pairs, bs->npairs, coef4(bs->h),
-            skin);
-        MGX_CHECK_LAUNCH(ctx);
+        if (bs->nilist > 0)
+        {
+            size_t lds        = (size_t)bs->maxedge * bs->maxedge * sizeof(double);
+            const int use_lds = lds <= 64 * 1024;
+            if (!use_lds) lds = 0;
+            hipLaunchKernelGGL(k_slab_skin, dim3(12 * bs->nbox), dim3(256), lds, ctx->stream, psi,
+                (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_ilist_ptr,
+                (const int*)bs->d_ilist, coef4(bs->h), use_lds, skin);
+            MGX_CHECK_LAUNCH(ctx);
+        }
         const double vel = bs->h[0] * bs->h[1] * bs->h[2];
         hipLaunchKernelGGL(k_pair_reduce, dim3(nblocks(bs->npairs)), dim3(kBlock), 0, ctx->stream,
             (const double*)part, bs->nslot, (const int*)bs->d_pslot_ptr, (const int64_t*)bs->d_pslot,
-            (const double*)skin, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H);
+            (const double*)skin, (const int*)bs->d_pair_ient, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H, Sv,
+            Hv);
         MGX_CHECK_LAUNCH(ctx);
     }
     return MGX_OK;
 }
 
+// S, H dense (n x n) and, when Sv is not null, their values per pair (Sv, Hv:
+// npairs each, the layout the column-tile products read)
 int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, const double* hpsi, double* S,
-    double* H)
+    double* H, double* Sv, double* Hv)
 {
-    if (bs->use_cells) return box_gram_h_cells(ctx, bs, prec, psi, hpsi, S, H);
+    if (bs->use_cells) return box_gram_h_cells(ctx, bs, prec, psi, hpsi, S, H, Sv, Hv);
     const int n = bs->nbox;
     double* part = (double*)workspace(ctx, 4, 3 * std::max<int64_t>(bs->npairs, 1) * sizeof(double));
     if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
@@ -1269,7 +1383,7 @@ int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, 
         MGX_CHECK_LAUNCH(ctx);
         const double vel = bs->h[0] * bs->h[1] * bs->h[2];
         hipLaunchKernelGGL(k_pair_scatter, dim3(nblocks(bs->npairs)), dim3(kBlock), 0, ctx->stream,
-            (const double*)part, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H);
+            (const double*)part, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H, Sv, Hv);
     }
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
@@ -1322,6 +1436,19 @@ int box_normalize(mgx_ctx* ctx, const mgx_boxset* bs, double* psi, double* d)
 }
 
 int64_t box_size(const mgx_boxset* bs, int level) { return bs->off[level][bs->nbox]; }
+
+PairTiles boxset_tiles(const mgx_boxset* bs)
+{
+    PairTiles t;
+    t.ntile = bs->ntile;
+    t.ptr   = bs->d_sp_ptr;
+    t.col   = bs->d_sp_col;
+    t.k     = bs->d_sp_k;
+    t.e     = bs->d_sp_e;
+    return t;
+}
+
+int64_t boxset_npairs(const mgx_boxset* bs) { return bs->npairs; }
 
 void boxset_geom(const mgx_boxset* bs, int gdim[3], double h[3], int* nbox)
 {
@@ -1436,6 +1563,115 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset
                                               bb[1] + sh[1], bb[2] + sh[2], -sh[0], -sh[1], -sh[2] });
         }
     s->npairs = (int64_t)(pairs.size() / 11);
+    {
+        // interaction lists for the skin terms (other boxes only)
+        std::vector<std::vector<int>> il(s->nbox);
+        std::vector<int> pient(2 * std::max<int64_t>(s->npairs, 1), -1);
+        for (int64_t q = 0; q < s->npairs; q++)
+        {
+            const int* P = &pairs[11 * q];
+            const int i = P[0], j = P[1];
+            if (i == j) continue;
+            // entry of i in j's list: sh maps j-local -> i-local = -sh_ij
+            pient[2 * q] = (int)(il[j].size() / 4);
+            il[j].insert(il[j].end(), { i, -P[8], -P[9], -P[10] });
+            pient[2 * q + 1] = (int)(il[i].size() / 4);
+            il[i].insert(il[i].end(), { j, P[8], P[9], P[10] });
+        }
+        std::vector<int> iptr(s->nbox + 1, 0), iflat;
+        for (int j = 0; j < s->nbox; j++)
+        {
+            iptr[j] = (int)(iflat.size() / 4);
+            iflat.insert(iflat.end(), il[j].begin(), il[j].end());
+        }
+        iptr[s->nbox] = (int)(iflat.size() / 4);
+        // pair entries are local to each box's list: make them global
+        for (int64_t q = 0; q < s->npairs; q++)
+        {
+            const int i = pairs[11 * q], j = pairs[11 * q + 1];
+            if (i == j) continue;
+            pient[2 * q] += iptr[j];
+            pient[2 * q + 1] += iptr[i];
+        }
+        s->nilist = iptr[s->nbox];
+        if (hipMalloc(&s->d_ilist_ptr, sizeof(int) * iptr.size()) != hipSuccess
+            || hipMalloc(&s->d_ilist, sizeof(int) * std::max<size_t>(iflat.size(), 1)) != hipSuccess
+            || hipMalloc(&s->d_pair_ient, sizeof(int) * pient.size()) != hipSuccess)
+            return fail("hipMalloc interaction lists");
+        hipMemcpy(s->d_ilist_ptr, iptr.data(), sizeof(int) * iptr.size(), hipMemcpyHostToDevice);
+        if (!iflat.empty()) hipMemcpy(s->d_ilist, iflat.data(), sizeof(int) * iflat.size(), hipMemcpyHostToDevice);
+        hipMemcpy(s->d_pair_ient, pient.data(), sizeof(int) * pient.size(), hipMemcpyHostToDevice);
+    }
+    {
+        // column tiles for the pair-sparse products: boxes sorted along a
+        // Morton curve of their centres, kSpTile consecutive ones per tile,
+        // so that a tile's columns share most of their rows
+        auto spread = [](uint64_t v) {
+            uint64_t r = 0;
+            for (int b = 0; b < 21; b++)
+                r |= ((v >> b) & 1ull) << (3 * b);
+            return r;
+        };
+        std::vector<std::pair<uint64_t, int>> key(s->nbox);
+        for (int j = 0; j < s->nbox; j++)
+        {
+            uint64_t c[3];
+            for (int d = 0; d < 3; d++)
+                c[d] = (uint64_t)((s->origin[3 * j + d] + s->edge[j] / 2) % s->gdim[d]);
+            key[j] = { (spread(c[0]) << 2) | (spread(c[1]) << 1) | spread(c[2]), j };
+        }
+        std::sort(key.begin(), key.end());
+        std::vector<std::vector<std::pair<int, int>>> colk(s->nbox);  // (row k, pair)
+        for (int64_t q = 0; q < s->npairs; q++)
+        {
+            const int i = pairs[11 * q], j = pairs[11 * q + 1];
+            colk[j].push_back({ i, (int)q });
+            if (i != j) colk[i].push_back({ j, (int)q });
+        }
+        s->ntile = (s->nbox + kSpTile - 1) / kSpTile;
+        std::vector<int> tptr(s->ntile + 1, 0), tcol((size_t)s->ntile * kSpTile, -1), tk, te;
+        std::vector<int> pos(s->nbox, -1);
+        for (int t = 0; t < s->ntile; t++)
+        {
+            tptr[t] = (int)tk.size();
+            std::vector<int> rows;
+            for (int q = 0; q < kSpTile && t * kSpTile + q < s->nbox; q++)
+            {
+                const int r              = key[t * kSpTile + q].second;
+                tcol[t * kSpTile + q]    = r;
+                for (auto& kp : colk[r])
+                    if (pos[kp.first] < 0)
+                    {
+                        pos[kp.first] = 0;
+                        rows.push_back(kp.first);
+                    }
+            }
+            std::sort(rows.begin(), rows.end());
+            const size_t base = tk.size();
+            for (size_t u = 0; u < rows.size(); u++)
+                pos[rows[u]] = (int)u;
+            tk.insert(tk.end(), rows.begin(), rows.end());
+            te.resize(te.size() + rows.size() * kSpTile, -1);
+            for (int q = 0; q < kSpTile && t * kSpTile + q < s->nbox; q++)
+                for (auto& kp : colk[tcol[t * kSpTile + q]])
+                    te[(base + pos[kp.first]) * kSpTile + q] = kp.second;
+            for (int r : rows)
+                pos[r] = -1;
+        }
+        tptr[s->ntile] = (int)tk.size();
+        if (hipMalloc(&s->d_sp_ptr, sizeof(int) * tptr.size()) != hipSuccess
+            || hipMalloc(&s->d_sp_col, sizeof(int) * tcol.size()) != hipSuccess
+            || hipMalloc(&s->d_sp_k, sizeof(int) * std::max<size_t>(tk.size(), 1)) != hipSuccess
+            || hipMalloc(&s->d_sp_e, sizeof(int) * std::max<size_t>(te.size(), 1)) != hipSuccess)
+            return fail("hipMalloc column tiles");
+        hipMemcpy(s->d_sp_ptr, tptr.data(), sizeof(int) * tptr.size(), hipMemcpyHostToDevice);
+        hipMemcpy(s->d_sp_col, tcol.data(), sizeof(int) * tcol.size(), hipMemcpyHostToDevice);
+        if (!tk.empty())
+        {
+            hipMemcpy(s->d_sp_k, tk.data(), sizeof(int) * tk.size(), hipMemcpyHostToDevice);
+            hipMemcpy(s->d_sp_e, te.data(), sizeof(int) * te.size(), hipMemcpyHostToDevice);
+        }
+    }
     std::vector<int> ptr(s->nbox + 1, 0), flat;
     for (int j = 0; j < s->nbox; j++)
     {
@@ -1578,6 +1814,11 @@ int mgx_boxset_destroy(mgx_boxset* s)
     if (s->d_cell_slot) hipFree(s->d_cell_slot);
     if (s->d_pslot_ptr) hipFree(s->d_pslot_ptr);
     if (s->d_pslot) hipFree(s->d_pslot);
+    if (s->d_ilist_ptr) hipFree(s->d_ilist_ptr);
+    if (s->d_ilist) hipFree(s->d_ilist);
+    if (s->d_pair_ient) hipFree(s->d_pair_ient);
+    for (int* p : { s->d_sp_ptr, s->d_sp_col, s->d_sp_k, s->d_sp_e })
+        if (p) hipFree(p);
     delete s;
     return MGX_OK;
 }
@@ -1674,7 +1915,7 @@ int mgx_box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* p
     BOX_ARGS_OK(ctx, bs, 0);
     MGX_REQUIRE(ctx, psi && hpsi && S && H, "null data");
     MGX_REQUIRE(ctx, prec == MGX_PREC_F64 || prec == MGX_PREC_SPLIT, "bad precision mode");
-    int rc = box_gram_h(ctx, bs, prec, psi, hpsi, S, H);
+    int rc = box_gram_h(ctx, bs, prec, psi, hpsi, S, H, nullptr, nullptr);
     return rc ? rc : sync(ctx);
 }
 

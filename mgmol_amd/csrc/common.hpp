@@ -178,6 +178,24 @@ int prof_begin(mgx_ctx* ctx, int group);
 void prof_end(mgx_ctx* ctx, int token);
 void prof_collect(mgx_ctx* ctx);
 
+// Column tiles of a box set's pair-sparse matrices (S and H have a nonzero
+// (i, j) only for interacting pairs): tile t covers kSpTile columns (boxes
+// grouped by space), ptr[t]..ptr[t+1] index the union of their rows k, and
+// e[kSpTile*u + q] is the pair whose value sits at (k_u, col_q), or -1.
+constexpr int kSpTile = 8;
+struct PairTiles
+{
+    int ntile      = 0;
+    const int* ptr = nullptr;
+    const int* col = nullptr;  // kSpTile per tile, -1 padded
+    const int* k   = nullptr;
+    const int* e   = nullptr;
+};
+
+// Y = X A for dense symmetric-storage X (n x n, column-major) and the
+// pair-sparse symmetric A given by its per-pair values
+int spmm_pairs(mgx_ctx* ctx, int n, const double* X, const double* val, const PairTiles& pt, double* Y);
+
 // grid-family internals reused by the solver and tests
 int halo(mgx_ctx* ctx, int dtype, void* data, int nfunc, const mgx_grid_t& g);
 

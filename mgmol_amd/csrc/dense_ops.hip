@@ -313,7 +313,66 @@ __global__ void __launch_bounds__(256) k_diag_dot(const double* d, const double*
     if (threadIdx.x == 0) out[0] = 0.5 * sqrt(red[0]);
 }
 
+// Y = X A, A pair-sparse symmetric, by column tiles: block (t, chunk) holds
+// kSpTile output columns of tile t for 256 consecutive rows c, walks the
+// union of the tile's rows k once (X[c, k] = X[k, c] read down column k,
+// coalesced over c) and applies the tile's coefficients from LDS.
+__global__ void __launch_bounds__(256) k_spmm_tiles(const double* __restrict__ X, int n,
+    const double* __restrict__ val, const int* tptr, const int* tcol, const int* tk, const int* te,
+    double* __restrict__ Y)
+{
+    constexpr int U = 64;
+    __shared__ double cf[U][kSpTile];
+    __shared__ int kk[U];
+    const int t = blockIdx.x;
+    const int c = blockIdx.y * 256 + threadIdx.x;
+    double acc[kSpTile];
+    for (int q = 0; q < kSpTile; q++)
+        acc[q] = 0.;
+    const int u0 = tptr[t], u1 = tptr[t + 1];
+    for (int ub = u0; ub < u1; ub += U)
+    {
+        const int m = min(U, u1 - ub);
+        for (int z = threadIdx.x; z < U * kSpTile; z += 256)
+        {
+            const int u = z / kSpTile, q = z - (z / kSpTile) * kSpTile;
+            double v    = 0.;
+            if (u < m)
+            {
+                const int e = te[(int64_t)(ub + u) * kSpTile + q];
+                if (e >= 0) v = val[e];
+            }
+            cf[u][q] = v;
+        }
+        if (threadIdx.x < U) kk[threadIdx.x] = threadIdx.x < m ? tk[ub + threadIdx.x] : 0;
+        __syncthreads();
+        if (c < n)
+            for (int u = 0; u < m; u++)
+            {
+                const double x = X[c + (int64_t)kk[u] * n];
+                for (int q = 0; q < kSpTile; q++)
+                    acc[q] += cf[u][q] * x;
+            }
+        __syncthreads();
+    }
+    if (c < n)
+        for (int q = 0; q < kSpTile; q++)
+        {
+            const int r = tcol[t * kSpTile + q];
+            if (r >= 0) Y[c + (int64_t)r * n] = acc[q];
+        }
+}
+
 }  // namespace
+
+int spmm_pairs(mgx_ctx* ctx, int n, const double* X, const double* val, const PairTiles& pt, double* Y)
+{
+    if (pt.ntile <= 0) return set_err(ctx, MGX_EINVAL, "no column tiles");
+    hipLaunchKernelGGL(k_spmm_tiles, dim3(pt.ntile, (n + 255) / 256), dim3(256), 0, ctx->stream, X, n, val, pt.ptr,
+        pt.col, pt.k, pt.e, Y);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
 
 int tsgemm(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_t ld, int64_t lnp, int n, int subdivx,
     double vel, int lower_only, double* ss)
@@ -424,6 +483,56 @@ int inverse_update(mgx_ctx* ctx, int n, const double* S, double* L, double* X, d
         prev = delta;
         if (delta < 3e-8) break;  // the error after this step is ~delta^2
     }
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, X, n);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+// inverse_update for a pair-sparse S (values Sv per pair, column tiles pt):
+// T = X S on the sparse product, X <- 2X - T X as one dense GEMM (written to
+// the other half of the work buffer, operands swapped, no copies).
+int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv, const PairTiles& pt, double* L,
+    double* X, double* work, int* steps)
+{
+    const double mone = -1., two = 2.;
+    const size_t nn = (size_t)n * n;
+    double* T   = work;
+    double* cur = X;
+    double* nxt = work + nn;
+    double* bm  = (double*)workspace(ctx, 4, 1024 * sizeof(double));
+    if (!bm) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    double prev = 1e300;
+    *steps      = 0;
+    int rc;
+    for (int it = 0; it < 8; it++)
+    {
+        if ((rc = spmm_pairs(ctx, n, cur, Sv, pt, T))) return rc;
+        const int nb = 512;
+        hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
+        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double*)bm, nb, bm + nb);
+        MGX_CHECK_LAUNCH(ctx);
+        double delta = 0.;
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(&delta, bm + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (!(delta < 0.25) || delta > 0.5 * prev)
+        {
+            *steps = -1;
+            return chol_inv(ctx, n, S, L, X);
+        }
+        if (delta < 1e-15) break;
+        // nxt = 2 cur - T cur
+        if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
+                rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n, nxt,
+                rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
+            != rocblas_status_success)
+            return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex failed");
+        std::swap(cur, nxt);
+        (*steps)++;
+        prev = delta;
+        if (delta < 3e-8) break;  // the error after this step is ~delta^2
+    }
+    if (cur != X) MGX_CHECK_HIP(ctx, hipMemcpyAsync(X, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, X, n);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;

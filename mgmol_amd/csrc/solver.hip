@@ -15,13 +15,18 @@
 // equals the S the reference recomputes after normalize()).
 #include "common.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
 namespace mgx
 {
 int box_hloc(mgx_ctx*, const mgx_boxset*, int, const void*, const double*, void*);
-int box_gram_h(mgx_ctx*, const mgx_boxset*, int, const double*, const double*, double*, double*);
+int box_gram_h(mgx_ctx*, const mgx_boxset*, int, const double*, const double*, double*, double*, double*, double*);
+PairTiles boxset_tiles(const mgx_boxset*);
+int64_t boxset_npairs(const mgx_boxset*);
+int inverse_update_pairs(mgx_ctx*, int, const double*, const double*, const PairTiles&, double*, double*, double*,
+    int*);
 int box_residual(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, double*);
 int box_sumsq(mgx_ctx*, const mgx_boxset*, const double*, double*);
 int box_norm(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
@@ -52,6 +57,8 @@ struct mgx_solver
     MgxProf prof;
     double *S = nullptr, *H = nullptr, *L = nullptr, *Sinv = nullptr, *th = nullptr;
     double *eig = nullptr, *d = nullptr, *scal = nullptr, *nswork = nullptr;
+    double *Sv = nullptr, *Hv = nullptr;  // S, H per interacting pair
+    mgx::PairTiles tiles;
     bool have_inv = false;  // Sinv holds the inverse of a recent S
     int64_t ns_steps = 0, chol_calls = 0;
     double* host = nullptr;  // pinned: [energy, norm]
@@ -65,7 +72,7 @@ namespace
 void free_solver(mgx_solver* s)
 {
     double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
-        s->scal, s->nswork };
+        s->scal, s->nswork, s->Sv, s->Hv };
     for (double* b : bufs)
         if (b) hipFree(b);
     if (s->host) hipHostFree(s->host);
@@ -83,7 +90,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     if ((rc = box_hloc(ctx, s->bs, MGX_F64, s->psi, s->vtot, s->hpsi))) return rc;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_GRAM);
-    if ((rc = box_gram_h(ctx, s->bs, s->cfg.gram_prec, s->psi, s->hpsi, s->S, s->H))) return rc;
+    if ((rc = box_gram_h(ctx, s->bs, s->cfg.gram_prec, s->psi, s->hpsi, s->S, s->H, s->Sv, s->Hv))) return rc;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_DENSE);
     // S^-1: Cholesky + inverse (GramMatrix::updateLS / computeInverse) on the
@@ -97,13 +104,14 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     else
     {
         int steps = 0;
-        if ((rc = inverse_update(ctx, n, s->S, s->L, s->Sinv, s->nswork, &steps))) return rc;
+        if ((rc = inverse_update_pairs(ctx, n, s->S, s->Sv, s->tiles, s->L, s->Sinv, s->nswork, &steps))) return rc;
         if (steps < 0)
             s->chol_calls++;
         else
             s->ns_steps += steps;
     }
-    if ((rc = theta(ctx, n, s->Sinv, s->H, s->th))) return rc;
+    // theta = S^-1 H with H pair-sparse
+    if ((rc = spmm_pairs(ctx, n, s->Sinv, s->Hv, s->tiles, s->th))) return rc;
     // tr(H S^-1) = tr(S^-1 H) = tr(theta)   (ProjectedMatrices::getEigSum)
     if ((rc = diag_sum(ctx, n, s->th, s->scal))) return rc;
     prof_end(ctx, tk);
@@ -187,12 +195,15 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     mgx_boxset_size(bs, 0, &s->npts);
     int g[3];
     boxset_geom(bs, g, s->h, &s->n);
+    s->tiles = boxset_tiles(bs);
     const int64_t gsize = (int64_t)g[0] * g[1] * g[2];
     const size_t nn     = (size_t)s->n * s->n;
     auto alloc = [&](double** p, size_t cnt) { return hipMalloc((void**)p, cnt * sizeof(double)) == hipSuccess; };
     bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
               && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
               && alloc(&s->eig, s->n) && alloc(&s->d, s->n + 8) && alloc(&s->scal, 8) && alloc(&s->nswork, 2 * nn)
+              && alloc(&s->Sv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
+              && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {

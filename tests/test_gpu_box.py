@@ -109,7 +109,7 @@ def test_box_gram_h_residual(ctx, boxes, cfg, prec):
     S, H = O.b_gram_h(ob, cfg.psi, hp)
     es, eh = relmax(to_np(tS), S), relmax(to_np(tH), H)
     say("gram/H prec", prec, es, eh)
-    # split: FP32 accumulation over 16-point chunks (measured ~2e-8)
+    # split: FP32 accumulation over 32-point chunks, FP64 beyond
     tol = 1e-13 if prec == mgx.PREC_F64 else 1e-7
     assert es <= tol and eh <= tol
     assert bs.npairs() == len(O.b_pairs(ob)[0])
