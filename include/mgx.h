@@ -220,6 +220,33 @@ int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, cons
 int mgx_box_residual_norm(mgx_ctx* ctx, const mgx_boxset* bs, const double* res,
     const double* Sinv, double* norm);
 
+/* ------------------------------------------------ learned preconditioner ---- */
+/* No counterpart in the reference: a 4-layer 3D CNN (1->16->16->16->1 channels,
+ * 3^3 taps, zero padding per box, ReLU, scalar skip y = s x + conv(h)) that
+ * maps a box's residual to a correction in place of the V-cycle of
+ * OrbitalsPreconditioning::precond_mg (src/OrbitalsPreconditioning.cc:87-117),
+ * which stays its judge and fallback.  BF16 weights/activations, FP32
+ * accumulation on MFMA; FP32 master parameters trained with Adam.  Samples
+ * are cubes of edge `edge`, packed one after another. */
+typedef struct mgx_cnn mgx_cnn;
+int mgx_cnn_nparams(void);
+int mgx_cnn_create(mgx_ctx* ctx, int edge, int max_batch, mgx_cnn** out);
+int mgx_cnn_destroy(mgx_cnn* c);
+/* FP32 host parameters: w0[27][16][1] b0[16] w1[27][16][16] b1[16]
+ * w2[27][16][16] b2[16] w3[27][1][16] b3[1] s; tap = (dx*3+dy)*3+dz */
+int mgx_cnn_set_params(mgx_cnn* c, const float* host);
+int mgx_cnn_get_params(mgx_cnn* c, float* host);
+/* y = net(x) for nb samples (FP32, device) */
+int mgx_cnn_forward(mgx_cnn* c, const float* x, int nb, float* y);
+/* mean squared error of net(x) against t and its gradient (host, nparams) */
+int mgx_cnn_grad(mgx_cnn* c, const float* x, const float* t, int nb, double* loss, float* grad_host);
+/* one Adam step (beta 0.9/0.999, eps 1e-8) on a batch; loss before the step */
+int mgx_cnn_train_step(mgx_cnn* c, const float* x, const float* t, int nb, float lr, double* loss);
+/* out_j = rms_j * net(R_j / rms_j) over nbox boxes of the CNN's edge (FP64, device) */
+int mgx_cnn_apply(mgx_cnn* c, int nbox, const double* R, double* out);
+/* training pairs x = R_j / rms_j, t = KR_j / rms_j (FP32, device) */
+int mgx_cnn_make_pairs(mgx_cnn* c, int nbox, const double* R, const double* KR, float* x, float* t);
+
 /* ------------------------------------------------------------- solver ---- */
 /* ABPG<T>::updateWF / update_states with DFTsolver's per-step normalisation,
  * src/ABPG.cc:41-140, src/DFTsolver.cc:262-401, on a box set. */
@@ -233,10 +260,28 @@ typedef struct
     int ritz_every;      /* recompute lowest Ritz value every k steps (0 = first only) */
     int precond_start;   /* MGX_START_GAMMA (reference) or MGX_START_ZERO */
     int profile;         /* 1: time kernel groups with hipEvents (mgx_solver_profile) */
+    int anderson_m;      /* ABPG.m: 0 = plain preconditioned steps (PSD), 1 = Anderson
+                            extrapolation with one previous step (src/AndersonMix.cc) */
+    double anderson_beta;/* ABPG.beta (1.0) */
+    int conv_criterion;  /* Quench.conv_criterion: MGX_CONV_DELTAE or MGX_CONV_RESIDUAL */
+    int precond_kind;    /* MGX_PRECOND_VCYCLE, or MGX_PRECOND_LEARNED (mgx_solver_set_cnn) */
 } mgx_solver_cfg_t;
+
+/* MGX_PRECOND_LEARNED: the step is alpha * CNN(R); when the next residual
+ * norm is larger, psi and R are restored, the V-cycle step is taken instead
+ * and the V-cycle is used for the next MGX_LEARNED_BACKOFF steps. */
+#define MGX_PRECOND_VCYCLE 0
+#define MGX_PRECOND_LEARNED 1
+#define MGX_LEARNED_BACKOFF 4
 
 #define MGX_START_GAMMA 0
 #define MGX_START_ZERO 1
+
+/* DFTsolver::checkConvergenceEnergy (src/DFTsolver.cc:138-167): after step 2,
+ * converged when max(|dE_k|, |dE_k-1|) of the band energy is below tol (Ha);
+ * or ABPG::updateWF's residual test |R| < tol (Ha). */
+#define MGX_CONV_DELTAE 0
+#define MGX_CONV_RESIDUAL 1
 
 /* kernel groups timed when cfg.profile = 1 */
 #define MGX_PROF_HLOC 0      /* H psi */
@@ -256,6 +301,8 @@ typedef struct
     double band_energy;  /* tr(S^-1 H), Ha */
     double seconds;
     double lowest_eig;   /* Ry */
+    int learned_steps;   /* steps taken with the CNN (accepted or not), cumulative */
+    int fallbacks;       /* CNN steps rejected and redone with the V-cycle, cumulative */
 } mgx_solver_stats_t;
 
 int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t* cfg,
@@ -270,8 +317,13 @@ int mgx_solver_reset(mgx_solver* s);
  * arrays of MGX_PROF_N); reset=1 clears the accumulators afterwards */
 int mgx_solver_profile(mgx_solver* s, double* ms, int64_t* launches, int reset);
 int mgx_solver_get_psi(mgx_solver* s, double* psi_host);
-/* device pointers of the solver's psi / hpsi / S / H (for inspection) */
-int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H);
+/* device pointers of the solver's psi / hpsi / S / H / residual (for inspection; the
+ * residual is that of the last evaluated step when cfg.anderson_m = 0 and
+ * precond_kind = MGX_PRECOND_VCYCLE, otherwise overwritten by the step) */
+int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H, double** res);
+/* attach a trained network (uniform box edge equal to the network's) for
+ * cfg.precond_kind = MGX_PRECOND_LEARNED; the solver does not own it */
+int mgx_solver_set_cnn(mgx_solver* s, mgx_cnn* cnn);
 
 #ifdef __cplusplus
 }

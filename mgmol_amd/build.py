@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libmgx.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ctx.hip", "grid_ops.hip", "box_ops.hip", "dense_ops.hip", "solver.hip"]
+SOURCES = ["ctx.hip", "grid_ops.hip", "box_ops.hip", "dense_ops.hip", "solver.hip", "cnn_ops.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",

@@ -1020,6 +1020,65 @@ __global__ void __launch_bounds__(256) k_box_sumsq(const double* r, const int64_
     if (threadIdx.x == 0) d[j] = acc[0] * vel;
 }
 
+// Anderson mixing inner products (AndersonMix::update, src/AndersonMix.cc,
+// m = 1, with LocGridOrbitals::dotProductDiagonal): per box j,
+//   dd_j = vel <af - f0, af - f0>,  df_j = vel <af - f0, af>   with f = a*kr
+__global__ void __launch_bounds__(256) k_box_and_dots(const double* kr, double a, const double* f0,
+    const int64_t* off, int nbox, double vel, double* d)
+{
+    __shared__ double lds[8];
+    const int j = blockIdx.x;
+    if (j >= nbox) return;
+    double acc[2] = { 0., 0. };
+    for (int64_t q = off[j] + threadIdx.x; q < off[j + 1]; q += blockDim.x)
+    {
+        const double f = a * kr[q], df = f - f0[q];
+        acc[0] += df * df;
+        acc[1] += df * f;
+    }
+    block_sum<2>(acc, lds);
+    if (threadIdx.x == 0)
+    {
+        d[j]        = acc[0] * vel;
+        d[nbox + j] = acc[1] * vel;
+    }
+}
+
+// out[k] = sum_j d[k*n + j] Sinv_jj, k = 0, 1 (single block)
+__global__ void __launch_bounds__(256) k_diag_wsum2(const double* d, const double* Sinv, int n, double* out)
+{
+    __shared__ double lds[8];
+    double acc[2] = { 0., 0. };
+    for (int j = threadIdx.x; j < n; j += blockDim.x)
+    {
+        const double w = Sinv[(int64_t)j * n + j];
+        acc[0] += d[j] * w;
+        acc[1] += d[n + j] * w;
+    }
+    block_sum<2>(acc, lds);
+    if (threadIdx.x == 0)
+    {
+        out[0] = acc[0];
+        out[1] = acc[1];
+    }
+}
+
+// x_bar = (1-th) x + th x0, f_bar = (1-th) f + th f0 (f = a*kr); then
+// x0 <- x, f0 <- f, x <- x_bar + beta f_bar
+__global__ void k_and_update(double* x, double* x0, const double* kr, double a, double* f0, int64_t n, double th,
+    double beta)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const double xv = x[t], x0v = x0[t], fv = a * kr[t], f0v = f0[t];
+        const double xb = (1. - th) * xv + th * x0v;
+        const double fb = (1. - th) * fv + th * f0v;
+        x0[t] = xv;
+        f0[t] = fv;
+        x[t]  = xb + beta * fb;
+    }
+}
+
 // 0.5*sqrt(sum_j d_j Sinv_jj): single block
 __global__ void __launch_bounds__(256) k_norm(const double* d, const double* Sinv, int n, double* out)
 {
@@ -1407,6 +1466,28 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
     return MGX_OK;
 }
 
+// Anderson m = 1 inner products, weighted by diag(S^-1): out[0] =
+// <f - f0, f - f0>, out[1] = <f - f0, f> (device), f = a*kr.  d: 2 n scratch.
+int box_anderson_dots(mgx_ctx* ctx, const mgx_boxset* bs, const double* kr, double a, const double* f0,
+    const double* Sinv, double* d, double* out)
+{
+    const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+    hipLaunchKernelGGL(k_box_and_dots, dim3(bs->nbox), dim3(256), 0, ctx->stream, kr, a, f0,
+        (const int64_t*)bs->d_off[0], bs->nbox, vel, d);
+    hipLaunchKernelGGL(k_diag_wsum2, dim3(1), dim3(256), 0, ctx->stream, (const double*)d, Sinv, bs->nbox, out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_anderson_update(mgx_ctx* ctx, const mgx_boxset* bs, double* x, double* x0, const double* kr, double a,
+    double* f0, double th, double beta)
+{
+    const int64_t n = bs->off[0][bs->nbox];
+    hipLaunchKernelGGL(k_and_update, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, x, x0, kr, a, f0, n, th, beta);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
 int box_sumsq(mgx_ctx* ctx, const mgx_boxset* bs, const double* r, double* d)
 {
     const double vel = bs->h[0] * bs->h[1] * bs->h[2];
@@ -1449,6 +1530,14 @@ PairTiles boxset_tiles(const mgx_boxset* bs)
 }
 
 int64_t boxset_npairs(const mgx_boxset* bs) { return bs->npairs; }
+
+bool boxset_uniform_edge(const mgx_boxset* bs, int* edge)
+{
+    *edge = bs->edge.empty() ? 0 : bs->edge[0];
+    for (int e : bs->edge)
+        if (e != *edge) return false;
+    return true;
+}
 
 void boxset_geom(const mgx_boxset* bs, int gdim[3], double h[3], int* nbox)
 {

@@ -40,6 +40,11 @@ int trace_prod(mgx_ctx*, int, const double*, const double*, double*);
 int diag_sum(mgx_ctx*, int, const double*, double*);
 int inverse_update(mgx_ctx*, int, const double*, double*, double*, double*, int*);
 int ritz(mgx_ctx*, int, const double*, const double*, double*, double*);
+int box_anderson_dots(mgx_ctx*, const mgx_boxset*, const double*, double, const double*, const double*, double*,
+    double*);
+int box_anderson_update(mgx_ctx*, const mgx_boxset*, double*, double*, const double*, double, double*, double, double);
+int cnn_apply_boxes(mgx_cnn*, int, int, const double*, double*);
+bool boxset_uniform_edge(const mgx_boxset*, int*);
 }  // namespace mgx
 
 struct mgx_solver
@@ -58,6 +63,18 @@ struct mgx_solver
     double *S = nullptr, *H = nullptr, *L = nullptr, *Sinv = nullptr, *th = nullptr;
     double *eig = nullptr, *d = nullptr, *scal = nullptr, *nswork = nullptr;
     double *Sv = nullptr, *Hv = nullptr;  // S, H per interacting pair
+    double *x0 = nullptr, *f0 = nullptr;  // Anderson history: previous psi and step
+    // DFTsolver::checkConvergenceEnergy state (band energy, Ha)
+    double e_prev = 0., deig = 1e9, deig2 = 1e9;
+    bool accelerate = false;
+    int mm          = 0;  // Anderson history length in use
+    // learned preconditioner: the state before the last CNN step, to undo it
+    mgx_cnn* cnn       = nullptr;
+    double *psi_prev = nullptr, *r_prev = nullptr;
+    bool cnn_pending   = false;
+    double r_prev_norm = 0.;
+    int backoff        = 0;
+    int learned_steps = 0, fallbacks = 0;
     mgx::PairTiles tiles;
     bool have_inv = false;  // Sinv holds the inverse of a recent S
     int64_t ns_steps = 0, chol_calls = 0;
@@ -69,10 +86,17 @@ namespace mgx
 namespace
 {
 
+// psi += alpha x (ABPG::update_states without extrapolation)
+__global__ void k_step(double* psi, const double* x, int64_t n, double alpha)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        psi[t] += alpha * x[t];
+}
+
 void free_solver(mgx_solver* s)
 {
     double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
-        s->scal, s->nswork, s->Sv, s->Hv };
+        s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->f0, s->psi_prev, s->r_prev };
     for (double* b : bufs)
         if (b) hipFree(b);
     if (s->host) hipHostFree(s->host);
@@ -127,8 +151,51 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         return set_err(ctx, MGX_ENUMERIC, "non-finite energy or residual");
     st->band_energy = 0.5 * e_ry;  // Ry -> Ha
     st->residual    = rnorm;
-    *done           = (tol > 0. && rnorm < tol);
+    // energy history (DFTsolver::checkConvergenceEnergy): deig2 = max of the
+    // last two band-energy changes, 1e9 until two steps exist
+    if (s->iter > 0)
+    {
+        const double deig_old = s->deig;
+        s->deig               = std::fabs(st->band_energy - s->e_prev);
+        s->deig2              = std::max(s->deig, deig_old);
+    }
+    s->e_prev = st->band_energy;
+    // Anderson starts once the eigenvalue-sum change (2 x Ry = 4 x Ha of the
+    // band energy) is below 1e-2 per state
+    s->accelerate = s->accelerate || (s->cfg.anderson_m > 0 && 4. * s->deig2 < 1e-2 * n);
+    if (s->cfg.conv_criterion == MGX_CONV_DELTAE)
+        *done = (tol > 0. && s->iter > 1 && s->deig2 < tol);
+    else
+        *done = (tol > 0. && rnorm < tol);
     if (*done) return MGX_OK;
+    // learned preconditioner: judge the last CNN step by the residual norm
+    bool use_cnn = false, revert = false;
+    if (s->cfg.precond_kind == MGX_PRECOND_LEARNED)
+    {
+        if (s->cnn_pending && rnorm > s->r_prev_norm)
+        {
+            revert = true;
+            s->fallbacks++;
+            s->backoff = MGX_LEARNED_BACKOFF;
+        }
+        s->cnn_pending = false;
+        if (revert)
+        {
+            // back to the state before the CNN step; redo it with the V-cycle
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi, s->psi_prev, s->npts * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->res, s->r_prev, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+            s->mm = 0;
+        }
+        else if (s->backoff > 0)
+            s->backoff--;
+        else
+        {
+            if (!s->cnn) return set_err(ctx, MGX_EINVAL, "no network attached (mgx_solver_set_cnn)");
+            use_cnn = true;
+        }
+    }
     double gamma = 0.;
     if (s->cfg.precond_start == MGX_START_GAMMA)
     {
@@ -146,11 +213,85 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         gamma = mgx_gamma(s->h, s->cfg.mg_levels, s->vmax, s->lowest);
     }
     st->lowest_eig     = s->lowest;
+    st->learned_steps  = s->learned_steps;
+    st->fallbacks      = s->fallbacks;
     const double alpha = 0.5 * s->cfg.step_length;
     tk = prof_begin(ctx, MGX_PROF_PRECOND);
-    if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, s->cfg.precond_dtype, s->res, nullptr,
-             s->psi, alpha)))
-        return rc;
+    if (use_cnn)
+    {
+        // keep the pre-step state, then R -> CNN(R) in place
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi_prev, s->psi, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->r_prev, s->res, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        int edge = 0;
+        boxset_uniform_edge(s->bs, &edge);
+        if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->res))) return rc;
+        s->cnn_pending = true;
+        s->r_prev_norm = rnorm;
+        s->learned_steps++;
+        if (s->accelerate)
+        {
+            double th = 0., beta = 1.;
+            if (s->mm > 0)
+            {
+                if ((rc = box_anderson_dots(ctx, s->bs, s->res, alpha, s->f0, s->Sinv, s->d, s->scal + 2)))
+                    return rc;
+                MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host + 3, s->scal + 2, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+                MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                const double mat = s->host[3], rhs = s->host[4];
+                th = mat > 0. ? rhs / mat : 0.;
+                if (th > 0.5)
+                    th = th > 1. ? -0.5 : 0.;
+                else if (th < -3.)
+                    th = -3.;
+                beta = s->cfg.anderson_beta;
+            }
+            if ((rc = box_anderson_update(ctx, s->bs, s->psi, s->x0, s->res, alpha, s->f0, th, beta))) return rc;
+            s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
+        }
+        else
+        {
+            hipLaunchKernelGGL(k_step, dim3(nblocks(s->npts)), dim3(kBlock), 0, ctx->stream, s->psi,
+                (const double*)s->res, s->npts, alpha);
+            MGX_CHECK_LAUNCH(ctx);
+        }
+    }
+    else if (!s->accelerate)
+    {
+        // preconditioned step psi += alpha K(R) (ABPG::update_states, no extrapolation)
+        if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, s->cfg.precond_dtype, s->res,
+                 nullptr, s->psi, alpha)))
+            return rc;
+    }
+    else
+    {
+        // K(R) into res (the V-cycle has consumed it), then AndersonMix::update
+        // with m = 1 on f = alpha K(R)
+        if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, s->cfg.precond_dtype, s->res,
+                 s->res, nullptr, 0.)))
+            return rc;
+        double th = 0., beta = 1.;
+        if (s->mm > 0)
+        {
+            if ((rc = box_anderson_dots(ctx, s->bs, s->res, alpha, s->f0, s->Sinv, s->d, s->scal + 2))) return rc;
+            MGX_CHECK_HIP(ctx,
+                hipMemcpyAsync(s->host + 3, s->scal + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            const double mat = s->host[3], rhs = s->host[4];
+            th = mat > 0. ? rhs / mat : 0.;
+            // theta bounds of AndersonMix (max 0.5, min -3; with one history
+            // vector an out-of-range theta is reset, not the history shortened)
+            if (th > 0.5)
+                th = th > 1. ? -0.5 : 0.;
+            else if (th < -3.)
+                th = -3.;
+            beta = s->cfg.anderson_beta;
+        }
+        if ((rc = box_anderson_update(ctx, s->bs, s->psi, s->x0, s->res, alpha, s->f0, th, beta))) return rc;
+        s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
+    }
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
     if ((rc = box_normalize(ctx, s->bs, s->psi, s->d))) return rc;
@@ -186,6 +327,16 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     MGX_REQUIRE(ctx, cfg->precond_start == MGX_START_GAMMA || cfg->precond_start == MGX_START_ZERO,
         "bad precond start");
     MGX_REQUIRE(ctx, cfg->step_length > 0., "step length must be positive");
+    MGX_REQUIRE(ctx, cfg->anderson_m == 0 || cfg->anderson_m == 1, "Anderson history length must be 0 or 1");
+    MGX_REQUIRE(ctx, cfg->precond_kind == MGX_PRECOND_VCYCLE || cfg->precond_kind == MGX_PRECOND_LEARNED,
+        "bad preconditioner kind");
+    if (cfg->precond_kind == MGX_PRECOND_LEARNED)
+    {
+        int e = 0;
+        MGX_REQUIRE(ctx, boxset_uniform_edge(bs, &e), "the learned preconditioner needs boxes of one edge");
+    }
+    MGX_REQUIRE(ctx, cfg->conv_criterion == MGX_CONV_DELTAE || cfg->conv_criterion == MGX_CONV_RESIDUAL,
+        "bad convergence criterion");
     int64_t dummy = 0;
     MGX_REQUIRE(ctx, mgx_boxset_size(bs, cfg->mg_levels, &dummy) == MGX_OK, "box set has too few levels");
     mgx_solver* s = new mgx_solver();
@@ -201,9 +352,12 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     auto alloc = [&](double** p, size_t cnt) { return hipMalloc((void**)p, cnt * sizeof(double)) == hipSuccess; };
     bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
               && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
-              && alloc(&s->eig, s->n) && alloc(&s->d, s->n + 8) && alloc(&s->scal, 8) && alloc(&s->nswork, 2 * nn)
+              && alloc(&s->eig, s->n) && alloc(&s->d, 2 * (size_t)s->n + 8) && alloc(&s->scal, 8) && alloc(&s->nswork, 2 * nn)
               && alloc(&s->Sv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
+              && (cfg->anderson_m == 0 || (alloc(&s->x0, s->npts) && alloc(&s->f0, s->npts)))
+              && (cfg->precond_kind != MGX_PRECOND_LEARNED
+                  || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts)))
               && hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {
@@ -273,9 +427,18 @@ int mgx_solver_reset(mgx_solver* s)
     MGX_CHECK_HIP(s->ctx, hipMemcpyAsync(s->psi, s->psi0, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
                               s->ctx->stream));
     MGX_CHECK_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
-    s->iter     = 0;
-    s->lowest   = 0.;
-    s->have_inv = false;
+    s->iter       = 0;
+    s->lowest     = 0.;
+    s->have_inv   = false;
+    s->e_prev     = 0.;
+    s->deig       = 1e9;
+    s->deig2      = 1e9;
+    s->accelerate = false;
+    s->mm         = 0;
+    s->cnn_pending   = false;
+    s->backoff       = 0;
+    s->learned_steps = 0;
+    s->fallbacks     = 0;
     return MGX_OK;
 }
 
@@ -302,13 +465,21 @@ int mgx_solver_get_psi(mgx_solver* s, double* host)
     return MGX_OK;
 }
 
-int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H)
+int mgx_solver_set_cnn(mgx_solver* s, mgx_cnn* cnn)
+{
+    if (!s) return MGX_EINVAL;
+    s->cnn = cnn;
+    return MGX_OK;
+}
+
+int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H, double** res)
 {
     if (!s) return MGX_EINVAL;
     if (psi) *psi = s->psi;
     if (hpsi) *hpsi = s->hpsi;
     if (S) *S = s->S;
     if (H) *H = s->H;
+    if (res) *res = s->res;
     return MGX_OK;
 }
 

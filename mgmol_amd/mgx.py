@@ -14,6 +14,8 @@ F32, F64 = 0, 1
 LAP_MEHRSTELLEN, LAP_2ND, LAP_4TH = 0, 1, 2
 PREC_F64, PREC_SPLIT = 0, 1
 START_GAMMA, START_ZERO = 0, 1
+CONV_DELTAE, CONV_RESIDUAL = 0, 1
+PRECOND_VCYCLE, PRECOND_LEARNED = 0, 1
 PROF_GROUPS = ["hloc", "gram", "dense", "resid", "jacobi0", "precond", "normalize"]
 
 _LIB = None
@@ -31,7 +33,9 @@ EXPORTS = [
     "mgx_box_lap", "mgx_box_hloc", "mgx_box_jacobi", "mgx_box_restrict", "mgx_box_prolong", "mgx_box_precond_mg",
     "mgx_box_gram_h", "mgx_box_residual", "mgx_box_residual_norm",
     "mgx_solver_create", "mgx_solver_destroy", "mgx_solver_step", "mgx_solver_run", "mgx_solver_reset",
-    "mgx_solver_profile", "mgx_solver_get_psi", "mgx_solver_buffers",
+    "mgx_solver_profile", "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_set_cnn",
+    "mgx_cnn_nparams", "mgx_cnn_create", "mgx_cnn_destroy", "mgx_cnn_set_params", "mgx_cnn_get_params",
+    "mgx_cnn_forward", "mgx_cnn_grad", "mgx_cnn_train_step", "mgx_cnn_apply", "mgx_cnn_make_pairs",
 ]
 
 
@@ -61,12 +65,14 @@ class Boxes(C.Structure):
 class SolverCfg(C.Structure):
     _fields_ = [("lap_type", C.c_int), ("mg_levels", C.c_int), ("precond_dtype", C.c_int),
                 ("gram_prec", C.c_int), ("step_length", C.c_double), ("ritz_every", C.c_int),
-                ("precond_start", C.c_int), ("profile", C.c_int)]
+                ("precond_start", C.c_int), ("profile", C.c_int), ("anderson_m", C.c_int),
+                ("anderson_beta", C.c_double), ("conv_criterion", C.c_int), ("precond_kind", C.c_int)]
 
 
 class SolverStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
-                ("band_energy", C.c_double), ("seconds", C.c_double), ("lowest_eig", C.c_double)]
+                ("band_energy", C.c_double), ("seconds", C.c_double), ("lowest_eig", C.c_double),
+                ("learned_steps", C.c_int), ("fallbacks", C.c_int)]
 
 
 class MgxError(RuntimeError):
@@ -112,7 +118,11 @@ def lib():
             "mgx_solver_destroy": (P,), "mgx_solver_step": (P, C.POINTER(SolverStats)),
             "mgx_solver_run": (P, D, I, C.POINTER(SolverStats)), "mgx_solver_get_psi": (P, P),
             "mgx_solver_reset": (P,), "mgx_solver_profile": (P, C.POINTER(D), C.POINTER(I64), I),
-            "mgx_solver_buffers": (P, PP, PP, PP, PP),
+            "mgx_solver_buffers": (P, PP, PP, PP, PP, PP), "mgx_solver_set_cnn": (P, P),
+            "mgx_cnn_create": (P, I, I, C.POINTER(P)), "mgx_cnn_destroy": (P,),
+            "mgx_cnn_set_params": (P, P), "mgx_cnn_get_params": (P, P), "mgx_cnn_forward": (P, P, I, P),
+            "mgx_cnn_grad": (P, P, P, I, C.POINTER(D), P), "mgx_cnn_train_step": (P, P, P, I, C.c_float, C.POINTER(D)),
+            "mgx_cnn_apply": (P, I, P, P), "mgx_cnn_make_pairs": (P, I, P, P, P, P),
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -123,6 +133,8 @@ def lib():
         L.mgx_last_error.restype = C.c_char_p
         L.mgx_gamma.argtypes = [C.POINTER(D), I, D, D]
         L.mgx_gamma.restype = D
+        L.mgx_cnn_nparams.argtypes = []
+        L.mgx_cnn_nparams.restype = I
         _LIB = L
     return _LIB
 
@@ -174,7 +186,7 @@ def dtype_code(t):
 class DeviceArray:
     """A device buffer owned through the C ABI (mgx_malloc / mgx_free)."""
 
-    def __init__(self, ctx, src=None, shape=None, dtype=None):
+    def __init__(self, ctx, src=None, shape=None, dtype=None, zero=True):
         import numpy as np
         self.ctx = ctx
         if src is not None:
@@ -187,7 +199,7 @@ class DeviceArray:
         ctx.check(lib().mgx_malloc(ctx.h, self.nbytes, C.byref(self._p)), "mgx_malloc")
         if src is not None:
             self.set(src)
-        elif self.nbytes:
+        elif self.nbytes and zero:
             self.set(np.zeros(self.shape, self.dtype))
 
     def data_ptr(self):
@@ -273,18 +285,25 @@ class BoxSet:
 
 class Solver:
     def __init__(self, ctx, boxset, vtot, psi0, mg_levels=2, precond_dtype=F64, gram_prec=PREC_F64,
-                 step_length=1.5, ritz_every=0, precond_start=START_ZERO, profile=0):
+                 step_length=1.5, ritz_every=0, precond_start=START_ZERO, profile=0, anderson_m=0,
+                 anderson_beta=1.0, conv_criterion=CONV_RESIDUAL, precond_kind=PRECOND_VCYCLE, cnn=None):
         import numpy as np
         self.ctx = ctx
         self.boxset = boxset
         cfg = SolverCfg(LAP_4TH, mg_levels, precond_dtype, gram_prec, step_length, ritz_every, precond_start,
-                        profile)
+                        profile, anderson_m, anderson_beta, conv_criterion, precond_kind)
         v = np.ascontiguousarray(vtot, dtype=np.float64)
         p = np.ascontiguousarray(psi0, dtype=np.float64)
         self.npts = p.size
         self.h = C.c_void_p()
         ctx.check(lib().mgx_solver_create(ctx.h, boxset.h, C.byref(cfg), v.ctypes.data_as(C.c_void_p),
                                           p.ctypes.data_as(C.c_void_p), C.byref(self.h)), "mgx_solver_create")
+        if cnn is not None:
+            self.set_cnn(cnn)
+
+    def set_cnn(self, cnn):
+        self.cnn = cnn  # keep it alive while attached
+        self.ctx.check(lib().mgx_solver_set_cnn(self.h, cnn.h), "mgx_solver_set_cnn")
 
     def step(self):
         st = SolverStats()

@@ -72,11 +72,33 @@ class BoxConfig:
         return len(self.edge)
 
 
-def box_config(nbox, gdim, h, edges, nwells, seed, levels=3, sigma=1.0, noise=1e-3):
+def centers(n, L, rng, min_sep=None):
+    """n positions ~ U([0, L)^3); with ``min_sep`` (bohr), candidates closer
+    than min_sep (minimum image) to an accepted centre are redrawn (random
+    sequential addition), so that no two orbitals start nearly identical."""
+    L = np.asarray(L, dtype=np.float64)
+    if not min_sep:
+        return rng.uniform(0.0, 1.0, (n, 3)) * L
+    out = np.zeros((n, 3))
+    k = 0
+    tries = 0
+    while k < n:
+        c = rng.uniform(0.0, 1.0, 3) * L
+        tries += 1
+        if tries > 1000 * n:
+            raise RuntimeError("cannot place %d centres %.2f bohr apart" % (n, min_sep))
+        if k and np.min(np.sum(_mi(out[:k] - c, L) ** 2, axis=1)) < min_sep * min_sep:
+            continue
+        out[k] = c
+        k += 1
+    return out
+
+
+def box_config(nbox, gdim, h, edges, nwells, seed, levels=3, sigma=1.0, noise=1e-3, min_sep=None):
     gdim = (int(gdim),) * 3 if np.isscalar(gdim) else tuple(int(x) for x in gdim)
     rng = np.random.default_rng(seed)
     L = np.array(gdim) * h
-    cen = rng.uniform(0.0, 1.0, (nbox, 3)) * L
+    cen = centers(nbox, L, rng, min_sep)
     edge = rng.choice(np.asarray(edges, dtype=np.int32), nbox).astype(np.int32)
     m = 1 << (levels - 1)
     origin = np.zeros((nbox, 3), dtype=np.int32)

@@ -518,3 +518,76 @@ def b_solver_step(b, psi, vtot, mg_levels=2, prec=F64, step_length=1.5, lowest=N
     for j in range(b.nbox):
         new[off[j]:off[j + 1]] *= 1.0 / np.sqrt(d[j])
     return new, 0.5 * e_ry, rn, lowest
+
+
+def _anderson_theta(mat, rhs):
+    """AndersonMix::update coefficient for one history vector
+    (src/AndersonMix.cc): theta = rhs/mat, reset to -0.5 (theta > 1) or 0
+    (0.5 < theta <= 1) above max_theta = 0.5, clamped to min_theta = -3."""
+    th = rhs / mat if mat > 0.0 else 0.0
+    if th > 0.5:
+        th = -0.5 if th > 1.0 else 0.0
+    elif th < -3.0:
+        th = -3.0
+    return th
+
+
+class BSolver:
+    """The orbital iteration with DFTsolver's energy bookkeeping and ABPG's
+    optional Anderson extrapolation (m <= 1), stateful like the engine's
+    solver: src/DFTsolver.cc:138-167 (deig, deig2, accelerate),
+    src/ABPG.cc:73-140 and src/AndersonMix.cc (m = 1, dotProductDiagonal).
+    Fine-level V-cycle from zero (zero_start)."""
+
+    def __init__(self, b, psi, vtot, mg_levels=2, prec=F64, step_length=1.5, anderson_m=0, beta=1.0):
+        self.b, self.psi, self.vtot = b, np.array(psi, dtype=np.float64), vtot
+        self.mg_levels, self.prec, self.alpha = mg_levels, prec, 0.5 * step_length
+        self.m, self.beta = anderson_m, beta
+        self.iter, self.e_prev, self.deig, self.deig2 = 0, 0.0, 1e9, 1e9
+        self.accelerate, self.mm = False, 0
+        self.x0 = self.f0 = None
+
+    def _dot(self, a, c, Sinv):
+        off = self.b.offsets(0)
+        vel = float(np.prod(list(self.b.h)))
+        return sum(Sinv[j, j] * vel * float(a[off[j]:off[j + 1]] @ c[off[j]:off[j + 1]])
+                   for j in range(self.b.nbox))
+
+    def step(self):
+        """One step; returns (band_energy_Ha, residual_Ha, deig2_Ha)."""
+        b, psi = self.b, self.psi
+        hp = b_hloc(b, self.vtot, psi)
+        S, H = b_gram_h(b, psi, hp)
+        _, Sinv = chol_inv(S)
+        th = theta(Sinv, H)
+        e = 0.5 * eigsum(H, Sinv)
+        R = b_residual(b, psi, hp, th)
+        rn = residual_norm(b_diag_dots(b, R), Sinv)
+        if self.iter > 0:
+            old = self.deig
+            self.deig = abs(e - self.e_prev)
+            self.deig2 = max(self.deig, old)
+        self.e_prev = e
+        n = b.nbox
+        self.accelerate = self.accelerate or (self.m > 0 and 4.0 * self.deig2 < 1e-2 * n)
+        f = self.alpha * b_precond(b, 2, self.mg_levels, 0.0, R, dtype=self.prec)
+        if not self.accelerate:
+            psi = psi + f
+        else:
+            t, beta = 0.0, 1.0
+            if self.mm > 0:
+                df = f - self.f0
+                t = _anderson_theta(self._dot(df, df, Sinv), self._dot(df, f, Sinv))
+                beta = self.beta
+            xb = (1.0 - t) * psi + t * (self.x0 if self.mm > 0 else psi)
+            fb = (1.0 - t) * f + t * (self.f0 if self.mm > 0 else f)
+            self.x0, self.f0 = psi.copy(), f.copy()
+            psi = xb + beta * fb
+            self.mm = min(self.mm + 1, self.m)
+        d = b_diag_dots(b, psi)
+        off = b.offsets(0)
+        for j in range(n):
+            psi[off[j]:off[j + 1]] *= 1.0 / np.sqrt(d[j])
+        self.psi = psi
+        self.iter += 1
+        return e, rn, self.deig2
