@@ -112,7 +112,10 @@ def main():
     # roofline of the fine-level Jacobi sweep of the headline path
     ms, launches = head["profile"]["jacobi0"]
     s_bytes = 4 if "mixed" in res else 8
-    per_launch_bytes = 3.25 * s_bytes * npts  # read x, f; write x (+ w on 1 of 4 sweeps)
+    # the 3 fine sweeps launched per V-cycle (the zero-start first sweep is
+    # pointwise, fused with the cast): read x, f and write x, plus w on the
+    # last pre-sweep -> (4 + 3 + 3) / 3 values per point
+    per_launch_bytes = (10.0 / 3.0) * s_bytes * npts
     avg_s = (ms / 1e3) / max(launches, 1)
     achieved = per_launch_bytes / avg_s / 1e9 if launches else 0.0
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",

@@ -455,6 +455,29 @@ __global__ void k_prep(const double* res, T* f, T* x, int64_t n, double gamma)
     }
 }
 
+// The first Jacobi sweep from x = 0, done pointwise: w = A*0 - f = -f, so
+// x = 0 + (T)(-jf * w) = 0 + (T)(jf * f), the value GridFuncVector::jacobi
+// produces (the mask is 1 on the box).  With res non-null it also forms
+// f = (T)res (the fine level's FP64 -> T conversion).
+template <typename T>
+__global__ void k_first_sweep(const double* res, T* f, T* x, int64_t n, double jf)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        T fv;
+        if (res)
+        {
+            fv   = (T)res[t];
+            f[t] = fv;
+        }
+        else
+            fv = f[t];
+        T xv = (T)0;
+        xv += static_cast<T>(jf * static_cast<double>(fv));
+        x[t] = xv;
+    }
+}
+
 template <typename T>
 __global__ void k_to_f64(const T* x, double* out, int64_t n)
 {
@@ -1297,15 +1320,17 @@ int box_mg_setup(mgx_ctx* ctx, const mgx_boxset* bs, int max_levels, BoxMG<T>& B
 // result is back in v.
 template <typename T>
 int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, T* v, const T* f, int type,
-    int level)
+    int level, bool first_done)
 {
     const int ncycl = (level == max_levels) ? 4 : 2;
     double h[3]     = { bs->h[0] * (1 << level), bs->h[1] * (1 << level), bs->h[2] * (1 << level) };
     const double jf = jacobi_factor(type, h);
-    T* a = v;
-    T* b = B.tmp[level];
+    // first_done: the zero-start first sweep has been written to tmp; the
+    // remaining sweeps ping-pong so that the iterate still ends in v
+    T* a = first_done ? B.tmp[level] : v;
+    T* b = first_done ? v : B.tmp[level];
     int rc;
-    for (int it = 0; it < ncycl; it++)
+    for (int it = first_done ? 1 : 0; it < ncycl; it++)
     {
         T* w          = (it == ncycl - 1 && level < max_levels) ? B.w[level] : nullptr;
         const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0) : -1;
@@ -1318,9 +1343,12 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
     T* nv      = B.nv[level];
     if ((rc = box_restrict_t(ctx, bs, level, (const T*)B.w[level], rcoarse))) return rc;
     const int64_t nc = bs->off[level + 1][bs->nbox];
-    hipLaunchKernelGGL(k_fill_b<T>, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, nv, nc, (T)0);
+    // the coarse level starts from 0: its first sweep is pointwise
+    double hc[3] = { 2. * h[0], 2. * h[1], 2. * h[2] };
+    hipLaunchKernelGGL(k_first_sweep<T>, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, (const double*)nullptr,
+        rcoarse, B.tmp[level + 1], nc, jacobi_factor(MGX_LAP_2ND, hc));
     MGX_CHECK_LAUNCH(ctx);
-    if ((rc = box_mg_rec(ctx, bs, B, max_levels, nv, (const T*)rcoarse, MGX_LAP_2ND, level + 1))) return rc;
+    if ((rc = box_mg_rec(ctx, bs, B, max_levels, nv, (const T*)rcoarse, MGX_LAP_2ND, level + 1, true))) return rc;
     if ((rc = box_prolong_t<T, true>(ctx, bs, level, (const T*)nv, v))) return rc;
     a = v;
     b = B.tmp[level];
@@ -1347,12 +1375,18 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     if (!xf) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     T* x = xf;
     T* f = xf + n;
-    hipLaunchKernelGGL(k_prep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, x, n, gamma);
-    MGX_CHECK_LAUNCH(ctx);
     BoxMG<T> B;
     int rc = box_mg_setup(ctx, bs, max_levels, B, 3);
     if (rc) return rc;
-    if ((rc = box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0))) return rc;
+    // from x0 = 0 the first fine sweep is pointwise and fused with the cast
+    const bool zero = gamma == 0.;
+    if (zero)
+        hipLaunchKernelGGL(k_first_sweep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, B.tmp[0], n,
+            jacobi_factor(type, bs->h));
+    else
+        hipLaunchKernelGGL(k_prep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, x, n, gamma);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0, zero))) return rc;
     if (out)
         hipLaunchKernelGGL(k_to_f64<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, (const T*)x, out, n);
     else if (psi)
