@@ -1,17 +1,24 @@
 """Benchmark: time per converged orbital solve on one MI355X.
 
-Workload (default): 2048 localized orbitals in cubic 32^3 boxes on a 256^3
-periodic grid, h = 0.2 bohr, 64 Gaussian wells, seeded Gaussian orbitals;
-iterate until the residual norm (Ha) is below 1e-6.  One "step" is one such
-converged solve.  Two paths run on identical inputs:
-  plain : FP64 stencils, FP64 V-cycle, FP64 S/H, FP64 dense layer;
-  mixed : FP64 stencils/residual, FP32 V-cycle, hi/lo-split S/H, FP64 dense.
-value = seconds per converged solve of the mixed path (lower is better);
-the plain path's time, both iteration counts and the band-energy difference
+Workload (default c5): 2048 localized orbitals in cubic 32^3 boxes on a
+256^3 periodic grid, h = 0.2 bohr, 64 Gaussian wells, seeded Gaussian
+orbitals (centres at least 2 bohr apart); ABPG steps with Anderson
+extrapolation (the reference's default quench solver, m = 1, beta = 1) until
+the residual norm is below 1e-6 Ha.  One "step" is one such converged solve.
+Three paths run on identical inputs:
+  plain   : FP64 V-cycle, FP64 S/H, FP64 dense layer;
+  mixed   : FP32 V-cycle, hi/lo-split S/H on FP32 MFMA, FP64 dense layer;
+  learned : mixed, with the trained CNN in place of the V-cycle (BF16 MFMA),
+            the V-cycle judging each step and taking over when it fails.
+Hpsi, the residual and the energy are FP64 in all three.
+value = seconds per converged solve of the mixed+learned path (lower is
+better); plain and mixed times, iteration counts and band-energy differences
 are reported beside it.
 
-Roofline: the fine-level Jacobi sweep (the launch that moves the most bytes),
-timed with hipEvents on the engine's stream during the timed solves.
+Roofline: the dominant kernel of the headline path, timed with hipEvents on
+the engine's stream inside the timed solves: the CNN inference (BF16 MFMA)
+for the learned path, the fine-level Jacobi sweep (HBM) otherwise; the other
+is reported as roofline_other.
 CPU baseline: the oracle (C++/OpenMP restatement of the reference) timed on
 this host for one full-size orbital step, times the iteration count.
 """
@@ -29,6 +36,20 @@ sys.path.insert(0, ROOT)
 from mgmol_amd import mgx, synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFS = 2500.0
+CNN_FLOP_PER_VOXEL = 2 * 27 * (1 * 16 + 16 * 16 + 16 * 16 + 16 * 1)
+WEIGHTS = os.path.join(ROOT, "mgmol_amd", "weights", "cnn_e32.npz")
+
+PATHS = {
+    "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64),
+    "mixed": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT),
+    "learned": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT, precond_kind=mgx.PRECOND_LEARNED),
+}
+LABELS = {
+    "plain": "plain FP64 (FP64 V-cycle, FP64 S/H)",
+    "mixed": "mixed (FP32 V-cycle, hi/lo-split S/H)",
+    "learned": "mixed+learned (BF16 CNN preconditioner, V-cycle fallback)",
+}
 
 
 def log(*a):
@@ -42,13 +63,11 @@ def make_config(name, seed):
     return synth.box_config(seed=seed, **c), c
 
 
-def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup):
-    kw = dict(mg_levels=cfg.levels - 1, profile=1, precond_start=mgx.START_ZERO)
-    if mode == "plain":
-        kw.update(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64)
-    else:
-        kw.update(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT)
-    s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, **kw)
+def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
+    kw = dict(mg_levels=cfg.levels - 1, profile=1, precond_start=mgx.START_ZERO, anderson_m=1,
+              conv_criterion=mgx.CONV_RESIDUAL)
+    kw.update(PATHS[mode])
+    s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, cnn=cnn if mode == "learned" else None, **kw)
     for _ in range(warmup):
         s.run(tol, 3)
         s.reset()
@@ -59,12 +78,14 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup):
         t0 = time.perf_counter()
         stats = s.run(tol, maxit)
         times.append(time.perf_counter() - t0)
-        log("[%s] solve %d: %d iterations, |R|=%.3e Ha, E=%.12f Ha, %.3f s (converged=%d)"
-            % (mode, k, stats.iterations, stats.residual, stats.band_energy, times[-1], stats.converged))
+        log("[%s] solve %d: %d iterations, |R|=%.3e Ha, E=%.12f Ha, %.3f s (converged=%d, learned steps %d, "
+            "fallbacks %d)" % (mode, k, stats.iterations, stats.residual, stats.band_energy, times[-1],
+                               stats.converged, stats.learned_steps, stats.fallbacks))
     prof = s.profile()
     s.close()
     return dict(seconds=float(np.mean(times)), iterations=stats.iterations, residual=stats.residual,
-                energy=stats.band_energy, converged=bool(stats.converged), profile=prof)
+                energy=stats.band_energy, converged=bool(stats.converged), learned_steps=stats.learned_steps,
+                fallbacks=stats.fallbacks, profile=prof)
 
 
 def cpu_baseline(cfg, iters, mg_levels):
@@ -76,8 +97,37 @@ def cpu_baseline(cfg, iters, mg_levels):
     O.b_solver_step(ob, cfg.psi.copy(), cfg.vtot, mg_levels, O.F32, 1.5, None, zero_start=True)
     t_step = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    return dict(value=t_step * iters, unit="s per converged solve", cores=cores, kind="port",
-                sample="1 oracle orbital step on the full workload (%.1f s) x %d iterations" % (t_step, iters))
+    return dict(value=round(t_step * iters, 2), unit="s per converged solve", cores=cores, kind="port",
+                sample="1 oracle orbital step on the full workload (%.2f s) x %d iterations" % (t_step, iters))
+
+
+def jacobi_roofline(r, s_bytes, npts):
+    ms, launches = r["profile"]["jacobi0"]
+    # the 3 fine sweeps launched per V-cycle (the zero-start first sweep is
+    # pointwise, fused with the cast): read x, f and write x, plus w on the
+    # last pre-sweep -> (4 + 3 + 3) / 3 values per point
+    per_launch = (10.0 / 3.0) * s_bytes * npts
+    if not launches:
+        return None
+    avg_s = (ms / 1e3) / launches
+    achieved = per_launch / avg_s / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_box_march<JACOBI> fine-level sweep (%s)" % ("FP32" if s_bytes == 4 else "FP64"),
+                avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches), bytes_per_launch=int(per_launch))
+
+
+def cnn_roofline(r, npts):
+    ms, launches = r["profile"]["learned"]
+    if not launches:
+        return None
+    per_launch = CNN_FLOP_PER_VOXEL * npts
+    avg_s = (ms / 1e3) / launches
+    achieved = per_launch / avg_s / 1e12
+    return dict(bound="mfma", achieved=round(achieved, 2), peak=BF16_PEAK_TFS, unit="TFLOP/s",
+                frac=round(achieved / BF16_PEAK_TFS, 5), traffic=None,
+                kernel="CNN inference (4 conv3d layers, BF16 MFMA) over all boxes",
+                avg_launch_ms=round(1e3 * avg_s, 3), launches=int(launches), flop_per_launch=int(per_launch))
 
 
 def main():
@@ -88,13 +138,15 @@ def main():
     ap.add_argument("--config", default="c5")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--maxit", type=int, default=3000)
-    ap.add_argument("--modes", default="plain,mixed")
+    ap.add_argument("--maxit", type=int, default=2000)
+    ap.add_argument("--modes", default="plain,mixed,learned")
+    ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    modes = args.modes.split(",")
 
     t0 = time.perf_counter()
     cfg, meta = make_config(args.config, args.seed)
@@ -103,26 +155,25 @@ def main():
     ctx = mgx.Ctx(0)
     bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, cfg.levels)
     log("box pairs:", bs.npairs())
+    cnn, wmeta = None, None
+    if "learned" in modes:
+        from mgmol_amd import learned, train_cnn
+        params, wmeta = train_cnn.load(args.weights)
+        cnn = learned.Cnn(ctx, wmeta["edge"], 256, params)
     res = {}
-    for mode in args.modes.split(","):
-        res[mode] = run_mode(ctx, bs, cfg, mode, args.tol, args.maxit, args.steps, args.warmup)
-    head = res.get("mixed") or res[list(res)[0]]
+    for mode in modes:
+        res[mode] = run_mode(ctx, bs, cfg, mode, args.tol, args.maxit, args.steps, args.warmup, cnn)
+    headname = "learned" if "learned" in res else ("mixed" if "mixed" in res else modes[0])
+    head = res[headname]
     npts = cfg.psi.size
 
-    # roofline of the fine-level Jacobi sweep of the headline path
-    ms, launches = head["profile"]["jacobi0"]
-    s_bytes = 4 if "mixed" in res else 8
-    # the 3 fine sweeps launched per V-cycle (the zero-start first sweep is
-    # pointwise, fused with the cast): read x, f and write x, plus w on the
-    # last pre-sweep -> (4 + 3 + 3) / 3 values per point
-    per_launch_bytes = (10.0 / 3.0) * s_bytes * npts
-    avg_s = (ms / 1e3) / max(launches, 1)
-    achieved = per_launch_bytes / avg_s / 1e9 if launches else 0.0
-    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_box_jacobi (fine level, %s)" % ("FP32" if s_bytes == 4 else "FP64"),
-                avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches),
-                bytes_per_launch=int(per_launch_bytes))
+    vc_name = "mixed" if "mixed" in res else headname
+    roof_vc = jacobi_roofline(res[vc_name], 4 if PATHS[vc_name]["precond_dtype"] == mgx.F32 else 8, npts)
+    roof_cnn = cnn_roofline(res["learned"], npts) if "learned" in res else None
+    if headname == "learned" and roof_cnn:
+        roof, other = roof_cnn, roof_vc
+    else:
+        roof, other = roof_vc, roof_cnn
 
     cpu = None
     if not args.no_cpu:
@@ -142,20 +193,33 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if "mixed" in res else "f64",
+        "dtype": "bf16+f32+f64" if headname == "learned" else ("f32+f64" if headname == "mixed" else "f64"),
         "data": "synthetic (seeded Gaussian orbitals and wells)",
-        "config": {"workload": "%s: %d orbitals in %s boxes, %d^3 periodic grid, h=%.2f bohr, %d wells, |R|<%g Ha"
-                   % (args.config, cfg.nbox, "/".join(str(e) for e in meta["edges"]), cfg.gdim[0], cfg.h,
-                      meta["nwells"], args.tol),
-                   "path": "mixed (FP32 V-cycle, split S/H)" if "mixed" in res else "plain FP64"},
+        "config": {"workload": "%s: %d orbitals in %s^3 boxes, %d^3 periodic grid, h=%.2f bohr, %d wells, "
+                               "centres >= %s bohr apart, ABPG (Anderson m=1) to |R| < %g Ha"
+                               % (args.config, cfg.nbox, "/".join(str(e) for e in meta["edges"]), cfg.gdim[0],
+                                  cfg.h, meta["nwells"], meta.get("min_sep"), args.tol),
+                   "path": LABELS[headname]},
+        "iterations": head["iterations"],
+        "converged": head["converged"],
         "paths": {m: {k: v for k, v in r.items() if k != "profile"} for m, r in res.items()},
         "kernel_ms": {m: {g: round(t[0], 2) for g, t in r["profile"].items()} for m, r in res.items()},
         "roofline": roof,
+        "roofline_other": other,
         "cpu_baseline": cpu,
     }
-    if "plain" in res and "mixed" in res:
-        out["speedup_vs_plain"] = round(res["plain"]["seconds"] / res["mixed"]["seconds"], 3)
-        out["energy_rel_diff"] = abs(res["mixed"]["energy"] - res["plain"]["energy"]) / abs(res["plain"]["energy"])
+    if "plain" in res:
+        for m in res:
+            if m == "plain":
+                continue
+            out["paths"][m]["speedup_vs_plain"] = round(res["plain"]["seconds"] / res[m]["seconds"], 3)
+            out["paths"][m]["energy_rel_diff_vs_plain"] = (abs(res[m]["energy"] - res["plain"]["energy"])
+                                                           / abs(res["plain"]["energy"]))
+    if wmeta:
+        out["learned_training"] = {k: wmeta[k] for k in ("pairs", "epochs", "heldout_rel_error", "seconds_train",
+                                                         "seconds_dataset") if k in wmeta}
+    if cnn:
+        cnn.close()
     bs.close()
     ctx.close()
     print(json.dumps(out))

@@ -291,7 +291,8 @@ typedef struct
 #define MGX_PROF_JACOBI0 4   /* fine-level Jacobi sweeps (one launch each) */
 #define MGX_PROF_PRECOND 5   /* whole V-cycle incl. update */
 #define MGX_PROF_NORMALIZE 6 /* orbital normalisation */
-#define MGX_PROF_N 7
+#define MGX_PROF_LEARNED 7   /* CNN inference of the learned preconditioner */
+#define MGX_PROF_N 8
 
 typedef struct
 {

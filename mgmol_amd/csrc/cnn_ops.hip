@@ -105,16 +105,22 @@ __global__ void __launch_bounds__(64 * kWaves) k_conv(const void* __restrict__ i
     f4 acc = { 0.f, 0.f, 0.f, 0.f };
     if (CIN == 16)
     {
+        // all 27 input loads in flight before the MFMA chain
         const uint16_t* X = (const uint16_t*)in;
-#pragma unroll 3
+        s4 b[kTaps];
+#pragma unroll
+        for (int tap = 0; tap < kTaps; tap++)
+        {
+            const int xi = T.x + tap_dx(tap), yi = T.y + tap_dy(tap), zi = T.z0 + n + tap_dz(tap);
+            b[tap] = (s4){ 0, 0, 0, 0 };
+            if (xi >= 0 && xi < e && yi >= 0 && yi < e && zi >= 0 && zi < e)
+                b[tap] = *(const s4*)(X + (sbase + ((int64_t)xi * e + yi) * e + zi) * 16 + 4 * g);
+        }
+#pragma unroll
         for (int tap = 0; tap < kTaps; tap++)
         {
             const s4 a = *(const s4*)(wt + ((size_t)tap * 16 + n) * 16 + 4 * g);
-            const int xi = T.x + tap_dx(tap), yi = T.y + tap_dy(tap), zi = T.z0 + n + tap_dz(tap);
-            s4 b = { 0, 0, 0, 0 };
-            if (xi >= 0 && xi < e && yi >= 0 && yi < e && zi >= 0 && zi < e)
-                b = *(const s4*)(X + (sbase + ((int64_t)xi * e + yi) * e + zi) * 16 + 4 * g);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            acc        = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[tap], acc, 0, 0, 0);
         }
     }
     else

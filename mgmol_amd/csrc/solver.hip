@@ -226,7 +226,9 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
                                ctx->stream));
         int edge = 0;
         boxset_uniform_edge(s->bs, &edge);
+        const int tl = prof_begin(ctx, MGX_PROF_LEARNED);
         if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->res))) return rc;
+        prof_end(ctx, tl);
         s->cnn_pending = true;
         s->r_prev_norm = rnorm;
         s->learned_steps++;

@@ -124,10 +124,12 @@ def train(cnn, x, t, nsamples, epochs, batch, lr, seed=0, log=None):
         # shuffled once when the set is built)
         order = rng.permutation(nbat)
         tot = 0.0
-        for b in order:
+        for i, b in enumerate(order):
             s0 = int(b) * batch
             nb = min(batch, nsamples - s0)
             tot += cnn.train_step(x.data_ptr() + s0 * bytes_per, t.data_ptr() + s0 * bytes_per, nb, lr)
+            if log and (i + 1) % 500 == 0:
+                log("epoch %d batch %d/%d running loss %.6e" % (ep, i + 1, nbat, tot / (i + 1)))
         hist.append(tot / nbat)
         if log:
             log("epoch %d loss %.6e" % (ep, hist[-1]))

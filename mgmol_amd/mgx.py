@@ -16,7 +16,7 @@ PREC_F64, PREC_SPLIT = 0, 1
 START_GAMMA, START_ZERO = 0, 1
 CONV_DELTAE, CONV_RESIDUAL = 0, 1
 PRECOND_VCYCLE, PRECOND_LEARNED = 0, 1
-PROF_GROUPS = ["hloc", "gram", "dense", "resid", "jacobi0", "precond", "normalize"]
+PROF_GROUPS = ["hloc", "gram", "dense", "resid", "jacobi0", "precond", "normalize", "learned"]
 
 _LIB = None
 
@@ -319,8 +319,8 @@ class Solver:
         self.ctx.check(lib().mgx_solver_reset(self.h), "mgx_solver_reset")
 
     def profile(self, reset=False):
-        ms = (C.c_double * 7)()
-        cnt = (C.c_int64 * 7)()
+        ms = (C.c_double * len(PROF_GROUPS))()
+        cnt = (C.c_int64 * len(PROF_GROUPS))()
         self.ctx.check(lib().mgx_solver_profile(self.h, ms, cnt, 1 if reset else 0), "mgx_solver_profile")
         return {g: (ms[i], cnt[i]) for i, g in enumerate(PROF_GROUPS)}
 
