@@ -1045,9 +1045,12 @@ __global__ void __launch_bounds__(256) k_box_sumsq(const double* r, const int64_
 
 // Anderson mixing inner products (AndersonMix::update, src/AndersonMix.cc,
 // m = 1, with LocGridOrbitals::dotProductDiagonal): per box j,
-//   dd_j = vel <af - f0, af - f0>,  df_j = vel <af - f0, af>   with f = a*kr
-__global__ void __launch_bounds__(256) k_box_and_dots(const double* kr, double a, const double* f0,
-    const int64_t* off, int nbox, double vel, double* d)
+//   dd_j = vel <f - f0, f - f0>,  df_j = vel <f - f0, f>
+// with f = a*kr and f0 = a*kr0 formed in FP64 from the preconditioner's
+// outputs of this and the previous step (type T, as the V-cycle returns them)
+template <typename T>
+__global__ void __launch_bounds__(256) k_box_and_dots(const T* kr, double a, const T* kr0, const int64_t* off,
+    int nbox, double vel, double* d)
 {
     __shared__ double lds[8];
     const int j = blockIdx.x;
@@ -1055,7 +1058,7 @@ __global__ void __launch_bounds__(256) k_box_and_dots(const double* kr, double a
     double acc[2] = { 0., 0. };
     for (int64_t q = off[j] + threadIdx.x; q < off[j + 1]; q += blockDim.x)
     {
-        const double f = a * kr[q], df = f - f0[q];
+        const double f = a * (double)kr[q], df = f - a * (double)kr0[q];
         acc[0] += df * df;
         acc[1] += df * f;
     }
@@ -1086,20 +1089,30 @@ __global__ void __launch_bounds__(256) k_diag_wsum2(const double* d, const doubl
     }
 }
 
-// x_bar = (1-th) x + th x0, f_bar = (1-th) f + th f0 (f = a*kr); then
-// x0 <- x, f0 <- f, x <- x_bar + beta f_bar
-__global__ void k_and_update(double* x, double* x0, const double* kr, double a, double* f0, int64_t n, double th,
+// x_bar = (1-th) x + th x0, f_bar = (1-th) f + th f0 (f = a*kr, f0 = a*kr0);
+// then x0 <- x, kr0 <- kr, x <- x_bar + beta f_bar
+template <typename T>
+__global__ void k_and_update(double* x, double* x0, const T* kr, double a, T* kr0, int64_t n, double th,
     double beta)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
     {
-        const double xv = x[t], x0v = x0[t], fv = a * kr[t], f0v = f0[t];
-        const double xb = (1. - th) * xv + th * x0v;
-        const double fb = (1. - th) * fv + th * f0v;
-        x0[t] = xv;
-        f0[t] = fv;
-        x[t]  = xb + beta * fb;
+        const T k         = kr[t];
+        const double xv   = x[t], x0v = x0[t], fv = a * (double)k, f0v = a * (double)kr0[t];
+        const double xb   = (1. - th) * xv + th * x0v;
+        const double fb   = (1. - th) * fv + th * f0v;
+        x0[t]             = xv;
+        kr0[t]            = k;
+        x[t]              = xb + beta * fb;
     }
+}
+
+// psi += a * kr (ABPG::update_states without extrapolation)
+template <typename T>
+__global__ void k_psi_step(double* psi, const T* kr, int64_t n, double a)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        psi[t] += a * (double)kr[t];
 }
 
 // 0.5*sqrt(sum_j d_j Sinv_jj): single block
@@ -1395,6 +1408,18 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     return MGX_OK;
 }
 
+// the V-cycle of box_precond_t, result left in the context workspace (type
+// prec); *x points at it until the next preconditioner call
+int box_precond_x(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
+    const double* res, const void** x)
+{
+    int rc = prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0.)
+                             : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0.);
+    if (rc) return rc;
+    *x = workspace(ctx, 2, 0);
+    return MGX_OK;
+}
+
 int box_precond(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
     const double* res, double* out, double* psi, double alpha)
 {
@@ -1501,23 +1526,46 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
 }
 
 // Anderson m = 1 inner products, weighted by diag(S^-1): out[0] =
-// <f - f0, f - f0>, out[1] = <f - f0, f> (device), f = a*kr.  d: 2 n scratch.
-int box_anderson_dots(mgx_ctx* ctx, const mgx_boxset* bs, const double* kr, double a, const double* f0,
+// <f - f0, f - f0>, out[1] = <f - f0, f> (device), f = a*kr, f0 = a*kr0;
+// kr, kr0 of type prec (MGX_F32 / MGX_F64).  d: 2 n scratch.
+int box_anderson_dots(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const void* kr, double a, const void* kr0,
     const double* Sinv, double* d, double* out)
 {
     const double vel = bs->h[0] * bs->h[1] * bs->h[2];
-    hipLaunchKernelGGL(k_box_and_dots, dim3(bs->nbox), dim3(256), 0, ctx->stream, kr, a, f0,
-        (const int64_t*)bs->d_off[0], bs->nbox, vel, d);
+    if (prec == MGX_F32)
+        hipLaunchKernelGGL(k_box_and_dots<float>, dim3(bs->nbox), dim3(256), 0, ctx->stream, (const float*)kr, a,
+            (const float*)kr0, (const int64_t*)bs->d_off[0], bs->nbox, vel, d);
+    else
+        hipLaunchKernelGGL(k_box_and_dots<double>, dim3(bs->nbox), dim3(256), 0, ctx->stream, (const double*)kr, a,
+            (const double*)kr0, (const int64_t*)bs->d_off[0], bs->nbox, vel, d);
     hipLaunchKernelGGL(k_diag_wsum2, dim3(1), dim3(256), 0, ctx->stream, (const double*)d, Sinv, bs->nbox, out);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
 
-int box_anderson_update(mgx_ctx* ctx, const mgx_boxset* bs, double* x, double* x0, const double* kr, double a,
-    double* f0, double th, double beta)
+int box_anderson_update(mgx_ctx* ctx, const mgx_boxset* bs, int prec, double* x, double* x0, const void* kr,
+    double a, void* kr0, double th, double beta)
 {
     const int64_t n = bs->off[0][bs->nbox];
-    hipLaunchKernelGGL(k_and_update, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, x, x0, kr, a, f0, n, th, beta);
+    if (prec == MGX_F32)
+        hipLaunchKernelGGL(k_and_update<float>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, x, x0,
+            (const float*)kr, a, (float*)kr0, n, th, beta);
+    else
+        hipLaunchKernelGGL(k_and_update<double>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, x, x0,
+            (const double*)kr, a, (double*)kr0, n, th, beta);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_psi_step(mgx_ctx* ctx, const mgx_boxset* bs, int prec, double* psi, const void* kr, double a)
+{
+    const int64_t n = bs->off[0][bs->nbox];
+    if (prec == MGX_F32)
+        hipLaunchKernelGGL(k_psi_step<float>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, psi,
+            (const float*)kr, n, a);
+    else
+        hipLaunchKernelGGL(k_psi_step<double>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, psi,
+            (const double*)kr, n, a);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }

@@ -419,10 +419,11 @@ __global__ void k_box_scale_to_f32(const double* R, const double* rms, int64_t m
         x[i] = (float)(R[i] / rms[i / m]);
 }
 
-__global__ void k_box_scale_from_f32(const float* y, const double* rms, int64_t m, int64_t n, double* out)
+template <typename T>
+__global__ void k_box_scale_from_f32(const float* y, const double* rms, int64_t m, int64_t n, T* out)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (double)y[i] * rms[i / m];
+        out[i] = (T)((double)y[i] * rms[i / m]);
 }
 
 }  // namespace
@@ -551,7 +552,7 @@ bool batch_ok(const mgx_cnn* c, int nb) { return nb >= 1 && nb <= c->max_batch; 
 
 }  // namespace
 
-int cnn_apply_boxes(mgx_cnn* c, int nbox, int edge, const double* R, double* out);
+int cnn_apply_boxes(mgx_cnn* c, int nbox, int edge, const double* R, void* out, int out_dtype);
 
 }  // namespace mgx
 
@@ -690,7 +691,7 @@ int mgx_cnn_train_step(mgx_cnn* c, const float* x, const float* t, int nb, float
 int mgx_cnn_apply(mgx_cnn* c, int nbox, const double* R, double* out)
 {
     if (!c || !R || !out || nbox < 1) return MGX_EINVAL;
-    int rc = cnn_apply_boxes(c, nbox, c->edge, R, out);
+    int rc = cnn_apply_boxes(c, nbox, c->edge, R, out, MGX_F64);
     return rc ? rc : sync(c->ctx);
 }
 
@@ -718,7 +719,8 @@ int mgx_cnn_make_pairs(mgx_cnn* c, int nbox, const double* R, const double* KR, 
 
 namespace mgx
 {
-int cnn_apply_boxes(mgx_cnn* c, int nbox, int edge, const double* R, double* out)
+// out (type out_dtype) = rms_j * net(R_j / rms_j) per box
+int cnn_apply_boxes(mgx_cnn* c, int nbox, int edge, const double* R, void* out, int out_dtype)
 {
     mgx_ctx* ctx = c->ctx;
     MGX_REQUIRE(ctx, edge == c->edge, "box edge differs from the CNN's");
@@ -735,8 +737,12 @@ int cnn_apply_boxes(mgx_cnn* c, int nbox, int edge, const double* R, double* out
         if ((rc = forward(c, c->xf, nb))) return rc;
         hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, ctx->stream, (const float*)c->xf,
             (const float*)c->z, (const float*)nullptr, (const float*)c->p, m * nb, c->y, (float*)nullptr, c->lpart);
-        hipLaunchKernelGGL(k_box_scale_from_f32, dim3(4096), dim3(256), 0, ctx->stream, (const float*)c->y,
-            (const double*)c->rms, m, m * nb, out + (int64_t)b0 * m);
+        if (out_dtype == MGX_F32)
+            hipLaunchKernelGGL(k_box_scale_from_f32<float>, dim3(4096), dim3(256), 0, ctx->stream,
+                (const float*)c->y, (const double*)c->rms, m, m * nb, (float*)out + (int64_t)b0 * m);
+        else
+            hipLaunchKernelGGL(k_box_scale_from_f32<double>, dim3(4096), dim3(256), 0, ctx->stream,
+                (const float*)c->y, (const double*)c->rms, m, m * nb, (double*)out + (int64_t)b0 * m);
         MGX_CHECK_LAUNCH(ctx);
     }
     return MGX_OK;

@@ -40,10 +40,13 @@ int trace_prod(mgx_ctx*, int, const double*, const double*, double*);
 int diag_sum(mgx_ctx*, int, const double*, double*);
 int inverse_update(mgx_ctx*, int, const double*, double*, double*, double*, int*);
 int ritz(mgx_ctx*, int, const double*, const double*, double*, double*);
-int box_anderson_dots(mgx_ctx*, const mgx_boxset*, const double*, double, const double*, const double*, double*,
+int box_anderson_dots(mgx_ctx*, const mgx_boxset*, int, const void*, double, const void*, const double*, double*,
     double*);
-int box_anderson_update(mgx_ctx*, const mgx_boxset*, double*, double*, const double*, double, double*, double, double);
-int cnn_apply_boxes(mgx_cnn*, int, int, const double*, double*);
+int box_anderson_update(mgx_ctx*, const mgx_boxset*, int, double*, double*, const void*, double, void*, double,
+    double);
+int box_psi_step(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double);
+int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, const void**);
+int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
 }  // namespace mgx
 
@@ -63,7 +66,9 @@ struct mgx_solver
     double *S = nullptr, *H = nullptr, *L = nullptr, *Sinv = nullptr, *th = nullptr;
     double *eig = nullptr, *d = nullptr, *scal = nullptr, *nswork = nullptr;
     double *Sv = nullptr, *Hv = nullptr;  // S, H per interacting pair
-    double *x0 = nullptr, *f0 = nullptr;  // Anderson history: previous psi and step
+    double* x0 = nullptr;  // Anderson history: previous psi
+    void* kr0  = nullptr;  // and previous correction, in the preconditioner's type
+    void* kcnn = nullptr;  // the network's correction (learned steps)
     // DFTsolver::checkConvergenceEnergy state (band energy, Ha)
     double e_prev = 0., deig = 1e9, deig2 = 1e9;
     bool accelerate = false;
@@ -86,19 +91,14 @@ namespace mgx
 namespace
 {
 
-// psi += alpha x (ABPG::update_states without extrapolation)
-__global__ void k_step(double* psi, const double* x, int64_t n, double alpha)
-{
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-        psi[t] += alpha * x[t];
-}
-
 void free_solver(mgx_solver* s)
 {
     double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
-        s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->f0, s->psi_prev, s->r_prev };
+        s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->psi_prev, s->r_prev };
     for (double* b : bufs)
         if (b) hipFree(b);
+    if (s->kr0) (void)hipFree(s->kr0);
+    if (s->kcnn) (void)hipFree(s->kcnn);
     if (s->host) hipHostFree(s->host);
     for (hipEvent_t e : s->prof.pool)
         hipEventDestroy(e);
@@ -217,82 +217,66 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     st->fallbacks      = s->fallbacks;
     const double alpha = 0.5 * s->cfg.step_length;
     tk = prof_begin(ctx, MGX_PROF_PRECOND);
-    if (use_cnn)
+    const int prec = s->cfg.precond_dtype;
+    if (!use_cnn && !s->accelerate)
     {
-        // keep the pre-step state, then R -> CNN(R) in place
-        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi_prev, s->psi, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
-                               ctx->stream));
-        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->r_prev, s->res, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
-                               ctx->stream));
-        int edge = 0;
-        boxset_uniform_edge(s->bs, &edge);
-        const int tl = prof_begin(ctx, MGX_PROF_LEARNED);
-        if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->res))) return rc;
-        prof_end(ctx, tl);
-        s->cnn_pending = true;
-        s->r_prev_norm = rnorm;
-        s->learned_steps++;
-        if (s->accelerate)
+        // preconditioned step psi += alpha K(R) (ABPG::update_states, no
+        // extrapolation), fused with the V-cycle's last kernel
+        if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, nullptr, s->psi,
+                 alpha)))
+            return rc;
+    }
+    else
+    {
+        // the correction K(R) (or the network's), in the preconditioner's type
+        const void* kr = nullptr;
+        if (use_cnn)
         {
+            // keep the pre-step state for the residual test of the next step
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi_prev, s->psi, s->npts * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->r_prev, s->res, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+            int edge = 0;
+            boxset_uniform_edge(s->bs, &edge);
+            const int tl = prof_begin(ctx, MGX_PROF_LEARNED);
+            if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->kcnn, prec))) return rc;
+            prof_end(ctx, tl);
+            kr             = s->kcnn;
+            s->cnn_pending = true;
+            s->r_prev_norm = rnorm;
+            s->learned_steps++;
+        }
+        else if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, &kr)))
+            return rc;
+        if (!s->accelerate)
+        {
+            if ((rc = box_psi_step(ctx, s->bs, prec, s->psi, kr, alpha))) return rc;
+        }
+        else
+        {
+            // AndersonMix::update with m = 1 on f = alpha K(R)
             double th = 0., beta = 1.;
             if (s->mm > 0)
             {
-                if ((rc = box_anderson_dots(ctx, s->bs, s->res, alpha, s->f0, s->Sinv, s->d, s->scal + 2)))
+                if ((rc = box_anderson_dots(ctx, s->bs, prec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2)))
                     return rc;
-                MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host + 3, s->scal + 2, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                                       ctx->stream));
+                MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host + 3, s->scal + 2, 2 * sizeof(double),
+                                       hipMemcpyDeviceToHost, ctx->stream));
                 MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
                 const double mat = s->host[3], rhs = s->host[4];
                 th = mat > 0. ? rhs / mat : 0.;
+                // theta bounds of AndersonMix (max 0.5, min -3; with one history
+                // vector an out-of-range theta is reset, not the history shortened)
                 if (th > 0.5)
                     th = th > 1. ? -0.5 : 0.;
                 else if (th < -3.)
                     th = -3.;
                 beta = s->cfg.anderson_beta;
             }
-            if ((rc = box_anderson_update(ctx, s->bs, s->psi, s->x0, s->res, alpha, s->f0, th, beta))) return rc;
+            if ((rc = box_anderson_update(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0, th, beta))) return rc;
             s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
         }
-        else
-        {
-            hipLaunchKernelGGL(k_step, dim3(nblocks(s->npts)), dim3(kBlock), 0, ctx->stream, s->psi,
-                (const double*)s->res, s->npts, alpha);
-            MGX_CHECK_LAUNCH(ctx);
-        }
-    }
-    else if (!s->accelerate)
-    {
-        // preconditioned step psi += alpha K(R) (ABPG::update_states, no extrapolation)
-        if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, s->cfg.precond_dtype, s->res,
-                 nullptr, s->psi, alpha)))
-            return rc;
-    }
-    else
-    {
-        // K(R) into res (the V-cycle has consumed it), then AndersonMix::update
-        // with m = 1 on f = alpha K(R)
-        if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, s->cfg.precond_dtype, s->res,
-                 s->res, nullptr, 0.)))
-            return rc;
-        double th = 0., beta = 1.;
-        if (s->mm > 0)
-        {
-            if ((rc = box_anderson_dots(ctx, s->bs, s->res, alpha, s->f0, s->Sinv, s->d, s->scal + 2))) return rc;
-            MGX_CHECK_HIP(ctx,
-                hipMemcpyAsync(s->host + 3, s->scal + 2, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            const double mat = s->host[3], rhs = s->host[4];
-            th = mat > 0. ? rhs / mat : 0.;
-            // theta bounds of AndersonMix (max 0.5, min -3; with one history
-            // vector an out-of-range theta is reset, not the history shortened)
-            if (th > 0.5)
-                th = th > 1. ? -0.5 : 0.;
-            else if (th < -3.)
-                th = -3.;
-            beta = s->cfg.anderson_beta;
-        }
-        if ((rc = box_anderson_update(ctx, s->bs, s->psi, s->x0, s->res, alpha, s->f0, th, beta))) return rc;
-        s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
     }
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
@@ -352,14 +336,17 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     const int64_t gsize = (int64_t)g[0] * g[1] * g[2];
     const size_t nn     = (size_t)s->n * s->n;
     auto alloc = [&](double** p, size_t cnt) { return hipMalloc((void**)p, cnt * sizeof(double)) == hipSuccess; };
+    const size_t tsize = cfg->precond_dtype == MGX_F32 ? sizeof(float) : sizeof(double);
     bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
               && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
               && alloc(&s->eig, s->n) && alloc(&s->d, 2 * (size_t)s->n + 8) && alloc(&s->scal, 8) && alloc(&s->nswork, 2 * nn)
               && alloc(&s->Sv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
-              && (cfg->anderson_m == 0 || (alloc(&s->x0, s->npts) && alloc(&s->f0, s->npts)))
+              && (cfg->anderson_m == 0
+                  || (alloc(&s->x0, s->npts) && hipMalloc(&s->kr0, s->npts * tsize) == hipSuccess))
               && (cfg->precond_kind != MGX_PRECOND_LEARNED
-                  || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts)))
+                  || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts)
+                      && hipMalloc(&s->kcnn, s->npts * tsize) == hipSuccess))
               && hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {
