@@ -168,6 +168,126 @@ __global__ void __launch_bounds__(64 * kWaves) k_conv(const void* __restrict__ i
     *(s4*)((uint16_t*)out + vox * 16 + 4 * g) = o;
 }
 
+// A 16 -> 16 layer with the input staged in LDS: a block (4 waves) owns
+// rows y0..y0+3 and z0..z0+31 of one sample and marches along x, keeping the
+// planes x-1, x, x+1 of its (6 x 34)-voxel window in a ring of 3 LDS slots
+// (one new plane loaded per step, 16-byte coalesced loads, zeros outside the
+// box).  Wave w computes row y0+w, two 16-voxel z-tiles, 27 MFMAs each, with
+// all 27 taps' weights in registers and the B operands read from LDS.
+constexpr int kLY = 4, kLZ = 32, kLW = kLZ + 2, kLH = kLY + 2;
+constexpr int kPlane = kLH * kLW * 16;  // BF16 values per staged plane
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv16_lds(const uint16_t* __restrict__ in, const uint16_t* __restrict__ wt,
+    const float* __restrict__ bias, uint16_t* __restrict__ out, const uint16_t* __restrict__ hmask, int e, int nyb,
+    int nzb)
+{
+    __shared__ __align__(16) uint16_t lds[3 * kPlane];
+    const int b = blockIdx.x;
+    const int s = b / (nyb * nzb), yb = (b / nzb) % nyb, zb = b % nzb;
+    const int y0 = yb * kLY, z0 = zb * kLZ;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, n = lane & 15, g = lane >> 4;
+    const int64_t e3 = (int64_t)e * e * e;
+    const uint16_t* X = in + (int64_t)s * e3 * 16;
+    s4 wa[kTaps];
+#pragma unroll
+    for (int tap = 0; tap < kTaps; tap++)
+        wa[tap] = *(const s4*)(wt + ((size_t)tap * 16 + n) * 16 + 4 * g);
+    // stage plane xp into slot, as 16-byte chunks (2 per voxel)
+    auto stage = [&](int xp, int slot) {
+        uint4* dst = (uint4*)(lds + slot * kPlane);
+        for (int q = threadIdx.x; q < kLH * kLW * 2; q += blockDim.x)
+        {
+            const int vox = q >> 1, half = q & 1;
+            const int yy = vox / kLW, zz = vox - (vox / kLW) * kLW;
+            const int y = y0 - 1 + yy, z = z0 - 1 + zz;
+            uint4 v = { 0u, 0u, 0u, 0u };
+            if (xp >= 0 && xp < e && y >= 0 && y < e && z >= 0 && z < e)
+                v = *(const uint4*)(X + (((int64_t)xp * e + y) * e + z) * 16 + 8 * half);
+            dst[q] = v;
+        }
+    };
+    stage(-1, 2);  // zeros
+    stage(0, 0);
+    const int y = y0 + wid;
+    for (int x = 0; x < e; x++)
+    {
+        stage(x + 1, (x + 1) % 3);
+        __syncthreads();
+        f4 acc0 = { 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
+#pragma unroll
+        for (int tap = 0; tap < kTaps; tap++)
+        {
+            const int slot = (x + 3 + tap_dx(tap)) % 3;
+            const int yy   = wid + 1 + tap_dy(tap);
+            const uint16_t* row = lds + slot * kPlane + (yy * kLW) * 16;
+            const int zz = n + 1 + tap_dz(tap);
+            const s4 b0 = *(const s4*)(row + zz * 16 + 4 * g);
+            const s4 b1 = *(const s4*)(row + (zz + 16) * 16 + 4 * g);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[tap], b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[tap], b1, acc1, 0, 0, 0);
+        }
+        if (y < e)
+        {
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+            {
+                const int z = z0 + 16 * t + n;
+                if (z >= e) continue;
+                const int64_t vox = (int64_t)s * e3 + ((int64_t)x * e + y) * e + z;
+                const f4 a = t == 0 ? acc0 : acc1;
+                s4 o;
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                {
+                    float v = a[i];
+                    if (MODE == CONV_FWD_RELU) v = fmaxf(v + bias[4 * g + i], 0.f);
+                    if (MODE == CONV_BWD_MASK && hmask[vox * 16 + 4 * g + i] == 0) v = 0.f;
+                    o[i] = (short)f2bf(v);
+                }
+                *(s4*)(out + vox * 16 + 4 * g) = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The last layer (16 -> 1 channel) on the vector units: one output voxel per
+// thread, the 27 x 16 weights and the bias in LDS (FP32 values of the BF16
+// weights), two 16-byte loads per tap.  z = b + sum_tap sum_ci W[tap][ci] h[v + off(tap)][ci]
+// in FP32, as the MFMA path accumulates it.
+__global__ void __launch_bounds__(256) k_conv_out(const uint16_t* __restrict__ h, const uint16_t* __restrict__ wt,
+    const float* __restrict__ bias, float* __restrict__ out, int e, int64_t nvox)
+{
+    __shared__ float w[kTaps * 16];
+    for (int q = threadIdx.x; q < kTaps * 16; q += blockDim.x)
+        w[q] = bf2f(wt[(q / 16) * 256 + (q % 16)]);  // row 0 of the padded [27][16][16] layout
+    __syncthreads();
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const int64_t e3 = (int64_t)e * e * e;
+    const int64_t s  = v / e3;
+    int r            = (int)(v - s * e3);
+    const int x = r / (e * e), y = (r / e) % e, z = r % e;
+    float acc = bias[0];
+    for (int tap = 0; tap < kTaps; tap++)
+    {
+        const int xi = x + tap_dx(tap), yi = y + tap_dy(tap), zi = z + tap_dz(tap);
+        if (xi < 0 || xi >= e || yi < 0 || yi >= e || zi < 0 || zi >= e) continue;
+        const uint4* p = (const uint4*)(h + (s * e3 + ((int64_t)xi * e + yi) * e + zi) * 16);
+        const uint4 a = p[0], b = p[1];
+        const uint32_t u[8] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
+        const float* wr = w + tap * 16;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+        {
+            acc += wr[2 * k] * bf2f((uint16_t)(u[k] & 0xFFFFu));
+            acc += wr[2 * k + 1] * bf2f((uint16_t)(u[k] >> 16));
+        }
+    }
+    out[v] = acc;
+}
+
 // Weight / bias gradient partials of one layer, per wave:
 //   dW[tap][cout][cin] = sum_v dZ[v][cout] H[v + off(tap)][cin],  db = sum_v dZ
 // dZ: BF16 [voxel][16] (COUT = 16) or FP32 scalar (COUT = 1); H: BF16
@@ -476,12 +596,15 @@ int forward(mgx_cnn* c, const float* x, int nb)
     const dim3 grid((unsigned)((nt + kWaves - 1) / kWaves)), blk(64 * kWaves);
     hipLaunchKernelGGL((k_conv<1, 16, CONV_FWD_RELU>), grid, blk, 0, ctx->stream, (const void*)x, c->f0,
         (const float*)c->p + kB0, (void*)c->h1, nullptr, e, nzt, nt);
-    hipLaunchKernelGGL((k_conv<16, 16, CONV_FWD_RELU>), grid, blk, 0, ctx->stream, (const void*)c->h1, c->f1,
-        (const float*)c->p + kB1, (void*)c->h2, nullptr, e, nzt, nt);
-    hipLaunchKernelGGL((k_conv<16, 16, CONV_FWD_RELU>), grid, blk, 0, ctx->stream, (const void*)c->h2, c->f2,
-        (const float*)c->p + kB2, (void*)c->h3, nullptr, e, nzt, nt);
-    hipLaunchKernelGGL((k_conv<16, 1, CONV_FWD_LIN>), grid, blk, 0, ctx->stream, (const void*)c->h3, c->f3,
-        (const float*)c->p + kB3, (void*)c->z, nullptr, e, nzt, nt);
+    const int nyb = (e + kLY - 1) / kLY, nzb = (e + kLZ - 1) / kLZ;
+    const dim3 lgrid((unsigned)(nb * nyb * nzb));
+    hipLaunchKernelGGL((k_conv16_lds<CONV_FWD_RELU>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->h1,
+        (const uint16_t*)c->f1, (const float*)c->p + kB1, c->h2, (const uint16_t*)nullptr, e, nyb, nzb);
+    hipLaunchKernelGGL((k_conv16_lds<CONV_FWD_RELU>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->h2,
+        (const uint16_t*)c->f2, (const float*)c->p + kB2, c->h3, (const uint16_t*)nullptr, e, nyb, nzb);
+    const int64_t nvox = (int64_t)nb * e * e * e;
+    hipLaunchKernelGGL(k_conv_out, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream,
+        (const uint16_t*)c->h3, (const uint16_t*)c->f3, (const float*)c->p + kB3, c->z, e, nvox);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
@@ -521,13 +644,15 @@ int grad(mgx_cnn* c, const float* x, const float* t, int nb, double* loss)
     MGX_CHECK_LAUNCH(ctx);
     // layer 2: dZ2 = dza
     if ((rc = dw<16, 16>(c, c->dza, c->h2, nb, kW2, kB2, 16, 16))) return rc;
-    hipLaunchKernelGGL((k_conv<16, 16, CONV_BWD_MASK>), grid, blk, 0, ctx->stream, (const void*)c->dza, c->r2,
-        (const float*)nullptr, (void*)c->dzb, (const uint16_t*)c->h2, e, nzt, nt);
+    const int nyb = (e + kLY - 1) / kLY, nzb = (e + kLZ - 1) / kLZ;
+    const dim3 lgrid((unsigned)(nb * nyb * nzb));
+    hipLaunchKernelGGL((k_conv16_lds<CONV_BWD_MASK>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->dza,
+        (const uint16_t*)c->r2, (const float*)nullptr, c->dzb, (const uint16_t*)c->h2, e, nyb, nzb);
     MGX_CHECK_LAUNCH(ctx);
     // layer 1: dZ1 = dzb
     if ((rc = dw<16, 16>(c, c->dzb, c->h1, nb, kW1, kB1, 16, 16))) return rc;
-    hipLaunchKernelGGL((k_conv<16, 16, CONV_BWD_MASK>), grid, blk, 0, ctx->stream, (const void*)c->dzb, c->r1,
-        (const float*)nullptr, (void*)c->dza, (const uint16_t*)c->h1, e, nzt, nt);
+    hipLaunchKernelGGL((k_conv16_lds<CONV_BWD_MASK>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->dzb,
+        (const uint16_t*)c->r1, (const float*)nullptr, c->dza, (const uint16_t*)c->h1, e, nyb, nzb);
     MGX_CHECK_LAUNCH(ctx);
     // layer 0: dZ0 = dza, input x
     if ((rc = dw<1, 16>(c, c->dza, x, nb, kW0, kB0, 1, 16))) return rc;
