@@ -416,6 +416,70 @@ __device__ __forceinline__ T box_prolong_val(const T* uc, int ec, int x, int y, 
 #undef I
 }
 
+// The 8 fine values of the coarse cell at (xc, yc, zc): fine point
+// (2xc+ox, 2yc+oy, 2zc+oz) reads coarse indices xc (and xc+1 when ox = 1),
+// so with c[a][b][d] = coarse(xc+a, yc+b, zc+d) the same expressions as
+// box_prolong_val apply with I(dx, dy, dz) = c[dx>0][dy>0][dz>0].
+template <typename T>
+__device__ __forceinline__ T prolong8_val(const T (&c)[2][2][2], int ox, int oy, int oz)
+{
+#define I(dx, dy, dz) c[(dx) > 0][(dy) > 0][(dz) > 0]
+    if (!ox && !oy && !oz) return I(0, 0, 0);
+    if (!ox && !oy && oz) return (T)(0.5 * (T)(I(0, 0, -1) + I(0, 0, 1)));
+    if (!ox && oy && !oz) return (T)(0.5 * (T)(I(0, 1, 0) + I(0, -1, 0)));
+    if (!ox && oy && oz) return (T)(0.25 * (T)(I(0, 1, 1) + I(0, -1, 1) + I(0, 1, -1) + I(0, -1, -1)));
+    if (ox && !oy && oz) return (T)(0.25 * (T)(I(1, 0, 1) + I(1, 0, -1) + I(-1, 0, 1) + I(-1, 0, -1)));
+    if (ox && !oy && !oz) return (T)(0.5 * (T)(I(1, 0, 0) + I(-1, 0, 0)));
+    if (ox && oy && oz)
+        return (T)(0.125
+                   * (T)(I(1, 1, 1) + I(1, 1, -1) + I(1, -1, 1) + I(1, -1, -1) + I(-1, 1, 1) + I(-1, 1, -1)
+                         + I(-1, -1, 1) + I(-1, -1, -1)));
+    return (T)(0.25 * (T)(I(1, 1, 0) + I(1, -1, 0) + I(-1, 1, 0) + I(-1, -1, 0)));
+#undef I
+}
+
+// prolongation by coarse cells: one thread per coarse point writes its 8
+// fine points (no parity divergence, each coarse value loaded once per
+// cell); with SUB: v -= P(coarse)
+template <typename T, bool SUB>
+__global__ void __launch_bounds__(256) k_box_prolong8(const T* coarse, const int64_t* off_f, T* fine, BoxView vc)
+{
+    int j, ec, xc, yc, zc, p;
+    int64_t cbase;
+    if (!tile_point(vc, &j, &ec, &xc, &yc, &zc, &cbase, &p)) return;
+    const T* uc = coarse + cbase;
+    T c[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int d = 0; d < 2; d++)
+                c[a][b][d] = bv(uc, ec, xc + a, yc + b, zc + d);
+    const int ef      = 2 * ec;
+    T* uf             = fine + off_f[j];
+#pragma unroll
+    for (int ox = 0; ox < 2; ox++)
+#pragma unroll
+        for (int oy = 0; oy < 2; oy++)
+        {
+            const int64_t row = ((int64_t)(2 * xc + ox) * ef + (2 * yc + oy)) * ef + 2 * zc;
+#pragma unroll
+            for (int oz = 0; oz < 2; oz++)
+            {
+                const T pv = prolong8_val(c, ox, oy, oz);
+                if (SUB)
+                {
+                    T v = uf[row + oz];
+                    v += static_cast<T>(-1. * static_cast<double>(pv));
+                    uf[row + oz] = v;
+                }
+                else
+                    uf[row + oz] = pv;
+            }
+        }
+}
+
 // prolongation; with SUB: v -= P(coarse) (extend3D, app_mask, operator-=)
 template <typename T, bool SUB>
 __global__ void __launch_bounds__(256) k_box_prolong(const T* coarse, const int64_t* off_c, T* fine, BoxView vf)
@@ -971,17 +1035,19 @@ __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const 
                 if (ia >= 0) xa = psi[ia];
             }
             const double tb = th[aa * kCellK + b0 + c];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, tb, acc, 0, 0, 0);
+            // D[m = box b0+m][n = point p0+n] = sum_a theta(a, b) psi_a(point):
+            // A = theta^T (lane: m = c, k = r), B = psi (k = r, n = c)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(tb, xa, acc, 0, 0, 0);
         }
-        const int bb = b0 + c;
-        if (bb < k)
+        // lane (c, r) holds boxes b0 + r + 4v at point p0 + c: 16 lanes write
+        // 16 consecutive points of one box
+        const int pp = p0 + c;
+        for (int v = 0; v < 4; v++)
         {
-            for (int v = 0; v < 4; v++)
-            {
-                const int pp     = p0 + r + 4 * v;
-                const int64_t ib = cell_idx(ce[bb], pp);
-                if (ib >= 0) res[ib] = acc[v] + (-1. * hpsi[ib]);
-            }
+            const int bb = b0 + r + 4 * v;
+            if (bb >= k) continue;
+            const int64_t ib = cell_idx(ce[bb], pp);
+            if (ib >= 0) res[ib] = acc[v] + (-1. * hpsi[ib]);
         }
     }
 }
@@ -1284,9 +1350,9 @@ int box_restrict_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, const T* fine,
 template <typename T, bool SUB>
 int box_prolong_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, const T* coarse, T* fine)
 {
-    BoxView vf = view(bs, level);
-    hipLaunchKernelGGL((k_box_prolong<T, SUB>), dim3(tile_grid(vf.ntiles)), dim3(kTile), 0, ctx->stream, coarse,
-        (const int64_t*)bs->d_off[level + 1], fine, vf);
+    BoxView vc = view(bs, level + 1);
+    hipLaunchKernelGGL((k_box_prolong8<T, SUB>), dim3(tile_grid(vc.ntiles)), dim3(kTile), 0, ctx->stream, coarse,
+        (const int64_t*)bs->d_off[level], fine, vc);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
