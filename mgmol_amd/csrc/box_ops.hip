@@ -876,83 +876,103 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
         }
 }
 
-// Stencil reach of H psi_j outside box j, organised by slab: block (j, s)
-// with s = face*2 + layer-1 (12 slabs of one plane each, at distance 1 and 2
-// beyond each face).  The reach values hv(X) of the slab are computed once
-// into LDS; then every interacting box i contributes
-//   out[12*q + s] = sum_{X in slab, X in box i} psi_i(X) hv(X)
-// for its entry q in j's interaction list.
+// Stencil reach of H psi_j outside box j, organised by face: block (j, f)
+// with f = d*2 + side covers the two slabs (planes at distance 1 and 2
+// beyond the face; slab index s = f*2 + layer-1).  The reach values hv(X) of
+// both planes are computed once into LDS; then every interacting box i
+// (one wave each) contributes
+//   out[12*q + s] = sum_{X in slab s, X in box i} psi_i(X) hv(X)
+// for its entry q in j's interaction list, reading box i's two planes
+// together (adjacent values for z faces).
 __global__ void __launch_bounds__(256) k_slab_skin(const double* psi, const int64_t* off, const int* edge,
     const int* ilist_ptr, const int* ilist, Coef4 c4, int use_lds, double* out)
 {
-    extern __shared__ double hv[];
-    const int j = blockIdx.x / 12, sl = blockIdx.x - (blockIdx.x / 12) * 12;
-    const int d = sl >> 2, side = (sl >> 1) & 1, layer = (sl & 1) + 1;
+    extern __shared__ double hv[];  // 2 planes of ej^2
+    const int j = blockIdx.x / 6, f = blockIdx.x - (blockIdx.x / 6) * 6;
+    const int d = f >> 1, side = f & 1;
     const int ej = edge[j];
     const double* pj = psi + off[j];
     const double c1 = d == 0 ? c4.c1x : (d == 1 ? c4.c1y : c4.c1z);
     const double c2 = d == 0 ? c4.c2x : (d == 1 ? c4.c2y : c4.c2z);
     const int u = (d + 1) % 3, w = (d + 2) % 3;
-    const int Xd = side == 0 ? -layer : ej - 1 + layer;
-    // hv over the (u, w) plane of the slab (recomputed per use when the
-    // plane does not fit in LDS)
-    for (int t = threadIdx.x; use_lds && t < ej * ej; t += blockDim.x)
+    // plane positions along d (j-local) of layers 1 and 2
+    const int Xd1 = side == 0 ? -1 : ej, Xd2 = side == 0 ? -2 : ej + 1;
+    // the two box values next to the face: f0 at the face, f1 one further in
+    auto reach = [&](int a, int b, double* h1, double* h2) {
+        int q[3];
+        q[u]        = a;
+        q[w]        = b;
+        q[d]        = side == 0 ? 0 : ej - 1;
+        const double f0 = pj[((int64_t)q[0] * ej + q[1]) * ej + q[2]];
+        double f1       = 0.;
+        if (ej > 1)
+        {
+            q[d] = side == 0 ? 1 : ej - 2;
+            f1   = pj[((int64_t)q[0] * ej + q[1]) * ej + q[2]];
+        }
+        *h1 = c1 * f0 + c2 * f1;
+        *h2 = c2 * f0;
+    };
+    const int e2 = ej * ej;
+    for (int t = threadIdx.x; use_lds && t < e2; t += blockDim.x)
     {
         const int a = t / ej, b = t - (t / ej) * ej;
-        int q[3];
-        q[u] = a;
-        q[w] = b;
-        auto at = [&](int v) {
-            q[d] = v;
-            return v < ej ? pj[((int64_t)q[0] * ej + q[1]) * ej + q[2]] : 0.;  // edge-1 boxes
-        };
-        double val;
-        if (side == 0)
-            val = layer == 1 ? c1 * at(0) + c2 * at(1) : c2 * at(0);
-        else
-            val = layer == 1 ? c1 * at(ej - 1) + (ej > 1 ? c2 * at(ej - 2) : 0.) : c2 * at(ej - 1);
-        hv[t] = val;
+        reach(a, b, hv + t, hv + e2 + t);
     }
     __syncthreads();
     // one wave per interacting box: no block barriers in the loop
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    for (int qq = ilist_ptr[j] + wid; qq < ilist_ptr[j + 1]; qq += nw)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int s1 = 2 * f, s2 = 2 * f + 1;
+    for (int qq = ilist_ptr[j] + wid; qq < ilist_ptr[j + 1]; qq += nwv)
     {
         const int* E = ilist + 4 * qq;
         const int i = E[0], ei = edge[i];
         const int sh[3] = { E[1], E[2], E[3] };
-        double acc = 0.;
-        const int xd = Xd + sh[d];
-        if (xd >= 0 && xd < ei)
+        const int xd1 = Xd1 + sh[d], xd2 = Xd2 + sh[d];
+        const bool in1 = xd1 >= 0 && xd1 < ei, in2 = xd2 >= 0 && xd2 < ei;
+        double acc1 = 0., acc2 = 0.;
+        if (in1 || in2)
         {
             const int lo_u = max(0, -sh[u]), hi_u = min(ej, ei - sh[u]);
             const int lo_w = max(0, -sh[w]), hi_w = min(ej, ei - sh[w]);
-            const int nu = hi_u - lo_u, nwid = hi_w - lo_w;
+            const int nu = hi_u - lo_u, nww = hi_w - lo_w;
             const double* pi = psi + off[i];
-            for (int t = lane; t < nu * nwid; t += 64)
+            for (int t = lane; t < nu * nww; t += 64)
             {
-                const int a = lo_u + t / nwid, b = lo_w + (t - (t / nwid) * nwid);
+                const int a = lo_u + t / nww, b = lo_w + (t - (t / nww) * nww);
                 int x[3];
-                x[d] = xd;
                 x[u] = a + sh[u];
                 x[w] = b + sh[w];
-                double h;
+                double h1, h2;
                 if (use_lds)
-                    h = hv[a * ej + b];
-                else
                 {
-                    int X[3];
-                    X[d] = Xd;
-                    X[u] = a;
-                    X[w] = b;
-                    h = skin_hv(pj, ej, d, side, layer, X[0], X[1], X[2], c1, c2);
+                    h1 = hv[a * ej + b];
+                    h2 = hv[e2 + a * ej + b];
                 }
-                acc += pi[((int64_t)x[0] * ei + x[1]) * ei + x[2]] * h;
+                else
+                    reach(a, b, &h1, &h2);
+                if (in1)
+                {
+                    x[d] = xd1;
+                    acc1 += pi[((int64_t)x[0] * ei + x[1]) * ei + x[2]] * h1;
+                }
+                if (in2)
+                {
+                    x[d] = xd2;
+                    acc2 += pi[((int64_t)x[0] * ei + x[1]) * ei + x[2]] * h2;
+                }
             }
             for (int o = 32; o > 0; o >>= 1)
-                acc += __shfl_xor(acc, o, 64);
+            {
+                acc1 += __shfl_xor(acc1, o, 64);
+                acc2 += __shfl_xor(acc2, o, 64);
+            }
         }
-        if (lane == 0) out[12 * (int64_t)qq + sl] = acc;
+        if (lane == 0)
+        {
+            out[12 * (int64_t)qq + s1] = acc1;
+            out[12 * (int64_t)qq + s2] = acc2;
+        }
     }
 }
 
@@ -1524,10 +1544,10 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
     {
         if (bs->nilist > 0)
         {
-            size_t lds        = (size_t)bs->maxedge * bs->maxedge * sizeof(double);
+            size_t lds        = 2 * (size_t)bs->maxedge * bs->maxedge * sizeof(double);
             const int use_lds = lds <= 64 * 1024;
             if (!use_lds) lds = 0;
-            hipLaunchKernelGGL(k_slab_skin, dim3(12 * bs->nbox), dim3(256), lds, ctx->stream, psi,
+            hipLaunchKernelGGL(k_slab_skin, dim3(6 * bs->nbox), dim3(256), lds, ctx->stream, psi,
                 (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_ilist_ptr,
                 (const int*)bs->d_ilist, coef4(bs->h), use_lds, skin);
             MGX_CHECK_LAUNCH(ctx);
