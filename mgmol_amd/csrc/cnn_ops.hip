@@ -41,7 +41,17 @@ constexpr int kW0 = 0, kB0 = kW0 + 27 * 16, kW1 = kB0 + 16, kB1 = kW1 + 27 * 256
               kB2 = kW2 + 27 * 256, kW3 = kB2 + 16, kB3 = kW3 + 27 * 16, kS = kB3 + 1, kNP = kS + 1;
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)..8(l/16)+7],
+// B[8(l/16)..8(l/16)+7][l%16], D[4(l/16)+i][l%16]
+__device__ __forceinline__ f4 mfma32(s8 a, s8 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0,
+        0);
+}
 
 __host__ __device__ inline uint16_t f2bf(float f)
 {
@@ -189,10 +199,17 @@ __global__ void __launch_bounds__(256) k_conv16_lds(const uint16_t* __restrict__
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, n = lane & 15, g = lane >> 4;
     const int64_t e3 = (int64_t)e * e * e;
     const uint16_t* X = in + (int64_t)s * e3 * 16;
-    s4 wa[kTaps];
+    // K = 32 per MFMA: tap pair tp, lane group g: tap 2tp + (g >> 1), channels
+    // 8(g & 1)..8(g & 1)+7 (the 28th tap is zero)
+    constexpr int kPairs = (kTaps + 1) / 2;
+    const int tsel = g >> 1, cs = 8 * (g & 1);
+    s8 wa[kPairs];
 #pragma unroll
-    for (int tap = 0; tap < kTaps; tap++)
-        wa[tap] = *(const s4*)(wt + ((size_t)tap * 16 + n) * 16 + 4 * g);
+    for (int tp = 0; tp < kPairs; tp++)
+    {
+        const int tap = 2 * tp + tsel;
+        wa[tp]        = tap < kTaps ? *(const s8*)(wt + ((size_t)tap * 16 + n) * 16 + cs) : (s8){ 0, 0, 0, 0, 0, 0, 0, 0 };
+    }
     // stage plane xp into slot, as 16-byte chunks (2 per voxel)
     auto stage = [&](int xp, int slot) {
         uint4* dst = (uint4*)(lds + slot * kPlane);
@@ -216,16 +233,18 @@ __global__ void __launch_bounds__(256) k_conv16_lds(const uint16_t* __restrict__
         __syncthreads();
         f4 acc0 = { 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
 #pragma unroll
-        for (int tap = 0; tap < kTaps; tap++)
+        for (int tp = 0; tp < kPairs; tp++)
         {
+            // the padding tap (27) reads tap 26's operands against zero weights
+            const int tap  = min(2 * tp + tsel, kTaps - 1);
             const int slot = (x + 3 + tap_dx(tap)) % 3;
             const int yy   = wid + 1 + tap_dy(tap);
             const uint16_t* row = lds + slot * kPlane + (yy * kLW) * 16;
             const int zz = n + 1 + tap_dz(tap);
-            const s4 b0 = *(const s4*)(row + zz * 16 + 4 * g);
-            const s4 b1 = *(const s4*)(row + (zz + 16) * 16 + 4 * g);
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[tap], b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[tap], b1, acc1, 0, 0, 0);
+            const s8 b0 = *(const s8*)(row + zz * 16 + cs);
+            const s8 b1 = *(const s8*)(row + (zz + 16) * 16 + cs);
+            acc0 = mfma32(wa[tp], b0, acc0);
+            acc1 = mfma32(wa[tp], b1, acc1);
         }
         if (y < e)
         {
@@ -245,6 +264,83 @@ __global__ void __launch_bounds__(256) k_conv16_lds(const uint16_t* __restrict__
                     if (MODE == CONV_BWD_MASK && hmask[vox * 16 + 4 * g + i] == 0) v = 0.f;
                     o[i] = (short)f2bf(v);
                 }
+                *(s4*)(out + vox * 16 + 4 * g) = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The first layer (1 -> 16 channels) with the scalar input staged in LDS
+// as for k_conv16_lds: a ring of 3 planes of (6 x 34) values; one 16x16x32
+// MFMA per 16-voxel tile covers all 27 taps (K = tap, lane group g holds taps
+// 8g..8g+7, the last 5 zero).  Bias and ReLU; BF16 out.
+__global__ void __launch_bounds__(256) k_conv1_lds(const float* __restrict__ in, const uint16_t* __restrict__ wt,
+    const float* __restrict__ bias, uint16_t* __restrict__ out, int e, int nyb, int nzb)
+{
+    constexpr int kP1 = kLH * kLW;
+    __shared__ float lds[3 * kP1];
+    const int b = blockIdx.x;
+    const int s = b / (nyb * nzb), yb = (b / nzb) % nyb, zb = b % nzb;
+    const int y0 = yb * kLY, z0 = zb * kLZ;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, n = lane & 15, g = lane >> 4;
+    const int64_t e3 = (int64_t)e * e * e;
+    const float* X = in + (int64_t)s * e3;
+    // A: W0[cout = n][taps 8g..8g+7] from the [16][32] layout
+    const s8 wa = *(const s8*)(wt + (size_t)n * 32 + 8 * g);
+    float bv4[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        bv4[i] = bias[4 * g + i];
+    auto stage = [&](int xp, int slot) {
+        float* dst = lds + slot * kP1;
+        for (int q = threadIdx.x; q < kP1; q += blockDim.x)
+        {
+            const int yy = q / kLW, zz = q - (q / kLW) * kLW;
+            const int y = y0 - 1 + yy, z = z0 - 1 + zz;
+            float v = 0.f;
+            if (xp >= 0 && xp < e && y >= 0 && y < e && z >= 0 && z < e) v = X[((int64_t)xp * e + y) * e + z];
+            dst[q] = v;
+        }
+    };
+    stage(-1, 2);
+    stage(0, 0);
+    const int y = y0 + wid;
+    for (int x = 0; x < e; x++)
+    {
+        stage(x + 1, (x + 1) % 3);
+        __syncthreads();
+        s8 b0, b1;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+        {
+            const int tap = 8 * g + i;
+            float v0 = 0.f, v1 = 0.f;
+            if (tap < kTaps)
+            {
+                const float* row = lds + ((x + 3 + tap_dx(tap)) % 3) * kP1 + (wid + 1 + tap_dy(tap)) * kLW;
+                const int zz     = n + 1 + tap_dz(tap);
+                v0               = row[zz];
+                v1               = row[zz + 16];
+            }
+            b0[i] = (short)f2bf(v0);
+            b1[i] = (short)f2bf(v1);
+        }
+        const f4 z4 = { 0.f, 0.f, 0.f, 0.f };
+        const f4 acc0 = mfma32(wa, b0, z4), acc1 = mfma32(wa, b1, z4);
+        if (y < e)
+        {
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+            {
+                const int z = z0 + 16 * t + n;
+                if (z >= e) continue;
+                const int64_t vox = (int64_t)s * e3 + ((int64_t)x * e + y) * e + z;
+                const f4 a = t == 0 ? acc0 : acc1;
+                s4 o;
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    o[i] = (short)f2bf(fmaxf(a[i] + bv4[i], 0.f));
                 *(s4*)(out + vox * 16 + 4 * g) = o;
             }
         }
@@ -594,10 +690,10 @@ int forward(mgx_cnn* c, const float* x, int nb)
     const int e = c->edge, nzt = (e + 15) / 16;
     const int64_t nt = ntiles(e, nb);
     const dim3 grid((unsigned)((nt + kWaves - 1) / kWaves)), blk(64 * kWaves);
-    hipLaunchKernelGGL((k_conv<1, 16, CONV_FWD_RELU>), grid, blk, 0, ctx->stream, (const void*)x, c->f0,
-        (const float*)c->p + kB0, (void*)c->h1, nullptr, e, nzt, nt);
     const int nyb = (e + kLY - 1) / kLY, nzb = (e + kLZ - 1) / kLZ;
     const dim3 lgrid((unsigned)(nb * nyb * nzb));
+    hipLaunchKernelGGL(k_conv1_lds, lgrid, dim3(256), 0, ctx->stream, x, (const uint16_t*)c->f0,
+        (const float*)c->p + kB0, c->h1, e, nyb, nzb);
     hipLaunchKernelGGL((k_conv16_lds<CONV_FWD_RELU>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->h1,
         (const uint16_t*)c->f1, (const float*)c->p + kB1, c->h2, (const uint16_t*)nullptr, e, nyb, nzb);
     hipLaunchKernelGGL((k_conv16_lds<CONV_FWD_RELU>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->h2,
