@@ -43,6 +43,7 @@ constexpr int kW0 = 0, kB0 = kW0 + 27 * 16, kW1 = kB0 + 16, kB1 = kW1 + 27 * 256
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)..8(l/16)+7],
@@ -275,8 +276,10 @@ __global__ void __launch_bounds__(256) k_conv16_lds(const uint16_t* __restrict__
 // as for k_conv16_lds: a ring of 3 planes of (6 x 34) values; one 16x16x32
 // MFMA per 16-voxel tile covers all 27 taps (K = tap, lane group g holds taps
 // 8g..8g+7, the last 5 zero).  Bias and ReLU; BF16 out.
-__global__ void __launch_bounds__(256) k_conv1_lds(const float* __restrict__ in, const uint16_t* __restrict__ wt,
-    const float* __restrict__ bias, uint16_t* __restrict__ out, int e, int nyb, int nzb)
+template <typename TIN>
+__global__ void __launch_bounds__(256) k_conv1_lds(const TIN* __restrict__ in, const double* __restrict__ rms,
+    const uint16_t* __restrict__ wt, const float* __restrict__ bias, uint16_t* __restrict__ out, int e, int nyb,
+    int nzb)
 {
     constexpr int kP1 = kLH * kLW;
     __shared__ float lds[3 * kP1];
@@ -285,7 +288,9 @@ __global__ void __launch_bounds__(256) k_conv1_lds(const float* __restrict__ in,
     const int y0 = yb * kLY, z0 = zb * kLZ;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, n = lane & 15, g = lane >> 4;
     const int64_t e3 = (int64_t)e * e * e;
-    const float* X = in + (int64_t)s * e3;
+    const TIN* X = in + (int64_t)s * e3;
+    // inputs R / rms (the box's normalised residual) when rms is given
+    const double rs = rms ? rms[s] : 1.;
     // A: W0[cout = n][taps 8g..8g+7] from the [16][32] layout
     const s8 wa = *(const s8*)(wt + (size_t)n * 32 + 8 * g);
     float bv4[4];
@@ -299,7 +304,11 @@ __global__ void __launch_bounds__(256) k_conv1_lds(const float* __restrict__ in,
             const int yy = q / kLW, zz = q - (q / kLW) * kLW;
             const int y = y0 - 1 + yy, z = z0 - 1 + zz;
             float v = 0.f;
-            if (xp >= 0 && xp < e && y >= 0 && y < e && z >= 0 && z < e) v = X[((int64_t)xp * e + y) * e + z];
+            if (xp >= 0 && xp < e && y >= 0 && y < e && z >= 0 && z < e)
+            {
+                const TIN r = X[((int64_t)xp * e + y) * e + z];
+                v           = rms ? (float)((double)r / rs) : (float)r;
+            }
             dst[q] = v;
         }
     };
@@ -343,6 +352,85 @@ __global__ void __launch_bounds__(256) k_conv1_lds(const float* __restrict__ in,
                     o[i] = (short)f2bf(fmaxf(a[i] + bv4[i], 0.f));
                 *(s4*)(out + vox * 16 + 4 * g) = o;
             }
+        }
+        __syncthreads();
+    }
+}
+
+// The last layer (16 -> 1 channel) with the input staged in LDS: a block
+// owns rows y0..y0+7, z0..z0+31 of one sample (one voxel per thread) and
+// marches along x with a ring of 3 planes of (10 x 34) voxels; the 27 x 16
+// weights are LDS broadcasts of BF16 channel pairs, applied with the BF16
+// dot2 instruction (FP32 accumulation):
+//   z = b + sum_tap sum_ci W[tap][ci] h[v + off(tap)][ci]   (tap-major)
+// With R non-null it also finishes the network on the box: y = s x + z with
+// x = (float)(R / rms), written as (T)(y * rms) to out; otherwise z to zout.
+constexpr int kOY = 8, kOH = kOY + 2;
+template <typename T>
+__global__ void __launch_bounds__(256) k_conv_out_lds(const uint16_t* __restrict__ h, const uint16_t* __restrict__ wt,
+    const float* __restrict__ prm, float* __restrict__ zout, const double* __restrict__ R,
+    const double* __restrict__ rms, T* __restrict__ out, int e, int nyb, int nzb)
+{
+    constexpr int kPO = kOH * kLW * 16;
+    __shared__ __align__(16) uint16_t lds[3 * kPO];
+    __shared__ uint32_t w[kTaps * 8];  // channel pairs (2k, 2k+1) of W[tap], BF16
+    for (int q = threadIdx.x; q < kTaps * 8; q += blockDim.x)
+    {
+        const int tap = q / 8, k = q % 8;  // row 0 of the padded [27][16][16] layout
+        w[q] = (uint32_t)wt[tap * 256 + 2 * k] | ((uint32_t)wt[tap * 256 + 2 * k + 1] << 16);
+    }
+    const int b = blockIdx.x;
+    const int s = b / (nyb * nzb), yb = (b / nzb) % nyb, zb = b % nzb;
+    const int y0 = yb * kOY, z0 = zb * kLZ;
+    const int64_t e3 = (int64_t)e * e * e;
+    const uint16_t* X = h + (int64_t)s * e3 * 16;
+    auto stage = [&](int xp, int slot) {
+        uint4* dst = (uint4*)(lds + slot * kPO);
+        for (int q = threadIdx.x; q < kOH * kLW * 2; q += blockDim.x)
+        {
+            const int vox = q >> 1, half = q & 1;
+            const int yy = vox / kLW, zz = vox - (vox / kLW) * kLW;
+            const int y = y0 - 1 + yy, z = z0 - 1 + zz;
+            uint4 v = { 0u, 0u, 0u, 0u };
+            if (xp >= 0 && xp < e && y >= 0 && y < e && z >= 0 && z < e)
+                v = *(const uint4*)(X + (((int64_t)xp * e + y) * e + z) * 16 + 8 * half);
+            dst[q] = v;
+        }
+    };
+    stage(-1, 2);
+    stage(0, 0);
+    const int ry = threadIdx.x >> 5, rz = threadIdx.x & 31;
+    const int y = y0 + ry, z = z0 + rz;
+    const float bias = prm[kB3], sk = prm[kS];
+    const double rs  = rms ? rms[s] : 1.;
+    for (int x = 0; x < e; x++)
+    {
+        stage(x + 1, (x + 1) % 3);
+        __syncthreads();
+        float acc = bias;
+        for (int tap = 0; tap < kTaps; tap++)
+        {
+            const uint16_t* src = lds + ((x + 3 + tap_dx(tap)) % 3) * kPO
+                                  + ((ry + 1 + tap_dy(tap)) * kLW + (rz + 1 + tap_dz(tap))) * 16;
+            const uint4 a = ((const uint4*)src)[0], bq = ((const uint4*)src)[1];
+            const uint32_t u[8] = { a.x, a.y, a.z, a.w, bq.x, bq.y, bq.z, bq.w };
+            const uint32_t* wr  = w + tap * 8;
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                acc = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf2, u[k]), __builtin_bit_cast(bf2, wr[k]),
+                    acc, false);
+        }
+        if (y < e && z < e)
+        {
+            const int64_t vox = (int64_t)s * e3 + ((int64_t)x * e + y) * e + z;
+            if (R)
+            {
+                const float xv = (float)(R[vox] / rs);
+                const float yv = sk * xv + acc;
+                out[vox]       = (T)((double)yv * rs);
+            }
+            else
+                zout[vox] = acc;
         }
         __syncthreads();
     }
@@ -682,27 +770,38 @@ int derive(mgx_cnn* c)
 }
 
 // forward of nb samples from FP32 x; z (pre-skip last layer) into c->z
-int forward(mgx_cnn* c, const float* x, int nb)
+// the network over nb samples: from FP32 x (training, z of the last layer to
+// c->z), or from FP64 box residuals R with per-box rms (inference: the
+// rescaled output rms * (s x + z) written to out in the type of T)
+template <typename T>
+int forward_any(mgx_cnn* c, const float* x, const double* R, const double* rms, int nb, T* out)
 {
     mgx_ctx* ctx = c->ctx;
     int rc;
     if ((rc = derive(c))) return rc;
-    const int e = c->edge, nzt = (e + 15) / 16;
-    const int64_t nt = ntiles(e, nb);
-    const dim3 grid((unsigned)((nt + kWaves - 1) / kWaves)), blk(64 * kWaves);
+    const int e   = c->edge;
     const int nyb = (e + kLY - 1) / kLY, nzb = (e + kLZ - 1) / kLZ;
     const dim3 lgrid((unsigned)(nb * nyb * nzb));
-    hipLaunchKernelGGL(k_conv1_lds, lgrid, dim3(256), 0, ctx->stream, x, (const uint16_t*)c->f0,
-        (const float*)c->p + kB0, c->h1, e, nyb, nzb);
+    if (R)
+        hipLaunchKernelGGL(k_conv1_lds<double>, lgrid, dim3(256), 0, ctx->stream, R, rms, (const uint16_t*)c->f0,
+            (const float*)c->p + kB0, c->h1, e, nyb, nzb);
+    else
+        hipLaunchKernelGGL(k_conv1_lds<float>, lgrid, dim3(256), 0, ctx->stream, x, (const double*)nullptr,
+            (const uint16_t*)c->f0, (const float*)c->p + kB0, c->h1, e, nyb, nzb);
     hipLaunchKernelGGL((k_conv16_lds<CONV_FWD_RELU>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->h1,
         (const uint16_t*)c->f1, (const float*)c->p + kB1, c->h2, (const uint16_t*)nullptr, e, nyb, nzb);
     hipLaunchKernelGGL((k_conv16_lds<CONV_FWD_RELU>), lgrid, dim3(256), 0, ctx->stream, (const uint16_t*)c->h2,
         (const uint16_t*)c->f2, (const float*)c->p + kB2, c->h3, (const uint16_t*)nullptr, e, nyb, nzb);
-    const int64_t nvox = (int64_t)nb * e * e * e;
-    hipLaunchKernelGGL(k_conv_out, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, ctx->stream,
-        (const uint16_t*)c->h3, (const uint16_t*)c->f3, (const float*)c->p + kB3, c->z, e, nvox);
+    const int noy = (e + kOY - 1) / kOY;
+    hipLaunchKernelGGL(k_conv_out_lds<T>, dim3((unsigned)(nb * noy * nzb)), dim3(256), 0, ctx->stream,
+        (const uint16_t*)c->h3, (const uint16_t*)c->f3, (const float*)c->p, c->z, R, rms, out, e, noy, nzb);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
+}
+
+int forward(mgx_cnn* c, const float* x, int nb)
+{
+    return forward_any<float>(c, x, nullptr, nullptr, nb, nullptr);
 }
 
 template <int CIN, int COUT>
@@ -949,22 +1048,14 @@ int cnn_apply_boxes(mgx_cnn* c, int nbox, int edge, const double* R, void* out, 
     int rc;
     for (int b0 = 0; b0 < nbox; b0 += c->max_batch)
     {
-        const int nb = std::min(c->max_batch, nbox - b0);
+        const int nb     = std::min(c->max_batch, nbox - b0);
         const double* Rb = R + (int64_t)b0 * m;
         hipLaunchKernelGGL(k_box_rms, dim3(nb), dim3(256), 0, ctx->stream, Rb, m, c->rms);
-        hipLaunchKernelGGL(k_box_scale_to_f32, dim3(4096), dim3(256), 0, ctx->stream, Rb, (const double*)c->rms, m,
-            m * nb, c->xf);
         MGX_CHECK_LAUNCH(ctx);
-        if ((rc = forward(c, c->xf, nb))) return rc;
-        hipLaunchKernelGGL(k_loss, dim3(kLossBlocks), dim3(256), 0, ctx->stream, (const float*)c->xf,
-            (const float*)c->z, (const float*)nullptr, (const float*)c->p, m * nb, c->y, (float*)nullptr, c->lpart);
-        if (out_dtype == MGX_F32)
-            hipLaunchKernelGGL(k_box_scale_from_f32<float>, dim3(4096), dim3(256), 0, ctx->stream,
-                (const float*)c->y, (const double*)c->rms, m, m * nb, (float*)out + (int64_t)b0 * m);
-        else
-            hipLaunchKernelGGL(k_box_scale_from_f32<double>, dim3(4096), dim3(256), 0, ctx->stream,
-                (const float*)c->y, (const double*)c->rms, m, m * nb, (double*)out + (int64_t)b0 * m);
-        MGX_CHECK_LAUNCH(ctx);
+        rc = out_dtype == MGX_F32
+                 ? forward_any<float>(c, nullptr, Rb, c->rms, nb, (float*)out + (int64_t)b0 * m)
+                 : forward_any<double>(c, nullptr, Rb, c->rms, nb, (double*)out + (int64_t)b0 * m);
+        if (rc) return rc;
     }
     return MGX_OK;
 }
