@@ -779,6 +779,74 @@ __device__ __forceinline__ int64_t cell_idx3(const CellEnt& c, int x, int y, int
 // in a fixed order; partials go to part[slot + a*k + b] (G) and
 // part[nslot + slot + a*k + b] (P).  MFMA step s of wave w covers the four
 // points x = 2w + s/16, y = (s/2)%8, z = 4(s%2) + lane/16.
+// one 16 x 16 tile pair (ti, tj) of a cell's S / H partials, summed over
+// this wave's 128 points; SAME: ti == tj, so the B operand of S is A's
+template <int PREC, bool SAME>
+__device__ __forceinline__ void cell_gram_tile(const double* psi, const double* hpsi, const CellEnt& A,
+    const CellEnt& B, int wid, int r, double (&accG)[4], double (&accP)[4])
+{
+    if (PREC == MGX_PREC_F64)
+    {
+        d4c g = { 0., 0., 0., 0. }, h = { 0., 0., 0., 0. };
+#pragma unroll 4
+        for (int s = 0; s < 32; s++)
+        {
+            const int x = 2 * wid + (s >> 4), y = (s >> 1) & 7, z = 4 * (s & 1) + r;
+            const int64_t ia = cell_idx3(A, x, y, z);
+            const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
+            const double xa = ia >= 0 ? psi[ia] : 0.;
+            const double xb = SAME ? xa : (ib >= 0 ? psi[ib] : 0.);
+            const double hb = ib >= 0 ? hpsi[ib] : 0.;
+            g = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, xb, g, 0, 0, 0);
+            h = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, hb, h, 0, 0, 0);
+        }
+        for (int v = 0; v < 4; v++)
+        {
+            accG[v] = g[v];
+            accP[v] = h[v];
+        }
+    }
+    else
+    {
+        // hi/lo split: hi*hi + (hi*lo + lo*hi) on FP32 MFMA; FP32 sums
+        // over 8 steps (32 points), FP64 across them
+        for (int v = 0; v < 4; v++)
+            accG[v] = accP[v] = 0.;
+        for (int s0 = 0; s0 < 32; s0 += 8)
+        {
+            f4c g1 = { 0.f, 0.f, 0.f, 0.f }, g2 = g1, h1 = g1, h2 = g1;
+#pragma unroll 4
+            for (int s = s0; s < s0 + 8; s++)
+            {
+                const int x = 2 * wid + (s >> 4), y = (s >> 1) & 7, z = 4 * (s & 1) + r;
+                const int64_t ia = cell_idx3(A, x, y, z);
+                const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
+                const double xa = ia >= 0 ? psi[ia] : 0.;
+                const double xb = SAME ? xa : (ib >= 0 ? psi[ib] : 0.);
+                const double hb = ib >= 0 ? hpsi[ib] : 0.;
+                const float ah = (float)xa, al = (float)(xa - (double)ah);
+                const float bh = SAME ? ah : (float)xb, bl = SAME ? al : (float)(xb - (double)bh);
+                const float hh = (float)hb, hl = (float)(hb - (double)hh);
+                g1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, g1, 0, 0, 0);
+                g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, g2, 0, 0, 0);
+                g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, g2, 0, 0, 0);
+                h1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hh, h1, 0, 0, 0);
+                h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hl, h2, 0, 0, 0);
+                h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, hh, h2, 0, 0, 0);
+            }
+            for (int v = 0; v < 4; v++)
+            {
+                accG[v] += (double)g1[v] + (double)g2[v];
+                accP[v] += (double)h1[v] + (double)h2[v];
+            }
+        }
+    }
+}
+
+// S / H partials per 8^3 cell over all box pairs covering it (k <= 32):
+// G_ab = sum_p psi_a psi_b, P_ab = sum_p psi_a hpsi_b, as 16 x 16 box tiles
+// on MFMA with K = points (4 per step); wave w takes x-planes 2w, 2w+1, and
+// the four waves' sums are added in a fixed order.
 template <int PREC>
 __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
     const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part)
@@ -800,61 +868,11 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
             const int a = ti * 16 + c, b = tj * 16 + c;
             const CellEnt A = a < k ? ce[a] : none;
             const CellEnt B = b < k ? ce[b] : none;
-            double accG[4] = { 0., 0., 0., 0. }, accP[4] = { 0., 0., 0., 0. };
-            if (PREC == MGX_PREC_F64)
-            {
-                d4c g = { 0., 0., 0., 0. }, h = { 0., 0., 0., 0. };
-#pragma unroll 4
-                for (int s = 0; s < 32; s++)
-                {
-                    const int x = 2 * wid + (s >> 4), y = (s >> 1) & 7, z = 4 * (s & 1) + r;
-                    const int64_t ia = cell_idx3(A, x, y, z);
-                    const int64_t ib = cell_idx3(B, x, y, z);
-                    const double xa = ia >= 0 ? psi[ia] : 0.;
-                    const double xb = ib >= 0 ? psi[ib] : 0.;
-                    const double hb = ib >= 0 ? hpsi[ib] : 0.;
-                    g = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, xb, g, 0, 0, 0);
-                    h = __builtin_amdgcn_mfma_f64_16x16x4f64(xa, hb, h, 0, 0, 0);
-                }
-                for (int v = 0; v < 4; v++)
-                {
-                    accG[v] = g[v];
-                    accP[v] = h[v];
-                }
-            }
+            double accG[4], accP[4];
+            if (ti == tj)
+                cell_gram_tile<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
             else
-            {
-                // hi/lo split: hi*hi + (hi*lo + lo*hi) on FP32 MFMA; FP32 sums
-                // over 8 steps (32 points), FP64 across them
-                for (int s0 = 0; s0 < 32; s0 += 8)
-                {
-                    f4c g1 = { 0.f, 0.f, 0.f, 0.f }, g2 = g1, h1 = g1, h2 = g1;
-#pragma unroll 4
-                    for (int s = s0; s < s0 + 8; s++)
-                    {
-                        const int x = 2 * wid + (s >> 4), y = (s >> 1) & 7, z = 4 * (s & 1) + r;
-                        const int64_t ia = cell_idx3(A, x, y, z);
-                        const int64_t ib = cell_idx3(B, x, y, z);
-                        const double xa = ia >= 0 ? psi[ia] : 0.;
-                        const double xb = ib >= 0 ? psi[ib] : 0.;
-                        const double hb = ib >= 0 ? hpsi[ib] : 0.;
-                        const float ah = (float)xa, al = (float)(xa - (double)ah);
-                        const float bh = (float)xb, bl = (float)(xb - (double)bh);
-                        const float hh = (float)hb, hl = (float)(hb - (double)hh);
-                        g1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, g1, 0, 0, 0);
-                        g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, g2, 0, 0, 0);
-                        g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, g2, 0, 0, 0);
-                        h1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hh, h1, 0, 0, 0);
-                        h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hl, h2, 0, 0, 0);
-                        h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, hh, h2, 0, 0, 0);
-                    }
-                    for (int v = 0; v < 4; v++)
-                    {
-                        accG[v] += (double)g1[v] + (double)g2[v];
-                        accP[v] += (double)h1[v] + (double)h2[v];
-                    }
-                }
-            }
+                cell_gram_tile<PREC, false>(psi, hpsi, A, B, wid, r, accG, accP);
             for (int pass = 0; pass < 2; pass++)
             {
                 const double* acc = pass == 0 ? accG : accP;
