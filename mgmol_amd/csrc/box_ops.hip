@@ -268,6 +268,10 @@ struct MarchArgs
     int g0, g1, g2;
     int lds_w;  // LDS row width = max edge + 4
     double mjf;
+    // HLOC only: a pending normalisation psi_j *= 1/sqrt(nrm[j]), applied as
+    // psi is loaded (and written back to psi_w, the same array as in)
+    const double* nrm = nullptr;
+    double* psi_w     = nullptr;
 };
 
 template <typename T, int TYPE, int MODE, int NT, int P>
@@ -294,6 +298,17 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
         qy[k]       = act[k] ? q / e : 0;
         qz[k]       = act[k] ? q - (q / e) * e : 0;
     }
+    // the pending normalisation (HLOC on FP64 only): every value is loaded
+    // once, scaled with LocGridOrbitals::normalize's factor and written back
+    const bool scl  = MODE == MARCH_HLOC && sizeof(T) == sizeof(double) && a.nrm != nullptr;
+    const double fs = scl ? 1. / sqrt(a.nrm[j]) : 1.;
+    auto ld = [&](int64_t idx) -> T {
+        if (!scl) return in[idx];
+        double v = a.psi_w[idx];
+        v *= fs;
+        a.psi_w[idx] = v;
+        return (T)v;
+    };
     T qv[P][5];
 #pragma unroll
     for (int k = 0; k < P; k++)
@@ -301,9 +316,9 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
         const int q = threadIdx.x + k * NT;
         qv[k][0]    = (T)0;
         qv[k][1]    = (T)0;
-        qv[k][2]    = act[k] ? in[base + q] : (T)0;
-        qv[k][3]    = (act[k] && e > 1) ? in[base + E2 + q] : (T)0;
-        qv[k][4]    = (act[k] && e > 2) ? in[base + 2 * E2 + q] : (T)0;
+        qv[k][2]    = act[k] ? ld(base + q) : (T)0;
+        qv[k][3]    = (act[k] && e > 1) ? ld(base + E2 + q) : (T)0;
+        qv[k][4]    = (act[k] && e > 2) ? ld(base + 2 * E2 + q) : (T)0;
     }
     __syncthreads();
     for (int x = 0; x < e; x++)
@@ -314,7 +329,7 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
         for (int k = 0; k < P; k++)
         {
             const int q = threadIdx.x + k * NT;
-            nx[k]       = (act[k] && x + 3 < e) ? in[base + (int64_t)(x + 3) * E2 + q] : (T)0;
+            nx[k]       = (act[k] && x + 3 < e) ? ld(base + (int64_t)(x + 3) * E2 + q) : (T)0;
         }
 #pragma unroll
         for (int k = 0; k < P; k++)
@@ -1265,7 +1280,7 @@ void march_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, const 
 // sizes at this level need another kernel
 template <typename T, int TYPE, int MODE>
 bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* in, const T* f, const double* v,
-    T* out, T* w)
+    T* out, T* w, const double* nrm = nullptr)
 {
     const int emax = bs->maxedge >> level;
     const int E2   = emax * emax;
@@ -1279,6 +1294,11 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
     a.g2     = bs->gdim[2] >> level;
     a.lds_w  = emax + 4;
     a.mjf    = mjf;
+    if (nrm)
+    {
+        a.nrm   = nrm;
+        a.psi_w = (double*)in;
+    }
     Coefs c  = coefs_at(bs, level);
     if ((size_t)a.lds_w * a.lds_w * sizeof(T) > 60 * 1024) return false;
     if (E2 <= 64)
@@ -1326,13 +1346,15 @@ int box_lap_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, const T* 
 }
 
 template <typename T>
-int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vtot, T* out)
+int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vtot, T* out,
+    const double* nrm = nullptr)
 {
-    if (march<T, MGX_LAP_4TH, MARCH_HLOC>(ctx, bs, 0, 0., in, nullptr, vtot, out, nullptr))
+    if (march<T, MGX_LAP_4TH, MARCH_HLOC>(ctx, bs, 0, 0., in, nullptr, vtot, out, nullptr, nrm))
     {
         MGX_CHECK_LAUNCH(ctx);
         return MGX_OK;
     }
+    if (nrm) return set_err(ctx, MGX_EINVAL, "fused normalisation needs the marching kernel");
     BoxView v = view(bs, 0);
     Coef4 c4  = coef4(bs->h);
     hipLaunchKernelGGL(k_box_hloc<T>, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, in, vtot, out, v, c4);
@@ -1686,6 +1708,23 @@ int box_sumsq(mgx_ctx* ctx, const mgx_boxset* bs, const double* r, double* d)
 int box_norm(mgx_ctx* ctx, const mgx_boxset* bs, const double* d, const double* Sinv, double* dev_out)
 {
     hipLaunchKernelGGL(k_norm, dim3(1), dim3(256), 0, ctx->stream, d, Sinv, bs->nbox, dev_out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+// Hpsi of the FP64 orbitals with the pending normalisation d (vel*|psi_j|^2
+// of the unnormalised psi) applied on the way, psi rewritten normalised
+int box_hloc_normalize(mgx_ctx* ctx, const mgx_boxset* bs, double* psi, const double* vtot, double* hpsi,
+    const double* d)
+{
+    return box_hloc_t(ctx, bs, (const double*)psi, vtot, hpsi, d);
+}
+
+int box_scale(mgx_ctx* ctx, const mgx_boxset* bs, double* psi, const double* d)
+{
+    BoxView v = view(bs, 0);
+    hipLaunchKernelGGL(k_box_scale, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi,
+        (const int64_t*)bs->d_off[0], (const int*)bs->d_tile[0], bs->ntiles[0], (const int*)bs->d_edge, d);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }

@@ -31,6 +31,8 @@ int box_residual(mgx_ctx*, const mgx_boxset*, const double*, const double*, cons
 int box_sumsq(mgx_ctx*, const mgx_boxset*, const double*, double*);
 int box_norm(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
 int box_normalize(mgx_ctx*, const mgx_boxset*, double*, double*);
+int box_hloc_normalize(mgx_ctx*, const mgx_boxset*, double*, const double*, double*, const double*);
+int box_scale(mgx_ctx*, const mgx_boxset*, double*, const double*);
 int box_precond(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, double*, double*, double);
 int64_t box_size(const mgx_boxset*, int);
 void boxset_geom(const mgx_boxset*, int gdim[3], double h[3], int* nbox);
@@ -73,6 +75,9 @@ struct mgx_solver
     double e_prev = 0., deig = 1e9, deig2 = 1e9;
     bool accelerate = false;
     int mm          = 0;  // Anderson history length in use
+    // the normalisation of the last update is pending: d holds vel*|psi_j|^2
+    // of the unnormalised psi; the next Hpsi applies it (or flush_norm)
+    bool pending_norm = false;
     // learned preconditioner: the state before the last CNN step, to undo it
     mgx_cnn* cnn       = nullptr;
     double *psi_prev = nullptr, *r_prev = nullptr;
@@ -111,7 +116,14 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     int rc, tk;
     const int n = s->n;
     tk = prof_begin(ctx, MGX_PROF_HLOC);
-    if ((rc = box_hloc(ctx, s->bs, MGX_F64, s->psi, s->vtot, s->hpsi))) return rc;
+    if (s->pending_norm)
+    {
+        // LocGridOrbitals::normalize of the last update, fused with Hpsi
+        if ((rc = box_hloc_normalize(ctx, s->bs, s->psi, s->vtot, s->hpsi, s->d))) return rc;
+        s->pending_norm = false;
+    }
+    else if ((rc = box_hloc(ctx, s->bs, MGX_F64, s->psi, s->vtot, s->hpsi)))
+        return rc;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_GRAM);
     if ((rc = box_gram_h(ctx, s->bs, s->cfg.gram_prec, s->psi, s->hpsi, s->S, s->H, s->Sv, s->Hv))) return rc;
@@ -279,10 +291,23 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         }
     }
     prof_end(ctx, tk);
+    // LocGridOrbitals::normalize: the norms now, the scaling with the next Hpsi
     tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
-    if ((rc = box_normalize(ctx, s->bs, s->psi, s->d))) return rc;
+    if ((rc = box_sumsq(ctx, s->bs, s->psi, s->d))) return rc;
+    s->pending_norm = true;
     prof_end(ctx, tk);
     s->iter++;
+    return MGX_OK;
+}
+
+// apply a pending normalisation, so that psi is normalised when seen from outside
+int flush_norm(mgx_solver* s)
+{
+    if (!s->pending_norm) return MGX_OK;
+    int rc = box_scale(s->ctx, s->bs, s->psi, s->d);
+    if (rc) return rc;
+    s->pending_norm = false;
+    MGX_CHECK_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
     return MGX_OK;
 }
 
@@ -386,6 +411,7 @@ int mgx_solver_step(mgx_solver* s, mgx_solver_stats_t* st)
     bool done = false;
     auto t0   = std::chrono::steady_clock::now();
     int rc    = step_impl(s, -1., st, &done);
+    if (!rc) rc = flush_norm(s);
     st->seconds    = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st->iterations = s->iter;
     st->converged  = 0;
@@ -404,6 +430,7 @@ int mgx_solver_run(mgx_solver* s, double tol, int maxit, mgx_solver_stats_t* st)
         rc = step_impl(s, tol, st, &done);
         if (rc) break;
     }
+    if (!rc) rc = flush_norm(s);
     st->seconds    = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     st->iterations = s->iter - it0;
     st->converged  = done ? 1 : 0;
@@ -419,6 +446,7 @@ int mgx_solver_reset(mgx_solver* s)
     s->iter       = 0;
     s->lowest     = 0.;
     s->have_inv   = false;
+    s->pending_norm = false;
     s->e_prev     = 0.;
     s->deig       = 1e9;
     s->deig2      = 1e9;
@@ -450,6 +478,8 @@ int mgx_solver_profile(mgx_solver* s, double* ms, int64_t* launches, int reset)
 int mgx_solver_get_psi(mgx_solver* s, double* host)
 {
     if (!s || !host) return MGX_EINVAL;
+    int rc = flush_norm(s);
+    if (rc) return rc;
     MGX_CHECK_HIP(s->ctx, hipMemcpy(host, s->psi, s->npts * sizeof(double), hipMemcpyDeviceToHost));
     return MGX_OK;
 }
@@ -464,6 +494,8 @@ int mgx_solver_set_cnn(mgx_solver* s, mgx_cnn* cnn)
 int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H, double** res)
 {
     if (!s) return MGX_EINVAL;
+    int rc = flush_norm(s);
+    if (rc) return rc;
     if (psi) *psi = s->psi;
     if (hpsi) *hpsi = s->hpsi;
     if (S) *S = s->S;
