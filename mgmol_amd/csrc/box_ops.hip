@@ -739,6 +739,9 @@ __global__ void __launch_bounds__(256) k_pair_gram(const double* psi, const doub
     }
 }
 
+// blocks for ncell cells: a multiple of 8, so that xcd_block is a bijection
+inline unsigned cell_grid(int ncell) { return (unsigned)((ncell + 7) / 8 * 8); }
+
 constexpr int kCell  = 8;    // cell edge
 constexpr int kCellK = 32;   // max boxes per cell handled by the cell kernels
 
@@ -864,11 +867,15 @@ __device__ __forceinline__ void cell_gram_tile(const double* psi, const double* 
 // the four waves' sums are added in a fixed order.
 template <int PREC>
 __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
-    const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part)
+    const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part,
+    int ncell)
 {
     __shared__ CellEnt ce[kCellK];
     __shared__ double red[4][64][4];
-    const int cell = blockIdx.x;
+    // consecutive cells (z-neighbours share cache lines) on one XCD
+    const int64_t cl = xcd_block(blockIdx.x, gridDim.x);
+    if (cl >= ncell) return;
+    const int cell = (int)cl;
     const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
     load_cell(cell_ent + 10 * e0, off, edge, k, ce);
     __syncthreads();
@@ -1052,12 +1059,15 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
 // point p of the cell, as a 512 x k by k x k product on FP64 MFMA:
 // A = psi gathered per (point, box a), B = theta sub-block from LDS.
 __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
-    int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res)
+    int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res, int ncell)
 {
     __shared__ CellEnt ce[kCellK];
     __shared__ double th[kCellK * kCellK];
     __shared__ int jid[kCellK];
-    const int cell = blockIdx.x;
+    // consecutive cells (z-neighbours share cache lines) on one XCD
+    const int64_t cl = xcd_block(blockIdx.x, gridDim.x);
+    if (cl >= ncell) return;
+    const int cell = (int)cl;
     const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
     load_cell(cell_ent + 10 * e0, off, edge, k, ce);
     for (int t = threadIdx.x; t < k; t += blockDim.x)
@@ -1572,13 +1582,13 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, S, nn, 0.);
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, H, nn, 0.);
     if (prec == MGX_PREC_SPLIT)
-        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_SPLIT>, dim3(bs->ncell), dim3(256), 0, ctx->stream, psi, hpsi,
-            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part);
+        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_SPLIT>, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi,
+            hpsi, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part, bs->ncell);
     else
-        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_F64>, dim3(bs->ncell), dim3(256), 0, ctx->stream, psi, hpsi,
-            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part);
+        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_F64>, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi,
+            hpsi, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part, bs->ncell);
     MGX_CHECK_LAUNCH(ctx);
     if (bs->npairs > 0)
     {
@@ -1638,9 +1648,9 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
 {
     if (bs->use_cells)
     {
-        hipLaunchKernelGGL(k_cell_residual, dim3(bs->ncell), dim3(256), 0, ctx->stream, psi, hpsi, theta, bs->nbox,
-            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-            (const int*)bs->d_cell_ent, res);
+        hipLaunchKernelGGL(k_cell_residual, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, hpsi, theta,
+            bs->nbox, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+            (const int*)bs->d_cell_ent, res, bs->ncell);
         MGX_CHECK_LAUNCH(ctx);
         return MGX_OK;
     }
