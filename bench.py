@@ -39,6 +39,10 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFS = 2500.0
 CNN_FLOP_PER_VOXEL = 2 * 27 * (1 * 16 + 16 * 16 + 16 * 16 + 16 * 1)
 WEIGHTS = os.path.join(ROOT, "mgmol_amd", "weights", "cnn_e32.npz")
+# HBM bytes per launch at C5, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes
+# (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000, both calibrated on
+# streaming kernels of known size; profiles/pmc_traffic_c5.md)
+TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895255953}
 
 PATHS = {
     "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64),
@@ -111,8 +115,9 @@ def jacobi_roofline(r, s_bytes, npts):
         return None
     avg_s = (ms / 1e3) / launches
     achieved = per_launch / avg_s / 1e9
+    traffic = TRAFFIC_C5["jacobi_fp32_sweep"] if (s_bytes == 4 and npts == 2048 * 32 ** 3) else None
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_box_march<JACOBI> fine-level sweep (%s)" % ("FP32" if s_bytes == 4 else "FP64"),
                 avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches), bytes_per_launch=int(per_launch))
 
@@ -124,8 +129,9 @@ def cnn_roofline(r, npts):
     per_launch = CNN_FLOP_PER_VOXEL * npts
     avg_s = (ms / 1e3) / launches
     achieved = per_launch / avg_s / 1e12
+    traffic = TRAFFIC_C5["cnn_application"] if npts == 2048 * 32 ** 3 else None
     return dict(bound="mfma", achieved=round(achieved, 2), peak=BF16_PEAK_TFS, unit="TFLOP/s",
-                frac=round(achieved / BF16_PEAK_TFS, 5), traffic=None,
+                frac=round(achieved / BF16_PEAK_TFS, 5), traffic=traffic,
                 kernel="CNN inference (4 conv3d layers, BF16 MFMA) over all boxes",
                 avg_launch_ms=round(1e3 * avg_s, 3), launches=int(launches), flop_per_launch=int(per_launch))
 
