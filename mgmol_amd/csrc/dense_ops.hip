@@ -374,12 +374,167 @@ int spmm_pairs(mgx_ctx* ctx, int n, const double* X, const double* val, const Pa
     return MGX_OK;
 }
 
+// Wide products (n >= kWideN): the tiles of k_tsgemm would re-read each
+// column strip n/16 times, so these go to the library GEMMs, split over K
+// into batches (an n x n output alone is too few tiles for the chip when K
+// is hundreds of thousands of rows) whose partial products are summed in
+// FP64.  FP64: strided-batched dgemm over chunks of kF64Chunk rows or more
+// (at most kF64MaxParts partials) plus one dgemm for the remainder.  Split:
+// the rows are taken in groups of kWideGroup chunks of kWideChunk rows; each
+// group is converted to FP32 hi / lo copies (hi = fl32(a), lo = fl32(a - hi))
+// and multiplied chunk by chunk with strided-batched FP32 GEMMs, hi*hi and
+// the cross terms in separate FP32 partials per chunk; for a Gram the cross
+// term is hi^T lo plus its transpose.  A Gram (A == B) is mirrored from its
+// lower triangle, so it is exactly symmetric.
+constexpr int kWideN       = 256;
+constexpr int kWideChunk   = 1024;
+constexpr int kWideGroup   = 32;
+constexpr int kF64Chunk    = 16384;
+constexpr int kF64MaxParts = 64;
+
+// C = sum_c parts[c] (FP64, np partials of n x n); sym: both triangles from
+// the lower
+__global__ void k_sum_parts(const double* parts, int np, int n, int sym, double* C)
+{
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t / n), row = (int)(t - (int64_t)col * n);
+        if (sym && row < col) continue;
+        double v = 0.;
+        for (int c = 0; c < np; c++)
+            v += parts[c * nn + t];
+        C[t] = v;
+        if (sym) C[(int64_t)row * n + col] = v;
+    }
+}
+
+// hi / lo of rows [k0, k0 + rows) of each column of A (leading dimension
+// ld, lnp valid rows), stored with leading dimension kp = group rows;
+// rows past lnp are written as zeros
+__global__ void k_split_hilo_ld(const double* A, int64_t ld, int64_t lnp, int64_t k0, int64_t kp, int n, float* hi,
+    float* lo)
+{
+    const int64_t tot = kp * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t col = t / kp, k = k0 + (t - col * kp);
+        const double a    = k < lnp ? A[col * ld + k] : 0.;
+        const float h     = (float)a;
+        hi[t]             = h;
+        lo[t]             = (float)(a - (double)h);
+    }
+}
+
+// C (+)= vel * sum_c (p0[c] + p1[c]) over nchunk n x n FP32 partials,
+// summed in FP64.  sym: the cross term is p1 + p1^T and both triangles of C
+// are written from the lower.
+__global__ void k_wide_sum(const float* p0, const float* p1, int nchunk, int n, double vel, int accumulate, int sym,
+    double* C)
+{
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t / n), row = (int)(t - (int64_t)col * n);
+        if (sym && row < col) continue;
+        const int64_t tt = (int64_t)row * n + col;
+        double hh = 0., x = 0.;
+        for (int c = 0; c < nchunk; c++)
+        {
+            hh += (double)p0[c * nn + t];
+            x += (double)p1[c * nn + t];
+            if (sym) x += (double)p1[c * nn + tt];
+        }
+        const double v = vel * (hh + x) + (accumulate ? C[t] : 0.);
+        C[t]           = v;
+        if (sym) C[tt] = v;
+    }
+}
+
+int tsgemm_wide(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_t ld, int64_t lnp, int n,
+    int subdivx, double vel, double* ss)
+{
+    const int64_t nn = (int64_t)n * n;
+    const bool same  = A == B;
+    rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    for (int sl = 0; sl < subdivx; sl++)
+    {
+        const double* As = A + (int64_t)sl * lnp;
+        const double* Bs = B + (int64_t)sl * lnp;
+        double* C        = ss + sl * nn;
+        if (prec == MGX_PREC_F64)
+        {
+            const double zero  = 0.;
+            const int64_t kc   = std::max<int64_t>(kF64Chunk, (lnp + kF64MaxParts - 1) / kF64MaxParts);
+            const int64_t nful = lnp / kc, rem = lnp - nful * kc;
+            const int np       = (int)(nful + (rem > 0));
+            double* parts      = (double*)workspace(ctx, 7, (size_t)np * nn * sizeof(double));
+            if (!parts) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+            rocblas_status st = rocblas_status_success;
+            if (nful > 0)
+                st = rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n,
+                    n, (rocblas_int)kc, &vel, As, (rocblas_int)ld, kc, Bs, (rocblas_int)ld, kc, &zero, parts, n, nn,
+                    (rocblas_int)nful);
+            if (st == rocblas_status_success && rem > 0)
+                st = rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
+                    (rocblas_int)rem, &vel, As + nful * kc, (rocblas_int)ld, Bs + nful * kc, (rocblas_int)ld, &zero,
+                    parts + nful * nn, n);
+            if (st != rocblas_status_success) return set_err(ctx, MGX_EHIP, "rocBLAS FP64 product failed");
+            hipLaunchKernelGGL(k_sum_parts, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, (const double*)parts, np,
+                n, same ? 1 : 0, C);
+            MGX_CHECK_LAUNCH(ctx);
+            continue;
+        }
+        const int64_t nchunk_all = (lnp + kWideChunk - 1) / kWideChunk;
+        const int gch            = (int)std::min<int64_t>(nchunk_all, kWideGroup);
+        const int64_t kp         = (int64_t)gch * kWideChunk;
+        const size_t opf         = (size_t)kp * n;
+        const size_t need        = ((same ? 2 : 4) * opf + 2 * (size_t)gch * nn) * sizeof(float);
+        float* w                 = (float*)workspace(ctx, 7, need);
+        if (!w) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+        float* ahi = w;
+        float* alo = ahi + opf;
+        float* bhi = same ? ahi : alo + opf;
+        float* blo = same ? alo : bhi + opf;
+        float* p0  = (same ? alo : blo) + opf;
+        float* p1  = p0 + (size_t)gch * nn;
+        const float one = 1.f, zero = 0.f;
+        const rocblas_stride sk = kWideChunk, sc = nn;
+        for (int64_t c0 = 0; c0 < nchunk_all; c0 += gch)
+        {
+            const int nc    = (int)std::min<int64_t>(gch, nchunk_all - c0);
+            const int64_t k0 = c0 * kWideChunk;
+            hipLaunchKernelGGL(k_split_hilo_ld, dim3(4096), dim3(256), 0, ctx->stream, As, ld, lnp, k0, kp, n, ahi,
+                alo);
+            if (!same)
+                hipLaunchKernelGGL(k_split_hilo_ld, dim3(4096), dim3(256), 0, ctx->stream, Bs, ld, lnp, k0, kp, n,
+                    bhi, blo);
+            MGX_CHECK_LAUNCH(ctx);
+            rocblas_status st = rocblas_sgemm_strided_batched(ctx->blas, rocblas_operation_transpose,
+                rocblas_operation_none, n, n, kWideChunk, &one, ahi, (rocblas_int)kp, sk, bhi, (rocblas_int)kp, sk,
+                &zero, p0, n, sc, nc);
+            if (st == rocblas_status_success)
+                st = rocblas_sgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n,
+                    n, kWideChunk, &one, ahi, (rocblas_int)kp, sk, blo, (rocblas_int)kp, sk, &zero, p1, n, sc, nc);
+            if (st == rocblas_status_success && !same)
+                st = rocblas_sgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n,
+                    n, kWideChunk, &one, alo, (rocblas_int)kp, sk, bhi, (rocblas_int)kp, sk, &one, p1, n, sc, nc);
+            if (st != rocblas_status_success) return set_err(ctx, MGX_EHIP, "rocBLAS FP32 product failed");
+            hipLaunchKernelGGL(k_wide_sum, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, (const float*)p0,
+                (const float*)p1, nc, n, vel, c0 > 0 ? 1 : 0, same ? 1 : 0, C);
+            MGX_CHECK_LAUNCH(ctx);
+        }
+    }
+    return MGX_OK;
+}
+
 int tsgemm(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_t ld, int64_t lnp, int n, int subdivx,
     double vel, int lower_only, double* ss)
 {
     MGX_REQUIRE(ctx, A && B && ss && n > 0 && subdivx > 0 && lnp > 0, "bad product arguments");
     MGX_REQUIRE(ctx, (int64_t)subdivx * lnp <= ld, "slabs exceed the leading dimension");
     MGX_REQUIRE(ctx, prec == MGX_PREC_F64 || prec == MGX_PREC_SPLIT, "bad precision mode");
+    if (n >= kWideN) return tsgemm_wide(ctx, prec, A, B, ld, lnp, n, subdivx, vel, ss);
     const int ntile  = (n + 15) / 16;
     const int ntiles = ntile * ntile;
     // enough chunks to fill the chip, each at least 256 rows

@@ -158,10 +158,12 @@ def grid_orbitals(n, gdim, h, seed, sigma=1.0, noise=1e-3, nwells=16):
 # The five workloads of the measurement plan.
 CONFIGS = {
     "c1": dict(kind="grid", n=64, gdim=96, h=0.3, nwells=16, levels=3),
-    "c2": dict(kind="box", nbox=512, gdim=192, h=0.25, edges=(24, 28, 32, 36, 40), nwells=16, levels=3),
+    # c2, c5: centres at least 2 bohr apart: uniform centres put
+    # near-duplicate orbitals in the set (c5: overlap 0.98, cond(S) ~ 225 at
+    # seed 1), with which the reference's update direction never settles
+    # (residual stalls at 1e-3 .. 1e-2)
+    "c2": dict(kind="box", nbox=512, gdim=192, h=0.25, edges=(24, 28, 32, 36, 40), nwells=16, levels=3,
+               min_sep=2.0),
     "c3": dict(kind="dense", n=1024, gdim=256, h=0.2),
-    # centres at least 2 bohr apart: uniform centres put near-duplicate
-    # orbitals in the set (overlap 0.98, cond(S) ~ 225 at seed 1), with which
-    # the reference's update direction never settles (residual stalls ~1e-2)
     "c5": dict(kind="box", nbox=2048, gdim=256, h=0.2, edges=(32,), nwells=64, levels=3, min_sep=2.0),
 }

@@ -66,6 +66,38 @@ def test_local_gram_split_error(ctx):
     assert err < 2e-8
 
 
+def test_wide_products(ctx):
+    """n >= 256 goes to the library path: dsyrk / dgemm in FP64, grouped
+    FP32 hi/lo chunk products summed in FP64 for the split.  Two slabs of
+    35,000 rows each: 35 chunks of 1024 (the last one zero-padded), two
+    groups of chunks."""
+    rng = np.random.default_rng(21)
+    n, lnp, sub = 256, 35000, 2
+    ld = sub * lnp + 17
+    psi = rng.standard_normal((n, ld)) * 1e-2
+    hp = psi + 1e-3 * rng.standard_normal((n, ld))
+    vel = 0.008
+    tp, th = to_dev(psi), to_dev(hp)
+    ts = to_dev(np.zeros((sub, n, n)))
+    gref = O.local_gram(psi, lnp, sub, vel)
+    pref = O.local_product(psi, hp, lnp, sub, vel)
+    ctx.call("mgx_local_gram", mgx.PREC_F64, mgx.ptr(tp), ld, lnp, n, sub, vel, mgx.ptr(ts))
+    g = to_np(ts)
+    assert relmax(g, gref) <= 1e-13
+    assert np.array_equal(g, np.transpose(g, (0, 2, 1)))
+    ctx.call("mgx_local_product", mgx.PREC_F64, mgx.ptr(tp), mgx.ptr(th), ld, lnp, n, sub, vel, mgx.ptr(ts))
+    assert relmax(to_np(ts), pref) <= 1e-13
+    ctx.call("mgx_local_gram", mgx.PREC_SPLIT, mgx.ptr(tp), ld, lnp, n, sub, vel, mgx.ptr(ts))
+    g = to_np(ts)
+    eg = relmax(g, gref)
+    assert np.array_equal(g, np.transpose(g, (0, 2, 1)))
+    ctx.call("mgx_local_product", mgx.PREC_SPLIT, mgx.ptr(tp), mgx.ptr(th), ld, lnp, n, sub, vel, mgx.ptr(ts))
+    ep = relmax(to_np(ts), pref)
+    say("wide split rel err (max-norm): gram %.3e product %.3e" % (eg, ep))
+    # FP32 accumulation within 1024-row chunks, FP64 over the chunks
+    assert eg < 2e-7 and ep < 2e-7
+
+
 def test_local_to_global(ctx):
     rng = np.random.default_rng(12)
     sub, m, numst = 3, 4, 7
