@@ -274,20 +274,26 @@ struct MarchArgs
     double* psi_w     = nullptr;
 };
 
-template <typename T, int TYPE, int MODE, int NT, int P>
+// NB: number of LDS plane buffers.  With 2 the planes alternate and one
+// barrier per plane suffices (plane x+1 is written to the buffer that plane
+// x-1 was read from, and every thread has passed plane x's barrier, so it is
+// done reading plane x-1).  PD: planes loaded ahead (1: plane x+3 is loaded
+// while plane x is computed; 2: plane x+4).
+template <typename T, int TYPE, int MODE, int NT, int P, int NB, int PD>
 __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, const T* __restrict__ f,
     const double* __restrict__ vtot, T* __restrict__ out, T* __restrict__ wout, MarchArgs a, Coef4 c4, Coef2 c2)
 {
     extern __shared__ unsigned char lds_raw[];
-    T* pl       = reinterpret_cast<T*>(lds_raw);
+    T* pl0      = reinterpret_cast<T*>(lds_raw);
     const int j = blockIdx.x;
     const int e = a.edge[j] >> a.level;
     const int E2 = e * e;
     const int W  = a.lds_w;
     const int64_t base = a.off[j];
-    // zero the plane including its ring once; the ring is never written again
-    for (int t = threadIdx.x; t < W * W; t += NT)
-        pl[t] = (T)0;
+    // zero the plane buffers including their rings once; the rings are never
+    // written again
+    for (int t = threadIdx.x; t < NB * W * W; t += NT)
+        pl0[t] = (T)0;
     int qy[P], qz[P];
     bool act[P];
 #pragma unroll
@@ -309,7 +315,9 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
         a.psi_w[idx] = v;
         return (T)v;
     };
+    // qv[k][d]: plane x-2+d at this thread's point; nx[i]: plane x+3+i
     T qv[P][5];
+    T nx[PD][P];
 #pragma unroll
     for (int k = 0; k < P; k++)
     {
@@ -319,17 +327,32 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
         qv[k][2]    = act[k] ? ld(base + q) : (T)0;
         qv[k][3]    = (act[k] && e > 1) ? ld(base + E2 + q) : (T)0;
         qv[k][4]    = (act[k] && e > 2) ? ld(base + 2 * E2 + q) : (T)0;
+#pragma unroll
+        for (int i = 0; i + 1 < PD; i++)
+            nx[i][k] = (act[k] && e > 3 + i) ? ld(base + (int64_t)(3 + i) * E2 + q) : (T)0;
     }
     __syncthreads();
     for (int x = 0; x < e; x++)
     {
-        // prefetch plane x+3
-        T nx[P];
+        T* pl = (NB == 2 && (x & 1)) ? pl0 + W * W : pl0;
+        // load plane x+2+PD; the potential of plane x (HLOC) is gathered here
+        // too, so its latency overlaps the LDS store and the barrier
+        double pot[P];
 #pragma unroll
         for (int k = 0; k < P; k++)
         {
             const int q = threadIdx.x + k * NT;
-            nx[k]       = (act[k] && x + 3 < e) ? ld(base + (int64_t)(x + 3) * E2 + q) : (T)0;
+            const int xn = x + 2 + PD;
+            nx[PD - 1][k] = (act[k] && xn < e) ? ld(base + (int64_t)xn * E2 + q) : (T)0;
+            pot[k]        = 0.;
+            if (MODE == MARCH_HLOC && TYPE == MGX_LAP_4TH && act[k])
+            {
+                int gx = a.origin[3 * j] + x, gy = a.origin[3 * j + 1] + qy[k], gz = a.origin[3 * j + 2] + qz[k];
+                if (gx >= a.g0) gx -= a.g0;
+                if (gy >= a.g1) gy -= a.g1;
+                if (gz >= a.g2) gz -= a.g2;
+                pot[k] = vtot[((int64_t)gx * a.g1 + gy) * a.g2 + gz];
+            }
         }
 #pragma unroll
         for (int k = 0; k < P; k++)
@@ -346,14 +369,7 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
             if (TYPE == MGX_LAP_4TH)
             {
                 double diag = c4.c0;
-                if (MODE == MARCH_HLOC)
-                {
-                    int gx = a.origin[3 * j] + x, gy = a.origin[3 * j + 1] + qy[k], gz = a.origin[3 * j + 2] + qz[k];
-                    if (gx >= a.g0) gx -= a.g0;
-                    if (gy >= a.g1) gy -= a.g1;
-                    if (gz >= a.g2) gz -= a.g2;
-                    diag = c4.c0 + vtot[((int64_t)gx * a.g1 + gy) * a.g2 + gz];
-                }
+                if (MODE == MARCH_HLOC) diag = c4.c0 + pot[k];
                 r = diag * (double)qv[k][2] + c4.c1x * ((double)qv[k][1] + (double)qv[k][3])
                     + c4.c1y * ((double)ctr[-W] + (double)ctr[W]) + c4.c1z * ((double)ctr[-1] + (double)ctr[1])
                     + c4.c2x * ((double)qv[k][0] + (double)qv[k][4])
@@ -377,7 +393,181 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
             else
                 out[gi] = (T)r;
         }
+        if (NB == 1) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < P; k++)
+        {
+            qv[k][0] = qv[k][1];
+            qv[k][1] = qv[k][2];
+            qv[k][2] = qv[k][3];
+            qv[k][3] = qv[k][4];
+            qv[k][4] = nx[0][k];
+#pragma unroll
+            for (int i = 0; i + 1 < PD; i++)
+                nx[i][k] = nx[i + 1][k];
+        }
+    }
+}
+
+// The same sweeps with V consecutive z-points per thread (V = 4 for FP32,
+// 2 for FP64: 16 bytes; every box edge at this level a multiple of 4): one
+// 16-byte load or store per array and plane instead of V, and the y- and
+// z-neighbours read from LDS as whole V-wide pieces.  LDS rows have a ring of
+// ZR = max(V, 2) in z (so the pieces stay aligned) and 2 in y; the arithmetic
+// per point is that of k_box_march.
+template <typename T, int TYPE, int MODE, int NT, int P, int NB, int V>
+__global__ void __launch_bounds__(NT, 8) k_box_march4(const T* __restrict__ in, const T* __restrict__ f,
+    const double* __restrict__ vtot, T* __restrict__ out, T* __restrict__ wout, MarchArgs a, Coef4 c4, Coef2 c2)
+{
+    typedef T V4 __attribute__((ext_vector_type(V)));
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    constexpr int ZR = V > 2 ? V : 2;
+    extern __shared__ unsigned char lds_raw[];
+    T* pl0      = reinterpret_cast<T*>(lds_raw);
+    const int j = blockIdx.x;
+    const int e = a.edge[j] >> a.level;
+    const int E2 = e * e;
+    const int W  = a.lds_w;  // max edge + 2 ZR
+    const int HW = W * (W - 2 * ZR + 4);
+    const int64_t base = a.off[j];
+    for (int t = threadIdx.x; t < NB * HW; t += NT)
+        pl0[t] = (T)0;
+    int qy[P], qz[P];
+    bool act[P];
+#pragma unroll
+    for (int k = 0; k < P; k++)
+    {
+        const int q = V * (threadIdx.x + k * NT);
+        act[k]      = q < E2;
+        qy[k]       = act[k] ? q / e : 0;
+        qz[k]       = act[k] ? q - (q / e) * e : 0;
+    }
+    const bool scl  = MODE == MARCH_HLOC && sizeof(T) == sizeof(double) && a.nrm != nullptr;
+    const double fs = scl ? 1. / sqrt(a.nrm[j]) : 1.;
+    auto ld = [&](int64_t idx) -> V4 {
+        if (!scl) return *reinterpret_cast<const V4*>(in + idx);
+        V4 v = *reinterpret_cast<const V4*>(a.psi_w + idx);
+#pragma unroll
+        for (int i = 0; i < V; i++)
+        {
+            double d = (double)v[i];
+            d *= fs;
+            v[i] = (T)d;
+        }
+        *reinterpret_cast<V4*>(a.psi_w + idx) = v;
+        return v;
+    };
+    V4 zero4;
+#pragma unroll
+    for (int i = 0; i < V; i++)
+        zero4[i] = (T)0;
+    V4 qv[P][5];
+    V4 nx[P];
+#pragma unroll
+    for (int k = 0; k < P; k++)
+    {
+        const int q = V * (threadIdx.x + k * NT);
+        qv[k][0]    = zero4;
+        qv[k][1]    = zero4;
+        qv[k][2]    = act[k] ? ld(base + q) : zero4;
+        qv[k][3]    = (act[k] && e > 1) ? ld(base + E2 + q) : zero4;
+        qv[k][4]    = (act[k] && e > 2) ? ld(base + 2 * E2 + q) : zero4;
+    }
+    __syncthreads();
+    for (int x = 0; x < e; x++)
+    {
+        T* pl = (NB == 2 && (x & 1)) ? pl0 + HW : pl0;
+        double pot[P][V];
+#pragma unroll
+        for (int k = 0; k < P; k++)
+        {
+            const int q = V * (threadIdx.x + k * NT);
+            nx[k]       = (act[k] && x + 3 < e) ? ld(base + (int64_t)(x + 3) * E2 + q) : zero4;
+#pragma unroll
+            for (int i = 0; i < V; i++)
+                pot[k][i] = 0.;
+            if (MODE == MARCH_HLOC && TYPE == MGX_LAP_4TH && act[k])
+            {
+                int gx = a.origin[3 * j] + x, gy = a.origin[3 * j + 1] + qy[k];
+                if (gx >= a.g0) gx -= a.g0;
+                if (gy >= a.g1) gy -= a.g1;
+                const double* vrow = vtot + ((int64_t)gx * a.g1 + gy) * a.g2;
+#pragma unroll
+                for (int i = 0; i < V; i++)
+                {
+                    int gz = a.origin[3 * j + 2] + qz[k] + i;
+                    if (gz >= a.g2) gz -= a.g2;
+                    pot[k][i] = vrow[gz];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < P; k++)
+            if (act[k]) *reinterpret_cast<V4*>(pl + (qy[k] + 2) * W + qz[k] + ZR) = qv[k][2];
         __syncthreads();
+#pragma unroll
+        for (int k = 0; k < P; k++)
+        {
+            if (!act[k]) continue;
+            const int q      = V * (threadIdx.x + k * NT);
+            const T* ctr     = pl + (qy[k] + 2) * W + qz[k] + ZR;
+            const int64_t gi = base + (int64_t)x * E2 + q;
+            const V4 ym1 = *reinterpret_cast<const V4*>(ctr - W);
+            const V4 yp1 = *reinterpret_cast<const V4*>(ctr + W);
+            const V2 zl  = *reinterpret_cast<const V2*>(ctr - 2);
+            const V2 zr  = *reinterpret_cast<const V2*>(ctr + V);
+            V4 ym2 = zero4, yp2 = zero4;
+            if (TYPE == MGX_LAP_4TH)
+            {
+                ym2 = *reinterpret_cast<const V4*>(ctr - 2 * W);
+                yp2 = *reinterpret_cast<const V4*>(ctr + 2 * W);
+            }
+            const V4 cc = qv[k][2];
+            T zz[V + 4];
+            zz[0] = zl[0];
+            zz[1] = zl[1];
+#pragma unroll
+            for (int i = 0; i < V; i++)
+                zz[2 + i] = cc[i];
+            zz[V + 2] = zr[0];
+            zz[V + 3] = zr[1];
+            V4 fv = zero4;
+            if (MODE == MARCH_JACOBI) fv = *reinterpret_cast<const V4*>(f + gi);
+            V4 ov, wv4;
+#pragma unroll
+            for (int i = 0; i < V; i++)
+            {
+                double r;
+                if (TYPE == MGX_LAP_4TH)
+                {
+                    double diag = c4.c0;
+                    if (MODE == MARCH_HLOC) diag = c4.c0 + pot[k][i];
+                    r = diag * (double)cc[i] + c4.c1x * ((double)qv[k][1][i] + (double)qv[k][3][i])
+                        + c4.c1y * ((double)ym1[i] + (double)yp1[i]) + c4.c1z * ((double)zz[i + 1] + (double)zz[i + 3])
+                        + c4.c2x * ((double)qv[k][0][i] + (double)qv[k][4][i])
+                        + c4.c2y * ((double)ym2[i] + (double)yp2[i]) + c4.c2z * ((double)zz[i] + (double)zz[i + 4]);
+                }
+                else
+                {
+                    r = c2.c0 * cc[i] + c2.c1x * (T)(qv[k][1][i] + qv[k][3][i]) + c2.c1y * (T)(ym1[i] + yp1[i])
+                        + c2.c1z * (T)(zz[i + 1] + zz[i + 3]);
+                }
+                if (MODE == MARCH_JACOBI)
+                {
+                    T wv = (T)r;
+                    wv += static_cast<T>(-1. * static_cast<double>(fv[i]));
+                    wv4[i] = wv;
+                    T xv   = cc[i];
+                    xv += static_cast<T>(a.mjf * static_cast<double>(wv));
+                    ov[i] = xv;
+                }
+                else
+                    ov[i] = (T)r;
+            }
+            if (MODE == MARCH_JACOBI && wout) *reinterpret_cast<V4*>(wout + gi) = wv4;
+            *reinterpret_cast<V4*>(out + gi) = ov;
+        }
+        if (NB == 1) __syncthreads();
 #pragma unroll
         for (int k = 0; k < P; k++)
         {
@@ -1281,9 +1471,32 @@ template <typename T, int TYPE, int MODE, int NT, int P>
 void march_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, const T* in, const T* f, const double* v,
     T* out, T* w, const Coefs& c)
 {
-    const size_t lds = (size_t)a.lds_w * a.lds_w * sizeof(T);
-    hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P>), dim3(bs->nbox), dim3(NT), lds, ctx->stream, in, f, v, out,
-        w, a, c.c4, c.c2);
+    // two plane buffers unless that costs occupancy (8 blocks of this size
+    // per CU must fit the 160 KB of LDS); loads two planes ahead when the
+    // extra registers are few
+    const size_t plane = (size_t)a.lds_w * a.lds_w * sizeof(T);
+    constexpr int PD   = (P * sizeof(T) <= 16) ? 2 : 1;
+    if (2 * plane * 8 <= 160 * 1024)
+        hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P, 2, PD>), dim3(bs->nbox), dim3(NT), 2 * plane,
+            ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
+    else
+        hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P, 1, PD>), dim3(bs->nbox), dim3(NT), plane, ctx->stream,
+            in, f, v, out, w, a, c.c4, c.c2);
+}
+
+template <typename T, int TYPE, int MODE, int NT, int P>
+void march4_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
+    const double* v, T* out, T* w, const Coefs& c)
+{
+    constexpr int V = 16 / sizeof(T);
+    // two plane buffers when the blocks that fill a CU (32 waves) still fit
+    // its 160 KB of LDS
+    if (2 * plane * (2048 / NT) <= 160 * 1024)
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V>), dim3(bs->nbox), dim3(NT), 2 * plane,
+            ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
+    else
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V>), dim3(bs->nbox), dim3(NT), plane, ctx->stream,
+            in, f, v, out, w, a, c.c4, c.c2);
 }
 
 // the marching sweep for 2nd/4th-order operators; returns false if the box
@@ -1310,6 +1523,31 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
         a.psi_w = (double*)in;
     }
     Coefs c  = coefs_at(bs, level);
+    bool by4 = emax % 4 == 0;
+    for (int jb = 0; by4 && jb < bs->nbox; jb++)
+        by4 = ((bs->edge[jb] >> level) % 4) == 0;
+    if (by4)
+    {
+        // V points per thread, z ring ZR (see k_box_march4)
+        constexpr int V    = 16 / sizeof(T);
+        constexpr int ZR   = V > 2 ? V : 2;
+        a.lds_w            = emax + 2 * ZR;
+        const size_t plane = (size_t)a.lds_w * (emax + 4) * sizeof(T);
+        const int groups   = E2 / V;
+        if (plane <= 60 * 1024 && groups <= 1024)
+        {
+            if (groups <= 64)
+                march4_launch<T, TYPE, MODE, 64, 1>(ctx, bs, a, plane, in, f, v, out, w, c);
+            else if (groups <= 256)
+                march4_launch<T, TYPE, MODE, 256, 1>(ctx, bs, a, plane, in, f, v, out, w, c);
+            else if (groups <= 512)
+                march4_launch<T, TYPE, MODE, 512, 1>(ctx, bs, a, plane, in, f, v, out, w, c);
+            else
+                march4_launch<T, TYPE, MODE, 1024, 1>(ctx, bs, a, plane, in, f, v, out, w, c);
+            return true;
+        }
+        a.lds_w = emax + 4;
+    }
     if ((size_t)a.lds_w * a.lds_w * sizeof(T) > 60 * 1024) return false;
     if (E2 <= 64)
         march_launch<T, TYPE, MODE, 64, 1>(ctx, bs, a, in, f, v, out, w, c);
