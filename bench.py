@@ -136,6 +136,32 @@ def cnn_roofline(r, npts):
                 avg_launch_ms=round(1e3 * avg_s, 3), launches=int(launches), flop_per_launch=int(per_launch))
 
 
+def stream_reference(ctx, n=1 << 28, reps=20):
+    """Achievable HBM rate on this device: the engine's FP32 axpy (y += a x,
+    STREAM-triad form: two reads and one write per element) over 1 GiB
+    arrays; bytes / wall time of back-to-back launches."""
+    import ctypes as C
+    a = mgx.DeviceArray(ctx, shape=(n,), dtype=np.float32)
+    b = mgx.DeviceArray(ctx, shape=(n,), dtype=np.float32)
+    sync = mgx.DeviceArray(ctx, shape=(1,), dtype=np.float64)
+    L = mgx.lib()
+
+    def run(k):
+        for _ in range(k):
+            ctx.check(L.mgx_axpy(ctx.h, mgx.F32, C.c_int64(n), C.c_double(0.5), mgx.ptr(a), mgx.ptr(b)), "mgx_axpy")
+        sync.get()
+
+    run(3)
+    t0 = time.perf_counter()
+    run(reps)
+    dt = (time.perf_counter() - t0) / reps
+    for d in (a, b, sync):
+        d.free()
+    gbs = 3 * 4 * n / dt / 1e9
+    return dict(kernel="k_axpy<float> (y += a x, 1 GiB arrays: 2 reads + 1 write per element)",
+                GBps=round(gbs, 1), frac_of_peak=round(gbs / HBM_PEAK_GBS, 4))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +207,11 @@ def main():
     else:
         roof, other = roof_vc, roof_cnn
 
+    stream = stream_reference(ctx)
+    for r in (roof, other):
+        if r and r["unit"] == "GB/s":
+            r["frac_of_stream"] = round(r["achieved"] / stream["GBps"], 4)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -212,6 +243,7 @@ def main():
         "kernel_ms": {m: {g: round(t[0], 2) for g, t in r["profile"].items()} for m, r in res.items()},
         "roofline": roof,
         "roofline_other": other,
+        "stream": stream,
         "cpu_baseline": cpu,
     }
     if "plain" in res:

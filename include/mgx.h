@@ -269,10 +269,14 @@ typedef struct
 
 /* MGX_PRECOND_LEARNED: the step is alpha * CNN(R); when the next residual
  * norm is larger, psi and R are restored, the V-cycle step is taken instead
- * and the V-cycle is used for the next MGX_LEARNED_BACKOFF steps. */
+ * and the V-cycle is used for the next MGX_LEARNED_BACKOFF steps, a number
+ * that doubles with each rejection in a row (at most
+ * MGX_LEARNED_BACKOFF_MAX) and returns to MGX_LEARNED_BACKOFF once a
+ * network step is accepted. */
 #define MGX_PRECOND_VCYCLE 0
 #define MGX_PRECOND_LEARNED 1
 #define MGX_LEARNED_BACKOFF 4
+#define MGX_LEARNED_BACKOFF_MAX 256
 
 #define MGX_START_GAMMA 0
 #define MGX_START_ZERO 1

@@ -84,6 +84,7 @@ struct mgx_solver
     bool cnn_pending   = false;
     double r_prev_norm = 0.;
     int backoff        = 0;
+    int backoff_len    = MGX_LEARNED_BACKOFF;
     int learned_steps = 0, fallbacks = 0;
     mgx::PairTiles tiles;
     bool have_inv = false;  // Sinv holds the inverse of a recent S
@@ -186,10 +187,15 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     {
         if (s->cnn_pending && rnorm > s->r_prev_norm)
         {
+            // rejected: the V-cycle for the next backoff_len steps; each
+            // rejection in a row doubles that (up to MGX_LEARNED_BACKOFF_MAX)
             revert = true;
             s->fallbacks++;
-            s->backoff = MGX_LEARNED_BACKOFF;
+            s->backoff     = s->backoff_len;
+            s->backoff_len = std::min(2 * s->backoff_len, MGX_LEARNED_BACKOFF_MAX);
         }
+        else if (s->cnn_pending)
+            s->backoff_len = MGX_LEARNED_BACKOFF;
         s->cnn_pending = false;
         if (revert)
         {
@@ -454,6 +460,7 @@ int mgx_solver_reset(mgx_solver* s)
     s->mm         = 0;
     s->cnn_pending   = false;
     s->backoff       = 0;
+    s->backoff_len   = MGX_LEARNED_BACKOFF;
     s->learned_steps = 0;
     s->fallbacks     = 0;
     return MGX_OK;

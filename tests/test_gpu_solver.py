@@ -134,7 +134,8 @@ def test_solver_learned_fallback_to_vcycle(ctx):
     """A network that points uphill (y = -10 x) is rejected by the residual
     test: psi and R are restored and the V-cycle step is taken, so after the
     rejection the run retraces the plain V-cycle run, and the CNN is tried
-    again after MGX_LEARNED_BACKOFF V-cycle steps."""
+    again after MGX_LEARNED_BACKOFF V-cycle steps, then after twice as
+    many once it has been rejected again."""
     from mgmol_amd import learned
     cfg = synth.box_config(nbox=16, gdim=48, h=0.3, edges=(16,), nwells=4, seed=11, levels=3)
     bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
@@ -146,7 +147,7 @@ def test_solver_learned_fallback_to_vcycle(ctx):
     plain = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, **kw)
     lrn = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, precond_kind=mgx.PRECOND_LEARNED, cnn=net, **kw)
     ep = [plain.step().band_energy for _ in range(6)]
-    sts = [lrn.step() for _ in range(8)]
+    sts = [lrn.step() for _ in range(18)]
     el = [st.band_energy for st in sts]
     say("plain", ep)
     say("learned", el, [(st.learned_steps, st.fallbacks) for st in sts])
@@ -160,6 +161,11 @@ def test_solver_learned_fallback_to_vcycle(ctx):
     # rejects it
     assert sts[6].learned_steps == 1 and sts[6].fallbacks == 1
     assert sts[7].learned_steps == 2 and sts[7].fallbacks == 2
+    # the second rejection in a row: 8 V-cycle steps (8..15), then the
+    # network at step 16, rejected at step 17
+    for k in range(8, 17):
+        assert sts[k].learned_steps == 2 and sts[k].fallbacks == 2
+    assert sts[17].learned_steps == 3 and sts[17].fallbacks == 3
     for s in (plain, lrn):
         s.close()
     net.close()
