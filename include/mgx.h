@@ -31,7 +31,7 @@ extern "C" {
 #define MGX_EINVAL -1   /* bad argument (reference: assert / exit(2)) */
 #define MGX_EHIP -2     /* HIP runtime or library failure */
 #define MGX_ENOTSPD -3  /* Cholesky failed: S not SPD (reference: mmpi.abort) */
-#define MGX_ENUMERIC -4 /* NaN/Inf detected */
+#define MGX_ENUMERIC -4 /* NaN/Inf detected, or a density charge off the electron count */
 #define MGX_ENOMEM -5   /* device allocation failed */
 
 #define MGX_F32 0
@@ -219,6 +219,17 @@ int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, cons
 /* d_j = vel*|r_j|^2 (device n); norm = 0.5*sqrt(sum d_j Sinv_jj) host out */
 int mgx_box_residual_norm(mgx_ctx* ctx, const mgx_boxset* bs, const double* res,
     const double* Sinv, double* norm);
+/* Electron density rho(r) = sum_ij dm_ij psi_i(r) psi_j(r) on the level-0 periodic
+ * grid (device, gdim[0]*gdim[1]*gdim[2] FP64, x slowest, z fastest; overwritten);
+ * dm: n x n column-major, n = number of boxes.  *charge (host, may be NULL) =
+ * vel * sum rho.  nel > 0 rescales rho to nel electrons and fails with
+ * MGX_ENUMERIC (rho unscaled) when nel / charge is off 1 by more than 2%.
+ * Needs the box set's cell tables (grid edges multiples of 8, at most 32 boxes
+ * covering any 8^3 cell), else MGX_EINVAL.
+ * Replaces Rho<LocGridOrbitals>::computeRhoSubdomainUsingBlas3 (src/Rho.cc:360-449)
+ * with computeTotalCharge / rescaleTotalCharge (src/Rho.cc:122-208). */
+int mgx_box_density(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* dm,
+    int n, double nel, double* rho, double* charge);
 
 /* ------------------------------------------------ learned preconditioner ---- */
 /* No counterpart in the reference: a 4-layer 3D CNN (1->16->16->16->1 channels,
@@ -329,6 +340,12 @@ int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, d
 /* attach a trained network (uniform box edge equal to the network's) for
  * cfg.precond_kind = MGX_PRECOND_LEARNED; the solver does not own it */
 int mgx_solver_set_cnn(mgx_solver* s, mgx_cnn* cnn);
+/* Density of the solver's current (normalised) orbitals with the density matrix
+ * of fully occupied non-orthogonal orbitals, dm = 2 S^-1 (S recomputed for the
+ * current psi; ProjectedMatrices::setDMto2InvS, src/ProjectedMatrices.cc:265-274),
+ * then as mgx_box_density.  Overwrites the solver's S, H, S^-1 and theta, which the
+ * next step recomputes. */
+int mgx_solver_density(mgx_solver* s, double nel, double* rho, double* charge);
 
 #ifdef __cplusplus
 }

@@ -1305,6 +1305,119 @@ __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const 
     }
 }
 
+// Electron density on the cells (Rho::computeRhoSubdomainUsingBlas3,
+// src/Rho.cc:360-449: product = Phi X, then rho += product o Phi):
+//   rho(p) = sum_b psi_b(p) sum_a X_ab psi_a(p)
+// over the k boxes covering the cell, the inner sum as the (k x k)(k x 512)
+// product on FP64 MFMA (as in k_cell_residual), the outer one per lane and
+// over the four lanes of a point.  Each point of the periodic grid lies in
+// one cell, so rho is written once per point (points no box covers stay as
+// the caller set them); csum[cell] = the cell's sum of rho, in a fixed order.
+__global__ void __launch_bounds__(256) k_cell_density(const double* psi, const double* X, int n, const int64_t* off,
+    const int* edge, const int* origin, int g0, int g1, int g2, const int* cell_ptr, const int* cell_ent, double* rho,
+    double* csum, int ncell)
+{
+    __shared__ CellEnt ce[kCellK];
+    __shared__ double xs[kCellK * kCellK];
+    __shared__ int jid[kCellK];
+    __shared__ double wsum[4];
+    const int64_t cl = xcd_block(blockIdx.x, gridDim.x);
+    if (cl >= ncell) return;
+    const int cell = (int)cl;
+    const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
+    load_cell(cell_ent + 10 * e0, off, edge, k, ce);
+    for (int t = threadIdx.x; t < k; t += blockDim.x)
+        jid[t] = cell_ent[10 * (e0 + t)];
+    __syncthreads();
+    for (int t = threadIdx.x; t < kCellK * kCellK; t += blockDim.x)
+    {
+        const int aa = t / kCellK, bb = t - (t / kCellK) * kCellK;
+        xs[t] = (aa < k && bb < k) ? X[(int64_t)jid[bb] * n + jid[aa]] : 0.;
+    }
+    __syncthreads();
+    // global coordinates of the cell origin (from any covering box)
+    int gc[3];
+    {
+        const int gd[3] = { g0, g1, g2 };
+        const int bl[3] = { ce[0].b0, ce[0].b1, ce[0].b2 };
+        for (int d = 0; d < 3; d++)
+        {
+            int v = (origin[3 * jid[0] + d] + bl[d]) % gd[d];
+            gc[d] = v < 0 ? v + gd[d] : v;
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int c = lane & 15, r = lane >> 4;
+    const int nct = (k + 15) / 16;
+    const int ksteps = (k + 3) / 4;
+    double wacc = 0.;
+    for (int job = wid; job < 32; job += 4)
+    {
+        const int p0 = job * 16, pp = p0 + c;
+        double rs = 0.;
+        for (int bt = 0; bt < nct; bt++)
+        {
+            const int b0 = bt * 16;
+            d4c acc = { 0., 0., 0., 0. };
+            for (int st = 0; st < ksteps; st++)
+            {
+                const int aa = 4 * st + r;
+                double xa    = 0.;
+                if (aa < k)
+                {
+                    const int64_t ia = cell_idx(ce[aa], pp);
+                    if (ia >= 0) xa = psi[ia];
+                }
+                const double tb = xs[aa * kCellK + b0 + c];
+                // D[m = box b0+m][point p0+n] = sum_a X(a, b) psi_a(point)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(tb, xa, acc, 0, 0, 0);
+            }
+            // lane (c, r) holds boxes b0 + r + 4v at point p0 + c
+            for (int v = 0; v < 4; v++)
+            {
+                const int bb = b0 + r + 4 * v;
+                if (bb >= k) continue;
+                const int64_t ib = cell_idx(ce[bb], pp);
+                if (ib >= 0) rs += acc[v] * psi[ib];
+            }
+        }
+        rs += __shfl_xor(rs, 16, 64);
+        rs += __shfl_xor(rs, 32, 64);
+        if (r == 0)
+        {
+            int gx = gc[0] + (pp >> 6), gy = gc[1] + ((pp >> 3) & 7), gz = gc[2] + (pp & 7);
+            if (gx >= g0) gx -= g0;
+            if (gy >= g1) gy -= g1;
+            if (gz >= g2) gz -= g2;
+            rho[((int64_t)gx * g1 + gy) * g2 + gz] = rs;
+            wacc += rs;
+        }
+    }
+    // lanes 0..15 hold the wave's point values: add them in lane order
+    for (int o = 1; o < 16; o <<= 1)
+        wacc += __shfl_xor(wacc, o, 64);
+    if (lane == 0) wsum[wid] = wacc;
+    __syncthreads();
+    if (threadIdx.x == 0) csum[cell] = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+}
+
+// out[0] = vel * sum_i v[i] in a fixed order (one block)
+__global__ void __launch_bounds__(256) k_sum_scaled(const double* v, int n, double vel, double* out)
+{
+    __shared__ double lds[4];
+    double acc[1] = { 0. };
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        acc[0] += v[i];
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) out[0] = acc[0] * vel;
+}
+
+__global__ void k_scale_d(double* u, int64_t n, double a)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        u[t] *= a;
+}
+
 __global__ void k_pair_scatter(const double* part, const int* pairs, int64_t npairs, int n, double vel, double* S,
     double* H, double* Sv, double* Hv)
 {
@@ -1899,6 +2012,45 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
     return MGX_OK;
 }
 
+// rho on the level-0 grid (x slowest, z fastest) from the boxes' psi and
+// the density matrix X (n x n, column-major): zeroed, then written cell by
+// cell; *charge = vel * sum rho (Rho::computeTotalCharge,
+// src/Rho.cc:122-153).  With nel > 0, rho is rescaled to nel electrons
+// (Rho::rescaleTotalCharge, src/Rho.cc:155-208), which fails with
+// MGX_ENUMERIC, rho left unscaled, when the factor is off 1 by more than 2%.
+int box_density(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* X, int n, double nel,
+    double* rho, double* charge)
+{
+    if (!bs->use_cells)
+        return set_err(ctx, MGX_EINVAL,
+            "the density needs the cell tables (grid edges multiples of 8, at most 32 boxes per cell)");
+    if (n != bs->nbox) return set_err(ctx, MGX_EINVAL, "density matrix size differs from the number of boxes");
+    const int64_t ng = (int64_t)bs->gdim[0] * bs->gdim[1] * bs->gdim[2];
+    double* cs       = (double*)workspace(ctx, 6, (bs->ncell + 8) * sizeof(double));
+    if (!cs) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(ng)), dim3(kBlock), 0, ctx->stream, rho, ng, 0.);
+    hipLaunchKernelGGL(k_cell_density, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, X, n,
+        (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_origin, bs->gdim[0], bs->gdim[1],
+        bs->gdim[2], (const int*)bs->d_cell_ptr, (const int*)bs->d_cell_ent, rho, cs, bs->ncell);
+    const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+    hipLaunchKernelGGL(k_sum_scaled, dim3(1), dim3(256), 0, ctx->stream, (const double*)cs, bs->ncell, vel,
+        cs + bs->ncell);
+    MGX_CHECK_LAUNCH(ctx);
+    double tc = 0.;
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(&tc, cs + bs->ncell, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (charge) *charge = tc;
+    if (nel > 0. && tc > 0.)
+    {
+        const double t1 = nel / tc;
+        if (std::fabs(t1 - 1.) > 0.02)
+            return set_err(ctx, MGX_ENUMERIC, "density charge differs from the electron count by more than 2%");
+        hipLaunchKernelGGL(k_scale_d, dim3(nblocks(ng)), dim3(kBlock), 0, ctx->stream, rho, ng, t1);
+        MGX_CHECK_LAUNCH(ctx);
+    }
+    return MGX_OK;
+}
+
 // Anderson m = 1 inner products, weighted by diag(S^-1): out[0] =
 // <f - f0, f - f0>, out[1] = <f - f0, f> (device), f = a*kr, f0 = a*kr0;
 // kr, kr0 of type prec (MGX_F32 / MGX_F64).  d: 2 n scratch.
@@ -2487,6 +2639,15 @@ int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, cons
     BOX_ARGS_OK(ctx, bs, 0);
     MGX_REQUIRE(ctx, psi && hpsi && theta && res, "null data");
     int rc = box_residual(ctx, bs, psi, hpsi, theta, res);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_density(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* dm, int n, double nel,
+    double* rho, double* charge)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, psi && dm && rho, "null data");
+    int rc = box_density(ctx, bs, psi, dm, n, nel, rho, charge);
     return rc ? rc : sync(ctx);
 }
 

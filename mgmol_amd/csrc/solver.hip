@@ -50,6 +50,14 @@ int box_psi_step(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double)
 int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, const void**);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
+int box_density(mgx_ctx*, const mgx_boxset*, const double*, const double*, int, double, double*, double*);
+
+// y = a x (n values)
+__global__ void k_scaled_copy(const double* x, int64_t n, double a, double* y)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        y[t] = a * x[t];
+}
 }  // namespace mgx
 
 struct mgx_solver
@@ -495,6 +503,27 @@ int mgx_solver_set_cnn(mgx_solver* s, mgx_cnn* cnn)
 {
     if (!s) return MGX_EINVAL;
     s->cnn = cnn;
+    return MGX_OK;
+}
+
+int mgx_solver_density(mgx_solver* s, double nel, double* rho, double* charge)
+{
+    if (!s) return MGX_EINVAL;
+    MGX_REQUIRE(s->ctx, rho, "null rho");
+    mgx_ctx* ctx = s->ctx;
+    int rc       = flush_norm(s);
+    if (rc) return rc;
+    // S of the current psi (FP64; H is a by-product here), its inverse, and
+    // dm = 2 S^-1 in theta's buffer
+    if ((rc = box_gram_h(ctx, s->bs, MGX_PREC_F64, s->psi, s->psi, s->S, s->H, s->Sv, s->Hv))) return rc;
+    if ((rc = chol_inv(ctx, s->n, s->S, s->L, s->Sinv))) return rc;
+    s->have_inv      = true;
+    const int64_t nn = (int64_t)s->n * s->n;
+    hipLaunchKernelGGL(k_scaled_copy, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, (const double*)s->Sinv, nn, 2.,
+        s->th);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = box_density(ctx, s->bs, s->psi, s->th, s->n, nel, rho, charge))) return rc;
+    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return MGX_OK;
 }
 

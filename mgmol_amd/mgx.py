@@ -31,9 +31,9 @@ EXPORTS = [
     "mgx_ritz", "mgx_residual", "mgx_residual_norm",
     "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
     "mgx_box_lap", "mgx_box_hloc", "mgx_box_jacobi", "mgx_box_restrict", "mgx_box_prolong", "mgx_box_precond_mg",
-    "mgx_box_gram_h", "mgx_box_residual", "mgx_box_residual_norm",
+    "mgx_box_gram_h", "mgx_box_residual", "mgx_box_residual_norm", "mgx_box_density",
     "mgx_solver_create", "mgx_solver_destroy", "mgx_solver_step", "mgx_solver_run", "mgx_solver_reset",
-    "mgx_solver_profile", "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_set_cnn",
+    "mgx_solver_profile", "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_set_cnn", "mgx_solver_density",
     "mgx_cnn_nparams", "mgx_cnn_create", "mgx_cnn_destroy", "mgx_cnn_set_params", "mgx_cnn_get_params",
     "mgx_cnn_forward", "mgx_cnn_grad", "mgx_cnn_train_step", "mgx_cnn_apply", "mgx_cnn_make_pairs",
 ]
@@ -114,11 +114,13 @@ def lib():
             "mgx_box_prolong": (P, P, I, I, P, P), "mgx_box_precond_mg": (P, P, I, I, D, I, P, P),
             "mgx_box_gram_h": (P, P, I, P, P, P, P), "mgx_box_residual": (P, P, P, P, P, P),
             "mgx_box_residual_norm": (P, P, P, P, C.POINTER(D)),
+            "mgx_box_density": (P, P, P, P, I, D, P, C.POINTER(D)),
             "mgx_solver_create": (P, P, C.POINTER(SolverCfg), P, P, C.POINTER(P)),
             "mgx_solver_destroy": (P,), "mgx_solver_step": (P, C.POINTER(SolverStats)),
             "mgx_solver_run": (P, D, I, C.POINTER(SolverStats)), "mgx_solver_get_psi": (P, P),
             "mgx_solver_reset": (P,), "mgx_solver_profile": (P, C.POINTER(D), C.POINTER(I64), I),
             "mgx_solver_buffers": (P, PP, PP, PP, PP, PP), "mgx_solver_set_cnn": (P, P),
+            "mgx_solver_density": (P, D, P, C.POINTER(D)),
             "mgx_cnn_create": (P, I, I, C.POINTER(P)), "mgx_cnn_destroy": (P,),
             "mgx_cnn_set_params": (P, P), "mgx_cnn_get_params": (P, P), "mgx_cnn_forward": (P, P, I, P),
             "mgx_cnn_grad": (P, P, P, I, C.POINTER(D), P), "mgx_cnn_train_step": (P, P, P, I, C.c_float, C.POINTER(D)),
@@ -271,6 +273,15 @@ class BoxSet:
         self.ctx.check(lib().mgx_boxset_npairs(self.h, C.byref(n)), "mgx_boxset_npairs")
         return n.value
 
+    def density(self, psi, dm, rho, nel=0.0):
+        """rho (device, gdim^3) = sum_ij dm_ij psi_i psi_j (mgx_box_density);
+        psi, dm, rho: DeviceArray or raw device pointers.  Returns the charge
+        vel * sum rho before any rescaling to nel electrons."""
+        q = C.c_double()
+        self.ctx.check(lib().mgx_box_density(self.ctx.h, self.h, ptr(psi), ptr(dm), self.nbox, nel, ptr(rho),
+                                             C.byref(q)), "mgx_box_density")
+        return q.value
+
     def close(self):
         if self.h:
             lib().mgx_boxset_destroy(self.h)
@@ -300,6 +311,13 @@ class Solver:
                                           p.ctypes.data_as(C.c_void_p), C.byref(self.h)), "mgx_solver_create")
         if cnn is not None:
             self.set_cnn(cnn)
+
+    def density(self, rho, nel=0.0):
+        """rho (device, gdim^3) of the current orbitals with dm = 2 S^-1
+        (mgx_solver_density); returns the charge before rescaling."""
+        q = C.c_double()
+        self.ctx.check(lib().mgx_solver_density(self.h, nel, ptr(rho), C.byref(q)), "mgx_solver_density")
+        return q.value
 
     def set_cnn(self, cnn):
         self.cnn = cnn  # keep it alive while attached

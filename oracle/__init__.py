@@ -450,6 +450,38 @@ def b_diag_dots(b, r):
     return d
 
 
+def b_density(b, psi, dm):
+    """rho(r) = sum_ij dm_ij psi_i(r) psi_j(r) on the periodic level-0 grid
+    (gx, gy, gz), following Rho::computeRhoSubdomainUsingBlas3
+    (src/Rho.cc:360-449): product = Phi dm (MPgemmNN, :395-397), then
+    lrho(r) += sum_j product(r, j) phi_j(r) (:426-433), here over the box
+    orbitals embedded in the full grid (zero outside their boxes, so the
+    reference's symmetric mask on dm, src/Rho.cc:303-307, changes nothing)."""
+    E = b_embed(b, psi)
+    n = E.shape[0]
+    flat = E.reshape(n, -1)
+    prod = np.asarray(dm, dtype=F64).T @ flat  # prod_j(r) = sum_i dm_ij phi_i(r)
+    return np.einsum("jr,jr->r", prod, flat).reshape(E.shape[1:])
+
+
+def charge(rho, vel):
+    """Rho::computeTotalCharge (src/Rho.cc:122-153): vel * sum rho."""
+    return float(np.sum(rho)) * vel
+
+
+def rescale_charge(rho, vel, nel):
+    """Rho::rescaleTotalCharge (src/Rho.cc:155-208): rho *= nel / charge, and
+    the reference aborts when that factor is off 1 by more than 2%
+    (ValueError here).  Returns (rho, charge before scaling)."""
+    q = charge(rho, vel)
+    if q > 0.0:
+        t1 = nel / q
+        if abs(t1 - 1.0) > 0.02:
+            raise ValueError("charge %g vs %g electrons" % (q, nel))
+        rho = rho * t1
+    return rho, q
+
+
 def b_embed(b, data, level=0):
     """Packed boxes -> (nbox, gx, gy, gz) periodic full grids (zero outside)."""
     gd = [b.gdim[i] >> level for i in range(3)]
