@@ -50,6 +50,7 @@ struct mgx_boxset
     int* d_pslot_ptr = nullptr;
     int64_t* d_pslot = nullptr;
     bool use_cells   = false;
+    bool pair_z      = false;  // every origin and edge even: paired z loads in the cell kernels
     // per box j: the other boxes i it interacts with (overlap or within 2
     // points), 4 ints (i, sh[3]) with sh mapping j-local -> i-local; and per
     // pair the entries (i in j's list, j in i's list) for the skin terms
@@ -982,6 +983,8 @@ __device__ __forceinline__ int64_t cell_idx3(const CellEnt& c, int x, int y, int
     return c.base + ((int64_t)(c.b0 + x) * c.e + (c.b1 + y)) * c.e + (c.b2 + z);
 }
 
+constexpr int RJB = 2;  // point tiles per load batch in k_cell_residual and k_cell_density
+
 // Per cell: G = A^T A and P = A^T B (A = psi, B = H psi, 512 x k) on MFMA.
 // Four waves split the 512 points (two x-planes each), then combine in LDS
 // in a fixed order; partials go to part[slot + a*k + b] (G) and
@@ -1051,11 +1054,86 @@ __device__ __forceinline__ void cell_gram_tile(const double* psi, const double* 
     }
 }
 
+// The same tile with each lane loading two adjacent z-points (16 bytes):
+// lanes r = 0..3 of a box cover one 8-point cell row, and the two values feed
+// two MFMA steps.  Valid when every box origin and edge is even, so the
+// covered z-range of a cell row starts and ends at even positions and the
+// pair is aligned.  The split keeps FP32 partial sums over 32 points.
+template <int PREC, bool SAME>
+__device__ __forceinline__ void cell_gram_tile2(const double* psi, const double* hpsi, const CellEnt& A,
+    const CellEnt& B, int wid, int r, double (&accG)[4], double (&accP)[4])
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 z2 = { 0., 0. };
+    auto ld2 = [](const double* p, int64_t i) -> d2 { return *reinterpret_cast<const d2*>(p + i); };
+    if (PREC == MGX_PREC_F64)
+    {
+        d4c g = { 0., 0., 0., 0. }, h = { 0., 0., 0., 0. };
+#pragma unroll 4
+        for (int s = 0; s < 16; s++)
+        {
+            const int x = 2 * wid + (s >> 3), y = s & 7, z = 2 * r;
+            const int64_t ia = cell_idx3(A, x, y, z);
+            const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
+            const d2 xa = ia >= 0 ? ld2(psi, ia) : z2;
+            const d2 xb = SAME ? xa : (ib >= 0 ? ld2(psi, ib) : z2);
+            const d2 hb = ib >= 0 ? ld2(hpsi, ib) : z2;
+            g = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[0], xb[0], g, 0, 0, 0);
+            h = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[0], hb[0], h, 0, 0, 0);
+            g = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[1], xb[1], g, 0, 0, 0);
+            h = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[1], hb[1], h, 0, 0, 0);
+        }
+        for (int v = 0; v < 4; v++)
+        {
+            accG[v] = g[v];
+            accP[v] = h[v];
+        }
+    }
+    else
+    {
+        for (int v = 0; v < 4; v++)
+            accG[v] = accP[v] = 0.;
+        for (int s0 = 0; s0 < 16; s0 += 4)
+        {
+            f4c g1 = { 0.f, 0.f, 0.f, 0.f }, g2 = g1, h1 = g1, h2 = g1;
+#pragma unroll 2
+            for (int s = s0; s < s0 + 4; s++)
+            {
+                const int x = 2 * wid + (s >> 3), y = s & 7, z = 2 * r;
+                const int64_t ia = cell_idx3(A, x, y, z);
+                const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
+                const d2 xa2 = ia >= 0 ? ld2(psi, ia) : z2;
+                const d2 xb2 = SAME ? xa2 : (ib >= 0 ? ld2(psi, ib) : z2);
+                const d2 hb2 = ib >= 0 ? ld2(hpsi, ib) : z2;
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+                {
+                    const double xa = xa2[q], xb = xb2[q], hb = hb2[q];
+                    const float ah = (float)xa, al = (float)(xa - (double)ah);
+                    const float bh = SAME ? ah : (float)xb, bl = SAME ? al : (float)(xb - (double)bh);
+                    const float hh = (float)hb, hl = (float)(hb - (double)hh);
+                    g1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bh, g1, 0, 0, 0);
+                    g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, bl, g2, 0, 0, 0);
+                    g2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, bh, g2, 0, 0, 0);
+                    h1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hh, h1, 0, 0, 0);
+                    h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ah, hl, h2, 0, 0, 0);
+                    h2 = __builtin_amdgcn_mfma_f32_16x16x4f32(al, hh, h2, 0, 0, 0);
+                }
+            }
+            for (int v = 0; v < 4; v++)
+            {
+                accG[v] += (double)g1[v] + (double)g2[v];
+                accP[v] += (double)h1[v] + (double)h2[v];
+            }
+        }
+    }
+}
+
 // S / H partials per 8^3 cell over all box pairs covering it (k <= 32):
 // G_ab = sum_p psi_a psi_b, P_ab = sum_p psi_a hpsi_b, as 16 x 16 box tiles
 // on MFMA with K = points (4 per step); wave w takes x-planes 2w, 2w+1, and
 // the four waves' sums are added in a fixed order.
-template <int PREC>
+template <int PREC, bool PAIR>
 __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
     const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part,
     int ncell)
@@ -1081,7 +1159,11 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
             const CellEnt A = a < k ? ce[a] : none;
             const CellEnt B = b < k ? ce[b] : none;
             double accG[4], accP[4];
-            if (ti == tj)
+            if (PAIR && ti == tj)
+                cell_gram_tile2<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
+            else if (PAIR)
+                cell_gram_tile2<PREC, false>(psi, hpsi, A, B, wid, r, accG, accP);
+            else if (ti == tj)
                 cell_gram_tile<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
             else
                 cell_gram_tile<PREC, false>(psi, hpsi, A, B, wid, r, accG, accP);
@@ -1273,6 +1355,53 @@ __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const 
     const int c = lane & 15, r = lane >> 4;
     const int nct = (k + 15) / 16;
     const int ksteps = (k + 3) / 4;
+    if (k <= 16)
+    {
+        // one box tile: the wave's 8 point tiles in batches of RJB, every
+        // load of a batch issued before its products (same sums, same order)
+        for (int jb = 0; jb < 8; jb += RJB)
+        {
+            double xa[RJB][4], hv[RJB][4];
+            int64_t ib[RJB][4];
+#pragma unroll
+            for (int u = 0; u < RJB; u++)
+            {
+                const int pp = (wid + 4 * (jb + u)) * 16 + c;
+#pragma unroll
+                for (int st = 0; st < 4; st++)
+                {
+                    const int aa = 4 * st + r;
+                    xa[u][st]    = 0.;
+                    if (aa < k)
+                    {
+                        const int64_t ia = cell_idx(ce[aa], pp);
+                        if (ia >= 0) xa[u][st] = psi[ia];
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                {
+                    const int bb = r + 4 * v;
+                    ib[u][v]     = bb < k ? cell_idx(ce[bb], pp) : -1;
+                    hv[u][v]     = ib[u][v] >= 0 ? hpsi[ib[u][v]] : 0.;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < RJB; u++)
+            {
+                d4c acc = { 0., 0., 0., 0. };
+#pragma unroll
+                for (int st = 0; st < 4; st++)
+                    if (st < ksteps)
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(th[(4 * st + r) * kCellK + c], xa[u][st], acc, 0,
+                            0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (ib[u][v] >= 0) res[ib[u][v]] = acc[v] + (-1. * hv[u][v]);
+            }
+        }
+        return;
+    }
     for (int job = wid; job < 32 * nct; job += 4)
     {
         const int p0 = (job & 31) * 16, b0 = (job >> 5) * 16;
@@ -1351,7 +1480,55 @@ __global__ void __launch_bounds__(256) k_cell_density(const double* psi, const d
     const int nct = (k + 15) / 16;
     const int ksteps = (k + 3) / 4;
     double wacc = 0.;
-    for (int job = wid; job < 32; job += 4)
+    // up to 16 boxes (one box tile): point tiles in batches of RJB with all
+    // loads of a batch issued first (same sums in the same order)
+    for (int jb = 0; k <= 16 && jb < 8; jb += RJB)
+    {
+        double xa[RJB][4];
+#pragma unroll
+        for (int u = 0; u < RJB; u++)
+        {
+            const int pp = (wid + 4 * (jb + u)) * 16 + c;
+#pragma unroll
+            for (int st = 0; st < 4; st++)
+            {
+                const int aa = 4 * st + r;
+                xa[u][st]    = 0.;
+                if (aa < k)
+                {
+                    const int64_t ia = cell_idx(ce[aa], pp);
+                    if (ia >= 0) xa[u][st] = psi[ia];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RJB; u++)
+        {
+            const int pp = (wid + 4 * (jb + u)) * 16 + c;
+            d4c acc      = { 0., 0., 0., 0. };
+#pragma unroll
+            for (int st = 0; st < 4; st++)
+                if (st < ksteps)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xs[(4 * st + r) * kCellK + c], xa[u][st], acc, 0, 0, 0);
+            // the outer factor psi_b(p), b = r + 4v, is this lane's xa[u][v]
+            double rs = 0.;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (r + 4 * v < k) rs += acc[v] * xa[u][v];
+            rs += __shfl_xor(rs, 16, 64);
+            rs += __shfl_xor(rs, 32, 64);
+            if (r == 0)
+            {
+                int gx = gc[0] + (pp >> 6), gy = gc[1] + ((pp >> 3) & 7), gz = gc[2] + (pp & 7);
+                if (gx >= g0) gx -= g0;
+                if (gy >= g1) gy -= g1;
+                if (gz >= g2) gz -= g2;
+                rho[((int64_t)gx * g1 + gy) * g2 + gz] = rs;
+                wacc += rs;
+            }
+        }
+    }
+    for (int job = wid; k > 16 && job < 32; job += 4)
     {
         const int p0 = job * 16, pp = p0 + c;
         double rs = 0.;
@@ -1932,14 +2109,15 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
     const int64_t nn = (int64_t)n * n;
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, S, nn, 0.);
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, H, nn, 0.);
-    if (prec == MGX_PREC_SPLIT)
-        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_SPLIT>, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi,
-            hpsi, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+    {
+        auto kern = prec == MGX_PREC_SPLIT ? (bs->pair_z ? k_cell_gram<MGX_PREC_SPLIT, true>
+                                                         : k_cell_gram<MGX_PREC_SPLIT, false>)
+                                           : (bs->pair_z ? k_cell_gram<MGX_PREC_F64, true>
+                                                         : k_cell_gram<MGX_PREC_F64, false>);
+        hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, hpsi,
+            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
             (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part, bs->ncell);
-    else
-        hipLaunchKernelGGL(k_cell_gram<MGX_PREC_F64>, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi,
-            hpsi, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part, bs->ncell);
+    }
     MGX_CHECK_LAUNCH(ctx);
     if (bs->npairs > 0)
     {
@@ -2504,6 +2682,9 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset
             if (!pslot.empty())
                 hipMemcpy(s->d_pslot, pslot.data(), sizeof(int64_t) * pslot.size(), hipMemcpyHostToDevice);
             s->use_cells = true;
+            s->pair_z    = true;
+            for (int j = 0; j < s->nbox; j++)
+                s->pair_z = s->pair_z && s->edge[j] % 2 == 0 && s->origin[3 * j + 2] % 2 == 0;
         }
     }
     if (hipDeviceSynchronize() != hipSuccess) return fail("box table upload");

@@ -16,6 +16,9 @@ c3: one z-slab of the dense 256^3 x 1024 problem (8 planes, K = 524,288 grid
     the FP64 result.
 c4: inference over 2048 boxes of 32^3 (29,376 flop per voxel) and Adam steps
     on 64-box batches (forward + backward + weight gradients).
+density: rho = sum_ij (2 S^-1)_ij psi_i psi_j on the c5 boxes (2048 x 32^3 on
+    a 256^3 grid), mgx_box_density; algorithmic bytes: psi read once (8 per
+    orbital point) and rho zeroed and written (16 per grid point).
 """
 import ctypes as C
 import json
@@ -142,11 +145,40 @@ def c4(ctx):
     return out
 
 
+def density(ctx):
+    kw = dict(synth.CONFIGS["c5"])
+    kw.pop("kind")
+    cfg = synth.box_config(seed=1, **kw)
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, cfg.levels)
+    n = cfg.nbox
+    psi = mgx.DeviceArray(ctx, cfg.psi)
+    S = mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64)
+    H = mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64)
+    ctx.check(mgx.lib().mgx_box_gram_h(ctx.h, bs.h, mgx.PREC_F64, mgx.ptr(psi), mgx.ptr(psi), mgx.ptr(S),
+                                       mgx.ptr(H)), "gram")
+    dm = mgx.DeviceArray(ctx, 2.0 * np.linalg.inv(S.get()))
+    ng = int(np.prod(cfg.gdim))
+    rho = mgx.DeviceArray(ctx, shape=(ng,), dtype=np.float64, zero=False)
+    q = [0.0]
+
+    def run():
+        q[0] = bs.density(psi, dm, rho)
+
+    t = _timeit(run, 10)
+    nbytes = 8 * cfg.psi.size + 16 * ng
+    out = {"config": "density (c5 boxes)", "ms": round(1e3 * t, 3), "charge": q[0], "electrons_expected": 2 * n,
+           "algorithmic_MB": round(nbytes / 1e6, 1), "GBps": round(nbytes / t / 1e9, 1)}
+    for a in (psi, S, H, dm, rho):
+        a.free()
+    bs.close()
+    return out
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     ctx = mgx.Ctx(0)
-    for name in argv or ["c2", "c3", "c4"]:
-        res = {"c2": c2, "c3": c3, "c4": c4}[name](ctx)
+    for name in argv or ["c2", "c3", "c4", "density"]:
+        res = {"c2": c2, "c3": c3, "c4": c4, "density": density}[name](ctx)
         print(json.dumps(res), flush=True)
     ctx.close()
 
