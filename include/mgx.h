@@ -128,6 +128,29 @@ int mgx_mask(mgx_ctx* ctx, mgx_block_t* u, const uint8_t* mask);
 int mgx_mg(mgx_ctx* ctx, int lap_type, int max_levels, mgx_block_t* v, const mgx_block_t* f,
     const uint8_t* const* masks);
 
+/* Hartree potential by multigrid: pb::Mgm (src/pb/Mgm.h:12-111) with pb::Vcycle
+ * (src/pb/Vcycle.h:28-249), as pb::SolverLap::solve runs them
+ * (src/pb/SolverLap.cc:49-63) for Hartree::solve (src/Hartree.cc:27-100; the
+ * caller passes rho already multiplied by 4 pi).  rhs = A.rhs(rho) (the
+ * Mehrstellen B operator for lap_type 0, rho itself otherwise); up to max_sweeps
+ * times: stop when |rhs - A vh| / |rhs| < tol, else one V-cycle on the residual
+ * (nu1 / nu2 Jacobi sweeps, coarse levels while every dimension is even and at
+ * least twice the ghost width, at most max_nlevels coarsenings, 4th order coarse
+ * levels in 2nd order) added to vh; then vh -= mean(vh) (GridFunc::average0) and
+ * its ghosts are refreshed.  rho, vh: one FP64 function each, same periodic
+ * grid (bc = 1); vh is the initial guess and the result.  The reference's
+ * defaults are nu1 = nu2 = 2, max_sweeps = 10, tol = 1e-16, max_nlevels = 10. */
+typedef struct
+{
+    int converged;
+    int nb_sweeps;
+    double final_residual;          /* pb::norm of rhs - A vh */
+    double final_relative_residual; /* final_residual / |rhs| */
+    double residual_reduction;      /* final_residual / initial residual */
+} mgx_poisson_stats_t;
+int mgx_poisson_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, mgx_block_t* vh,
+    int nu1, int nu2, int max_sweeps, double tol, int max_nlevels, mgx_poisson_stats_t* stats);
+
 /* ------------------------------------ Hamiltonian / preconditioner (packed) */
 /* Hamiltonian<T>::applyLocal(ncolors, phi, hphi), src/Hamiltonian.cc:86-159
  * (4th order: FDoper::del2_4th_withPot, src/pb/FDoper.cc:322-399; Mehrstellen:

@@ -700,6 +700,246 @@ int precond_run(mgx_ctx* ctx, int type, int max_levels, double gamma, const mgx_
     return MGX_OK;
 }
 
+// ------------------------------------------------- Hartree multigrid ----
+// vel * sum of squares (or the plain sum) over the interior of one function:
+// per-block partials, then one block adds them in a fixed order
+template <bool SQ>
+__global__ void __launch_bounds__(256) k_int_sum(const double* u, GGeom g, double* part)
+{
+    __shared__ double lds[4];
+    double acc = 0.;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        int r       = (int)t;
+        const int x = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y  = r / g.d2;
+        const int z  = r - y * g.d2;
+        const double v = u[(int64_t)(x + g.ng) * g.incx + (int64_t)(y + g.ng) * g.incy + z + g.ng];
+        acc += SQ ? v * v : v;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        acc += __shfl_xor(acc, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+__global__ void __launch_bounds__(256) k_part_sum(const double* part, int n, double* out)
+{
+    __shared__ double lds[4];
+    double acc = 0.;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        acc += part[i];
+    for (int o = 32; o > 0; o >>= 1)
+        acc += __shfl_xor(acc, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+// c = a - b (GridFunc::diff, src/pb/GridFunc.cc:557-572), all sizeg points
+__global__ void k_diff(const double* a, const double* b, double* c, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        c[t] = a[t] - b[t];
+}
+
+__global__ void k_add_const(double* u, int64_t n, const double* s, double scale)
+{
+    const double v = -scale * s[0];
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        u[t] += v;
+}
+
+struct PoissonMG
+{
+    std::vector<mgx_grid_t> grids;
+    std::vector<int> types;
+    std::vector<double*> res, x, f;
+    double* part = nullptr;  // kPartBlocks partials + 8 results
+};
+
+constexpr int kPartBlocks = 1024;
+
+// one sum over the interior to dev_out (device); SQ: of squares
+template <bool SQ>
+int int_sum(mgx_ctx* ctx, PoissonMG& P, const double* u, const mgx_grid_t& gr, double* dev_out)
+{
+    GGeom g = ggeom(gr);
+    const int nb = (int)std::min<int64_t>(kPartBlocks, (g.size + 255) / 256);
+    hipLaunchKernelGGL(k_int_sum<SQ>, dim3(nb), dim3(256), 0, ctx->stream, u, g, P.part);
+    hipLaunchKernelGGL(k_part_sum, dim3(1), dim3(256), 0, ctx->stream, (const double*)P.part, nb, dev_out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+// pb::norm: sqrt(vel * sum u^2) over the interior (host result)
+int int_norm(mgx_ctx* ctx, PoissonMG& P, const double* u, const mgx_grid_t& gr, double* out)
+{
+    int rc = int_sum<true>(ctx, P, u, gr, P.part + kPartBlocks);
+    if (rc) return rc;
+    double v = 0.;
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(&v, P.part + kPartBlocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = std::sqrt(v * gr.h[0] * gr.h[1] * gr.h[2]);
+    return MGX_OK;
+}
+
+// pb::Vcycle, src/pb/Vcycle.h:28-249 (periodic, one task, USE_LOWER_ORDER)
+int poisson_vcycle(mgx_ctx* ctx, PoissonMG& P, int l, double* x, const double* f, int nu1, int nu2)
+{
+    const mgx_grid_t& g = P.grids[l];
+    const int t         = P.types[l];
+    const double jf     = jacobi_factor(t, g.h);
+    double* res         = P.res[l];
+    const int64_t n     = ggeom(g).sizeg;
+    int rc;
+    for (int i = 0; i < nu1; i++)
+        if ((rc = launch_jacobi(ctx, t, jf, f, x, res, 1, g))) return rc;
+    if (l + 1 < (int)P.grids.size())
+    {
+        const mgx_grid_t& gc = P.grids[l + 1];
+        // fresh residual A x - rhs, restricted
+        if ((rc = launch_halo(ctx, x, 1, g))) return rc;
+        if ((rc = launch_lap(ctx, t, (const double*)x, res, 1, g))) return rc;
+        if ((rc = launch_axpy(ctx, n, -1., f, res))) return rc;
+        if ((rc = launch_halo(ctx, res, 1, g))) return rc;
+        if ((rc = launch_restrict(ctx, (const double*)res, g, P.f[l + 1], gc, 1))) return rc;
+        // coarse correction from zero, prolonged and subtracted
+        if ((rc = launch_fill(ctx, P.x[l + 1], ggeom(gc).sizeg, 0.))) return rc;
+        if ((rc = poisson_vcycle(ctx, P, l + 1, P.x[l + 1], P.f[l + 1], nu1, nu2))) return rc;
+        if ((rc = launch_halo(ctx, P.x[l + 1], 1, gc))) return rc;
+        if ((rc = launch_extend(ctx, (const double*)P.x[l + 1], gc, res, g, 1))) return rc;
+        if ((rc = launch_axpy(ctx, n, -1., (const double*)res, x))) return rc;
+    }
+    for (int i = 0; i < nu2; i++)
+        if ((rc = launch_jacobi(ctx, t, jf, f, x, res, 1, g))) return rc;
+    return MGX_OK;
+}
+
+// pb::Mgm, src/pb/Mgm.h:12-111, as SolverLap::solve calls it
+// (src/pb/SolverLap.cc:49-63): periodic grids only
+int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho, double* vh, int nu1, int nu2,
+    int max_sweeps, double tol, int max_nlevels, mgx_poisson_stats_t* st)
+{
+    PoissonMG P;
+    P.grids.push_back(g0);
+    P.types.push_back(type);
+    // Vcycle coarsens level l while l < max_nlevels and every dimension is
+    // even and at least twice the ghost width; coarse grids keep one ghost
+    // and the lower-order operator (Laph4 -> Laph2)
+    while ((int)P.grids.size() - 1 < max_nlevels)
+    {
+        const mgx_grid_t& p = P.grids.back();
+        bool ok = true;
+        for (int i = 0; i < 3; i++)
+            ok = ok && p.dim[i] % 2 == 0 && p.dim[i] >= 2 * p.ghosts;
+        if (!ok) break;
+        mgx_grid_t c = coarse_of(p);
+        c.ghosts     = 1;
+        P.grids.push_back(c);
+        P.types.push_back(type == MGX_LAP_4TH ? MGX_LAP_2ND : type);
+    }
+    const int nl = (int)P.grids.size();
+    const int64_t n0 = ggeom(g0).sizeg;
+    size_t total = 4 * (size_t)n0 + kPartBlocks + 8;  // rhs, lhs, r0 (then res0), w
+    for (int l = 0; l < nl; l++)
+        total += (size_t)ggeom(P.grids[l]).sizeg * (l == 0 ? 1 : 3);
+    double* base = (double*)workspace(ctx, 0, total * sizeof(double));
+    if (!base) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    double* rhs = base;
+    double* lhs = rhs + n0;
+    double* r0  = lhs + n0;
+    double* w   = r0 + n0;
+    double* q   = w + n0;
+    P.part      = q;
+    q += kPartBlocks + 8;
+    for (int l = 0; l < nl; l++)
+    {
+        const int64_t m = ggeom(P.grids[l]).sizeg;
+        P.res.push_back(q);
+        q += m;
+        if (l == 0)
+        {
+            P.x.push_back(nullptr);
+            P.f.push_back(nullptr);
+            continue;
+        }
+        P.x.push_back(q);
+        q += m;
+        P.f.push_back(q);
+        q += m;
+    }
+    int rc;
+    // rhs = A.rhs(rho): B rho (Mehrstellen), rho otherwise
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(r0, rho, n0 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = launch_halo(ctx, r0, 1, g0))) return rc;
+    if (type == MGX_LAP_MEHRSTELLEN)
+    {
+        GGeom a = ggeom(g0);
+        hipLaunchKernelGGL(k_rhs<double>, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)r0, rhs,
+            a, a, a.size);
+        MGX_CHECK_LAUNCH(ctx);
+    }
+    else
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(rhs, r0, n0 * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    double rhs_norm = 0.;
+    if ((rc = int_norm(ctx, P, rhs, g0, &rhs_norm))) return rc;
+    MGX_REQUIRE(ctx, rhs_norm > 0., "zero right-hand side");
+    const double inv_rhs_norm = 1. / rhs_norm;
+    double init = 1., fin = 1., rel = 1.;
+    int conv = 0, sweeps = 0;
+    for (int i = 0; i < max_sweeps; i++)
+    {
+        if ((rc = launch_halo(ctx, vh, 1, g0))) return rc;
+        if ((rc = launch_lap(ctx, type, (const double*)vh, lhs, 1, g0))) return rc;
+        hipLaunchKernelGGL(k_diff, dim3(nblocks(n0)), dim3(kBlock), 0, ctx->stream, (const double*)rhs,
+            (const double*)lhs, r0, n0);
+        MGX_CHECK_LAUNCH(ctx);
+        double rn = 0.;
+        if ((rc = int_norm(ctx, P, r0, g0, &rn))) return rc;
+        if (i == 0) init = rn;
+        if (rn * inv_rhs_norm < tol)
+        {
+            fin  = rn;
+            rel  = rn * inv_rhs_norm;
+            conv = 1;
+            break;
+        }
+        if ((rc = launch_fill(ctx, w, n0, 0.))) return rc;
+        if ((rc = poisson_vcycle(ctx, P, 0, w, r0, nu1, nu2))) return rc;
+        sweeps++;
+        if ((rc = launch_axpy(ctx, n0, 1., (const double*)w, vh))) return rc;
+    }
+    if (!conv)
+    {
+        if ((rc = launch_halo(ctx, vh, 1, g0))) return rc;
+        if ((rc = launch_lap(ctx, type, (const double*)vh, lhs, 1, g0))) return rc;
+        if ((rc = launch_axpy(ctx, n0, -1., (const double*)rhs, lhs))) return rc;
+        if ((rc = int_norm(ctx, P, lhs, g0, &fin))) return rc;
+        rel = fin * inv_rhs_norm;
+    }
+    // GridFunc::average0 for the fully periodic grid, then fresh ghosts
+    GGeom a = ggeom(g0);
+    if ((rc = int_sum<false>(ctx, P, vh, g0, P.part + kPartBlocks))) return rc;
+    hipLaunchKernelGGL(k_add_const, dim3(nblocks(n0)), dim3(kBlock), 0, ctx->stream, vh, n0,
+        (const double*)(P.part + kPartBlocks), 1. / (double)a.size);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = launch_halo(ctx, vh, 1, g0))) return rc;
+    if (st)
+    {
+        st->converged               = conv;
+        st->nb_sweeps               = sweeps;
+        st->final_residual          = fin;
+        st->final_relative_residual = rel;
+        st->residual_reduction      = fin / init;
+    }
+    return MGX_OK;
+}
+
 }  // namespace
 
 int halo(mgx_ctx* ctx, int dtype, void* data, int nfunc, const mgx_grid_t& g)
@@ -844,6 +1084,24 @@ int mgx_mg(mgx_ctx* ctx, int type, int max_levels, mgx_block_t* v, const mgx_blo
              ? mg_run(ctx, type, max_levels, (float*)v->data, (const float*)f->data, v->nfunc, v->grid, masks, 1)
              : mg_run(ctx, type, max_levels, (double*)v->data, (const double*)f->data, v->nfunc, v->grid, masks,
                    1);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_poisson_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, mgx_block_t* vh, int nu1, int nu2,
+    int max_sweeps, double tol, int max_nlevels, mgx_poisson_stats_t* stats)
+{
+    int rc = check_block(ctx, rho);
+    if (!rc) rc = check_block(ctx, vh);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, same_shape(rho, vh), "rho/vh blocks differ in shape");
+    MGX_REQUIRE(ctx, rho->dtype == MGX_F64 && rho->nfunc == 1, "one FP64 function");
+    MGX_REQUIRE(ctx, lap_type >= 0 && lap_type <= 2, "LapFactory::createLap() --- option invalid");
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, vh->grid.bc[i] == 1, "the multigrid Poisson solver takes periodic grids");
+    MGX_REQUIRE(ctx, vh->grid.ghosts >= (lap_type == MGX_LAP_4TH ? 2 : 1), "too few ghosts for the operator");
+    MGX_REQUIRE(ctx, nu1 >= 0 && nu2 >= 0 && max_sweeps >= 0 && max_nlevels >= 0, "bad cycle parameters");
+    rc = poisson_run(ctx, lap_type, vh->grid, (const double*)rho->data, (double*)vh->data, nu1, nu2, max_sweeps, tol,
+        max_nlevels, stats);
     return rc ? rc : sync(ctx);
 }
 

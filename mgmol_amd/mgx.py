@@ -26,7 +26,7 @@ EXPORTS = [
     "mgx_malloc", "mgx_free", "mgx_memcpy_htod", "mgx_memcpy_dtoh",
     "mgx_halo", "mgx_lap_apply", "mgx_rhs_apply", "mgx_pointwise_product", "mgx_axpy", "mgx_jacobi",
     "mgx_restrict", "mgx_prolong", "mgx_mask", "mgx_mg",
-    "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma",
+    "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg",
     "mgx_local_gram", "mgx_local_product", "mgx_local_to_global", "mgx_chol_inv", "mgx_theta", "mgx_eigsum",
     "mgx_ritz", "mgx_residual", "mgx_residual_norm",
     "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
@@ -75,6 +75,11 @@ class SolverStats(C.Structure):
                 ("learned_steps", C.c_int), ("fallbacks", C.c_int)]
 
 
+class PoissonStats(C.Structure):
+    _fields_ = [("converged", C.c_int), ("nb_sweeps", C.c_int), ("final_residual", C.c_double),
+                ("final_relative_residual", C.c_double), ("residual_reduction", C.c_double)]
+
+
 class MgxError(RuntimeError):
     pass
 
@@ -97,6 +102,7 @@ def lib():
             "mgx_pointwise_product": (P, BP, P, BP), "mgx_axpy": (P, I, I64, D, P, P),
             "mgx_jacobi": (P, I, D, BP, BP, BP), "mgx_restrict": (P, BP, BP), "mgx_prolong": (P, BP, BP),
             "mgx_mask": (P, BP, P), "mgx_mg": (P, I, I, BP, BP, PP),
+            "mgx_poisson_mg": (P, I, BP, BP, I, I, I, D, I, C.POINTER(PoissonStats)),
             "mgx_hloc_apply": (P, I, GP, I, P, I64, I, P, P, I64),
             "mgx_precond_mg": (P, I, I, D, I, GP, P, I64, I, PP, P, I64),
             "mgx_local_gram": (P, I, P, I64, I64, I, I, D, P),

@@ -291,6 +291,86 @@ def mg(g, lap_type, max_levels, v, f, bc=(1, 1, 1), masks=None):
     return v
 
 
+def gnorm(g, u):
+    """pb::norm (GridFunc::norm2, src/pb/GridFunc.cc:2856-2861): sqrt(vel *
+    sum of squares over the interior), one function."""
+    v = interior(g, u)
+    return float(np.sqrt(np.sum(v.astype(F64) ** 2) * g.h[0] * g.h[1] * g.h[2]))
+
+
+def _lower_order(lap_type):
+    """The coarse-level operator of pb::Vcycle with USE_LOWER_ORDER
+    (src/pb/Vcycle.h:14, 178-203): Laph4M stays Mehrstellen
+    (src/pb/Laph4M.h:57), Laph4 drops to Laph2 (src/pb/Laph4.h:75-83), Laph2
+    stays Laph2."""
+    return 1 if lap_type == 2 else lap_type
+
+
+def _vcycle(g, lap_type, x, rhs, level, cogr, nu1, nu2):
+    """pb::Vcycle (src/pb/Vcycle.h:28-249), periodic, one process: nu1
+    Jacobi sweeps; on a coarsenable level the fresh residual A x - rhs is
+    restricted, the coarse problem solved from zero by the same cycle, its
+    prolongation subtracted; nu2 Jacobi sweeps."""
+    res = np.zeros_like(x)
+    jf = jacobi_factor(lap_type, list(g.h))
+    for _ in range(nu1):
+        jacobi(g, lap_type, rhs, x, res, jf)
+    coarsen = all(g.dim[i] % 2 == 0 and g.dim[i] >= 2 * g.ghosts for i in range(3))
+    if level < cogr and coarsen:
+        trade(g, x)
+        res = lap(g, lap_type, x)
+        res -= rhs
+        trade(g, res)
+        gc = Grid.make([d // 2 for d in g.dim], 1, [2 * v for v in g.h])
+        rc = restrict(g, res, gc)
+        uc = empty_block(gc, 1, x.dtype)
+        _vcycle(gc, _lower_order(lap_type), uc, rc, level + 1, cogr, nu1, nu2)
+        trade(gc, uc)
+        p = extend(gc, uc, g)
+        x -= p
+    for _ in range(nu2):
+        jacobi(g, lap_type, rhs, x, res, jf)
+
+
+def poisson_mgm(g, lap_type, rho, vh, nu1=2, nu2=2, max_sweeps=10, tol=1e-16, max_nlevels=10):
+    """pb::Mgm (src/pb/Mgm.h:12-111) as SolverLap::solve calls it
+    (src/pb/SolverLap.cc:49-63), periodic: rhs = A.rhs(rho) (the Mehrstellen
+    B operator for Laph4M, src/pb/Laph4M.h:68-71; a copy otherwise,
+    src/pb/FDoper.h:68-73); up to max_sweeps times: res = rhs - A vh, stop when
+    |res| / |rhs| < tol, else vh += Vcycle(0, res); then vh -= mean(vh)
+    (GridFunc::average0).  rho, vh: ghosted (1, ...) blocks; vh is updated.
+    Returns (converged, final_residual, final_relative_residual,
+    residual_reduction, nb_sweeps)."""
+    r = rho.copy()
+    trade(g, r)
+    rhs = rhs_mehr(g, r) if lap_type == 0 else r.copy()
+    inv_rhs_norm = 1.0 / gnorm(g, rhs)
+    converged, sweeps = False, 0
+    init = final = rel = 1.0
+    for i in range(max_sweeps):
+        trade(g, vh)
+        lhs = lap(g, lap_type, vh)
+        res = rhs - lhs
+        rn = gnorm(g, res)
+        if i == 0:
+            init = rn
+        if rn * inv_rhs_norm < tol:
+            final, rel, converged = rn, rn * inv_rhs_norm, True
+            break
+        w = np.zeros_like(vh)
+        _vcycle(g, lap_type, w, res, 0, max_nlevels, nu1, nu2)
+        sweeps += 1
+        vh += w
+    if not converged:
+        trade(g, vh)
+        lhs = lap(g, lap_type, vh)
+        lhs -= rhs
+        final = gnorm(g, lhs)
+        rel = final * inv_rhs_norm
+    vh -= np.mean(interior(g, vh))
+    return converged, final, rel, final / init, sweeps
+
+
 def precond(g, lap_type, max_levels, gamma_, res, dtype=F32, bc=(1, 1, 1), masks=None):
     """res: (nfunc, numpt) FP64 packed -> preconditioned (nfunc, numpt) FP64."""
     res = np.ascontiguousarray(res, dtype=F64)
