@@ -231,9 +231,11 @@ int mgx_box_prolong(mgx_ctx* ctx, const mgx_boxset* bs, int level, int dtype, co
 int mgx_box_precond_mg(mgx_ctx* ctx, const mgx_boxset* bs, int lap_type, int max_levels,
     double gamma, int prec_dtype, const double* res, double* out);
 /* S_ij = vel*<psi_i, psi_j>; H_ij = vel*<psi_i, H psi_j> (Galerkin: hpsi on the
- * overlap plus the stencil's reach 2 points outside each box, averaged over
- * both sides so H is symmetric); dense n x n FP64, zero for pairs that do not
- * interact.  vtot is not needed: outside a box psi is zero. */
+ * overlap plus the stencil's reach 2 points outside each box; with the cell
+ * tables the value for i < j is written to both sides, without them the two
+ * one-sided values are averaged: either way H is exactly symmetric); dense
+ * n x n FP64, zero for pairs that do not interact.  vtot is not needed:
+ * outside a box psi is zero. */
 int mgx_box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi,
     const double* hpsi, double* S, double* H);
 /* res_j = sum_k psi_k theta_kj - hpsi_j on box j */

@@ -1239,6 +1239,8 @@ __global__ void __launch_bounds__(256) k_slab_skin(const double* psi, const int6
     {
         const int* E = ilist + 4 * qq;
         const int i = E[0], ei = edge[i];
+        // only <psi_i, H psi_j> with i < j is used (k_pair_reduce)
+        if (i >= j) continue;
         const int sh[3] = { E[1], E[2], E[3] };
         const int xd1 = Xd1 + sh[d], xd2 = Xd2 + sh[d];
         const bool in1 = xd1 >= 0 && xd1 < ei, in2 = xd2 >= 0 && xd2 < ei;
@@ -1295,26 +1297,25 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x)
     {
-        double g = 0., h1 = 0., h2 = 0.;
+        double g = 0., h1 = 0.;
         for (int q = pslot_ptr[t]; q < pslot_ptr[t + 1]; q++)
         {
-            const int64_t ab = pslot[2 * q], ba = pslot[2 * q + 1];
+            const int64_t ab = pslot[2 * q];
             g += part[ab];
             h1 += part[nslot + ab];
-            h2 += part[nslot + ba];
         }
         const int i = pairs[11 * t], j = pairs[11 * t + 1];
         if (i != j)
         {
-            // h1 = <psi_i, H psi_j>: skin of j seen by i (entry of i in j's list)
-            const int q1 = pair_ient[2 * t], q2 = pair_ient[2 * t + 1];
+            // h1 = <psi_i, H psi_j> (i < j): the in-box part plus the skin of
+            // j seen by i (entry of i in j's list).  The stencil is symmetric,
+            // so <psi_j, H psi_i> is the same number up to rounding; H takes
+            // h1 on both sides and is exactly symmetric.
+            const int q1 = pair_ient[2 * t];
             for (int sl = 0; sl < 12; sl++)
-            {
                 h1 += skin[12 * (int64_t)q1 + sl];
-                h2 += skin[12 * (int64_t)q2 + sl];
-            }
         }
-        const double sv = g * vel, hv = 0.5 * (h1 + h2) * vel;
+        const double sv = g * vel, hv = h1 * vel;
         S[(int64_t)j * n + i] = sv;
         S[(int64_t)i * n + j] = sv;
         H[(int64_t)j * n + i] = hv;

@@ -16,6 +16,9 @@ c3: one z-slab of the dense 256^3 x 1024 problem (8 planes, K = 524,288 grid
     the FP64 result.
 c4: inference over 2048 boxes of 32^3 (29,376 flop per voxel) and Adam steps
     on 64-box batches (forward + backward + weight gradients).
+poisson: the Hartree multigrid (mgx_poisson_mg, Mehrstellen, nu1 = nu2 = 2) on
+    the 256^3 grid of c5 (h = 0.2 bohr) from zero to |r| / |rhs| < 1e-10, for a
+    seeded zero-mean 4 pi rho.
 density: rho = sum_ij (2 S^-1)_ij psi_i psi_j on the c5 boxes (2048 x 32^3 on
     a 256^3 grid), mgx_box_density; algorithmic bytes: psi read once (8 per
     orbital point) and rho zeroed and written (16 per grid point).
@@ -174,11 +177,45 @@ def density(ctx):
     return out
 
 
+def poisson(ctx):
+    n, h = 256, 0.2
+    g = mgx.Grid.make((n, n, n), 1, [h] * 3)
+    rng = np.random.default_rng(9)
+    rho = np.zeros((n + 2,) * 3)
+    inner = rng.standard_normal((n, n, n))
+    rho[1:-1, 1:-1, 1:-1] = 4 * np.pi * (inner - inner.mean())
+    trho = mgx.DeviceArray(ctx, rho)
+    tvh = mgx.DeviceArray(ctx, shape=rho.shape, dtype=np.float64)
+    st = mgx.PoissonStats()
+
+    zero = np.zeros_like(rho)
+
+    def run():
+        ctx.check(mgx.lib().mgx_memcpy_htod(ctx.h, mgx.ptr(tvh), zero.ctypes.data_as(C.c_void_p), tvh.nbytes),
+                  "mgx_memcpy_htod")
+        ctx.check(mgx.lib().mgx_poisson_mg(ctx.h, 0, mgx.block(trho, g, 1), mgx.block(tvh, g, 1), 2, 2, 40, 1e-10, 10,
+                                           C.byref(st)), "mgx_poisson_mg")
+
+    run()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        run()
+    t = (time.perf_counter() - t0) / reps
+    out = {"config": "poisson 256^3 Mehrstellen", "ms_per_solve_incl_zeroing": round(1e3 * t, 2),
+           "v_cycles": st.nb_sweeps, "converged": bool(st.converged),
+           "final_relative_residual": st.final_relative_residual,
+           "ms_per_v_cycle": round(1e3 * t / max(st.nb_sweeps, 1), 3)}
+    trho.free()
+    tvh.free()
+    return out
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     ctx = mgx.Ctx(0)
-    for name in argv or ["c2", "c3", "c4", "density"]:
-        res = {"c2": c2, "c3": c3, "c4": c4, "density": density}[name](ctx)
+    for name in argv or ["c2", "c3", "c4", "density", "poisson"]:
+        res = {"c2": c2, "c3": c3, "c4": c4, "density": density, "poisson": poisson}[name](ctx)
         print(json.dumps(res), flush=True)
     ctx.close()
 
