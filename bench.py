@@ -15,10 +15,12 @@ value = seconds per converged solve of the mixed+learned path (lower is
 better); plain and mixed times, iteration counts and band-energy differences
 are reported beside it.
 
-Roofline: the dominant kernel of the headline path, timed with hipEvents on
-the engine's stream inside the timed solves: the CNN inference (BF16 MFMA)
-for the learned path, the fine-level Jacobi sweep (HBM) otherwise; the other
-is reported as roofline_other.
+Roofline: the dominant hand-written kernel of the headline path, timed with
+hipEvents on the engine's stream inside the timed solves: the S / H cell
+partials (k_cell_gram, HBM-bound: psi and H psi read once).  roofline_other
+lists the fine-level FP32 Jacobi sweep (HBM) and the CNN inference (BF16
+MFMA, learned steps only); stream is the achievable HBM rate of a plain
+axpy on this device.
 CPU baseline: the oracle (C++/OpenMP restatement of the reference) timed on
 this host for one full-size orbital step, times the iteration count.
 """
@@ -42,7 +44,7 @@ WEIGHTS = os.path.join(ROOT, "mgmol_amd", "weights", "cnn_e32.npz")
 # HBM bytes per launch at C5, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes
 # (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000, both calibrated on
 # streaming kernels of known size; profiles/pmc_traffic_c5.md)
-TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895255953}
+TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895234355, "cell_gram_split": 1103204147}
 
 PATHS = {
     "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64),
@@ -119,6 +121,23 @@ def jacobi_roofline(r, s_bytes, npts):
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_box_march<JACOBI> fine-level sweep (%s)" % ("FP32" if s_bytes == 4 else "FP64"),
+                avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches), bytes_per_launch=int(per_launch))
+
+
+def cellgram_roofline(r, npts, split):
+    """k_cell_gram, the S / H cell partials: psi and H psi (FP64) read once,
+    16 bytes per orbital point per launch (the partials written are < 2%)."""
+    ms, launches = r["profile"]["cellgram"]
+    if not launches:
+        return None
+    per_launch = 16.0 * npts
+    avg_s = (ms / 1e3) / launches
+    achieved = per_launch / avg_s / 1e9
+    key = "cell_gram_split" if split else "cell_gram_f64"
+    traffic = TRAFFIC_C5.get(key) if npts == 2048 * 32 ** 3 else None
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel="k_cell_gram<%s> S/H partials per 8^3 cell (MFMA)" % ("SPLIT" if split else "F64"),
                 avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches), bytes_per_launch=int(per_launch))
 
 
@@ -199,16 +218,19 @@ def main():
     head = res[headname]
     npts = cfg.psi.size
 
+    # the headline path's dominant hand-written kernel: the S / H cell
+    # partials (one launch per step); beside it the fine FP32 Jacobi sweep
+    # (three per step) and the network (learned steps only)
     vc_name = "mixed" if "mixed" in res else headname
-    roof_vc = jacobi_roofline(res[vc_name], 4 if PATHS[vc_name]["precond_dtype"] == mgx.F32 else 8, npts)
-    roof_cnn = cnn_roofline(res["learned"], npts) if "learned" in res else None
-    if headname == "learned" and roof_cnn:
-        roof, other = roof_cnn, roof_vc
-    else:
-        roof, other = roof_vc, roof_cnn
+    roof = cellgram_roofline(head, npts, PATHS[headname]["gram_prec"] == mgx.PREC_SPLIT)
+    other = [x for x in (jacobi_roofline(res[vc_name], 4 if PATHS[vc_name]["precond_dtype"] == mgx.F32 else 8,
+                                         npts),
+                         cnn_roofline(res["learned"], npts) if "learned" in res else None) if x]
+    if roof is None and other:
+        roof, other = other[0], other[1:]
 
     stream = stream_reference(ctx)
-    for r in (roof, other):
+    for r in [roof] + other:
         if r and r["unit"] == "GB/s":
             r["frac_of_stream"] = round(r["achieved"] / stream["GBps"], 4)
 

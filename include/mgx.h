@@ -332,7 +332,8 @@ typedef struct
 #define MGX_PROF_PRECOND 5   /* whole V-cycle incl. update */
 #define MGX_PROF_NORMALIZE 6 /* orbital normalisation */
 #define MGX_PROF_LEARNED 7   /* CNN inference of the learned preconditioner */
-#define MGX_PROF_N 8
+#define MGX_PROF_CELLGRAM 8  /* the S / H cell-partials kernel alone (one launch per step) */
+#define MGX_PROF_N 9
 
 typedef struct
 {

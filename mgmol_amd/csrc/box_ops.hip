@@ -984,6 +984,7 @@ __device__ __forceinline__ int64_t cell_idx3(const CellEnt& c, int x, int y, int
 }
 
 constexpr int RJB = 2;  // point tiles per load batch in k_cell_residual and k_cell_density
+constexpr int kGramWaves = 4;  // waves per cell in k_cell_gram with paired loads (8 measured slower)
 
 // Per cell: G = A^T A and P = A^T B (A = psi, B = H psi, 512 x k) on MFMA.
 // Four waves split the 512 points (two x-planes each), then combine in LDS
@@ -1059,10 +1060,12 @@ __device__ __forceinline__ void cell_gram_tile(const double* psi, const double* 
 // two MFMA steps.  Valid when every box origin and edge is even, so the
 // covered z-range of a cell row starts and ends at even positions and the
 // pair is aligned.  The split keeps FP32 partial sums over 32 points.
-template <int PREC, bool SAME>
+template <int PREC, bool SAME, int NW>
 __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double* hpsi, const CellEnt& A,
     const CellEnt& B, int wid, int r, double (&accG)[4], double (&accP)[4])
 {
+    // NW waves share the cell's 64 rows of 8 points: NS rows each
+    constexpr int NS = 64 / NW;
     typedef double d2 __attribute__((ext_vector_type(2)));
     const d2 z2 = { 0., 0. };
     auto ld2 = [](const double* p, int64_t i) -> d2 { return *reinterpret_cast<const d2*>(p + i); };
@@ -1070,9 +1073,10 @@ __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double*
     {
         d4c g = { 0., 0., 0., 0. }, h = { 0., 0., 0., 0. };
 #pragma unroll 4
-        for (int s = 0; s < 16; s++)
+        for (int s = 0; s < NS; s++)
         {
-            const int x = 2 * wid + (s >> 3), y = s & 7, z = 2 * r;
+            const int row = wid * NS + s;
+            const int x = row >> 3, y = row & 7, z = 2 * r;
             const int64_t ia = cell_idx3(A, x, y, z);
             const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
             const d2 xa = ia >= 0 ? ld2(psi, ia) : z2;
@@ -1093,13 +1097,14 @@ __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double*
     {
         for (int v = 0; v < 4; v++)
             accG[v] = accP[v] = 0.;
-        for (int s0 = 0; s0 < 16; s0 += 4)
+        for (int s0 = 0; s0 < NS; s0 += 4)
         {
             f4c g1 = { 0.f, 0.f, 0.f, 0.f }, g2 = g1, h1 = g1, h2 = g1;
 #pragma unroll 2
             for (int s = s0; s < s0 + 4; s++)
             {
-                const int x = 2 * wid + (s >> 3), y = s & 7, z = 2 * r;
+                const int row = wid * NS + s;
+                const int x = row >> 3, y = row & 7, z = 2 * r;
                 const int64_t ia = cell_idx3(A, x, y, z);
                 const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
                 const d2 xa2 = ia >= 0 ? ld2(psi, ia) : z2;
@@ -1133,13 +1138,14 @@ __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double*
 // G_ab = sum_p psi_a psi_b, P_ab = sum_p psi_a hpsi_b, as 16 x 16 box tiles
 // on MFMA with K = points (4 per step); wave w takes x-planes 2w, 2w+1, and
 // the four waves' sums are added in a fixed order.
-template <int PREC, bool PAIR>
-__global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
+template <int PREC, bool PAIR, int NW>
+__global__ void __launch_bounds__(64 * NW) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
     const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part,
     int ncell)
 {
+    static_assert(PAIR || NW == 4, "the unpaired tiles take four waves");
     __shared__ CellEnt ce[kCellK];
-    __shared__ double red[4][64][4];
+    __shared__ double red[NW][64][4];
     // consecutive cells (z-neighbours share cache lines) on one XCD
     const int64_t cl = xcd_block(blockIdx.x, gridDim.x);
     if (cl >= ncell) return;
@@ -1160,9 +1166,9 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
             const CellEnt B = b < k ? ce[b] : none;
             double accG[4], accP[4];
             if (PAIR && ti == tj)
-                cell_gram_tile2<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
+                cell_gram_tile2<PREC, true, NW>(psi, hpsi, A, B, wid, r, accG, accP);
             else if (PAIR)
-                cell_gram_tile2<PREC, false>(psi, hpsi, A, B, wid, r, accG, accP);
+                cell_gram_tile2<PREC, false, NW>(psi, hpsi, A, B, wid, r, accG, accP);
             else if (ti == tj)
                 cell_gram_tile<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
             else
@@ -1177,7 +1183,9 @@ __global__ void __launch_bounds__(256) k_cell_gram(const double* psi, const doub
                 {
                     for (int v = 0; v < 4; v++)
                     {
-                        const double sum = red[0][lane][v] + red[1][lane][v] + red[2][lane][v] + red[3][lane][v];
+                        double sum = red[0][lane][v];
+                        for (int w = 1; w < NW; w++)
+                            sum += red[w][lane][v];
                         const int row = ti * 16 + ((PREC == MGX_PREC_F64) ? r + 4 * v : 4 * r + v);
                         const int col = tj * 16 + c;
                         if (row < k && col < k) part[pass * nslot + slot + row * k + col] = sum;
@@ -2111,13 +2119,17 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, S, nn, 0.);
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, H, nn, 0.);
     {
-        auto kern = prec == MGX_PREC_SPLIT ? (bs->pair_z ? k_cell_gram<MGX_PREC_SPLIT, true>
-                                                         : k_cell_gram<MGX_PREC_SPLIT, false>)
-                                           : (bs->pair_z ? k_cell_gram<MGX_PREC_F64, true>
-                                                         : k_cell_gram<MGX_PREC_F64, false>);
-        hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, hpsi,
+        // paired loads: kGramWaves waves per cell, else four
+        auto kern = prec == MGX_PREC_SPLIT ? (bs->pair_z ? k_cell_gram<MGX_PREC_SPLIT, true, kGramWaves>
+                                                         : k_cell_gram<MGX_PREC_SPLIT, false, 4>)
+                                           : (bs->pair_z ? k_cell_gram<MGX_PREC_F64, true, kGramWaves>
+                                                         : k_cell_gram<MGX_PREC_F64, false, 4>);
+        const int nthr = 64 * (bs->pair_z ? kGramWaves : 4);
+        const int tk   = prof_begin(ctx, MGX_PROF_CELLGRAM);
+        hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(nthr), 0, ctx->stream, psi, hpsi,
             (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
             (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part, bs->ncell);
+        prof_end(ctx, tk);
     }
     MGX_CHECK_LAUNCH(ctx);
     if (bs->npairs > 0)

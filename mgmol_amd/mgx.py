@@ -16,7 +16,7 @@ PREC_F64, PREC_SPLIT = 0, 1
 START_GAMMA, START_ZERO = 0, 1
 CONV_DELTAE, CONV_RESIDUAL = 0, 1
 PRECOND_VCYCLE, PRECOND_LEARNED = 0, 1
-PROF_GROUPS = ["hloc", "gram", "dense", "resid", "jacobi0", "precond", "normalize", "learned"]
+PROF_GROUPS = ["hloc", "gram", "dense", "resid", "jacobi0", "precond", "normalize", "learned", "cellgram"]
 
 _LIB = None
 
