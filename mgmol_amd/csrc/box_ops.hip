@@ -1707,31 +1707,55 @@ __global__ void __launch_bounds__(256) k_diag_wsum2(const double* d, const doubl
     }
 }
 
-// x_bar = (1-th) x + th x0, f_bar = (1-th) f + th f0 (f = a*kr, f0 = a*kr0);
-// then x0 <- x, kr0 <- kr, x <- x_bar + beta f_bar
+// Anderson m = 1 update (AndersonMix::update): x_bar = (1-th) x + th x0,
+// f_bar = (1-th) f + th f0 with f = a*kr, f0 = a*kr0; kr0 <- kr and the new
+// x = x_bar + beta f_bar written to xo (the buffer holding x0, which the
+// caller then swaps with x: the old x becomes x0 without a copy), with the
+// normalisation sums of the new x, d_j = vel |x_j|^2, in k_box_sumsq's
+// order: one block per box
 template <typename T>
-__global__ void k_and_update(double* x, double* x0, const T* kr, double a, T* kr0, int64_t n, double th,
-    double beta)
+__global__ void __launch_bounds__(256) k_box_and_update_sq(const double* x, double* xo, const T* kr, double a,
+    T* kr0, const int64_t* off, int nbox, double th, double beta, double vel, double* d)
 {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    __shared__ double lds[4];
+    const int j = blockIdx.x;
+    if (j >= nbox) return;
+    double acc[1] = { 0. };
+    for (int64_t q = off[j] + threadIdx.x; q < off[j + 1]; q += blockDim.x)
     {
-        const T k         = kr[t];
-        const double xv   = x[t], x0v = x0[t], fv = a * (double)k, f0v = a * (double)kr0[t];
-        const double xb   = (1. - th) * xv + th * x0v;
-        const double fb   = (1. - th) * fv + th * f0v;
-        x0[t]             = xv;
-        kr0[t]            = k;
-        x[t]              = xb + beta * fb;
+        const T k       = kr[q];
+        const double xv = x[q], x0v = xo[q], fv = a * (double)k, f0v = a * (double)kr0[q];
+        const double xb = (1. - th) * xv + th * x0v;
+        const double fb = (1. - th) * fv + th * f0v;
+        kr0[q]          = k;
+        const double nv = xb + beta * fb;
+        xo[q]           = nv;
+        acc[0] += nv * nv;
     }
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) d[j] = acc[0] * vel;
 }
 
-// psi += a * kr (ABPG::update_states without extrapolation)
+// psi += a * kr with the normalisation sums of the new psi (k_box_sumsq's order)
 template <typename T>
-__global__ void k_psi_step(double* psi, const T* kr, int64_t n, double a)
+__global__ void __launch_bounds__(256) k_box_psi_step_sq(double* psi, const T* kr, double a, const int64_t* off,
+    int nbox, double vel, double* d)
 {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-        psi[t] += a * (double)kr[t];
+    __shared__ double lds[4];
+    const int j = blockIdx.x;
+    if (j >= nbox) return;
+    double acc[1] = { 0. };
+    for (int64_t q = off[j] + threadIdx.x; q < off[j + 1]; q += blockDim.x)
+    {
+        double v = psi[q];
+        v += a * (double)kr[q];
+        psi[q] = v;
+        acc[0] += v * v;
+    }
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) d[j] = acc[0] * vel;
 }
+
 
 // 0.5*sqrt(sum_j d_j Sinv_jj): single block
 __global__ void __launch_bounds__(256) k_norm(const double* d, const double* Sinv, int n, double* out)
@@ -2260,29 +2284,32 @@ int box_anderson_dots(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const void* 
     return MGX_OK;
 }
 
-int box_anderson_update(mgx_ctx* ctx, const mgx_boxset* bs, int prec, double* x, double* x0, const void* kr,
-    double a, void* kr0, double th, double beta)
+// Anderson update into xo (see k_box_and_update_sq) + the norms d of the new x
+int box_anderson_update_sq(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* x, double* xo, const void* kr,
+    double a, void* kr0, double th, double beta, double* d)
 {
-    const int64_t n = bs->off[0][bs->nbox];
+    const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+    const int64_t* off = (const int64_t*)bs->d_off[0];
     if (prec == MGX_F32)
-        hipLaunchKernelGGL(k_and_update<float>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, x, x0,
-            (const float*)kr, a, (float*)kr0, n, th, beta);
+        hipLaunchKernelGGL(k_box_and_update_sq<float>, dim3(bs->nbox), dim3(256), 0, ctx->stream, x, xo,
+            (const float*)kr, a, (float*)kr0, off, bs->nbox, th, beta, vel, d);
     else
-        hipLaunchKernelGGL(k_and_update<double>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, x, x0,
-            (const double*)kr, a, (double*)kr0, n, th, beta);
+        hipLaunchKernelGGL(k_box_and_update_sq<double>, dim3(bs->nbox), dim3(256), 0, ctx->stream, x, xo,
+            (const double*)kr, a, (double*)kr0, off, bs->nbox, th, beta, vel, d);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
 
-int box_psi_step(mgx_ctx* ctx, const mgx_boxset* bs, int prec, double* psi, const void* kr, double a)
+int box_psi_step_sq(mgx_ctx* ctx, const mgx_boxset* bs, int prec, double* psi, const void* kr, double a, double* d)
 {
-    const int64_t n = bs->off[0][bs->nbox];
+    const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+    const int64_t* off = (const int64_t*)bs->d_off[0];
     if (prec == MGX_F32)
-        hipLaunchKernelGGL(k_psi_step<float>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, psi,
-            (const float*)kr, n, a);
+        hipLaunchKernelGGL(k_box_psi_step_sq<float>, dim3(bs->nbox), dim3(256), 0, ctx->stream, psi, (const float*)kr,
+            a, off, bs->nbox, vel, d);
     else
-        hipLaunchKernelGGL(k_psi_step<double>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, psi,
-            (const double*)kr, n, a);
+        hipLaunchKernelGGL(k_box_psi_step_sq<double>, dim3(bs->nbox), dim3(256), 0, ctx->stream, psi,
+            (const double*)kr, a, off, bs->nbox, vel, d);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }

@@ -44,9 +44,9 @@ int inverse_update(mgx_ctx*, int, const double*, double*, double*, double*, int*
 int ritz(mgx_ctx*, int, const double*, const double*, double*, double*);
 int box_anderson_dots(mgx_ctx*, const mgx_boxset*, int, const void*, double, const void*, const double*, double*,
     double*);
-int box_anderson_update(mgx_ctx*, const mgx_boxset*, int, double*, double*, const void*, double, void*, double,
-    double);
-int box_psi_step(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double);
+int box_anderson_update_sq(mgx_ctx*, const mgx_boxset*, int, const double*, double*, const void*, double, void*,
+    double, double, double*);
+int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double, double*);
 int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, const void**);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
@@ -242,6 +242,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     st->learned_steps  = s->learned_steps;
     st->fallbacks      = s->fallbacks;
     const double alpha = 0.5 * s->cfg.step_length;
+    bool norms_done    = false;  // d already holds the norms of the updated psi
     tk = prof_begin(ctx, MGX_PROF_PRECOND);
     const int prec = s->cfg.precond_dtype;
     if (!use_cnn && !s->accelerate)
@@ -277,7 +278,9 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             return rc;
         if (!s->accelerate)
         {
-            if ((rc = box_psi_step(ctx, s->bs, prec, s->psi, kr, alpha))) return rc;
+            // psi += alpha K(R), with the norms of the new psi
+            if ((rc = box_psi_step_sq(ctx, s->bs, prec, s->psi, kr, alpha, s->d))) return rc;
+            norms_done = true;
         }
         else
         {
@@ -300,14 +303,20 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
                     th = -3.;
                 beta = s->cfg.anderson_beta;
             }
-            if ((rc = box_anderson_update(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0, th, beta))) return rc;
+            // the new psi goes to the history buffer, and the buffers swap:
+            // the old psi is the next step's x0; the norms of the new psi too
+            if ((rc = box_anderson_update_sq(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0, th, beta, s->d)))
+                return rc;
+            std::swap(s->psi, s->x0);
+            norms_done = true;
             s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
         }
     }
     prof_end(ctx, tk);
-    // LocGridOrbitals::normalize: the norms now, the scaling with the next Hpsi
+    // LocGridOrbitals::normalize: the norms now (fused into the update when
+    // it was not the V-cycle's fused step), the scaling with the next Hpsi
     tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
-    if ((rc = box_sumsq(ctx, s->bs, s->psi, s->d))) return rc;
+    if (!norms_done && (rc = box_sumsq(ctx, s->bs, s->psi, s->d))) return rc;
     s->pending_norm = true;
     prof_end(ctx, tk);
     s->iter++;
