@@ -91,7 +91,8 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
     s.close()
     return dict(seconds=float(np.mean(times)), iterations=stats.iterations, residual=stats.residual,
                 energy=stats.band_energy, converged=bool(stats.converged), learned_steps=stats.learned_steps,
-                fallbacks=stats.fallbacks, profile=prof)
+                fallbacks=stats.fallbacks, newton_schulz_steps=stats.ns_steps, cholesky_calls=stats.chol_calls,
+                profile=prof)
 
 
 def cpu_baseline(cfg, iters, mg_levels):

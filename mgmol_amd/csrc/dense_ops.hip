@@ -645,9 +645,11 @@ int inverse_update(mgx_ctx* ctx, int n, const double* S, double* L, double* X, d
 
 // inverse_update for a pair-sparse S (values Sv per pair, column tiles pt):
 // T = X S on the sparse product, X <- 2X - T X as one dense GEMM (written to
-// the other half of the work buffer, operands swapped, no copies).
+// the other half of the work buffer, operands swapped, no copies).  Stops
+// before a step once delta = max|I - X S| < tol, and after a step once
+// delta^2 (the error that step leaves) < tol.
 int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv, const PairTiles& pt, double* L,
-    double* X, double* work, int* steps)
+    double* X, double* work, int* steps, double tol)
 {
     const double mone = -1., two = 2.;
     const size_t nn = (size_t)n * n;
@@ -675,7 +677,7 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
             *steps = -1;
             return chol_inv(ctx, n, S, L, X);
         }
-        if (delta < 1e-15) break;
+        if (delta < tol) break;
         // nxt = 2 cur - T cur
         if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
                 rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n, nxt,
@@ -685,7 +687,7 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
         std::swap(cur, nxt);
         (*steps)++;
         prev = delta;
-        if (delta < 3e-8) break;  // the error after this step is ~delta^2
+        if (delta * delta < tol) break;  // the error after this step is ~delta^2
     }
     if (cur != X) MGX_CHECK_HIP(ctx, hipMemcpyAsync(X, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, X, n);

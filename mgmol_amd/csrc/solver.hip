@@ -26,7 +26,7 @@ int box_gram_h(mgx_ctx*, const mgx_boxset*, int, const double*, const double*, d
 PairTiles boxset_tiles(const mgx_boxset*);
 int64_t boxset_npairs(const mgx_boxset*);
 int inverse_update_pairs(mgx_ctx*, int, const double*, const double*, const PairTiles&, double*, double*, double*,
-    int*);
+    int*, double);
 int box_residual(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, double*);
 int box_sumsq(mgx_ctx*, const mgx_boxset*, const double*, double*);
 int box_norm(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
@@ -149,7 +149,11 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     else
     {
         int steps = 0;
-        if ((rc = inverse_update_pairs(ctx, n, s->S, s->Sv, s->tiles, s->L, s->Sinv, s->nswork, &steps))) return rc;
+        // accuracy of the refreshed inverse: FP64 (1e-15) with FP64 S; with the
+        // split S (entries good to ~1e-8) 1e-10 suffices, one step fewer
+        const double tol = s->cfg.gram_prec == MGX_PREC_SPLIT ? 1e-10 : 1e-15;
+        if ((rc = inverse_update_pairs(ctx, n, s->S, s->Sv, s->tiles, s->L, s->Sinv, s->nswork, &steps, tol)))
+            return rc;
         if (steps < 0)
             s->chol_calls++;
         else
@@ -241,6 +245,8 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     st->lowest_eig     = s->lowest;
     st->learned_steps  = s->learned_steps;
     st->fallbacks      = s->fallbacks;
+    st->ns_steps       = (int)s->ns_steps;
+    st->chol_calls     = (int)s->chol_calls;
     const double alpha = 0.5 * s->cfg.step_length;
     bool norms_done    = false;  // d already holds the norms of the updated psi
     tk = prof_begin(ctx, MGX_PROF_PRECOND);
@@ -480,6 +486,8 @@ int mgx_solver_reset(mgx_solver* s)
     s->backoff_len   = MGX_LEARNED_BACKOFF;
     s->learned_steps = 0;
     s->fallbacks     = 0;
+    s->ns_steps      = 0;
+    s->chol_calls    = 0;
     return MGX_OK;
 }
 
