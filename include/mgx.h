@@ -199,6 +199,24 @@ int mgx_ritz(mgx_ctx* ctx, int n, const double* H, const double* S, double* eigv
  * res = psi*theta - hpsi (packed, ld, numpt rows, n functions) */
 int mgx_residual(mgx_ctx* ctx, const double* psi, const double* hpsi, const double* theta,
     int64_t ld, int64_t numpt, int n, double* res);
+/* Mehrstellen discretisation (FDtype = Mehrstellen) on packed orbitals (ld, numpt =
+ * prod(grid->dim) rows, n functions, FP64; halo from grid->bc):
+ * B-matrix: LocGridOrbitals::computeMatB, src/LocGridOrbitals.cc:1428-1486:
+ *   B = vel * Psi^T (B Psi), B the Mehrstellen right-hand-side operator
+ *   (FDkernelRHS_4th_Mehr1, src/pb/FDkernels.cc:523-584); n x n col-major */
+int mgx_mehr_bmatrix(mgx_ctx* ctx, const mgx_grid_t* grid, const double* psi, int64_t ld, int n,
+    double* B);
+/* ProjectedMatricesMehrstellen::computeInvB / updateTheta / updateHB,
+ * src/ProjectedMatricesMehrstellen.cc:38-64, .h:46-69, and getEigSum,
+ * src/ProjectedMatrices.cc:550-569: invB = B^-1 (Cholesky; MGX_ENOTSPD),
+ * theta = invB H, HB = (S theta + (S theta)^T) / 2, *eigsum (host, may be
+ * NULL) = tr(HB Sinv) [Ry] */
+int mgx_mehr_projected(mgx_ctx* ctx, int n, const double* S, const double* Sinv, const double* B,
+    const double* H, double* invB, double* theta, double* HB, double* eigsum);
+/* MGmol::computeResidualUsingHPhi with applyB, src/MGmol.cc:1227-1291:
+ * res = (B psi) theta - hpsi */
+int mgx_mehr_residual(mgx_ctx* ctx, const mgx_grid_t* grid, const double* psi, const double* hpsi,
+    const double* theta, int64_t ld, int n, double* res);
 /* MGmol::computeConstraintResidual with LocGridOrbitals::dotProductDiagonal,
  * src/MGmol.cc:1296-1332, src/LocGridOrbitals.cc:1606-1630:
  * 0.5*sqrt(sum_i Sinv_ii * vel*|r_i|^2) [Ha], host out */

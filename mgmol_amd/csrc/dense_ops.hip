@@ -363,6 +363,17 @@ __global__ void __launch_bounds__(256) k_spmm_tiles(const double* __restrict__ X
         }
 }
 
+// out = 0.5 (a + a^T)  (MatrixType::transpose(0.5, work, 0.5) after symm)
+__global__ void k_sym_half(const double* a, int n, double* out)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)n * n;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t / n), row = (int)(t - (int64_t)col * n);
+        out[t] = 0.5 * a[t] + 0.5 * a[(int64_t)row * n + col];
+    }
+}
+
 }  // namespace
 
 int spmm_pairs(mgx_ctx* ctx, int n, const double* X, const double* val, const PairTiles& pt, double* Y)
@@ -795,6 +806,37 @@ int mgx_ritz(mgx_ctx* ctx, int n, const double* H, const double* S, double* eig,
     MGX_REQUIRE(ctx, n > 0 && H && S && eig, "bad ritz arguments");
     int rc = ritz(ctx, n, H, S, eig, Z);
     return rc ? rc : sync(ctx);
+}
+
+int mgx_mehr_projected(mgx_ctx* ctx, int n, const double* S, const double* Sinv, const double* B, const double* H,
+    double* invB, double* theta, double* HB, double* eigsum)
+{
+    MGX_REQUIRE(ctx, n > 0 && S && Sinv && B && H && invB && theta && HB, "bad projected-matrix arguments");
+    const size_t nn = (size_t)n * n;
+    double* w       = (double*)workspace(ctx, 5, 2 * nn * sizeof(double) + 64);
+    if (!w) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    // invB (ProjectedMatricesMehrstellen::computeInvB: potrf + potri)
+    int rc = chol_inv(ctx, n, B, w, invB);
+    if (rc) return rc;
+    // theta = invB H (updateTheta)
+    if ((rc = mgx::theta(ctx, n, invB, H, theta))) return rc;
+    // HB = 0.5 (S theta + (S theta)^T) (updateHB)
+    const double one = 1., zero = 0.;
+    rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, S, n, theta, n, &zero,
+            w, n)
+        != rocblas_status_success)
+        return set_err(ctx, MGX_EHIP, "rocblas_dgemm failed");
+    hipLaunchKernelGGL(k_sym_half, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, (const double*)w, n, HB);
+    MGX_CHECK_LAUNCH(ctx);
+    // eigsum = tr(HB Sinv) (getEigSum)
+    if (eigsum)
+    {
+        double* dv = w + nn;
+        if ((rc = trace_prod(ctx, n, HB, Sinv, dv))) return rc;
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(eigsum, dv, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    return sync(ctx);
 }
 
 int mgx_residual(mgx_ctx* ctx, const double* psi, const double* hpsi, const double* th, int64_t ld, int64_t numpt,

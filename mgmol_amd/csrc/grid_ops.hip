@@ -439,6 +439,31 @@ __global__ void k_hlocM_packed(const T* psi, int64_t ld, const double* vtot, T* 
     }
 }
 
+// FDkernelRHS_4th_Mehr1 (src/pb/FDkernels.cc:523-584), i.e. Lap::rhs of the
+// Mehrstellen operator, on packed data with the halo by index wrapping:
+// out = (T)(0.5 psi + (1/12) (T)(sum of the 6 face neighbours))
+template <typename T>
+__global__ void k_rhsM_packed(const T* psi, int64_t ld, T* out, int64_t ldo, PGeom g, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.size;
+        const int64_t p = t - f * g.size;
+        int r           = (int)p;
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y = r / g.d2;
+        const int z = r - y * g.d2;
+        const T* a  = psi + f * ld;
+#define P(dx, dy, dz) pvalT(a, g, x, y, z, dx, dy, dz)
+        out[f * ldo + p] = (T)(0.5 * (double)P(0, 0, 0)
+                               + kInv12 * (double)(T)(P(-1, 0, 0) + P(1, 0, 0) + P(0, -1, 0) + P(0, 1, 0)
+                                                      + P(0, 0, -1) + P(0, 0, 1)));
+#undef P
+    }
+}
+
 // --------------------------------------------------------------- drivers --
 int check_block(mgx_ctx* ctx, const mgx_block_t* b)
 {
@@ -940,6 +965,25 @@ int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho,
     return MGX_OK;
 }
 
+// B psi on packed FP64 data (ld), numpt rows
+int rhsM_packed(mgx_ctx* ctx, const mgx_grid_t* grid, const double* psi, int64_t ld, int n, double* out)
+{
+    PGeom g;
+    g.d0   = grid->dim[0];
+    g.d1   = grid->dim[1];
+    g.d2   = grid->dim[2];
+    g.bc0  = grid->bc[0] == 1;
+    g.bc1  = grid->bc[1] == 1;
+    g.bc2  = grid->bc[2] == 1;
+    g.size = (int64_t)g.d0 * g.d1 * g.d2;
+    MGX_REQUIRE(ctx, ld >= g.size, "leading dimension smaller than numpt");
+    const int64_t total = (int64_t)n * g.size;
+    hipLaunchKernelGGL(k_rhsM_packed<double>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, psi, ld, out, ld,
+        g, total);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
 }  // namespace
 
 int halo(mgx_ctx* ctx, int dtype, void* data, int nfunc, const mgx_grid_t& g)
@@ -1147,6 +1191,32 @@ int mgx_hloc_apply(mgx_ctx* ctx, int type, const mgx_grid_t* grid, int dtype, co
         return set_err(ctx, MGX_EINVAL, "Hamiltonian::applyLocal supports lap types 0 and 2");
     MGX_CHECK_LAUNCH(ctx);
     return sync(ctx);
+}
+
+int mgx_mehr_bmatrix(mgx_ctx* ctx, const mgx_grid_t* grid, const double* psi, int64_t ld, int n, double* B)
+{
+    MGX_REQUIRE(ctx, grid && psi && B && n > 0, "bad B-matrix arguments");
+    const int64_t numpt = (int64_t)grid->dim[0] * grid->dim[1] * grid->dim[2];
+    MGX_REQUIRE(ctx, ld >= numpt, "leading dimension smaller than numpt");
+    double* bpsi = (double*)workspace(ctx, 0, (size_t)ld * n * sizeof(double));
+    if (!bpsi) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    int rc = rhsM_packed(ctx, grid, psi, ld, n, bpsi);
+    if (rc) return rc;
+    const double vel = grid->h[0] * grid->h[1] * grid->h[2];
+    return mgx_local_product(ctx, MGX_PREC_F64, psi, bpsi, ld, numpt, n, 1, vel, B);
+}
+
+int mgx_mehr_residual(mgx_ctx* ctx, const mgx_grid_t* grid, const double* psi, const double* hpsi,
+    const double* theta, int64_t ld, int n, double* res)
+{
+    MGX_REQUIRE(ctx, grid && psi && hpsi && theta && res && n > 0, "bad residual arguments");
+    const int64_t numpt = (int64_t)grid->dim[0] * grid->dim[1] * grid->dim[2];
+    MGX_REQUIRE(ctx, ld >= numpt, "leading dimension smaller than numpt");
+    double* bpsi = (double*)workspace(ctx, 0, (size_t)ld * n * sizeof(double));
+    if (!bpsi) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    int rc = rhsM_packed(ctx, grid, psi, ld, n, bpsi);
+    if (rc) return rc;
+    return mgx_residual(ctx, bpsi, hpsi, theta, ld, numpt, n, res);
 }
 
 int mgx_precond_mg(mgx_ctx* ctx, int type, int max_levels, double gamma, int prec_dtype, const mgx_grid_t* grid,

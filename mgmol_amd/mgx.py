@@ -28,7 +28,7 @@ EXPORTS = [
     "mgx_restrict", "mgx_prolong", "mgx_mask", "mgx_mg",
     "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg",
     "mgx_local_gram", "mgx_local_product", "mgx_local_to_global", "mgx_chol_inv", "mgx_theta", "mgx_eigsum",
-    "mgx_ritz", "mgx_residual", "mgx_residual_norm",
+    "mgx_ritz", "mgx_residual", "mgx_residual_norm", "mgx_mehr_bmatrix", "mgx_mehr_projected", "mgx_mehr_residual",
     "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
     "mgx_box_lap", "mgx_box_hloc", "mgx_box_jacobi", "mgx_box_restrict", "mgx_box_prolong", "mgx_box_precond_mg",
     "mgx_box_gram_h", "mgx_box_residual", "mgx_box_residual_norm", "mgx_box_density",
@@ -112,6 +112,9 @@ def lib():
             "mgx_eigsum": (P, I, P, P, C.POINTER(D)), "mgx_ritz": (P, I, P, P, P, P),
             "mgx_residual": (P, P, P, P, I64, I64, I, P),
             "mgx_residual_norm": (P, P, I64, I64, I, D, P, C.POINTER(D)),
+            "mgx_mehr_bmatrix": (P, GP, P, I64, I, P),
+            "mgx_mehr_projected": (P, I, P, P, P, P, P, P, P, C.POINTER(D)),
+            "mgx_mehr_residual": (P, GP, P, P, P, I64, I, P),
             "mgx_boxset_create": (P, C.POINTER(Boxes), I, C.POINTER(P)), "mgx_boxset_destroy": (P,),
             "mgx_boxset_size": (P, I, C.POINTER(I64)), "mgx_boxset_offset": (P, I, I, C.POINTER(I64)),
             "mgx_boxset_npairs": (P, C.POINTER(I64)),

@@ -454,6 +454,43 @@ def residual_norm(d, Sinv):
     return 0.5 * float(np.sqrt(np.sum(np.asarray(d) * np.diag(Sinv))))
 
 
+def mehr_bpsi(g, psi, bc=(1, 1, 1)):
+    """B psi for packed orbitals (n, numpt): Lap::rhs of the Mehrstellen
+    operator (FDkernelRHS_4th_Mehr1, src/pb/FDkernels.cc:523-584) after the
+    halo trade, as LocGridOrbitals::computeMatB and
+    MGmol::computeResidualUsingHPhi apply it (src/LocGridOrbitals.cc:1444-1462,
+    src/MGmol.cc:1250-1262)."""
+    psi = np.ascontiguousarray(psi, dtype=F64)
+    n = psi.shape[0]
+    blk = embed(g, psi.reshape((n,) + tuple(g.dim)))
+    trade(g, blk, bc)
+    return interior(g, rhs_mehr(g, blk)).reshape(n, -1)
+
+
+def mehr_bmatrix(g, psi, bc=(1, 1, 1)):
+    """LocGridOrbitals::computeMatB (src/LocGridOrbitals.cc:1428-1486):
+    B[i, j] = vel <psi_i, B psi_j>."""
+    vel = g.h[0] * g.h[1] * g.h[2]
+    return vel * (np.asarray(psi, dtype=F64) @ mehr_bpsi(g, psi, bc).T)
+
+
+def mehr_projected(S, Sinv, B, H):
+    """ProjectedMatricesMehrstellen::computeInvB / updateTheta / updateHB
+    (src/ProjectedMatricesMehrstellen.cc:38-64, .h:46-69) and getEigSum
+    (src/ProjectedMatrices.cc:550-569): (invB, theta, HB, eigsum)."""
+    _, invB = chol_inv(B)
+    th = invB @ H
+    st = np.asarray(S, dtype=F64) @ th
+    HB = 0.5 * st + 0.5 * st.T
+    return invB, th, HB, float(np.sum(HB * np.asarray(Sinv).T))
+
+
+def mehr_residual(g, psi, hpsi, th, bc=(1, 1, 1)):
+    """MGmol::computeResidualUsingHPhi with applyB (src/MGmol.cc:1227-1291):
+    res = (B psi) theta - hpsi."""
+    return residual(mehr_bpsi(g, psi, bc), hpsi, th)
+
+
 # ------------------------------------------------------------------ box family
 def b_call(name, b, *args):
     return getattr(lib(), name)(C.byref(b), *args)
