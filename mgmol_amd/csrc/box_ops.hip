@@ -49,6 +49,10 @@ struct mgx_boxset
     // per pair: the partial slots (ab, ba) of every cell both boxes cover
     int* d_pslot_ptr = nullptr;
     int64_t* d_pslot = nullptr;
+    // per box: its cell entries in cell order (sums of per-entry partials)
+    int* d_bent_ptr  = nullptr;
+    int* d_bent      = nullptr;
+    int nent         = 0;  // cell entries in all
     bool use_cells   = false;
     bool pair_z      = false;  // every origin and edge even: paired z loads in the cell kernels
     // per box j: the other boxes i it interacts with (overlap or within 2
@@ -1340,11 +1344,13 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
 // point p of the cell, as a 512 x k by k x k product on FP64 MFMA:
 // A = psi gathered per (point, box a), B = theta sub-block from LDS.
 __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
-    int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res, int ncell)
+    int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res, double* esq,
+    int ncell)
 {
     __shared__ CellEnt ce[kCellK];
     __shared__ double th[kCellK * kCellK];
     __shared__ int jid[kCellK];
+    __shared__ double wsq[4][kCellK];
     // consecutive cells (z-neighbours share cache lines) on one XCD
     const int64_t cl = xcd_block(blockIdx.x, gridDim.x);
     if (cl >= ncell) return;
@@ -1364,6 +1370,8 @@ __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const 
     const int c = lane & 15, r = lane >> 4;
     const int nct = (k + 15) / 16;
     const int ksteps = (k + 3) / 4;
+    // this lane's sums of R^2 for boxes 16 t + r + 4 v (esq: per cell entry)
+    double sq[2][4] = { { 0., 0., 0., 0. }, { 0., 0., 0., 0. } };
     if (k <= 16)
     {
         // one box tile: the wave's 8 point tiles in batches of RJB, every
@@ -1406,12 +1414,16 @@ __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const 
                             0, 0);
 #pragma unroll
                 for (int v = 0; v < 4; v++)
-                    if (ib[u][v] >= 0) res[ib[u][v]] = acc[v] + (-1. * hv[u][v]);
+                    if (ib[u][v] >= 0)
+                    {
+                        const double rv = acc[v] + (-1. * hv[u][v]);
+                        res[ib[u][v]]   = rv;
+                        sq[0][v] += rv * rv;
+                    }
             }
         }
-        return;
     }
-    for (int job = wid; job < 32 * nct; job += 4)
+    for (int job = wid; k > 16 && job < 32 * nct; job += 4)
     {
         const int p0 = (job & 31) * 16, b0 = (job >> 5) * 16;
         d4c acc = { 0., 0., 0., 0. };
@@ -1438,9 +1450,43 @@ __global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const 
             const int bb = b0 + r + 4 * v;
             if (bb >= k) continue;
             const int64_t ib = cell_idx(ce[bb], pp);
-            if (ib >= 0) res[ib] = acc[v] + (-1. * hpsi[ib]);
+            if (ib >= 0)
+            {
+                const double rv = acc[v] + (-1. * hpsi[ib]);
+                res[ib]         = rv;
+                sq[job >> 5][v] += rv * rv;
+            }
         }
     }
+    if (!esq) return;
+    // sums over the 16 points of a lane group, then over the waves in order
+    for (int t = 0; t < 2; t++)
+        for (int v = 0; v < 4; v++)
+        {
+            double x = sq[t][v];
+            for (int o = 1; o < 16; o <<= 1)
+                x += __shfl_xor(x, o, 64);
+            const int bb = 16 * t + r + 4 * v;
+            if (c == 0 && bb < kCellK) wsq[wid][bb] = x;
+        }
+    __syncthreads();
+    for (int b = threadIdx.x; b < k; b += blockDim.x)
+        esq[e0 + b] = ((wsq[0][b] + wsq[1][b]) + wsq[2][b]) + wsq[3][b];
+}
+
+// d_j = vel * sum of the box's cell-entry partials (entries listed per box
+// in cell order: a fixed summation order)
+__global__ void __launch_bounds__(64) k_box_from_entries(const double* esq, const int* bent_ptr, const int* bent,
+    int nbox, double vel, double* d)
+{
+    const int j = blockIdx.x;
+    if (j >= nbox) return;
+    double acc = 0.;
+    for (int q = bent_ptr[j] + threadIdx.x; q < bent_ptr[j + 1]; q += 64)
+        acc += esq[bent[q]];
+    for (int o = 32; o > 0; o >>= 1)
+        acc += __shfl_xor(acc, o, 64);
+    if (threadIdx.x == 0) d[j] = acc * vel;
 }
 
 // Electron density on the cells (Rho::computeRhoSubdomainUsingBlas3,
@@ -2209,14 +2255,30 @@ int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, 
     return MGX_OK;
 }
 
+int box_sumsq(mgx_ctx* ctx, const mgx_boxset* bs, const double* r, double* d);
+
+// R = psi theta - hpsi per box; with d: also d_j = vel |R_j|^2 (computed in
+// the cell kernel when the cell tables exist, else by k_box_sumsq)
 int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
-    double* res)
+    double* res, double* d)
 {
     if (bs->use_cells)
     {
+        double* esq = nullptr;
+        if (d)
+        {
+            esq = (double*)workspace(ctx, 3, (size_t)std::max(bs->nent, 1) * sizeof(double));
+            if (!esq) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+        }
         hipLaunchKernelGGL(k_cell_residual, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, hpsi, theta,
             bs->nbox, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-            (const int*)bs->d_cell_ent, res, bs->ncell);
+            (const int*)bs->d_cell_ent, res, esq, bs->ncell);
+        if (d)
+        {
+            const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+            hipLaunchKernelGGL(k_box_from_entries, dim3(bs->nbox), dim3(64), 0, ctx->stream, (const double*)esq,
+                (const int*)bs->d_bent_ptr, (const int*)bs->d_bent, bs->nbox, vel, d);
+        }
         MGX_CHECK_LAUNCH(ctx);
         return MGX_OK;
     }
@@ -2224,7 +2286,7 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
     hipLaunchKernelGGL(k_box_residual, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi, hpsi, theta,
         bs->nbox, (const int*)bs->d_nbr_ptr, (const int*)bs->d_nbr, v, res);
     MGX_CHECK_LAUNCH(ctx);
-    return MGX_OK;
+    return d ? box_sumsq(ctx, bs, res, d) : MGX_OK;
 }
 
 // rho on the level-0 grid (x slowest, z fastest) from the boxes' psi and
@@ -2721,6 +2783,24 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset
             hipMemcpy(s->d_pslot_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice);
             if (!pslot.empty())
                 hipMemcpy(s->d_pslot, pslot.data(), sizeof(int64_t) * pslot.size(), hipMemcpyHostToDevice);
+            {
+                std::vector<std::vector<int>> be(s->nbox);
+                const int nent = (int)(cent.size() / 10);
+                for (int q = 0; q < nent; q++)
+                    be[cent[10 * q]].push_back(q);
+                std::vector<int> bptr(s->nbox + 1, 0), bl;
+                for (int j = 0; j < s->nbox; j++)
+                {
+                    bptr[j + 1] = bptr[j] + (int)be[j].size();
+                    bl.insert(bl.end(), be[j].begin(), be[j].end());
+                }
+                if (hipMalloc(&s->d_bent_ptr, sizeof(int) * bptr.size()) != hipSuccess
+                    || hipMalloc(&s->d_bent, sizeof(int) * std::max<size_t>(bl.size(), 1)) != hipSuccess)
+                    return fail("hipMalloc box entry lists");
+                hipMemcpy(s->d_bent_ptr, bptr.data(), sizeof(int) * bptr.size(), hipMemcpyHostToDevice);
+                if (!bl.empty()) hipMemcpy(s->d_bent, bl.data(), sizeof(int) * bl.size(), hipMemcpyHostToDevice);
+                s->nent = nent;
+            }
             s->use_cells = true;
             s->pair_z    = true;
             for (int j = 0; j < s->nbox; j++)
@@ -2752,7 +2832,7 @@ int mgx_boxset_destroy(mgx_boxset* s)
     if (s->d_ilist_ptr) hipFree(s->d_ilist_ptr);
     if (s->d_ilist) hipFree(s->d_ilist);
     if (s->d_pair_ient) hipFree(s->d_pair_ient);
-    for (int* p : { s->d_sp_ptr, s->d_sp_col, s->d_sp_k, s->d_sp_e })
+    for (int* p : { s->d_sp_ptr, s->d_sp_col, s->d_sp_k, s->d_sp_e, s->d_bent_ptr, s->d_bent })
         if (p) hipFree(p);
     delete s;
     return MGX_OK;
@@ -2859,7 +2939,7 @@ int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, cons
 {
     BOX_ARGS_OK(ctx, bs, 0);
     MGX_REQUIRE(ctx, psi && hpsi && theta && res, "null data");
-    int rc = box_residual(ctx, bs, psi, hpsi, theta, res);
+    int rc = box_residual(ctx, bs, psi, hpsi, theta, res, nullptr);
     return rc ? rc : sync(ctx);
 }
 

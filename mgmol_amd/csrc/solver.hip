@@ -27,7 +27,7 @@ PairTiles boxset_tiles(const mgx_boxset*);
 int64_t boxset_npairs(const mgx_boxset*);
 int inverse_update_pairs(mgx_ctx*, int, const double*, const double*, const PairTiles&, double*, double*, double*,
     int*, double);
-int box_residual(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, double*);
+int box_residual(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, double*, double*);
 int box_sumsq(mgx_ctx*, const mgx_boxset*, const double*, double*);
 int box_norm(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
 int box_normalize(mgx_ctx*, const mgx_boxset*, double*, double*);
@@ -165,8 +165,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     if ((rc = diag_sum(ctx, n, s->th, s->scal))) return rc;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_RESID);
-    if ((rc = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res))) return rc;
-    if ((rc = box_sumsq(ctx, s->bs, s->res, s->d))) return rc;
+    if ((rc = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d))) return rc;
     if ((rc = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return rc;
     prof_end(ctx, tk);
     MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
