@@ -277,6 +277,19 @@ struct MarchArgs
     // psi is loaded (and written back to psi_w, the same array as in)
     const double* nrm = nullptr;
     double* psi_w     = nullptr;
+    // JACOBI only, last fine sweep of an Anderson step: per-box sums of
+    // (f - f0)^2 and (f - f0) f with f = da * x_out, f0 = da * dkr0 (type T),
+    // times dvel, to dout[j] and dout[nbox + j] (AndersonMix::update's dots)
+    const void* dkr0 = nullptr;
+    double da = 0., dvel = 0.;
+    double* dout = nullptr;
+};
+
+struct MarchDots
+{
+    const void* kr0;
+    double a, vel;
+    double* out;
 };
 
 // NB: number of LDS plane buffers.  With 2 the planes alternate and one
@@ -420,8 +433,8 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
 // z-neighbours read from LDS as whole V-wide pieces.  LDS rows have a ring of
 // ZR = max(V, 2) in z (so the pieces stay aligned) and 2 in y; the arithmetic
 // per point is that of k_box_march.
-template <typename T, int TYPE, int MODE, int NT, int P, int NB, int V>
-__global__ void __launch_bounds__(NT, 8) k_box_march4(const T* __restrict__ in, const T* __restrict__ f,
+template <typename T, int TYPE, int MODE, int NT, int P, int NB, int V, bool DOTS>
+__global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __restrict__ in, const T* __restrict__ f,
     const double* __restrict__ vtot, T* __restrict__ out, T* __restrict__ wout, MarchArgs a, Coef4 c4, Coef2 c2)
 {
     typedef T V4 __attribute__((ext_vector_type(V)));
@@ -468,6 +481,9 @@ __global__ void __launch_bounds__(NT, 8) k_box_march4(const T* __restrict__ in, 
         zero4[i] = (T)0;
     V4 qv[P][5];
     V4 nx[P];
+    constexpr bool dots = DOTS && MODE == MARCH_JACOBI;
+    const T* dkr0   = reinterpret_cast<const T*>(a.dkr0);
+    double dacc0 = 0., dacc1 = 0.;
 #pragma unroll
     for (int k = 0; k < P; k++)
     {
@@ -571,6 +587,17 @@ __global__ void __launch_bounds__(NT, 8) k_box_march4(const T* __restrict__ in, 
             }
             if (MODE == MARCH_JACOBI && wout) *reinterpret_cast<V4*>(wout + gi) = wv4;
             *reinterpret_cast<V4*>(out + gi) = ov;
+            if (dots)
+            {
+                const V4 k0 = *reinterpret_cast<const V4*>(dkr0 + gi);
+#pragma unroll
+                for (int i = 0; i < V; i++)
+                {
+                    const double fa = a.da * (double)ov[i], df = fa - a.da * (double)k0[i];
+                    dacc0 += df * df;
+                    dacc1 += df * fa;
+                }
+            }
         }
         if (NB == 1) __syncthreads();
 #pragma unroll
@@ -581,6 +608,33 @@ __global__ void __launch_bounds__(NT, 8) k_box_march4(const T* __restrict__ in, 
             qv[k][2] = qv[k][3];
             qv[k][3] = qv[k][4];
             qv[k][4] = nx[k];
+        }
+    }
+    if (dots)
+    {
+        __shared__ double dred[2][16];
+        for (int o = 32; o > 0; o >>= 1)
+        {
+            dacc0 += __shfl_xor(dacc0, o, 64);
+            dacc1 += __shfl_xor(dacc1, o, 64);
+        }
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane == 0)
+        {
+            dred[0][wv] = dacc0;
+            dred[1][wv] = dacc1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            double s0 = 0., s1 = 0.;
+            for (int w = 0; w < NT / 64; w++)
+            {
+                s0 += dred[0][w];
+                s1 += dred[1][w];
+            }
+            a.dout[j]             = s0 * a.dvel;
+            a.dout[gridDim.x + j] = s1 * a.dvel;
         }
     }
 }
@@ -1853,26 +1907,37 @@ void march_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, const 
             in, f, v, out, w, a, c.c4, c.c2);
 }
 
-template <typename T, int TYPE, int MODE, int NT, int P>
-void march4_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
+template <typename T, int TYPE, int MODE, int NT, int P, bool DOTS>
+void march4_launch_d(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
     const double* v, T* out, T* w, const Coefs& c)
 {
     constexpr int V = 16 / sizeof(T);
     // two plane buffers when the blocks that fill a CU (32 waves) still fit
     // its 160 KB of LDS
     if (2 * plane * (2048 / NT) <= 160 * 1024)
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V>), dim3(bs->nbox), dim3(NT), 2 * plane,
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V, DOTS>), dim3(bs->nbox), dim3(NT), 2 * plane,
             ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
     else
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V>), dim3(bs->nbox), dim3(NT), plane, ctx->stream,
-            in, f, v, out, w, a, c.c4, c.c2);
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V, DOTS>), dim3(bs->nbox), dim3(NT), plane,
+            ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
+}
+
+// the Anderson dots exist for the Jacobi sweep only (a.dout set)
+template <typename T, int TYPE, int MODE, int NT, int P>
+void march4_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
+    const double* v, T* out, T* w, const Coefs& c)
+{
+    if (MODE == MARCH_JACOBI && a.dout)
+        march4_launch_d<T, TYPE, MODE, NT, P, MODE == MARCH_JACOBI>(ctx, bs, a, plane, in, f, v, out, w, c);
+    else
+        march4_launch_d<T, TYPE, MODE, NT, P, false>(ctx, bs, a, plane, in, f, v, out, w, c);
 }
 
 // the marching sweep for 2nd/4th-order operators; returns false if the box
 // sizes at this level need another kernel
 template <typename T, int TYPE, int MODE>
 bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* in, const T* f, const double* v,
-    T* out, T* w, const double* nrm = nullptr)
+    T* out, T* w, const double* nrm = nullptr, const MarchDots* dots = nullptr)
 {
     const int emax = bs->maxedge >> level;
     const int E2   = emax * emax;
@@ -1895,6 +1960,14 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
     bool by4 = emax % 4 == 0;
     for (int jb = 0; by4 && jb < bs->nbox; jb++)
         by4 = ((bs->edge[jb] >> level) % 4) == 0;
+    if (dots)
+    {
+        if (!by4) return false;  // the dot products exist in k_box_march4 only
+        a.dkr0 = dots->kr0;
+        a.da   = dots->a;
+        a.dvel = dots->vel;
+        a.dout = dots->out;
+    }
     if (by4)
     {
         // V points per thread, z ring ZR (see k_box_march4)
@@ -1915,6 +1988,7 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
                 march4_launch<T, TYPE, MODE, 1024, 1>(ctx, bs, a, plane, in, f, v, out, w, c);
             return true;
         }
+        if (dots) return false;
         a.lds_w = emax + 4;
     }
     if ((size_t)a.lds_w * a.lds_w * sizeof(T) > 60 * 1024) return false;
@@ -1981,9 +2055,16 @@ int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vt
 
 template <typename T>
 int box_jacobi_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, double jf, const T* x, const T* f, T* xo,
-    T* w)
+    T* w, const MarchDots* dots = nullptr, bool* dots_done = nullptr)
 {
     const double mjf = -1. * jf;
+    if (dots && type == MGX_LAP_4TH
+        && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, w, nullptr, dots))
+    {
+        MGX_CHECK_LAUNCH(ctx);
+        if (dots_done) *dots_done = true;
+        return MGX_OK;
+    }
     if (type == MGX_LAP_4TH && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, w))
     {
         MGX_CHECK_LAUNCH(ctx);
@@ -2076,7 +2157,7 @@ int box_mg_setup(mgx_ctx* ctx, const mgx_boxset* bs, int max_levels, BoxMG<T>& B
 // result is back in v.
 template <typename T>
 int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, T* v, const T* f, int type,
-    int level, bool first_done)
+    int level, bool first_done, const MarchDots* dots = nullptr, bool* dots_done = nullptr)
 {
     const int ncycl = (level == max_levels) ? 4 : 2;
     double h[3]     = { bs->h[0] * (1 << level), bs->h[1] * (1 << level), bs->h[2] * (1 << level) };
@@ -2111,7 +2192,9 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
     for (int it = 0; it < 2; it++)
     {
         const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0) : -1;
-        if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr))) return rc;
+        // the last fine sweep (it writes v) may also form the Anderson dots
+        const MarchDots* dd = (level == 0 && it == 1) ? dots : nullptr;
+        if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr, dd, dots_done))) return rc;
         prof_end(ctx, tok);
         std::swap(a, b);
     }
@@ -2122,7 +2205,7 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
 // out (FP64) written if non-null, else psi += alpha*x if psi non-null.
 template <typename T>
 int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, const double* res,
-    double* out, double* psi, double alpha)
+    double* out, double* psi, double alpha, const MarchDots* dots = nullptr, bool* dots_done = nullptr)
 {
     MGX_REQUIRE(ctx, type >= 0 && type <= 2, "LapFactory::createLap() --- option invalid");
     MGX_REQUIRE(ctx, gamma >= 0., "gamma must be non-negative");
@@ -2142,7 +2225,7 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     else
         hipLaunchKernelGGL(k_prep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, x, n, gamma);
     MGX_CHECK_LAUNCH(ctx);
-    if ((rc = box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0, zero))) return rc;
+    if ((rc = box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0, zero, dots, dots_done))) return rc;
     if (out)
         hipLaunchKernelGGL(k_to_f64<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, (const T*)x, out, n);
     else if (psi)
@@ -2153,11 +2236,18 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
 
 // the V-cycle of box_precond_t, result left in the context workspace (type
 // prec); *x points at it until the next preconditioner call
+// the V-cycle's correction left in workspace slot 2 (*x); with kr0 non-null
+// the last fine sweep also forms the Anderson dot products of
+// box_anderson_dots into d (2 nbox) when its kernel can (*dots_done)
 int box_precond_x(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
-    const double* res, const void** x)
+    const double* res, const void** x, const void* kr0, double a, double* d, bool* dots_done)
 {
-    int rc = prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0.)
-                             : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0.);
+    MarchDots md{ kr0, a, bs->h[0] * bs->h[1] * bs->h[2], d };
+    const MarchDots* dots = kr0 ? &md : nullptr;
+    if (dots_done) *dots_done = false;
+    int rc = prec == MGX_F32
+                 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots, dots_done)
+                 : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots, dots_done);
     if (rc) return rc;
     *x = workspace(ctx, 2, 0);
     return MGX_OK;
@@ -2331,6 +2421,14 @@ int box_density(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const dou
 // Anderson m = 1 inner products, weighted by diag(S^-1): out[0] =
 // <f - f0, f - f0>, out[1] = <f - f0, f> (device), f = a*kr, f0 = a*kr0;
 // kr, kr0 of type prec (MGX_F32 / MGX_F64).  d: 2 n scratch.
+// the S^-1-weighted totals of per-box dots already in d (see box_precond_x)
+int box_anderson_dots_total(mgx_ctx* ctx, const mgx_boxset* bs, const double* Sinv, const double* d, double* out)
+{
+    hipLaunchKernelGGL(k_diag_wsum2, dim3(1), dim3(256), 0, ctx->stream, d, Sinv, bs->nbox, out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
 int box_anderson_dots(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const void* kr, double a, const void* kr0,
     const double* Sinv, double* d, double* out)
 {

@@ -47,7 +47,9 @@ int box_anderson_dots(mgx_ctx*, const mgx_boxset*, int, const void*, double, con
 int box_anderson_update_sq(mgx_ctx*, const mgx_boxset*, int, const double*, double*, const void*, double, void*,
     double, double, double*);
 int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double, double*);
-int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, const void**);
+int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, const void**, const void*,
+    double, double*, bool*);
+int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
 int box_density(mgx_ctx*, const mgx_boxset*, const double*, const double*, int, double, double*, double*);
@@ -248,6 +250,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     st->chol_calls     = (int)s->chol_calls;
     const double alpha = 0.5 * s->cfg.step_length;
     bool norms_done    = false;  // d already holds the norms of the updated psi
+    bool dots_done     = false;  // the V-cycle formed the Anderson dots in d
     tk = prof_begin(ctx, MGX_PROF_PRECOND);
     const int prec = s->cfg.precond_dtype;
     if (!use_cnn && !s->accelerate)
@@ -279,7 +282,8 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             s->r_prev_norm = rnorm;
             s->learned_steps++;
         }
-        else if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, &kr)))
+        else if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, &kr,
+                      (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done)))
             return rc;
         if (!s->accelerate)
         {
@@ -293,8 +297,11 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             double th = 0., beta = 1.;
             if (s->mm > 0)
             {
-                if ((rc = box_anderson_dots(ctx, s->bs, prec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2)))
-                    return rc;
+                if (dots_done)
+                    rc = box_anderson_dots_total(ctx, s->bs, s->Sinv, s->d, s->scal + 2);
+                else
+                    rc = box_anderson_dots(ctx, s->bs, prec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2);
+                if (rc) return rc;
                 MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host + 3, s->scal + 2, 2 * sizeof(double),
                                        hipMemcpyDeviceToHost, ctx->stream));
                 MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
