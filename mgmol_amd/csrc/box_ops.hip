@@ -1042,7 +1042,7 @@ __device__ __forceinline__ int64_t cell_idx3(const CellEnt& c, int x, int y, int
 }
 
 constexpr int RJB = 2;  // point tiles per load batch in k_cell_residual and k_cell_density
-constexpr int kGramWaves = 4;  // waves per cell in k_cell_gram with paired loads (8 measured slower)
+constexpr int kGramWaves = 1;  // waves per cell in k_cell_gram with paired loads (1 measured fastest of 1, 2, 4, 8)
 
 // Per cell: G = A^T A and P = A^T B (A = psi, B = H psi, 512 x k) on MFMA.
 // Four waves split the 512 points (two x-planes each), then combine in LDS
@@ -1118,12 +1118,15 @@ __device__ __forceinline__ void cell_gram_tile(const double* psi, const double* 
 // two MFMA steps.  Valid when every box origin and edge is even, so the
 // covered z-range of a cell row starts and ends at even positions and the
 // pair is aligned.  The split keeps FP32 partial sums over 32 points.
-template <int PREC, bool SAME, int NW>
+// G > 1 packs G row groups into the 16 box columns (16/G boxes, k <= 16/G):
+// lane column c = g (16/G) + a loads box a's row of group g, so one step
+// covers G rows; the product's diagonal blocks (same g) are the partials.
+template <int PREC, bool SAME, int NW, int G>
 __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double* hpsi, const CellEnt& A,
-    const CellEnt& B, int wid, int r, double (&accG)[4], double (&accP)[4])
+    const CellEnt& B, int wid, int r, int grp, double (&accG)[4], double (&accP)[4])
 {
-    // NW waves share the cell's 64 rows of 8 points: NS rows each
-    constexpr int NS = 64 / NW;
+    // NW waves share the cell's 64 rows of 8 points: NS steps of G rows each
+    constexpr int NS = 64 / (NW * G);
     typedef double d2 __attribute__((ext_vector_type(2)));
     const d2 z2 = { 0., 0. };
     auto ld2 = [](const double* p, int64_t i) -> d2 { return *reinterpret_cast<const d2*>(p + i); };
@@ -1133,7 +1136,7 @@ __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double*
 #pragma unroll 4
         for (int s = 0; s < NS; s++)
         {
-            const int row = wid * NS + s;
+            const int row = (wid * NS + s) * G + grp;
             const int x = row >> 3, y = row & 7, z = 2 * r;
             const int64_t ia = cell_idx3(A, x, y, z);
             const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
@@ -1161,7 +1164,7 @@ __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double*
 #pragma unroll 2
             for (int s = s0; s < s0 + 4; s++)
             {
-                const int row = wid * NS + s;
+                const int row = (wid * NS + s) * G + grp;
                 const int x = row >> 3, y = row & 7, z = 2 * r;
                 const int64_t ia = cell_idx3(A, x, y, z);
                 const int64_t ib = SAME ? ia : cell_idx3(B, x, y, z);
@@ -1192,10 +1195,52 @@ __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double*
     }
 }
 
+// A cell with k <= 16/G boxes as one packed tile (G row groups per step):
+// the partial (a, b) is the sum over the groups g of the product's entry
+// (i, j) = (g kp + a, g kp + b), kp = 16/G, read from the lanes that hold it
+// (FP64 MFMA: lane j + 16 (i % 4), value i / 4; FP32: lane j + 16 (i / 4),
+// value i % 4), over the waves and groups in a fixed order.
+template <int PREC, int NW, int G>
+__device__ __forceinline__ void cell_gram_packed(const double* psi, const double* hpsi, const CellEnt* ce, int k,
+    int wid, int lane, double (*red)[64][4], double* part, int64_t nslot)
+{
+    constexpr int kp = 16 / G;
+    const int c = lane & 15, r = lane >> 4;
+    const int a = c % kp, g = c / kp;
+    CellEnt none{};
+    const CellEnt A = a < k ? ce[a] : none;
+    double accG[4], accP[4];
+    cell_gram_tile2<PREC, true, NW, G>(psi, hpsi, A, A, wid, r, g, accG, accP);
+    for (int pass = 0; pass < 2; pass++)
+    {
+        const double* acc = pass == 0 ? accG : accP;
+        for (int v = 0; v < 4; v++)
+            red[wid][lane][v] = acc[v];
+        __syncthreads();
+        for (int o = threadIdx.x; o < k * k; o += 64 * NW)
+        {
+            const int aa = o / k, bb = o - (o / k) * k;
+            double sum = 0.;
+            for (int w = 0; w < NW; w++)
+                for (int gg = 0; gg < G; gg++)
+                {
+                    const int i = gg * kp + aa, j = gg * kp + bb;
+                    const int ln = j + 16 * (PREC == MGX_PREC_F64 ? (i & 3) : (i >> 2));
+                    sum += red[w][ln][PREC == MGX_PREC_F64 ? (i >> 2) : (i & 3)];
+                }
+            part[pass * nslot + o] = sum;
+        }
+        __syncthreads();
+    }
+}
+
 // S / H partials per 8^3 cell over all box pairs covering it (k <= 32):
 // G_ab = sum_p psi_a psi_b, P_ab = sum_p psi_a hpsi_b, as 16 x 16 box tiles
-// on MFMA with K = points (4 per step); wave w takes x-planes 2w, 2w+1, and
-// the four waves' sums are added in a fixed order.
+// on MFMA with K = points (4 per step); the NW waves split the cell's rows,
+// and their sums are added in a fixed order.  With paired loads, cells of
+// k <= 8 boxes pack 2 or 4 row groups into the tile's 16 columns, so that
+// the loads of a step fill the wave (a cell is covered by 5-6 boxes on
+// average when 32^3 boxes cover the grid 4 times).
 template <int PREC, bool PAIR, int NW>
 __global__ void __launch_bounds__(64 * NW) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
     const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part,
@@ -1216,6 +1261,15 @@ __global__ void __launch_bounds__(64 * NW) k_cell_gram(const double* psi, const 
     const int nt = (k + 15) / 16;
     const int64_t slot = cell_slot[cell];
     CellEnt none{};
+    if (PAIR && k <= 8)
+    {
+        // k <= 4: four row groups per step, k <= 8: two
+        if (k <= 4)
+            cell_gram_packed<PREC, NW, 4>(psi, hpsi, ce, k, wid, lane, red, part + slot, nslot);
+        else
+            cell_gram_packed<PREC, NW, 2>(psi, hpsi, ce, k, wid, lane, red, part + slot, nslot);
+        return;
+    }
     for (int ti = 0; ti < nt; ti++)
         for (int tj = 0; tj < nt; tj++)
         {
@@ -1224,9 +1278,9 @@ __global__ void __launch_bounds__(64 * NW) k_cell_gram(const double* psi, const 
             const CellEnt B = b < k ? ce[b] : none;
             double accG[4], accP[4];
             if (PAIR && ti == tj)
-                cell_gram_tile2<PREC, true, NW>(psi, hpsi, A, B, wid, r, accG, accP);
+                cell_gram_tile2<PREC, true, NW, 1>(psi, hpsi, A, B, wid, r, 0, accG, accP);
             else if (PAIR)
-                cell_gram_tile2<PREC, false, NW>(psi, hpsi, A, B, wid, r, accG, accP);
+                cell_gram_tile2<PREC, false, NW, 1>(psi, hpsi, A, B, wid, r, 0, accG, accP);
             else if (ti == tj)
                 cell_gram_tile<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
             else
@@ -1397,7 +1451,7 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
 // R_b(p) = sum_a psi_a(p) theta(a,b) - hpsi_b(p) for every box b covering
 // point p of the cell, as a 512 x k by k x k product on FP64 MFMA:
 // A = psi gathered per (point, box a), B = theta sub-block from LDS.
-__global__ void __launch_bounds__(256) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
+__global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
     int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res, double* esq,
     int ncell)
 {

@@ -126,6 +126,47 @@ def test_box_gram_h_residual(ctx, boxes, cfg, prec):
     assert out.value == pytest.approx(O.residual_norm(O.b_diag_dots(ob, rref), Sinv), rel=1e-12)
 
 
+def _cell_multiplicity(cfg):
+    g = np.asarray(cfg.gdim) // 8
+    cnt = np.zeros(tuple(g), dtype=int)
+    for o, e in zip(cfg.origin, cfg.edge):
+        ix = [np.arange(o[d] // 8, (o[d] + e - 1) // 8 + 1) % g[d] for d in range(3)]
+        cnt[np.ix_(*ix)] += 1
+    return cnt.ravel()
+
+
+@pytest.mark.parametrize("prec", [mgx.PREC_F64, mgx.PREC_SPLIT])
+@pytest.mark.parametrize("crowd", ["mixed", "crowded"])
+def test_box_gram_cell_multiplicity(ctx, prec, crowd):
+    """Cells covered by 1-4, 5-8, 9-16 and more than 16 boxes take different
+    tilings of the cell kernel (row groups packed into the MFMA columns, one
+    16-box tile, 2 x 2 tiles); each class is present in one of the two sets."""
+    if crowd == "mixed":
+        cfg = synth.box_config(nbox=80, gdim=48, h=0.3, edges=(8, 12, 16, 20), nwells=4, seed=5, levels=3)
+    else:
+        cfg = synth.box_config(nbox=220, gdim=48, h=0.3, edges=(12, 16, 20), nwells=4, seed=7, levels=3)
+    k = _cell_multiplicity(cfg)
+    if crowd == "mixed":
+        assert k.min() <= 4 and np.any((k > 4) & (k <= 8)) and np.any(k > 8)
+    else:
+        assert k.max() > 16 and k.max() <= 32
+    ob = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    try:
+        hp = O.b_hloc(ob, cfg.vtot, cfg.psi)
+        n = ob.nbox
+        tp, th = to_dev(cfg.psi), to_dev(hp)
+        tS, tH = to_dev(np.zeros((n, n))), to_dev(np.zeros((n, n)))
+        ctx.call("mgx_box_gram_h", bs.h, prec, mgx.ptr(tp), mgx.ptr(th), mgx.ptr(tS), mgx.ptr(tH))
+        S, H = O.b_gram_h(ob, cfg.psi, hp)
+        es, eh = relmax(to_np(tS), S), relmax(to_np(tH), H)
+        say("gram/H", crowd, prec, es, eh)
+        tol = 1e-13 if prec == mgx.PREC_F64 else 1e-7
+        assert es <= tol and eh <= tol
+    finally:
+        bs.close()
+
+
 def test_boxset_rejects_misaligned(ctx):
     with pytest.raises(mgx.MgxError):
         mgx.BoxSet(ctx, [[2, 0, 0]], [8], (32, 32, 32), 0.3, 3)
