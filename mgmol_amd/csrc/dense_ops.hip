@@ -239,6 +239,40 @@ __global__ void __launch_bounds__(256) k_id_err(const double* T, int n, double* 
     if (threadIdx.x == 0) blockmax[blockIdx.x] = red[0];
 }
 
+// k_id_err that also writes E = I - T and X in FP32 (the operands of the
+// FP32 correction product of a Newton-Schulz step)
+__global__ void __launch_bounds__(256) k_id_err_f32(const double* T, const double* X, int n, double* blockmax,
+    float* E32, float* X32)
+{
+    __shared__ double red[256];
+    double m = 0.;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)n * n;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t / n), row = (int)(t - (t / n) * n);
+        const double e = (row == col ? 1. : 0.) - T[t];
+        const double v = fabs(e);
+        m              = v > m ? v : m;
+        E32[t]         = (float)e;
+        X32[t]         = (float)X[t];
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) blockmax[blockIdx.x] = red[0];
+}
+
+// out = x + (double)y
+__global__ void k_add_f32(const double* x, const float* y, double* out, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        out[t] = x[t] + (double)y[t];
+}
+
 __global__ void __launch_bounds__(256) k_max_reduce(const double* v, int n, double* out)
 {
     __shared__ double red[256];
@@ -670,6 +704,19 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
     double* bm  = (double*)workspace(ctx, 4, 1024 * sizeof(double));
     if (!bm) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    // a loose tolerance (an S from the split products) takes the correction
+    // E X of a step, E = I - X S, as one FP32 product: its rounding error,
+    // ~1e-7 |E| |X|, is below |E|^2 while |E| > 1e-5 and below tol / 10
+    // after the last step (|E| < 1e-5, |X| ~ 1-10)
+    const bool f32 = tol >= 1e-12;
+    float *E32 = nullptr, *X32 = nullptr, *Y32 = nullptr;
+    if (f32)
+    {
+        E32 = (float*)workspace(ctx, 7, 3 * nn * sizeof(float));
+        if (!E32) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+        X32 = E32 + nn;
+        Y32 = X32 + nn;
+    }
     double prev = 1e300;
     *steps      = 0;
     int rc;
@@ -677,7 +724,11 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
     {
         if ((rc = spmm_pairs(ctx, n, cur, Sv, pt, T))) return rc;
         const int nb = 512;
-        hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
+        if (f32)
+            hipLaunchKernelGGL(k_id_err_f32, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T,
+                (const double*)cur, n, bm, E32, X32);
+        else
+            hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
         hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double*)bm, nb, bm + nb);
         MGX_CHECK_LAUNCH(ctx);
         double delta = 0.;
@@ -689,11 +740,23 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
             return chol_inv(ctx, n, S, L, X);
         }
         if (delta < tol) break;
+        if (f32)
+        {
+            // nxt = cur + E cur
+            const float onef = 1.f, zerof = 0.f;
+            if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &onef, E32, n, X32,
+                    n, &zerof, Y32, n)
+                != rocblas_status_success)
+                return set_err(ctx, MGX_EHIP, "rocblas_sgemm failed");
+            hipLaunchKernelGGL(k_add_f32, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream,
+                (const double*)cur, (const float*)Y32, nxt, (int64_t)nn);
+            MGX_CHECK_LAUNCH(ctx);
+        }
         // nxt = 2 cur - T cur
-        if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
-                rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n, nxt,
-                rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
-            != rocblas_status_success)
+        else if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
+                     rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n,
+                     nxt, rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
+                 != rocblas_status_success)
             return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex failed");
         std::swap(cur, nxt);
         (*steps)++;
