@@ -1858,30 +1858,40 @@ __global__ void __launch_bounds__(256) k_diag_wsum2(const double* d, const doubl
     {
         out[0] = acc[0];
         out[1] = acc[1];
+        // AndersonMix's theta for one history vector: rhs / mat, out of
+        // range values reset (above 0.5: 0, or -0.5 above 1), at least -3
+        double th = acc[0] > 0. ? acc[1] / acc[0] : 0.;
+        if (th > 0.5)
+            th = th > 1. ? -0.5 : 0.;
+        else if (th < -3.)
+            th = -3.;
+        out[2] = th;
     }
 }
 
 // Anderson m = 1 update (AndersonMix::update): x_bar = (1-th) x + th x0,
-// f_bar = (1-th) f + th f0 with f = a*kr, f0 = a*kr0; kr0 <- kr and the new
+// f_bar = (1-th) f + th f0 with f = a*kr, f0 = a*kr0 (th from thp, kr0 not
+// read when th = 0); kr0_out <- kr when non-null, and the new
 // x = x_bar + beta f_bar written to xo (the buffer holding x0, which the
 // caller then swaps with x: the old x becomes x0 without a copy), with the
 // normalisation sums of the new x, d_j = vel |x_j|^2, in k_box_sumsq's
 // order: one block per box
 template <typename T>
 __global__ void __launch_bounds__(256) k_box_and_update_sq(const double* x, double* xo, const T* kr, double a,
-    T* kr0, const int64_t* off, int nbox, double th, double beta, double vel, double* d)
+    const T* kr0, T* kr0_out, const int64_t* off, int nbox, const double* thp, double beta, double vel, double* d)
 {
     __shared__ double lds[4];
     const int j = blockIdx.x;
     if (j >= nbox) return;
+    const double th = thp ? *thp : 0.;
     double acc[1] = { 0. };
     for (int64_t q = off[j] + threadIdx.x; q < off[j + 1]; q += blockDim.x)
     {
         const T k       = kr[q];
-        const double xv = x[q], x0v = xo[q], fv = a * (double)k, f0v = a * (double)kr0[q];
+        const double xv = x[q], x0v = xo[q], fv = a * (double)k, f0v = th != 0. ? a * (double)kr0[q] : 0.;
         const double xb = (1. - th) * xv + th * x0v;
         const double fb = (1. - th) * fv + th * f0v;
-        kr0[q]          = k;
+        if (kr0_out) kr0_out[q] = k;
         const double nv = xb + beta * fb;
         xo[q]           = nv;
         acc[0] += nv * nv;
@@ -2259,14 +2269,15 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
 // out (FP64) written if non-null, else psi += alpha*x if psi non-null.
 template <typename T>
 int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, const double* res,
-    double* out, double* psi, double alpha, const MarchDots* dots = nullptr, bool* dots_done = nullptr)
+    double* out, double* psi, double alpha, const MarchDots* dots = nullptr, bool* dots_done = nullptr,
+    T* xdst = nullptr)
 {
     MGX_REQUIRE(ctx, type >= 0 && type <= 2, "LapFactory::createLap() --- option invalid");
     MGX_REQUIRE(ctx, gamma >= 0., "gamma must be non-negative");
     const int64_t n = bs->off[0][bs->nbox];
     T* xf           = (T*)workspace(ctx, 2, 2 * n * sizeof(T));
     if (!xf) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
-    T* x = xf;
+    T* x = xdst ? xdst : xf;
     T* f = xf + n;
     BoxMG<T> B;
     int rc = box_mg_setup(ctx, bs, max_levels, B, 3);
@@ -2288,23 +2299,21 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     return MGX_OK;
 }
 
-// the V-cycle of box_precond_t, result left in the context workspace (type
-// prec); *x points at it until the next preconditioner call
-// the V-cycle's correction left in workspace slot 2 (*x); with kr0 non-null
-// the last fine sweep also forms the Anderson dot products of
-// box_anderson_dots into d (2 nbox) when its kernel can (*dots_done)
+// the V-cycle of box_precond_t, its correction (type prec) written to xdst
+// (nbox points of the boxes); with kr0 non-null the last fine sweep also
+// forms the Anderson dot products of box_anderson_dots into d (2 nbox) when
+// its kernel can (*dots_done)
 int box_precond_x(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
-    const double* res, const void** x, const void* kr0, double a, double* d, bool* dots_done)
+    const double* res, void* xdst, const void* kr0, double a, double* d, bool* dots_done)
 {
     MarchDots md{ kr0, a, bs->h[0] * bs->h[1] * bs->h[2], d };
     const MarchDots* dots = kr0 ? &md : nullptr;
     if (dots_done) *dots_done = false;
-    int rc = prec == MGX_F32
-                 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots, dots_done)
-                 : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots, dots_done);
-    if (rc) return rc;
-    *x = workspace(ctx, 2, 0);
-    return MGX_OK;
+    if (!xdst) return set_err(ctx, MGX_EINVAL, "no destination for the correction");
+    return prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots,
+                                 dots_done, (float*)xdst)
+                           : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots,
+                                 dots_done, (double*)xdst);
 }
 
 int box_precond(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
@@ -2498,18 +2507,20 @@ int box_anderson_dots(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const void* 
     return MGX_OK;
 }
 
-// Anderson update into xo (see k_box_and_update_sq) + the norms d of the new x
+// Anderson update into xo (see k_box_and_update_sq) + the norms d of the new
+// x; theta read from device memory (null: 0); kr0_out, if non-null, receives
+// a copy of kr (the next step's history when kr's buffer is not kept)
 int box_anderson_update_sq(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* x, double* xo, const void* kr,
-    double a, void* kr0, double th, double beta, double* d)
+    double a, const void* kr0, void* kr0_out, const double* th, double beta, double* d)
 {
     const double vel = bs->h[0] * bs->h[1] * bs->h[2];
     const int64_t* off = (const int64_t*)bs->d_off[0];
     if (prec == MGX_F32)
         hipLaunchKernelGGL(k_box_and_update_sq<float>, dim3(bs->nbox), dim3(256), 0, ctx->stream, x, xo,
-            (const float*)kr, a, (float*)kr0, off, bs->nbox, th, beta, vel, d);
+            (const float*)kr, a, (const float*)kr0, (float*)kr0_out, off, bs->nbox, th, beta, vel, d);
     else
         hipLaunchKernelGGL(k_box_and_update_sq<double>, dim3(bs->nbox), dim3(256), 0, ctx->stream, x, xo,
-            (const double*)kr, a, (double*)kr0, off, bs->nbox, th, beta, vel, d);
+            (const double*)kr, a, (const double*)kr0, (double*)kr0_out, off, bs->nbox, th, beta, vel, d);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }

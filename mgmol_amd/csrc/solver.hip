@@ -44,11 +44,11 @@ int inverse_update(mgx_ctx*, int, const double*, double*, double*, double*, int*
 int ritz(mgx_ctx*, int, const double*, const double*, double*, double*);
 int box_anderson_dots(mgx_ctx*, const mgx_boxset*, int, const void*, double, const void*, const double*, double*,
     double*);
-int box_anderson_update_sq(mgx_ctx*, const mgx_boxset*, int, const double*, double*, const void*, double, void*,
-    double, double, double*);
+int box_anderson_update_sq(mgx_ctx*, const mgx_boxset*, int, const double*, double*, const void*, double,
+    const void*, void*, const double*, double, double*);
 int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double, double*);
-int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, const void**, const void*,
-    double, double*, bool*);
+int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, void*, const void*, double,
+    double*, bool*);
 int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
@@ -80,6 +80,7 @@ struct mgx_solver
     double *Sv = nullptr, *Hv = nullptr;  // S, H per interacting pair
     double* x0 = nullptr;  // Anderson history: previous psi
     void* kr0  = nullptr;  // and previous correction, in the preconditioner's type
+    void* kr1  = nullptr;  // the V-cycle's correction (swapped with kr0 after the update)
     void* kcnn = nullptr;  // the network's correction (learned steps)
     // DFTsolver::checkConvergenceEnergy state (band energy, Ha)
     double e_prev = 0., deig = 1e9, deig2 = 1e9;
@@ -114,6 +115,7 @@ void free_solver(mgx_solver* s)
     for (double* b : bufs)
         if (b) hipFree(b);
     if (s->kr0) (void)hipFree(s->kr0);
+    if (s->kr1) (void)hipFree(s->kr1);
     if (s->kcnn) (void)hipFree(s->kcnn);
     if (s->host) hipHostFree(s->host);
     for (hipEvent_t e : s->prof.pool)
@@ -282,9 +284,13 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             s->r_prev_norm = rnorm;
             s->learned_steps++;
         }
-        else if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, &kr,
-                      (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done)))
-            return rc;
+        else
+        {
+            if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, s->kr1,
+                     (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done)))
+                return rc;
+            kr = s->kr1;
+        }
         if (!s->accelerate)
         {
             // psi += alpha K(R), with the norms of the new psi
@@ -293,8 +299,10 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         }
         else
         {
-            // AndersonMix::update with m = 1 on f = alpha K(R)
-            double th = 0., beta = 1.;
+            // AndersonMix::update with m = 1 on f = alpha K(R); theta is
+            // formed on the device with the dot products (no host wait)
+            const double* thp = nullptr;
+            double beta       = 1.;
             if (s->mm > 0)
             {
                 if (dots_done)
@@ -302,24 +310,19 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
                 else
                     rc = box_anderson_dots(ctx, s->bs, prec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2);
                 if (rc) return rc;
-                MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host + 3, s->scal + 2, 2 * sizeof(double),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-                MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
-                const double mat = s->host[3], rhs = s->host[4];
-                th = mat > 0. ? rhs / mat : 0.;
-                // theta bounds of AndersonMix (max 0.5, min -3; with one history
-                // vector an out-of-range theta is reset, not the history shortened)
-                if (th > 0.5)
-                    th = th > 1. ? -0.5 : 0.;
-                else if (th < -3.)
-                    th = -3.;
+                thp  = s->scal + 4;
                 beta = s->cfg.anderson_beta;
             }
             // the new psi goes to the history buffer, and the buffers swap:
-            // the old psi is the next step's x0; the norms of the new psi too
-            if ((rc = box_anderson_update_sq(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0, th, beta, s->d)))
+            // the old psi is the next step's x0; the norms of the new psi too.
+            // The V-cycle's correction buffer becomes the history (kr0, kr1
+            // swap); the network's correction is copied into kr0 instead
+            const bool keep = kr == s->kr1;
+            if ((rc = box_anderson_update_sq(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0,
+                     keep ? nullptr : s->kr0, thp, beta, s->d)))
                 return rc;
             std::swap(s->psi, s->x0);
+            if (keep) std::swap(s->kr0, s->kr1);
             norms_done = true;
             s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
         }
@@ -403,7 +406,8 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
               && alloc(&s->Sv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && (cfg->anderson_m == 0
-                  || (alloc(&s->x0, s->npts) && hipMalloc(&s->kr0, s->npts * tsize) == hipSuccess))
+                  || (alloc(&s->x0, s->npts) && hipMalloc(&s->kr0, s->npts * tsize) == hipSuccess
+                      && hipMalloc(&s->kr1, s->npts * tsize) == hipSuccess))
               && (cfg->precond_kind != MGX_PRECOND_LEARNED
                   || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts)
                       && hipMalloc(&s->kcnn, s->npts * tsize) == hipSuccess))
