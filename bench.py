@@ -44,7 +44,7 @@ WEIGHTS = os.path.join(ROOT, "mgmol_amd", "weights", "cnn_e32.npz")
 # HBM bytes per launch at C5, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes
 # (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000, both calibrated on
 # streaming kernels of known size; profiles/pmc_traffic_c5.md)
-TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895234355, "cell_gram_split": 1103204147}
+TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895234355, "cell_gram_split": 1416436634}
 
 PATHS = {
     "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64),
@@ -92,7 +92,7 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
     return dict(seconds=float(np.mean(times)), iterations=stats.iterations, residual=stats.residual,
                 energy=stats.band_energy, converged=bool(stats.converged), learned_steps=stats.learned_steps,
                 fallbacks=stats.fallbacks, newton_schulz_steps=stats.ns_steps, cholesky_calls=stats.chol_calls,
-                profile=prof)
+                dots_sweeps=stats.dots_sweeps, profile=prof)
 
 
 def cpu_baseline(cfg, iters, mg_levels):
@@ -110,12 +110,13 @@ def cpu_baseline(cfg, iters, mg_levels):
 
 def jacobi_roofline(r, s_bytes, npts):
     ms, launches = r["profile"]["jacobi0"]
-    # the 3 fine sweeps launched per V-cycle (the zero-start first sweep is
-    # pointwise, fused with the cast): read x, f and write x, plus w on the
-    # last pre-sweep -> (4 + 3 + 3) / 3 values per point
-    per_launch = (10.0 / 3.0) * s_bytes * npts
     if not launches:
         return None
+    # the 3 fine sweeps launched per V-cycle (the zero-start first sweep is
+    # pointwise, fused with the cast): read x, f and write x, plus w on the
+    # last pre-sweep -> (4 + 3 + 3) / 3 values per point; a last sweep that
+    # also forms the Anderson dots reads the previous correction as well
+    per_launch = (10.0 * launches + r["dots_sweeps"]) / (3.0 * launches) * s_bytes * npts
     avg_s = (ms / 1e3) / launches
     achieved = per_launch / avg_s / 1e9
     traffic = TRAFFIC_C5["jacobi_fp32_sweep"] if (s_bytes == 4 and npts == 2048 * 32 ** 3) else None

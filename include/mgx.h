@@ -365,6 +365,7 @@ typedef struct
     int fallbacks;       /* CNN steps rejected and redone with the V-cycle, cumulative */
     int ns_steps;        /* Newton-Schulz refresh steps of S^-1 (one dgemm each), cumulative */
     int chol_calls;      /* Cholesky + inverse of S (first step and refresh failures), cumulative */
+    int dots_sweeps;     /* V-cycles whose last fine sweep also formed the Anderson dots, cumulative */
 } mgx_solver_stats_t;
 
 int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t* cfg,

@@ -99,7 +99,7 @@ struct mgx_solver
     int learned_steps = 0, fallbacks = 0;
     mgx::PairTiles tiles;
     bool have_inv = false;  // Sinv holds the inverse of a recent S
-    int64_t ns_steps = 0, chol_calls = 0;
+    int64_t ns_steps = 0, chol_calls = 0, dots_sweeps = 0;
     double* host = nullptr;  // pinned: [energy, norm]
 };
 
@@ -250,6 +250,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     st->fallbacks      = s->fallbacks;
     st->ns_steps       = (int)s->ns_steps;
     st->chol_calls     = (int)s->chol_calls;
+    st->dots_sweeps    = (int)s->dots_sweeps;
     const double alpha = 0.5 * s->cfg.step_length;
     bool norms_done    = false;  // d already holds the norms of the updated psi
     bool dots_done     = false;  // the V-cycle formed the Anderson dots in d
@@ -306,7 +307,10 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             if (s->mm > 0)
             {
                 if (dots_done)
+                {
                     rc = box_anderson_dots_total(ctx, s->bs, s->Sinv, s->d, s->scal + 2);
+                    s->dots_sweeps++;
+                }
                 else
                     rc = box_anderson_dots(ctx, s->bs, prec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2);
                 if (rc) return rc;
@@ -498,6 +502,7 @@ int mgx_solver_reset(mgx_solver* s)
     s->fallbacks     = 0;
     s->ns_steps      = 0;
     s->chol_calls    = 0;
+    s->dots_sweeps   = 0;
     return MGX_OK;
 }
 

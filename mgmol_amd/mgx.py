@@ -72,7 +72,8 @@ class SolverCfg(C.Structure):
 class SolverStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("band_energy", C.c_double), ("seconds", C.c_double), ("lowest_eig", C.c_double),
-                ("learned_steps", C.c_int), ("fallbacks", C.c_int), ("ns_steps", C.c_int), ("chol_calls", C.c_int)]
+                ("learned_steps", C.c_int), ("fallbacks", C.c_int), ("ns_steps", C.c_int), ("chol_calls", C.c_int),
+                ("dots_sweeps", C.c_int)]
 
 
 class PoissonStats(C.Structure):
