@@ -659,6 +659,70 @@ __global__ void __launch_bounds__(256) k_box_restrict2(const T* fine, const int6
 #undef F
 }
 
+// k_box_restrict2 for two z-adjacent coarse points per thread (every coarse
+// edge even, so a pair never leaves its row): the fine values z = 2z-1 ..
+// 2z+3 of the 9 (x, y) rows around them, the four from 2z in 16-byte loads;
+// per coarse point the same double sums in the same order as k_box_restrict2
+template <typename T>
+__global__ void __launch_bounds__(kTile / 2) k_box_restrict2p(const T* fine, const int64_t* off_f, T* coarse,
+    BoxView vc)
+{
+    typedef T vec16 __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int VN = 16 / sizeof(T);
+    const int64_t t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= vc.ntiles) return;
+    const int j  = vc.tile[2 * t];
+    const int ec = vc.edge[j] >> vc.level;
+    const int p  = vc.tile[2 * t + 1] + 2 * (int)threadIdx.x;
+    if (p >= ec * ec * ec) return;
+    int r       = p;
+    const int x = r / (ec * ec);
+    r -= x * ec * ec;
+    const int y = r / ec;
+    const int z = r - y * ec;
+    const int ef = 2 * ec;
+    const T* u   = fine + off_f[j];
+    const int X = 2 * x, Y = 2 * y, Z = 2 * z;
+    // w[dx + 1][dy + 1][k] = fine(X + dx, Y + dy, Z - 1 + k), zero outside the box
+    T w[3][3][5];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+        {
+            const int xx = X + a - 1, yy = Y + b - 1;
+            if ((unsigned)xx >= (unsigned)ef || (unsigned)yy >= (unsigned)ef)
+            {
+                for (int k = 0; k < 5; k++)
+                    w[a][b][k] = (T)0;
+                continue;
+            }
+            const T* row = u + ((int64_t)xx * ef + yy) * ef;
+            w[a][b][0]   = Z > 0 ? row[Z - 1] : (T)0;
+#pragma unroll
+            for (int q = 0; q < 4; q += VN)
+            {
+                const vec16 l = *reinterpret_cast<const vec16*>(row + Z + q);
+#pragma unroll
+                for (int i = 0; i < VN; i++)
+                    w[a][b][1 + q + i] = l[i];
+            }
+        }
+    const int64_t cbase = vc.off[j];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+    {
+#define F(dx, dy, dz) (double)w[(dx) + 1][(dy) + 1][2 * h + 1 + (dz)]
+        double face = F(1, 0, 0) + F(-1, 0, 0) + F(0, 1, 0) + F(0, -1, 0) + F(0, 0, -1) + F(0, 0, 1);
+        double corner = F(1, 1, -1) + F(1, 1, 1) + F(1, -1, -1) + F(1, -1, 1) + F(-1, 1, -1) + F(-1, 1, 1)
+                        + F(-1, -1, -1) + F(-1, -1, 1);
+        double edge = F(0, 1, -1) + F(0, 1, 1) + F(0, -1, -1) + F(0, -1, 1) + F(1, 0, -1) + F(1, 0, 1)
+                      + F(-1, 0, -1) + F(-1, 0, 1) + F(-1, -1, 0) + F(1, -1, 0) + F(-1, 1, 0) + F(1, 1, 0);
+        coarse[cbase + p + h] = (T)(kInv64 * (8. * F(0, 0, 0) + 4. * face + 2. * edge + corner));
+#undef F
+    }
+}
+
 // MGkernelExtend3D value at a fine box point (injected coarse values, zero
 // outside the coarse box)
 template <typename T>
@@ -2163,8 +2227,16 @@ template <typename T>
 int box_restrict_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, const T* fine, T* coarse)
 {
     BoxView vc = view(bs, level + 1);
-    hipLaunchKernelGGL(k_box_restrict2<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile), 0, ctx->stream, fine,
-        (const int64_t*)bs->d_off[level], coarse, vc);
+    // pairs of coarse points per thread when every coarse edge is even
+    bool even = ((uintptr_t)fine & 15) == 0;
+    for (int e : bs->edge)
+        even = even && ((e >> (level + 1)) & 1) == 0;
+    if (even)
+        hipLaunchKernelGGL(k_box_restrict2p<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile / 2), 0, ctx->stream, fine,
+            (const int64_t*)bs->d_off[level], coarse, vc);
+    else
+        hipLaunchKernelGGL(k_box_restrict2<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile), 0, ctx->stream, fine,
+            (const int64_t*)bs->d_off[level], coarse, vc);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }

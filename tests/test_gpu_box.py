@@ -78,6 +78,14 @@ def test_box_jacobi_transfer(ctx, boxes, dt):
     tc = to_dev(np.zeros(ob.size(1), DT[dt]))
     ctx.call("mgx_box_restrict", bs.h, 0, CODE[dt], mgx.ptr(tf), mgx.ptr(tc))
     assert np.array_equal(to_np(tc), O.b_restrict(ob, 0, f))
+    # level 1 -> 2 has odd coarse edges (2, 3, 4, 5): one point per thread
+    # there, pairs of points per thread above (all coarse edges even)
+    edges = [int(ob.edge[i]) for i in range(ob.nbox)]
+    assert any((e >> 2) % 2 for e in edges) and not any((e >> 1) % 2 for e in edges)
+    f1 = rng.standard_normal(ob.size(1)).astype(DT[dt])
+    tf1, tc1 = to_dev(f1), to_dev(np.zeros(ob.size(2), DT[dt]))
+    ctx.call("mgx_box_restrict", bs.h, 1, CODE[dt], mgx.ptr(tf1), mgx.ptr(tc1))
+    assert np.array_equal(to_np(tc1), O.b_restrict(ob, 1, f1))
     c = rng.standard_normal(ob.size(1)).astype(DT[dt])
     tc2, tfi = to_dev(c), to_dev(np.zeros(ob.size(0), DT[dt]))
     ctx.call("mgx_box_prolong", bs.h, 0, CODE[dt], mgx.ptr(tc2), mgx.ptr(tfi))
