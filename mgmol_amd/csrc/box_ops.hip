@@ -1964,6 +1964,65 @@ __global__ void __launch_bounds__(256) k_box_and_update_sq(const double* x, doub
     if (threadIdx.x == 0) d[j] = acc[0] * vel;
 }
 
+// k_box_and_update_sq with the normalisation done in the same pass: one
+// block of 1024 threads per box keeps the box's new values in registers
+// (PT per thread, boxes of at most 1024 PT points), forms the box's norm
+// d_j = vel |x_j|^2 and writes x_j / sqrt(d_j) (LocGridOrbitals::normalize,
+// with no pending scale left for the next H psi)
+template <typename T, int PT>
+__global__ void __launch_bounds__(1024) k_box_and_update_nrm(const double* x, double* xo, const T* kr, double a,
+    const T* kr0, T* kr0_out, const int64_t* off, int nbox, const double* thp, double beta, double vel, double* d)
+{
+    __shared__ double lds[17];
+    const int j = blockIdx.x;
+    if (j >= nbox) return;
+    const double th = thp ? *thp : 0.;
+    const int64_t q0 = off[j];
+    const int n      = (int)(off[j + 1] - q0);
+    const double* xj = x + q0;
+    double* xoj      = xo + q0;
+    const T* kj      = kr + q0;
+    const T* k0j     = kr0 + q0;
+    T* k0o           = kr0_out ? kr0_out + q0 : nullptr;
+    double nv[PT];
+    double acc[1] = { 0. };
+#pragma unroll
+    for (int c = 0; c < PT; c += 8)
+    {
+        // eight points in flight at a time (the scheduler may not hoist the
+        // next group's loads: registers hold the PT values already formed)
+#pragma unroll
+        for (int i = c; i < c + 8; i++)
+        {
+            const int t = threadIdx.x + 1024 * i;
+            nv[i]       = 0.;
+            if (t < n)
+            {
+                const T k       = kj[t];
+                const double xv = xj[t], x0v = xoj[t], fv = a * (double)k, f0v = th != 0. ? a * (double)k0j[t] : 0.;
+                const double xb = (1. - th) * xv + th * x0v;
+                const double fb = (1. - th) * fv + th * f0v;
+                if (k0o) k0o[t] = k;
+                nv[i] = xb + beta * fb;
+                acc[0] += nv[i] * nv[i];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    block_sum<1>(acc, lds);
+    if (threadIdx.x == 0) lds[16] = acc[0] * vel;
+    __syncthreads();
+    const double dj = lds[16];
+    const double fs = 1. / sqrt(dj);
+#pragma unroll
+    for (int i = 0; i < PT; i++)
+    {
+        const int t = threadIdx.x + 1024 * i;
+        if (t < n) xoj[t] = nv[i] * fs;
+    }
+    if (threadIdx.x == 0) d[j] = dj;
+}
+
 // psi += a * kr with the normalisation sums of the new psi (k_box_sumsq's order)
 template <typename T>
 __global__ void __launch_bounds__(256) k_box_psi_step_sq(double* psi, const T* kr, double a, const int64_t* off,
@@ -2597,6 +2656,24 @@ int box_anderson_update_sq(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const d
     return MGX_OK;
 }
 
+// box_anderson_update_sq with the normalisation applied in the same kernel
+// (boxes of at most 32^3 points); returns MGX_EINVAL for larger boxes
+int box_anderson_update_nrm(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* x, double* xo,
+    const void* kr, double a, const void* kr0, void* kr0_out, const double* th, double beta, double* d)
+{
+    if (bs->maxedge > 32) return set_err(ctx, MGX_EINVAL, "in-register normalisation takes boxes up to 32^3");
+    const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+    const int64_t* off = (const int64_t*)bs->d_off[0];
+    if (prec == MGX_F32)
+        hipLaunchKernelGGL((k_box_and_update_nrm<float, 32>), dim3(bs->nbox), dim3(1024), 0, ctx->stream, x, xo,
+            (const float*)kr, a, (const float*)kr0, (float*)kr0_out, off, bs->nbox, th, beta, vel, d);
+    else
+        hipLaunchKernelGGL((k_box_and_update_nrm<double, 32>), dim3(bs->nbox), dim3(1024), 0, ctx->stream, x, xo,
+            (const double*)kr, a, (const double*)kr0, (double*)kr0_out, off, bs->nbox, th, beta, vel, d);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
 int box_psi_step_sq(mgx_ctx* ctx, const mgx_boxset* bs, int prec, double* psi, const void* kr, double a, double* d)
 {
     const double vel = bs->h[0] * bs->h[1] * bs->h[2];
@@ -2670,6 +2747,8 @@ PairTiles boxset_tiles(const mgx_boxset* bs)
 }
 
 int64_t boxset_npairs(const mgx_boxset* bs) { return bs->npairs; }
+
+int boxset_maxedge(const mgx_boxset* bs) { return bs->maxedge; }
 
 bool boxset_uniform_edge(const mgx_boxset* bs, int* edge)
 {

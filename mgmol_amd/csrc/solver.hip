@@ -52,6 +52,9 @@ int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const doub
 int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
+int boxset_maxedge(const mgx_boxset*);
+int box_anderson_update_nrm(mgx_ctx*, const mgx_boxset*, int, const double*, double*, const void*, double,
+    const void*, void*, const double*, double, double*);
 int box_density(mgx_ctx*, const mgx_boxset*, const double*, const double*, int, double, double*, double*);
 
 // y = a x (n values)
@@ -253,6 +256,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     st->dots_sweeps    = (int)s->dots_sweeps;
     const double alpha = 0.5 * s->cfg.step_length;
     bool norms_done    = false;  // d already holds the norms of the updated psi
+    bool normalized    = false;  // and psi is normalised already
     bool dots_done     = false;  // the V-cycle formed the Anderson dots in d
     tk = prof_begin(ctx, MGX_PROF_PRECOND);
     const int prec = s->cfg.precond_dtype;
@@ -321,9 +325,13 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             // the old psi is the next step's x0; the norms of the new psi too.
             // The V-cycle's correction buffer becomes the history (kr0, kr1
             // swap); the network's correction is copied into kr0 instead
+            // boxes up to 32^3: the normalisation in the same pass
             const bool keep = kr == s->kr1;
-            if ((rc = box_anderson_update_sq(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0,
-                     keep ? nullptr : s->kr0, thp, beta, s->d)))
+            normalized      = boxset_maxedge(s->bs) <= 32;
+            if ((rc = normalized ? box_anderson_update_nrm(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0,
+                                       keep ? nullptr : s->kr0, thp, beta, s->d)
+                                 : box_anderson_update_sq(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0,
+                                       keep ? nullptr : s->kr0, thp, beta, s->d)))
                 return rc;
             std::swap(s->psi, s->x0);
             if (keep) std::swap(s->kr0, s->kr1);
@@ -334,9 +342,10 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     prof_end(ctx, tk);
     // LocGridOrbitals::normalize: the norms now (fused into the update when
     // it was not the V-cycle's fused step), the scaling with the next Hpsi
+    // unless the Anderson update has applied it already
     tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
     if (!norms_done && (rc = box_sumsq(ctx, s->bs, s->psi, s->d))) return rc;
-    s->pending_norm = true;
+    s->pending_norm = !normalized;
     prof_end(ctx, tk);
     s->iter++;
     return MGX_OK;
