@@ -1306,7 +1306,7 @@ __device__ __forceinline__ void cell_gram_packed(const double* psi, const double
 // the loads of a step fill the wave (a cell is covered by 5-6 boxes on
 // average when 32^3 boxes cover the grid 4 times).
 template <int PREC, bool PAIR, int NW>
-__global__ void __launch_bounds__(64 * NW) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
+__global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
     const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part,
     int ncell)
 {
@@ -1479,15 +1479,25 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
     const double* skin, const int* pair_ient, const int* pairs, int64_t npairs, int n, double vel, double* S, double* H,
     double* Sv, double* Hv)
 {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x)
+    // eight lanes per pair: lane k sums the slots k, k + 8, ..., then the
+    // eight partial sums meet in a fixed butterfly
+    const int sub = threadIdx.x & 7;
+    for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 3; t < npairs;
+         t += ((int64_t)gridDim.x * blockDim.x) >> 3)
     {
         double g = 0., h1 = 0.;
-        for (int q = pslot_ptr[t]; q < pslot_ptr[t + 1]; q++)
+        for (int q = pslot_ptr[t] + sub; q < pslot_ptr[t + 1]; q += 8)
         {
             const int64_t ab = pslot[2 * q];
             g += part[ab];
             h1 += part[nslot + ab];
         }
+        for (int o = 4; o > 0; o >>= 1)
+        {
+            g += __shfl_xor(g, o, 64);
+            h1 += __shfl_xor(h1, o, 64);
+        }
+        if (sub) continue;
         const int i = pairs[11 * t], j = pairs[11 * t + 1];
         if (i != j)
         {
@@ -2499,7 +2509,7 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
             MGX_CHECK_LAUNCH(ctx);
         }
         const double vel = bs->h[0] * bs->h[1] * bs->h[2];
-        hipLaunchKernelGGL(k_pair_reduce, dim3(nblocks(bs->npairs)), dim3(kBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_pair_reduce, dim3(nblocks(8 * bs->npairs)), dim3(kBlock), 0, ctx->stream,
             (const double*)part, bs->nslot, (const int*)bs->d_pslot_ptr, (const int64_t*)bs->d_pslot,
             (const double*)skin, (const int*)bs->d_pair_ient, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H, Sv,
             Hv);
