@@ -1106,6 +1106,7 @@ __device__ __forceinline__ int64_t cell_idx3(const CellEnt& c, int x, int y, int
 }
 
 constexpr int RJB = 2;  // point tiles per load batch in k_cell_residual and k_cell_density
+constexpr int RJP = 1;  // 32-point tiles per load batch in k_cell_residual with paired loads
 constexpr int kGramWaves = 1;  // waves per cell in k_cell_gram with paired loads (1 measured fastest of 1, 2, 4, 8)
 
 // Per cell: G = A^T A and P = A^T B (A = psi, B = H psi, 512 x k) on MFMA.
@@ -1525,6 +1526,7 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
 // R_b(p) = sum_a psi_a(p) theta(a,b) - hpsi_b(p) for every box b covering
 // point p of the cell, as a 512 x k by k x k product on FP64 MFMA:
 // A = psi gathered per (point, box a), B = theta sub-block from LDS.
+template <bool PAIR>
 __global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
     int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res, double* esq,
     int ncell)
@@ -1554,7 +1556,69 @@ __global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, con
     const int ksteps = (k + 3) / 4;
     // this lane's sums of R^2 for boxes 16 t + r + 4 v (esq: per cell entry)
     double sq[2][4] = { { 0., 0., 0., 0. }, { 0., 0., 0., 0. } };
-    if (k <= 16)
+    if (PAIR && k <= 16)
+    {
+        // even box origins and edges: each lane takes two adjacent z-points
+        // (16-byte loads and stores), which feed two MFMAs with the same
+        // theta operand; 32-point tiles, wave w takes tiles w + 4 jj, RJP
+        // per load batch (the pairs hold twice the registers)
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 z2 = { 0., 0. };
+#pragma unroll 1
+        for (int jb = 0; jb < 4; jb += RJP)
+        {
+            d2 xa[RJP][4], hv[RJP][4];
+            int64_t ib[RJP][4];
+#pragma unroll
+            for (int u = 0; u < RJP; u++)
+            {
+                const int pp = (wid + 4 * (jb + u)) * 32 + 2 * c;
+#pragma unroll
+                for (int st = 0; st < 4; st++)
+                {
+                    const int aa = 4 * st + r;
+                    xa[u][st]    = z2;
+                    if (aa < k)
+                    {
+                        const int64_t ia = cell_idx(ce[aa], pp);
+                        if (ia >= 0) xa[u][st] = *reinterpret_cast<const d2*>(psi + ia);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                {
+                    const int bb = r + 4 * v;
+                    ib[u][v]     = bb < k ? cell_idx(ce[bb], pp) : -1;
+                    hv[u][v]     = ib[u][v] >= 0 ? *reinterpret_cast<const d2*>(hpsi + ib[u][v]) : z2;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < RJP; u++)
+            {
+                d4c acc0 = { 0., 0., 0., 0. }, acc1 = acc0;
+#pragma unroll
+                for (int st = 0; st < 4; st++)
+                    if (st < ksteps)
+                    {
+                        const double t = th[(4 * st + r) * kCellK + c];
+                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[u][st][0], acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[u][st][1], acc1, 0, 0, 0);
+                    }
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    if (ib[u][v] >= 0)
+                    {
+                        d2 rv;
+                        rv[0] = acc0[v] + (-1. * hv[u][v][0]);
+                        rv[1] = acc1[v] + (-1. * hv[u][v][1]);
+                        *reinterpret_cast<d2*>(res + ib[u][v]) = rv;
+                        sq[0][v] += rv[0] * rv[0];
+                        sq[0][v] += rv[1] * rv[1];
+                    }
+            }
+        }
+    }
+    else if (k <= 16)
     {
         // one box tile: the wave's 8 point tiles in batches of RJB, every
         // load of a batch issued before its products (same sums, same order)
@@ -2564,7 +2628,8 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
             esq = (double*)workspace(ctx, 3, (size_t)std::max(bs->nent, 1) * sizeof(double));
             if (!esq) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
         }
-        hipLaunchKernelGGL(k_cell_residual, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, hpsi, theta,
+        hipLaunchKernelGGL(bs->pair_z ? k_cell_residual<true> : k_cell_residual<false>, dim3(cell_grid(bs->ncell)),
+            dim3(256), 0, ctx->stream, psi, hpsi, theta,
             bs->nbox, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
             (const int*)bs->d_cell_ent, res, esq, bs->ncell);
         if (d)
