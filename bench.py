@@ -7,7 +7,10 @@ extrapolation (the reference's default quench solver, m = 1, beta = 1) until
 the residual norm is below 1e-6 Ha.  One "step" is one such converged solve.
 Three paths run on identical inputs:
   plain   : FP64 V-cycle, FP64 S/H, FP64 dense layer;
-  mixed   : FP32 V-cycle, hi/lo-split S/H on FP32 MFMA, FP64 dense layer;
+  mixed   : FP16 V-cycle (FP32 arithmetic, the residual scaled per box, the
+            correction in FP32; the FP32 V-cycle where box edges are not
+            multiples of 8 on every level), hi/lo-split S/H on FP32 MFMA,
+            FP64 dense layer with the FP32 Newton-Schulz correction;
   learned : mixed, with the trained CNN in place of the V-cycle (BF16 MFMA),
             the V-cycle judging each step and taking over when it fails.
 Hpsi, the residual and the energy are FP64 in all three.
@@ -18,7 +21,7 @@ are reported beside it.
 Roofline: the dominant hand-written kernel of the headline path, timed with
 hipEvents on the engine's stream inside the timed solves: the S / H cell
 partials (k_cell_gram, HBM-bound: psi and H psi read once).  roofline_other
-lists the fine-level FP32 Jacobi sweep (HBM) and the CNN inference (BF16
+lists the fine-level Jacobi sweep of the mixed path (HBM) and the CNN inference (BF16
 MFMA, learned steps only); stream is the achievable HBM rate of a plain
 axpy on this device.
 CPU baseline: the oracle (C++/OpenMP restatement of the reference) timed on
@@ -46,16 +49,28 @@ WEIGHTS = os.path.join(ROOT, "mgmol_amd", "weights", "cnn_e32.npz")
 # streaming kernels of known size; profiles/pmc_traffic_c5.md)
 TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895234355, "cell_gram_split": 1416436634}
 
+# the mixed paths take the FP16 V-cycle (FP32 arithmetic, residual scaled per
+# box, FP32 correction) where every box edge is a multiple of 8 on every
+# level, else the FP32 V-cycle; set_mixed_precond picks it for the workload
 PATHS = {
     "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64),
-    "mixed": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT),
-    "learned": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT, precond_kind=mgx.PRECOND_LEARNED),
+    "mixed": dict(precond_dtype=mgx.F16, gram_prec=mgx.PREC_SPLIT),
+    "learned": dict(precond_dtype=mgx.F16, gram_prec=mgx.PREC_SPLIT, precond_kind=mgx.PRECOND_LEARNED),
 }
 LABELS = {
     "plain": "plain FP64 (FP64 V-cycle, FP64 S/H)",
-    "mixed": "mixed (FP32 V-cycle, hi/lo-split S/H)",
-    "learned": "mixed+learned (BF16 CNN preconditioner, V-cycle fallback)",
+    "mixed": "mixed (FP16 V-cycle, hi/lo-split S/H)",
+    "learned": "mixed+learned (BF16 CNN preconditioner, FP16 V-cycle fallback)",
 }
+
+
+def set_mixed_precond(cfg):
+    m = 8 << (cfg.levels - 1)
+    pd = mgx.F16 if all(int(e) % m == 0 for e in cfg.edge) else mgx.F32
+    for k in ("mixed", "learned"):
+        PATHS[k]["precond_dtype"] = pd
+        LABELS[k] = LABELS[k].replace("FP16", "FP32") if pd == mgx.F32 else LABELS[k]
+    return pd
 
 
 def log(*a):
@@ -108,21 +123,28 @@ def cpu_baseline(cfg, iters, mg_levels):
                 sample="1 oracle orbital step on the full workload (%.2f s) x %d iterations" % (t_step, iters))
 
 
-def jacobi_roofline(r, s_bytes, npts):
+def jacobi_roofline(r, precond_dtype, npts):
     ms, launches = r["profile"]["jacobi0"]
     if not launches:
         return None
     # the 3 fine sweeps launched per V-cycle (the zero-start first sweep is
     # pointwise, fused with the cast): read x, f and write x, plus w on the
     # last pre-sweep -> (4 + 3 + 3) / 3 values per point; a last sweep that
-    # also forms the Anderson dots reads the previous correction as well
-    per_launch = (10.0 * launches + r["dots_sweeps"]) / (3.0 * launches) * s_bytes * npts
+    # also forms the Anderson dots reads the previous correction as well.
+    # FP16: 2-byte values, but the last sweep writes (and the dots read) the
+    # FP32 correction: (8 + 6 + 8) / 3 bytes per point, + 4 with the dots
+    if precond_dtype == mgx.F16:
+        per_launch = (22.0 / 3.0 * launches + 4.0 * r["dots_sweeps"]) / launches * npts
+        s_bytes = 2
+    else:
+        s_bytes = 4 if precond_dtype == mgx.F32 else 8
+        per_launch = (10.0 * launches + r["dots_sweeps"]) / (3.0 * launches) * s_bytes * npts
     avg_s = (ms / 1e3) / launches
     achieved = per_launch / avg_s / 1e9
     traffic = TRAFFIC_C5["jacobi_fp32_sweep"] if (s_bytes == 4 and npts == 2048 * 32 ** 3) else None
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="k_box_march<JACOBI> fine-level sweep (%s)" % ("FP32" if s_bytes == 4 else "FP64"),
+                kernel="k_box_march<JACOBI> fine-level sweep (%s)" % {2: "FP16", 4: "FP32", 8: "FP64"}[s_bytes],
                 avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches), bytes_per_launch=int(per_launch))
 
 
@@ -203,6 +225,7 @@ def main():
 
     t0 = time.perf_counter()
     cfg, meta = make_config(args.config, args.seed)
+    set_mixed_precond(cfg)
     log("config %s: %d boxes, %d points, grid %s, h=%.3f (generated in %.1f s)"
         % (args.config, cfg.nbox, cfg.psi.size, cfg.gdim, cfg.h, time.perf_counter() - t0))
     ctx = mgx.Ctx(0)
@@ -225,8 +248,7 @@ def main():
     # (three per step) and the network (learned steps only)
     vc_name = "mixed" if "mixed" in res else headname
     roof = cellgram_roofline(head, npts, PATHS[headname]["gram_prec"] == mgx.PREC_SPLIT)
-    other = [x for x in (jacobi_roofline(res[vc_name], 4 if PATHS[vc_name]["precond_dtype"] == mgx.F32 else 8,
-                                         npts),
+    other = [x for x in (jacobi_roofline(res[vc_name], PATHS[vc_name]["precond_dtype"], npts),
                          cnn_roofline(res["learned"], npts) if "learned" in res else None) if x]
     if roof is None and other:
         roof, other = other[0], other[1:]
@@ -254,7 +276,9 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16+f32+f64" if headname == "learned" else ("f32+f64" if headname == "mixed" else "f64"),
+        "dtype": ({"learned": "bf16+", "mixed": ""}.get(headname, "")
+                  + ("f16+f32+f64" if PATHS[headname]["precond_dtype"] == mgx.F16 else
+                     ("f32+f64" if headname != "plain" else "f64"))),
         "data": "synthetic (seeded Gaussian orbitals and wells)",
         "config": {"workload": "%s: %d orbitals in %s^3 boxes, %d^3 periodic grid, h=%.2f bohr, %d wells, "
                                "centres >= %s bohr apart, ABPG (Anderson m=1) to |R| < %g Ha"

@@ -36,6 +36,9 @@ extern "C" {
 
 #define MGX_F32 0
 #define MGX_F64 1
+/* preconditioner storage only (mgx_box_precond_mg, mgx_solver_cfg_t.precond_dtype):
+   the V-cycle in FP16 with the residual scaled per box, its correction in FP32 */
+#define MGX_F16 2
 
 /* lap_type values of LapFactory<T>::createLap, src/LapFactory.h:26-56 */
 #define MGX_LAP_MEHRSTELLEN 0

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 struct mgx_boxset
 {
@@ -283,6 +284,10 @@ struct MarchArgs
     const void* dkr0 = nullptr;
     double da = 0., dvel = 0.;
     double* dout = nullptr;
+    // JACOBI with an output type other than T (the FP16 V-cycle's last fine
+    // sweep): x_out = x * oinv[j], undoing the box's scale of the residual;
+    // dkr0 then has the output type
+    const double* oinv = nullptr;
 };
 
 struct MarchDots
@@ -433,11 +438,13 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
 // z-neighbours read from LDS as whole V-wide pieces.  LDS rows have a ring of
 // ZR = max(V, 2) in z (so the pieces stay aligned) and 2 in y; the arithmetic
 // per point is that of k_box_march.
-template <typename T, int TYPE, int MODE, int NT, int P, int NB, int V, bool DOTS>
+template <typename T, int TYPE, int MODE, int NT, int P, int NB, int V, bool DOTS, typename TO = T>
 __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __restrict__ in, const T* __restrict__ f,
-    const double* __restrict__ vtot, T* __restrict__ out, T* __restrict__ wout, MarchArgs a, Coef4 c4, Coef2 c2)
+    const double* __restrict__ vtot, TO* __restrict__ out, T* __restrict__ wout, MarchArgs a, Coef4 c4, Coef2 c2)
 {
     typedef T V4 __attribute__((ext_vector_type(V)));
+    typedef TO VO __attribute__((ext_vector_type(V)));
+    constexpr bool mixed = !std::is_same<T, TO>::value;
     typedef T V2 __attribute__((ext_vector_type(2)));
     constexpr int ZR = V > 2 ? V : 2;
     extern __shared__ unsigned char lds_raw[];
@@ -482,7 +489,8 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
     V4 qv[P][5];
     V4 nx[P];
     constexpr bool dots = DOTS && MODE == MARCH_JACOBI;
-    const T* dkr0   = reinterpret_cast<const T*>(a.dkr0);
+    const TO* dkr0  = reinterpret_cast<const TO*>(a.dkr0);
+    const double oj = mixed ? a.oinv[j] : 1.;
     double dacc0 = 0., dacc1 = 0.;
 #pragma unroll
     for (int k = 0; k < P; k++)
@@ -558,15 +566,24 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
 #pragma unroll
             for (int i = 0; i < V; i++)
             {
-                double r;
+                // FP64 arithmetic on FP32 / FP64 values as the reference; FP32
+                // on FP16 values (the FP16 V-cycle is a preconditioner of
+                // 11-bit values: FP64 would only cost issue slots)
+                using C = typename std::conditional<sizeof(T) == 2, float, double>::type;
+                C r;
                 if (TYPE == MGX_LAP_4TH)
                 {
-                    double diag = c4.c0;
-                    if (MODE == MARCH_HLOC) diag = c4.c0 + pot[k][i];
-                    r = diag * (double)cc[i] + c4.c1x * ((double)qv[k][1][i] + (double)qv[k][3][i])
-                        + c4.c1y * ((double)ym1[i] + (double)yp1[i]) + c4.c1z * ((double)zz[i + 1] + (double)zz[i + 3])
-                        + c4.c2x * ((double)qv[k][0][i] + (double)qv[k][4][i])
-                        + c4.c2y * ((double)ym2[i] + (double)yp2[i]) + c4.c2z * ((double)zz[i] + (double)zz[i + 4]);
+                    C diag = (C)c4.c0;
+                    if (MODE == MARCH_HLOC) diag = (C)(c4.c0 + pot[k][i]);
+                    r = diag * (C)cc[i] + (C)c4.c1x * ((C)qv[k][1][i] + (C)qv[k][3][i])
+                        + (C)c4.c1y * ((C)ym1[i] + (C)yp1[i]) + (C)c4.c1z * ((C)zz[i + 1] + (C)zz[i + 3])
+                        + (C)c4.c2x * ((C)qv[k][0][i] + (C)qv[k][4][i])
+                        + (C)c4.c2y * ((C)ym2[i] + (C)yp2[i]) + (C)c4.c2z * ((C)zz[i] + (C)zz[i + 4]);
+                }
+                else if (sizeof(T) == 2)
+                {
+                    r = (C)c2.c0 * (C)cc[i] + (C)c2.c1x * ((C)qv[k][1][i] + (C)qv[k][3][i])
+                        + (C)c2.c1y * ((C)ym1[i] + (C)yp1[i]) + (C)c2.c1z * ((C)zz[i + 1] + (C)zz[i + 3]);
                 }
                 else
                 {
@@ -576,24 +593,36 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
                 if (MODE == MARCH_JACOBI)
                 {
                     T wv = (T)r;
-                    wv += static_cast<T>(-1. * static_cast<double>(fv[i]));
+                    wv += static_cast<T>((C)-1. * static_cast<C>(fv[i]));
                     wv4[i] = wv;
                     T xv   = cc[i];
-                    xv += static_cast<T>(a.mjf * static_cast<double>(wv));
+                    xv += static_cast<T>((C)a.mjf * static_cast<C>(wv));
                     ov[i] = xv;
                 }
                 else
                     ov[i] = (T)r;
             }
             if (MODE == MARCH_JACOBI && wout) *reinterpret_cast<V4*>(wout + gi) = wv4;
-            *reinterpret_cast<V4*>(out + gi) = ov;
+            VO oo;
+            if constexpr (mixed)
+            {
+#pragma unroll
+                for (int i = 0; i < V; i++)
+                    oo[i] = (TO)((double)ov[i] * oj);
+                *reinterpret_cast<VO*>(out + gi) = oo;
+            }
+            else
+            {
+                oo = ov;
+                *reinterpret_cast<V4*>(out + gi) = ov;
+            }
             if (dots)
             {
-                const V4 k0 = *reinterpret_cast<const V4*>(dkr0 + gi);
+                const VO k0 = *reinterpret_cast<const VO*>(dkr0 + gi);
 #pragma unroll
                 for (int i = 0; i < V; i++)
                 {
-                    const double fa = a.da * (double)ov[i], df = fa - a.da * (double)k0[i];
+                    const double fa = a.da * (double)oo[i], df = fa - a.da * (double)k0[i];
                     dacc0 += df * df;
                     dacc1 += df * fa;
                 }
@@ -667,6 +696,7 @@ template <typename T>
 __global__ void __launch_bounds__(kTile / 2) k_box_restrict2p(const T* fine, const int64_t* off_f, T* coarse,
     BoxView vc)
 {
+    static_assert(sizeof(T) >= 4, "16-byte loads of at most four values");
     typedef T vec16 __attribute__((ext_vector_type(16 / sizeof(T))));
     constexpr int VN = 16 / sizeof(T);
     const int64_t t = xcd_block(blockIdx.x, gridDim.x);
@@ -868,6 +898,38 @@ __global__ void k_first_sweep(const double* res, T* f, T* x, int64_t n, double j
         xv += static_cast<T>(jf * static_cast<double>(fv));
         x[t] = xv;
     }
+}
+
+// per-box scale of the residual for the FP16 V-cycle: sc_j brings the box's
+// RMS of R to kF16Rms (FP16 keeps 11 bits between 6e-5 and 65504; the
+// V-cycle is linear, so its FP32 output is divided by sc_j); inv_j = 1/sc_j.
+// dres_j = vel |R_j|^2 (may alias inv)
+constexpr double kF16Rms = 8.;
+__global__ void k_res_scale(const double* dres, const int64_t* off, int nbox, double vel, double* sc, double* inv)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nbox) return;
+    const double ms = dres[j] / (vel * (double)(off[j + 1] - off[j]));
+    const double c  = (ms > 0. && ms < 1e300) ? kF16Rms / sqrt(ms) : 1.;
+    sc[j]           = c;
+    inv[j]          = 1. / c;
+}
+
+// k_first_sweep / k_prep with the box's scale: f = (T)(sc_j res); the zero
+// start writes x0 = 0 + (T)(c f) with c = the Jacobi factor, the gamma start
+// c = gamma
+template <typename T>
+__global__ void k_first_sweep_box(const double* res, const double* sc, T* f, T* x, double c, BoxView v)
+{
+    int j, e, xx, y, z, p;
+    int64_t base;
+    if (!tile_point(v, &j, &e, &xx, &y, &z, &base, &p)) return;
+    const int64_t t = base + p;
+    const T fv      = (T)(sc[j] * res[t]);
+    f[t]            = fv;
+    T xv            = (T)0;
+    xv += static_cast<T>(c * static_cast<double>(fv));
+    x[t] = xv;
 }
 
 template <typename T>
@@ -2168,38 +2230,48 @@ void march_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, const 
             in, f, v, out, w, a, c.c4, c.c2);
 }
 
-template <typename T, int TYPE, int MODE, int NT, int P, bool DOTS>
-void march4_launch_d(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
-    const double* v, T* out, T* w, const Coefs& c)
+// points per lane of k_box_march4: 16 bytes of FP32 / FP64, 4 FP16 values
+// (8 bytes: 16-byte groups leave half the lanes of a 32^3 box idle)
+template <typename T>
+constexpr int march_v()
 {
-    constexpr int V = 16 / sizeof(T);
+    return sizeof(T) == 2 ? 4 : 16 / (int)sizeof(T);
+}
+
+template <typename T, int TYPE, int MODE, int NT, int P, bool DOTS, typename TO>
+void march4_launch_d(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
+    const double* v, TO* out, T* w, const Coefs& c)
+{
+    constexpr int V = march_v<T>();
     // two plane buffers when the blocks that fill a CU (32 waves) still fit
     // its 160 KB of LDS
     if (2 * plane * (2048 / NT) <= 160 * 1024)
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V, DOTS>), dim3(bs->nbox), dim3(NT), 2 * plane,
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V, DOTS, TO>), dim3(bs->nbox), dim3(NT), 2 * plane,
             ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
     else
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V, DOTS>), dim3(bs->nbox), dim3(NT), plane,
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V, DOTS, TO>), dim3(bs->nbox), dim3(NT), plane,
             ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
 }
 
 // the Anderson dots exist for the Jacobi sweep only (a.dout set)
-template <typename T, int TYPE, int MODE, int NT, int P>
+template <typename T, int TYPE, int MODE, int NT, int P, typename TO>
 void march4_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
-    const double* v, T* out, T* w, const Coefs& c)
+    const double* v, TO* out, T* w, const Coefs& c)
 {
     if (MODE == MARCH_JACOBI && a.dout)
-        march4_launch_d<T, TYPE, MODE, NT, P, MODE == MARCH_JACOBI>(ctx, bs, a, plane, in, f, v, out, w, c);
+        march4_launch_d<T, TYPE, MODE, NT, P, MODE == MARCH_JACOBI, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
     else
-        march4_launch_d<T, TYPE, MODE, NT, P, false>(ctx, bs, a, plane, in, f, v, out, w, c);
+        march4_launch_d<T, TYPE, MODE, NT, P, false, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
 }
 
 // the marching sweep for 2nd/4th-order operators; returns false if the box
 // sizes at this level need another kernel
-template <typename T, int TYPE, int MODE>
+template <typename T, int TYPE, int MODE, typename TO = T>
 bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* in, const T* f, const double* v,
-    T* out, T* w, const double* nrm = nullptr, const MarchDots* dots = nullptr)
+    TO* out, T* w, const double* nrm = nullptr, const MarchDots* dots = nullptr, const double* oinv = nullptr)
 {
+    constexpr bool mixed = !std::is_same<T, TO>::value;
+    static_assert(!mixed || MODE == MARCH_JACOBI, "another output type for the Jacobi sweep only");
     const int emax = bs->maxedge >> level;
     const int E2   = emax * emax;
     MarchArgs a;
@@ -2218,9 +2290,13 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
         a.psi_w = (double*)in;
     }
     Coefs c  = coefs_at(bs, level);
-    bool by4 = emax % 4 == 0;
+    // rows split into lane groups of march_v points (at least 4)
+    constexpr int VM = march_v<T>() > 4 ? march_v<T>() : 4;
+    bool by4 = emax % VM == 0;
     for (int jb = 0; by4 && jb < bs->nbox; jb++)
-        by4 = ((bs->edge[jb] >> level) % 4) == 0;
+        by4 = ((bs->edge[jb] >> level) % VM) == 0;
+    a.oinv = oinv;
+    if (mixed && (!by4 || !oinv)) return false;
     if (dots)
     {
         if (!by4) return false;  // the dot products exist in k_box_march4 only
@@ -2232,7 +2308,7 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
     if (by4)
     {
         // V points per thread, z ring ZR (see k_box_march4)
-        constexpr int V    = 16 / sizeof(T);
+        constexpr int V    = march_v<T>();
         constexpr int ZR   = V > 2 ? V : 2;
         a.lds_w            = emax + 2 * ZR;
         const size_t plane = (size_t)a.lds_w * (emax + 4) * sizeof(T);
@@ -2252,6 +2328,10 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
         if (dots) return false;
         a.lds_w = emax + 4;
     }
+    if constexpr (mixed)
+        return false;
+    else
+    {
     if ((size_t)a.lds_w * a.lds_w * sizeof(T) > 60 * 1024) return false;
     if (E2 <= 64)
         march_launch<T, TYPE, MODE, 64, 1>(ctx, bs, a, in, f, v, out, w, c);
@@ -2266,6 +2346,7 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
     else
         return false;
     return true;
+    }
 }
 
 template <typename T>
@@ -2361,13 +2442,19 @@ int box_restrict_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, const T* fine,
 {
     BoxView vc = view(bs, level + 1);
     // pairs of coarse points per thread when every coarse edge is even
+    // (FP32 / FP64: four values of a row per 16 bytes or less)
     bool even = ((uintptr_t)fine & 15) == 0;
     for (int e : bs->edge)
         even = even && ((e >> (level + 1)) & 1) == 0;
-    if (even)
-        hipLaunchKernelGGL(k_box_restrict2p<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile / 2), 0, ctx->stream, fine,
-            (const int64_t*)bs->d_off[level], coarse, vc);
-    else
+    bool done = false;
+    if constexpr (sizeof(T) >= 4)
+        if (even)
+        {
+            hipLaunchKernelGGL(k_box_restrict2p<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile / 2), 0, ctx->stream,
+                fine, (const int64_t*)bs->d_off[level], coarse, vc);
+            done = true;
+        }
+    if (!done)
         hipLaunchKernelGGL(k_box_restrict2<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile), 0, ctx->stream, fine,
             (const int64_t*)bs->d_off[level], coarse, vc);
     MGX_CHECK_LAUNCH(ctx);
@@ -2389,6 +2476,31 @@ struct BoxMG
 {
     std::vector<T*> tmp, w, rc, nv;
 };
+
+// the fine-level Jacobi sweep x (T) -> out (FP32), out = x_new * inv[j]
+// (the FP16 V-cycle's last sweep); the marching kernel only
+template <typename T>
+int box_jacobi_out(mgx_ctx* ctx, const mgx_boxset* bs, int type, double jf, const T* x, const T* f, float* out,
+    const double* inv, const MarchDots* dots, bool* dots_done)
+{
+    if constexpr (std::is_same<T, float>::value)
+        return set_err(ctx, MGX_EINVAL, "FP32 output from an FP32 V-cycle");
+    else
+    {
+        const double mjf = -1. * jf;
+        bool ok          = false;
+        if (type == MGX_LAP_4TH)
+            ok = march<T, MGX_LAP_4TH, MARCH_JACOBI, float>(ctx, bs, 0, mjf, x, f, nullptr, out, nullptr, nullptr,
+                dots, inv);
+        else if (type == MGX_LAP_2ND)
+            ok = march<T, MGX_LAP_2ND, MARCH_JACOBI, float>(ctx, bs, 0, mjf, x, f, nullptr, out, nullptr, nullptr,
+                dots, inv);
+        if (!ok) return set_err(ctx, MGX_EINVAL, "FP16 V-cycle needs the marching kernel (edges multiple of 8 per level)");
+        MGX_CHECK_LAUNCH(ctx);
+        if (dots && dots_done) *dots_done = true;
+        return MGX_OK;
+    }
+}
 
 template <typename T>
 int box_mg_setup(mgx_ctx* ctx, const mgx_boxset* bs, int max_levels, BoxMG<T>& B, int slot)
@@ -2426,7 +2538,8 @@ int box_mg_setup(mgx_ctx* ctx, const mgx_boxset* bs, int max_levels, BoxMG<T>& B
 // result is back in v.
 template <typename T>
 int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, T* v, const T* f, int type,
-    int level, bool first_done, const MarchDots* dots = nullptr, bool* dots_done = nullptr)
+    int level, bool first_done, const MarchDots* dots = nullptr, bool* dots_done = nullptr, float* last_out = nullptr,
+    const double* last_inv = nullptr)
 {
     const int ncycl = (level == max_levels) ? 4 : 2;
     double h[3]     = { bs->h[0] * (1 << level), bs->h[1] * (1 << level), bs->h[2] * (1 << level) };
@@ -2461,9 +2574,14 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
     for (int it = 0; it < 2; it++)
     {
         const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0) : -1;
-        // the last fine sweep (it writes v) may also form the Anderson dots
+        // the last fine sweep (it writes v) may also form the Anderson dots;
+        // with last_out it writes FP32 there instead, times last_inv per box
         const MarchDots* dd = (level == 0 && it == 1) ? dots : nullptr;
-        if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr, dd, dots_done))) return rc;
+        if (level == 0 && it == 1 && last_out)
+            rc = box_jacobi_out(ctx, bs, type, jf, (const T*)a, f, last_out, last_inv, dd, dots_done);
+        else
+            rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr, dd, dots_done);
+        if (rc) return rc;
         prof_end(ctx, tok);
         std::swap(a, b);
     }
@@ -2504,17 +2622,64 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     return MGX_OK;
 }
 
-// the V-cycle of box_precond_t, its correction (type prec) written to xdst
-// (nbox points of the boxes); with kr0 non-null the last fine sweep also
-// forms the Anderson dot products of box_anderson_dots into d (2 nbox) when
-// its kernel can (*dots_done)
+int box_sumsq(mgx_ctx* ctx, const mgx_boxset* bs, const double* r, double* d);
+
+// The V-cycle with FP16 storage (arithmetic in FP64 as in the other types):
+// the residual scaled per box (k_res_scale; dres = vel |R_j|^2 per box, or
+// null to form it here), the zero-start or gamma first sweep with the
+// scale, FP16 sweeps and transfers on every level, and the last fine sweep
+// writing FP32 to xdst with the scale undone (and the Anderson dots, as in
+// the other types, from that FP32 output)
+int box_precond_h(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, const double* res,
+    const double* dres, float* xdst, const MarchDots* dots, bool* dots_done)
+{
+    using T = _Float16;
+    MGX_REQUIRE(ctx, type == MGX_LAP_4TH || type == MGX_LAP_2ND, "FP16 V-cycle: 2nd or 4th order operators only");
+    MGX_REQUIRE(ctx, gamma >= 0., "gamma must be non-negative");
+    for (int l = 0; l <= max_levels; l++)
+        for (int e : bs->edge)
+            MGX_REQUIRE(ctx, ((e >> l) % 8) == 0, "FP16 V-cycle: box edges must be multiples of 8 on every level");
+    const int64_t n = bs->off[0][bs->nbox];
+    T* xf           = (T*)workspace(ctx, 2, 2 * n * sizeof(T));
+    double* sc      = (double*)workspace(ctx, 8, 2 * (size_t)bs->nbox * sizeof(double));
+    if (!xf || !sc) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    double* inv = sc + bs->nbox;
+    T* x        = xf;
+    T* f        = xf + n;
+    int rc;
+    if (!dres)
+    {
+        if ((rc = box_sumsq(ctx, bs, res, inv))) return rc;
+        dres = inv;
+    }
+    const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+    hipLaunchKernelGGL(k_res_scale, dim3((bs->nbox + 255) / 256), dim3(256), 0, ctx->stream, dres,
+        (const int64_t*)bs->d_off[0], bs->nbox, vel, sc, inv);
+    BoxMG<T> B;
+    if ((rc = box_mg_setup(ctx, bs, max_levels, B, 3))) return rc;
+    const bool zero = gamma == 0.;
+    BoxView v       = view(bs, 0);
+    hipLaunchKernelGGL(k_first_sweep_box<T>, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, res,
+        (const double*)sc, f, zero ? B.tmp[0] : x, zero ? jacobi_factor(type, bs->h) : gamma, v);
+    MGX_CHECK_LAUNCH(ctx);
+    return box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0, zero, dots, dots_done, xdst,
+        (const double*)inv);
+}
+
+// the V-cycle of box_precond_t, its correction (type prec; FP32 for
+// MGX_F16) written to xdst (nbox points of the boxes); with kr0 non-null the
+// last fine sweep also forms the Anderson dot products of box_anderson_dots
+// into d (2 nbox) when its kernel can (*dots_done); dres: the residual's
+// per-box vel |R_j|^2 when at hand (FP16 scaling), else null
 int box_precond_x(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
-    const double* res, void* xdst, const void* kr0, double a, double* d, bool* dots_done)
+    const double* res, void* xdst, const void* kr0, double a, double* d, bool* dots_done, const double* dres)
 {
     MarchDots md{ kr0, a, bs->h[0] * bs->h[1] * bs->h[2], d };
     const MarchDots* dots = kr0 ? &md : nullptr;
     if (dots_done) *dots_done = false;
     if (!xdst) return set_err(ctx, MGX_EINVAL, "no destination for the correction");
+    // FP16 storage: the correction comes out in FP32
+    if (prec == MGX_F16) return box_precond_h(ctx, bs, type, max_levels, gamma, res, dres, (float*)xdst, dots, dots_done);
     return prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots,
                                  dots_done, (float*)xdst)
                            : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots,
@@ -3308,8 +3473,19 @@ int mgx_box_precond_mg(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_lev
 {
     BOX_ARGS_OK(ctx, bs, max_levels);
     MGX_REQUIRE(ctx, res && out, "null data");
-    MGX_REQUIRE(ctx, prec == MGX_F32 || prec == MGX_F64, "bad precision");
-    int rc = box_precond(ctx, bs, type, max_levels, gamma, prec, res, out, nullptr, 0.);
+    MGX_REQUIRE(ctx, prec == MGX_F32 || prec == MGX_F64 || prec == MGX_F16, "bad precision");
+    int rc;
+    if (prec == MGX_F16)
+    {
+        const int64_t n = bs->off[0][bs->nbox];
+        float* x32      = (float*)workspace(ctx, 5, n * sizeof(float));
+        if (!x32) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+        if ((rc = box_precond_h(ctx, bs, type, max_levels, gamma, res, nullptr, x32, nullptr, nullptr))) return rc;
+        hipLaunchKernelGGL(k_to_f64<float>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, (const float*)x32, out, n);
+        MGX_CHECK_LAUNCH(ctx);
+        return sync(ctx);
+    }
+    rc = box_precond(ctx, bs, type, max_levels, gamma, prec, res, out, nullptr, 0.);
     return rc ? rc : sync(ctx);
 }
 

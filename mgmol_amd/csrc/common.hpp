@@ -31,7 +31,7 @@ struct mgx_ctx
     std::string err;
     // grow-only device workspaces; a call owns them for its duration (the
     // context is not re-entrant), slot = independent buffer
-    static constexpr int kSlots = 8;
+    static constexpr int kSlots = 9;
     void* ws[kSlots]     = {};
     size_t ws_sz[kSlots] = {};
     double* host_pinned  = nullptr;  // small pinned host buffer for scalars

@@ -48,7 +48,7 @@ int box_anderson_update_sq(mgx_ctx*, const mgx_boxset*, int, const double*, doub
     const void*, void*, const double*, double, double*);
 int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double, double*);
 int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, void*, const void*, double,
-    double*, bool*);
+    double*, bool*, const double*);
 int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
@@ -260,7 +260,9 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     bool dots_done     = false;  // the V-cycle formed the Anderson dots in d
     tk = prof_begin(ctx, MGX_PROF_PRECOND);
     const int prec = s->cfg.precond_dtype;
-    if (!use_cnn && !s->accelerate)
+    // the correction's type: FP32 out of the FP16 V-cycle
+    const int kprec = prec == MGX_F16 ? MGX_F32 : prec;
+    if (!use_cnn && !s->accelerate && prec != MGX_F16)
     {
         // preconditioned step psi += alpha K(R) (ABPG::update_states, no
         // extrapolation), fused with the V-cycle's last kernel
@@ -282,7 +284,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             int edge = 0;
             boxset_uniform_edge(s->bs, &edge);
             const int tl = prof_begin(ctx, MGX_PROF_LEARNED);
-            if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->kcnn, prec))) return rc;
+            if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->kcnn, kprec))) return rc;
             prof_end(ctx, tl);
             kr             = s->kcnn;
             s->cnn_pending = true;
@@ -291,15 +293,17 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         }
         else
         {
+            // d holds the residual's per-box sums unless R was just restored
             if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, s->kr1,
-                     (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done)))
+                     (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done,
+                     revert ? nullptr : s->d)))
                 return rc;
             kr = s->kr1;
         }
         if (!s->accelerate)
         {
             // psi += alpha K(R), with the norms of the new psi
-            if ((rc = box_psi_step_sq(ctx, s->bs, prec, s->psi, kr, alpha, s->d))) return rc;
+            if ((rc = box_psi_step_sq(ctx, s->bs, kprec, s->psi, kr, alpha, s->d))) return rc;
             norms_done = true;
         }
         else
@@ -316,7 +320,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
                     s->dots_sweeps++;
                 }
                 else
-                    rc = box_anderson_dots(ctx, s->bs, prec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2);
+                    rc = box_anderson_dots(ctx, s->bs, kprec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2);
                 if (rc) return rc;
                 thp  = s->scal + 4;
                 beta = s->cfg.anderson_beta;
@@ -328,9 +332,9 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             // boxes up to 32^3: the normalisation in the same pass
             const bool keep = kr == s->kr1;
             normalized      = boxset_maxedge(s->bs) <= 32;
-            if ((rc = normalized ? box_anderson_update_nrm(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0,
+            if ((rc = normalized ? box_anderson_update_nrm(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
                                        keep ? nullptr : s->kr0, thp, beta, s->d)
-                                 : box_anderson_update_sq(ctx, s->bs, prec, s->psi, s->x0, kr, alpha, s->kr0,
+                                 : box_anderson_update_sq(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
                                        keep ? nullptr : s->kr0, thp, beta, s->d)))
                 return rc;
             std::swap(s->psi, s->x0);
@@ -384,7 +388,8 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
 {
     MGX_REQUIRE(ctx, bs && cfg && vtot_host && psi0_host && out, "bad solver arguments");
     MGX_REQUIRE(ctx, cfg->lap_type == MGX_LAP_4TH, "solver supports the 4th-order operator");
-    MGX_REQUIRE(ctx, cfg->precond_dtype == MGX_F32 || cfg->precond_dtype == MGX_F64, "bad precond dtype");
+    MGX_REQUIRE(ctx, cfg->precond_dtype == MGX_F32 || cfg->precond_dtype == MGX_F64 || cfg->precond_dtype == MGX_F16,
+        "bad precond dtype");
     MGX_REQUIRE(ctx, cfg->gram_prec == MGX_PREC_F64 || cfg->gram_prec == MGX_PREC_SPLIT, "bad gram precision");
     MGX_REQUIRE(ctx, cfg->precond_start == MGX_START_GAMMA || cfg->precond_start == MGX_START_ZERO,
         "bad precond start");
@@ -412,15 +417,16 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     const int64_t gsize = (int64_t)g[0] * g[1] * g[2];
     const size_t nn     = (size_t)s->n * s->n;
     auto alloc = [&](double** p, size_t cnt) { return hipMalloc((void**)p, cnt * sizeof(double)) == hipSuccess; };
-    const size_t tsize = cfg->precond_dtype == MGX_F32 ? sizeof(float) : sizeof(double);
+    const size_t tsize = cfg->precond_dtype == MGX_F64 ? sizeof(double) : sizeof(float);
     bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
               && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
               && alloc(&s->eig, s->n) && alloc(&s->d, 2 * (size_t)s->n + 8) && alloc(&s->scal, 8) && alloc(&s->nswork, 2 * nn)
               && alloc(&s->Sv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && (cfg->anderson_m == 0
-                  || (alloc(&s->x0, s->npts) && hipMalloc(&s->kr0, s->npts * tsize) == hipSuccess
-                      && hipMalloc(&s->kr1, s->npts * tsize) == hipSuccess))
+                  || (alloc(&s->x0, s->npts) && hipMalloc(&s->kr0, s->npts * tsize) == hipSuccess))
+              && ((cfg->anderson_m == 0 && cfg->precond_dtype != MGX_F16)
+                  || hipMalloc(&s->kr1, s->npts * tsize) == hipSuccess)
               && (cfg->precond_kind != MGX_PRECOND_LEARNED
                   || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts)
                       && hipMalloc(&s->kcnn, s->npts * tsize) == hipSuccess))

@@ -11,6 +11,7 @@ from . import LIB_PATH
 
 MGX_OK, MGX_EINVAL, MGX_EHIP, MGX_ENOTSPD, MGX_ENUMERIC, MGX_ENOMEM = 0, -1, -2, -3, -4, -5
 F32, F64 = 0, 1
+F16 = 2  # preconditioner storage only: FP16 V-cycle, FP32 correction
 LAP_MEHRSTELLEN, LAP_2ND, LAP_4TH = 0, 1, 2
 PREC_F64, PREC_SPLIT = 0, 1
 START_GAMMA, START_ZERO = 0, 1
