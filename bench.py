@@ -47,7 +47,8 @@ WEIGHTS = os.path.join(ROOT, "mgmol_amd", "weights", "cnn_e32.npz")
 # HBM bytes per launch at C5, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes
 # (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000, both calibrated on
 # streaming kernels of known size; profiles/pmc_traffic_c5.md)
-TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895234355, "cell_gram_split": 1416436634}
+TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895234355, "jacobi_fp16_sweep": 492663945,
+              "cell_gram_split": 1540153344}
 
 # the mixed paths take the FP16 V-cycle (FP32 arithmetic, residual scaled per
 # box, FP32 correction) where every box edge is a multiple of 8 on every
@@ -141,7 +142,9 @@ def jacobi_roofline(r, precond_dtype, npts):
         per_launch = (10.0 * launches + r["dots_sweeps"]) / (3.0 * launches) * s_bytes * npts
     avg_s = (ms / 1e3) / launches
     achieved = per_launch / avg_s / 1e9
-    traffic = TRAFFIC_C5["jacobi_fp32_sweep"] if (s_bytes == 4 and npts == 2048 * 32 ** 3) else None
+    # measured without the Anderson dots (the first 12 steps of a solve)
+    key = {4: "jacobi_fp32_sweep", 2: "jacobi_fp16_sweep"}.get(s_bytes)
+    traffic = TRAFFIC_C5[key] if (key and npts == 2048 * 32 ** 3) else None
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_box_march<JACOBI> fine-level sweep (%s)" % {2: "FP16", 4: "FP32", 8: "FP64"}[s_bytes],

@@ -690,15 +690,15 @@ __global__ void __launch_bounds__(256) k_box_restrict2(const T* fine, const int6
 
 // k_box_restrict2 for two z-adjacent coarse points per thread (every coarse
 // edge even, so a pair never leaves its row): the fine values z = 2z-1 ..
-// 2z+3 of the 9 (x, y) rows around them, the four from 2z in 16-byte loads;
+// 2z+3 of the 9 (x, y) rows around them, the four from 2z in vector loads;
 // per coarse point the same double sums in the same order as k_box_restrict2
 template <typename T>
 __global__ void __launch_bounds__(kTile / 2) k_box_restrict2p(const T* fine, const int64_t* off_f, T* coarse,
     BoxView vc)
 {
-    static_assert(sizeof(T) >= 4, "16-byte loads of at most four values");
-    typedef T vec16 __attribute__((ext_vector_type(16 / sizeof(T))));
-    constexpr int VN = 16 / sizeof(T);
+    // loads of four values (16 bytes FP32, 8 bytes FP16) or two (FP64)
+    constexpr int VN = sizeof(T) == 8 ? 2 : 4;
+    typedef T vec16 __attribute__((ext_vector_type(VN)));
     const int64_t t = xcd_block(blockIdx.x, gridDim.x);
     if (t >= vc.ntiles) return;
     const int j  = vc.tile[2 * t];
@@ -917,19 +917,33 @@ __global__ void k_res_scale(const double* dres, const int64_t* off, int nbox, do
 
 // k_first_sweep / k_prep with the box's scale: f = (T)(sc_j res); the zero
 // start writes x0 = 0 + (T)(c f) with c = the Jacobi factor, the gamma start
-// c = gamma
+// c = gamma.  One block per box, four points per thread and step (box sizes
+// multiples of 4)
 template <typename T>
-__global__ void k_first_sweep_box(const double* res, const double* sc, T* f, T* x, double c, BoxView v)
+__global__ void __launch_bounds__(256) k_first_sweep_box(const double* res, const double* sc, T* f, T* x, double c,
+    const int64_t* off, int nbox)
 {
-    int j, e, xx, y, z, p;
-    int64_t base;
-    if (!tile_point(v, &j, &e, &xx, &y, &z, &base, &p)) return;
-    const int64_t t = base + p;
-    const T fv      = (T)(sc[j] * res[t]);
-    f[t]            = fv;
-    T xv            = (T)0;
-    xv += static_cast<T>(c * static_cast<double>(fv));
-    x[t] = xv;
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    typedef T t4 __attribute__((ext_vector_type(4)));
+    const int j = blockIdx.x;
+    if (j >= nbox) return;
+    const double s   = sc[j];
+    const int64_t q0 = off[j], q1 = off[j + 1];
+    for (int64_t q = q0 + 4 * (int64_t)threadIdx.x; q < q1; q += 4 * (int64_t)blockDim.x)
+    {
+        const d4 r = *reinterpret_cast<const d4*>(res + q);
+        t4 fv, xv;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+        {
+            fv[i]  = (T)(s * r[i]);
+            T xi   = (T)0;
+            xi += static_cast<T>(c * static_cast<double>(fv[i]));
+            xv[i] = xi;
+        }
+        *reinterpret_cast<t4*>(f + q) = fv;
+        *reinterpret_cast<t4*>(x + q) = xv;
+    }
 }
 
 template <typename T>
@@ -2442,19 +2456,13 @@ int box_restrict_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, const T* fine,
 {
     BoxView vc = view(bs, level + 1);
     // pairs of coarse points per thread when every coarse edge is even
-    // (FP32 / FP64: four values of a row per 16 bytes or less)
     bool even = ((uintptr_t)fine & 15) == 0;
     for (int e : bs->edge)
         even = even && ((e >> (level + 1)) & 1) == 0;
-    bool done = false;
-    if constexpr (sizeof(T) >= 4)
-        if (even)
-        {
-            hipLaunchKernelGGL(k_box_restrict2p<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile / 2), 0, ctx->stream,
-                fine, (const int64_t*)bs->d_off[level], coarse, vc);
-            done = true;
-        }
-    if (!done)
+    if (even)
+        hipLaunchKernelGGL(k_box_restrict2p<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile / 2), 0, ctx->stream, fine,
+            (const int64_t*)bs->d_off[level], coarse, vc);
+    else
         hipLaunchKernelGGL(k_box_restrict2<T>, dim3(tile_grid(vc.ntiles)), dim3(kTile), 0, ctx->stream, fine,
             (const int64_t*)bs->d_off[level], coarse, vc);
     MGX_CHECK_LAUNCH(ctx);
@@ -2658,9 +2666,8 @@ int box_precond_h(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     BoxMG<T> B;
     if ((rc = box_mg_setup(ctx, bs, max_levels, B, 3))) return rc;
     const bool zero = gamma == 0.;
-    BoxView v       = view(bs, 0);
-    hipLaunchKernelGGL(k_first_sweep_box<T>, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, res,
-        (const double*)sc, f, zero ? B.tmp[0] : x, zero ? jacobi_factor(type, bs->h) : gamma, v);
+    hipLaunchKernelGGL(k_first_sweep_box<T>, dim3(bs->nbox), dim3(256), 0, ctx->stream, res, (const double*)sc, f,
+        zero ? B.tmp[0] : x, zero ? jacobi_factor(type, bs->h) : gamma, (const int64_t*)bs->d_off[0], bs->nbox);
     MGX_CHECK_LAUNCH(ctx);
     return box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0, zero, dots, dots_done, xdst,
         (const double*)inv);
