@@ -57,11 +57,14 @@ PATHS = {
     "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64),
     "mixed": dict(precond_dtype=mgx.F16, gram_prec=mgx.PREC_SPLIT),
     "learned": dict(precond_dtype=mgx.F16, gram_prec=mgx.PREC_SPLIT, precond_kind=mgx.PRECOND_LEARNED),
+    # not in the default modes: the mixed path with the reference's FP32 V-cycle
+    "mixed32": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT),
 }
 LABELS = {
     "plain": "plain FP64 (FP64 V-cycle, FP64 S/H)",
     "mixed": "mixed (FP16 V-cycle, hi/lo-split S/H)",
     "learned": "mixed+learned (BF16 CNN preconditioner, FP16 V-cycle fallback)",
+    "mixed32": "mixed with the FP32 V-cycle (hi/lo-split S/H)",
 }
 
 
