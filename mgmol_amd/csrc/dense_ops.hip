@@ -769,6 +769,94 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
     return MGX_OK;
 }
 
+// inverse_update_pairs without host waits: nsteps Newton-Schulz steps from
+// X, each recording delta_k = max|I - X_k S| (before step k) in deltas[k]
+// on the device.  The caller reads the deltas later with its other scalars
+// and checks them (ns_spec_check); the steps and their arithmetic are those
+// of inverse_update_pairs.
+int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& pt, double* X, double* work,
+    int nsteps, double tol, double* deltas)
+{
+    const double mone = -1., two = 2.;
+    const size_t nn = (size_t)n * n;
+    double* T   = work;
+    double* cur = X;
+    double* nxt = work + nn;
+    double* bm  = (double*)workspace(ctx, 4, 1024 * sizeof(double));
+    if (!bm) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
+    const bool f32 = tol >= 1e-12;
+    float *E32 = nullptr, *X32 = nullptr, *Y32 = nullptr;
+    if (f32)
+    {
+        E32 = (float*)workspace(ctx, 7, 3 * nn * sizeof(float));
+        if (!E32) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+        X32 = E32 + nn;
+        Y32 = X32 + nn;
+    }
+    int rc;
+    for (int it = 0; it < nsteps; it++)
+    {
+        if ((rc = spmm_pairs(ctx, n, cur, Sv, pt, T))) return rc;
+        const int nb = 512;
+        if (f32)
+            hipLaunchKernelGGL(k_id_err_f32, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T,
+                (const double*)cur, n, bm, E32, X32);
+        else
+            hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
+        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double*)bm, nb, deltas + it);
+        MGX_CHECK_LAUNCH(ctx);
+        if (f32)
+        {
+            const float onef = 1.f, zerof = 0.f;
+            if (rocblas_sgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &onef, E32, n, X32,
+                    n, &zerof, Y32, n)
+                != rocblas_status_success)
+                return set_err(ctx, MGX_EHIP, "rocblas_sgemm failed");
+            hipLaunchKernelGGL(k_add_f32, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream,
+                (const double*)cur, (const float*)Y32, nxt, (int64_t)nn);
+            MGX_CHECK_LAUNCH(ctx);
+        }
+        else if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
+                     rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n,
+                     nxt, rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
+                 != rocblas_status_success)
+            return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex failed");
+        std::swap(cur, nxt);
+    }
+    if (cur != X) MGX_CHECK_HIP(ctx, hipMemcpyAsync(X, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, X, n);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+// The checks inverse_update_pairs makes step by step, on the deltas of
+// inverse_update_spec: 0 = the steps were enough (*needed = the steps the
+// checked loop would have taken), 1 = they converge but more are needed,
+// -1 = they do not converge (Cholesky)
+int ns_spec_check(const double* deltas, int nsteps, double tol, int* needed)
+{
+    double prev = 1e300;
+    for (int k = 0; k < nsteps; k++)
+    {
+        const double d = deltas[k];
+        if (!(d < 0.25) || d > 0.5 * prev) return -1;
+        if (d < tol)
+        {
+            *needed = k;
+            return 0;
+        }
+        if (d * d < tol)
+        {
+            *needed = k + 1;
+            return 0;
+        }
+        prev = d;
+    }
+    *needed = nsteps;
+    return 1;
+}
+
 int theta(mgx_ctx* ctx, int n, const double* Sinv, const double* H, double* th)
 {
     const double one = 1., zero = 0.;

@@ -50,6 +50,8 @@ int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, doub
 int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, void*, const void*, double,
     double*, bool*, const double*);
 int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
+int inverse_update_spec(mgx_ctx*, int, const double*, const PairTiles&, double*, double*, int, double, double*);
+int ns_spec_check(const double*, int, double, int*);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
 int boxset_maxedge(const mgx_boxset*);
@@ -103,6 +105,7 @@ struct mgx_solver
     mgx::PairTiles tiles;
     bool have_inv = false;  // Sinv holds the inverse of a recent S
     int64_t ns_steps = 0, chol_calls = 0, dots_sweeps = 0;
+    int ns_pred = 2;  // Newton-Schulz steps taken without host checks (the last step's need)
     double* host = nullptr;  // pinned: [energy, norm]
 };
 
@@ -147,6 +150,10 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     tk = prof_begin(ctx, MGX_PROF_DENSE);
     // S^-1: Cholesky + inverse (GramMatrix::updateLS / computeInverse) on the
     // first step, then Newton-Schulz refresh from the previous inverse
+    // accuracy of the refreshed inverse: FP64 (1e-15) with FP64 S; with the
+    // split S (entries good to ~1e-8) 1e-10 suffices, one step fewer
+    const double ns_tol = s->cfg.gram_prec == MGX_PREC_SPLIT ? 1e-10 : 1e-15;
+    int spec = 0;  // Newton-Schulz steps taken unchecked, their deltas in scal[8..]
     if (!s->have_inv)
     {
         if ((rc = chol_inv(ctx, n, s->S, s->L, s->Sinv))) return rc;
@@ -155,28 +162,61 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     }
     else
     {
-        int steps = 0;
-        // accuracy of the refreshed inverse: FP64 (1e-15) with FP64 S; with the
-        // split S (entries good to ~1e-8) 1e-10 suffices, one step fewer
-        const double tol = s->cfg.gram_prec == MGX_PREC_SPLIT ? 1e-10 : 1e-15;
-        if ((rc = inverse_update_pairs(ctx, n, s->S, s->Sv, s->tiles, s->L, s->Sinv, s->nswork, &steps, tol)))
+        // as many steps as the last refresh needed, checked with the energy
+        // and the residual norm below (one host wait per step)
+        spec = std::max(1, std::min(s->ns_pred, 4));
+        if ((rc = inverse_update_spec(ctx, n, s->Sv, s->tiles, s->Sinv, s->nswork, spec, ns_tol, s->scal + 8)))
             return rc;
-        if (steps < 0)
-            s->chol_calls++;
-        else
-            s->ns_steps += steps;
     }
-    // theta = S^-1 H with H pair-sparse
-    if ((rc = spmm_pairs(ctx, n, s->Sinv, s->Hv, s->tiles, s->th))) return rc;
-    // tr(H S^-1) = tr(S^-1 H) = tr(theta)   (ProjectedMatrices::getEigSum)
-    if ((rc = diag_sum(ctx, n, s->th, s->scal))) return rc;
-    prof_end(ctx, tk);
-    tk = prof_begin(ctx, MGX_PROF_RESID);
-    if ((rc = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d))) return rc;
-    if ((rc = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return rc;
-    prof_end(ctx, tk);
-    MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // theta, the band energy, the residual and its norm from the current S^-1
+    auto project = [&]() -> int {
+        int r;
+        // theta = S^-1 H with H pair-sparse
+        if ((r = spmm_pairs(ctx, n, s->Sinv, s->Hv, s->tiles, s->th))) return r;
+        // tr(H S^-1) = tr(S^-1 H) = tr(theta)   (ProjectedMatrices::getEigSum)
+        if ((r = diag_sum(ctx, n, s->th, s->scal))) return r;
+        prof_end(ctx, tk);
+        tk = prof_begin(ctx, MGX_PROF_RESID);
+        if ((r = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d))) return r;
+        if ((r = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return r;
+        prof_end(ctx, tk);
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, (8 + spec) * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
+        MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        return MGX_OK;
+    };
+    if ((rc = project())) return rc;
+    if (spec > 0)
+    {
+        int needed     = 0;
+        const int more = ns_spec_check(s->host + 8, spec, ns_tol, &needed);
+        if (more == 0)
+        {
+            s->ns_steps += spec;
+            s->ns_pred = std::max(needed, 1);
+        }
+        else
+        {
+            // the unchecked steps were too few or did not converge: finish
+            // the inverse as inverse_update_pairs does (Cholesky when the
+            // steps diverged), then theta, the residual and its norm again
+            tk = prof_begin(ctx, MGX_PROF_DENSE);
+            int steps = 0;
+            if (more < 0)
+            {
+                if ((rc = chol_inv(ctx, n, s->S, s->L, s->Sinv))) return rc;
+                steps = -1;
+            }
+            else if ((rc = inverse_update_pairs(ctx, n, s->S, s->Sv, s->tiles, s->L, s->Sinv, s->nswork, &steps,
+                          ns_tol)))
+                return rc;
+            s->ns_steps += spec + std::max(steps, 0);
+            if (steps < 0) s->chol_calls++;
+            s->ns_pred = steps < 0 ? 2 : spec + steps;
+            spec       = 0;
+            if ((rc = project())) return rc;
+        }
+    }
     const double e_ry = s->host[0], rnorm = s->host[1];
     if (!std::isfinite(e_ry) || !std::isfinite(rnorm))
         return set_err(ctx, MGX_ENUMERIC, "non-finite energy or residual");
@@ -420,7 +460,7 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     const size_t tsize = cfg->precond_dtype == MGX_F64 ? sizeof(double) : sizeof(float);
     bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
               && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
-              && alloc(&s->eig, s->n) && alloc(&s->d, 2 * (size_t)s->n + 8) && alloc(&s->scal, 8) && alloc(&s->nswork, 2 * nn)
+              && alloc(&s->eig, s->n) && alloc(&s->d, 2 * (size_t)s->n + 8) && alloc(&s->scal, 16) && alloc(&s->nswork, 2 * nn)
               && alloc(&s->Sv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && (cfg->anderson_m == 0
@@ -430,7 +470,7 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
               && (cfg->precond_kind != MGX_PRECOND_LEARNED
                   || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts)
                       && hipMalloc(&s->kcnn, s->npts * tsize) == hipSuccess))
-              && hipHostMalloc((void**)&s->host, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+              && hipHostMalloc((void**)&s->host, 16 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {
         free_solver(s);
@@ -516,6 +556,7 @@ int mgx_solver_reset(mgx_solver* s)
     s->learned_steps = 0;
     s->fallbacks     = 0;
     s->ns_steps      = 0;
+    s->ns_pred       = 2;
     s->chol_calls    = 0;
     s->dots_sweeps   = 0;
     return MGX_OK;
