@@ -1819,6 +1819,7 @@ __global__ void __launch_bounds__(64) k_box_from_entries(const double* esq, cons
 // over the four lanes of a point.  Each point of the periodic grid lies in
 // one cell, so rho is written once per point (points no box covers stay as
 // the caller set them); csum[cell] = the cell's sum of rho, in a fixed order.
+template <bool PAIR>
 __global__ void __launch_bounds__(256) k_cell_density(const double* psi, const double* X, int n, const int64_t* off,
     const int* edge, const int* origin, int g0, int g1, int g2, const int* cell_ptr, const int* cell_ent, double* rho,
     double* csum, int ncell)
@@ -1857,9 +1858,69 @@ __global__ void __launch_bounds__(256) k_cell_density(const double* psi, const d
     const int nct = (k + 15) / 16;
     const int ksteps = (k + 3) / 4;
     double wacc = 0.;
+    if (PAIR && k <= 16)
+    {
+        // even box origins and edges: two adjacent z-points per lane (16-byte
+        // loads and stores), two MFMAs per X operand; 32-point tiles, wave w
+        // takes tiles w + 4 jb
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        const d2 z2 = { 0., 0. };
+#pragma unroll 1
+        for (int jb = 0; jb < 4; jb++)
+        {
+            const int pp = (wid + 4 * jb) * 32 + 2 * c;
+            d2 xa[4];
+#pragma unroll
+            for (int st = 0; st < 4; st++)
+            {
+                const int aa = 4 * st + r;
+                xa[st]       = z2;
+                if (aa < k)
+                {
+                    const int64_t ia = cell_idx(ce[aa], pp);
+                    if (ia >= 0) xa[st] = *reinterpret_cast<const d2*>(psi + ia);
+                }
+            }
+            d4c acc0 = { 0., 0., 0., 0. }, acc1 = acc0;
+#pragma unroll
+            for (int st = 0; st < 4; st++)
+                if (st < ksteps)
+                {
+                    const double t = xs[(4 * st + r) * kCellK + c];
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[st][0], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[st][1], acc1, 0, 0, 0);
+                }
+            double rs0 = 0., rs1 = 0.;
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if (r + 4 * v < k)
+                {
+                    rs0 += acc0[v] * xa[v][0];
+                    rs1 += acc1[v] * xa[v][1];
+                }
+            rs0 += __shfl_xor(rs0, 16, 64);
+            rs1 += __shfl_xor(rs1, 16, 64);
+            rs0 += __shfl_xor(rs0, 32, 64);
+            rs1 += __shfl_xor(rs1, 32, 64);
+            if (r == 0)
+            {
+                // the pair stays in the cell's z-row (cells are 8-aligned)
+                int gx = gc[0] + (pp >> 6), gy = gc[1] + ((pp >> 3) & 7), gz = gc[2] + (pp & 7);
+                if (gx >= g0) gx -= g0;
+                if (gy >= g1) gy -= g1;
+                if (gz >= g2) gz -= g2;
+                d2 o;
+                o[0] = rs0;
+                o[1] = rs1;
+                *reinterpret_cast<d2*>(rho + ((int64_t)gx * g1 + gy) * g2 + gz) = o;
+                wacc += rs0;
+                wacc += rs1;
+            }
+        }
+    }
     // up to 16 boxes (one box tile): point tiles in batches of RJB with all
     // loads of a batch issued first (same sums in the same order)
-    for (int jb = 0; k <= 16 && jb < 8; jb += RJB)
+    for (int jb = 0; !PAIR && k <= 16 && jb < 8; jb += RJB)
     {
         double xa[RJB][4];
 #pragma unroll
@@ -2837,7 +2898,8 @@ int box_density(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const dou
     double* cs       = (double*)workspace(ctx, 6, (bs->ncell + 8) * sizeof(double));
     if (!cs) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(ng)), dim3(kBlock), 0, ctx->stream, rho, ng, 0.);
-    hipLaunchKernelGGL(k_cell_density, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, X, n,
+    hipLaunchKernelGGL(bs->pair_z ? k_cell_density<true> : k_cell_density<false>, dim3(cell_grid(bs->ncell)),
+        dim3(256), 0, ctx->stream, psi, X, n,
         (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_origin, bs->gdim[0], bs->gdim[1],
         bs->gdim[2], (const int*)bs->d_cell_ptr, (const int*)bs->d_cell_ent, rho, cs, bs->ncell);
     const double vel = bs->h[0] * bs->h[1] * bs->h[2];
