@@ -87,3 +87,37 @@ def test_f16_solver_converges_to_the_fp64_energy(ctx):
     assert abs(out["f16"].band_energy - out["plain"].band_energy) <= 5e-6
     # no more than a quarter more steps than the FP64 preconditioner
     assert out["f16"].iterations <= 1.25 * out["plain"].iterations + 10
+
+
+def test_f16_learned_fallback_to_the_fp16_vcycle(ctx):
+    """The learned path over the FP16 V-cycle: a network that points uphill
+    is rejected, psi and R are restored and the FP16 V-cycle step is taken
+    (its box scales then formed from the restored R), after which the run
+    follows the plain FP16 run.  The scales come from sums in another order
+    there, so the match is to the FP16 V-cycle's accuracy, not bitwise."""
+    from mgmol_amd import learned
+    cfg = _cfg()
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    p = learned.init_params(seed=0, skip=-10.0)
+    for k in range(4):
+        p["w%d" % k][:] = 0.0
+    net = learned.Cnn(ctx, 32, 8, p)
+    kw = dict(mg_levels=2, precond_dtype=mgx.F16)
+    plain = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, **kw)
+    lrn = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, precond_kind=mgx.PRECOND_LEARNED, cnn=net, **kw)
+    try:
+        ep = [plain.step().band_energy for _ in range(6)]
+        sts = [lrn.step() for _ in range(6)]
+        el = [st.band_energy for st in sts]
+        say("plain f16", ep)
+        say("learned f16", el)
+        assert el[0] == pytest.approx(ep[0], rel=1e-13)
+        assert el[1] > el[0]  # the uphill network step
+        assert sts[1].fallbacks == 1 and sts[1].learned_steps == 1
+        for k in range(1, 5):
+            assert el[k + 1] == pytest.approx(ep[k], rel=1e-6)
+    finally:
+        for s in (plain, lrn):
+            s.close()
+        net.close()
+        bs.close()
