@@ -182,7 +182,7 @@ void prof_collect(mgx_ctx* ctx);
 // (i, j) only for interacting pairs): tile t covers kSpTile columns (boxes
 // grouped by space), ptr[t]..ptr[t+1] index the union of their rows k, and
 // e[kSpTile*u + q] is the pair whose value sits at (k_u, col_q), or -1.
-constexpr int kSpTile = 8;
+constexpr int kSpTile = 4;  // measured at C5: 4 columns per tile 0.90, 2: 0.92, 8: 0.97, 16: 1.08 ms of dense phase per step
 struct PairTiles
 {
     int ntile      = 0;
