@@ -290,6 +290,30 @@ __global__ void __launch_bounds__(256) k_max_reduce(const double* v, int n, doub
 }
 
 // X = (X + X^T)/2
+// X = (X + X^T)/2 by 32 x 32 tile pairs through LDS (coalesced on both
+// sides); each pair of entries gets 0.5 (x_pq + x_qp), as k_symmetrize
+__global__ void __launch_bounds__(256) k_symmetrize_tiles(double* X, int n)
+{
+    __shared__ double a[32][33], b[32][33];
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (tj < ti) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8)
+    {
+        // a[k][tx] = X(ti*32 + tx, tj*32 + k), b[k][tx] = X(tj*32 + tx, ti*32 + k)
+        const int r = ti * 32 + tx, c = tj * 32 + k, r2 = tj * 32 + tx, c2 = ti * 32 + k;
+        a[k][tx] = (r < n && c < n) ? X[r + (int64_t)c * n] : 0.;
+        b[k][tx] = (r2 < n && c2 < n) ? X[r2 + (int64_t)c2 * n] : 0.;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8)
+    {
+        const int r = ti * 32 + tx, c = tj * 32 + k, r2 = tj * 32 + tx, c2 = ti * 32 + k;
+        if (r < n && c < n) X[r + (int64_t)c * n] = 0.5 * (a[k][tx] + b[tx][k]);
+        if (r2 < n && c2 < n) X[r2 + (int64_t)c2 * n] = 0.5 * (b[k][tx] + a[tx][k]);
+    }
+}
+
 __global__ void k_symmetrize(double* X, int n)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)n * n;
@@ -825,7 +849,7 @@ int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& 
         std::swap(cur, nxt);
     }
     if (cur != X) MGX_CHECK_HIP(ctx, hipMemcpyAsync(X, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, X, n);
+    hipLaunchKernelGGL(k_symmetrize_tiles, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, ctx->stream, X, n);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
