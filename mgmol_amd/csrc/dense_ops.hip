@@ -277,9 +277,11 @@ __device__ __forceinline__ uint16_t bf16_rn(double v)
 __device__ __forceinline__ double bf16_val(uint16_t h) { return (double)__uint_as_float((uint32_t)h << 16); }
 
 // k_id_err with E = I - T and X as BF16 hi/lo pairs (hi = bf16(v),
-// lo = bf16(v - hi)), the operands of the split BF16 correction product
+// lo = bf16(v - hi)), laid out as the operands of one product with K = 3n:
+// A = [Eh Eh El] (n x 3n) and B = [Xh; Xl; Xh] (3n x n), both column-major,
+// so that A B = Eh Xh + Eh Xl + El Xh
 __global__ void __launch_bounds__(256) k_id_err_bf16(const double* T, const double* X, int n, double* blockmax,
-    uint16_t* Eh, uint16_t* El, uint16_t* Xh, uint16_t* Xl)
+    uint16_t* A, uint16_t* B)
 {
     __shared__ double red[256];
     double m = 0.;
@@ -290,13 +292,17 @@ __global__ void __launch_bounds__(256) k_id_err_bf16(const double* T, const doub
         const double e = (row == col ? 1. : 0.) - T[t];
         const double v = fabs(e);
         m              = v > m ? v : m;
+        const int64_t nn  = (int64_t)n * n;
         const uint16_t eh = bf16_rn(e);
-        Eh[t]             = eh;
-        El[t]             = bf16_rn(e - bf16_val(eh));
+        A[t]              = eh;
+        A[nn + t]         = eh;
+        A[2 * nn + t]     = bf16_rn(e - bf16_val(eh));
         const double x    = X[t];
         const uint16_t xh = bf16_rn(x);
-        Xh[t]             = xh;
-        Xl[t]             = bf16_rn(x - bf16_val(xh));
+        const int64_t bj  = (int64_t)col * 3 * n + row;
+        B[bj]             = xh;
+        B[bj + n]         = bf16_rn(x - bf16_val(xh));
+        B[bj + 2 * n]     = xh;
     }
     red[threadIdx.x] = m;
     __syncthreads();
@@ -851,8 +857,8 @@ int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& 
     double* bm  = (double*)workspace(ctx, 4, 1024 * sizeof(double));
     if (!bm) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
-    // the correction E X at the split tolerance: three BF16 products of the
-    // hi/lo parts (Eh Xh + Eh Xl + El Xh, FP32 sums) -- relative error ~1e-5
+    // the correction E X at the split tolerance: one BF16 product with K = 3n
+    // of the hi/lo parts (Eh Xh + Eh Xl + El Xh, FP32 sums) -- relative error ~1e-5
     // of |E| |X|, below |E|^2 while |E| > 1e-5; the deltas measured in FP64
     // before each step decide the step count either way
     const bool f32 = tol >= 1e-12;
@@ -860,7 +866,7 @@ int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& 
     uint16_t* H16  = nullptr;
     if (f32)
     {
-        Y32 = (float*)workspace(ctx, 7, nn * sizeof(float) + 4 * nn * sizeof(uint16_t));
+        Y32 = (float*)workspace(ctx, 7, nn * sizeof(float) + 6 * nn * sizeof(uint16_t));
         if (!Y32) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
         H16 = reinterpret_cast<uint16_t*>(Y32 + nn);
     }
@@ -871,7 +877,7 @@ int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& 
         const int nb = 512;
         if (f32)
             hipLaunchKernelGGL(k_id_err_bf16, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T,
-                (const double*)cur, n, bm, H16, H16 + nn, H16 + 2 * nn, H16 + 3 * nn);
+                (const double*)cur, n, bm, H16, H16 + 3 * nn);
         else
             hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
         hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double*)bm, nb, deltas + it);
@@ -879,16 +885,12 @@ int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& 
         if (f32)
         {
             const float onef = 1.f, zerof = 0.f;
-            const uint16_t *Eh = H16, *El = H16 + nn, *Xh = H16 + 2 * nn, *Xl = H16 + 3 * nn;
-            const void* As[3] = { Eh, Eh, El };
-            const void* Bs[3] = { Xh, Xl, Xh };
-            for (int q = 0; q < 3; q++)
-                if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &onef, As[q],
-                        rocblas_datatype_bf16_r, n, Bs[q], rocblas_datatype_bf16_r, n, q ? &onef : &zerof, Y32,
-                        rocblas_datatype_f32_r, n, Y32, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
-                        rocblas_gemm_algo_standard, 0, 0)
-                    != rocblas_status_success)
-                    return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex (bf16) failed");
+            if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, 3 * n, &onef, H16,
+                    rocblas_datatype_bf16_r, n, H16 + 3 * nn, rocblas_datatype_bf16_r, 3 * n, &zerof, Y32,
+                    rocblas_datatype_f32_r, n, Y32, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
+                    rocblas_gemm_algo_standard, 0, 0)
+                != rocblas_status_success)
+                return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex (bf16) failed");
             hipLaunchKernelGGL(k_add_f32, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream,
                 (const double*)cur, (const float*)Y32, nxt, (int64_t)nn);
             MGX_CHECK_LAUNCH(ctx);
