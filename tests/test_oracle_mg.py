@@ -185,3 +185,114 @@ def test_precond_constant_and_gamma():
     assert np.max(np.abs(out64 - 0.5 * (gam + 4 * (jf0 + jf1 + jf2)))) < 1e-13
     out32 = O.precond(g, 2, 2, gam, res, dtype=np.float32)
     assert np.max(np.abs(out32 - out64)) < 1e-6 * np.max(np.abs(out64))
+
+
+# ---------------------------------------------------------------------------
+# Two-level V-cycles on one Fourier mode, in closed form.
+#
+# On a periodic grid every stencil is diagonal in Fourier space, full
+# weighting maps the fine mode cos(k.r) to the coarse mode cos(k.r_c) times
+# W_0 = prod_d (1 + cos k_d h) / 2, and trilinear prolongation maps the coarse
+# mode to the 8 fine modes cos((k + pi/h (.) a).r), a in {0,1}^3, with weights
+# W_a = prod_d (1 + (-1)^a_d cos k_d h) / 2 (the tensor product of the 1-D
+# prolongation symbol).  A two-level cycle on f = cos(k.r) from x = gamma f is
+# then: Jacobi sweeps c <- c - jf (lam c - 1) on the mode; the restricted
+# residual r_c = W_0 w, with w = lam c - 1 taken either before the last
+# pre-sweep's update (the stale residual the last `jacobi` call leaves in w,
+# Preconditioning::mg, src/Preconditioning.cc:170-186) or after it (the fresh
+# residual A x - rhs of pb::Vcycle, src/pb/Vcycle.h:28-249); four coarse
+# sweeps from zero with the coarse operator's symbol; x -= P e, which puts
+# -W_a v_c on mode a; and the post-sweeps on every mode.  A dropped sweep, a
+# wrong restriction or prolongation weight, a sign of the correction, or a
+# fresh residual where a stale one belongs each change the predicted output by
+# far more than the 1e-12 tolerance (checked below for the residual).
+# ---------------------------------------------------------------------------
+def _sym4(k, h):
+    return sum((30 - 32 * np.cos(kk * hh) + 2 * np.cos(2 * kk * hh)) / (12 * hh * hh) for kk, hh in zip(k, h))
+
+
+def _sym2(k, h):
+    return sum((2 - 2 * np.cos(kk * hh)) / (hh * hh) for kk, hh in zip(k, h))
+
+
+def _two_level_modes(k, h, gam, stale, nu1=2, nu2=2, ncoarse=4):
+    """[(coefficient, wave vector)] of the cycle's output on f = cos(k.r)."""
+    hc = [2 * x for x in h]
+    jf = O.jacobi_factor(2, h)
+    jfc = O.jacobi_factor(1, hc)
+    lam0 = _sym4(k, h)
+    c, w = gam, None
+    for _ in range(nu1):
+        w = lam0 * c - 1.0
+        c = c - jf * w
+    if not stale:
+        w = lam0 * c - 1.0
+    w0 = np.prod([(1 + np.cos(kk * hh)) / 2 for kk, hh in zip(k, h)])
+    rc = w0 * w
+    lamc = _sym2(k, hc)
+    vc = 0.0
+    for _ in range(ncoarse):
+        vc = vc - jfc * (lamc * vc - rc)
+    out = []
+    for a in np.ndindex(2, 2, 2):
+        wa = np.prod([(1 + (-1) ** ad * np.cos(kk * hh)) / 2 for ad, kk, hh in zip(a, k, h)])
+        ka = [kk + np.pi / hh * ad for ad, kk, hh in zip(a, k, h)]
+        lam = _sym4(ka, h)
+        ca = (c if a == (0, 0, 0) else 0.0) - wa * vc
+        for _ in range(nu2):
+            ca = ca - jf * (lam * ca - (1.0 if a == (0, 0, 0) else 0.0))
+        out.append((ca, ka))
+    return out
+
+
+def _eval_modes(modes, dims, h):
+    x = [np.arange(dims[i]) * h[i] for i in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    return sum(c * np.cos(ka[0] * X + ka[1] * Y + ka[2] * Z) for c, ka in modes)
+
+
+@pytest.mark.parametrize("m", [(1, 2, 0), (1, 1, 1), (3, 0, 2)])
+@pytest.mark.parametrize("gam", [0.0, 0.013])
+def test_vcycle_two_level_fourier_mode_stale_residual(m, gam):
+    dims = (16, 16, 16)
+    g = _g(dims, ng=2, ll=4.8)
+    h = list(g.h)
+    k = [2 * np.pi * mm / (dims[i] * h[i]) for i, mm in enumerate(m)]
+    x = [np.arange(dims[i]) * h[i] for i in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    u = np.cos(k[0] * X + k[1] * Y + k[2] * Z)
+    f = O.embed(g, u[None])
+    v = np.zeros_like(f)
+    O.axpy(gam, f, v)
+    O.mg(g, 2, 1, v, f)
+    out = O.interior(g, v)[0]
+    stale = _eval_modes(_two_level_modes(k, h, gam, stale=True), dims, h)
+    fresh = _eval_modes(_two_level_modes(k, h, gam, stale=False), dims, h)
+    scale = np.max(np.abs(stale))
+    assert np.max(np.abs(out - stale)) < 1e-12 * scale
+    # the closed form tells the two schedules apart by far more than its tolerance
+    assert np.max(np.abs(fresh - stale)) > 1e-4 * scale
+
+
+@pytest.mark.parametrize("m", [(1, 2, 0), (2, 1, 1)])
+def test_poisson_vcycle_two_level_fourier_mode_fresh_residual(m):
+    """pb::Vcycle with one coarsening (max_nlevels = 1): nu1 = nu2 = 2 fine
+    sweeps around the fresh residual's coarse correction, nu1 + nu2 = 4
+    sweeps from zero on the coarsest level (its operator Laph2)."""
+    dims = (16, 16, 16)
+    g = _g(dims, ng=2, ll=4.8)
+    h = list(g.h)
+    k = [2 * np.pi * mm / (dims[i] * h[i]) for i, mm in enumerate(m)]
+    x = [np.arange(dims[i]) * h[i] for i in range(3)]
+    X, Y, Z = np.meshgrid(*x, indexing="ij")
+    u = np.cos(k[0] * X + k[1] * Y + k[2] * Z)
+    rhs = O.embed(g, u[None])
+    O.trade(g, rhs)
+    w = np.zeros_like(rhs)
+    O._vcycle(g, 2, w, rhs, 0, 1, 2, 2)
+    out = O.interior(g, w)[0]
+    fresh = _eval_modes(_two_level_modes(k, h, 0.0, stale=False), dims, h)
+    stale = _eval_modes(_two_level_modes(k, h, 0.0, stale=True), dims, h)
+    scale = np.max(np.abs(fresh))
+    assert np.max(np.abs(out - fresh)) < 1e-12 * scale
+    assert np.max(np.abs(fresh - stale)) > 1e-4 * scale
