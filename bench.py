@@ -5,27 +5,32 @@ Workload (default c5): 2048 localized orbitals in cubic 32^3 boxes on a
 orbitals (centres at least 2 bohr apart); ABPG steps with Anderson
 extrapolation (the reference's default quench solver, m = 1, beta = 1) until
 the residual norm is below 1e-6 Ha.  One "step" is one such converged solve.
-Three paths run on identical inputs:
-  plain   : FP64 V-cycle, FP64 S/H, FP64 dense layer;
-  mixed   : FP16 V-cycle (FP32 arithmetic, the residual scaled per box, the
-            correction in FP32; the FP32 V-cycle where box edges are not
-            multiples of 8 on every level), hi/lo-split S/H on FP32 MFMA,
-            FP64 dense layer with the FP32 Newton-Schulz correction;
-  learned : mixed, with the trained CNN in place of the V-cycle (BF16 MFMA),
-            the V-cycle judging each step and taking over when it fails.
-Hpsi, the residual and the energy are FP64 in all three.
+Paths, on identical inputs:
+  plain   : FP64 V-cycle, FP64 S/H, S^-1 refreshed with FP64 Newton-Schulz
+            products to 1e-15;
+  mixed   : FP32 V-cycle (the reference's MGPRECONDTYPE), hi/lo-split S/H on
+            FP32 MFMA with FP64 totals, S^-1 refreshed with FP64 Newton-Schulz
+            products (to 1e-10, the split S's own accuracy) and Cholesky in
+            FP64 where the refresh fails;
+  learned : mixed, with the trained BF16 CNN tried in place of the V-cycle
+            (a plain preconditioned step, compared with the V-cycle step
+            from the same state; the V-cycle takes over after a rejection);
+  ext16   : an extension beyond the specified precisions: FP16 V-cycle
+            storage (FP32 arithmetic, residual scaled per box) and the
+            Newton-Schulz correction as one BF16 hi/lo product.
+H psi, the residual and the energy are FP64 in all paths.
 value = seconds per converged solve of the mixed+learned path (lower is
-better); plain and mixed times, iteration counts and band-energy differences
-are reported beside it.
+better), timed with kernel profiling off; the other paths' times, iteration
+counts and band-energy differences are reported beside it.
 
-Roofline: the dominant hand-written kernel of the headline path, timed with
-hipEvents on the engine's stream inside the timed solves: the S / H cell
-partials (k_cell_gram, HBM-bound: psi and H psi read once).  roofline_other
-lists the fine-level Jacobi sweep of the mixed path (HBM) and the CNN inference (BF16
-MFMA, learned steps only); stream is the achievable HBM rate of a plain
-axpy on this device.
+Roofline: every kernel group the solver brackets with hipEvents (one
+profiled solve per path after the timed ones) carries its algorithmic bytes
+(or flops); the headline path's groups are ranked by time per solve, the
+top one is `roofline`, the others above 5% of a solve are `roofline_other`.
+`stream` is the measured HBM ceiling: a STREAM triad with 16-byte lanes.
 CPU baseline: the oracle (C++/OpenMP restatement of the reference) timed on
-this host for one full-size orbital step, times the iteration count.
+this host for a bounded number of full-size orbital steps (about 12 s),
+scaled by the headline path's iteration count.
 """
 import argparse
 import json
@@ -41,40 +46,51 @@ sys.path.insert(0, ROOT)
 from mgmol_amd import mgx, synth  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-BF16_PEAK_TFS = 2500.0
-CNN_FLOP_PER_VOXEL = 2 * 27 * (1 * 16 + 16 * 16 + 16 * 16 + 16 * 1)
+PEAK_TFS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
 WEIGHTS = os.path.join(ROOT, "mgmol_amd", "weights", "cnn_e32.npz")
-# HBM bytes per launch at C5, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes
-# (FETCH_SIZE scaled by 2.000, WRITE_SIZE by 1.000, both calibrated on
-# streaming kernels of known size; profiles/pmc_traffic_c5.md)
-TRAFFIC_C5 = {"cnn_application": 17287778496, "jacobi_fp32_sweep": 895234355, "jacobi_fp16_sweep": 492663945,
-              "cell_gram_split": 1540153344}
+C5_POINTS = 2048 * 32 ** 3
+# HBM bytes per launch at C5 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, both calibrated on
+# streaming kernels of known size), copied from profiles/pmc_traffic_c5.md;
+# keyed by (group, precision of the path's kernel)
+TRAFFIC_C5 = {("jacobi0", "f32"): 895234355, ("jacobi0", "f16"): 492663945, ("cellgram", "split"): 1540153344,
+              ("residk", "f64"): 1774300000, ("learned", "bf16"): 17287778496}
+TRAFFIC_SOURCE = "profiles/pmc_traffic_c5.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
 
-# the mixed paths take the FP16 V-cycle (FP32 arithmetic, residual scaled per
-# box, FP32 correction) where every box edge is a multiple of 8 on every
-# level, else the FP32 V-cycle; set_mixed_precond picks it for the workload
 PATHS = {
-    "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64),
-    "mixed": dict(precond_dtype=mgx.F16, gram_prec=mgx.PREC_SPLIT),
-    "learned": dict(precond_dtype=mgx.F16, gram_prec=mgx.PREC_SPLIT, precond_kind=mgx.PRECOND_LEARNED),
-    # not in the default modes: the mixed path with the reference's FP32 V-cycle
-    "mixed32": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT),
+    "plain": dict(precond_dtype=mgx.F64, gram_prec=mgx.PREC_F64, inverse_mode=mgx.INV_NS_F64),
+    "mixed": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT, inverse_mode=mgx.INV_NS_F64),
+    "learned": dict(precond_dtype=mgx.F32, gram_prec=mgx.PREC_SPLIT, inverse_mode=mgx.INV_NS_F64,
+                    precond_kind=mgx.PRECOND_LEARNED),
+    "ext16": dict(precond_dtype=mgx.F16, gram_prec=mgx.PREC_SPLIT, inverse_mode=mgx.INV_NS_BF16),
 }
 LABELS = {
-    "plain": "plain FP64 (FP64 V-cycle, FP64 S/H)",
-    "mixed": "mixed (FP16 V-cycle, hi/lo-split S/H)",
-    "learned": "mixed+learned (BF16 CNN preconditioner, FP16 V-cycle fallback)",
-    "mixed32": "mixed with the FP32 V-cycle (hi/lo-split S/H)",
+    "plain": "plain FP64 (FP64 V-cycle, FP64 S/H, FP64 Newton-Schulz S^-1)",
+    "mixed": "mixed (FP32 V-cycle, hi/lo-split S/H, FP64 Newton-Schulz / Cholesky S^-1)",
+    "learned": "mixed+learned (BF16 CNN preconditioner with the FP32 V-cycle as judge and fallback, "
+               "hi/lo-split S/H, FP64 S^-1)",
+    "ext16": "extension, not the specified precisions: FP16-storage V-cycle, hi/lo-split S/H, "
+             "S^-1 Newton-Schulz correction as one BF16 hi/lo product",
 }
+DTYPES = {"plain": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64", "ext16": "f16+bf16+f32+f64"}
 
-
-def set_mixed_precond(cfg):
-    m = 8 << (cfg.levels - 1)
-    pd = mgx.F16 if all(int(e) % m == 0 for e in cfg.edge) else mgx.F32
-    for k in ("mixed", "learned"):
-        PATHS[k]["precond_dtype"] = pd
-        LABELS[k] = LABELS[k].replace("FP16", "FP32") if pd == mgx.F32 else LABELS[k]
-    return pd
+# kernel groups timed by the solver: rocprof kernel name (C5 instantiation)
+# and what bounds it; unit of the accumulated work: bytes ("hbm") or flops
+GROUPS = {
+    "hloc": ("k_box_march4<double, 2, 1, 512, ...> H psi (FP64, V gathered per box)", "hbm"),
+    "cellgram": ("k_cell_gram<PREC, true, 1> S/H partials per 8^3 cell (MFMA)", "hbm"),
+    "skin": ("k_slab_skin Galerkin H outside each box", "gather"),
+    "residk": ("k_cell_residual<true> R = psi theta - H psi per 8^3 cell (FP64 MFMA)", "hbm"),
+    "update": ("k_box_and_update_nrm<T, 32> Anderson update + normalisation", "hbm"),
+    "jacobi0": ("k_box_march4<T, 2, 2, ...> fine-level Jacobi sweep", "hbm"),
+    "jacobi0d": ("k_box_march4<T, 2, 2, ..., DOTS=true> last fine sweep with the Anderson dots", "hbm"),
+    "first": ("k_first_sweep<T> zero-start first fine sweep (pointwise)", "hbm"),
+    "transfer": ("k_box_restrict2p<T> + k_box_prolong8<T, true> fine-level transfers (2 kernels)", "hbm"),
+    "spmm": ("k_spmm_tiles pair-sparse X S and S^-1 H", "hbm"),
+    "nsgemm": ("rocBLAS dgemm (Cijk_..._DB_MT64x64x16) Newton-Schulz X <- 2X - (XS)X", "mfma:f64"),
+    "nsgemm16": ("rocBLAS gemm_ex BF16 (Cijk_..._BSS_BH_...) Newton-Schulz correction, K = 3n", "mfma:bf16"),
+    "learned": ("CNN inference (4 conv3d layers, BF16 MFMA) over all boxes", "mfma:bf16"),
+}
 
 
 def log(*a):
@@ -88,15 +104,22 @@ def make_config(name, seed):
     return synth.box_config(seed=seed, **c), c
 
 
+def fit_precond(cfg):
+    """FP16 storage needs box edges that are multiples of 8 on every level."""
+    m = 8 << (cfg.levels - 1)
+    if not all(int(e) % m == 0 for e in cfg.edge):
+        PATHS["ext16"]["precond_dtype"] = mgx.F32
+        LABELS["ext16"] = LABELS["ext16"].replace("FP16-storage V-cycle", "FP32 V-cycle (edges not multiples of 8)")
+
+
 def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
-    kw = dict(mg_levels=cfg.levels - 1, profile=1, precond_start=mgx.START_ZERO, anderson_m=1,
+    kw = dict(mg_levels=cfg.levels - 1, profile=0, precond_start=mgx.START_ZERO, anderson_m=1,
               conv_criterion=mgx.CONV_RESIDUAL)
     kw.update(PATHS[mode])
     s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, cnn=cnn if mode == "learned" else None, **kw)
     for _ in range(warmup):
         s.run(tol, 3)
         s.reset()
-    s.profile(reset=True)
     times, stats = [], None
     for k in range(steps):
         s.reset()
@@ -106,121 +129,137 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
         log("[%s] solve %d: %d iterations, |R|=%.3e Ha, E=%.12f Ha, %.3f s (converged=%d, learned steps %d, "
             "fallbacks %d)" % (mode, k, stats.iterations, stats.residual, stats.band_energy, times[-1],
                                stats.converged, stats.learned_steps, stats.fallbacks))
-    prof = s.profile()
-    s.close()
-    return dict(seconds=float(np.mean(times)), iterations=stats.iterations, residual=stats.residual,
-                energy=stats.band_energy, converged=bool(stats.converged), learned_steps=stats.learned_steps,
-                fallbacks=stats.fallbacks, newton_schulz_steps=stats.ns_steps, cholesky_calls=stats.chol_calls,
-                dots_sweeps=stats.dots_sweeps, profile=prof)
-
-
-def cpu_baseline(cfg, iters, mg_levels):
-    """One full-size orbital step of the oracle (FP32 V-cycle as the reference's
-    MGPRECONDTYPE) on this host, scaled by the iteration count."""
-    import oracle as O
-    ob = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    # one more solve with the kernel groups timed (hipEvents on the engine's stream)
+    s.reset()
+    s.set_profile(True)
+    s.profile(reset=True)
     t0 = time.perf_counter()
-    O.b_solver_step(ob, cfg.psi.copy(), cfg.vtot, mg_levels, O.F32, 1.5, None, zero_start=True)
-    t_step = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    return dict(value=round(t_step * iters, 2), unit="s per converged solve", cores=cores, kind="port",
-                sample="1 oracle orbital step on the full workload (%.2f s) x %d iterations" % (t_step, iters))
+    pst = s.run(tol, maxit)
+    t_prof = time.perf_counter() - t0
+    prof = s.profile()
+    log("[%s] profiled solve: %d iterations, %.3f s" % (mode, pst.iterations, t_prof))
+    s.close()
+    return dict(seconds=float(np.mean(times)), seconds_min=float(np.min(times)), iterations=stats.iterations,
+                residual=stats.residual, energy=stats.band_energy, converged=bool(stats.converged),
+                learned_steps=stats.learned_steps, fallbacks=stats.fallbacks, newton_schulz_steps=stats.ns_steps,
+                cholesky_calls=stats.chol_calls, ns_continued=stats.ns_continued, ns_restarts=stats.ns_restarts,
+                dots_sweeps=stats.dots_sweeps, profiled_seconds=t_prof, profiled_iterations=pst.iterations,
+                profile=prof)
 
 
-def jacobi_roofline(r, precond_dtype, npts):
-    ms, launches = r["profile"]["jacobi0"]
-    if not launches:
-        return None
-    # the 3 fine sweeps launched per V-cycle (the zero-start first sweep is
-    # pointwise, fused with the cast): read x, f and write x, plus w on the
-    # last pre-sweep -> (4 + 3 + 3) / 3 values per point; a last sweep that
-    # also forms the Anderson dots reads the previous correction as well.
-    # FP16: 2-byte values, but the last sweep writes (and the dots read) the
-    # FP32 correction: (8 + 6 + 8) / 3 bytes per point, + 4 with the dots
-    if precond_dtype == mgx.F16:
-        per_launch = (22.0 / 3.0 * launches + 4.0 * r["dots_sweeps"]) / launches * npts
-        s_bytes = 2
-    else:
-        s_bytes = 4 if precond_dtype == mgx.F32 else 8
-        per_launch = (10.0 * launches + r["dots_sweeps"]) / (3.0 * launches) * s_bytes * npts
-    avg_s = (ms / 1e3) / launches
-    achieved = per_launch / avg_s / 1e9
-    # measured without the Anderson dots (the first 12 steps of a solve)
-    key = {4: "jacobi_fp32_sweep", 2: "jacobi_fp16_sweep"}.get(s_bytes)
-    traffic = TRAFFIC_C5[key] if (key and npts == 2048 * 32 ** 3) else None
-    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="k_box_march<JACOBI> fine-level sweep (%s)" % {2: "FP16", 4: "FP32", 8: "FP64"}[s_bytes],
-                avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches), bytes_per_launch=int(per_launch))
-
-
-def cellgram_roofline(r, npts, split):
-    """k_cell_gram, the S / H cell partials: psi and H psi (FP64) read once,
-    16 bytes per orbital point per launch (the partials written are < 2%)."""
-    ms, launches = r["profile"]["cellgram"]
-    if not launches:
-        return None
-    per_launch = 16.0 * npts
-    avg_s = (ms / 1e3) / launches
-    achieved = per_launch / avg_s / 1e9
-    key = "cell_gram_split" if split else "cell_gram_f64"
-    traffic = TRAFFIC_C5.get(key) if npts == 2048 * 32 ** 3 else None
-    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="k_cell_gram<%s> S/H partials per 8^3 cell (MFMA)" % ("SPLIT" if split else "F64"),
-                avg_launch_ms=round(1e3 * avg_s, 4), launches=int(launches), bytes_per_launch=int(per_launch))
-
-
-def cnn_roofline(r, npts):
-    ms, launches = r["profile"]["learned"]
-    if not launches:
-        return None
-    per_launch = CNN_FLOP_PER_VOXEL * npts
-    avg_s = (ms / 1e3) / launches
-    achieved = per_launch / avg_s / 1e12
-    traffic = TRAFFIC_C5["cnn_application"] if npts == 2048 * 32 ** 3 else None
-    return dict(bound="mfma", achieved=round(achieved, 2), peak=BF16_PEAK_TFS, unit="TFLOP/s",
-                frac=round(achieved / BF16_PEAK_TFS, 5), traffic=traffic,
-                kernel="CNN inference (4 conv3d layers, BF16 MFMA) over all boxes",
-                avg_launch_ms=round(1e3 * avg_s, 3), launches=int(launches), flop_per_launch=int(per_launch))
+def group_rates(r, mode, npts):
+    """Per kernel group of one profiled solve: ms per solve, launches, average
+    launch, algorithmic work per launch and the rate against its peak."""
+    out = {}
+    prec = {"plain": "f64", "mixed": "f32", "learned": "f32", "ext16": "f16"}[mode]
+    gprec = "split" if PATHS[mode]["gram_prec"] == mgx.PREC_SPLIT else "f64"
+    for g, (name, bound) in GROUPS.items():
+        ms, launches, work = r["profile"][g]
+        if not launches:
+            continue
+        avg_s = ms / 1e3 / launches
+        e = dict(kernel=name.replace("<T", "<" + {"f64": "double", "f32": "float", "f16": "_Float16"}[prec])
+                 .replace("<PREC", "<" + ("1" if gprec == "split" else "0")),
+                 group=g, ms_per_solve=round(ms, 3), launches=int(launches), avg_launch_ms=round(1e3 * avg_s, 4))
+        if work > 0 and bound == "hbm":
+            per = work / launches
+            ach = per / avg_s / 1e9
+            key = (g, {"cellgram": gprec, "residk": "f64", "learned": "bf16"}.get(g, prec))
+            e.update(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=round(ach / HBM_PEAK_GBS, 4), bytes_per_launch=int(per),
+                     traffic=TRAFFIC_C5.get(key) if npts == C5_POINTS else None)
+        elif work > 0 and bound.startswith("mfma"):
+            dt = bound.split(":")[1]
+            per = work / launches
+            ach = per / avg_s / 1e12
+            e.update(bound="mfma", achieved=round(ach, 2), peak=PEAK_TFS[dt], unit="TFLOP/s",
+                     frac=round(ach / PEAK_TFS[dt], 4), flop_per_launch=int(per), mfma_dtype=dt,
+                     traffic=TRAFFIC_C5.get((g, dt)) if npts == C5_POINTS else None)
+        else:
+            e.update(bound=bound, achieved=None, frac=None, traffic=None)
+        if e.get("traffic"):
+            e["traffic_source"] = TRAFFIC_SOURCE
+        out[g] = e
+    return out
 
 
 def stream_reference(ctx, n=1 << 28, reps=20):
-    """Achievable HBM rate on this device: the engine's FP32 axpy (y += a x,
-    STREAM-triad form: two reads and one write per element) over 1 GiB
-    arrays; bytes / wall time of back-to-back launches."""
+    """Measured HBM ceiling: STREAM triad a = b + s c over three 1 GiB FP32
+    arrays, 16-byte lanes, 2/4/8 loads in flight per lane (best depth)."""
     import ctypes as C
-    a = mgx.DeviceArray(ctx, shape=(n,), dtype=np.float32)
-    b = mgx.DeviceArray(ctx, shape=(n,), dtype=np.float32)
-    sync = mgx.DeviceArray(ctx, shape=(1,), dtype=np.float64)
-    L = mgx.lib()
-
-    def run(k):
-        for _ in range(k):
-            ctx.check(L.mgx_axpy(ctx.h, mgx.F32, C.c_int64(n), C.c_double(0.5), mgx.ptr(a), mgx.ptr(b)), "mgx_axpy")
-        sync.get()
-
-    run(3)
-    t0 = time.perf_counter()
-    run(reps)
-    dt = (time.perf_counter() - t0) / reps
-    for d in (a, b, sync):
+    arrs = [mgx.DeviceArray(ctx, shape=(n,), dtype=np.float32) for _ in range(3)]
+    gbps, unroll = C.c_double(), C.c_int()
+    ctx.check(mgx.lib().mgx_stream_triad(ctx.h, C.c_int64(n), mgx.ptr(arrs[0]), mgx.ptr(arrs[1]), mgx.ptr(arrs[2]),
+                                         reps, C.byref(gbps), C.byref(unroll)), "mgx_stream_triad")
+    for d in arrs:
         d.free()
-    gbs = 3 * 4 * n / dt / 1e9
-    return dict(kernel="k_axpy<float> (y += a x, 1 GiB arrays: 2 reads + 1 write per element)",
-                GBps=round(gbs, 1), frac_of_peak=round(gbs / HBM_PEAK_GBS, 4))
+    return dict(kernel="k_triad4<%d> (a = b + s c, 3 x 1 GiB FP32, 16-byte lanes, %d loads per operand in flight)"
+                % (unroll.value, unroll.value), GBps=round(gbps.value, 1),
+                frac_of_peak=round(gbps.value / HBM_PEAK_GBS, 4))
+
+
+def host_cpu():
+    """CPU model, logical CPUs, physical cores (from /proc/cpuinfo) and the
+    CPUs this process may run on."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None and core is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return dict(model=model, logical_cpus=os.cpu_count(), physical_cores=len(cores) or None, cpus_available=avail)
+
+
+def cpu_baseline(cfg, iters, mg_levels, budget_s=12.0):
+    """Full-size orbital steps of the oracle (FP32 V-cycle, the reference's
+    MGPRECONDTYPE; C++/OpenMP) on this host until about budget_s seconds
+    have passed (at least 2 steps), scaled by the iteration count."""
+    import oracle as O
+    ob = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    psi = cfg.psi.copy()
+    ts = []
+    while len(ts) < 2 or sum(ts) < budget_s:
+        t0 = time.perf_counter()
+        psi, _, _, _ = O.b_solver_step(ob, psi, cfg.vtot, mg_levels, O.F32, 1.5, None, zero_start=True)
+        ts.append(time.perf_counter() - t0)
+    t_step = float(np.mean(ts))
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or host_cpu()["cpus_available"]
+    cpu = host_cpu()
+    return dict(value=round(t_step * iters, 2), unit="s per converged solve", cores=threads, kind="port",
+                sample="%d oracle orbital steps on the full workload (%.2f s each, %d OpenMP threads), "
+                       "scaled by %d iterations" % (len(ts), t_step, threads, iters),
+                cpu_model=cpu["model"], physical_cores=cpu["physical_cores"], logical_cpus=cpu["logical_cpus"],
+                cpus_available=cpu["cpus_available"])
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1, help="timed converged solves per path")
+    ap.add_argument("--steps", type=int, default=2, help="timed converged solves per path")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c5")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--maxit", type=int, default=2000)
-    ap.add_argument("--modes", default="plain,mixed,learned")
+    ap.add_argument("--modes", default="plain,mixed,learned,ext16")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -231,7 +270,7 @@ def main():
 
     t0 = time.perf_counter()
     cfg, meta = make_config(args.config, args.seed)
-    set_mixed_precond(cfg)
+    fit_precond(cfg)
     log("config %s: %d boxes, %d points, grid %s, h=%.3f (generated in %.1f s)"
         % (args.config, cfg.nbox, cfg.psi.size, cfg.gdim, cfg.h, time.perf_counter() - t0))
     ctx = mgx.Ctx(0)
@@ -245,24 +284,25 @@ def main():
     res = {}
     for mode in modes:
         res[mode] = run_mode(ctx, bs, cfg, mode, args.tol, args.maxit, args.steps, args.warmup, cnn)
-    headname = "learned" if "learned" in res else ("mixed" if "mixed" in res else modes[0])
+    headname = next(m for m in ("learned", "mixed", "ext16", "plain") if m in res)
     head = res[headname]
     npts = cfg.psi.size
 
-    # the headline path's dominant hand-written kernel: the S / H cell
-    # partials (one launch per step); beside it the fine FP32 Jacobi sweep
-    # (three per step) and the network (learned steps only)
-    vc_name = "mixed" if "mixed" in res else headname
-    roof = cellgram_roofline(head, npts, PATHS[headname]["gram_prec"] == mgx.PREC_SPLIT)
-    other = [x for x in (jacobi_roofline(res[vc_name], PATHS[vc_name]["precond_dtype"], npts),
-                         cnn_roofline(res["learned"], npts) if "learned" in res else None) if x]
-    if roof is None and other:
-        roof, other = other[0], other[1:]
-
     stream = stream_reference(ctx)
-    for r in [roof] + other:
-        if r and r["unit"] == "GB/s":
-            r["frac_of_stream"] = round(r["achieved"] / stream["GBps"], 4)
+    kernels = {m: group_rates(r, m, npts) for m, r in res.items()}
+    for ks in kernels.values():
+        for e in ks.values():
+            if e.get("unit") == "GB/s":
+                e["frac_of_stream"] = round(e["achieved"] / stream["GBps"], 4)
+    # the headline path's kernel groups by time per solve; the top one with a
+    # roofline is `roofline`, the others above 5% of a solve follow it
+    ranked = sorted(kernels[headname].values(), key=lambda e: -e["ms_per_solve"])
+    solve_ms = 1e3 * head["profiled_seconds"]
+    for e in ranked:
+        e["share_of_solve"] = round(e["ms_per_solve"] / solve_ms, 4)
+    with_roof = [e for e in ranked if e.get("frac") is not None]
+    roof = with_roof[0] if with_roof else None
+    other = [e for e in ranked if e is not roof and e["share_of_solve"] > 0.05]
 
     cpu = None
     if not args.no_cpu:
@@ -282,9 +322,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ({"learned": "bf16+", "mixed": ""}.get(headname, "")
-                  + ("f16+f32+f64" if PATHS[headname]["precond_dtype"] == mgx.F16 else
-                     ("f32+f64" if headname != "plain" else "f64"))),
+        "dtype": DTYPES[headname],
         "data": "synthetic (seeded Gaussian orbitals and wells)",
         "config": {"workload": "%s: %d orbitals in %s^3 boxes, %d^3 periodic grid, h=%.2f bohr, %d wells, "
                                "centres >= %s bohr apart, ABPG (Anderson m=1) to |R| < %g Ha"
@@ -293,13 +331,19 @@ def main():
                    "path": LABELS[headname]},
         "iterations": head["iterations"],
         "converged": head["converged"],
-        "paths": {m: {k: v for k, v in r.items() if k != "profile"} for m, r in res.items()},
-        "kernel_ms": {m: {g: round(t[0], 2) for g, t in r["profile"].items()} for m, r in res.items()},
+        "paths": {m: dict({k: v for k, v in r.items() if k != "profile"}, label=LABELS[m], dtype=DTYPES[m])
+                  for m, r in res.items()},
         "roofline": roof,
         "roofline_other": other,
+        "kernels": {m: sorted(ks.values(), key=lambda e: -e["ms_per_solve"]) for m, ks in kernels.items()},
+        "phase_ms_per_solve": {m: {g: round(r["profile"][g][0], 2) for g in ("hloc", "gram", "dense", "resid",
+                                                                             "precond", "normalize", "learned")}
+                               for m, r in res.items()},
         "stream": stream,
         "cpu_baseline": cpu,
     }
+    if roof and roof.get("unit") == "GB/s":
+        roof["frac_of_stream"] = round(roof["achieved"] / stream["GBps"], 4)
     if "plain" in res:
         for m in res:
             if m == "plain":

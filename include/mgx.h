@@ -112,6 +112,13 @@ int mgx_pointwise_product(mgx_ctx* ctx, const mgx_block_t* a, const double* v, m
 /* LinearAlgebraUtils::MPaxpy, src/linear_algebra/mputils.cc:221-244:
  * y += (T)(alpha*(double)x), n elements of dtype */
 int mgx_axpy(mgx_ctx* ctx, int dtype, int64_t n, double alpha, const void* x, void* y);
+/* No counterpart in the reference: the HBM ceiling for the roofline fractions.
+ * STREAM triad a = b + 0.5 c over n FP32 values (n % 4 == 0, 16-byte aligned
+ * device arrays), 16-byte lanes with 2, 4 or 8 loads of each operand in
+ * flight per lane, reps launches each timed with hipEvents on the context
+ * stream; *gbps = the best rate (12 n bytes per launch), *unroll = its depth. */
+int mgx_stream_triad(mgx_ctx* ctx, int64_t n, float* a, const float* b, const float* c, int reps,
+    double* gbps, int* unroll);
 /* GridFuncVector::jacobi, src/pb/GridFuncVector.cc:2416-2425 (trades x):
  * w = L x; w -= f; x += -jacobi_factor * w */
 int mgx_jacobi(mgx_ctx* ctx, int lap_type, double jacobi_factor, const mgx_block_t* f,
@@ -322,14 +329,28 @@ typedef struct
     double anderson_beta;/* ABPG.beta (1.0) */
     int conv_criterion;  /* Quench.conv_criterion: MGX_CONV_DELTAE or MGX_CONV_RESIDUAL */
     int precond_kind;    /* MGX_PRECOND_VCYCLE, or MGX_PRECOND_LEARNED (mgx_solver_set_cnn) */
+    int inverse_mode;    /* S^-1 after the first step: MGX_INV_NS_F64 (Newton-Schulz refresh,
+                            FP64 products), MGX_INV_NS_BF16 (its correction as one BF16 hi/lo
+                            product, no counterpart in the reference) or MGX_INV_CHOL (Cholesky +
+                            inverse every step, GramMatrix::updateLS / computeInverse) */
+    double ns_tol;       /* Newton-Schulz target max|I - S^-1 S|; 0 = 1e-15 with FP64 S,
+                            1e-10 with the split S */
+    int ns_max_spec;     /* most refresh steps taken before the host checks them (0 = 4) */
 } mgx_solver_cfg_t;
 
-/* MGX_PRECOND_LEARNED: the step is alpha * CNN(R); when the next residual
- * norm is larger, psi and R are restored, the V-cycle step is taken instead
- * and the V-cycle is used for the next MGX_LEARNED_BACKOFF steps, a number
- * that doubles with each rejection in a row (at most
- * MGX_LEARNED_BACKOFF_MAX) and returns to MGX_LEARNED_BACKOFF once a
- * network step is accepted. */
+#define MGX_INV_NS_F64 0
+#define MGX_INV_NS_BF16 1
+#define MGX_INV_CHOL 2
+
+/* MGX_PRECOND_LEARNED: a trial takes psi += alpha * CNN(R) as a plain
+ * preconditioned step (Anderson history cleared) from the current state;
+ * the next step evaluates it, returns to the saved state and takes the
+ * V-cycle's plain step from there; the step after that evaluates the V-cycle
+ * step and keeps whichever of the two reached the smaller residual norm
+ * (both evaluations count as iterations).  The V-cycle alone then runs for
+ * MGX_LEARNED_BACKOFF steps, a number that doubles with each rejected trial
+ * in a row (at most MGX_LEARNED_BACKOFF_MAX) and returns to
+ * MGX_LEARNED_BACKOFF after an accepted one. */
 #define MGX_PRECOND_VCYCLE 0
 #define MGX_PRECOND_LEARNED 1
 #define MGX_LEARNED_BACKOFF 4
@@ -344,17 +365,29 @@ typedef struct
 #define MGX_CONV_DELTAE 0
 #define MGX_CONV_RESIDUAL 1
 
-/* kernel groups timed when cfg.profile = 1 */
-#define MGX_PROF_HLOC 0      /* H psi */
+/* kernel groups timed when cfg.profile = 1.  Phase groups (several kernels):
+ * GRAM, DENSE, RESID, PRECOND, NORMALIZE.  The others bracket one kernel (or
+ * one kind of launch) each and also accumulate its algorithmic work: HBM bytes,
+ * or flops for LEARNED, NSGEMM and NSGEMM16 (mgx_solver_profile_work). */
+#define MGX_PROF_HLOC 0      /* H psi (one marching kernel) */
 #define MGX_PROF_GRAM 1      /* S, H pair contractions */
 #define MGX_PROF_DENSE 2     /* Cholesky, inverse, theta, trace, Ritz */
 #define MGX_PROF_RESID 3     /* residual + norm */
-#define MGX_PROF_JACOBI0 4   /* fine-level Jacobi sweeps (one launch each) */
+#define MGX_PROF_JACOBI0 4   /* fine-level Jacobi sweeps without the Anderson dots */
 #define MGX_PROF_PRECOND 5   /* whole V-cycle incl. update */
 #define MGX_PROF_NORMALIZE 6 /* orbital normalisation */
 #define MGX_PROF_LEARNED 7   /* CNN inference of the learned preconditioner */
-#define MGX_PROF_CELLGRAM 8  /* the S / H cell-partials kernel alone (one launch per step) */
-#define MGX_PROF_N 9
+#define MGX_PROF_CELLGRAM 8  /* the S / H cell-partials kernel */
+#define MGX_PROF_SKIN 9      /* the Galerkin skin kernel (stencil reach outside each box) */
+#define MGX_PROF_RESIDK 10   /* the cell residual kernel */
+#define MGX_PROF_UPDATE 11   /* the orbital update (Anderson or preconditioned step) */
+#define MGX_PROF_JACOBI0D 12 /* the last fine sweep forming the Anderson dots */
+#define MGX_PROF_FIRST 13    /* the zero-start first fine sweep (pointwise, with the cast) */
+#define MGX_PROF_NSGEMM 14   /* Newton-Schulz FP64 dense products */
+#define MGX_PROF_SPMM 15     /* pair-sparse products (X S, S^-1 H) */
+#define MGX_PROF_TRANSFER 16 /* fine-level restriction and prolongation */
+#define MGX_PROF_NSGEMM16 17 /* Newton-Schulz BF16 correction products */
+#define MGX_PROF_N 18
 
 typedef struct
 {
@@ -364,11 +397,13 @@ typedef struct
     double band_energy;  /* tr(S^-1 H), Ha */
     double seconds;
     double lowest_eig;   /* Ry */
-    int learned_steps;   /* steps taken with the CNN (accepted or not), cumulative */
-    int fallbacks;       /* CNN steps rejected and redone with the V-cycle, cumulative */
+    int learned_steps;   /* CNN trials (accepted or not), cumulative */
+    int fallbacks;       /* CNN trials whose step lost to the V-cycle step from the same state */
     int ns_steps;        /* Newton-Schulz refresh steps of S^-1 (one dgemm each), cumulative */
     int chol_calls;      /* Cholesky + inverse of S (first step and refresh failures), cumulative */
     int dots_sweeps;     /* V-cycles whose last fine sweep also formed the Anderson dots, cumulative */
+    int ns_continued;    /* refreshes whose unchecked steps fell short and were continued with checks */
+    int ns_restarts;     /* refreshes that diverged and were replaced by Cholesky + inverse */
 } mgx_solver_stats_t;
 
 int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t* cfg,
@@ -382,11 +417,24 @@ int mgx_solver_reset(mgx_solver* s);
 /* accumulated milliseconds and launch counts per MGX_PROF_* group (host
  * arrays of MGX_PROF_N); reset=1 clears the accumulators afterwards */
 int mgx_solver_profile(mgx_solver* s, double* ms, int64_t* launches, int reset);
+/* the algorithmic work accumulated per group (host array of MGX_PROF_N; bytes or
+ * flops, see above), read before a reset of mgx_solver_profile */
+int mgx_solver_profile_work(mgx_solver* s, double* work);
+/* switch the kernel-group timing (cfg.profile) on or off between steps */
+int mgx_solver_set_profile(mgx_solver* s, int on);
 int mgx_solver_get_psi(mgx_solver* s, double* psi_host);
 /* device pointers of the solver's psi / hpsi / S / H / residual (for inspection; the
  * residual is that of the last evaluated step when cfg.anderson_m = 0 and
  * precond_kind = MGX_PRECOND_VCYCLE, otherwise overwritten by the step) */
 int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H, double** res);
+/* device pointer of the solver's S^-1 (n x n, column-major): the inverse of the
+ * S of the last evaluated step (refreshed, or Cholesky) */
+int mgx_solver_inverse(mgx_solver* s, double** Sinv);
+/* replace the orbitals (host, the box set's packed layout), e.g. extrapolated
+ * orbitals after an ionic move (MGmol::extrapolate_orbitals keeps the solver
+ * state); the Anderson history is cleared, S^-1 stays as the start of the next
+ * refresh (which falls back to Cholesky when it is too far off) */
+int mgx_solver_set_psi(mgx_solver* s, const double* psi_host);
 /* attach a trained network (uniform box edge equal to the network's) for
  * cfg.precond_kind = MGX_PRECOND_LEARNED; the solver does not own it */
 int mgx_solver_set_cnn(mgx_solver* s, mgx_cnn* cnn);

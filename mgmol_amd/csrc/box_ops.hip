@@ -2618,10 +2618,15 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
     T* a = first_done ? B.tmp[level] : v;
     T* b = first_done ? v : B.tmp[level];
     int rc;
+    // algorithmic bytes of a fine sweep: x and f read, x written (s each),
+    // + w written on the last pre-sweep; the last sweep may write FP32
+    // (last_out) and read the previous correction for the Anderson dots
+    const double np0 = (double)bs->off[0][bs->nbox];
+    const double st  = sizeof(T);
     for (int it = first_done ? 1 : 0; it < ncycl; it++)
     {
         T* w          = (it == ncycl - 1 && level < max_levels) ? B.w[level] : nullptr;
-        const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0) : -1;
+        const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0, (w ? 4. : 3.) * st * np0) : -1;
         if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, w))) return rc;
         prof_end(ctx, tok);
         std::swap(a, b);
@@ -2629,7 +2634,9 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
     if (level == max_levels) return MGX_OK;
     T* rcoarse = B.rc[level];
     T* nv      = B.nv[level];
+    int tok    = level == 0 ? prof_begin(ctx, MGX_PROF_TRANSFER, st * np0 * 9. / 8.) : -1;
     if ((rc = box_restrict_t(ctx, bs, level, (const T*)B.w[level], rcoarse))) return rc;
+    prof_end(ctx, tok);
     const int64_t nc = bs->off[level + 1][bs->nbox];
     // the coarse level starts from 0: its first sweep is pointwise
     double hc[3] = { 2. * h[0], 2. * h[1], 2. * h[2] };
@@ -2637,21 +2644,32 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
         rcoarse, B.tmp[level + 1], nc, jacobi_factor(MGX_LAP_2ND, hc));
     MGX_CHECK_LAUNCH(ctx);
     if ((rc = box_mg_rec(ctx, bs, B, max_levels, nv, (const T*)rcoarse, MGX_LAP_2ND, level + 1, true))) return rc;
+    tok = level == 0 ? prof_begin(ctx, MGX_PROF_TRANSFER, st * np0 * 17. / 8.) : -1;
     if ((rc = box_prolong_t<T, true>(ctx, bs, level, (const T*)nv, v))) return rc;
+    prof_end(ctx, tok);
     a = v;
     b = B.tmp[level];
     for (int it = 0; it < 2; it++)
     {
-        const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0) : -1;
+        const int tk = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0, 3. * st * np0) : -1;
         // the last fine sweep (it writes v) may also form the Anderson dots;
         // with last_out it writes FP32 there instead, times last_inv per box
         const MarchDots* dd = (level == 0 && it == 1) ? dots : nullptr;
+        bool dd_done        = false;
         if (level == 0 && it == 1 && last_out)
-            rc = box_jacobi_out(ctx, bs, type, jf, (const T*)a, f, last_out, last_inv, dd, dots_done);
+            rc = box_jacobi_out(ctx, bs, type, jf, (const T*)a, f, last_out, last_inv, dd, &dd_done);
         else
-            rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr, dd, dots_done);
+            rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr, dd, &dd_done);
         if (rc) return rc;
-        prof_end(ctx, tok);
+        prof_end(ctx, tk);
+        if (level == 0 && it == 1)
+        {
+            // the previous correction (FP32 from the FP16 V-cycle, else T) is read for the dots
+            const double so = last_out ? 4. : st;
+            prof_retag(ctx, tk, dd_done ? MGX_PROF_JACOBI0D : MGX_PROF_JACOBI0,
+                (2. * st + so + (dd_done ? so : 0.)) * np0);
+        }
+        if (dd_done && dots_done) *dots_done = true;
         std::swap(a, b);
     }
     return MGX_OK;
@@ -2676,17 +2694,25 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     if (rc) return rc;
     // from x0 = 0 the first fine sweep is pointwise and fused with the cast
     const bool zero = gamma == 0.;
+    // the residual read (FP64), f and x written
+    const int tk = prof_begin(ctx, MGX_PROF_FIRST, (8. + 2. * sizeof(T)) * (double)n);
     if (zero)
         hipLaunchKernelGGL(k_first_sweep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, B.tmp[0], n,
             jacobi_factor(type, bs->h));
     else
         hipLaunchKernelGGL(k_prep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, x, n, gamma);
+    prof_end(ctx, tk);
     MGX_CHECK_LAUNCH(ctx);
     if ((rc = box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0, zero, dots, dots_done))) return rc;
     if (out)
         hipLaunchKernelGGL(k_to_f64<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, (const T*)x, out, n);
     else if (psi)
+    {
+        // psi read and written, the correction read
+        const int tu = prof_begin(ctx, MGX_PROF_UPDATE, (16. + sizeof(T)) * (double)n);
         hipLaunchKernelGGL(k_update<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, psi, (const T*)x, n, alpha);
+        prof_end(ctx, tu);
+    }
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
@@ -2727,8 +2753,10 @@ int box_precond_h(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     BoxMG<T> B;
     if ((rc = box_mg_setup(ctx, bs, max_levels, B, 3))) return rc;
     const bool zero = gamma == 0.;
+    const int tk    = prof_begin(ctx, MGX_PROF_FIRST, (8. + 2. * sizeof(T)) * (double)n);
     hipLaunchKernelGGL(k_first_sweep_box<T>, dim3(bs->nbox), dim3(256), 0, ctx->stream, res, (const double*)sc, f,
         zero ? B.tmp[0] : x, zero ? jacobi_factor(type, bs->h) : gamma, (const int64_t*)bs->d_off[0], bs->nbox);
+    prof_end(ctx, tk);
     MGX_CHECK_LAUNCH(ctx);
     return box_mg_rec(ctx, bs, B, max_levels, x, (const T*)f, type, 0, zero, dots, dots_done, xdst,
         (const double*)inv);
@@ -2786,7 +2814,8 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
                                            : (bs->pair_z ? k_cell_gram<MGX_PREC_F64, true, kGramWaves>
                                                          : k_cell_gram<MGX_PREC_F64, false, 4>);
         const int nthr = 64 * (bs->pair_z ? kGramWaves : 4);
-        const int tk   = prof_begin(ctx, MGX_PROF_CELLGRAM);
+        // psi and H psi read once (FP64)
+        const int tk   = prof_begin(ctx, MGX_PROF_CELLGRAM, 16. * (double)bs->off[0][bs->nbox]);
         hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(nthr), 0, ctx->stream, psi, hpsi,
             (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
             (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part, bs->ncell);
@@ -2800,9 +2829,11 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
             size_t lds        = 2 * (size_t)bs->maxedge * bs->maxedge * sizeof(double);
             const int use_lds = lds <= 64 * 1024;
             if (!use_lds) lds = 0;
+            const int tk = prof_begin(ctx, MGX_PROF_SKIN);
             hipLaunchKernelGGL(k_slab_skin, dim3(6 * bs->nbox), dim3(256), lds, ctx->stream, psi,
                 (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_ilist_ptr,
                 (const int*)bs->d_ilist, coef4(bs->h), use_lds, skin);
+            prof_end(ctx, tk);
             MGX_CHECK_LAUNCH(ctx);
         }
         const double vel = bs->h[0] * bs->h[1] * bs->h[2];
@@ -2861,10 +2892,13 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
             esq = (double*)workspace(ctx, 3, (size_t)std::max(bs->nent, 1) * sizeof(double));
             if (!esq) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
         }
+        // psi, H psi read, R written (FP64)
+        const int tk = prof_begin(ctx, MGX_PROF_RESIDK, 24. * (double)bs->off[0][bs->nbox]);
         hipLaunchKernelGGL(bs->pair_z ? k_cell_residual<true> : k_cell_residual<false>, dim3(cell_grid(bs->ncell)),
             dim3(256), 0, ctx->stream, psi, hpsi, theta,
             bs->nbox, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
             (const int*)bs->d_cell_ent, res, esq, bs->ncell);
+        prof_end(ctx, tk);
         if (d)
         {
             const double vel = bs->h[0] * bs->h[1] * bs->h[2];

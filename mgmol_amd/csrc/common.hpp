@@ -13,14 +13,19 @@
 #include "mgx.h"
 
 // hipEvent pairs around kernel groups, recorded on the context stream;
-// collected (elapsed ms accumulated per group) after a synchronisation
+// collected (elapsed ms accumulated per group) after a synchronisation.
+// Each pair also carries the algorithmic work of what it brackets (HBM bytes
+// or flops, the unit fixed per group), so that rates come out per group.
+constexpr int kProfGroups = 32;
 struct MgxProf
 {
     std::vector<hipEvent_t> pool;
-    std::vector<int> group;  // group of each recorded pair
-    size_t used = 0;         // events handed out
-    double ms[16]     = {};
-    int64_t cnt[16]   = {};
+    std::vector<int> group;     // group of each recorded pair
+    std::vector<double> work;   // algorithmic work of each recorded pair
+    size_t used = 0;            // events handed out
+    double ms[kProfGroups]   = {};
+    int64_t cnt[kProfGroups] = {};
+    double wsum[kProfGroups] = {};
 };
 
 struct mgx_ctx
@@ -173,9 +178,12 @@ inline double jacobi_factor(int type, const double h[3])
 
 int sync(mgx_ctx* ctx);
 
-// profiling tokens (no-ops when ctx->prof is null)
-int prof_begin(mgx_ctx* ctx, int group);
+// profiling tokens (no-ops when ctx->prof is null); work = the bracketed
+// launches' algorithmic bytes or flops (the group's unit)
+int prof_begin(mgx_ctx* ctx, int group, double work = 0.);
 void prof_end(mgx_ctx* ctx, int token);
+// change the group and work of a recorded token (once a launch knows what it did)
+void prof_retag(mgx_ctx* ctx, int token, int group, double work);
 void prof_collect(mgx_ctx* ctx);
 
 // Column tiles of a box set's pair-sparse matrices (S and H have a nonzero

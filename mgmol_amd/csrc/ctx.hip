@@ -38,10 +38,10 @@ int sync(mgx_ctx* ctx)
     return MGX_OK;
 }
 
-int prof_begin(mgx_ctx* ctx, int group)
+int prof_begin(mgx_ctx* ctx, int group, double work)
 {
     MgxProf* p = ctx->prof;
-    if (!p) return -1;
+    if (!p || group < 0 || group >= kProfGroups) return -1;
     while (p->pool.size() < p->used + 2)
     {
         hipEvent_t e;
@@ -51,8 +51,13 @@ int prof_begin(mgx_ctx* ctx, int group)
     const int tok = (int)(p->used / 2);
     hipEventRecord(p->pool[p->used], ctx->stream);
     p->used += 2;
-    if ((int)p->group.size() <= tok) p->group.resize(tok + 1);
+    if ((int)p->group.size() <= tok)
+    {
+        p->group.resize(tok + 1);
+        p->work.resize(tok + 1);
+    }
     p->group[tok] = group;
+    p->work[tok]  = work;
     return tok;
 }
 
@@ -60,6 +65,14 @@ void prof_end(mgx_ctx* ctx, int tok)
 {
     if (!ctx->prof || tok < 0) return;
     hipEventRecord(ctx->prof->pool[2 * tok + 1], ctx->stream);
+}
+
+void prof_retag(mgx_ctx* ctx, int tok, int group, double work)
+{
+    MgxProf* p = ctx->prof;
+    if (!p || tok < 0 || tok >= (int)p->group.size() || group < 0 || group >= kProfGroups) return;
+    p->group[tok] = group;
+    p->work[tok]  = work;
 }
 
 void prof_collect(mgx_ctx* ctx)
@@ -74,6 +87,7 @@ void prof_collect(mgx_ctx* ctx)
         {
             p->ms[p->group[t]] += ms;
             p->cnt[p->group[t]] += 1;
+            p->wsum[p->group[t]] += p->work[t];
         }
     }
     p->used = 0;

@@ -485,8 +485,11 @@ __global__ void k_sym_half(const double* a, int n, double* out)
 int spmm_pairs(mgx_ctx* ctx, int n, const double* X, const double* val, const PairTiles& pt, double* Y)
 {
     if (pt.ntile <= 0) return set_err(ctx, MGX_EINVAL, "no column tiles");
+    // X read and Y written once: 16 n^2 bytes
+    const int tk = prof_begin(ctx, MGX_PROF_SPMM, 16. * (double)n * n);
     hipLaunchKernelGGL(k_spmm_tiles, dim3(pt.ntile, (n + 255) / 256), dim3(256), 0, ctx->stream, X, n, val, pt.ptr,
         pt.col, pt.k, pt.e, Y);
+    prof_end(ctx, tk);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
@@ -763,10 +766,12 @@ int inverse_update(mgx_ctx* ctx, int n, const double* S, double* L, double* X, d
 // inverse_update for a pair-sparse S (values Sv per pair, column tiles pt):
 // T = X S on the sparse product, X <- 2X - T X as one dense GEMM (written to
 // the other half of the work buffer, operands swapped, no copies).  Stops
-// before a step once delta = max|I - X S| < tol, and after a step once
-// delta^2 (the error that step leaves) < tol.
+// before a step once delta = max|I - X S| < tol.  With FP64 products it also
+// stops after a step once delta^2 (the error that step leaves) < tol; with
+// lowp the correction E X, E = I - X S, is one FP32 product, whose rounding
+// (~1e-7 |E| |X|) can exceed delta^2, so only measured deltas end the loop.
 int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv, const PairTiles& pt, double* L,
-    double* X, double* work, int* steps, double tol)
+    double* X, double* work, int* steps, double tol, bool lowp)
 {
     const double mone = -1., two = 2.;
     const size_t nn = (size_t)n * n;
@@ -776,27 +781,23 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
     double* bm  = (double*)workspace(ctx, 4, 1024 * sizeof(double));
     if (!bm) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
-    // a loose tolerance (an S from the split products) takes the correction
-    // E X of a step, E = I - X S, as one FP32 product: its rounding error,
-    // ~1e-7 |E| |X|, is below |E|^2 while |E| > 1e-5 and below tol / 10
-    // after the last step (|E| < 1e-5, |X| ~ 1-10)
-    const bool f32 = tol >= 1e-12;
     float *E32 = nullptr, *X32 = nullptr, *Y32 = nullptr;
-    if (f32)
+    if (lowp)
     {
         E32 = (float*)workspace(ctx, 7, 3 * nn * sizeof(float));
         if (!E32) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
         X32 = E32 + nn;
         Y32 = X32 + nn;
     }
+    const double gflop = 2. * (double)n * n * n;
     double prev = 1e300;
     *steps      = 0;
     int rc;
-    for (int it = 0; it < 8; it++)
+    for (int it = 0; it < 10; it++)
     {
         if ((rc = spmm_pairs(ctx, n, cur, Sv, pt, T))) return rc;
         const int nb = 512;
-        if (f32)
+        if (lowp)
             hipLaunchKernelGGL(k_id_err_f32, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T,
                 (const double*)cur, n, bm, E32, X32);
         else
@@ -812,7 +813,7 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
             return chol_inv(ctx, n, S, L, X);
         }
         if (delta < tol) break;
-        if (f32)
+        if (lowp)
         {
             // nxt = cur + E cur
             const float onef = 1.f, zerof = 0.f;
@@ -824,16 +825,21 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
                 (const double*)cur, (const float*)Y32, nxt, (int64_t)nn);
             MGX_CHECK_LAUNCH(ctx);
         }
-        // nxt = 2 cur - T cur
-        else if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
-                     rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n,
-                     nxt, rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
-                 != rocblas_status_success)
-            return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex failed");
+        else
+        {
+            // nxt = 2 cur - T cur
+            const int tk = prof_begin(ctx, MGX_PROF_NSGEMM, gflop);
+            if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
+                    rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n,
+                    nxt, rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
+                != rocblas_status_success)
+                return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex failed");
+            prof_end(ctx, tk);
+        }
         std::swap(cur, nxt);
         (*steps)++;
         prev = delta;
-        if (delta * delta < tol) break;  // the error after this step is ~delta^2
+        if (!lowp && delta * delta < tol) break;  // the error after this step is ~delta^2
     }
     if (cur != X) MGX_CHECK_HIP(ctx, hipMemcpyAsync(X, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_symmetrize, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream, X, n);
@@ -843,11 +849,14 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
 
 // inverse_update_pairs without host waits: nsteps Newton-Schulz steps from
 // X, each recording delta_k = max|I - X_k S| (before step k) in deltas[k]
-// on the device.  The caller reads the deltas later with its other scalars
-// and checks them (ns_spec_check); the steps and their arithmetic are those
-// of inverse_update_pairs.
+// on the device.  With bf16 the correction E X of a step is one BF16 product
+// with K = 3n of hi/lo parts (Eh Xh + Eh Xl + El Xh, FP32 sums; relative
+// error ~1e-5 of |E| |X|), so delta^2 does not bound the error a step leaves:
+// the delta of the final X is measured too, into deltas[nsteps].  The caller
+// reads the deltas later with its other scalars and checks them
+// (ns_spec_check).
 int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& pt, double* X, double* work,
-    int nsteps, double tol, double* deltas)
+    int nsteps, bool bf16, double* deltas)
 {
     const double mone = -1., two = 2.;
     const size_t nn = (size_t)n * n;
@@ -857,62 +866,75 @@ int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& 
     double* bm  = (double*)workspace(ctx, 4, 1024 * sizeof(double));
     if (!bm) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host);
-    // the correction E X at the split tolerance: one BF16 product with K = 3n
-    // of the hi/lo parts (Eh Xh + Eh Xl + El Xh, FP32 sums) -- relative error ~1e-5
-    // of |E| |X|, below |E|^2 while |E| > 1e-5; the deltas measured in FP64
-    // before each step decide the step count either way
-    const bool f32 = tol >= 1e-12;
     float* Y32     = nullptr;
     uint16_t* H16  = nullptr;
-    if (f32)
+    if (bf16)
     {
         Y32 = (float*)workspace(ctx, 7, nn * sizeof(float) + 6 * nn * sizeof(uint16_t));
         if (!Y32) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
         H16 = reinterpret_cast<uint16_t*>(Y32 + nn);
     }
+    const double flop = 2. * (double)n * n * n;
     int rc;
+    const int nb = 512;
     for (int it = 0; it < nsteps; it++)
     {
         if ((rc = spmm_pairs(ctx, n, cur, Sv, pt, T))) return rc;
-        const int nb = 512;
-        if (f32)
+        if (bf16)
             hipLaunchKernelGGL(k_id_err_bf16, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T,
                 (const double*)cur, n, bm, H16, H16 + 3 * nn);
         else
             hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
         hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double*)bm, nb, deltas + it);
         MGX_CHECK_LAUNCH(ctx);
-        if (f32)
+        if (bf16)
         {
             const float onef = 1.f, zerof = 0.f;
+            const int tk = prof_begin(ctx, MGX_PROF_NSGEMM16, 3. * flop);
             if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, 3 * n, &onef, H16,
                     rocblas_datatype_bf16_r, n, H16 + 3 * nn, rocblas_datatype_bf16_r, 3 * n, &zerof, Y32,
                     rocblas_datatype_f32_r, n, Y32, rocblas_datatype_f32_r, n, rocblas_datatype_f32_r,
                     rocblas_gemm_algo_standard, 0, 0)
                 != rocblas_status_success)
                 return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex (bf16) failed");
+            prof_end(ctx, tk);
             hipLaunchKernelGGL(k_add_f32, dim3(nblocks((int64_t)nn)), dim3(kBlock), 0, ctx->stream,
                 (const double*)cur, (const float*)Y32, nxt, (int64_t)nn);
             MGX_CHECK_LAUNCH(ctx);
         }
-        else if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
-                     rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n,
-                     nxt, rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
-                 != rocblas_status_success)
-            return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex failed");
+        else
+        {
+            const int tk = prof_begin(ctx, MGX_PROF_NSGEMM, flop);
+            if (rocblas_gemm_ex(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &mone, T,
+                    rocblas_datatype_f64_r, n, cur, rocblas_datatype_f64_r, n, &two, cur, rocblas_datatype_f64_r, n,
+                    nxt, rocblas_datatype_f64_r, n, rocblas_datatype_f64_r, rocblas_gemm_algo_standard, 0, 0)
+                != rocblas_status_success)
+                return set_err(ctx, MGX_EHIP, "rocblas_gemm_ex failed");
+            prof_end(ctx, tk);
+        }
         std::swap(cur, nxt);
     }
     if (cur != X) MGX_CHECK_HIP(ctx, hipMemcpyAsync(X, cur, nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_symmetrize_tiles, dim3((n + 31) / 32, (n + 31) / 32), dim3(256), 0, ctx->stream, X, n);
     MGX_CHECK_LAUNCH(ctx);
+    if (bf16)
+    {
+        // the error the last BF16 step left, on the symmetrised X
+        if ((rc = spmm_pairs(ctx, n, X, Sv, pt, T))) return rc;
+        hipLaunchKernelGGL(k_id_err, dim3(nb), dim3(256), 0, ctx->stream, (const double*)T, n, bm);
+        hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, ctx->stream, (const double*)bm, nb, deltas + nsteps);
+        MGX_CHECK_LAUNCH(ctx);
+    }
     return MGX_OK;
 }
 
 // The checks inverse_update_pairs makes step by step, on the deltas of
 // inverse_update_spec: 0 = the steps were enough (*needed = the steps the
 // checked loop would have taken), 1 = they converge but more are needed,
-// -1 = they do not converge (Cholesky)
-int ns_spec_check(const double* deltas, int nsteps, double tol, int* needed)
+// -1 = they do not converge (Cholesky).  With final (the BF16 steps)
+// deltas[nsteps] holds the measured error of the final X, and only a
+// measured delta below tol accepts.
+int ns_spec_check(const double* deltas, int nsteps, double tol, bool final, int* needed)
 {
     double prev = 1e300;
     for (int k = 0; k < nsteps; k++)
@@ -924,12 +946,22 @@ int ns_spec_check(const double* deltas, int nsteps, double tol, int* needed)
             *needed = k;
             return 0;
         }
-        if (d * d < tol)
+        if (!final && d * d < tol)
         {
             *needed = k + 1;
             return 0;
         }
         prev = d;
+    }
+    if (final)
+    {
+        const double d = deltas[nsteps];
+        if (!(d < 0.25) || d > 0.5 * prev) return -1;
+        if (d < tol)
+        {
+            *needed = nsteps;
+            return 0;
+        }
     }
     *needed = nsteps;
     return 1;

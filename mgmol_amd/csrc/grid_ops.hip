@@ -159,6 +159,41 @@ __global__ void k_axpy(int64_t n, double alpha, const T* x, T* y)
         y[t] += static_cast<T>(alpha * static_cast<double>(x[t]));
 }
 
+// STREAM triad a = b + s c over FP32 arrays, the HBM ceiling the bandwidth-
+// bound kernels are measured against: 16-byte lanes, U independent loads of
+// each operand in flight per lane, one pass (n4 float4 values per array)
+template <int U>
+__global__ void __launch_bounds__(256) k_triad4(const float4* __restrict__ b, const float4* __restrict__ c,
+    float4* __restrict__ a, int64_t n4, float sc)
+{
+    const int64_t base = (int64_t)blockIdx.x * (256 * U) + threadIdx.x;
+    float4 vb[U], vc[U];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+    {
+        const int64_t i = base + (int64_t)u * 256;
+        if (i < n4)
+        {
+            vb[u] = b[i];
+            vc[u] = c[i];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+    {
+        const int64_t i = base + (int64_t)u * 256;
+        if (i < n4)
+        {
+            float4 r;
+            r.x  = vb[u].x + sc * vc[u].x;
+            r.y  = vb[u].y + sc * vc[u].y;
+            r.z  = vb[u].z + sc * vc[u].z;
+            r.w  = vb[u].w + sc * vc[u].w;
+            a[i] = r;
+        }
+    }
+}
+
 // second half of GridFuncVector::jacobi over all sizeg points:
 // w -= f (MPaxpy -1); x += -jf * w (MPaxpy)
 template <typename T>
@@ -1064,6 +1099,57 @@ int mgx_axpy(mgx_ctx* ctx, int dtype, int64_t n, double alpha, const void* x, vo
     int rc = dtype == MGX_F32 ? launch_axpy(ctx, n, alpha, (const float*)x, (float*)y)
                               : launch_axpy(ctx, n, alpha, (const double*)x, (double*)y);
     return rc ? rc : sync(ctx);
+}
+
+int mgx_stream_triad(mgx_ctx* ctx, int64_t n, float* a, const float* b, const float* c, int reps, double* gbps,
+    int* unroll)
+{
+    MGX_REQUIRE(ctx, n > 0 && n % 4 == 0 && a && b && c && gbps && reps > 0, "bad stream triad arguments");
+    MGX_REQUIRE(ctx, ((uintptr_t)a | (uintptr_t)b | (uintptr_t)c) % 16 == 0, "stream triad needs 16-byte alignment");
+    const int64_t n4 = n / 4;
+    hipEvent_t e0, e1;
+    MGX_CHECK_HIP(ctx, hipEventCreate(&e0));
+    MGX_CHECK_HIP(ctx, hipEventCreate(&e1));
+    double best = 0.;
+    int best_u  = 0;
+    const int us[3] = { 2, 4, 8 };
+    for (int k = 0; k < 3; k++)
+    {
+        const int u           = us[k];
+        const unsigned blocks = (unsigned)((n4 + 256 * u - 1) / (256 * u));
+        auto launch           = [&]() {
+            if (u == 2)
+                hipLaunchKernelGGL(k_triad4<2>, dim3(blocks), dim3(256), 0, ctx->stream, (const float4*)b,
+                    (const float4*)c, (float4*)a, n4, 0.5f);
+            else if (u == 4)
+                hipLaunchKernelGGL(k_triad4<4>, dim3(blocks), dim3(256), 0, ctx->stream, (const float4*)b,
+                    (const float4*)c, (float4*)a, n4, 0.5f);
+            else
+                hipLaunchKernelGGL(k_triad4<8>, dim3(blocks), dim3(256), 0, ctx->stream, (const float4*)b,
+                    (const float4*)c, (float4*)a, n4, 0.5f);
+        };
+        launch();
+        launch();
+        MGX_CHECK_HIP(ctx, hipEventRecord(e0, ctx->stream));
+        for (int r = 0; r < reps; r++)
+            launch();
+        MGX_CHECK_HIP(ctx, hipEventRecord(e1, ctx->stream));
+        MGX_CHECK_LAUNCH(ctx);
+        MGX_CHECK_HIP(ctx, hipEventSynchronize(e1));
+        float ms = 0.f;
+        MGX_CHECK_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+        const double rate = 12. * (double)n * reps / (ms * 1e-3) / 1e9;
+        if (rate > best)
+        {
+            best   = rate;
+            best_u = u;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *gbps = best;
+    if (unroll) *unroll = best_u;
+    return MGX_OK;
 }
 
 int mgx_jacobi(mgx_ctx* ctx, int type, double jf, const mgx_block_t* f, mgx_block_t* x, mgx_block_t* w)

@@ -26,7 +26,7 @@ int box_gram_h(mgx_ctx*, const mgx_boxset*, int, const double*, const double*, d
 PairTiles boxset_tiles(const mgx_boxset*);
 int64_t boxset_npairs(const mgx_boxset*);
 int inverse_update_pairs(mgx_ctx*, int, const double*, const double*, const PairTiles&, double*, double*, double*,
-    int*, double);
+    int*, double, bool);
 int box_residual(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, double*, double*);
 int box_sumsq(mgx_ctx*, const mgx_boxset*, const double*, double*);
 int box_norm(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
@@ -50,8 +50,8 @@ int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, doub
 int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, void*, const void*, double,
     double*, bool*, const double*);
 int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
-int inverse_update_spec(mgx_ctx*, int, const double*, const PairTiles&, double*, double*, int, double, double*);
-int ns_spec_check(const double*, int, double, int*);
+int inverse_update_spec(mgx_ctx*, int, const double*, const PairTiles&, double*, double*, int, bool, double*);
+int ns_spec_check(const double*, int, double, bool, int*);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
 int boxset_maxedge(const mgx_boxset*);
@@ -96,9 +96,10 @@ struct mgx_solver
     bool pending_norm = false;
     // learned preconditioner: the state before the last CNN step, to undo it
     mgx_cnn* cnn       = nullptr;
-    double *psi_prev = nullptr, *r_prev = nullptr;
-    bool cnn_pending   = false;
-    double r_prev_norm = 0.;
+    double *psi_prev = nullptr, *r_prev = nullptr;  // the saved state of a trial
+    double *psi_cnn = nullptr, *r_cnn = nullptr;    // the network's step from it, evaluated
+    int trial         = 0;  // 0: none, 1: network step taken, 2: V-cycle step from the same state taken
+    double r_cnn_norm = 0.;
     int backoff        = 0;
     int backoff_len    = MGX_LEARNED_BACKOFF;
     int learned_steps = 0, fallbacks = 0;
@@ -106,6 +107,11 @@ struct mgx_solver
     bool have_inv = false;  // Sinv holds the inverse of a recent S
     int64_t ns_steps = 0, chol_calls = 0, dots_sweeps = 0;
     int ns_pred = 2;  // Newton-Schulz steps taken without host checks (the last step's need)
+    int ns_max_spec = 4;
+    double ns_tol   = 1e-15;
+    int last_more   = 0;  // outcome of the last ns_spec_check (0 accepted, 1 continued, -1 Cholesky)
+    int64_t ns_continued = 0, ns_restarts = 0;  // checked continuations, Cholesky restarts of the refresh
+    int64_t gsize = 0;
     double* host = nullptr;  // pinned: [energy, norm]
 };
 
@@ -117,7 +123,7 @@ namespace
 void free_solver(mgx_solver* s)
 {
     double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
-        s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->psi_prev, s->r_prev };
+        s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->psi_prev, s->r_prev, s->psi_cnn, s->r_cnn };
     for (double* b : bufs)
         if (b) hipFree(b);
     if (s->kr0) (void)hipFree(s->kr0);
@@ -134,7 +140,10 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     mgx_ctx* ctx = s->ctx;
     int rc, tk;
     const int n = s->n;
-    tk = prof_begin(ctx, MGX_PROF_HLOC);
+    // H psi: psi read, H psi written (FP64), V once; psi written back when
+    // the pending normalisation is applied on the way
+    const double np = (double)s->npts;
+    tk = prof_begin(ctx, MGX_PROF_HLOC, (s->pending_norm ? 24. : 16.) * np + 8. * (double)s->gsize);
     if (s->pending_norm)
     {
         // LocGridOrbitals::normalize of the last update, fused with Hpsi
@@ -149,12 +158,12 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_DENSE);
     // S^-1: Cholesky + inverse (GramMatrix::updateLS / computeInverse) on the
-    // first step, then Newton-Schulz refresh from the previous inverse
-    // accuracy of the refreshed inverse: FP64 (1e-15) with FP64 S; with the
-    // split S (entries good to ~1e-8) 1e-10 suffices, one step fewer
-    const double ns_tol = s->cfg.gram_prec == MGX_PREC_SPLIT ? 1e-10 : 1e-15;
+    // first step (every step with MGX_INV_CHOL), then Newton-Schulz refresh
+    // from the previous inverse, to ns_tol
+    const double ns_tol = s->ns_tol;
+    const bool bf16     = s->cfg.inverse_mode == MGX_INV_NS_BF16;
     int spec = 0;  // Newton-Schulz steps taken unchecked, their deltas in scal[8..]
-    if (!s->have_inv)
+    if (!s->have_inv || s->cfg.inverse_mode == MGX_INV_CHOL)
     {
         if ((rc = chol_inv(ctx, n, s->S, s->L, s->Sinv))) return rc;
         s->chol_calls++;
@@ -164,8 +173,8 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     {
         // as many steps as the last refresh needed, checked with the energy
         // and the residual norm below (one host wait per step)
-        spec = std::max(1, std::min(s->ns_pred, 4));
-        if ((rc = inverse_update_spec(ctx, n, s->Sv, s->tiles, s->Sinv, s->nswork, spec, ns_tol, s->scal + 8)))
+        spec = std::max(1, std::min(s->ns_pred, s->ns_max_spec));
+        if ((rc = inverse_update_spec(ctx, n, s->Sv, s->tiles, s->Sinv, s->nswork, spec, bf16, s->scal + 8)))
             return rc;
     }
     // theta, the band energy, the residual and its norm from the current S^-1
@@ -180,8 +189,8 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         if ((r = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d))) return r;
         if ((r = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return r;
         prof_end(ctx, tk);
-        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, (8 + spec) * sizeof(double), hipMemcpyDeviceToHost,
-                               ctx->stream));
+        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, (8 + spec + (bf16 ? 1 : 0)) * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
         MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
         return MGX_OK;
     };
@@ -189,7 +198,8 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     if (spec > 0)
     {
         int needed     = 0;
-        const int more = ns_spec_check(s->host + 8, spec, ns_tol, &needed);
+        const int more = ns_spec_check(s->host + 8, spec, ns_tol, bf16, &needed);
+        s->last_more   = more;
         if (more == 0)
         {
             s->ns_steps += spec;
@@ -208,10 +218,14 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
                 steps = -1;
             }
             else if ((rc = inverse_update_pairs(ctx, n, s->S, s->Sv, s->tiles, s->L, s->Sinv, s->nswork, &steps,
-                          ns_tol)))
+                          ns_tol, bf16)))
                 return rc;
             s->ns_steps += spec + std::max(steps, 0);
             if (steps < 0) s->chol_calls++;
+            if (more < 0 || steps < 0)
+                s->ns_restarts++;
+            else
+                s->ns_continued++;
             s->ns_pred = steps < 0 ? 2 : spec + steps;
             spec       = 0;
             if ((rc = project())) return rc;
@@ -231,6 +245,10 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         s->deig2              = std::max(s->deig, deig_old);
     }
     s->e_prev = st->band_energy;
+    st->ns_steps     = (int)s->ns_steps;
+    st->chol_calls   = (int)s->chol_calls;
+    st->ns_continued = (int)s->ns_continued;
+    st->ns_restarts  = (int)s->ns_restarts;
     // Anderson starts once the eigenvalue-sum change (2 x Ry = 4 x Ha of the
     // band energy) is below 1e-2 per state
     s->accelerate = s->accelerate || (s->cfg.anderson_m > 0 && 4. * s->deig2 < 1e-2 * n);
@@ -239,38 +257,61 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     else
         *done = (tol > 0. && rnorm < tol);
     if (*done) return MGX_OK;
-    // learned preconditioner: judge the last CNN step by the residual norm
-    bool use_cnn = false, revert = false;
+    // learned preconditioner.  A trial takes the network's correction as a
+    // plain preconditioned step (Anderson history cleared) from the saved
+    // state (psi, R); the next step evaluates it, keeps it aside and takes
+    // the V-cycle's plain step from the same saved state; the step after that
+    // evaluates the V-cycle step and keeps whichever of the two reached the
+    // smaller residual norm.  The V-cycle alone then runs for backoff_len
+    // steps; each rejection in a row doubles that (up to
+    // MGX_LEARNED_BACKOFF_MAX), an accepted trial resets it.
+    bool use_cnn = false, restored = false, plain_step = false;
     if (s->cfg.precond_kind == MGX_PRECOND_LEARNED)
     {
-        if (s->cnn_pending && rnorm > s->r_prev_norm)
+        const size_t bytes = s->npts * sizeof(double);
+        if (s->trial == 1)
         {
-            // rejected: the V-cycle for the next backoff_len steps; each
-            // rejection in a row doubles that (up to MGX_LEARNED_BACKOFF_MAX)
-            revert = true;
-            s->fallbacks++;
-            s->backoff     = s->backoff_len;
-            s->backoff_len = std::min(2 * s->backoff_len, MGX_LEARNED_BACKOFF_MAX);
+            s->r_cnn_norm = rnorm;
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi_cnn, s->psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->r_cnn, s->res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi, s->psi_prev, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->res, s->r_prev, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            restored   = true;
+            plain_step = true;
+            s->trial   = 2;
         }
-        else if (s->cnn_pending)
-            s->backoff_len = MGX_LEARNED_BACKOFF;
-        s->cnn_pending = false;
-        if (revert)
+        else if (s->trial == 2)
         {
-            // back to the state before the CNN step; redo it with the V-cycle
-            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi, s->psi_prev, s->npts * sizeof(double),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->res, s->r_prev, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
-                                   ctx->stream));
-            s->mm = 0;
+            s->trial = 0;
+            if (s->r_cnn_norm < rnorm)
+            {
+                // the network's step wins: back to it
+                MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi, s->psi_cnn, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+                MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->res, s->r_cnn, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+                restored       = true;
+                s->backoff_len = MGX_LEARNED_BACKOFF;
+                s->backoff     = MGX_LEARNED_BACKOFF;
+            }
+            else
+            {
+                s->fallbacks++;
+                s->backoff     = s->backoff_len;
+                s->backoff_len = std::min(2 * s->backoff_len, MGX_LEARNED_BACKOFF_MAX);
+            }
         }
         else if (s->backoff > 0)
             s->backoff--;
         else
         {
             if (!s->cnn) return set_err(ctx, MGX_EINVAL, "no network attached (mgx_solver_set_cnn)");
-            use_cnn = true;
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi_prev, s->psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->r_prev, s->res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            use_cnn    = true;
+            plain_step = true;
+            s->trial   = 1;
+            s->learned_steps++;
         }
+        if (plain_step || restored) s->mm = 0;
     }
     double gamma = 0.;
     if (s->cfg.precond_start == MGX_START_GAMMA)
@@ -291,8 +332,6 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     st->lowest_eig     = s->lowest;
     st->learned_steps  = s->learned_steps;
     st->fallbacks      = s->fallbacks;
-    st->ns_steps       = (int)s->ns_steps;
-    st->chol_calls     = (int)s->chol_calls;
     st->dots_sweeps    = (int)s->dots_sweeps;
     const double alpha = 0.5 * s->cfg.step_length;
     bool norms_done    = false;  // d already holds the norms of the updated psi
@@ -302,7 +341,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     const int prec = s->cfg.precond_dtype;
     // the correction's type: FP32 out of the FP16 V-cycle
     const int kprec = prec == MGX_F16 ? MGX_F32 : prec;
-    if (!use_cnn && !s->accelerate && prec != MGX_F16)
+    if (!use_cnn && !s->accelerate && !restored && prec != MGX_F16)
     {
         // preconditioned step psi += alpha K(R) (ABPG::update_states, no
         // extrapolation), fused with the V-cycle's last kernel
@@ -316,34 +355,31 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         const void* kr = nullptr;
         if (use_cnn)
         {
-            // keep the pre-step state for the residual test of the next step
-            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->psi_prev, s->psi, s->npts * sizeof(double),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-            MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->r_prev, s->res, s->npts * sizeof(double), hipMemcpyDeviceToDevice,
-                                   ctx->stream));
             int edge = 0;
             boxset_uniform_edge(s->bs, &edge);
-            const int tl = prof_begin(ctx, MGX_PROF_LEARNED);
+            // 4 conv3d layers 1->16->16->16->1, 3^3 taps: 2 * 27 * 544 flops per voxel
+            const int tl = prof_begin(ctx, MGX_PROF_LEARNED, 2. * 27. * 544. * np);
             if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->kcnn, kprec))) return rc;
             prof_end(ctx, tl);
-            kr             = s->kcnn;
-            s->cnn_pending = true;
-            s->r_prev_norm = rnorm;
-            s->learned_steps++;
+            kr = s->kcnn;
         }
         else
         {
             // d holds the residual's per-box sums unless R was just restored
             if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, s->kr1,
                      (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done,
-                     revert ? nullptr : s->d)))
+                     restored ? nullptr : s->d)))
                 return rc;
             kr = s->kr1;
         }
-        if (!s->accelerate)
+        const double sk = kprec == MGX_F64 ? 8. : 4.;
+        if (!s->accelerate || plain_step)
         {
-            // psi += alpha K(R), with the norms of the new psi
+            // psi += alpha K(R), with the norms of the new psi: psi read and
+            // written, the correction read
+            const int tu = prof_begin(ctx, MGX_PROF_UPDATE, (16. + sk) * np);
             if ((rc = box_psi_step_sq(ctx, s->bs, kprec, s->psi, kr, alpha, s->d))) return rc;
+            prof_end(ctx, tu);
             norms_done = true;
         }
         else
@@ -372,11 +408,15 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
             // boxes up to 32^3: the normalisation in the same pass
             const bool keep = kr == s->kr1;
             normalized      = boxset_maxedge(s->bs) <= 32;
+            // psi and the previous psi read, the new psi written (FP64); the
+            // correction and the previous one read (+ the copy when not kept)
+            const int tu = prof_begin(ctx, MGX_PROF_UPDATE, (24. + (keep ? 2. : 3.) * sk) * np);
             if ((rc = normalized ? box_anderson_update_nrm(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
                                        keep ? nullptr : s->kr0, thp, beta, s->d)
                                  : box_anderson_update_sq(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
                                        keep ? nullptr : s->kr0, thp, beta, s->d)))
                 return rc;
+            prof_end(ctx, tu);
             std::swap(s->psi, s->x0);
             if (keep) std::swap(s->kr0, s->kr1);
             norms_done = true;
@@ -444,6 +484,11 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     }
     MGX_REQUIRE(ctx, cfg->conv_criterion == MGX_CONV_DELTAE || cfg->conv_criterion == MGX_CONV_RESIDUAL,
         "bad convergence criterion");
+    MGX_REQUIRE(ctx, cfg->inverse_mode == MGX_INV_NS_F64 || cfg->inverse_mode == MGX_INV_NS_BF16
+                         || cfg->inverse_mode == MGX_INV_CHOL,
+        "bad inverse mode");
+    MGX_REQUIRE(ctx, cfg->ns_tol >= 0. && cfg->ns_tol < 0.25, "Newton-Schulz tolerance must be in [0, 0.25)");
+    MGX_REQUIRE(ctx, cfg->ns_max_spec >= 0 && cfg->ns_max_spec <= 8, "ns_max_spec must be in [0, 8]");
     int64_t dummy = 0;
     MGX_REQUIRE(ctx, mgx_boxset_size(bs, cfg->mg_levels, &dummy) == MGX_OK, "box set has too few levels");
     mgx_solver* s = new mgx_solver();
@@ -455,12 +500,17 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     boxset_geom(bs, g, s->h, &s->n);
     s->tiles = boxset_tiles(bs);
     const int64_t gsize = (int64_t)g[0] * g[1] * g[2];
+    s->gsize            = gsize;
+    // accuracy of the refreshed inverse: FP64 (1e-15) with FP64 S; with the
+    // split S (entries good to ~1e-8) 1e-10 suffices, one step fewer
+    s->ns_tol      = cfg->ns_tol > 0. ? cfg->ns_tol : (cfg->gram_prec == MGX_PREC_SPLIT ? 1e-10 : 1e-15);
+    s->ns_max_spec = cfg->ns_max_spec > 0 ? cfg->ns_max_spec : 4;
     const size_t nn     = (size_t)s->n * s->n;
     auto alloc = [&](double** p, size_t cnt) { return hipMalloc((void**)p, cnt * sizeof(double)) == hipSuccess; };
     const size_t tsize = cfg->precond_dtype == MGX_F64 ? sizeof(double) : sizeof(float);
     bool ok = alloc(&s->psi, s->npts) && alloc(&s->psi0, s->npts) && alloc(&s->hpsi, s->npts) && alloc(&s->res, s->npts) && alloc(&s->vtot, gsize)
               && alloc(&s->S, nn) && alloc(&s->H, nn) && alloc(&s->L, nn) && alloc(&s->Sinv, nn) && alloc(&s->th, nn)
-              && alloc(&s->eig, s->n) && alloc(&s->d, 2 * (size_t)s->n + 8) && alloc(&s->scal, 16) && alloc(&s->nswork, 2 * nn)
+              && alloc(&s->eig, s->n) && alloc(&s->d, 2 * (size_t)s->n + 8) && alloc(&s->scal, 32) && alloc(&s->nswork, 2 * nn)
               && alloc(&s->Sv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && (cfg->anderson_m == 0
@@ -468,9 +518,10 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
               && ((cfg->anderson_m == 0 && cfg->precond_dtype != MGX_F16)
                   || hipMalloc(&s->kr1, s->npts * tsize) == hipSuccess)
               && (cfg->precond_kind != MGX_PRECOND_LEARNED
-                  || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts)
+                  || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts) && alloc(&s->psi_cnn, s->npts)
+                      && alloc(&s->r_cnn, s->npts)
                       && hipMalloc(&s->kcnn, s->npts * tsize) == hipSuccess))
-              && hipHostMalloc((void**)&s->host, 16 * sizeof(double), hipHostMallocDefault) == hipSuccess;
+              && hipHostMalloc((void**)&s->host, 32 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {
         free_solver(s);
@@ -550,7 +601,7 @@ int mgx_solver_reset(mgx_solver* s)
     s->deig2      = 1e9;
     s->accelerate = false;
     s->mm         = 0;
-    s->cnn_pending   = false;
+    s->trial         = 0;
     s->backoff       = 0;
     s->backoff_len   = MGX_LEARNED_BACKOFF;
     s->learned_steps = 0;
@@ -559,6 +610,9 @@ int mgx_solver_reset(mgx_solver* s)
     s->ns_pred       = 2;
     s->chol_calls    = 0;
     s->dots_sweeps   = 0;
+    s->ns_continued  = 0;
+    s->ns_restarts   = 0;
+    s->last_more     = 0;
     return MGX_OK;
 }
 
@@ -571,10 +625,26 @@ int mgx_solver_profile(mgx_solver* s, double* ms, int64_t* launches, int reset)
         if (launches) launches[g] = s->prof.cnt[g];
         if (reset)
         {
-            s->prof.ms[g]  = 0.;
-            s->prof.cnt[g] = 0;
+            s->prof.ms[g]   = 0.;
+            s->prof.cnt[g]  = 0;
+            s->prof.wsum[g] = 0.;
         }
     }
+    return MGX_OK;
+}
+
+int mgx_solver_set_profile(mgx_solver* s, int on)
+{
+    if (!s) return MGX_EINVAL;
+    s->cfg.profile = on ? 1 : 0;
+    return MGX_OK;
+}
+
+int mgx_solver_profile_work(mgx_solver* s, double* work)
+{
+    if (!s || !work) return MGX_EINVAL;
+    for (int g = 0; g < MGX_PROF_N; g++)
+        work[g] = s->prof.wsum[g];
     return MGX_OK;
 }
 
@@ -584,6 +654,28 @@ int mgx_solver_get_psi(mgx_solver* s, double* host)
     int rc = flush_norm(s);
     if (rc) return rc;
     MGX_CHECK_HIP(s->ctx, hipMemcpy(host, s->psi, s->npts * sizeof(double), hipMemcpyDeviceToHost));
+    return MGX_OK;
+}
+
+int mgx_solver_inverse(mgx_solver* s, double** Sinv)
+{
+    if (!s || !Sinv) return MGX_EINVAL;
+    *Sinv = s->Sinv;
+    return MGX_OK;
+}
+
+int mgx_solver_set_psi(mgx_solver* s, const double* host)
+{
+    if (!s || !host) return MGX_EINVAL;
+    MGX_CHECK_HIP(s->ctx, hipMemcpyAsync(s->psi, host, s->npts * sizeof(double), hipMemcpyHostToDevice,
+                              s->ctx->stream));
+    MGX_CHECK_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    // the caller's orbitals as they are: no pending normalisation, no
+    // Anderson history, no network trial in flight; S^-1 and the step
+    // counts stay as the refresh's start
+    s->pending_norm = false;
+    s->mm           = 0;
+    s->trial        = 0;
     return MGX_OK;
 }
 

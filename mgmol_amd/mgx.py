@@ -1,8 +1,9 @@
 """ctypes binding of libmgx.so (include/mgx.h).
 
-Device memory is plain pointers; callers here use torch tensors on
-``cuda:0`` for allocation only (``tensor.data_ptr()``).  There is no CPU
-path: if ``libmgx.so`` is missing or fails to load, ``lib()`` raises.
+Device memory is plain pointers, allocated and copied through the C ABI
+itself (``DeviceArray``: mgx_malloc / mgx_memcpy_*), so a process loads one
+HIP runtime only.  There is no CPU path: if ``libmgx.so`` is missing or
+fails to load, ``lib()`` raises.
 """
 import ctypes as C
 import os
@@ -17,7 +18,9 @@ PREC_F64, PREC_SPLIT = 0, 1
 START_GAMMA, START_ZERO = 0, 1
 CONV_DELTAE, CONV_RESIDUAL = 0, 1
 PRECOND_VCYCLE, PRECOND_LEARNED = 0, 1
-PROF_GROUPS = ["hloc", "gram", "dense", "resid", "jacobi0", "precond", "normalize", "learned", "cellgram"]
+INV_NS_F64, INV_NS_BF16, INV_CHOL = 0, 1, 2
+PROF_GROUPS = ["hloc", "gram", "dense", "resid", "jacobi0", "precond", "normalize", "learned", "cellgram",
+               "skin", "residk", "update", "jacobi0d", "first", "nsgemm", "spmm", "transfer", "nsgemm16"]
 
 _LIB = None
 
@@ -25,7 +28,8 @@ _LIB = None
 EXPORTS = [
     "mgx_ctx_create", "mgx_ctx_destroy", "mgx_last_error", "mgx_ctx_stream",
     "mgx_malloc", "mgx_free", "mgx_memcpy_htod", "mgx_memcpy_dtoh",
-    "mgx_halo", "mgx_lap_apply", "mgx_rhs_apply", "mgx_pointwise_product", "mgx_axpy", "mgx_jacobi",
+    "mgx_halo", "mgx_lap_apply", "mgx_rhs_apply", "mgx_pointwise_product", "mgx_axpy", "mgx_stream_triad",
+    "mgx_jacobi",
     "mgx_restrict", "mgx_prolong", "mgx_mask", "mgx_mg",
     "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg",
     "mgx_local_gram", "mgx_local_product", "mgx_local_to_global", "mgx_chol_inv", "mgx_theta", "mgx_eigsum",
@@ -34,7 +38,8 @@ EXPORTS = [
     "mgx_box_lap", "mgx_box_hloc", "mgx_box_jacobi", "mgx_box_restrict", "mgx_box_prolong", "mgx_box_precond_mg",
     "mgx_box_gram_h", "mgx_box_residual", "mgx_box_residual_norm", "mgx_box_density",
     "mgx_solver_create", "mgx_solver_destroy", "mgx_solver_step", "mgx_solver_run", "mgx_solver_reset",
-    "mgx_solver_profile", "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_set_cnn", "mgx_solver_density",
+    "mgx_solver_profile", "mgx_solver_profile_work", "mgx_solver_set_profile",
+    "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_inverse", "mgx_solver_set_psi", "mgx_solver_set_cnn", "mgx_solver_density",
     "mgx_cnn_nparams", "mgx_cnn_create", "mgx_cnn_destroy", "mgx_cnn_set_params", "mgx_cnn_get_params",
     "mgx_cnn_forward", "mgx_cnn_grad", "mgx_cnn_train_step", "mgx_cnn_apply", "mgx_cnn_make_pairs",
 ]
@@ -67,14 +72,15 @@ class SolverCfg(C.Structure):
     _fields_ = [("lap_type", C.c_int), ("mg_levels", C.c_int), ("precond_dtype", C.c_int),
                 ("gram_prec", C.c_int), ("step_length", C.c_double), ("ritz_every", C.c_int),
                 ("precond_start", C.c_int), ("profile", C.c_int), ("anderson_m", C.c_int),
-                ("anderson_beta", C.c_double), ("conv_criterion", C.c_int), ("precond_kind", C.c_int)]
+                ("anderson_beta", C.c_double), ("conv_criterion", C.c_int), ("precond_kind", C.c_int),
+                ("inverse_mode", C.c_int), ("ns_tol", C.c_double), ("ns_max_spec", C.c_int)]
 
 
 class SolverStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("band_energy", C.c_double), ("seconds", C.c_double), ("lowest_eig", C.c_double),
                 ("learned_steps", C.c_int), ("fallbacks", C.c_int), ("ns_steps", C.c_int), ("chol_calls", C.c_int),
-                ("dots_sweeps", C.c_int)]
+                ("dots_sweeps", C.c_int), ("ns_continued", C.c_int), ("ns_restarts", C.c_int)]
 
 
 class PoissonStats(C.Structure):
@@ -102,6 +108,7 @@ def lib():
             "mgx_memcpy_htod": (P, P, P, C.c_size_t), "mgx_memcpy_dtoh": (P, P, P, C.c_size_t),
             "mgx_halo": (P, BP), "mgx_lap_apply": (P, I, BP, BP), "mgx_rhs_apply": (P, BP, BP),
             "mgx_pointwise_product": (P, BP, P, BP), "mgx_axpy": (P, I, I64, D, P, P),
+            "mgx_stream_triad": (P, I64, P, P, P, I, C.POINTER(D), C.POINTER(I)),
             "mgx_jacobi": (P, I, D, BP, BP, BP), "mgx_restrict": (P, BP, BP), "mgx_prolong": (P, BP, BP),
             "mgx_mask": (P, BP, P), "mgx_mg": (P, I, I, BP, BP, PP),
             "mgx_poisson_mg": (P, I, BP, BP, I, I, I, D, I, C.POINTER(PoissonStats)),
@@ -130,7 +137,9 @@ def lib():
             "mgx_solver_destroy": (P,), "mgx_solver_step": (P, C.POINTER(SolverStats)),
             "mgx_solver_run": (P, D, I, C.POINTER(SolverStats)), "mgx_solver_get_psi": (P, P),
             "mgx_solver_reset": (P,), "mgx_solver_profile": (P, C.POINTER(D), C.POINTER(I64), I),
-            "mgx_solver_buffers": (P, PP, PP, PP, PP, PP), "mgx_solver_set_cnn": (P, P),
+            "mgx_solver_profile_work": (P, C.POINTER(D)), "mgx_solver_set_profile": (P, I),
+            "mgx_solver_buffers": (P, PP, PP, PP, PP, PP), "mgx_solver_inverse": (P, PP),
+            "mgx_solver_set_psi": (P, P), "mgx_solver_set_cnn": (P, P),
             "mgx_solver_density": (P, D, P, C.POINTER(D)),
             "mgx_cnn_create": (P, I, I, C.POINTER(P)), "mgx_cnn_destroy": (P,),
             "mgx_cnn_set_params": (P, P), "mgx_cnn_get_params": (P, P), "mgx_cnn_forward": (P, P, I, P),
@@ -308,12 +317,14 @@ class BoxSet:
 class Solver:
     def __init__(self, ctx, boxset, vtot, psi0, mg_levels=2, precond_dtype=F64, gram_prec=PREC_F64,
                  step_length=1.5, ritz_every=0, precond_start=START_ZERO, profile=0, anderson_m=0,
-                 anderson_beta=1.0, conv_criterion=CONV_RESIDUAL, precond_kind=PRECOND_VCYCLE, cnn=None):
+                 anderson_beta=1.0, conv_criterion=CONV_RESIDUAL, precond_kind=PRECOND_VCYCLE, cnn=None,
+                 inverse_mode=INV_NS_F64, ns_tol=0.0, ns_max_spec=0):
         import numpy as np
         self.ctx = ctx
         self.boxset = boxset
         cfg = SolverCfg(LAP_4TH, mg_levels, precond_dtype, gram_prec, step_length, ritz_every, precond_start,
-                        profile, anderson_m, anderson_beta, conv_criterion, precond_kind)
+                        profile, anderson_m, anderson_beta, conv_criterion, precond_kind, inverse_mode, ns_tol,
+                        ns_max_spec)
         v = np.ascontiguousarray(vtot, dtype=np.float64)
         p = np.ascontiguousarray(psi0, dtype=np.float64)
         self.npts = p.size
@@ -348,10 +359,37 @@ class Solver:
         self.ctx.check(lib().mgx_solver_reset(self.h), "mgx_solver_reset")
 
     def profile(self, reset=False):
+        """{group: (ms, launches, work)} accumulated since the last reset;
+        work = algorithmic bytes (flops for learned, nsgemm, nsgemm16)."""
         ms = (C.c_double * len(PROF_GROUPS))()
         cnt = (C.c_int64 * len(PROF_GROUPS))()
+        work = (C.c_double * len(PROF_GROUPS))()
+        self.ctx.check(lib().mgx_solver_profile_work(self.h, work), "mgx_solver_profile_work")
         self.ctx.check(lib().mgx_solver_profile(self.h, ms, cnt, 1 if reset else 0), "mgx_solver_profile")
-        return {g: (ms[i], cnt[i]) for i, g in enumerate(PROF_GROUPS)}
+        return {g: (ms[i], cnt[i], work[i]) for i, g in enumerate(PROF_GROUPS)}
+
+    def set_psi(self, psi):
+        import numpy as np
+        p = np.ascontiguousarray(psi, dtype=np.float64)
+        assert p.size == self.npts
+        self.ctx.check(lib().mgx_solver_set_psi(self.h, p.ctypes.data_as(C.c_void_p)), "mgx_solver_set_psi")
+
+    def matrices(self):
+        """host copies of (S, S^-1) of the last evaluated step (n x n)."""
+        import numpy as np
+        n = self.boxset.nbox
+        ps, pi = C.c_void_p(), C.c_void_p()
+        self.ctx.check(lib().mgx_solver_buffers(self.h, None, None, C.byref(ps), None, None), "mgx_solver_buffers")
+        self.ctx.check(lib().mgx_solver_inverse(self.h, C.byref(pi)), "mgx_solver_inverse")
+        out = []
+        for p in (ps, pi):
+            a = np.empty((n, n), dtype=np.float64)
+            self.ctx.check(lib().mgx_memcpy_dtoh(self.ctx.h, a.ctypes.data_as(C.c_void_p), p, a.nbytes), "dtoh")
+            out.append(a.T.copy())  # column-major on the device
+        return out[0], out[1]
+
+    def set_profile(self, on):
+        self.ctx.check(lib().mgx_solver_set_profile(self.h, 1 if on else 0), "mgx_solver_set_profile")
 
     def psi(self):
         import numpy as np

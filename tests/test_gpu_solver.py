@@ -131,11 +131,13 @@ def test_solver_deltae_stops_like_oracle(ctx):
 
 
 def test_solver_learned_fallback_to_vcycle(ctx):
-    """A network that points uphill (y = -10 x) is rejected by the residual
-    test: psi and R are restored and the V-cycle step is taken, so after the
-    rejection the run retraces the plain V-cycle run, and the CNN is tried
-    again after MGX_LEARNED_BACKOFF V-cycle steps, then after twice as
-    many once it has been rejected again."""
+    """A network that points uphill (y = -10 x): the trial's network step is
+    evaluated (the energy rises), the solver returns to the saved state and
+    takes the V-cycle step from it, evaluates that, and keeps the V-cycle
+    step.  So the learned run is the plain V-cycle run with one extra
+    evaluation per trial; the network is tried again after
+    MGX_LEARNED_BACKOFF V-cycle steps, then after twice as many once it has
+    lost again."""
     from mgmol_amd import learned
     cfg = synth.box_config(nbox=16, gdim=48, h=0.3, edges=(16,), nwells=4, seed=11, levels=3)
     bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
@@ -146,27 +148,72 @@ def test_solver_learned_fallback_to_vcycle(ctx):
     kw = dict(mg_levels=2, precond_dtype=mgx.F64)
     plain = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, **kw)
     lrn = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, precond_kind=mgx.PRECOND_LEARNED, cnn=net, **kw)
-    ep = [plain.step().band_energy for _ in range(6)]
-    sts = [lrn.step() for _ in range(18)]
+    ep = [plain.step().band_energy for _ in range(8)]
+    sts = [lrn.step() for _ in range(19)]
     el = [st.band_energy for st in sts]
     say("plain", ep)
     say("learned", el, [(st.learned_steps, st.fallbacks) for st in sts])
     assert el[0] == pytest.approx(ep[0], rel=1e-13)
-    assert el[1] > el[0]  # the uphill CNN step
-    for k in range(1, 6):
-        assert el[k + 1] == pytest.approx(ep[k], rel=1e-10)
-    # the counters a step reports are those before its own update
-    assert sts[1].fallbacks == 1 and sts[1].learned_steps == 1
-    # steps 2..5 use the V-cycle; step 6 tries the network again, and step 7
-    # rejects it
-    assert sts[6].learned_steps == 1 and sts[6].fallbacks == 1
-    assert sts[7].learned_steps == 2 and sts[7].fallbacks == 2
-    # the second rejection in a row: 8 V-cycle steps (8..15), then the
-    # network at step 16, rejected at step 17
-    for k in range(8, 17):
+    assert el[1] > el[0]  # the uphill network step, evaluated
+    # step 1 returns to the saved state and takes the V-cycle step, which
+    # step 2 evaluates: from there on the plain run's states
+    for k in range(2, 8):
+        assert el[k] == pytest.approx(ep[k - 1], rel=1e-10)
+    # a step reports the counters after its own decision
+    assert sts[0].learned_steps == 1 and sts[0].fallbacks == 0
+    assert sts[1].learned_steps == 1 and sts[1].fallbacks == 0
+    for k in range(2, 7):  # lost at step 2, then 4 V-cycle steps (3..6)
+        assert sts[k].learned_steps == 1 and sts[k].fallbacks == 1
+    assert sts[7].learned_steps == 2  # the second trial
+    assert el[8] > el[7]
+    assert el[9] == pytest.approx(ep[7], rel=1e-10)
+    # lost again at step 9: 8 V-cycle steps (10..17), the third trial at 18
+    for k in range(9, 18):
         assert sts[k].learned_steps == 2 and sts[k].fallbacks == 2
-    assert sts[17].learned_steps == 3 and sts[17].fallbacks == 3
+    assert sts[18].learned_steps == 3
     for s in (plain, lrn):
         s.close()
     net.close()
     bs.close()
+
+
+def test_solver_refresh_branches(ctx):
+    """The Newton-Schulz refresh's checked branches.  With at most one
+    unchecked step per refresh (ns_max_spec = 1) the solver continues with
+    checked steps whenever one is not enough (ns_continued); orbitals replaced
+    by very different ones make the refresh diverge from the old inverse, and
+    Cholesky takes over (ns_restarts).  Energies and residuals equal those of
+    a run with Cholesky + inverse every step (MGX_INV_CHOL, the reference's
+    GramMatrix::updateLS / computeInverse)."""
+    cfg = _cfg()
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    kw = dict(mg_levels=2, precond_dtype=mgx.F64, anderson_m=1)
+    chol = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, inverse_mode=mgx.INV_CHOL, **kw)
+    capped = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, ns_max_spec=1, **kw)
+    free = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, **kw)
+    try:
+        for k in range(12):
+            a, b, c = chol.step(), capped.step(), free.step()
+            say("refresh step", k, a.band_energy, b.band_energy, c.band_energy, b.ns_continued, c.ns_continued)
+            for x in (b, c):
+                assert x.band_energy == pytest.approx(a.band_energy, rel=1e-12)
+                assert x.residual == pytest.approx(a.residual, rel=1e-9)
+        assert a.chol_calls == 12 and a.ns_steps == 0
+        assert b.chol_calls == 1 and b.ns_continued > 0 and b.ns_restarts == 0
+        assert c.ns_restarts == 0
+        # very different orbitals: the old inverse is far off, the refresh
+        # diverges and Cholesky restarts it
+        rng = np.random.default_rng(4)
+        psi2 = cfg.psi + 0.3 * np.max(np.abs(cfg.psi)) * rng.uniform(-1.0, 1.0, cfg.psi.size)
+        capped.set_psi(psi2)
+        fresh = mgx.Solver(ctx, bs, cfg.vtot, psi2, **kw)
+        b2, f2 = capped.step(), fresh.step()
+        say("after set_psi", b2.band_energy, f2.band_energy, b2.ns_restarts, b2.chol_calls)
+        assert b2.ns_restarts == 1 and b2.chol_calls == 2
+        assert b2.band_energy == pytest.approx(f2.band_energy, rel=1e-12)
+        assert b2.residual == pytest.approx(f2.residual, rel=1e-9)
+        fresh.close()
+    finally:
+        for s in (chol, capped, free):
+            s.close()
+        bs.close()
