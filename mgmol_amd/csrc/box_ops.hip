@@ -1504,13 +1504,18 @@ __global__ void __launch_bounds__(256) k_slab_skin(const double* psi, const int6
         if (i >= j) continue;
         const int sh[3] = { E[1], E[2], E[3] };
         const int xd1 = Xd1 + sh[d], xd2 = Xd2 + sh[d];
-        const bool in1 = xd1 >= 0 && xd1 < ei, in2 = xd2 >= 0 && xd2 < ei;
+        const int lo_u = max(0, -sh[u]), hi_u = min(ej, ei - sh[u]);
+        const int lo_w = max(0, -sh[w]), hi_w = min(ej, ei - sh[w]);
+        const int nu = hi_u - lo_u, nww = hi_w - lo_w;
+        // box i must also lie across the face (u and w ranges not empty:
+        // boxes that are apart along u or w are in the list when they come
+        // within 2 points along every axis, and two negative extents would
+        // make a positive count)
+        const bool across = nu > 0 && nww > 0;
+        const bool in1 = across && xd1 >= 0 && xd1 < ei, in2 = across && xd2 >= 0 && xd2 < ei;
         double acc1 = 0., acc2 = 0.;
         if (in1 || in2)
         {
-            const int lo_u = max(0, -sh[u]), hi_u = min(ej, ei - sh[u]);
-            const int lo_w = max(0, -sh[w]), hi_w = min(ej, ei - sh[w]);
-            const int nu = hi_u - lo_u, nww = hi_w - lo_w;
             const double* pi = psi + off[i];
             for (int t = lane; t < nu * nww; t += 64)
             {

@@ -515,7 +515,10 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
               && alloc(&s->Hv, (size_t)std::max<int64_t>(boxset_npairs(bs), 1))
               && (cfg->anderson_m == 0
                   || (alloc(&s->x0, s->npts) && hipMalloc(&s->kr0, s->npts * tsize) == hipSuccess))
-              && ((cfg->anderson_m == 0 && cfg->precond_dtype != MGX_F16)
+              // the V-cycle's correction is stored for Anderson, for FP16
+              // storage and for the learned path's V-cycle step after a trial
+              && ((cfg->anderson_m == 0 && cfg->precond_dtype != MGX_F16
+                      && cfg->precond_kind != MGX_PRECOND_LEARNED)
                   || hipMalloc(&s->kr1, s->npts * tsize) == hipSuccess)
               && (cfg->precond_kind != MGX_PRECOND_LEARNED
                   || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts) && alloc(&s->psi_cnn, s->npts)

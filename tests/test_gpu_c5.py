@@ -5,7 +5,10 @@ Newton-Schulz refresh of S^-1 on its pair-sparse S at n = 2048.
 * plain, mixed (FP32 V-cycle, split S/H) and the FP16/BF16 extension from the
   same initial orbitals: the first step's band energy and residual agree to
   the split products' accuracy; later steps (whose states differ by the
-  V-cycle's rounding) to 1e-7 in energy; all paths lower the residual.
+  V-cycle's rounding) to 1e-7 in energy for the FP32 V-cycle and to 1e-5 for
+  the FP16-storage one, whose correction differs from the FP32 V-cycle's by
+  up to 1.4e-3 of each box's largest value (measured 2.6e-6 of the energy
+  after the second step); all paths lower the residual.
 * After the refresh (FP64 products, and the BF16 correction), max|I - X S|
   against the Cholesky inverse of the same S (mgx_chol_inv): FP64 refresh at
   1e-15 within 1e-12 (the dense products' rounding over 2048 terms), the split
@@ -66,8 +69,9 @@ def test_c5_paths_agree(ctx, c5):
         e, r = out[path][0]
         assert e == pytest.approx(e0, rel=1e-9)
         assert r == pytest.approx(r0, rel=1e-6)
+        tol = 1e-7 if path == "mixed" else 1e-5
         for k in range(1, 6):
-            assert out[path][k][0] == pytest.approx(out["plain"][k][0], rel=1e-7)
+            assert out[path][k][0] == pytest.approx(out["plain"][k][0], rel=tol)
         assert out[path][-1][1] < 0.5 * r
     assert out["plain"][-1][1] < 0.5 * r0
 
