@@ -186,6 +186,16 @@ double mgx_gamma(const double h[3], int mg_levels, double vmax, double small_eig
  * s < subdivx of loc_numpt rows; psi column-major (ld). */
 int mgx_local_gram(mgx_ctx* ctx, int prec, const double* psi, int64_t ld, int64_t loc_numpt, int n,
     int subdivx, double vel, double* ss);
+/* No counterpart in the reference: mgx_local_gram for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the chunk
+ * partials live in the library's fixed 16 MiB device array (one per GPU,
+ * kept until the process ends) instead of a workspace, so it calls no
+ * hipMalloc/hipFree.  Limits: n < 256 and subdivx * n * n * 8 <= 16 MiB
+ * (MGX_EINVAL beyond); calls that overlap on one GPU share the array and
+ * race. */
+int mgx_local_gram_noalloc(mgx_ctx* ctx, int prec, const double* psi, int64_t ld, int64_t loc_numpt, int n,
+    int subdivx, double vel, double* ss);
 /* LocGridOrbitals::computeLocalProduct -> LocalMatrices::gemm -> MPgemm('t','n'),
  * src/LocGridOrbitals.cc:1554-1604: ss[s](i,j) = vel * <a_i, b_j>_s */
 int mgx_local_product(mgx_ctx* ctx, int prec, const double* a, const double* b, int64_t ld,

@@ -51,6 +51,10 @@ int set_err(mgx_ctx* ctx, int code, const std::string& msg);
 // device workspace of at least `bytes` in slot `slot` (256-byte aligned)
 void* workspace(mgx_ctx* ctx, int slot, size_t bytes);
 
+// device address of this GPU's no-allocation scratch array (dense_ops.hip);
+// called once per device at context creation, with the device current
+int scratch_init(mgx_ctx* ctx, int device);
+
 inline size_t esize(int dtype) { return dtype == MGX_F32 ? sizeof(float) : sizeof(double); }
 
 constexpr int kBlock = 256;

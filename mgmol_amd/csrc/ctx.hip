@@ -122,6 +122,12 @@ int mgx_ctx_create(int device, mgx_ctx** out)
     rocblas_set_stream(c->blas, c->stream);
     if (hipHostMalloc((void**)&c->host_pinned, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         c->host_pinned = nullptr;
+    // the device address of this GPU's no-allocation scratch array
+    if (mgx::scratch_init(c, device) != MGX_OK)
+    {
+        mgx_ctx_destroy(c);
+        return MGX_EHIP;
+    }
     *out = c;
     return MGX_OK;
 }

@@ -12,6 +12,8 @@
 
 #include <rocsolver/rocsolver.h>
 
+#include <mutex>
+
 namespace mgx
 {
 namespace
@@ -648,8 +650,10 @@ int tsgemm_wide(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_
     return MGX_OK;
 }
 
+// part_ext: a caller-given buffer of part_cap bytes for the chunk partials
+// (the number of chunks is lowered to fit it), else the context's workspace
 int tsgemm(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_t ld, int64_t lnp, int n, int subdivx,
-    double vel, int lower_only, double* ss)
+    double vel, int lower_only, double* ss, double* part_ext = nullptr, size_t part_cap = 0)
 {
     MGX_REQUIRE(ctx, A && B && ss && n > 0 && subdivx > 0 && lnp > 0, "bad product arguments");
     MGX_REQUIRE(ctx, (int64_t)subdivx * lnp <= ld, "slabs exceed the leading dimension");
@@ -661,10 +665,19 @@ int tsgemm(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_t ld,
     int64_t want = (4096 + ntiles - 1) / ntiles;
     int64_t maxc = (lnp + 255) / 256;
     int nchunk   = (int)std::max<int64_t>(1, std::min<int64_t>(want, maxc));
+    const size_t per_chunk = (size_t)subdivx * n * n * sizeof(double);
+    double* part           = part_ext;
+    if (part_ext)
+    {
+        MGX_REQUIRE(ctx, per_chunk <= part_cap, "subdivx * n * n * 8 bytes exceed the scratch buffer");
+        nchunk = (int)std::min<size_t>((size_t)nchunk, part_cap / per_chunk);
+    }
     const int64_t rows = (lnp + nchunk - 1) / nchunk;
-    const size_t pbytes = (size_t)subdivx * nchunk * n * n * sizeof(double);
-    double* part        = (double*)workspace(ctx, 7, pbytes);
-    if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    if (!part_ext)
+    {
+        part = (double*)workspace(ctx, 7, per_chunk * nchunk);
+        if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    }
     dim3 grid(ntiles, nchunk, subdivx), blk(64 * kWaves);
     if (prec == MGX_PREC_F64)
         hipLaunchKernelGGL(k_tsgemm<MGX_PREC_F64>, grid, blk, 0, ctx->stream, A, B, ld, lnp, n, ntile, nchunk, rows,
@@ -1010,6 +1023,39 @@ int ritz(mgx_ctx* ctx, int n, const double* H, const double* S, double* eig, dou
 
 }  // namespace mgx
 
+// Scratch of mgx_local_gram_noalloc: the chunk partials of k_tsgemm, at most
+// kNoAllocBytes.  The array is the device memory itself (one copy per GPU,
+// held until the process ends); host code uses only the device address that
+// scratch_init() takes from hipGetSymbolAddress, never the array's name.
+constexpr size_t kNoAllocBytes = size_t(16) << 20;
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+namespace mgx
+{
+namespace
+{
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// once per device, with that device current (context creation): the device
+// address of this GPU's copy of `scratch`
+int scratch_init(mgx_ctx* ctx, int device)
+{
+    if (device < 0 || device >= kMaxDevices) return set_err(ctx, MGX_EINVAL, "device index out of range for scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MGX_OK;
+    void* p        = nullptr;
+    hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !p)
+        return set_err(ctx, MGX_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(err));
+    g_scratch_ptr[device] = p;
+    return MGX_OK;
+}
+
+}  // namespace mgx
+
 using namespace mgx;
 
 extern "C" {
@@ -1018,6 +1064,19 @@ int mgx_local_gram(mgx_ctx* ctx, int prec, const double* psi, int64_t ld, int64_
     double vel, double* ss)
 {
     int rc = tsgemm(ctx, prec, psi, psi, ld, lnp, n, subdivx, vel, 1, ss);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_local_gram_noalloc(mgx_ctx* ctx, int prec, const double* psi, int64_t ld, int64_t lnp, int n, int subdivx,
+    double vel, double* ss)
+{
+    if (!ctx) return MGX_EINVAL;
+    MGX_REQUIRE(ctx, n > 0 && n < kWideN, "mgx_local_gram_noalloc takes n < 256 (wider products use rocBLAS)");
+    void* p = ctx->device >= 0 && ctx->device < kMaxDevices ? g_scratch_ptr[ctx->device] : nullptr;
+    MGX_REQUIRE(ctx, p, "scratch address of this device not set up (mgx_ctx_create)");
+    // every partial the reduction reads is written by the same call: nothing
+    // to clear; calls that overlap on one GPU share the array and race
+    int rc = tsgemm(ctx, prec, psi, psi, ld, lnp, n, subdivx, vel, 1, ss, (double*)p, kNoAllocBytes);
     return rc ? rc : sync(ctx);
 }
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "mgx_jacobi",
     "mgx_restrict", "mgx_prolong", "mgx_mask", "mgx_mg",
     "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg",
-    "mgx_local_gram", "mgx_local_product", "mgx_local_to_global", "mgx_chol_inv", "mgx_theta", "mgx_eigsum",
+    "mgx_local_gram", "mgx_local_gram_noalloc", "mgx_local_product", "mgx_local_to_global", "mgx_chol_inv", "mgx_theta", "mgx_eigsum",
     "mgx_ritz", "mgx_residual", "mgx_residual_norm", "mgx_mehr_bmatrix", "mgx_mehr_projected", "mgx_mehr_residual",
     "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
     "mgx_box_lap", "mgx_box_hloc", "mgx_box_jacobi", "mgx_box_restrict", "mgx_box_prolong", "mgx_box_precond_mg",
@@ -115,6 +115,7 @@ def lib():
             "mgx_hloc_apply": (P, I, GP, I, P, I64, I, P, P, I64),
             "mgx_precond_mg": (P, I, I, D, I, GP, P, I64, I, PP, P, I64),
             "mgx_local_gram": (P, I, P, I64, I64, I, I, D, P),
+            "mgx_local_gram_noalloc": (P, I, P, I64, I64, I, I, D, P),
             "mgx_local_product": (P, I, P, P, I64, I64, I, I, D, P),
             "mgx_local_to_global": (P, P, P, I, I, I, D, P),
             "mgx_chol_inv": (P, I, P, P, P), "mgx_theta": (P, I, P, P, P),
