@@ -112,15 +112,28 @@ def fit_precond(cfg):
         LABELS["ext16"] = LABELS["ext16"].replace("FP16-storage V-cycle", "FP32 V-cycle (edges not multiples of 8)")
 
 
-def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
+def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn, graph=True, nograph_steps=0):
     kw = dict(mg_levels=cfg.levels - 1, profile=0, precond_start=mgx.START_ZERO, anderson_m=1,
               conv_criterion=mgx.CONV_RESIDUAL)
     kw.update(PATHS[mode])
     s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, cnn=cnn if mode == "learned" else None, **kw)
+    # the step's launch sequences replayed as HIP graphs (captured during the
+    # warm-up); the same solves with eager launches are timed for comparison
+    nograph = []
+    if nograph_steps:
+        s.set_graph(False)
+        for k in range(nograph_steps):
+            s.reset()
+            t0 = time.perf_counter()
+            st = s.run(tol, maxit)
+            nograph.append(time.perf_counter() - t0)
+            log("[%s] solve without graphs %d: %d iterations, %.3f s" % (mode, k, st.iterations, nograph[-1]))
+    s.set_graph(graph)
     for _ in range(warmup):
-        s.run(tol, 3)
         s.reset()
+        s.run(tol, maxit)
     times, stats = [], None
+    g0 = s.graph_stats()
     for k in range(steps):
         s.reset()
         t0 = time.perf_counter()
@@ -129,6 +142,7 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
         log("[%s] solve %d: %d iterations, |R|=%.3e Ha, E=%.12f Ha, %.3f s (converged=%d, learned steps %d, "
             "fallbacks %d)" % (mode, k, stats.iterations, stats.residual, stats.band_energy, times[-1],
                                stats.converged, stats.learned_steps, stats.fallbacks))
+    g1 = s.graph_stats()
     # one more solve with the kernel groups timed (hipEvents on the engine's stream)
     s.reset()
     s.set_profile(True)
@@ -144,7 +158,10 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn):
                 learned_steps=stats.learned_steps, fallbacks=stats.fallbacks, newton_schulz_steps=stats.ns_steps,
                 cholesky_calls=stats.chol_calls, ns_continued=stats.ns_continued, ns_restarts=stats.ns_restarts,
                 dots_sweeps=stats.dots_sweeps, profiled_seconds=t_prof, profiled_iterations=pst.iterations,
-                profile=prof)
+                profile=prof, graph=bool(graph),
+                graph_launches_per_solve=(g1[0] - g0[0]) / max(steps, 1),
+                graph_captures_in_timed_solves=g1[1] - g0[1], graphs_held=g1[2],
+                seconds_without_graphs=float(np.mean(nograph)) if nograph else None)
 
 
 def group_rates(r, mode, npts):
@@ -262,6 +279,7 @@ def main():
     ap.add_argument("--modes", default="plain,mixed,learned,ext16")
     ap.add_argument("--weights", default=WEIGHTS)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch every kernel eagerly")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -282,9 +300,10 @@ def main():
         params, wmeta = train_cnn.load(args.weights)
         cnn = learned.Cnn(ctx, wmeta["edge"], 256, params)
     res = {}
+    headname = next(m for m in ("learned", "mixed", "ext16", "plain") if m in modes)
     for mode in modes:
-        res[mode] = run_mode(ctx, bs, cfg, mode, args.tol, args.maxit, args.steps, args.warmup, cnn)
-    headname = next(m for m in ("learned", "mixed", "ext16", "plain") if m in res)
+        res[mode] = run_mode(ctx, bs, cfg, mode, args.tol, args.maxit, args.steps, args.warmup, cnn,
+                             graph=not args.no_graph, nograph_steps=args.steps if mode == headname else 0)
     head = res[headname]
     npts = cfg.psi.size
 
@@ -342,6 +361,12 @@ def main():
         "stream": stream,
         "cpu_baseline": cpu,
     }
+    out["hipgraph"] = {"path": headname, "seconds_graph_on": round(head["seconds"], 4),
+                       "seconds_graph_off": (round(head["seconds_without_graphs"], 4)
+                                             if head["seconds_without_graphs"] else None),
+                       "graph_launches_per_solve": head["graph_launches_per_solve"],
+                       "graphs_held": head["graphs_held"],
+                       "captures_in_timed_solves": head["graph_captures_in_timed_solves"]}
     if roof and roof.get("unit") == "GB/s":
         roof["frac_of_stream"] = round(roof["achieved"] / stream["GBps"], 4)
     if "plain" in res:

@@ -445,6 +445,17 @@ int mgx_solver_inverse(mgx_solver* s, double** Sinv);
  * state); the Anderson history is cleared, S^-1 stays as the start of the next
  * refresh (which falls back to Cholesky when it is too far off) */
 int mgx_solver_set_psi(mgx_solver* s, const double* psi_host);
+/* Run each step's launch sequences as HIP graphs (on = 1): the sequence from
+ * H psi to the read of the step's energy and residual norm, and the one from
+ * the preconditioner to the norms of the updated psi, each captured the
+ * second time the same host decisions shape it (same Newton-Schulz step
+ * count, Anderson state and buffers) and replayed from then on.  Steps with
+ * Cholesky, a network trial or a restore, and profiled steps, run eagerly.
+ * Results are those of the eager launches.  No counterpart in the reference
+ * (its steps are host loops). */
+int mgx_solver_set_graph(mgx_solver* s, int on);
+/* graph launches and captures so far, and graphs held (any pointer may be NULL) */
+int mgx_solver_graph_stats(mgx_solver* s, int64_t* launches, int64_t* captures, int64_t* cached);
 /* attach a trained network (uniform box edge equal to the network's) for
  * cfg.precond_kind = MGX_PRECOND_LEARNED; the solver does not own it */
 int mgx_solver_set_cnn(mgx_solver* s, mgx_cnn* cnn);

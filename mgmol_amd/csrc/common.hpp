@@ -39,6 +39,7 @@ struct mgx_ctx
     static constexpr int kSlots = 9;
     void* ws[kSlots]     = {};
     size_t ws_sz[kSlots] = {};
+    uint64_t ws_gen      = 0;  // bumped whenever a workspace moves (captured graphs hold the old address)
     double* host_pinned  = nullptr;  // small pinned host buffer for scalars
     MgxProf* prof        = nullptr;  // non-null while a profiled solver runs
 };

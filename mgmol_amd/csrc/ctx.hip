@@ -28,6 +28,7 @@ void* workspace(mgx_ctx* ctx, int slot, size_t bytes)
         if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
         ctx->ws[slot]    = p;
         ctx->ws_sz[slot] = bytes;
+        ctx->ws_gen++;
     }
     return ctx->ws[slot];
 }

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
+#include <unordered_map>
 
 namespace mgx
 {
@@ -65,6 +67,17 @@ __global__ void k_scaled_copy(const double* x, int64_t n, double a, double* y)
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
         y[t] = a * x[t];
 }
+// what a tail sequence leaves for the host to record (the same on every
+// replay of its graph)
+struct TailOut
+{
+    bool norms_done = false;  // d holds the norms of the updated psi
+    bool normalized = false;  // psi is normalised already
+    bool anderson   = false;  // the Anderson update ran: psi <-> x0 swap
+    bool keep       = false;  // and the V-cycle's correction becomes the history (kr0 <-> kr1)
+    bool dots_total = false;  // the V-cycle's last sweep formed the Anderson dots
+};
+
 }  // namespace mgx
 
 struct mgx_solver
@@ -113,6 +126,19 @@ struct mgx_solver
     int64_t ns_continued = 0, ns_restarts = 0;  // checked continuations, Cholesky restarts of the refresh
     int64_t gsize = 0;
     double* host = nullptr;  // pinned: [energy, norm]
+    // HIP graphs of the step's launch sequences, by the host decisions that
+    // shape them (mgx_solver_set_graph)
+    struct Graph
+    {
+        hipGraphExec_t exec = nullptr;
+        uint64_t ws_gen     = 0;  // the context's workspace generation at capture
+        int seen            = 0;
+        bool failed         = false;
+        mgx::TailOut out;
+    };
+    bool use_graph = false;
+    std::unordered_map<uint64_t, Graph> graphs;
+    int64_t graph_launches = 0, graph_captures = 0;
 };
 
 namespace mgx
@@ -130,25 +156,52 @@ void free_solver(mgx_solver* s)
     if (s->kr1) (void)hipFree(s->kr1);
     if (s->kcnn) (void)hipFree(s->kcnn);
     if (s->host) hipHostFree(s->host);
+    for (auto& kv : s->graphs)
+        if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+    s->graphs.clear();
     for (hipEvent_t e : s->prof.pool)
         hipEventDestroy(e);
     s->prof.pool.clear();
 }
 
-int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
+// theta, the band energy, the residual and its norm from the current S^-1,
+// and their copy to the host (read after the caller's stream wait); tk is
+// the open DENSE timing token
+int project(mgx_solver* s, int spec, bool bf16, int& tk)
 {
     mgx_ctx* ctx = s->ctx;
-    int rc, tk;
-    const int n = s->n;
+    const int n  = s->n;
+    int r;
+    // theta = S^-1 H with H pair-sparse
+    if ((r = spmm_pairs(ctx, n, s->Sinv, s->Hv, s->tiles, s->th))) return r;
+    // tr(H S^-1) = tr(S^-1 H) = tr(theta)   (ProjectedMatrices::getEigSum)
+    if ((r = diag_sum(ctx, n, s->th, s->scal))) return r;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_RESID);
+    if ((r = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d))) return r;
+    if ((r = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return r;
+    prof_end(ctx, tk);
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, (8 + spec + (bf16 ? 1 : 0)) * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    return MGX_OK;
+}
+
+// The launches of a step up to the host read of its scalars: H psi (with the
+// pending normalisation), S and H, S^-1 (Cholesky + inverse, or spec
+// Newton-Schulz steps whose deltas the host checks), theta, the energy, the
+// residual and its norm.  Changes no host state.
+int head_body(mgx_solver* s, bool pending, bool chol, int spec, bool bf16)
+{
+    mgx_ctx* ctx     = s->ctx;
+    const double np  = (double)s->npts;
+    int rc;
     // H psi: psi read, H psi written (FP64), V once; psi written back when
     // the pending normalisation is applied on the way
-    const double np = (double)s->npts;
-    tk = prof_begin(ctx, MGX_PROF_HLOC, (s->pending_norm ? 24. : 16.) * np + 8. * (double)s->gsize);
-    if (s->pending_norm)
+    int tk = prof_begin(ctx, MGX_PROF_HLOC, (pending ? 24. : 16.) * np + 8. * (double)s->gsize);
+    if (pending)
     {
         // LocGridOrbitals::normalize of the last update, fused with Hpsi
         if ((rc = box_hloc_normalize(ctx, s->bs, s->psi, s->vtot, s->hpsi, s->d))) return rc;
-        s->pending_norm = false;
     }
     else if ((rc = box_hloc(ctx, s->bs, MGX_F64, s->psi, s->vtot, s->hpsi)))
         return rc;
@@ -157,44 +210,208 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     if ((rc = box_gram_h(ctx, s->bs, s->cfg.gram_prec, s->psi, s->hpsi, s->S, s->H, s->Sv, s->Hv))) return rc;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_DENSE);
-    // S^-1: Cholesky + inverse (GramMatrix::updateLS / computeInverse) on the
-    // first step (every step with MGX_INV_CHOL), then Newton-Schulz refresh
-    // from the previous inverse, to ns_tol
-    const double ns_tol = s->ns_tol;
-    const bool bf16     = s->cfg.inverse_mode == MGX_INV_NS_BF16;
-    int spec = 0;  // Newton-Schulz steps taken unchecked, their deltas in scal[8..]
-    if (!s->have_inv || s->cfg.inverse_mode == MGX_INV_CHOL)
+    // S^-1: Cholesky + inverse (GramMatrix::updateLS / computeInverse), or
+    // the Newton-Schulz refresh from the previous inverse
+    if (chol)
     {
-        if ((rc = chol_inv(ctx, n, s->S, s->L, s->Sinv))) return rc;
-        s->chol_calls++;
-        s->have_inv = true;
+        if ((rc = chol_inv(ctx, s->n, s->S, s->L, s->Sinv))) return rc;
+    }
+    else if ((rc = inverse_update_spec(ctx, s->n, s->Sv, s->tiles, s->Sinv, s->nswork, spec, bf16, s->scal + 8)))
+        return rc;
+    return project(s, spec, bf16, tk);
+}
+
+
+// The launches of a step after the host's decisions: the preconditioner (or
+// the network), the orbital update and the norms.  Changes no host state.
+int tail_body(mgx_solver* s, bool use_cnn, bool restored, bool plain_step, double gamma, TailOut& o)
+{
+    mgx_ctx* ctx       = s->ctx;
+    const double np    = (double)s->npts;
+    const double alpha = 0.5 * s->cfg.step_length;
+    int rc;
+    o = TailOut();
+    bool dots_done = false;  // the V-cycle formed the Anderson dots in d
+    int tk         = prof_begin(ctx, MGX_PROF_PRECOND);
+    const int prec = s->cfg.precond_dtype;
+    // the correction's type: FP32 out of the FP16 V-cycle
+    const int kprec = prec == MGX_F16 ? MGX_F32 : prec;
+    if (!use_cnn && !s->accelerate && !restored && prec != MGX_F16)
+    {
+        // preconditioned step psi += alpha K(R) (ABPG::update_states, no
+        // extrapolation), fused with the V-cycle's last kernel
+        if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, nullptr, s->psi,
+                 alpha)))
+            return rc;
     }
     else
     {
-        // as many steps as the last refresh needed, checked with the energy
-        // and the residual norm below (one host wait per step)
-        spec = std::max(1, std::min(s->ns_pred, s->ns_max_spec));
-        if ((rc = inverse_update_spec(ctx, n, s->Sv, s->tiles, s->Sinv, s->nswork, spec, bf16, s->scal + 8)))
-            return rc;
+        // the correction K(R) (or the network's), in the preconditioner's type
+        const void* kr = nullptr;
+        if (use_cnn)
+        {
+            int edge = 0;
+            boxset_uniform_edge(s->bs, &edge);
+            // 4 conv3d layers 1->16->16->16->1, 3^3 taps: 2 * 27 * 544 flops per voxel
+            const int tl = prof_begin(ctx, MGX_PROF_LEARNED, 2. * 27. * 544. * np);
+            if ((rc = cnn_apply_boxes(s->cnn, s->n, edge, s->res, s->kcnn, kprec))) return rc;
+            prof_end(ctx, tl);
+            kr = s->kcnn;
+        }
+        else
+        {
+            // d holds the residual's per-box sums unless R was just restored
+            if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, s->kr1,
+                     (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done,
+                     restored ? nullptr : s->d)))
+                return rc;
+            kr = s->kr1;
+        }
+        const double sk = kprec == MGX_F64 ? 8. : 4.;
+        if (!s->accelerate || plain_step)
+        {
+            // psi += alpha K(R), with the norms of the new psi: psi read and
+            // written, the correction read
+            const int tu = prof_begin(ctx, MGX_PROF_UPDATE, (16. + sk) * np);
+            if ((rc = box_psi_step_sq(ctx, s->bs, kprec, s->psi, kr, alpha, s->d))) return rc;
+            prof_end(ctx, tu);
+            o.norms_done = true;
+        }
+        else
+        {
+            // AndersonMix::update with m = 1 on f = alpha K(R); theta is
+            // formed on the device with the dot products (no host wait)
+            const double* thp = nullptr;
+            double beta       = 1.;
+            if (s->mm > 0)
+            {
+                if (dots_done)
+                {
+                    rc           = box_anderson_dots_total(ctx, s->bs, s->Sinv, s->d, s->scal + 2);
+                    o.dots_total = true;
+                }
+                else
+                    rc = box_anderson_dots(ctx, s->bs, kprec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2);
+                if (rc) return rc;
+                thp  = s->scal + 4;
+                beta = s->cfg.anderson_beta;
+            }
+            // the new psi goes to the history buffer, and the buffers swap:
+            // the old psi is the next step's x0; the norms of the new psi too.
+            // The V-cycle's correction buffer becomes the history (kr0, kr1
+            // swap); the network's correction is copied into kr0 instead
+            // boxes up to 32^3: the normalisation in the same pass
+            o.keep       = kr == s->kr1;
+            o.normalized = boxset_maxedge(s->bs) <= 32;
+            // psi and the previous psi read, the new psi written (FP64); the
+            // correction and the previous one read (+ the copy when not kept)
+            const int tu = prof_begin(ctx, MGX_PROF_UPDATE, (24. + (o.keep ? 2. : 3.) * sk) * np);
+            if ((rc = o.normalized ? box_anderson_update_nrm(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
+                                         o.keep ? nullptr : s->kr0, thp, beta, s->d)
+                                   : box_anderson_update_sq(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
+                                         o.keep ? nullptr : s->kr0, thp, beta, s->d)))
+                return rc;
+            prof_end(ctx, tu);
+            o.anderson   = true;
+            o.norms_done = true;
+        }
     }
-    // theta, the band energy, the residual and its norm from the current S^-1
-    auto project = [&]() -> int {
-        int r;
-        // theta = S^-1 H with H pair-sparse
-        if ((r = spmm_pairs(ctx, n, s->Sinv, s->Hv, s->tiles, s->th))) return r;
-        // tr(H S^-1) = tr(S^-1 H) = tr(theta)   (ProjectedMatrices::getEigSum)
-        if ((r = diag_sum(ctx, n, s->th, s->scal))) return r;
-        prof_end(ctx, tk);
-        tk = prof_begin(ctx, MGX_PROF_RESID);
-        if ((r = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d))) return r;
-        if ((r = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return r;
-        prof_end(ctx, tk);
-        MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, (8 + spec + (bf16 ? 1 : 0)) * sizeof(double),
-                               hipMemcpyDeviceToHost, ctx->stream));
-        MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    prof_end(ctx, tk);
+    // LocGridOrbitals::normalize: the norms now (fused into the update when
+    // it was not the V-cycle's fused step), the scaling with the next Hpsi
+    // unless the Anderson update has applied it already
+    tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
+    if (!o.norms_done && (rc = box_sumsq(ctx, s->bs, s->psi, s->d))) return rc;
+    prof_end(ctx, tk);
+    return MGX_OK;
+}
+
+uint64_t mix_key(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
+    return h;
+}
+
+// Runs body, or replays its HIP graph.  A sequence seen for the first time
+// runs eagerly (its workspaces grow then); the second time it is captured
+// (relaxed mode) into a graph and launched; later steps with the same key
+// replay the graph.  A sequence that cannot be captured keeps running
+// eagerly.  body must enqueue work on the context's stream only and change
+// no host state; the TailOut it fills is stored with the graph.
+template <class F>
+int run_seq(mgx_solver* s, uint64_t key, TailOut* out, F&& body)
+{
+    mgx_ctx* ctx = s->ctx;
+    if (!s->use_graph || ctx->prof) return body();
+    mgx_solver::Graph& g = s->graphs[key];
+    if (g.exec && g.ws_gen != ctx->ws_gen)
+    {
+        // a workspace the graph writes has moved since: capture again
+        hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.seen = 0;
+    }
+    if (g.exec)
+    {
+        MGX_CHECK_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+        s->graph_launches++;
+        if (out) *out = g.out;
         return MGX_OK;
-    };
-    if ((rc = project())) return rc;
+    }
+    if (g.failed || g.seen++ == 0) return body();
+    MGX_CHECK_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+    const int rc     = body();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc == MGX_OK && e == hipSuccess && graph
+        && hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess)
+    {
+        hipGraphDestroy(graph);
+        g.ws_gen = ctx->ws_gen;
+        if (out) g.out = *out;
+        s->graph_captures++;
+        MGX_CHECK_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+        s->graph_launches++;
+        return MGX_OK;
+    }
+    // not capturable: run it eagerly, now and from here on
+    if (graph) hipGraphDestroy(graph);
+    g.exec   = nullptr;
+    g.failed = true;
+    (void)hipGetLastError();
+    return body();
+}
+
+int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
+{
+    mgx_ctx* ctx = s->ctx;
+    int rc, tk = -1;
+    const int n = s->n;
+    const double ns_tol = s->ns_tol;
+    const bool bf16     = s->cfg.inverse_mode == MGX_INV_NS_BF16;
+    // Cholesky on the first step (every step with MGX_INV_CHOL); else as
+    // many Newton-Schulz steps as the last refresh needed, unchecked, their
+    // deltas checked with the energy and the residual norm below
+    const bool chol    = !s->have_inv || s->cfg.inverse_mode == MGX_INV_CHOL;
+    int spec           = chol ? 0 : std::max(1, std::min(s->ns_pred, s->ns_max_spec));
+    const bool pending = s->pending_norm;
+    if (chol)
+        rc = head_body(s, pending, true, 0, bf16);  // waits for potrf's info: never captured
+    else
+    {
+        uint64_t key = 1;
+        for (uint64_t v : { (uint64_t)pending, (uint64_t)spec, (uint64_t)bf16, (uint64_t)(uintptr_t)s->psi })
+            key = mix_key(key, v);
+        rc = run_seq(s, key, nullptr, [&]() { return head_body(s, pending, false, spec, bf16); });
+    }
+    if (rc) return rc;
+    MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    s->pending_norm = false;
+    if (chol)
+    {
+        s->chol_calls++;
+        s->have_inv = true;
+    }
     if (spec > 0)
     {
         int needed     = 0;
@@ -228,7 +445,8 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
                 s->ns_continued++;
             s->ns_pred = steps < 0 ? 2 : spec + steps;
             spec       = 0;
-            if ((rc = project())) return rc;
+            if ((rc = project(s, 0, bf16, tk))) return rc;
+            MGX_CHECK_HIP(ctx, hipStreamSynchronize(ctx->stream));
         }
     }
     const double e_ry = s->host[0], rnorm = s->host[1];
@@ -329,108 +547,33 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         }
         gamma = mgx_gamma(s->h, s->cfg.mg_levels, s->vmax, s->lowest);
     }
-    st->lowest_eig     = s->lowest;
-    st->learned_steps  = s->learned_steps;
-    st->fallbacks      = s->fallbacks;
-    st->dots_sweeps    = (int)s->dots_sweeps;
-    const double alpha = 0.5 * s->cfg.step_length;
-    bool norms_done    = false;  // d already holds the norms of the updated psi
-    bool normalized    = false;  // and psi is normalised already
-    bool dots_done     = false;  // the V-cycle formed the Anderson dots in d
-    tk = prof_begin(ctx, MGX_PROF_PRECOND);
-    const int prec = s->cfg.precond_dtype;
-    // the correction's type: FP32 out of the FP16 V-cycle
-    const int kprec = prec == MGX_F16 ? MGX_F32 : prec;
-    if (!use_cnn && !s->accelerate && !restored && prec != MGX_F16)
+    st->lowest_eig    = s->lowest;
+    st->learned_steps = s->learned_steps;
+    st->fallbacks     = s->fallbacks;
+    TailOut o;
+    if (!use_cnn && !restored)
     {
-        // preconditioned step psi += alpha K(R) (ABPG::update_states, no
-        // extrapolation), fused with the V-cycle's last kernel
-        if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, nullptr, s->psi,
-                 alpha)))
-            return rc;
+        uint64_t key = 2;
+        uint64_t gbits;
+        std::memcpy(&gbits, &gamma, sizeof(gbits));
+        for (uint64_t v : { (uint64_t)s->accelerate, (uint64_t)plain_step, (uint64_t)(s->mm > 0), gbits,
+                 (uint64_t)(uintptr_t)s->psi, (uint64_t)(uintptr_t)s->x0, (uint64_t)(uintptr_t)s->kr0,
+                 (uint64_t)(uintptr_t)s->kr1 })
+            key = mix_key(key, v);
+        rc = run_seq(s, key, &o, [&]() { return tail_body(s, false, false, plain_step, gamma, o); });
     }
     else
+        rc = tail_body(s, use_cnn, restored, plain_step, gamma, o);
+    if (rc) return rc;
+    if (o.dots_total) s->dots_sweeps++;
+    if (o.anderson)
     {
-        // the correction K(R) (or the network's), in the preconditioner's type
-        const void* kr = nullptr;
-        if (use_cnn)
-        {
-            int edge = 0;
-            boxset_uniform_edge(s->bs, &edge);
-            // 4 conv3d layers 1->16->16->16->1, 3^3 taps: 2 * 27 * 544 flops per voxel
-            const int tl = prof_begin(ctx, MGX_PROF_LEARNED, 2. * 27. * 544. * np);
-            if ((rc = cnn_apply_boxes(s->cnn, n, edge, s->res, s->kcnn, kprec))) return rc;
-            prof_end(ctx, tl);
-            kr = s->kcnn;
-        }
-        else
-        {
-            // d holds the residual's per-box sums unless R was just restored
-            if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, s->kr1,
-                     (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done,
-                     restored ? nullptr : s->d)))
-                return rc;
-            kr = s->kr1;
-        }
-        const double sk = kprec == MGX_F64 ? 8. : 4.;
-        if (!s->accelerate || plain_step)
-        {
-            // psi += alpha K(R), with the norms of the new psi: psi read and
-            // written, the correction read
-            const int tu = prof_begin(ctx, MGX_PROF_UPDATE, (16. + sk) * np);
-            if ((rc = box_psi_step_sq(ctx, s->bs, kprec, s->psi, kr, alpha, s->d))) return rc;
-            prof_end(ctx, tu);
-            norms_done = true;
-        }
-        else
-        {
-            // AndersonMix::update with m = 1 on f = alpha K(R); theta is
-            // formed on the device with the dot products (no host wait)
-            const double* thp = nullptr;
-            double beta       = 1.;
-            if (s->mm > 0)
-            {
-                if (dots_done)
-                {
-                    rc = box_anderson_dots_total(ctx, s->bs, s->Sinv, s->d, s->scal + 2);
-                    s->dots_sweeps++;
-                }
-                else
-                    rc = box_anderson_dots(ctx, s->bs, kprec, kr, alpha, s->kr0, s->Sinv, s->d, s->scal + 2);
-                if (rc) return rc;
-                thp  = s->scal + 4;
-                beta = s->cfg.anderson_beta;
-            }
-            // the new psi goes to the history buffer, and the buffers swap:
-            // the old psi is the next step's x0; the norms of the new psi too.
-            // The V-cycle's correction buffer becomes the history (kr0, kr1
-            // swap); the network's correction is copied into kr0 instead
-            // boxes up to 32^3: the normalisation in the same pass
-            const bool keep = kr == s->kr1;
-            normalized      = boxset_maxedge(s->bs) <= 32;
-            // psi and the previous psi read, the new psi written (FP64); the
-            // correction and the previous one read (+ the copy when not kept)
-            const int tu = prof_begin(ctx, MGX_PROF_UPDATE, (24. + (keep ? 2. : 3.) * sk) * np);
-            if ((rc = normalized ? box_anderson_update_nrm(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
-                                       keep ? nullptr : s->kr0, thp, beta, s->d)
-                                 : box_anderson_update_sq(ctx, s->bs, kprec, s->psi, s->x0, kr, alpha, s->kr0,
-                                       keep ? nullptr : s->kr0, thp, beta, s->d)))
-                return rc;
-            prof_end(ctx, tu);
-            std::swap(s->psi, s->x0);
-            if (keep) std::swap(s->kr0, s->kr1);
-            norms_done = true;
-            s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
-        }
+        std::swap(s->psi, s->x0);
+        if (o.keep) std::swap(s->kr0, s->kr1);
+        s->mm = std::min(s->mm + 1, s->cfg.anderson_m);
     }
-    prof_end(ctx, tk);
-    // LocGridOrbitals::normalize: the norms now (fused into the update when
-    // it was not the V-cycle's fused step), the scaling with the next Hpsi
-    // unless the Anderson update has applied it already
-    tk = prof_begin(ctx, MGX_PROF_NORMALIZE);
-    if (!norms_done && (rc = box_sumsq(ctx, s->bs, s->psi, s->d))) return rc;
-    s->pending_norm = !normalized;
-    prof_end(ctx, tk);
+    st->dots_sweeps = (int)s->dots_sweeps;
+    s->pending_norm = !o.normalized;
     s->iter++;
     return MGX_OK;
 }
@@ -679,6 +822,28 @@ int mgx_solver_set_psi(mgx_solver* s, const double* host)
     s->pending_norm = false;
     s->mm           = 0;
     s->trial        = 0;
+    return MGX_OK;
+}
+
+int mgx_solver_set_graph(mgx_solver* s, int on)
+{
+    if (!s) return MGX_EINVAL;
+    s->use_graph = on != 0;
+    return MGX_OK;
+}
+
+int mgx_solver_graph_stats(mgx_solver* s, int64_t* launches, int64_t* captures, int64_t* cached)
+{
+    if (!s) return MGX_EINVAL;
+    if (launches) *launches = s->graph_launches;
+    if (captures) *captures = s->graph_captures;
+    if (cached)
+    {
+        int64_t c = 0;
+        for (auto& kv : s->graphs)
+            c += kv.second.exec ? 1 : 0;
+        *cached = c;
+    }
     return MGX_OK;
 }
 

@@ -40,6 +40,7 @@ EXPORTS = [
     "mgx_solver_create", "mgx_solver_destroy", "mgx_solver_step", "mgx_solver_run", "mgx_solver_reset",
     "mgx_solver_profile", "mgx_solver_profile_work", "mgx_solver_set_profile",
     "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_inverse", "mgx_solver_set_psi", "mgx_solver_set_cnn", "mgx_solver_density",
+    "mgx_solver_set_graph", "mgx_solver_graph_stats",
     "mgx_cnn_nparams", "mgx_cnn_create", "mgx_cnn_destroy", "mgx_cnn_set_params", "mgx_cnn_get_params",
     "mgx_cnn_forward", "mgx_cnn_grad", "mgx_cnn_train_step", "mgx_cnn_apply", "mgx_cnn_make_pairs",
 ]
@@ -141,6 +142,8 @@ def lib():
             "mgx_solver_profile_work": (P, C.POINTER(D)), "mgx_solver_set_profile": (P, I),
             "mgx_solver_buffers": (P, PP, PP, PP, PP, PP), "mgx_solver_inverse": (P, PP),
             "mgx_solver_set_psi": (P, P), "mgx_solver_set_cnn": (P, P),
+            "mgx_solver_set_graph": (P, I),
+            "mgx_solver_graph_stats": (P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)),
             "mgx_solver_density": (P, D, P, C.POINTER(D)),
             "mgx_cnn_create": (P, I, I, C.POINTER(P)), "mgx_cnn_destroy": (P,),
             "mgx_cnn_set_params": (P, P), "mgx_cnn_get_params": (P, P), "mgx_cnn_forward": (P, P, I, P),
@@ -391,6 +394,17 @@ class Solver:
 
     def set_profile(self, on):
         self.ctx.check(lib().mgx_solver_set_profile(self.h, 1 if on else 0), "mgx_solver_set_profile")
+
+    def set_graph(self, on):
+        """replay each step's launch sequences as HIP graphs (mgx_solver_set_graph)"""
+        self.ctx.check(lib().mgx_solver_set_graph(self.h, 1 if on else 0), "mgx_solver_set_graph")
+
+    def graph_stats(self):
+        """(graph launches, captures, graphs held)"""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.ctx.check(lib().mgx_solver_graph_stats(self.h, C.byref(a), C.byref(b), C.byref(c)),
+                       "mgx_solver_graph_stats")
+        return a.value, b.value, c.value
 
     def psi(self):
         import numpy as np
