@@ -119,6 +119,10 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn, graph=True, nog
     s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, cnn=cnn if mode == "learned" else None, **kw)
     # the step's launch sequences replayed as HIP graphs (captured during the
     # warm-up); the same solves with eager launches are timed for comparison
+    s.set_graph(graph)
+    for _ in range(warmup):
+        s.reset()
+        s.run(tol, maxit)
     nograph = []
     if nograph_steps:
         s.set_graph(False)
@@ -128,10 +132,7 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn, graph=True, nog
             st = s.run(tol, maxit)
             nograph.append(time.perf_counter() - t0)
             log("[%s] solve without graphs %d: %d iterations, %.3f s" % (mode, k, st.iterations, nograph[-1]))
-    s.set_graph(graph)
-    for _ in range(warmup):
-        s.reset()
-        s.run(tol, maxit)
+        s.set_graph(graph)
     times, stats = [], None
     g0 = s.graph_stats()
     for k in range(steps):
@@ -294,6 +295,9 @@ def main():
     ctx = mgx.Ctx(0)
     bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, cfg.levels)
     log("box pairs:", bs.npairs())
+    # launch shapes of the fine-level kernels for this box set (the solvers use them)
+    tune = bs.autotune()
+    log("autotune:", tune)
     cnn, wmeta = None, None
     if "learned" in modes:
         from mgmol_amd import learned, train_cnn
@@ -359,6 +363,10 @@ def main():
                                                                              "precond", "normalize", "learned")}
                                for m, r in res.items()},
         "stream": stream,
+        "autotune": {k: {"choice": c, "ms_per_launch": [round(t1, 4), round(t2, 4)],
+                         "candidates": ("lane groups per thread 1, 2" if not k.startswith("gram")
+                                        else "waves per cell 1, 2")}
+                     for k, (c, t1, t2) in tune.items()},
         "cpu_baseline": cpu,
     }
     out["hipgraph"] = {"path": headname, "seconds_graph_on": round(head["seconds"], 4),

@@ -245,6 +245,32 @@ int mgx_residual_norm(mgx_ctx* ctx, const double* res, int64_t ld, int64_t numpt
 
 /* --------------------------------------------------------- box family ---- */
 int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* boxes, int levels, mgx_boxset** bs);
+/* Autotuner of the box set's fine-level kernels (no counterpart in the
+ * reference).  Tunables (index into choice[MGX_TUNE_N]):
+ *   MGX_TUNE_MARCH_F16 / _F32 / _F64: 4th-order Jacobi sweep of the marching
+ *     kernel on FP16 / FP32 / FP64 values, MGX_TUNE_HLOC_F64: H psi (FP64):
+ *     lane groups (16 bytes of z-points, 8 for FP16) per thread, 1 or 2; the
+ *     block then has (e^2 / group size) / choice threads (64..1024);
+ *   MGX_TUNE_GRAM_F64 / _SPLIT: waves per 8^3 cell of the S / H cell kernel
+ *     with paired z-loads (FP64 / hi-lo split MFMA), 1 or 2.
+ * mgx_boxset_autotune times every candidate on the box set's own layout
+ * (scratch buffers of its size, 1 warm-up + 3 timed launches each, hipEvents)
+ * and keeps the fastest; ms (may be NULL) receives MGX_TUNE_N x 2 times (ms
+ * per launch, candidate 1 then 2; -1 where the kernel does not apply).  Any
+ * choice gives the same results (the per-point arithmetic does not depend
+ * on the lane mapping; only the Anderson dot sums of the last sweep are
+ * ordered differently).  A solver tunes its box set at creation unless it
+ * was tuned or set before.  mgx_boxset_set_tune forces a choice; get reads it. */
+#define MGX_TUNE_MARCH_F16 0
+#define MGX_TUNE_MARCH_F32 1
+#define MGX_TUNE_MARCH_F64 2
+#define MGX_TUNE_HLOC_F64 3
+#define MGX_TUNE_GRAM_F64 4
+#define MGX_TUNE_GRAM_SPLIT 5
+#define MGX_TUNE_N 6
+int mgx_boxset_autotune(mgx_ctx* ctx, mgx_boxset* bs, int* choice, double* ms);
+int mgx_boxset_set_tune(mgx_boxset* bs, const int* choice);
+int mgx_boxset_get_tune(const mgx_boxset* bs, int* choice, int* tuned);
 int mgx_boxset_destroy(mgx_boxset* bs);
 /* total points of all boxes at a level (host out) and offset of box j */
 int mgx_boxset_size(const mgx_boxset* bs, int level, int64_t* npoints);

@@ -69,6 +69,10 @@ struct mgx_boxset
     int* d_sp_col    = nullptr;
     int* d_sp_k      = nullptr;
     int* d_sp_e      = nullptr;
+    // autotuned launch shapes (MGX_TUNE_*) and whether the tuner ran or a
+    // caller set them
+    int tune[MGX_TUNE_N] = { 1, 1, 1, 1, 1, 1 };
+    bool tuned           = false;
 };
 
 namespace mgx
@@ -1183,7 +1187,6 @@ __device__ __forceinline__ int64_t cell_idx3(const CellEnt& c, int x, int y, int
 
 constexpr int RJB = 2;  // point tiles per load batch in k_cell_residual and k_cell_density
 constexpr int RJP = 1;  // 32-point tiles per load batch in k_cell_residual with paired loads
-constexpr int kGramWaves = 1;  // waves per cell in k_cell_gram with paired loads (1 measured fastest of 1, 2, 4, 8)
 
 // Per cell: G = A^T A and P = A^T B (A = psi, B = H psi, 512 x k) on MFMA.
 // Four waves split the 512 points (two x-planes each), then combine in LDS
@@ -2395,6 +2398,26 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
         const int groups   = E2 / V;
         if (plane <= 60 * 1024 && groups <= 1024)
         {
+            if constexpr (TYPE == MGX_LAP_4TH && (MODE == MARCH_JACOBI || MODE == MARCH_HLOC))
+            {
+                // two lane groups per thread when the autotuner chose so
+                constexpr int ti = MODE == MARCH_HLOC ? MGX_TUNE_HLOC_F64
+                                                      : (sizeof(T) == 2   ? MGX_TUNE_MARCH_F16
+                                                            : sizeof(T) == 4 ? MGX_TUNE_MARCH_F32
+                                                                             : MGX_TUNE_MARCH_F64);
+                // H psi is tuned for FP64 values only
+                const bool two = (MODE != MARCH_HLOC || sizeof(T) == 8) && bs->tune[ti] == 2;
+                if (level == 0 && two && groups > 128)
+                {
+                    if (groups <= 256)
+                        march4_launch<T, TYPE, MODE, 128, 2>(ctx, bs, a, plane, in, f, v, out, w, c);
+                    else if (groups <= 512)
+                        march4_launch<T, TYPE, MODE, 256, 2>(ctx, bs, a, plane, in, f, v, out, w, c);
+                    else
+                        march4_launch<T, TYPE, MODE, 512, 2>(ctx, bs, a, plane, in, f, v, out, w, c);
+                    return true;
+                }
+            }
             if (groups <= 64)
                 march4_launch<T, TYPE, MODE, 64, 1>(ctx, bs, a, plane, in, f, v, out, w, c);
             else if (groups <= 256)
@@ -2813,12 +2836,15 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, S, nn, 0.);
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, H, nn, 0.);
     {
-        // paired loads: kGramWaves waves per cell, else four
-        auto kern = prec == MGX_PREC_SPLIT ? (bs->pair_z ? k_cell_gram<MGX_PREC_SPLIT, true, kGramWaves>
-                                                         : k_cell_gram<MGX_PREC_SPLIT, false, 4>)
-                                           : (bs->pair_z ? k_cell_gram<MGX_PREC_F64, true, kGramWaves>
-                                                         : k_cell_gram<MGX_PREC_F64, false, 4>);
-        const int nthr = 64 * (bs->pair_z ? kGramWaves : 4);
+        // paired loads: the tuned number of waves per cell (1 or 2), else four
+        const int nw = bs->tune[prec == MGX_PREC_SPLIT ? MGX_TUNE_GRAM_SPLIT : MGX_TUNE_GRAM_F64] == 2 ? 2 : 1;
+        auto kern = prec == MGX_PREC_SPLIT
+                        ? (bs->pair_z ? (nw == 2 ? k_cell_gram<MGX_PREC_SPLIT, true, 2>
+                                                 : k_cell_gram<MGX_PREC_SPLIT, true, 1>)
+                                      : k_cell_gram<MGX_PREC_SPLIT, false, 4>)
+                        : (bs->pair_z ? (nw == 2 ? k_cell_gram<MGX_PREC_F64, true, 2> : k_cell_gram<MGX_PREC_F64, true, 1>)
+                                      : k_cell_gram<MGX_PREC_F64, false, 4>);
+        const int nthr = 64 * (bs->pair_z ? nw : 4);
         // psi and H psi read once (FP64)
         const int tk   = prof_begin(ctx, MGX_PROF_CELLGRAM, 16. * (double)bs->off[0][bs->nbox]);
         hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(nthr), 0, ctx->stream, psi, hpsi,
@@ -2879,6 +2905,84 @@ int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, 
             (const double*)part, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H, Sv, Hv);
     }
     MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+// the autotuner (mgx_boxset_autotune): each candidate of each tunable timed
+// on scratch buffers of the box set's size; the fastest is kept
+int box_autotune(mgx_ctx* ctx, mgx_boxset* bs, double* ms)
+{
+    const int64_t np = bs->off[0][bs->nbox];
+    const int64_t ng = (int64_t)bs->gdim[0] * bs->gdim[1] * bs->gdim[2];
+    const int n      = bs->nbox;
+    const int64_t nn = (int64_t)n * n;
+    const int64_t npr = std::max<int64_t>(bs->npairs, 1);
+    double* buf      = nullptr;
+    const int64_t total = 3 * np + ng + 2 * nn + 2 * npr;
+    if (hipMalloc(&buf, total * sizeof(double)) != hipSuccess)
+        return set_err(ctx, MGX_ENOMEM, "autotuner scratch allocation failed");
+    hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, buf, total, 0.);
+    double *a = buf, *b = a + np, *c = b + np, *v = c + np, *S = v + ng, *H = S + nn, *Sv = H + nn, *Hv = Sv + npr;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const double mjf = -1. * jacobi_factor(MGX_LAP_4TH, bs->h);
+    // one tunable: run(cand) launches the kernel with the candidate set;
+    // false when the kernel does not apply to this box set
+    auto tune1 = [&](int ti, int ncand, auto&& run) {
+        double best = 1e300;
+        int bestc    = 1;
+        for (int cand = 1; cand <= ncand; cand++)
+        {
+            bs->tune[ti] = cand;
+            double t     = -1.;
+            if (run())
+            {
+                float acc[3];
+                for (int r = 0; r < 3; r++)
+                {
+                    (void)hipEventRecord(e0, ctx->stream);
+                    run();
+                    (void)hipEventRecord(e1, ctx->stream);
+                    (void)hipEventSynchronize(e1);
+                    (void)hipEventElapsedTime(&acc[r], e0, e1);
+                }
+                std::sort(acc, acc + 3);
+                t = acc[1];
+                if (t < best)
+                {
+                    best  = t;
+                    bestc = cand;
+                }
+            }
+            if (ms) ms[2 * ti + cand - 1] = t;
+        }
+        bs->tune[ti] = bestc;
+    };
+    tune1(MGX_TUNE_MARCH_F16, 2, [&]() {
+        return march<_Float16, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, (const _Float16*)a, (const _Float16*)b,
+            nullptr, (_Float16*)c, nullptr);
+    });
+    tune1(MGX_TUNE_MARCH_F32, 2, [&]() {
+        return march<float, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, (const float*)a, (const float*)b, nullptr,
+            (float*)c, nullptr);
+    });
+    tune1(MGX_TUNE_MARCH_F64, 2, [&]() {
+        return march<double, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, a, b, nullptr, c, nullptr);
+    });
+    tune1(MGX_TUNE_HLOC_F64, 2, [&]() {
+        return march<double, MGX_LAP_4TH, MARCH_HLOC>(ctx, bs, 0, 0., a, nullptr, v, c, nullptr);
+    });
+    for (int prec : { MGX_PREC_F64, MGX_PREC_SPLIT })
+        tune1(prec == MGX_PREC_F64 ? MGX_TUNE_GRAM_F64 : MGX_TUNE_GRAM_SPLIT, 2, [&]() {
+            return bs->use_cells && bs->pair_z && box_gram_h_cells(ctx, bs, prec, a, b, S, H, Sv, Hv) == MGX_OK;
+        });
+    const hipError_t err = hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    if (err != hipSuccess) return set_err(ctx, MGX_EHIP, std::string("autotuner: ") + hipGetErrorString(err));
+    bs->tuned = true;
     return MGX_OK;
 }
 
@@ -3471,6 +3575,39 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset
     }
     if (hipDeviceSynchronize() != hipSuccess) return fail("box table upload");
     *out = s;
+    return MGX_OK;
+}
+
+int mgx_boxset_autotune(mgx_ctx* ctx, mgx_boxset* bs, int* choice, double* ms)
+{
+    if (!ctx) return MGX_EINVAL;
+    MGX_REQUIRE(ctx, bs, "null box set");
+    int rc = box_autotune(ctx, bs, ms);
+    if (rc) return rc;
+    if (choice)
+        for (int t = 0; t < MGX_TUNE_N; t++)
+            choice[t] = bs->tune[t];
+    return MGX_OK;
+}
+
+int mgx_boxset_set_tune(mgx_boxset* bs, const int* choice)
+{
+    if (!bs || !choice) return MGX_EINVAL;
+    for (int t = 0; t < MGX_TUNE_N; t++)
+        if (choice[t] != 1 && choice[t] != 2) return MGX_EINVAL;
+    for (int t = 0; t < MGX_TUNE_N; t++)
+        bs->tune[t] = choice[t];
+    bs->tuned = true;
+    return MGX_OK;
+}
+
+int mgx_boxset_get_tune(const mgx_boxset* bs, int* choice, int* tuned)
+{
+    if (!bs) return MGX_EINVAL;
+    if (choice)
+        for (int t = 0; t < MGX_TUNE_N; t++)
+            choice[t] = bs->tune[t];
+    if (tuned) *tuned = bs->tuned ? 1 : 0;
     return MGX_OK;
 }
 

@@ -634,6 +634,14 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     MGX_REQUIRE(ctx, cfg->ns_max_spec >= 0 && cfg->ns_max_spec <= 8, "ns_max_spec must be in [0, 8]");
     int64_t dummy = 0;
     MGX_REQUIRE(ctx, mgx_boxset_size(bs, cfg->mg_levels, &dummy) == MGX_OK, "box set has too few levels");
+    int tuned = 0;
+    mgx_boxset_get_tune(bs, nullptr, &tuned);
+    if (!tuned)
+    {
+        // launch shapes of the fine-level kernels for this box set
+        const int rc = mgx_boxset_autotune(ctx, const_cast<mgx_boxset*>(bs), nullptr, nullptr);
+        if (rc) return rc;
+    }
     mgx_solver* s = new mgx_solver();
     s->ctx        = ctx;
     s->bs         = bs;

@@ -34,7 +34,7 @@ EXPORTS = [
     "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg",
     "mgx_local_gram", "mgx_local_gram_noalloc", "mgx_local_product", "mgx_local_to_global", "mgx_chol_inv", "mgx_theta", "mgx_eigsum",
     "mgx_ritz", "mgx_residual", "mgx_residual_norm", "mgx_mehr_bmatrix", "mgx_mehr_projected", "mgx_mehr_residual",
-    "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
+    "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_autotune", "mgx_boxset_set_tune", "mgx_boxset_get_tune", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
     "mgx_box_lap", "mgx_box_hloc", "mgx_box_jacobi", "mgx_box_restrict", "mgx_box_prolong", "mgx_box_precond_mg",
     "mgx_box_gram_h", "mgx_box_residual", "mgx_box_residual_norm", "mgx_box_density",
     "mgx_solver_create", "mgx_solver_destroy", "mgx_solver_step", "mgx_solver_run", "mgx_solver_reset",
@@ -129,6 +129,8 @@ def lib():
             "mgx_boxset_create": (P, C.POINTER(Boxes), I, C.POINTER(P)), "mgx_boxset_destroy": (P,),
             "mgx_boxset_size": (P, I, C.POINTER(I64)), "mgx_boxset_offset": (P, I, I, C.POINTER(I64)),
             "mgx_boxset_npairs": (P, C.POINTER(I64)),
+            "mgx_boxset_autotune": (P, P, C.POINTER(I), C.POINTER(D)),
+            "mgx_boxset_set_tune": (P, C.POINTER(I)), "mgx_boxset_get_tune": (P, C.POINTER(I), C.POINTER(I)),
             "mgx_box_lap": (P, P, I, I, I, P, P), "mgx_box_hloc": (P, P, I, P, P, P),
             "mgx_box_jacobi": (P, P, I, I, I, D, P, P, P), "mgx_box_restrict": (P, P, I, I, P, P),
             "mgx_box_prolong": (P, P, I, I, P, P), "mgx_box_precond_mg": (P, P, I, I, D, I, P, P),
@@ -268,6 +270,10 @@ def block(t, grid, nfunc=None):
     return b
 
 
+# the autotuner's tunables, in MGX_TUNE_* order
+TUNE_NAMES = ["march_f16", "march_f32", "march_f64", "hloc_f64", "gram_f64", "gram_split"]
+
+
 class BoxSet:
     def __init__(self, ctx, origin, edge, gdim, h, levels):
         import numpy as np
@@ -296,6 +302,27 @@ class BoxSet:
         n = C.c_int64()
         self.ctx.check(lib().mgx_boxset_npairs(self.h, C.byref(n)), "mgx_boxset_npairs")
         return n.value
+
+    def autotune(self):
+        """run the autotuner; returns {tunable: (choice, ms of candidate 1, ms of candidate 2)}"""
+        ch = (C.c_int * len(TUNE_NAMES))()
+        ms = (C.c_double * (2 * len(TUNE_NAMES)))()
+        self.ctx.check(lib().mgx_boxset_autotune(self.ctx.h, self.h, ch, ms), "mgx_boxset_autotune")
+        return {k: (ch[i], ms[2 * i], ms[2 * i + 1]) for i, k in enumerate(TUNE_NAMES)}
+
+    def set_tune(self, choice):
+        """force the launch shapes: {tunable: 1 or 2} (missing ones stay)"""
+        cur = self.get_tune()[0]
+        cur.update(choice)
+        ch = (C.c_int * len(TUNE_NAMES))(*[cur[k] for k in TUNE_NAMES])
+        self.ctx.check(lib().mgx_boxset_set_tune(self.h, ch), "mgx_boxset_set_tune")
+
+    def get_tune(self):
+        """({tunable: choice}, tuned)"""
+        ch = (C.c_int * len(TUNE_NAMES))()
+        t = C.c_int()
+        self.ctx.check(lib().mgx_boxset_get_tune(self.h, ch, C.byref(t)), "mgx_boxset_get_tune")
+        return {k: ch[i] for i, k in enumerate(TUNE_NAMES)}, bool(t.value)
 
     def density(self, psi, dm, rho, nel=0.0):
         """rho (device, gdim^3) = sum_ij dm_ij psi_i psi_j (mgx_box_density);

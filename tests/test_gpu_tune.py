@@ -1,0 +1,150 @@
+"""The box set's autotuner (mgx_boxset_autotune) and every launch shape it
+can choose.
+
+* The tuner times both candidates of every tunable on the box set's layout
+  and keeps the faster one (times reported per candidate).
+* Each shape gives the oracle's results: the fine Jacobi sweep (FP32, FP64),
+  H psi (FP64) and the FP16 V-cycle bit-identical between one and two lane
+  groups per thread (the arithmetic per point does not depend on the lane
+  mapping); S / H with one or two waves per cell within the oracle's
+  tolerances (the point sums are split differently between the waves).
+* A solver on a box set that was not tuned tunes it at creation.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from mgmol_amd import mgx, synth
+from tests.gpu_util import relmax, say, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tests.gpu_util import ctx as shared
+    return shared()
+
+
+def _cfg():
+    # 32^3 boxes (all tunables apply: 256 / 512 lane groups of the fine sweeps)
+    return synth.box_config(nbox=12, gdim=72, h=0.3, edges=(32,), nwells=4, seed=17, levels=3)
+
+
+def test_autotune_reports_and_chooses(ctx):
+    cfg = _cfg()
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    try:
+        assert bs.get_tune()[1] is False
+        rep = bs.autotune()
+        say("autotune", rep)
+        for k, (choice, t1, t2) in rep.items():
+            assert choice in (1, 2)
+            assert t1 > 0 and t2 > 0
+            assert (choice == 1) == (t1 <= t2)
+        assert bs.get_tune()[1] is True
+    finally:
+        bs.close()
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_march_shapes_match_oracle(ctx, dt):
+    cfg = _cfg()
+    ob = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    DT = np.float32 if dt == "f32" else np.float64
+    code = mgx.F32 if dt == "f32" else mgx.F64
+    key = "march_" + dt
+    try:
+        rng = np.random.default_rng(3)
+        f = rng.standard_normal(ob.size(0)).astype(DT)
+        x = rng.standard_normal(ob.size(0)).astype(DT)
+        jf = O.jacobi_factor(2, list(ob.h))
+        xr, wr = x.copy(), np.zeros_like(x)
+        O.b_jacobi(ob, 0, 2, f, xr, wr, jf)
+        outs = []
+        for p in (1, 2):
+            bs.set_tune({key: p, "hloc_f64": p})
+            tf, tx, tw = to_dev(f), to_dev(x), to_dev(np.zeros_like(x))
+            ctx.call("mgx_box_jacobi", bs.h, 0, 2, code, jf, mgx.ptr(tf), mgx.ptr(tx), mgx.ptr(tw))
+            outs.append((to_np(tx), to_np(tw)))
+            say("jacobi", dt, "groups per thread", p, relmax(outs[-1][0], xr))
+            assert relmax(outs[-1][0], xr) <= (1e-6 if dt == "f32" else 1e-13)
+        assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+        if dt == "f64":
+            ref = O.b_hloc(ob, cfg.vtot, cfg.psi)
+            hs = []
+            for p in (1, 2):
+                bs.set_tune({"hloc_f64": p})
+                tp, tv, th = to_dev(cfg.psi), to_dev(cfg.vtot), to_dev(np.zeros_like(cfg.psi))
+                ctx.call("mgx_box_hloc", bs.h, mgx.F64, mgx.ptr(tp), mgx.ptr(tv), mgx.ptr(th))
+                hs.append(to_np(th))
+                assert relmax(hs[-1], ref) <= 1e-13
+            assert np.array_equal(hs[0], hs[1])
+    finally:
+        bs.close()
+
+
+def test_f16_vcycle_shapes_identical(ctx):
+    cfg = _cfg()
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    try:
+        r = np.random.default_rng(4).standard_normal(int(cfg.psi.size))
+        outs = []
+        for p in (1, 2):
+            bs.set_tune({"march_f16": p})
+            tr, to = to_dev(r), to_dev(np.zeros_like(r))
+            ctx.call("mgx_box_precond_mg", bs.h, 2, 2, 0.0, mgx.F16, mgx.ptr(tr), mgx.ptr(to))
+            outs.append(to_np(to))
+        assert np.array_equal(outs[0], outs[1])
+    finally:
+        bs.close()
+
+
+@pytest.mark.parametrize("prec", [mgx.PREC_F64, mgx.PREC_SPLIT])
+def test_gram_waves_match_oracle(ctx, prec):
+    cfg = _cfg()
+    ob = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    key = "gram_f64" if prec == mgx.PREC_F64 else "gram_split"
+    try:
+        hp = O.b_hloc(ob, cfg.vtot, cfg.psi)
+        S, H = O.b_gram_h(ob, cfg.psi, hp)
+        n = ob.nbox
+        res = []
+        for nw in (1, 2):
+            bs.set_tune({key: nw})
+            tp, th = to_dev(cfg.psi), to_dev(hp)
+            tS, tH = to_dev(np.zeros((n, n))), to_dev(np.zeros((n, n)))
+            ctx.call("mgx_box_gram_h", bs.h, prec, mgx.ptr(tp), mgx.ptr(th), mgx.ptr(tS), mgx.ptr(tH))
+            es, eh = relmax(to_np(tS), S), relmax(to_np(tH), H)
+            say("gram", prec, "waves", nw, es, eh)
+            tol = 1e-13 if prec == mgx.PREC_F64 else 1e-7
+            assert es <= tol and eh <= tol
+            res.append((to_np(tS), to_np(tH)))
+        assert relmax(res[1][0], res[0][0]) <= (1e-14 if prec == mgx.PREC_F64 else 1e-7)
+    finally:
+        bs.close()
+
+
+def test_solver_tunes_its_box_set(ctx):
+    cfg = _cfg()
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    try:
+        s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, mg_levels=2, anderson_m=1)
+        choice, tuned = bs.get_tune()
+        say("solver's box set tuned:", choice)
+        assert tuned
+        e = [s.step().band_energy for _ in range(6)]
+        s.close()
+        # the other shapes give the same steps to rounding
+        other = {k: 3 - v for k, v in choice.items()}
+        bs.set_tune(other)
+        s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, mg_levels=2, anderson_m=1)
+        assert bs.get_tune()[0] == other
+        e2 = [s.step().band_energy for _ in range(6)]
+        s.close()
+        for a, b in zip(e, e2):
+            assert a == pytest.approx(b, rel=1e-12)
+    finally:
+        bs.close()
