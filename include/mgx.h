@@ -109,6 +109,16 @@ int mgx_rhs_apply(mgx_ctx* ctx, const mgx_block_t* in, mgx_block_t* out);
 /* GridFuncVector::pointwiseProduct, src/pb/GridFuncVector.cc:90-136;
  * v: device FP64 ghosted single function */
 int mgx_pointwise_product(mgx_ctx* ctx, const mgx_block_t* a, const double* v, mgx_block_t* out);
+/* Synthetic input generator for the dense contraction workload (no
+ * counterpart in the reference): separable orbitals psi_j(x,y,z) =
+ * fx_j(x) fy_j(y) fz_j(z) on the z-planes [z0, z0 + nzs) of a periodic
+ * gdim grid, and H psi_j = (-Delta_h4 + V) psi_j from the factors' 1D
+ * 4th-order second differences lf (the stencil is a sum of 1D stencils).
+ * Factors n x gdim[d] row-major; vtot gdim[0]*gdim[1]*gdim[2] (z fastest);
+ * psi, hpsi column-major, ld = gdim[0]*gdim[1]*nzs (x slowest, z fastest). */
+int mgx_synth_separable(mgx_ctx* ctx, int n, const int gdim[3], int z0, int nzs, const double* fx,
+    const double* fy, const double* fz, const double* lfx, const double* lfy, const double* lfz,
+    const double* vtot, double* psi, double* hpsi);
 /* LinearAlgebraUtils::MPaxpy, src/linear_algebra/mputils.cc:221-244:
  * y += (T)(alpha*(double)x), n elements of dtype */
 int mgx_axpy(mgx_ctx* ctx, int dtype, int64_t n, double alpha, const void* x, void* y);

@@ -497,20 +497,15 @@ int spmm_pairs(mgx_ctx* ctx, int n, const double* X, const double* val, const Pa
 }
 
 // Wide products (n >= kWideN): the tiles of k_tsgemm would re-read each
-// column strip n/16 times, so these go to the library GEMMs, split over K
-// into batches (an n x n output alone is too few tiles for the chip when K
-// is hundreds of thousands of rows) whose partial products are summed in
-// FP64.  FP64: strided-batched dgemm over chunks of kF64Chunk rows or more
-// (at most kF64MaxParts partials) plus one dgemm for the remainder.  Split:
-// the rows are taken in groups of kWideGroup chunks of kWideChunk rows; each
-// group is converted to FP32 hi / lo copies (hi = fl32(a), lo = fl32(a - hi))
-// and multiplied chunk by chunk with strided-batched FP32 GEMMs, hi*hi and
-// the cross terms in separate FP32 partials per chunk; for a Gram the cross
-// term is hi^T lo plus its transpose.  A Gram (A == B) is mirrored from its
-// lower triangle, so it is exactly symmetric.
+// column strip n/16 times from scattered 8-byte loads.  FP64: the library's
+// dgemm (the reference's own DGEMM), split over K into batches (an n x n
+// output alone is too few tiles for the chip when K is hundreds of
+// thousands of rows) whose partial products are summed in FP64:
+// strided-batched dgemm over chunks of kF64Chunk rows or more (at most
+// kF64MaxParts partials) plus one dgemm for the remainder.  Split:
+// k_tsgemm_split_wide (below).  A Gram (A == B) is mirrored from its lower
+// triangle, so it is exactly symmetric.
 constexpr int kWideN       = 256;
-constexpr int kWideChunk   = 1024;
-constexpr int kWideGroup   = 32;
 constexpr int kF64Chunk    = 16384;
 constexpr int kF64MaxParts = 64;
 
@@ -531,46 +526,172 @@ __global__ void k_sum_parts(const double* parts, int np, int n, int sym, double*
     }
 }
 
-// hi / lo of rows [k0, k0 + rows) of each column of A (leading dimension
-// ld, lnp valid rows), stored with leading dimension kp = group rows;
-// rows past lnp are written as zeros
-__global__ void k_split_hilo_ld(const double* A, int64_t ld, int64_t lnp, int64_t k0, int64_t kp, int n, float* hi,
-    float* lo)
-{
-    const int64_t tot = kp * n;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x)
-    {
-        const int64_t col = t / kp, k = k0 + (t - col * kp);
-        const double a    = k < lnp ? A[col * ld + k] : 0.;
-        const float h     = (float)a;
-        hi[t]             = h;
-        lo[t]             = (float)(a - (double)h);
-    }
-}
+// The hi/lo split of a wide product on FP32 MFMA.  One block computes a
+// kWT x kWT output tile (rows from A's columns i0.., columns from B's j0..)
+// over one range of rows (a K-split).  Per chunk of kWK rows the block loads
+// both column panels once, coalesced (a wave reads two columns' 512
+// contiguous bytes per instruction), splits each value into hi = fl32(x),
+// lo = fl32(x - hi) in registers and stores them to LDS column by column
+// (stride kWL floats: the stores and the MFMA operand reads are free of bank
+// conflicts).  Each wave owns a 32 x 32 quadrant (2 x 2 tiles of
+// v_mfma_f32_16x16x4_f32): hi*hi and hi*lo + lo*hi accumulate in FP32 over
+// the chunk's kWK rows, then the chunk sums are added in FP64.  Partial
+// products of the K-splits go to part (FP64, n x n each, column-major).
+constexpr int kWT = 64;  // output tile edge
+constexpr int kWK = 64;  // rows per LDS panel
+constexpr int kWR = 4;   // rows per FP32 sum of hi*hi (then added in FP64)
+constexpr int kWL = 68;  // LDS column stride in floats (64 rows + 4)
 
-// C (+)= vel * sum_c (p0[c] + p1[c]) over nchunk n x n FP32 partials,
-// summed in FP64.  sym: the cross term is p1 + p1^T and both triangles of C
-// are written from the lower.
-__global__ void k_wide_sum(const float* p0, const float* p1, int nchunk, int n, double vel, int accumulate, int sym,
-    double* C)
+__global__ void __launch_bounds__(256, 2) k_tsgemm_split_wide(const double* A, const double* B, int64_t ld,
+    int64_t lnp, int n, int ntile, int64_t rows_per_split, int lower_only, double* part)
 {
-    const int64_t nn = (int64_t)n * n;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nn; t += (int64_t)gridDim.x * blockDim.x)
+    typedef double dd2 __attribute__((ext_vector_type(2)));
+    typedef float ff2 __attribute__((ext_vector_type(2)));
+    // lower_only: the grid enumerates the tiles on and below the diagonal
+    // only (so the round-robin of blocks over the XCDs stays balanced)
+    const int tile = blockIdx.x;
+    int ti, tj;
+    if (lower_only)
     {
-        const int col = (int)(t / n), row = (int)(t - (int64_t)col * n);
-        if (sym && row < col) continue;
-        const int64_t tt = (int64_t)row * n + col;
-        double hh = 0., x = 0.;
-        for (int c = 0; c < nchunk; c++)
-        {
-            hh += (double)p0[c * nn + t];
-            x += (double)p1[c * nn + t];
-            if (sym) x += (double)p1[c * nn + tt];
-        }
-        const double v = vel * (hh + x) + (accumulate ? C[t] : 0.);
-        C[t]           = v;
-        if (sym) C[tt] = v;
+        ti = (int)((sqrtf(8.f * (float)tile + 1.f) - 1.f) * 0.5f);
+        while ((ti + 1) * (ti + 2) / 2 <= tile)
+            ti++;
+        while (ti * (ti + 1) / 2 > tile)
+            ti--;
+        tj = tile - ti * (ti + 1) / 2;
     }
+    else
+    {
+        ti = tile / ntile;
+        tj = tile - ti * ntile;
+    }
+    const int split  = blockIdx.y;
+    const int64_t k0 = (int64_t)split * rows_per_split;
+    const int64_t k1 = min(k0 + rows_per_split, lnp);
+    // a diagonal tile of a Gram reads one panel for both operands
+    const bool one = A == B && ti == tj;
+    __shared__ float Ah[kWT * kWL], Al[kWT * kWL], Bh[kWT * kWL], Bl[kWT * kWL];
+    const float* Bhr = one ? Ah : Bh;
+    const float* Blr = one ? Al : Bl;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wi = w & 1, wj = w >> 1;
+    const int c = lane & 15, r = lane >> 4;
+    const int i0 = ti * kWT, j0 = tj * kWT;
+    // loader: rows 2 rr, 2 rr + 1 of column q * 8 + (t >> 5) for q = 0..7
+    const int rr = 2 * (t & 31), cq = t >> 5;
+    double acc[2][2][4] = {};
+    for (int64_t kc = k0; kc < k1; kc += kWK)
+    {
+        const int64_t row = kc + rr;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+        {
+            const int col = q * 8 + cq;
+            // two rows at once where they are 16-byte aligned (even ld),
+            // else one by one
+            auto ld2 = [&](const double* p) -> dd2 {
+                dd2 v = { 0., 0. };
+                if (row + 1 < k1 && (reinterpret_cast<uintptr_t>(p) & 15) == 0)
+                    v = *reinterpret_cast<const dd2*>(p);
+                else
+                {
+                    if (row < k1) v[0] = p[0];
+                    if (row + 1 < k1) v[1] = p[1];
+                }
+                return v;
+            };
+            dd2 a = { 0., 0. }, b = { 0., 0. };
+            if (i0 + col < n) a = ld2(A + (int64_t)(i0 + col) * ld + row);
+            if (!one && j0 + col < n) b = ld2(B + (int64_t)(j0 + col) * ld + row);
+            ff2 h, l;
+            h[0] = (float)a[0];
+            h[1] = (float)a[1];
+            l[0] = (float)(a[0] - (double)h[0]);
+            l[1] = (float)(a[1] - (double)h[1]);
+            *reinterpret_cast<ff2*>(Ah + col * kWL + rr) = h;
+            *reinterpret_cast<ff2*>(Al + col * kWL + rr) = l;
+            if (!one)
+            {
+                h[0] = (float)b[0];
+                h[1] = (float)b[1];
+                l[0] = (float)(b[0] - (double)h[0]);
+                l[1] = (float)(b[1] - (double)h[1]);
+                *reinterpret_cast<ff2*>(Bh + col * kWL + rr) = h;
+                *reinterpret_cast<ff2*>(Bl + col * kWL + rr) = l;
+            }
+        }
+        __syncthreads();
+        // hi*hi: FP32 sums over kWR rows (kWR / 4 MFMA steps), then into
+        // FP64; the cross terms (2^-24 of it) in FP32 over the panel's rows
+        f4 hx[2][2];
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+                hx[x][y] = f4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll 2
+        for (int s0 = 0; s0 < kWK / 4; s0 += kWR / 4)
+        {
+            f4 hh[2][2];
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++)
+                    hh[x][y] = f4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+            for (int s = s0; s < s0 + kWR / 4; s++)
+            {
+                const int kk = 4 * s + r;
+                float ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+                {
+                    const int ia = (wi * 32 + x * 16 + c) * kWL + kk;
+                    const int jb = (wj * 32 + x * 16 + c) * kWL + kk;
+                    ah[x]        = Ah[ia];
+                    al[x]        = Al[ia];
+                    bh[x]        = Bhr[jb];
+                    bl[x]        = Blr[jb];
+                }
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+                    {
+                        hh[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[x], bh[y], hh[x][y], 0, 0, 0);
+                        hx[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[x], bl[y], hx[x][y], 0, 0, 0);
+                        hx[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[x], bh[y], hx[x][y], 0, 0, 0);
+                    }
+            }
+#pragma unroll
+            for (int x = 0; x < 2; x++)
+#pragma unroll
+                for (int y = 0; y < 2; y++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        acc[x][y][v] += (double)hh[x][y][v];
+        }
+#pragma unroll
+        for (int x = 0; x < 2; x++)
+#pragma unroll
+            for (int y = 0; y < 2; y++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[x][y][v] += (double)hx[x][y][v];
+        __syncthreads();
+    }
+    // accumulator v of lane l: D[4 (l / 16) + v][l % 16] (FP32 MFMA layout)
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+            {
+                const int row = i0 + wi * 32 + x * 16 + 4 * r + v;
+                const int col = j0 + wj * 32 + y * 16 + c;
+                if (row < n && col < n) part[((int64_t)split * n + col) * n + row] = acc[x][y][v];
+            }
 }
 
 int tsgemm_wide(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_t ld, int64_t lnp, int n,
@@ -593,59 +714,38 @@ int tsgemm_wide(mgx_ctx* ctx, int prec, const double* A, const double* B, int64_
             double* parts      = (double*)workspace(ctx, 7, (size_t)np * nn * sizeof(double));
             if (!parts) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
             rocblas_status st = rocblas_status_success;
+            // full products also for a Gram: rocBLAS dsyrk measured 10x slower here
             if (nful > 0)
-                st = rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n,
-                    n, (rocblas_int)kc, &vel, As, (rocblas_int)ld, kc, Bs, (rocblas_int)ld, kc, &zero, parts, n, nn,
-                    (rocblas_int)nful);
+                st = rocblas_dgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                    n, n, (rocblas_int)kc, &vel, As, (rocblas_int)ld, kc, Bs, (rocblas_int)ld, kc, &zero, parts, n,
+                    nn, (rocblas_int)nful);
             if (st == rocblas_status_success && rem > 0)
                 st = rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n, n,
-                    (rocblas_int)rem, &vel, As + nful * kc, (rocblas_int)ld, Bs + nful * kc, (rocblas_int)ld, &zero,
-                    parts + nful * nn, n);
+                    (rocblas_int)rem, &vel, As + nful * kc, (rocblas_int)ld, Bs + nful * kc, (rocblas_int)ld,
+                    &zero, parts + nful * nn, n);
             if (st != rocblas_status_success) return set_err(ctx, MGX_EHIP, "rocBLAS FP64 product failed");
             hipLaunchKernelGGL(k_sum_parts, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, (const double*)parts, np,
                 n, same ? 1 : 0, C);
             MGX_CHECK_LAUNCH(ctx);
             continue;
         }
-        const int64_t nchunk_all = (lnp + kWideChunk - 1) / kWideChunk;
-        const int gch            = (int)std::min<int64_t>(nchunk_all, kWideGroup);
-        const int64_t kp         = (int64_t)gch * kWideChunk;
-        const size_t opf         = (size_t)kp * n;
-        const size_t need        = ((same ? 2 : 4) * opf + 2 * (size_t)gch * nn) * sizeof(float);
-        float* w                 = (float*)workspace(ctx, 7, need);
-        if (!w) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
-        float* ahi = w;
-        float* alo = ahi + opf;
-        float* bhi = same ? ahi : alo + opf;
-        float* blo = same ? alo : bhi + opf;
-        float* p0  = (same ? alo : blo) + opf;
-        float* p1  = p0 + (size_t)gch * nn;
-        const float one = 1.f, zero = 0.f;
-        const rocblas_stride sk = kWideChunk, sc = nn;
-        for (int64_t c0 = 0; c0 < nchunk_all; c0 += gch)
-        {
-            const int nc    = (int)std::min<int64_t>(gch, nchunk_all - c0);
-            const int64_t k0 = c0 * kWideChunk;
-            hipLaunchKernelGGL(k_split_hilo_ld, dim3(4096), dim3(256), 0, ctx->stream, As, ld, lnp, k0, kp, n, ahi,
-                alo);
-            if (!same)
-                hipLaunchKernelGGL(k_split_hilo_ld, dim3(4096), dim3(256), 0, ctx->stream, Bs, ld, lnp, k0, kp, n,
-                    bhi, blo);
-            MGX_CHECK_LAUNCH(ctx);
-            rocblas_status st = rocblas_sgemm_strided_batched(ctx->blas, rocblas_operation_transpose,
-                rocblas_operation_none, n, n, kWideChunk, &one, ahi, (rocblas_int)kp, sk, bhi, (rocblas_int)kp, sk,
-                &zero, p0, n, sc, nc);
-            if (st == rocblas_status_success)
-                st = rocblas_sgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n,
-                    n, kWideChunk, &one, ahi, (rocblas_int)kp, sk, blo, (rocblas_int)kp, sk, &zero, p1, n, sc, nc);
-            if (st == rocblas_status_success && !same)
-                st = rocblas_sgemm_strided_batched(ctx->blas, rocblas_operation_transpose, rocblas_operation_none, n,
-                    n, kWideChunk, &one, alo, (rocblas_int)kp, sk, bhi, (rocblas_int)kp, sk, &one, p1, n, sc, nc);
-            if (st != rocblas_status_success) return set_err(ctx, MGX_EHIP, "rocBLAS FP32 product failed");
-            hipLaunchKernelGGL(k_wide_sum, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, (const float*)p0,
-                (const float*)p1, nc, n, vel, c0 > 0 ? 1 : 0, same ? 1 : 0, C);
-            MGX_CHECK_LAUNCH(ctx);
-        }
+        // split: K divided among enough blocks to fill the chip (at least
+        // 32 chunks each), rows per split a multiple of the chunk
+        const int ntile     = (n + kWT - 1) / kWT;
+        const int active    = same ? ntile * (ntile + 1) / 2 : ntile * ntile;
+        const int64_t nchk  = (lnp + kWK - 1) / kWK;
+        const int64_t want  = std::max<int64_t>(1, (2048 + active - 1) / active);
+        const int nsplit    = (int)std::max<int64_t>(1, std::min<int64_t>(want, nchk / 32));
+        const int64_t rps   = ((nchk + nsplit - 1) / nsplit) * kWK;
+        double* parts       = (double*)workspace(ctx, 7, (size_t)nsplit * nn * sizeof(double));
+        if (!parts) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+        dim3 grid(active, nsplit), blk(256);
+        hipLaunchKernelGGL(k_tsgemm_split_wide, grid, blk, 0, ctx->stream, As, Bs, ld, lnp, n, ntile, rps,
+            same ? 1 : 0, parts);
+        MGX_CHECK_LAUNCH(ctx);
+        hipLaunchKernelGGL(k_chunk_reduce, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, (const double*)parts, n,
+            nsplit, 1, vel, same ? 1 : 0, C);
+        MGX_CHECK_LAUNCH(ctx);
     }
     return MGX_OK;
 }

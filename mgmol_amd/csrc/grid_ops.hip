@@ -1030,6 +1030,36 @@ int halo(mgx_ctx* ctx, int dtype, void* data, int nfunc, const mgx_grid_t& g)
 
 using namespace mgx;
 
+namespace mgx
+{
+namespace
+{
+// separable orbitals on z-planes [z0, z0 + nzs) of a periodic grid and
+// (-Delta_h4 + V) of them: the 4th-order stencil is a sum of 1D stencils, so
+// H psi = (L fx) fy fz + fx (L fy) fz + fx fy (L fz) + V psi exactly, with
+// the 1D second differences L f of the factors given
+__global__ void k_sep_orbitals(int n, int gx, int gy, int gz, int z0, int nzs, const double* fx, const double* fy,
+    const double* fz, const double* lfx, const double* lfy, const double* lfz, const double* vtot, double* psi,
+    double* hpsi)
+{
+    const int64_t K   = (int64_t)gx * gy * nzs;
+    const int64_t tot = K * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int j     = (int)(t / K);
+        const int64_t k = t - (int64_t)j * K;
+        const int zl = (int)(k % nzs), y = (int)((k / nzs) % gy), x = (int)(k / ((int64_t)nzs * gy));
+        const int z  = z0 + zl;
+        const double a = fx[(int64_t)j * gx + x], b = fy[(int64_t)j * gy + y], c = fz[(int64_t)j * gz + z];
+        const double p = a * b * c;
+        psi[t]         = p;
+        hpsi[t] = lfx[(int64_t)j * gx + x] * b * c + a * lfy[(int64_t)j * gy + y] * c + a * b * lfz[(int64_t)j * gz + z]
+                  + vtot[((int64_t)x * gy + y) * gz + z] * p;
+    }
+}
+}  // namespace
+}  // namespace mgx
+
 extern "C" {
 
 int mgx_halo(mgx_ctx* ctx, mgx_block_t* b)
@@ -1088,6 +1118,20 @@ int mgx_pointwise_product(mgx_ctx* ctx, const mgx_block_t* a, const double* v, m
     else
         hipLaunchKernelGGL(k_pointwise<double>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream,
             (const double*)a->data, v, (double*)out->data, g.sizeg, total);
+    MGX_CHECK_LAUNCH(ctx);
+    return sync(ctx);
+}
+
+int mgx_synth_separable(mgx_ctx* ctx, int n, const int gdim[3], int z0, int nzs, const double* fx,
+    const double* fy, const double* fz, const double* lfx, const double* lfy, const double* lfz, const double* vtot,
+    double* psi, double* hpsi)
+{
+    MGX_REQUIRE(ctx, n > 0 && gdim && gdim[0] > 0 && gdim[1] > 0 && gdim[2] > 0, "bad separable orbital sizes");
+    MGX_REQUIRE(ctx, z0 >= 0 && nzs > 0 && z0 + nzs <= gdim[2], "z-planes outside the grid");
+    MGX_REQUIRE(ctx, fx && fy && fz && lfx && lfy && lfz && vtot && psi && hpsi, "null data");
+    const int64_t tot = (int64_t)gdim[0] * gdim[1] * nzs * n;
+    hipLaunchKernelGGL(k_sep_orbitals, dim3(nblocks(tot)), dim3(kBlock), 0, ctx->stream, n, gdim[0], gdim[1],
+        gdim[2], z0, nzs, fx, fy, fz, lfx, lfy, lfz, vtot, psi, hpsi);
     MGX_CHECK_LAUNCH(ctx);
     return sync(ctx);
 }

@@ -8,12 +8,13 @@ each (the converged C5 solve is bench.py's):
 c2: 512 orbitals in boxes of edge {24..40} on a 192^3 grid, h = 0.25 bohr,
     16 wells, centres at least 2 bohr apart; ABPG (Anderson m = 1) to
     |R| < 1e-6 on the plain and mixed paths, kernel time per group.
-c3: one z-slab of the dense 256^3 x 1024 problem (8 planes, K = 524,288 grid
-    points, N = 1024 Gaussian orbitals with sigma ~ U(0.8, 1.5) bohr, h = 0.2):
-    S = h^3 Psi^T Psi and H = h^3 Psi^T (Psi') in FP64 (dsyrk / dgemm) and
-    with the hi/lo split (FP32 chunk products summed in FP64);
-    FP64-equivalent flops K N (N+1) and 2 K N^2; error of the split against
-    the FP64 result.
+c3: the dense 256^3 x 1024 problem (N = 1024 Gaussian orbitals with sigma ~
+    U(0.8, 1.5) bohr, h = 0.2, 64 wells) streamed through 32 z-slabs of 8
+    planes: S = h^3 Psi^T Psi and H = h^3 Psi^T (H Psi) in FP64 (library
+    dgemm) and with the hi/lo split (hand-written FP32 MFMA kernel), summed
+    over the slabs in FP64, then the FP64 Cholesky + inverse of S;
+    FP64-equivalent flops K N (N+1) and 2 K N^2, FP32 MFMA utilisation of the
+    split; error of the split against the FP64 result.
 c4: inference over 2048 boxes of 32^3 (29,376 flop per voxel) and Adam steps
     on 64-box batches (forward + backward + weight gradients).
 poisson: the Hartree multigrid (mgx_poisson_mg, Mehrstellen, nu1 = nu2 = 2) on
@@ -67,56 +68,105 @@ def c2(ctx):
     return out
 
 
-def _slab_orbitals(n, gdim, h, nz, seed):
-    """Gaussian orbitals (sigma ~ U(0.8, 1.5)) on the z-planes 0..nz-1 of a
-    periodic gdim^3 grid, normalised over the full grid: (n, gdim*gdim*nz)."""
+def _separable_factors(n, gdim, h, seed):
+    """1D factors of n Gaussian orbitals (sigma ~ U(0.8, 1.5) bohr, centres
+    ~ U([0, L)^3)) on a periodic gdim^3 grid, each orbital normalised to
+    h^3 sum psi^2 = 1, and the factors' 1D 4th-order second differences
+    (-d^2/dx^2: c0 = 30/(12 h^2), c1 = -16/(12 h^2), c2 = 1/(12 h^2))."""
     rng = np.random.default_rng(seed)
     L = gdim * h
     cen = rng.uniform(0.0, 1.0, (n, 3)) * L
     sig = rng.uniform(0.8, 1.5, n)
     x = np.arange(gdim) * h
-    z = np.arange(nz) * h
-    psi = np.zeros((n, gdim, gdim, nz))
-    for j in range(n):
-        f = []
-        for d, coords in enumerate((x, x, z)):
-            dd = coords - cen[j, d]
-            dd -= L * np.round(dd / L)
-            f.append(np.exp(-dd * dd / (2.0 * sig[j] ** 2)))
-        psi[j] = np.einsum("i,j,k->ijk", *f)
-        # normalisation of the full-grid Gaussian: h^3 sum = (sqrt(pi) sigma)^3
-        psi[j] /= np.sqrt((np.sqrt(np.pi) * sig[j]) ** 3)
-    return psi.reshape(n, -1)
+    c0, c1, c2 = 30.0 / (12 * h * h), -16.0 / (12 * h * h), 1.0 / (12 * h * h)
+    f, lf = [], []
+    for d in range(3):
+        dd = x[None, :] - cen[:, d:d + 1]
+        dd -= L * np.round(dd / L)
+        f.append(np.exp(-dd * dd / (2.0 * sig[:, None] ** 2)))
+    norm = np.sqrt(h ** 3 * np.prod([np.sum(g * g, axis=1) for g in f], axis=0))
+    f[0] /= norm[:, None]
+    for g in f:
+        lf.append(c0 * g + c1 * (np.roll(g, 1, 1) + np.roll(g, -1, 1)) + c2 * (np.roll(g, 2, 1) + np.roll(g, -2, 1)))
+    return f, lf, rng
 
 
-def c3(ctx):
-    n, gdim, h, nz = 1024, 256, 0.2, 8
-    psi = _slab_orbitals(n, gdim, h, nz, seed=3)
-    K = psi.shape[1]
-    # psi' = psi with a perturbation, standing in for H psi in the product
-    rng = np.random.default_rng(4)
-    hp = psi + 1e-3 * rng.standard_normal(psi.shape) * (psi != 0)
+def c3(ctx, nslab=None):
+    """The dense contraction workload in full: S = h^3 Psi^T Psi and
+    H = h^3 Psi^T (H Psi) over the 256^3 grid, N = 1024, streamed through
+    32 z-slabs of 8 planes (K = 524,288 rows each; Psi and H Psi of a slab
+    generated on the device, H = -Delta_h4 + V with 64 wells), slab results
+    summed in FP64; then the FP64 Cholesky + inverse of S.  FP64 (library
+    dgemm) and the hi/lo split (hand-written FP32 MFMA kernel) on the same
+    slabs; the split's error against FP64 for every entry."""
+    n, gdim, h, nzs, nwells = 1024, 256, 0.2, 8, 64
+    nslab = nslab or gdim // nzs
+    f, lf, rng = _separable_factors(n, gdim, h, seed=3)
+    v, _, _ = synth.wells((gdim,) * 3, h, nwells, rng)
+    K = gdim * gdim * nzs
     vel = h ** 3
-    dp, dh = mgx.DeviceArray(ctx, psi), mgx.DeviceArray(ctx, hp)
-    del psi, hp
-    S = {p: mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64) for p in (mgx.PREC_F64, mgx.PREC_SPLIT)}
-    H = {p: mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64) for p in (mgx.PREC_F64, mgx.PREC_SPLIT)}
-    out = {"config": "c3", "K": K, "N": n, "note": "one 8-plane z-slab of the 256^3 grid"}
-    for p, name in ((mgx.PREC_F64, "f64"), (mgx.PREC_SPLIT, "split")):
-        ts = _timeit(lambda: ctx.check(mgx.lib().mgx_local_gram(ctx.h, p, mgx.ptr(dp), K, K, n, 1, vel,
-                                                                mgx.ptr(S[p])), "gram"), 5)
-        th = _timeit(lambda: ctx.check(mgx.lib().mgx_local_product(ctx.h, p, mgx.ptr(dp), mgx.ptr(dh), K, K, n, 1,
-                                                                   vel, mgx.ptr(H[p])), "product"), 5)
-        fs, fh = K * n * (n + 1.0), 2.0 * K * n * n
-        out[name] = {"S_ms": round(1e3 * ts, 3), "H_ms": round(1e3 * th, 3),
+    dev = [mgx.DeviceArray(ctx, np.ascontiguousarray(a)) for a in f + lf]
+    dv = mgx.DeviceArray(ctx, np.ascontiguousarray(v))
+    del v
+    dp = mgx.DeviceArray(ctx, shape=(n, K), dtype=np.float64, zero=False)
+    dh = mgx.DeviceArray(ctx, shape=(n, K), dtype=np.float64, zero=False)
+    tmp = mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64)
+    precs = ((mgx.PREC_F64, "f64"), (mgx.PREC_SPLIT, "split"))
+    acc = {(p, m): mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64) for p, _ in precs for m in "SH"}
+    gd = (C.c_int * 3)(gdim, gdim, gdim)
+    t_gen, t = 0.0, {(p, m): 0.0 for p, _ in precs for m in "SH"}
+    L = mgx.lib()
+    for sl in range(nslab):
+        t0 = time.perf_counter()
+        ctx.check(L.mgx_synth_separable(ctx.h, n, gd, sl * nzs, nzs, *[mgx.ptr(d) for d in dev], mgx.ptr(dv),
+                                        mgx.ptr(dp), mgx.ptr(dh)), "mgx_synth_separable")
+        t_gen += time.perf_counter() - t0
+        for p, _ in precs:
+            t0 = time.perf_counter()
+            ctx.check(L.mgx_local_gram(ctx.h, p, mgx.ptr(dp), K, K, n, 1, vel, mgx.ptr(tmp)), "gram")
+            t[(p, "S")] += time.perf_counter() - t0
+            ctx.check(L.mgx_axpy(ctx.h, mgx.F64, n * n, 1.0, mgx.ptr(tmp), mgx.ptr(acc[(p, "S")])), "axpy")
+            t0 = time.perf_counter()
+            ctx.check(L.mgx_local_product(ctx.h, p, mgx.ptr(dp), mgx.ptr(dh), K, K, n, 1, vel, mgx.ptr(tmp)),
+                      "product")
+            t[(p, "H")] += time.perf_counter() - t0
+            ctx.check(L.mgx_axpy(ctx.h, mgx.F64, n * n, 1.0, mgx.ptr(tmp), mgx.ptr(acc[(p, "H")])), "axpy")
+    # GramMatrix::updateLS / computeInverse on the summed FP64 S
+    Lm = mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64)
+    Si = mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64)
+    t0 = time.perf_counter()
+    ctx.check(L.mgx_chol_inv(ctx.h, n, mgx.ptr(acc[(mgx.PREC_F64, "S")]), mgx.ptr(Lm), mgx.ptr(Si)), "chol_inv")
+    t_chol = time.perf_counter() - t0
+    Ktot = K * nslab
+    fs, fh = Ktot * n * (n + 1.0), 2.0 * Ktot * n * n
+    nt = n // 64
+    # executed FP32 MFMA flops of the split: three products per output; the
+    # Gram computes the 64 x 64 tiles on and below the diagonal
+    xs, xh = 3 * 2.0 * Ktot * 64 * 64 * nt * (nt + 1) / 2, 3 * 2.0 * Ktot * n * n
+    out = {"config": "c3", "K": Ktot, "N": n, "slabs": nslab, "rows_per_slab": K,
+           "note": "Psi and H Psi generated on the device per 8-plane z-slab, slab products summed in FP64",
+           "generate_s": round(t_gen, 3), "cholesky_inverse_ms": round(1e3 * t_chol, 3)}
+    for p, name in precs:
+        ts, th = t[(p, "S")], t[(p, "H")]
+        out[name] = {"S_s": round(ts, 4), "H_s": round(th, 4), "S_plus_H_s": round(ts + th, 4),
                      "S_tflops_fp64_equiv": round(fs / ts / 1e12, 2), "H_tflops_fp64_equiv": round(fh / th / 1e12, 2)}
-    s64, s32 = S[mgx.PREC_F64].get(), S[mgx.PREC_SPLIT].get()
-    h64, h32 = H[mgx.PREC_F64].get(), H[mgx.PREC_SPLIT].get()
-    out["split_error"] = {"S_max_abs": float(np.max(np.abs(s32 - s64))),
-                          "S_max_rel": float(np.max(np.abs(s32 - s64)) / np.max(np.abs(s64))),
-                          "H_max_abs": float(np.max(np.abs(h32 - h64))),
-                          "H_max_rel": float(np.max(np.abs(h32 - h64)) / np.max(np.abs(h64)))}
-    for a in list(S.values()) + list(H.values()) + [dp, dh]:
+        if p == mgx.PREC_SPLIT:
+            out[name]["S_fp32_mfma_tflops"] = round(xs / ts / 1e12, 2)
+            out[name]["H_fp32_mfma_tflops"] = round(xh / th / 1e12, 2)
+            out[name]["S_fp32_mfma_utilisation"] = round(xs / ts / 157.3e12, 4)
+            out[name]["H_fp32_mfma_utilisation"] = round(xh / th / 157.3e12, 4)
+        else:
+            out[name]["S_fp64_mfma_utilisation_of_equiv"] = round(fs / ts / 78.6e12, 4)
+            out[name]["H_fp64_mfma_utilisation"] = round(fh / th / 78.6e12, 4)
+    err = {}
+    for m in "SH":
+        a64, a32 = acc[(mgx.PREC_F64, m)].get(), acc[(mgx.PREC_SPLIT, m)].get()
+        d = np.abs(a32 - a64)
+        err[m + "_max_abs"] = float(np.max(d))
+        err[m + "_max_rel_to_largest"] = float(np.max(d) / np.max(np.abs(a64)))
+        err[m + "_max_rel_diag"] = float(np.max(np.abs(np.diag(a32) - np.diag(a64)) / np.abs(np.diag(a64))))
+    out["split_error_vs_fp64"] = err
+    for a in dev + [dv, dp, dh, tmp, Lm, Si] + list(acc.values()):
         a.free()
     return out
 

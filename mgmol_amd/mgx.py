@@ -28,7 +28,7 @@ _LIB = None
 EXPORTS = [
     "mgx_ctx_create", "mgx_ctx_destroy", "mgx_last_error", "mgx_ctx_stream",
     "mgx_malloc", "mgx_free", "mgx_memcpy_htod", "mgx_memcpy_dtoh",
-    "mgx_halo", "mgx_lap_apply", "mgx_rhs_apply", "mgx_pointwise_product", "mgx_axpy", "mgx_stream_triad",
+    "mgx_halo", "mgx_lap_apply", "mgx_rhs_apply", "mgx_pointwise_product", "mgx_axpy", "mgx_stream_triad", "mgx_synth_separable",
     "mgx_jacobi",
     "mgx_restrict", "mgx_prolong", "mgx_mask", "mgx_mg",
     "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg",
@@ -109,6 +109,7 @@ def lib():
             "mgx_memcpy_htod": (P, P, P, C.c_size_t), "mgx_memcpy_dtoh": (P, P, P, C.c_size_t),
             "mgx_halo": (P, BP), "mgx_lap_apply": (P, I, BP, BP), "mgx_rhs_apply": (P, BP, BP),
             "mgx_pointwise_product": (P, BP, P, BP), "mgx_axpy": (P, I, I64, D, P, P),
+            "mgx_synth_separable": (P, I, C.POINTER(I), I, I, P, P, P, P, P, P, P, P, P),
             "mgx_stream_triad": (P, I64, P, P, P, I, C.POINTER(D), C.POINTER(I)),
             "mgx_jacobi": (P, I, D, BP, BP, BP), "mgx_restrict": (P, BP, BP), "mgx_prolong": (P, BP, BP),
             "mgx_mask": (P, BP, P), "mgx_mg": (P, I, I, BP, BP, PP),

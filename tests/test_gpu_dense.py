@@ -67,10 +67,11 @@ def test_local_gram_split_error(ctx):
 
 
 def test_wide_products(ctx):
-    """n >= 256 goes to the library path: dsyrk / dgemm in FP64, grouped
-    FP32 hi/lo chunk products summed in FP64 for the split.  Two slabs of
-    35,000 rows each: 35 chunks of 1024 (the last one zero-padded), two
-    groups of chunks."""
+    """n >= 256: the library dgemm in FP64 (K-batches summed in FP64); the
+    split on the hand-written wide kernel (hi / lo formed in registers, FP32
+    MFMA sums over 64-row chunks, FP64 over the chunks).  Two slabs of 35,000
+    rows each with an odd leading dimension (unaligned columns load one row
+    at a time); the last chunk of a split is partial."""
     rng = np.random.default_rng(21)
     n, lnp, sub = 256, 35000, 2
     ld = sub * lnp + 17
@@ -94,8 +95,9 @@ def test_wide_products(ctx):
     ctx.call("mgx_local_product", mgx.PREC_SPLIT, mgx.ptr(tp), mgx.ptr(th), ld, lnp, n, sub, vel, mgx.ptr(ts))
     ep = relmax(to_np(ts), pref)
     say("wide split rel err (max-norm): gram %.3e product %.3e" % (eg, ep))
-    # FP32 accumulation within 1024-row chunks, FP64 over the chunks
-    assert eg < 2e-7 and ep < 2e-7
+    # hi*hi of every 4 rows (one MFMA step) added in FP64, the cross terms
+    # (2^-24 of it) in FP32 over 64 rows: measured 1.4e-9 / 1.9e-9
+    assert eg < 1e-8 and ep < 1e-8
 
 
 def test_local_to_global(ctx):
@@ -168,3 +170,29 @@ def test_residual_and_norm(ctx):
     ctx.call("mgx_residual_norm", mgx.ptr(tr), ld, ld, n, 0.01, mgx.ptr(tI), ctypes.byref(out))
     d = 0.01 * np.sum(ref * ref, axis=1)
     assert out.value == pytest.approx(O.residual_norm(d, Sinv), rel=1e-13)
+
+
+def test_synth_separable_against_oracle_operator(ctx):
+    """The dense workload's device generator: psi_j = fx fy fz on z-planes
+    [z0, z0 + nzs) and H psi_j from the 1D second differences equal the
+    oracle's (-Delta_h4 + V) psi_j on the full periodic grid (1e-12)."""
+    from mgmol_amd.measure import _separable_factors
+    import ctypes as C
+    n, gd, h, z0, nzs = 3, 16, 0.3, 5, 6
+    f, lf, rng = _separable_factors(n, gd, h, seed=7)
+    v = rng.standard_normal((gd, gd, gd))
+    full = np.einsum("ji,jk,jl->jikl", f[0], f[1], f[2])  # (n, x, y, z)
+    g = O.Grid.make((gd, gd, gd), 2, (h, h, h))
+    HF = O.lap4_withpot(g, O.trade(g, O.embed(g, full)), v).reshape(n, gd, gd, gd)
+    dev = [to_dev(np.ascontiguousarray(a)) for a in f + lf]
+    K = gd * gd * nzs
+    tp, th = to_dev(np.zeros((n, K))), to_dev(np.zeros((n, K)))
+    gdim = (C.c_int * 3)(gd, gd, gd)
+    ctx.check(mgx.lib().mgx_synth_separable(ctx.h, n, gdim, z0, nzs, *[mgx.ptr(d) for d in dev], mgx.ptr(to_dev(v)),
+                                            mgx.ptr(tp), mgx.ptr(th)), "mgx_synth_separable")
+    ps = full[:, :, :, z0:z0 + nzs].reshape(n, K)
+    hs = HF[:, :, :, z0:z0 + nzs].reshape(n, K)
+    assert relmax(to_np(tp), ps) <= 1e-14
+    assert relmax(to_np(th), hs) <= 1e-12
+    # normalised over the full grid
+    assert np.allclose(h ** 3 * np.sum(full * full, axis=(1, 2, 3)), 1.0, rtol=1e-12)
