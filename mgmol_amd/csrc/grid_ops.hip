@@ -320,6 +320,23 @@ __global__ void k_mask(T* u, const uint8_t* m, GGeom g, int64_t total)
 }
 
 template <typename T>
+__global__ void k_copy_interior(const T* src, T* dst, GGeom g, int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.size;
+        int r           = (int)(t - f * g.size);
+        const int x     = r / (g.d1 * g.d2);
+        r -= x * g.d1 * g.d2;
+        const int y     = r / g.d2;
+        const int z     = r - y * g.d2;
+        const int64_t i = f * g.sizeg + (int64_t)(x + g.ng) * g.incx + (int64_t)(y + g.ng) * g.incy + z + g.ng;
+        dst[i]          = src[i];
+    }
+}
+
+template <typename T>
 __global__ void k_fill(T* u, int64_t n, T v)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
@@ -499,6 +516,262 @@ __global__ void k_rhsM_packed(const T* psi, int64_t ld, T* out, int64_t ldo, PGe
     }
 }
 
+// ------------------------------------------------------ marching sweeps --
+// The 4th- and 2nd-order stencils on whole grids, marching along x: a block
+// owns a tile of TY y-rows (all z) of one function over a range of x-planes
+// (an x-segment).  Each lane owns pairs of z-points (8 or 16 bytes) and keeps
+// them for the 2R + 1 planes x - R .. x + R in registers; the centre plane of
+// the tile, plus R halo rows on each y side and R z-ghosts on each row end,
+// goes through LDS (two buffers, one barrier per plane).  WRAP: neighbours
+// outside the grid are periodic images (bc = 1) or zero (bc = 0) and the
+// input's ghosts are never read (the sweep needs no halo exchange before it);
+// otherwise the input's ghost cells are read as stored.  Arithmetic per
+// point is that of lap4_at / lap2_at / k_hloc4_packed / k_jacobi_update, in
+// the same order.
+struct GMarch
+{
+    int d0, d1, d2;
+    int bc0, bc1, bc2;
+    int64_t ix, iy, i0, istr;  // input (and f): strides, offset of interior (0,0,0), function stride
+    int64_t ox, oy, o0, ostr;  // output (and w)
+    int ty, nty, nxs, xlen;    // tile rows, y tiles, x segments, planes per segment
+    int64_t nreal;             // blocks with work (the grid is padded to a multiple of 8)
+};
+
+constexpr int MG_LAP = 0, MG_JACOBI = 1, MG_HLOC = 2;
+
+template <typename T, int TYPE, int MODE, bool WRAP, int NT, int PM = 1, int PD = 1>
+__global__ void __launch_bounds__(NT) k_grid_march(const T* __restrict__ in, const T* __restrict__ f,
+    const double* __restrict__ vtot, T* __restrict__ out, T* __restrict__ wout, GMarch g, Coef4 c4, Coef2 c2,
+    double mjf)
+{
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    constexpr int R  = TYPE == MGX_LAP_4TH ? 2 : 1;  // stencil reach
+    constexpr int NQ = 2 * R + 1;
+    extern __shared__ unsigned char lds_raw[];
+    T* lds0 = reinterpret_cast<T*>(lds_raw);
+    const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    if (L >= g.nreal) return;
+    const int per_f = g.nty * g.nxs;
+    const int fn    = (int)(L / per_f);
+    const int rem   = (int)(L - (int64_t)fn * per_f);
+    const int seg = rem / g.nty, tyi = rem - (rem / g.nty) * g.nty;
+    const int y0 = tyi * g.ty, ny = min(g.ty, g.d1 - y0);
+    const int xs = seg * g.xlen, xe = min(xs + g.xlen, g.d0);
+    const int G   = g.d2 >> 1;
+    const int LW  = g.d2 + 4;            // LDS row: z = -2 .. d2 + 1
+    const int PLS = (g.ty + 2 * R) * LW;  // one plane buffer
+    const T* inf = in + (int64_t)fn * g.istr + g.i0;
+    const T* fff = f ? f + (int64_t)fn * g.istr + g.i0 : nullptr;
+    T* of        = out + (int64_t)fn * g.ostr + g.o0;
+    T* wf        = wout ? wout + (int64_t)fn * g.ostr + g.o0 : nullptr;
+    T2 zero2;
+    zero2[0] = (T)0;
+    zero2[1] = (T)0;
+    // x / y coordinates of a neighbour (wrapped), false when it is zero
+    auto cx = [&](int x, int& r) -> bool {
+        if (!WRAP || (x >= 0 && x < g.d0))
+        {
+            r = x;
+            return true;
+        }
+        if (!g.bc0) return false;
+        r = x < 0 ? x + g.d0 : x - g.d0;
+        return true;
+    };
+    auto cy = [&](int y, int& r) -> bool {
+        if (!WRAP || (y >= 0 && y < g.d1))
+        {
+            r = y;
+            return true;
+        }
+        if (!g.bc1) return false;
+        r = y < 0 ? y + g.d1 : y - g.d1;
+        return true;
+    };
+    auto ld2 = [&](int x, int y, int z) -> T2 {
+        int xx, yy;
+        if (!cx(x, xx) || !cy(y, yy)) return zero2;
+        return *reinterpret_cast<const T2*>(inf + (int64_t)xx * g.ix + (int64_t)yy * g.iy + z);
+    };
+    // the z-ghost pair of a row end (side 0: z = -2, -1; side 1: z = d2, d2 + 1)
+    auto ghost2 = [&](int x, int y, int side, const T2& own) -> T2 {
+        if (WRAP) return g.bc2 ? own : zero2;
+        int xx, yy;
+        cx(x, xx);
+        cy(y, yy);
+        const T* p = inf + (int64_t)xx * g.ix + (int64_t)yy * g.iy + (side ? g.d2 : -2);
+        T2 v;
+        v[0] = p[0];
+        v[1] = p[1];
+        return v;
+    };
+    int qy[PM], qz[PM];
+    bool act[PM];
+#pragma unroll
+    for (int k = 0; k < PM; k++)
+    {
+        const int item = threadIdx.x + k * NT;
+        act[k]         = item < ny * G;
+        qy[k]          = act[k] ? item / G : 0;
+        qz[k]          = act[k] ? 2 * (item - (item / G) * G) : 0;
+    }
+    // halo rows: 2R rows of G pairs, at most 2 per thread
+    const int nh = 2 * R * G;
+    int hy[2], hz[2];
+    bool hact[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+    {
+        const int item = threadIdx.x + k * NT;
+        hact[k]        = item < nh;
+        const int hr   = hact[k] ? item / G : 0;  // 0 .. 2R-1
+        hz[k]          = hact[k] ? 2 * (item - (item / G) * G) : 0;
+        hy[k]          = hr < R ? y0 - R + hr : y0 + ny + (hr - R);
+    }
+    // zero the ghost columns once (Dirichlet z under WRAP never writes them)
+    for (int t = threadIdx.x; t < 2 * PLS; t += NT)
+        lds0[t] = (T)0;
+    T2 q[PM][NQ];
+#pragma unroll
+    for (int k = 0; k < PM; k++)
+#pragma unroll
+        for (int j = 0; j < NQ; j++)
+            q[k][j] = act[k] ? ld2(xs - R + j, y0 + qy[k], qz[k]) : zero2;
+    // loads PD planes ahead: pf[i] = own pairs of plane x + R + 1 + i, hf[i]
+    // = halo pairs of plane x + 1 + i
+    T2 pf[PD][PM], hv[2], hf[PD][2];
+#pragma unroll
+    for (int i = 0; i < PD; i++)
+    {
+#pragma unroll
+        for (int k = 0; k < PM; k++)
+            pf[i][k] = (act[k] && xs + 1 + i < xe) ? ld2(xs + R + 1 + i, y0 + qy[k], qz[k]) : zero2;
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            hf[i][k] = (hact[k] && xs + 1 + i < xe) ? ld2(xs + 1 + i, hy[k], hz[k]) : zero2;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        hv[k] = hact[k] ? ld2(xs, hy[k], hz[k]) : zero2;
+    __syncthreads();
+    for (int x = xs; x < xe; x++)
+    {
+        T* pl = lds0 + ((x - xs) & 1) * PLS;
+        // the centre plane into LDS: own pairs, halo pairs, row-end ghosts
+#pragma unroll
+        for (int k = 0; k < PM; k++)
+        {
+            if (!act[k]) continue;
+            T* row = pl + (qy[k] + R) * LW + 2;
+            *reinterpret_cast<T2*>(row + qz[k]) = q[k][R];
+            if (qz[k] == 0) *reinterpret_cast<T2*>(row + g.d2) = ghost2(x, y0 + qy[k], 1, q[k][R]);
+            if (qz[k] == g.d2 - 2) *reinterpret_cast<T2*>(row - 2) = ghost2(x, y0 + qy[k], 0, q[k][R]);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+        {
+            if (!hact[k]) continue;
+            const int hr = hy[k] - (y0 - R);
+            T* row       = pl + hr * LW + 2;
+            *reinterpret_cast<T2*>(row + hz[k]) = hv[k];
+            if (hz[k] == 0) *reinterpret_cast<T2*>(row + g.d2) = ghost2(x, hy[k], 1, hv[k]);
+            if (hz[k] == g.d2 - 2) *reinterpret_cast<T2*>(row - 2) = ghost2(x, hy[k], 0, hv[k]);
+        }
+        __syncthreads();
+        // the planes the step after next-PD needs: own pairs at x + R + 1 +
+        // PD, halo pairs at x + 1 + PD
+        T2 nxt[PM], hnx[2];
+#pragma unroll
+        for (int k = 0; k < PM; k++)
+            nxt[k] = (act[k] && x + 1 + PD < xe) ? ld2(x + R + 1 + PD, y0 + qy[k], qz[k]) : zero2;
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            hnx[k] = (hact[k] && x + 1 + PD < xe) ? ld2(x + 1 + PD, hy[k], hz[k]) : zero2;
+#pragma unroll
+        for (int k = 0; k < PM; k++)
+        {
+            if (!act[k]) continue;
+            const int y = y0 + qy[k], z = qz[k];
+            const T* ctr = pl + (qy[k] + R) * LW + 2 + z;
+            const T2 cc  = q[k][R];
+            const T2 ym1 = *reinterpret_cast<const T2*>(ctr - LW);
+            const T2 yp1 = *reinterpret_cast<const T2*>(ctr + LW);
+            const T2 zl  = *reinterpret_cast<const T2*>(ctr - 2);
+            const T2 zr  = *reinterpret_cast<const T2*>(ctr + 2);
+            T2 ym2 = zero2, yp2 = zero2;
+            if (TYPE == MGX_LAP_4TH)
+            {
+                ym2 = *reinterpret_cast<const T2*>(ctr - 2 * LW);
+                yp2 = *reinterpret_cast<const T2*>(ctr + 2 * LW);
+            }
+            // z-line around the pair: zz[0..5] = z-2 .. z+3
+            const T zz[6] = { zl[0], zl[1], cc[0], cc[1], zr[0], zr[1] };
+            T2 fv = zero2;
+            if (MODE == MG_JACOBI) fv = *reinterpret_cast<const T2*>(fff + (int64_t)x * g.ix + (int64_t)y * g.iy + z);
+            typedef double D2 __attribute__((ext_vector_type(2)));
+            D2 vv = { 0., 0. };
+            if (MODE == MG_HLOC) vv = *reinterpret_cast<const D2*>(vtot + ((int64_t)x * g.d1 + y) * g.d2 + z);
+            T2 ov, wv2;
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+            {
+                double r;
+                if (TYPE == MGX_LAP_4TH)
+                {
+                    double diag = c4.c0;
+                    if (MODE == MG_HLOC) diag = c4.c0 + vv[i];
+                    r = diag * (double)cc[i] + c4.c1x * ((double)q[k][1][i] + (double)q[k][3][i])
+                        + c4.c1y * ((double)ym1[i] + (double)yp1[i])
+                        + c4.c1z * ((double)zz[i + 1] + (double)zz[i + 3])
+                        + c4.c2x * ((double)q[k][0][i] + (double)q[k][4 % NQ][i])
+                        + c4.c2y * ((double)ym2[i] + (double)yp2[i]) + c4.c2z * ((double)zz[i] + (double)zz[i + 4]);
+                }
+                else
+                {
+                    r = c2.c0 * cc[i] + c2.c1x * (T)(q[k][0][i] + q[k][2][i]) + c2.c1y * (T)(ym1[i] + yp1[i])
+                        + c2.c1z * (T)(zz[i + 1] + zz[i + 3]);
+                }
+                if (MODE == MG_JACOBI)
+                {
+                    T wv = (T)r;
+                    wv += static_cast<T>(-1. * static_cast<double>(fv[i]));
+                    wv2[i] = wv;
+                    T xv   = cc[i];
+                    xv += static_cast<T>(mjf * static_cast<double>(wv));
+                    ov[i] = xv;
+                }
+                else
+                    ov[i] = (T)r;
+            }
+            const int64_t o = (int64_t)x * g.ox + (int64_t)y * g.oy + z;
+            *reinterpret_cast<T2*>(of + o) = ov;
+            if (MODE == MG_JACOBI && wf) *reinterpret_cast<T2*>(wf + o) = wv2;
+        }
+#pragma unroll
+        for (int k = 0; k < PM; k++)
+        {
+#pragma unroll
+            for (int j = 0; j + 1 < NQ; j++)
+                q[k][j] = q[k][j + 1];
+            q[k][NQ - 1] = pf[0][k];
+#pragma unroll
+            for (int i = 0; i + 1 < PD; i++)
+                pf[i][k] = pf[i + 1][k];
+            pf[PD - 1][k] = nxt[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+        {
+            hv[k] = hf[0][k];
+#pragma unroll
+            for (int i = 0; i + 1 < PD; i++)
+                hf[i][k] = hf[i + 1][k];
+            hf[PD - 1][k] = hnx[k];
+        }
+    }
+}
+
 // --------------------------------------------------------------- drivers --
 int check_block(mgx_ctx* ctx, const mgx_block_t* b)
 {
@@ -531,9 +804,126 @@ int launch_halo(mgx_ctx* ctx, T* u, int nfunc, const mgx_grid_t& gr)
     return MGX_OK;
 }
 
+// geometry of a block for the marching sweeps: strides, offset of interior
+// point (0,0,0), function stride
+struct MView
+{
+    int64_t ix, iy, i0, str;
+};
+
+inline MView mview_ghosted(const mgx_grid_t& gr)
+{
+    const GGeom g = ggeom(gr);
+    return MView{ g.incx, g.incy, (int64_t)g.ng * (g.incx + g.incy + 1), g.sizeg };
+}
+
+inline MView mview_packed(const mgx_grid_t& gr, int64_t ld)
+{
+    return MView{ (int64_t)gr.dim[1] * gr.dim[2], gr.dim[2], 0, ld };
+}
+
+template <typename T, int TYPE, int MODE, bool WRAP>
+void march_grid_nt(mgx_ctx* ctx, int items, dim3 grid, size_t lds, const T* in, const T* f, const double* v, T* out,
+    T* w, const GMarch& g, const Coef4& c4, const Coef2& c2, double mjf)
+{
+    // 256 threads, each with up to four lane groups of the tile's plane
+    const int pm = (items + 255) / 256;
+    if (pm <= 1)
+        hipLaunchKernelGGL((k_grid_march<T, TYPE, MODE, WRAP, 256, 1>), grid, dim3(256), lds, ctx->stream, in, f, v,
+            out, w, g, c4, c2, mjf);
+    else if (pm == 2)
+        hipLaunchKernelGGL((k_grid_march<T, TYPE, MODE, WRAP, 256, 2>), grid, dim3(256), lds, ctx->stream, in, f, v,
+            out, w, g, c4, c2, mjf);
+    else if (pm == 3)
+        hipLaunchKernelGGL((k_grid_march<T, TYPE, MODE, WRAP, 256, 3>), grid, dim3(256), lds, ctx->stream, in, f, v,
+            out, w, g, c4, c2, mjf);
+    else
+        hipLaunchKernelGGL((k_grid_march<T, TYPE, MODE, WRAP, 256, 4>), grid, dim3(256), lds, ctx->stream, in, f, v,
+            out, w, g, c4, c2, mjf);
+}
+
+// The marching sweep (k_grid_march) when it applies: 4th- or 2nd-order
+// operator, even z extent up to 256, even strides and offsets, 2-element
+// aligned pointers; MG_LAP reads the input's ghosts (at least 2), MG_JACOBI
+// and MG_HLOC wrap by the boundary conditions.  Returns 1 if launched, 0 if
+// not applicable (the caller then uses the point kernels), or an error.
+template <typename T>
+int march_grid(mgx_ctx* ctx, int type, int mode, const T* in, const MView& vi, const T* f, const double* vtot, T* out,
+    const MView& vo, T* w, int nfunc, const mgx_grid_t& gr, double mjf)
+{
+    if (type != MGX_LAP_4TH && type != MGX_LAP_2ND) return 0;
+    if (mode == MG_HLOC && type != MGX_LAP_4TH) return 0;
+    const bool wrap = mode != MG_LAP;
+    const int d0 = gr.dim[0], d1 = gr.dim[1], d2 = gr.dim[2];
+    if (d2 % 2 || d2 > 256 || d0 < 3 || d1 < 2) return 0;
+    if (!wrap && gr.ghosts < 2) return 0;
+    for (int64_t v : { vi.ix, vi.iy, vi.i0, vi.str, vo.ix, vo.iy, vo.i0, vo.str })
+        if (v & 1) return 0;
+    const uintptr_t al = 2 * sizeof(T) - 1;
+    for (const void* p : { (const void*)in, (const void*)out, (const void*)f, (const void*)w })
+        if (reinterpret_cast<uintptr_t>(p) & al) return 0;
+    const int G = d2 / 2, R = type == MGX_LAP_4TH ? 2 : 1;
+    GMarch g;
+    g.d0 = d0;
+    g.d1 = d1;
+    g.d2 = d2;
+    g.bc0 = gr.bc[0] == 1;
+    g.bc1 = gr.bc[1] == 1;
+    g.bc2 = gr.bc[2] == 1;
+    g.ix = vi.ix;
+    g.iy = vi.iy;
+    g.i0 = vi.i0;
+    g.istr = vi.str;
+    g.ox = vo.ix;
+    g.oy = vo.iy;
+    g.o0 = vo.i0;
+    g.ostr = vo.str;
+    g.ty = std::min(std::min(16, 1024 / G), d1);
+    const size_t lds = 2 * (size_t)(g.ty + 2 * R) * (d2 + 4) * sizeof(T);
+    if (lds > 64 * 1024) return 0;
+    g.nty = (d1 + g.ty - 1) / g.ty;
+    // enough blocks to fill the chip: split x into segments (each re-reads
+    // 2R planes) when the functions and y tiles alone are too few
+    const int64_t base = (int64_t)nfunc * g.nty;
+    g.nxs  = (int)std::max<int64_t>(1, std::min<int64_t>((512 + base - 1) / base, std::max(1, d0 / 8)));
+    g.xlen = (d0 + g.nxs - 1) / g.nxs;
+    g.nxs  = (d0 + g.xlen - 1) / g.xlen;
+    g.nreal = base * g.nxs;
+    const int items = g.ty * G;
+    const dim3 grid((unsigned)((g.nreal + 7) / 8 * 8));
+    const Coef4 c4 = coef4(gr.h);
+    const Coef2 c2 = coef2(gr.h);
+    if (mode == MG_LAP)
+    {
+        if (type == MGX_LAP_4TH)
+            march_grid_nt<T, MGX_LAP_4TH, MG_LAP, false>(ctx, items, grid, lds, in, f, vtot, out, w, g, c4, c2, mjf);
+        else
+            march_grid_nt<T, MGX_LAP_2ND, MG_LAP, false>(ctx, items, grid, lds, in, f, vtot, out, w, g, c4, c2, mjf);
+    }
+    else if (mode == MG_JACOBI)
+    {
+        if (type == MGX_LAP_4TH)
+            march_grid_nt<T, MGX_LAP_4TH, MG_JACOBI, true>(ctx, items, grid, lds, in, f, vtot, out, w, g, c4, c2, mjf);
+        else
+            march_grid_nt<T, MGX_LAP_2ND, MG_JACOBI, true>(ctx, items, grid, lds, in, f, vtot, out, w, g, c4, c2, mjf);
+    }
+    else
+        march_grid_nt<T, MGX_LAP_4TH, MG_HLOC, true>(ctx, items, grid, lds, in, f, vtot, out, w, g, c4, c2, mjf);
+    MGX_CHECK_LAUNCH(ctx);
+    return 1;
+}
+
 template <typename T>
 int launch_lap(mgx_ctx* ctx, int type, const T* in, T* out, int nfunc, const mgx_grid_t& gr)
 {
+    if ((type == MGX_LAP_4TH && gr.ghosts >= 2) || (type == MGX_LAP_2ND && gr.ghosts >= 1))
+    {
+        const MView v = mview_ghosted(gr);
+        const int m   = march_grid(ctx, type, MG_LAP, in, v, (const T*)nullptr, nullptr, out, v, (T*)nullptr, nfunc,
+            gr, 0.);
+        if (m < 0) return m;
+        if (m) return MGX_OK;
+    }
     GGeom g = ggeom(gr);
     AllCoef c;
     c.c4 = coef4(gr.h);
@@ -570,6 +960,33 @@ int launch_axpy(mgx_ctx* ctx, int64_t n, double a, const T* x, T* y)
     return MGX_OK;
 }
 
+// one weighted-Jacobi sweep with the result in xo when the marching kernel
+// applies (*moved = true: no halo exchange of x is needed, and the ghost
+// cells of xo and w are not written), else in place on x by launch_jacobi
+template <typename T>
+int launch_jacobi(mgx_ctx* ctx, int type, double jf, const T* f, T* x, T* w, int nfunc, const mgx_grid_t& gr);
+
+template <typename T>
+int jacobi_sweep(mgx_ctx* ctx, int type, double jf, const T* f, T* x, T* xo, T* w, int nfunc, const mgx_grid_t& gr,
+    bool* moved, bool need_w = true)
+{
+    *moved = false;
+    if (xo)
+    {
+        // the marching sweep writes w = A x - f only where the caller reads it
+        const MView v = mview_ghosted(gr);
+        const int m   = march_grid(ctx, type, MG_JACOBI, (const T*)x, v, f, nullptr, xo, v, need_w ? w : (T*)nullptr,
+            nfunc, gr, -1. * jf);
+        if (m < 0) return m;
+        if (m)
+        {
+            *moved = true;
+            return MGX_OK;
+        }
+    }
+    return launch_jacobi(ctx, type, jf, f, x, w, nfunc, gr);
+}
+
 template <typename T>
 int launch_jacobi(mgx_ctx* ctx, int type, double jf, const T* f, T* x, T* w, int nfunc, const mgx_grid_t& gr)
 {
@@ -579,6 +996,26 @@ int launch_jacobi(mgx_ctx* ctx, int type, double jf, const T* f, T* x, T* w, int
     if (rc) return rc;
     const int64_t n = (int64_t)nfunc * ggeom(gr).sizeg;
     hipLaunchKernelGGL(k_jacobi_update<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, n, -1. * jf, f, w, x);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+// GridFuncVector::jacobi on one block (mgx_jacobi): the marching sweep into a
+// workspace and the interior copied back into x (the ghost cells of x and w
+// keep what they held: every consumer exchanges halos first), else the
+// halo + stencil + update kernels in place
+template <typename T>
+int jacobi_entry(mgx_ctx* ctx, int type, double jf, const T* f, T* x, T* w, int nfunc, const mgx_grid_t& gr)
+{
+    const GGeom g = ggeom(gr);
+    T* xo         = (T*)workspace(ctx, 1, (size_t)nfunc * g.sizeg * sizeof(T));
+    if (!xo) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    bool moved = false;
+    int rc     = jacobi_sweep(ctx, type, jf, f, x, xo, w, nfunc, gr, &moved);
+    if (rc || !moved) return rc;
+    const int64_t total = (int64_t)nfunc * g.size;
+    hipLaunchKernelGGL(k_copy_interior<T>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, (const T*)xo, x, g,
+        total);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
@@ -647,7 +1084,7 @@ template <typename T>
 struct MGBufs
 {
     std::vector<mgx_grid_t> grids;
-    std::vector<T*> w, rc, nv;
+    std::vector<T*> w, t, rc, nv;  // t: the other buffer of the out-of-place sweeps
 };
 
 template <typename T>
@@ -667,7 +1104,7 @@ int mg_setup(mgx_ctx* ctx, MGBufs<T>& B, const mgx_grid_t& g0, int max_levels, i
     for (int l = 0; l <= max_levels; l++)
     {
         sz[l] = (size_t)nfunc * ggeom(B.grids[l]).sizeg;
-        total += sz[l];
+        total += 2 * sz[l];
         if (l > 0) total += 2 * sz[l];
     }
     T* base = (T*)workspace(ctx, slot, total * sizeof(T));
@@ -676,6 +1113,8 @@ int mg_setup(mgx_ctx* ctx, MGBufs<T>& B, const mgx_grid_t& g0, int max_levels, i
     for (int l = 0; l <= max_levels; l++)
     {
         B.w.push_back(p);
+        p += sz[l];
+        B.t.push_back(p);
         p += sz[l];
     }
     for (int l = 1; l <= max_levels; l++)
@@ -698,12 +1137,28 @@ int mg_rec(mgx_ctx* ctx, MGBufs<T>& B, int max_levels, T* v, const T* f, int typ
     const double jf     = jacobi_factor(type, g.h);
     T* w                = B.w[level];
     const uint8_t* mk   = masks ? masks[level] : nullptr;
+    const size_t bytes  = (size_t)nfunc * ggeom(g).sizeg * sizeof(T);
     int rc;
+    // the sweeps alternate between v and t when they run out of place
+    T* cur     = v;
+    // w is read after the last pre-smoothing sweep only (the stale residual
+    // the coarse correction restricts)
+    auto sweep = [&](bool need_w) -> int {
+        bool moved = false;
+        T* other   = cur == v ? B.t[level] : v;
+        int r      = jacobi_sweep(ctx, type, jf, f, cur, other, w, nfunc, g, &moved, need_w);
+        if (r) return r;
+        if (moved) cur = other;
+        return launch_mask(ctx, cur, mk, nfunc, g);
+    };
+    auto settle = [&]() -> int {
+        if (cur != v) MGX_CHECK_HIP(ctx, hipMemcpyAsync(v, cur, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        cur = v;
+        return MGX_OK;
+    };
     for (int it = 0; it < ncycl; it++)
-    {
-        if ((rc = launch_jacobi(ctx, type, jf, f, v, w, nfunc, g))) return rc;
-        if ((rc = launch_mask(ctx, v, mk, nfunc, g))) return rc;
-    }
+        if ((rc = sweep(it == ncycl - 1 && level < max_levels))) return rc;
+    if ((rc = settle())) return rc;
     if (level == max_levels) return MGX_OK;
     if ((rc = launch_mask(ctx, w, mk, nfunc, g))) return rc;
     const mgx_grid_t& gc = B.grids[level + 1];
@@ -719,10 +1174,8 @@ int mg_rec(mgx_ctx* ctx, MGBufs<T>& B, int max_levels, T* v, const T* f, int typ
     if ((rc = launch_mask(ctx, w, mk, nfunc, g))) return rc;
     if ((rc = launch_axpy(ctx, (int64_t)nfunc * ggeom(g).sizeg, -1., (const T*)w, v))) return rc;
     for (int it = 0; it < 2; it++)
-    {
-        if ((rc = launch_jacobi(ctx, type, jf, f, v, w, nfunc, g))) return rc;
-        if ((rc = launch_mask(ctx, v, mk, nfunc, g))) return rc;
-    }
+        if ((rc = sweep(false))) return rc;
+    if ((rc = settle())) return rc;
     if (g.bc[0] != 1 || g.bc[1] != 1 || g.bc[2] != 1) return launch_halo(ctx, v, nfunc, g);
     return MGX_OK;
 }
@@ -818,7 +1271,7 @@ struct PoissonMG
 {
     std::vector<mgx_grid_t> grids;
     std::vector<int> types;
-    std::vector<double*> res, x, f;
+    std::vector<double*> res, x, f, t;  // t: the other buffer of the out-of-place sweeps
     double* part = nullptr;  // kPartBlocks partials + 8 results
 };
 
@@ -857,8 +1310,24 @@ int poisson_vcycle(mgx_ctx* ctx, PoissonMG& P, int l, double* x, const double* f
     double* res         = P.res[l];
     const int64_t n     = ggeom(g).sizeg;
     int rc;
-    for (int i = 0; i < nu1; i++)
-        if ((rc = launch_jacobi(ctx, t, jf, f, x, res, 1, g))) return rc;
+    // the sweeps alternate between x and P.t[l] when they run out of place
+    double* cur = x;
+    auto sweeps = [&](int k) -> int {
+        for (int i = 0; i < k; i++)
+        {
+            bool moved    = false;
+            double* other = cur == x ? P.t[l] : x;
+            // the fresh residual is formed after the sweeps: w is not read
+            int r         = jacobi_sweep(ctx, t, jf, f, cur, other, res, 1, g, &moved, false);
+            if (r) return r;
+            if (moved) cur = other;
+        }
+        if (cur != x)
+            MGX_CHECK_HIP(ctx, hipMemcpyAsync(x, cur, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        cur = x;
+        return MGX_OK;
+    };
+    if ((rc = sweeps(nu1))) return rc;
     if (l + 1 < (int)P.grids.size())
     {
         const mgx_grid_t& gc = P.grids[l + 1];
@@ -875,9 +1344,7 @@ int poisson_vcycle(mgx_ctx* ctx, PoissonMG& P, int l, double* x, const double* f
         if ((rc = launch_extend(ctx, (const double*)P.x[l + 1], gc, res, g, 1))) return rc;
         if ((rc = launch_axpy(ctx, n, -1., (const double*)res, x))) return rc;
     }
-    for (int i = 0; i < nu2; i++)
-        if ((rc = launch_jacobi(ctx, t, jf, f, x, res, 1, g))) return rc;
-    return MGX_OK;
+    return sweeps(nu2);
 }
 
 // pb::Mgm, src/pb/Mgm.h:12-111, as SolverLap::solve calls it
@@ -907,7 +1374,7 @@ int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho,
     const int64_t n0 = ggeom(g0).sizeg;
     size_t total = 4 * (size_t)n0 + kPartBlocks + 8;  // rhs, lhs, r0 (then res0), w
     for (int l = 0; l < nl; l++)
-        total += (size_t)ggeom(P.grids[l]).sizeg * (l == 0 ? 1 : 3);
+        total += (size_t)ggeom(P.grids[l]).sizeg * (l == 0 ? 2 : 4);
     double* base = (double*)workspace(ctx, 0, total * sizeof(double));
     if (!base) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     double* rhs = base;
@@ -921,6 +1388,8 @@ int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho,
     {
         const int64_t m = ggeom(P.grids[l]).sizeg;
         P.res.push_back(q);
+        q += m;
+        P.t.push_back(q);
         q += m;
         if (l == 0)
         {
@@ -1203,9 +1672,9 @@ int mgx_jacobi(mgx_ctx* ctx, int type, double jf, const mgx_block_t* f, mgx_bloc
     if (!rc) rc = check_block(ctx, w);
     if (rc) return rc;
     MGX_REQUIRE(ctx, same_shape(f, x) && same_shape(x, w), "jacobi blocks differ in shape");
-    rc = x->dtype == MGX_F32 ? launch_jacobi(ctx, type, jf, (const float*)f->data, (float*)x->data,
-                                   (float*)w->data, x->nfunc, x->grid)
-                             : launch_jacobi(ctx, type, jf, (const double*)f->data, (double*)x->data,
+    rc = x->dtype == MGX_F32 ? jacobi_entry(ctx, type, jf, (const float*)f->data, (float*)x->data, (float*)w->data,
+                                   x->nfunc, x->grid)
+                             : jacobi_entry(ctx, type, jf, (const double*)f->data, (double*)x->data,
                                    (double*)w->data, x->nfunc, x->grid);
     return rc ? rc : sync(ctx);
 }
@@ -1299,6 +1768,15 @@ int mgx_hloc_apply(mgx_ctx* ctx, int type, const mgx_grid_t* grid, int dtype, co
     dim3 grd(nblocks(total)), blk(kBlock);
     if (type == MGX_LAP_4TH)
     {
+        // the marching kernel (packed rows, halo by wrapping) when it applies
+        const MView vi = mview_packed(*grid, ld), vo = mview_packed(*grid, ldo);
+        const int m    = dtype == MGX_F32
+                             ? march_grid(ctx, MGX_LAP_4TH, MG_HLOC, (const float*)psi, vi, (const float*)nullptr, vtot,
+                                   (float*)hpsi, vo, (float*)nullptr, nfunc, *grid, 0.)
+                             : march_grid(ctx, MGX_LAP_4TH, MG_HLOC, (const double*)psi, vi, (const double*)nullptr,
+                                   vtot, (double*)hpsi, vo, (double*)nullptr, nfunc, *grid, 0.);
+        if (m < 0) return m;
+        if (m) return sync(ctx);
         Coef4 c = coef4(grid->h);
         if (dtype == MGX_F32)
             hipLaunchKernelGGL(k_hloc4_packed<float>, grd, blk, 0, ctx->stream, (const float*)psi, ld, vtot,

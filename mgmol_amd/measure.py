@@ -1,6 +1,7 @@
 """Measurements of the other workloads of the measurement plan, one JSON line
 each (the converged C5 solve is bench.py's):
 
+    python -m mgmol_amd.measure c1     # full-grid orbitals through the drop-in entry points
     python -m mgmol_amd.measure c2     # batched uneven boxes: a converged solve
     python -m mgmol_amd.measure c3     # tall-skinny S / H contractions, FP64 and split
     python -m mgmol_amd.measure c4     # the CNN: inference and training throughput
@@ -40,6 +41,82 @@ def _timeit(fn, reps):
     for _ in range(reps):
         fn()
     return (time.perf_counter() - t0) / reps
+
+
+def _cpu_info():
+    import os
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"cpu_model": model, "cpus_available": avail,
+            "omp_threads": int(os.environ.get("OMP_NUM_THREADS") or 0) or avail}
+
+
+def c1(ctx, cpu=True):
+    """The plain FP64 method on full-grid orbitals through the drop-in entry
+    points: 64 orbitals on a 96^3 periodic grid (h = 0.3, 16 wells), H psi
+    (mgx_hloc_apply, 4th order), the FP64 3-level V-cycle (mgx_precond_mg:
+    2/2 sweeps, 4 at the coarsest, omega = 2/3 / 6/7), S (mgx_local_gram);
+    algorithmic HBM bytes per call (H psi: psi read, H psi written, V once;
+    V-cycle: 18.5 x 8 B per orbital point for the fused minimum schedule)
+    and the oracle's CPU time for the same calls on this host."""
+    kw = dict(synth.CONFIGS["c1"])
+    n, gd, h, nwells = kw["n"], kw["gdim"], kw["h"], kw["nwells"]
+    psi, v = synth.grid_orbitals(n, gd, h, seed=1, nwells=nwells)
+    K = gd ** 3
+    grid = mgx.Grid.make((gd, gd, gd), 2, (h, h, h))
+    dp, dv = mgx.DeviceArray(ctx, psi), mgx.DeviceArray(ctx, v)
+    dh = mgx.DeviceArray(ctx, shape=(n, K), dtype=np.float64)
+    dr = mgx.DeviceArray(ctx, shape=(n, K), dtype=np.float64)
+    dS = mgx.DeviceArray(ctx, shape=(n, n), dtype=np.float64)
+    L = mgx.lib()
+    vel = h ** 3
+    gam = mgx.lib().mgx_gamma((C.c_double * 3)(h, h, h), 2, float(np.max(v)), -1.0)
+    t_h = _timeit(lambda: ctx.check(L.mgx_hloc_apply(ctx.h, mgx.LAP_4TH, C.byref(grid), mgx.F64, mgx.ptr(dp), K, n,
+                                                     mgx.ptr(dv), mgx.ptr(dh), K), "hloc"), 10)
+    t_p = _timeit(lambda: ctx.check(L.mgx_precond_mg(ctx.h, mgx.LAP_4TH, 2, gam, mgx.F64, C.byref(grid), mgx.ptr(dh),
+                                                     K, n, None, mgx.ptr(dr), K), "precond"), 5)
+    t_s = _timeit(lambda: ctx.check(L.mgx_local_gram(ctx.h, mgx.PREC_F64, mgx.ptr(dp), K, K, n, 1, vel,
+                                                     mgx.ptr(dS)), "gram"), 10)
+    bh = 16.0 * n * K + 8.0 * K
+    bp = 18.5 * 8.0 * n * K
+    fs = K * n * (n + 1.0)
+    out = {"config": "c1", "orbitals": n, "grid": gd, "points": K,
+           "hloc_ms": round(1e3 * t_h, 3), "hloc_GBps": round(bh / t_h / 1e9, 1),
+           "hloc_frac_of_8TBps": round(bh / t_h / 8e12, 4),
+           "vcycle_f64_ms": round(1e3 * t_p, 3), "vcycle_GBps_algorithmic": round(bp / t_p / 1e9, 1),
+           "vcycle_frac_of_8TBps": round(bp / t_p / 8e12, 4),
+           "gram_ms": round(1e3 * t_s, 3), "gram_tflops": round(fs / t_s / 1e12, 2)}
+    if cpu:
+        import oracle as O
+        og = O.Grid.make((gd, gd, gd), 2, (h, h, h))
+        t0 = time.perf_counter()
+        hp = O.hloc(og, 2, O.embed(og, psi.reshape((n, gd, gd, gd))), v)
+        c_h = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.precond(og, 2, 2, gam, hp, dtype=np.float64)
+        c_p = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.local_gram(psi, K, 1, vel)
+        c_s = time.perf_counter() - t0
+        out["cpu_oracle"] = dict(_cpu_info(), hloc_ms=round(1e3 * c_h, 1), vcycle_f64_ms=round(1e3 * c_p, 1),
+                                 gram_ms=round(1e3 * c_s, 1),
+                                 note="oracle (C++/OpenMP restatement of the reference), one call each")
+        hg = dh.get()
+        out["hloc_max_rel_err_vs_oracle"] = float(np.max(np.abs(hg - hp)) / np.max(np.abs(hp)))
+    for a in (dp, dv, dh, dr, dS):
+        a.free()
+    return out
 
 
 def c2(ctx):
@@ -264,8 +341,9 @@ def poisson(ctx):
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     ctx = mgx.Ctx(0)
-    for name in argv or ["c2", "c3", "c4", "density", "poisson"]:
-        res = {"c2": c2, "c3": c3, "c4": c4, "density": density, "poisson": poisson}[name](ctx)
+    for name in argv or ["c1", "c2", "c3", "c4", "density", "poisson"]:
+        res = {"c1": c1, "c1_gpu": lambda c: c1(c, cpu=False), "c2": c2, "c3": c3, "c4": c4, "density": density,
+               "poisson": poisson}[name](ctx)
         print(json.dumps(res), flush=True)
     ctx.close()
 

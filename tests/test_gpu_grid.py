@@ -28,6 +28,39 @@ def _grids(dims=(16, 12, 20), ng=2, ll=2.4, bc=(1, 1, 1)):
     return O.Grid.make(dims, ng, h), mgx.Grid.make(dims, ng, h, bc)
 
 
+# the marching sweeps (periodic and zero boundaries wrapped in the kernel,
+# one or more x-segments and y-tiles, several lane groups per thread) and the
+# point kernels they fall back to (odd z extent)
+MARCH_CASES = [((16, 12, 20), (1, 1, 1)), ((16, 12, 20), (1, 0, 1)), ((24, 20, 96), (0, 1, 0)),
+               ((16, 12, 15), (1, 1, 1)), ((40, 36, 8), (1, 1, 1))]
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("dims,bc", MARCH_CASES)
+def test_jacobi_and_lap_march_cases(ctx, dt, kind, dims, bc):
+    og, mg_ = _grids(dims=dims, bc=bc)
+    rng = np.random.default_rng(31)
+    nf = 3
+    f = O.embed(og, rng.standard_normal((nf,) + tuple(og.dim)).astype(DT[dt]))
+    x = O.embed(og, rng.standard_normal((nf,) + tuple(og.dim)).astype(DT[dt]))
+    w = np.zeros_like(x)
+    jf = O.jacobi_factor(kind, list(og.h))
+    tf, tx, tw = to_dev(f), to_dev(x), to_dev(w)
+    ctx.call("mgx_jacobi", kind, jf, mgx.block(tf, mg_), mgx.block(tx, mg_), mgx.block(tw, mg_))
+    xr, wr = x.copy(), w.copy()
+    O.jacobi(og, kind, f, xr, wr, jf, bc=bc)
+    ex = relmax(O.interior(og, to_np(tx)), O.interior(og, xr))
+    ew = relmax(O.interior(og, to_np(tw)), O.interior(og, wr))
+    # the stencil on traded ghosts (Dirichlet ghosts zero)
+    blk = O.trade(og, x.copy(), bc=bc)
+    ti, to = to_dev(blk), to_dev(np.zeros_like(blk))
+    ctx.call("mgx_lap_apply", kind, mgx.block(ti, mg_), mgx.block(to, mg_))
+    el = relmax(O.interior(og, to_np(to)), O.interior(og, O.lap(og, kind, blk)))
+    say("march case", dt, kind, dims, bc, ex, ew, el)
+    assert ex <= TOL[dt] and ew <= TOL[dt] and el <= TOL[dt]
+
+
 @pytest.mark.parametrize("dt", ["f64", "f32"])
 @pytest.mark.parametrize("bc", [(1, 1, 1), (1, 0, 1)])
 def test_halo_bit_exact(ctx, dt, bc):
