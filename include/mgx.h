@@ -318,6 +318,22 @@ int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, cons
 /* d_j = vel*|r_j|^2 (device n); norm = 0.5*sqrt(sum d_j Sinv_jj) host out */
 int mgx_box_residual_norm(mgx_ctx* ctx, const mgx_boxset* bs, const double* res,
     const double* Sinv, double* norm);
+/* Mehrstellen (FDtype = Mehrstellen, src/Control.cc:1353) on boxes:
+ * Hamiltonian<T>::applyLocal's Mehrstellen branch, src/Hamiltonian.cc:106-131:
+ * hpsi = -Lap_M psi + B (V psi) on each box (psi zero outside, V gathered), FP64 */
+int mgx_box_hloc_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* vtot,
+    double* hpsi);
+/* LocGridOrbitals::computeGram / computeLocalProduct / computeMatB,
+ * src/LocGridOrbitals.cc:1428-1604, with Mehrstellen operators: S_ij = vel <psi_i, psi_j>,
+ * H_ij = vel <psi_i, H psi_j>, B_ij = vel <psi_i, B psi_j> (column-major, entry (i, j));
+ * H psi_j and B psi_j include their one-point reach outside box j (faces, and the
+ * edges of -Lap_M), so H is the Galerkin matrix; it is not symmetric (B V is not) */
+int mgx_box_gram_hb_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
+    const double* vtot, double* S, double* H, double* B);
+/* MGmol::computeResidualUsingHPhi with applyB, src/MGmol.cc:1227-1291, on boxes:
+ * res_j = sum_a (B psi_a) theta(a, j) - hpsi_j on box j, B psi_a with its reach */
+int mgx_box_residual_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
+    const double* theta, double* res);
 /* Electron density rho(r) = sum_ij dm_ij psi_i(r) psi_j(r) on the level-0 periodic
  * grid (device, gdim[0]*gdim[1]*gdim[2] FP64, x slowest, z fastest; overwritten);
  * dm: n x n column-major, n = number of boxes.  *charge (host, may be NULL) =

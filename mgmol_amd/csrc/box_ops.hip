@@ -1132,6 +1132,273 @@ __global__ void __launch_bounds__(256) k_pair_gram(const double* psi, const doub
     }
 }
 
+// ------------------------------------------------ Mehrstellen on boxes ----
+// Hamiltonian<T>::applyLocal, Mehrstellen branch (src/Hamiltonian.cc:106-131),
+// on a box (psi zero outside): w1 = psi V, work1 = B w1, w1 = -Lap_M psi,
+// work1 += w1; the arithmetic of k_hlocM_packed with the box's zero ring in
+// place of the periodic wrap.  V is gathered from the periodic grid.
+__global__ void __launch_bounds__(256) k_box_hlocM(const double* in, const double* vtot, double* out, BoxView v,
+    CoefM cm)
+{
+    int j, e, x, y, z, p;
+    int64_t base;
+    if (!tile_point(v, &j, &e, &x, &y, &z, &base, &p)) return;
+    const double* a = in + base;
+    const int ox = v.origin[3 * j], oy = v.origin[3 * j + 1], oz = v.origin[3 * j + 2];
+    auto vpsi = [&](int dx, int dy, int dz) -> double {
+        const int xx = x + dx, yy = y + dy, zz = z + dz;
+        if ((unsigned)xx >= (unsigned)e || (unsigned)yy >= (unsigned)e || (unsigned)zz >= (unsigned)e) return 0.;
+        int gx = ox + xx, gy = oy + yy, gz = oz + zz;
+        if (gx >= v.g0) gx -= v.g0;
+        if (gy >= v.g1) gy -= v.g1;
+        if (gz >= v.g2) gz -= v.g2;
+        return a[((int64_t)xx * e + yy) * e + zz] * vtot[((int64_t)gx * v.g1 + gy) * v.g2 + gz];
+    };
+    const double bw = 0.5 * vpsi(0, 0, 0)
+                      + kInv12 * (vpsi(-1, 0, 0) + vpsi(1, 0, 0) + vpsi(0, -1, 0) + vpsi(0, 1, 0) + vpsi(0, 0, -1)
+                                     + vpsi(0, 0, 1));
+    Coef4 c4{};
+    Coef2 c2{};
+    const double lm = box_stencil<double, MGX_LAP_MEHRSTELLEN>(a, e, x, y, z, c4, c2, cm, 0.);
+    double o = bw;
+    o += 1. * lm;
+    out[base + p] = o;
+}
+
+// Lap::rhs (FDkernelRHS_4th_Mehr1, src/pb/FDkernels.cc:523-584) on a box, zero
+// outside: the arithmetic of k_rhsM_packed
+__global__ void __launch_bounds__(256) k_box_bpsiM(const double* in, double* out, BoxView v)
+{
+    int j, e, x, y, z, p;
+    int64_t base;
+    if (!tile_point(v, &j, &e, &x, &y, &z, &base, &p)) return;
+    const double* a = in + base;
+#define A(dx, dy, dz) bv(a, e, x + (dx), y + (dy), z + (dz))
+    out[base + p] = 0.5 * A(0, 0, 0)
+                    + kInv12 * (A(-1, 0, 0) + A(1, 0, 0) + A(0, -1, 0) + A(0, 1, 0) + A(0, 0, -1) + A(0, 0, 1));
+#undef A
+}
+
+// The reach of the Mehrstellen H psi_B and B psi_B at X (B-local), one point
+// outside box B in one axis (a face: axis coefficient, the edge neighbours on
+// the face, (1/12) V psi of B(V psi)) or two axes (an edge: the edge
+// coefficient); q is the box point next to X.
+struct MehrReach
+{
+    double h, b;
+};
+
+__device__ __forceinline__ double coef_axis(const CoefM& c, int d) { return d == 0 ? c.cx : (d == 1 ? c.cy : c.cz); }
+
+__device__ __forceinline__ double coef_edge(const CoefM& c, int d, int u)
+{
+    const int m = d + u;  // 1: xy, 3: yz, 2: xz
+    return m == 1 ? c.cxy : (m == 3 ? c.cyz : c.cxz);
+}
+
+// sum over the points x of box A outside box B within its one-point reach of
+// psi_A(x) (H psi_B)(x) and psi_A(x) (B psi_B)(x); sh maps A-local -> B-local;
+// block-stride loops over the faces and edges of B
+__device__ void mehr_skin(const double* pA, int eA, const double* pB, int eB, const int* oB, const int sh[3],
+    const double* vtot, int g0, int g1, int g2, const CoefM& cm, double* hs, double* bs)
+{
+    // A's extent in B-local coordinates: [sh, sh + eA)
+    int lo[3], hi[3];
+    for (int k = 0; k < 3; k++)
+    {
+        lo[k] = sh[k];
+        hi[k] = sh[k] + eA;
+    }
+    auto pa = [&](int X0, int X1, int X2) -> double {
+        return pA[((int64_t)(X0 - sh[0]) * eA + (X1 - sh[1])) * eA + (X2 - sh[2])];
+    };
+    auto pb = [&](int q0, int q1, int q2) -> double { return pB[((int64_t)q0 * eB + q1) * eB + q2]; };
+    auto inB = [&](int q) -> bool { return q >= 0 && q < eB; };
+    const int gd[3] = { g0, g1, g2 };
+    auto vB = [&](const int q[3]) -> double {
+        int g[3];
+        for (int k = 0; k < 3; k++)
+        {
+            g[k] = oB[k] + q[k];
+            if (g[k] >= gd[k]) g[k] -= gd[k];
+        }
+        return vtot[((int64_t)g[0] * g1 + g[1]) * g2 + g[2]];
+    };
+    double h = 0., b = 0.;
+    // faces: X_d = -1 or eB, the other two coordinates inside B
+    for (int f = 0; f < 6; f++)
+    {
+        const int d = f >> 1, s = (f & 1) ? 1 : -1;
+        const int Xd = s < 0 ? -1 : eB;
+        if (Xd < lo[d] || Xd >= hi[d]) continue;
+        const int u = (d + 1) % 3, w = (d + 2) % 3;
+        const int u0 = max(0, lo[u]), u1 = min(eB, hi[u]), w0 = max(0, lo[w]), w1 = min(eB, hi[w]);
+        const int nu = u1 - u0, nw = w1 - w0;
+        if (nu <= 0 || nw <= 0) continue;
+        for (int t = threadIdx.x; t < nu * nw; t += blockDim.x)
+        {
+            int X[3];
+            X[d] = Xd;
+            X[u] = u0 + t / nw;
+            X[w] = w0 + (t - (t / nw) * nw);
+            int q[3] = { X[0], X[1], X[2] };
+            q[d]             = Xd - s;
+            const double psq = pb(q[0], q[1], q[2]);
+            double hv        = coef_axis(cm, d) * psq;
+            for (int e2 : { u, w })
+                for (int tt = -1; tt <= 1; tt += 2)
+                {
+                    int r[3] = { q[0], q[1], q[2] };
+                    r[e2] += tt;
+                    if (inB(r[e2])) hv += coef_edge(cm, d, e2) * pb(r[0], r[1], r[2]);
+                }
+            hv += kInv12 * (vB(q) * psq);
+            const double av = pa(X[0], X[1], X[2]);
+            h += av * hv;
+            b += av * (kInv12 * psq);
+        }
+    }
+    // edges: X_d, X_u outside by one, X_w inside
+    for (int d = 0; d < 3; d++)
+        for (int u = d + 1; u < 3; u++)
+        {
+            const int w = 3 - d - u;
+            const int w0 = max(0, lo[w]), w1 = min(eB, hi[w]);
+            if (w1 <= w0) continue;
+            for (int c = 0; c < 4; c++)
+            {
+                const int sd = (c & 1) ? 1 : -1, su = (c & 2) ? 1 : -1;
+                const int Xd = sd < 0 ? -1 : eB, Xu = su < 0 ? -1 : eB;
+                if (Xd < lo[d] || Xd >= hi[d] || Xu < lo[u] || Xu >= hi[u]) continue;
+                for (int t = threadIdx.x; t < w1 - w0; t += blockDim.x)
+                {
+                    int X[3];
+                    X[d] = Xd;
+                    X[u] = Xu;
+                    X[w] = w0 + t;
+                    int q[3] = { X[0], X[1], X[2] };
+                    q[d] -= sd;
+                    q[u] -= su;
+                    h += pa(X[0], X[1], X[2]) * (coef_edge(cm, d, u) * pb(q[0], q[1], q[2]));
+                }
+            }
+        }
+    *hs += h;
+    *bs += b;
+}
+
+// S, H, B-matrix partials of one interacting pair (i <= j), one block each:
+// the overlap from the stored box values, plus each side's reach outside the
+// other box; H and B in both orders (H is not symmetric).  part: 5 per pair
+// (s, h_ij, h_ji, b_ij, b_ji).
+__global__ void __launch_bounds__(256) k_pair_gram_mehr(const double* psi, const double* hpsi, const double* bpsi,
+    const double* vtot, const int64_t* off, const int* edge, const int* origin, const int* pairs, int64_t npairs,
+    int g0, int g1, int g2, CoefM cm, double* out)
+{
+    __shared__ double lds[5 * 4];
+    const int64_t pid = blockIdx.x;
+    if (pid >= npairs) return;
+    const int* P = pairs + 11 * pid;
+    const int i = P[0], j = P[1];
+    const int a0 = P[2], a1 = P[3], a2 = P[4], b0 = P[5], b1 = P[6], b2 = P[7];
+    const int sh[3] = { P[8], P[9], P[10] };
+    const int ei = edge[i], ej = edge[j];
+    const int n1 = max(b1 - a1, 0), n2 = max(b2 - a2, 0);
+    const int64_t npt = (int64_t)max(b0 - a0, 0) * n1 * n2;
+    const double *pi = psi + off[i], *pj = psi + off[j];
+    const double *hi = hpsi + off[i], *hj = hpsi + off[j];
+    const double *bi = bpsi + off[i], *bj = bpsi + off[j];
+    double acc[5] = { 0., 0., 0., 0., 0. };
+    for (int64_t q = threadIdx.x; q < npt; q += blockDim.x)
+    {
+        int r       = (int)q;
+        const int x = r / (n1 * n2);
+        r -= x * n1 * n2;
+        const int y = r / n2;
+        const int z = r - y * n2;
+        const int X = a0 + x, Y = a1 + y, Z = a2 + z;
+        const int64_t ii = ((int64_t)X * ei + Y) * ei + Z;
+        const int64_t jj = ((int64_t)(X + sh[0]) * ej + (Y + sh[1])) * ej + (Z + sh[2]);
+        const double vi = pi[ii], vj = pj[jj];
+        acc[0] += vi * vj;
+        acc[1] += vi * hj[jj];
+        acc[2] += hi[ii] * vj;
+        acc[3] += vi * bj[jj];
+        acc[4] += bi[ii] * vj;
+    }
+    if (i != j)
+    {
+        const int shji[3] = { -sh[0], -sh[1], -sh[2] };
+        mehr_skin(pi, ei, pj, ej, origin + 3 * j, sh, vtot, g0, g1, g2, cm, &acc[1], &acc[3]);
+        mehr_skin(pj, ej, pi, ei, origin + 3 * i, shji, vtot, g0, g1, g2, cm, &acc[2], &acc[4]);
+    }
+    block_sum<5>(acc, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; k++)
+            out[5 * pid + k] = acc[k];
+}
+
+// dense S (symmetric), H and B-matrix (column-major, entry (i, j) =
+// vel <psi_i, op psi_j>) from the pair partials
+__global__ void k_pair_scatter_mehr(const double* part, const int* pairs, int64_t npairs, int n, double vel,
+    double* S, double* H, double* B)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int i = pairs[11 * t], j = pairs[11 * t + 1];
+        const double* q = part + 5 * t;
+        S[(int64_t)j * n + i] = q[0] * vel;
+        S[(int64_t)i * n + j] = q[0] * vel;
+        H[(int64_t)j * n + i] = q[1] * vel;
+        H[(int64_t)i * n + j] = q[2] * vel;
+        B[(int64_t)j * n + i] = q[3] * vel;
+        B[(int64_t)i * n + j] = q[4] * vel;
+    }
+}
+
+// R_b += (1/12) theta(a, b) psi_a(q) at the points of box b one point outside
+// a face of box a (the reach of B psi_a), one block per box b over its
+// interaction list (entry (a, sh): sh maps b-local -> a-local), one entry at
+// a time (entries may touch the same points)
+__global__ void __launch_bounds__(256) k_resid_skin_mehr(const double* psi, const double* theta, int n,
+    const int64_t* off, const int* edge, const int* ilist_ptr, const int* ilist, double* res)
+{
+    const int b  = blockIdx.x;
+    const int eb = edge[b];
+    double* rb   = res + off[b];
+    for (int qq = ilist_ptr[b]; qq < ilist_ptr[b + 1]; qq++)
+    {
+        const int* E = ilist + 4 * qq;
+        const int a = E[0], ea = edge[a];
+        const int sh[3] = { E[1], E[2], E[3] };
+        const double t  = kInv12 * theta[(int64_t)b * n + a];
+        const double* pa = psi + off[a];
+        for (int f = 0; f < 6; f++)
+        {
+            const int d = f >> 1, s = (f & 1) ? 1 : -1;
+            const int u = (d + 1) % 3, w = (d + 2) % 3;
+            // the shell plane of a in b-local coordinates: Xa_d - sh_d
+            const int pd = (s < 0 ? -1 : ea) - sh[d];
+            if (pd < 0 || pd >= eb) continue;
+            const int u0 = max(0, -sh[u]), u1 = min(eb, ea - sh[u]);
+            const int w0 = max(0, -sh[w]), w1 = min(eb, ea - sh[w]);
+            const int nu = u1 - u0, nw = w1 - w0;
+            if (nu <= 0 || nw <= 0) continue;
+            for (int k = threadIdx.x; k < nu * nw; k += blockDim.x)
+            {
+                int p[3];
+                p[d] = pd;
+                p[u] = u0 + k / nw;
+                p[w] = w0 + (k - (k / nw) * nw);
+                int q[3] = { p[0] + sh[0], p[1] + sh[1], p[2] + sh[2] };
+                q[d] -= s;
+                rb[((int64_t)p[0] * eb + p[1]) * eb + p[2]] += t * pa[((int64_t)q[0] * ea + q[1]) * ea + q[2]];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // blocks for ncell cells: a multiple of 8, so that xcd_block is a bijection
 inline unsigned cell_grid(int ncell) { return (unsigned)((ncell + 7) / 8 * 8); }
 
@@ -2908,6 +3175,67 @@ int box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi, 
     return MGX_OK;
 }
 
+// Mehrstellen on boxes: H psi (box-truncated), B psi, the Galerkin S, H and
+// B-matrix with the operators' one-point reach, and the residual
+// (B Psi) theta - H Psi (B psi_a with its reach into the other boxes)
+int box_hloc_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* vtot, double* hpsi)
+{
+    BoxView v = view(bs, 0);
+    hipLaunchKernelGGL(k_box_hlocM, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi, vtot, hpsi, v,
+        coefm(bs->h));
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_bpsi_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, double* bpsi)
+{
+    BoxView v = view(bs, 0);
+    hipLaunchKernelGGL(k_box_bpsiM, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi, bpsi, v);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_gram_hb_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* bpsi,
+    const double* vtot, double* S, double* H, double* B)
+{
+    const int n      = bs->nbox;
+    const int64_t nn = (int64_t)n * n;
+    double* part     = (double*)workspace(ctx, 4, 5 * std::max<int64_t>(bs->npairs, 1) * sizeof(double));
+    if (!part) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    for (double* m : { S, H, B })
+        hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, m, nn, 0.);
+    if (bs->npairs > 0)
+    {
+        hipLaunchKernelGGL(k_pair_gram_mehr, dim3((unsigned)bs->npairs), dim3(256), 0, ctx->stream, psi, hpsi, bpsi,
+            vtot, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_origin,
+            (const int*)bs->d_pairs, bs->npairs, bs->gdim[0], bs->gdim[1], bs->gdim[2], coefm(bs->h), part);
+        MGX_CHECK_LAUNCH(ctx);
+        const double vel = bs->h[0] * bs->h[1] * bs->h[2];
+        hipLaunchKernelGGL(k_pair_scatter_mehr, dim3(nblocks(bs->npairs)), dim3(kBlock), 0, ctx->stream,
+            (const double*)part, (const int*)bs->d_pairs, bs->npairs, n, vel, S, H, B);
+    }
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
+    double* res, double* d);
+
+int box_residual_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* bpsi, const double* hpsi,
+    const double* theta, double* res)
+{
+    int rc = box_residual(ctx, bs, bpsi, hpsi, theta, res, nullptr);
+    if (rc) return rc;
+    if (bs->nilist > 0)
+    {
+        hipLaunchKernelGGL(k_resid_skin_mehr, dim3(bs->nbox), dim3(256), 0, ctx->stream, psi, theta, bs->nbox,
+            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_ilist_ptr,
+            (const int*)bs->d_ilist, res);
+        MGX_CHECK_LAUNCH(ctx);
+    }
+    return MGX_OK;
+}
+
 // the autotuner (mgx_boxset_autotune): each candidate of each tunable timed
 // on scratch buffers of the box set's size; the fastest is kept
 int box_autotune(mgx_ctx* ctx, mgx_boxset* bs, double* ms)
@@ -3750,6 +4078,40 @@ int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, cons
     BOX_ARGS_OK(ctx, bs, 0);
     MGX_REQUIRE(ctx, psi && hpsi && theta && res, "null data");
     int rc = box_residual(ctx, bs, psi, hpsi, theta, res, nullptr);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_hloc_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* vtot, double* hpsi)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, psi && vtot && hpsi, "null data");
+    int rc = box_hloc_mehr(ctx, bs, psi, vtot, hpsi);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_gram_hb_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
+    const double* vtot, double* S, double* H, double* B)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, psi && hpsi && vtot && S && H && B, "null data");
+    const int64_t np = bs->off[0][bs->nbox];
+    double* bpsi     = (double*)workspace(ctx, 5, np * sizeof(double));
+    if (!bpsi) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    int rc = box_bpsi_mehr(ctx, bs, psi, bpsi);
+    if (!rc) rc = box_gram_hb_mehr(ctx, bs, psi, hpsi, bpsi, vtot, S, H, B);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_box_residual_mehr(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
+    const double* theta, double* res)
+{
+    BOX_ARGS_OK(ctx, bs, 0);
+    MGX_REQUIRE(ctx, psi && hpsi && theta && res, "null data");
+    const int64_t np = bs->off[0][bs->nbox];
+    double* bpsi     = (double*)workspace(ctx, 5, np * sizeof(double));
+    if (!bpsi) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    int rc = box_bpsi_mehr(ctx, bs, psi, bpsi);
+    if (!rc) rc = box_residual_mehr(ctx, bs, psi, bpsi, hpsi, theta, res);
     return rc ? rc : sync(ctx);
 }
 

@@ -60,6 +60,12 @@ int boxset_maxedge(const mgx_boxset*);
 int box_anderson_update_nrm(mgx_ctx*, const mgx_boxset*, int, const double*, double*, const void*, double,
     const void*, void*, const double*, double, double*);
 int box_density(mgx_ctx*, const mgx_boxset*, const double*, const double*, int, double, double*, double*);
+int box_hloc_mehr(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
+int box_bpsi_mehr(mgx_ctx*, const mgx_boxset*, const double*, double*);
+int box_gram_hb_mehr(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, const double*, double*,
+    double*, double*);
+int box_residual_mehr(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, const double*,
+    double*);
 
 // y = a x (n values)
 __global__ void k_scaled_copy(const double* x, int64_t n, double a, double* y)
@@ -96,6 +102,8 @@ struct mgx_solver
     double *S = nullptr, *H = nullptr, *L = nullptr, *Sinv = nullptr, *th = nullptr;
     double *eig = nullptr, *d = nullptr, *scal = nullptr, *nswork = nullptr;
     double *Sv = nullptr, *Hv = nullptr;  // S, H per interacting pair
+    // Mehrstellen: B psi, the B-matrix and its inverse
+    double *bpsi = nullptr, *Bm = nullptr, *invB = nullptr;
     double* x0 = nullptr;  // Anderson history: previous psi
     void* kr0  = nullptr;  // and previous correction, in the preconditioner's type
     void* kr1  = nullptr;  // the V-cycle's correction (swapped with kr0 after the update)
@@ -149,7 +157,8 @@ namespace
 void free_solver(mgx_solver* s)
 {
     double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
-        s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->psi_prev, s->r_prev, s->psi_cnn, s->r_cnn };
+        s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->psi_prev, s->r_prev, s->psi_cnn, s->r_cnn, s->bpsi, s->Bm,
+        s->invB };
     for (double* b : bufs)
         if (b) hipFree(b);
     if (s->kr0) (void)hipFree(s->kr0);
@@ -326,6 +335,41 @@ int tail_body(mgx_solver* s, bool use_cnn, bool restored, bool plain_step, doubl
     return MGX_OK;
 }
 
+// The same for the Mehrstellen discretisation (FDtype = Mehrstellen): H psi
+// = -Lap_M psi + B V psi and B psi on the boxes, the Galerkin S, H and
+// B-matrix, S^-1 and B-matrix^-1 by Cholesky + inverse (GramMatrix,
+// ProjectedMatricesMehrstellen::computeInvB,
+// src/ProjectedMatricesMehrstellen.cc:38-64), theta = B-matrix^-1 H
+// (updateTheta, .h:57-62), the band energy tr(HB S^-1) = tr(theta)
+// (getEigSum, src/ProjectedMatrices.cc:550-569: HB = (S theta + theta^T S) / 2),
+// the residual (B Psi) theta - H Psi and its norm.  Launched eagerly.
+int head_mehr(mgx_solver* s, bool pending)
+{
+    mgx_ctx* ctx = s->ctx;
+    int rc;
+    int tk = prof_begin(ctx, MGX_PROF_HLOC);
+    if (pending && (rc = box_scale(ctx, s->bs, s->psi, s->d))) return rc;
+    if ((rc = box_hloc_mehr(ctx, s->bs, s->psi, s->vtot, s->hpsi))) return rc;
+    if ((rc = box_bpsi_mehr(ctx, s->bs, s->psi, s->bpsi))) return rc;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_GRAM);
+    if ((rc = box_gram_hb_mehr(ctx, s->bs, s->psi, s->hpsi, s->bpsi, s->vtot, s->S, s->H, s->Bm))) return rc;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_DENSE);
+    if ((rc = chol_inv(ctx, s->n, s->S, s->L, s->Sinv))) return rc;
+    if ((rc = chol_inv(ctx, s->n, s->Bm, s->L, s->invB))) return rc;
+    if ((rc = theta(ctx, s->n, s->invB, s->H, s->th))) return rc;
+    if ((rc = diag_sum(ctx, s->n, s->th, s->scal))) return rc;
+    prof_end(ctx, tk);
+    tk = prof_begin(ctx, MGX_PROF_RESID);
+    if ((rc = box_residual_mehr(ctx, s->bs, s->psi, s->bpsi, s->hpsi, s->th, s->res))) return rc;
+    if ((rc = box_sumsq(ctx, s->bs, s->res, s->d))) return rc;
+    if ((rc = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return rc;
+    prof_end(ctx, tk);
+    MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, 8 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return MGX_OK;
+}
+
 uint64_t mix_key(uint64_t h, uint64_t v)
 {
     h ^= v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
@@ -392,10 +436,13 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     // Cholesky on the first step (every step with MGX_INV_CHOL); else as
     // many Newton-Schulz steps as the last refresh needed, unchecked, their
     // deltas checked with the energy and the residual norm below
-    const bool chol    = !s->have_inv || s->cfg.inverse_mode == MGX_INV_CHOL;
+    const bool mehr    = s->cfg.lap_type == MGX_LAP_MEHRSTELLEN;
+    const bool chol    = mehr || !s->have_inv || s->cfg.inverse_mode == MGX_INV_CHOL;
     int spec           = chol ? 0 : std::max(1, std::min(s->ns_pred, s->ns_max_spec));
     const bool pending = s->pending_norm;
-    if (chol)
+    if (mehr)
+        rc = head_mehr(s, pending);  // Cholesky of S and of the B-matrix every step
+    else if (chol)
         rc = head_body(s, pending, true, 0, bf16);  // waits for potrf's info: never captured
     else
     {
@@ -610,7 +657,15 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     const double* psi0_host, mgx_solver** out)
 {
     MGX_REQUIRE(ctx, bs && cfg && vtot_host && psi0_host && out, "bad solver arguments");
-    MGX_REQUIRE(ctx, cfg->lap_type == MGX_LAP_4TH, "solver supports the 4th-order operator");
+    MGX_REQUIRE(ctx, cfg->lap_type == MGX_LAP_4TH || cfg->lap_type == MGX_LAP_MEHRSTELLEN,
+        "solver supports the 4th-order and the Mehrstellen operators");
+    if (cfg->lap_type == MGX_LAP_MEHRSTELLEN)
+    {
+        MGX_REQUIRE(ctx, cfg->precond_dtype == MGX_F32 || cfg->precond_dtype == MGX_F64,
+            "the Mehrstellen solver takes an FP32 or FP64 V-cycle");
+        MGX_REQUIRE(ctx, cfg->gram_prec == MGX_PREC_F64, "the Mehrstellen solver forms S, H and B in FP64");
+        MGX_REQUIRE(ctx, cfg->precond_kind == MGX_PRECOND_VCYCLE, "the Mehrstellen solver takes the V-cycle");
+    }
     MGX_REQUIRE(ctx, cfg->precond_dtype == MGX_F32 || cfg->precond_dtype == MGX_F64 || cfg->precond_dtype == MGX_F16,
         "bad precond dtype");
     MGX_REQUIRE(ctx, cfg->gram_prec == MGX_PREC_F64 || cfg->gram_prec == MGX_PREC_SPLIT, "bad gram precision");
@@ -675,6 +730,8 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
                   || (alloc(&s->psi_prev, s->npts) && alloc(&s->r_prev, s->npts) && alloc(&s->psi_cnn, s->npts)
                       && alloc(&s->r_cnn, s->npts)
                       && hipMalloc(&s->kcnn, s->npts * tsize) == hipSuccess))
+              && (cfg->lap_type != MGX_LAP_MEHRSTELLEN
+                  || (alloc(&s->bpsi, s->npts) && alloc(&s->Bm, nn) && alloc(&s->invB, nn)))
               && hipHostMalloc((void**)&s->host, 32 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {

@@ -46,6 +46,19 @@ def read_cfg(path):
     return out
 
 
+# Control::setOptions, src/Control.cc:1353-1359: FDtype name -> lap_type
+FD_TYPES = {"Mehrstellen": 0, "2nd": 1, "4th": 2, "6th": 3, "8th": 4, "Mehrstellen2": 10}
+
+
+def fd_type(cfg_path):
+    """The orbital discretisation of an MGmol config (default Mehrstellen,
+    src/read_config.cc:52) as a lap_type code."""
+    name = str(read_cfg(cfg_path)[""].get("FDtype", "Mehrstellen"))
+    if name not in FD_TYPES:
+        raise ValueError("%s: unknown FDtype %r" % (cfg_path, name))
+    return FD_TYPES[name]
+
+
 def read_lrs(path):
     """(n, 3) centres in bohr."""
     a = np.loadtxt(path, ndmin=2)

@@ -37,6 +37,7 @@ EXPORTS = [
     "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_autotune", "mgx_boxset_set_tune", "mgx_boxset_get_tune", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
     "mgx_box_lap", "mgx_box_hloc", "mgx_box_jacobi", "mgx_box_restrict", "mgx_box_prolong", "mgx_box_precond_mg",
     "mgx_box_gram_h", "mgx_box_residual", "mgx_box_residual_norm", "mgx_box_density",
+    "mgx_box_hloc_mehr", "mgx_box_gram_hb_mehr", "mgx_box_residual_mehr",
     "mgx_solver_create", "mgx_solver_destroy", "mgx_solver_step", "mgx_solver_run", "mgx_solver_reset",
     "mgx_solver_profile", "mgx_solver_profile_work", "mgx_solver_set_profile",
     "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_inverse", "mgx_solver_set_psi", "mgx_solver_set_cnn", "mgx_solver_density",
@@ -136,6 +137,8 @@ def lib():
             "mgx_box_jacobi": (P, P, I, I, I, D, P, P, P), "mgx_box_restrict": (P, P, I, I, P, P),
             "mgx_box_prolong": (P, P, I, I, P, P), "mgx_box_precond_mg": (P, P, I, I, D, I, P, P),
             "mgx_box_gram_h": (P, P, I, P, P, P, P), "mgx_box_residual": (P, P, P, P, P, P),
+            "mgx_box_hloc_mehr": (P, P, P, P, P), "mgx_box_gram_hb_mehr": (P, P, P, P, P, P, P, P),
+            "mgx_box_residual_mehr": (P, P, P, P, P, P),
             "mgx_box_residual_norm": (P, P, P, P, C.POINTER(D)),
             "mgx_box_density": (P, P, P, P, I, D, P, C.POINTER(D)),
             "mgx_solver_create": (P, P, C.POINTER(SolverCfg), P, P, C.POINTER(P)),
@@ -350,11 +353,11 @@ class Solver:
     def __init__(self, ctx, boxset, vtot, psi0, mg_levels=2, precond_dtype=F64, gram_prec=PREC_F64,
                  step_length=1.5, ritz_every=0, precond_start=START_ZERO, profile=0, anderson_m=0,
                  anderson_beta=1.0, conv_criterion=CONV_RESIDUAL, precond_kind=PRECOND_VCYCLE, cnn=None,
-                 inverse_mode=INV_NS_F64, ns_tol=0.0, ns_max_spec=0):
+                 inverse_mode=INV_NS_F64, ns_tol=0.0, ns_max_spec=0, lap_type=LAP_4TH):
         import numpy as np
         self.ctx = ctx
         self.boxset = boxset
-        cfg = SolverCfg(LAP_4TH, mg_levels, precond_dtype, gram_prec, step_length, ritz_every, precond_start,
+        cfg = SolverCfg(lap_type, mg_levels, precond_dtype, gram_prec, step_length, ritz_every, precond_start,
                         profile, anderson_m, anderson_beta, conv_criterion, precond_kind, inverse_mode, ns_tol,
                         ns_max_spec)
         v = np.ascontiguousarray(vtot, dtype=np.float64)

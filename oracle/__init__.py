@@ -669,6 +669,142 @@ def b_solver_step(b, psi, vtot, mg_levels=2, prec=F64, step_length=1.5, lowest=N
     return new, 0.5 * e_ry, rn, lowest
 
 
+# ------------------------------------------- Mehrstellen on per-orbital boxes
+# MGmol's default discretisation (FDtype = Mehrstellen, src/Control.cc:1353)
+# on boxes: H psi_j = -Lap_M psi_j + B (V psi_j) (src/Hamiltonian.cc:106-131)
+# with psi_j zero outside its box; the Galerkin H and B-matrix carry the
+# operators' one-point reach outside each box (faces, and edges for -Lap_M),
+# the residual is (B Psi) theta - H Psi (src/MGmol.cc:1227-1291), theta =
+# B-matrix^-1 H (src/ProjectedMatricesMehrstellen.h:46-69).  Computed by
+# applying the full-grid operators of this oracle to each orbital on its box
+# grown by one point (Dirichlet, so the values outside are the true zeros).
+
+def _box_v(b, vglob, j, grow):
+    """V on box j grown by `grow` points (periodic wrap), (e+2g)^3."""
+    e = int(b._edge[j]) + 2 * grow
+    gd = [int(b.gdim[d]) for d in range(3)]
+    idx = [(np.arange(e) + int(b._origin[j, d]) - grow) % gd[d] for d in range(3)]
+    return np.asarray(vglob).reshape(gd)[np.ix_(*idx)]
+
+
+def _box_grid(b, e, ghosts=1):
+    return Grid.make((e, e, e), ghosts, [float(b.h[d]) for d in range(3)])
+
+
+def b_hloc_mehr(b, vglob, psi):
+    """Box-truncated Mehrstellen H psi (packed boxes, FP64)."""
+    off = b.offsets(0)
+    out = np.zeros_like(psi, dtype=F64)
+    for j in range(b.nbox):
+        e = int(b._edge[j])
+        g = _box_grid(b, e)
+        u = embed(g, psi[off[j]:off[j + 1]].reshape(1, e, e, e).astype(F64))
+        out[off[j]:off[j + 1]] = hloc(g, 0, u, _box_v(b, vglob, j, 0), bc=(0, 0, 0))[0]
+    return out
+
+
+def b_bpsi_mehr(b, psi):
+    """Box-truncated B psi (FDkernelRHS_4th_Mehr1 with zero outside each box)."""
+    off = b.offsets(0)
+    out = np.zeros_like(psi, dtype=F64)
+    for j in range(b.nbox):
+        e = int(b._edge[j])
+        g = _box_grid(b, e)
+        u = trade(g, embed(g, psi[off[j]:off[j + 1]].reshape(1, e, e, e).astype(F64)), bc=(0, 0, 0))
+        out[off[j]:off[j + 1]] = rhs_mehr(g, u, 0).reshape(-1)
+    return out
+
+
+def _grown_ops(b, vglob, psi, j):
+    """(H psi_j, B psi_j) on box j grown by one point, (e+2)^3 each."""
+    off = b.offsets(0)
+    e = int(b._edge[j])
+    g = _box_grid(b, e + 2)
+    full = np.zeros((1, e + 2, e + 2, e + 2))
+    full[0, 1:e + 1, 1:e + 1, 1:e + 1] = psi[off[j]:off[j + 1]].reshape(e, e, e)
+    u = embed(g, full)
+    hp = hloc(g, 0, u, _box_v(b, vglob, j, 1), bc=(0, 0, 0))[0].reshape(e + 2, e + 2, e + 2)
+    bp = rhs_mehr(g, trade(g, u.copy(), bc=(0, 0, 0)), 0)[0]
+    return hp, bp
+
+
+def _in_grown(b, j, a):
+    """For the points of box j: index arrays into box a grown by one point
+    (periodic, the unique image), and the mask of those inside it."""
+    gd = [int(b.gdim[d]) for d in range(3)]
+    ej, ea = int(b._edge[j]), int(b._edge[a])
+    loc = []
+    for d in range(3):
+        xg = (np.arange(ej) + int(b._origin[j, d]) - int(b._origin[a, d]) + 1) % gd[d]
+        loc.append(xg)
+    m = (loc[0][:, None, None] < ea + 2) & (loc[1][None, :, None] < ea + 2) & (loc[2][None, None, :] < ea + 2)
+    ii = np.ix_(np.minimum(loc[0], ea + 1), np.minimum(loc[1], ea + 1), np.minimum(loc[2], ea + 1))
+    return ii, m
+
+
+def b_gram_hb_mehr(b, psi, vglob):
+    """S, H, B-matrix (n x n FP64): S_ij = vel <psi_i, psi_j>, H_ij = vel
+    <psi_i, H psi_j> and B_ij = vel <psi_i, B psi_j> with the operators'
+    reach outside box j (H is not symmetric: B V is not)."""
+    n = b.nbox
+    off = b.offsets(0)
+    vel = float(np.prod([float(b.h[d]) for d in range(3)]))
+    S, H, B = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    grown = [_grown_ops(b, vglob, psi, j) for j in range(n)]
+    for i in range(n):
+        ei = int(b._edge[i])
+        pi = psi[off[i]:off[i + 1]].reshape(ei, ei, ei)
+        for j in range(n):
+            ii, m = _in_grown(b, i, j)
+            if not m.any():
+                continue
+            hp, bp = grown[j]
+            H[i, j] = vel * float(np.sum(pi * hp[ii] * m))
+            B[i, j] = vel * float(np.sum(pi * bp[ii] * m))
+            ej = int(b._edge[j])
+            pj = np.zeros((ej + 2,) * 3)
+            pj[1:ej + 1, 1:ej + 1, 1:ej + 1] = psi[off[j]:off[j + 1]].reshape(ej, ej, ej)
+            S[i, j] = vel * float(np.sum(pi * pj[ii] * m))
+    return S, H, B
+
+
+def b_residual_mehr(b, psi, hpsi, th, vglob=None):
+    """R_j = ((B Psi) theta)_j - (H psi)_j on box j, (B psi_a) with its reach."""
+    n = b.nbox
+    off = b.offsets(0)
+    zero_v = np.zeros(int(np.prod([int(b.gdim[d]) for d in range(3)])))
+    bps = [_grown_ops(b, zero_v, psi, a)[1] for a in range(n)]
+    out = np.zeros_like(hpsi, dtype=F64)
+    for j in range(n):
+        ej = int(b._edge[j])
+        r = np.zeros((ej, ej, ej))
+        for a in range(n):
+            ii, m = _in_grown(b, j, a)
+            if m.any():
+                r += th[a, j] * bps[a][ii] * m
+        out[off[j]:off[j + 1]] = (r.reshape(-1) - hpsi[off[j]:off[j + 1]])
+    return out
+
+
+def b_solver_step_mehr(b, psi, vtot, mg_levels=2, prec=F64, step_length=2.0):
+    """One orbital step with the Mehrstellen discretisation (V-cycle from zero
+    with the Mehrstellen fine operator, Laph4M): returns (new_psi,
+    band_energy_Ha, residual_Ha)."""
+    hp = b_hloc_mehr(b, vtot, psi)
+    S, H, Bm = b_gram_hb_mehr(b, psi, vtot)
+    _, Sinv = chol_inv(S)
+    _, th, _, e_ry = mehr_projected(S, Sinv, Bm, H)
+    R = b_residual_mehr(b, psi, hp, th)
+    rn = residual_norm(b_diag_dots(b, R), Sinv)
+    KR = b_precond(b, 0, mg_levels, 0.0, R, dtype=prec)
+    new = psi + 0.5 * step_length * KR
+    d = b_diag_dots(b, new)
+    off = b.offsets(0)
+    for j in range(b.nbox):
+        new[off[j]:off[j + 1]] *= 1.0 / np.sqrt(d[j])
+    return new, 0.5 * e_ry, rn
+
+
 def _anderson_theta(mat, rhs):
     """AndersonMix::update coefficient for one history vector
     (src/AndersonMix.cc): theta = rhs/mat, reset to -0.5 (theta > 1) or 0
@@ -688,8 +824,9 @@ class BSolver:
     src/ABPG.cc:73-140 and src/AndersonMix.cc (m = 1, dotProductDiagonal).
     Fine-level V-cycle from zero (zero_start)."""
 
-    def __init__(self, b, psi, vtot, mg_levels=2, prec=F64, step_length=1.5, anderson_m=0, beta=1.0):
+    def __init__(self, b, psi, vtot, mg_levels=2, prec=F64, step_length=1.5, anderson_m=0, beta=1.0, lap_type=2):
         self.b, self.psi, self.vtot = b, np.array(psi, dtype=np.float64), vtot
+        self.lap_type = lap_type  # 2: 4th order; 0: Mehrstellen (b_*_mehr, theta = B-matrix^-1 H)
         self.mg_levels, self.prec, self.alpha = mg_levels, prec, 0.5 * step_length
         self.m, self.beta = anderson_m, beta
         self.iter, self.e_prev, self.deig, self.deig2 = 0, 0.0, 1e9, 1e9
@@ -705,12 +842,20 @@ class BSolver:
     def step(self):
         """One step; returns (band_energy_Ha, residual_Ha, deig2_Ha)."""
         b, psi = self.b, self.psi
-        hp = b_hloc(b, self.vtot, psi)
-        S, H = b_gram_h(b, psi, hp)
-        _, Sinv = chol_inv(S)
-        th = theta(Sinv, H)
-        e = 0.5 * eigsum(H, Sinv)
-        R = b_residual(b, psi, hp, th)
+        if self.lap_type == 0:
+            hp = b_hloc_mehr(b, self.vtot, psi)
+            S, H, Bm = b_gram_hb_mehr(b, psi, self.vtot)
+            _, Sinv = chol_inv(S)
+            _, th, _, e_ry = mehr_projected(S, Sinv, Bm, H)
+            e = 0.5 * e_ry
+            R = b_residual_mehr(b, psi, hp, th)
+        else:
+            hp = b_hloc(b, self.vtot, psi)
+            S, H = b_gram_h(b, psi, hp)
+            _, Sinv = chol_inv(S)
+            th = theta(Sinv, H)
+            e = 0.5 * eigsum(H, Sinv)
+            R = b_residual(b, psi, hp, th)
         rn = residual_norm(b_diag_dots(b, R), Sinv)
         if self.iter > 0:
             old = self.deig
@@ -719,7 +864,7 @@ class BSolver:
         self.e_prev = e
         n = b.nbox
         self.accelerate = self.accelerate or (self.m > 0 and 4.0 * self.deig2 < 1e-2 * n)
-        f = self.alpha * b_precond(b, 2, self.mg_levels, 0.0, R, dtype=self.prec)
+        f = self.alpha * b_precond(b, self.lap_type, self.mg_levels, 0.0, R, dtype=self.prec)
         if not self.accelerate:
             psi = psi + f
         else:

@@ -50,3 +50,10 @@ def test_box_setup():
 def test_box_setup_rejects_oversized_boxes():
     with pytest.raises(ValueError):
         inputs.box_setup(CFG, LRS, edge=32)
+
+
+def test_fd_type_of_the_reference_inputs():
+    from mgmol_amd import inputs
+    import os
+    d = os.path.join(os.path.dirname(__file__), "golden", "energy_and_forces")
+    assert inputs.fd_type(os.path.join(d, "mgmol.cfg")) == 0  # FDtype=Mehrstellen
