@@ -2437,10 +2437,12 @@ __global__ void __launch_bounds__(256) k_box_and_update_sq(const double* x, doub
     double acc[1] = { 0. };
     for (int64_t q = off[j] + threadIdx.x; q < off[j + 1]; q += blockDim.x)
     {
+        // without a history (th = 0) the history buffers are not read: they
+        // may not hold values yet
         const T k       = kr[q];
-        const double xv = x[q], x0v = xo[q], fv = a * (double)k, f0v = th != 0. ? a * (double)kr0[q] : 0.;
-        const double xb = (1. - th) * xv + th * x0v;
-        const double fb = (1. - th) * fv + th * f0v;
+        const double xv = x[q], fv = a * (double)k;
+        const double xb = th != 0. ? (1. - th) * xv + th * xo[q] : xv;
+        const double fb = th != 0. ? (1. - th) * fv + th * (a * (double)kr0[q]) : fv;
         if (kr0_out) kr0_out[q] = k;
         const double nv = xb + beta * fb;
         xo[q]           = nv;
@@ -2484,10 +2486,12 @@ __global__ void __launch_bounds__(1024) k_box_and_update_nrm(const double* x, do
             nv[i]       = 0.;
             if (t < n)
             {
+                // without a history (th = 0) the history buffers are not
+                // read: they may not hold values yet
                 const T k       = kj[t];
-                const double xv = xj[t], x0v = xoj[t], fv = a * (double)k, f0v = th != 0. ? a * (double)k0j[t] : 0.;
-                const double xb = (1. - th) * xv + th * x0v;
-                const double fb = (1. - th) * fv + th * f0v;
+                const double xv = xj[t], fv = a * (double)k;
+                const double xb = th != 0. ? (1. - th) * xv + th * xoj[t] : xv;
+                const double fb = th != 0. ? (1. - th) * fv + th * (a * (double)k0j[t]) : fv;
                 if (k0o) k0o[t] = k;
                 nv[i] = xb + beta * fb;
                 acc[0] += nv[i] * nv[i];

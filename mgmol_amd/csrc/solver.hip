@@ -746,6 +746,10 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
     hipMemcpy(s->vtot, vtot_host, gsize * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(s->psi0, psi0_host, s->npts * sizeof(double), hipMemcpyHostToDevice);
     hipMemcpy(s->psi, s->psi0, s->npts * sizeof(double), hipMemcpyDeviceToDevice);
+    // the history and correction buffers start at zero (the kernels read the
+    // history only once it exists; zero keeps a stray read finite)
+    for (void* p : { (void*)s->x0, s->kr0, s->kr1, s->kcnn })
+        if (p) hipMemset(p, 0, p == (void*)s->x0 ? s->npts * sizeof(double) : s->npts * tsize);
     if (hipDeviceSynchronize() != hipSuccess)
     {
         free_solver(s);
