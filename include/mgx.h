@@ -59,7 +59,11 @@ typedef struct
     int dim[3];  /* interior points */
     int ghosts;  /* ghost width */
     double h[3]; /* spacing */
-    int bc[3];   /* 1 = periodic, 0 = zero (Dirichlet) ghosts */
+    int bc[3];   /* 1 = periodic; 0 = Dirichlet as GridFunc::setBoundaryValues
+                  * (src/pb/GridFunc.cc:2189-2340) sets it: the ghosts and the
+                  * index-0 layer (the boundary plane) hold zero, a halo
+                  * exchange zeroes that layer in place, and the kernels that
+                  * wrap in place read it as zero; interior norms leave it out */
 } mgx_grid_t;
 
 /* pb::GridFuncVector<T> storage, src/pb/GridFuncVector.h:30-493 */
@@ -156,9 +160,12 @@ int mgx_mg(mgx_ctx* ctx, int lap_type, int max_levels, mgx_block_t* v, const mgx
  * times: stop when |rhs - A vh| / |rhs| < tol, else one V-cycle on the residual
  * (nu1 / nu2 Jacobi sweeps, coarse levels while every dimension is even and at
  * least twice the ghost width, at most max_nlevels coarsenings, 4th order coarse
- * levels in 2nd order) added to vh; then vh -= mean(vh) (GridFunc::average0) and
- * its ghosts are refreshed.  rho, vh: one FP64 function each, same periodic
- * grid (bc = 1); vh is the initial guess and the result.  The reference's
+ * levels in 2nd order) added to vh; then, on a fully periodic grid,
+ * vh -= mean(vh) (GridFunc::average0, src/pb/SolverLap.cc:60), and its ghosts
+ * are refreshed.  rho, vh: one FP64 function each, same grid, every direction
+ * periodic (bc 1) or Dirichlet (bc 0, zero boundary values; the multipole
+ * values of bc 2, src/Hartree.cc:50-76, are not built); vh is the initial
+ * guess and the result.  The reference's
  * defaults are nu1 = nu2 = 2, max_sweeps = 10, tol = 1e-16, max_nlevels = 10. */
 typedef struct
 {

@@ -22,8 +22,12 @@ __device__ __forceinline__ int wrapi(int i, int n)
 
 // ------------------------------------------------------------------ halo --
 // GridFuncVector::trade_boundaries single-task branches
-// (src/pb/GridFuncVector.cc:585-604, 1052-1070, 1426-1440): every ghost point
-// receives the periodic image, or zero along a Dirichlet direction.
+// (src/pb/GridFuncVector.cc:1544-1622, 585-604, 1052-1070, 1426-1440): every
+// ghost point receives the periodic image; along a Dirichlet direction
+// GridFunc::setBoundaryValues (src/pb/GridFunc.cc:2189-2340) zeroes the ghosts
+// and the first interior layer (index 0, the boundary plane of the
+// vertex-centred grid), so a point is zero when a Dirichlet coordinate lies
+// outside 1 .. n-1.
 template <typename T>
 __global__ void k_halo(T* u, GGeom g, int64_t total, int bc0, int bc1, int bc2)
 {
@@ -43,6 +47,7 @@ __global__ void k_halo(T* u, GGeom g, int64_t total, int bc0, int bc1, int bc2)
 #pragma unroll
         for (int k = 0; k < 3; k++)
         {
+            if (bc[k] != 1 && c[k] == 0) zero = true;
             if (c[k] < 0 || c[k] >= d[k])
             {
                 ghost = true;
@@ -52,7 +57,7 @@ __global__ void k_halo(T* u, GGeom g, int64_t total, int bc0, int bc1, int bc2)
                     zero = true;
             }
         }
-        if (!ghost) continue;
+        if (!ghost && !zero) continue;
         T* uf = u + f * g.sizeg;
         const int64_t dst = (int64_t)x * g.incx + (int64_t)y * g.incy + z;
         const int64_t src = (int64_t)(c[0] + g.ng) * g.incx + (int64_t)(c[1] + g.ng) * g.incy + (c[2] + g.ng);
@@ -389,9 +394,13 @@ struct PGeom
     int64_t size;
 };
 
+// the neighbour c + d along one direction, false when its value is zero: a
+// Dirichlet direction (bc 0) holds zero outside 1 .. n-1 (the index-0 layer
+// is the boundary plane, as after a halo exchange)
 __device__ __forceinline__ bool nb(int c, int d, int n, int bc, int* out)
 {
     int v = c + d;
+    if (!bc && v == 0) return false;
     if (v < 0 || v >= n)
     {
         if (!bc) return false;
@@ -568,31 +577,34 @@ __global__ void __launch_bounds__(NT) k_grid_march(const T* __restrict__ in, con
     T2 zero2;
     zero2[0] = (T)0;
     zero2[1] = (T)0;
-    // x / y coordinates of a neighbour (wrapped), false when it is zero
+    // x / y coordinates of a neighbour (wrapped), false when it is zero (a
+    // Dirichlet direction holds zero outside 1 .. n-1)
     auto cx = [&](int x, int& r) -> bool {
-        if (!WRAP || (x >= 0 && x < g.d0))
+        if (!WRAP || (x > 0 && x < g.d0))
         {
             r = x;
             return true;
         }
         if (!g.bc0) return false;
-        r = x < 0 ? x + g.d0 : x - g.d0;
+        r = x < 0 ? x + g.d0 : (x >= g.d0 ? x - g.d0 : x);
         return true;
     };
     auto cy = [&](int y, int& r) -> bool {
-        if (!WRAP || (y >= 0 && y < g.d1))
+        if (!WRAP || (y > 0 && y < g.d1))
         {
             r = y;
             return true;
         }
         if (!g.bc1) return false;
-        r = y < 0 ? y + g.d1 : y - g.d1;
+        r = y < 0 ? y + g.d1 : (y >= g.d1 ? y - g.d1 : y);
         return true;
     };
     auto ld2 = [&](int x, int y, int z) -> T2 {
         int xx, yy;
         if (!cx(x, xx) || !cy(y, yy)) return zero2;
-        return *reinterpret_cast<const T2*>(inf + (int64_t)xx * g.ix + (int64_t)yy * g.iy + z);
+        T2 v = *reinterpret_cast<const T2*>(inf + (int64_t)xx * g.ix + (int64_t)yy * g.iy + z);
+        if (WRAP && !g.bc2 && z == 0) v[0] = (T)0;
+        return v;
     };
     // the z-ghost pair of a row end (side 0: z = -2, -1; side 1: z = d2, d2 + 1)
     auto ghost2 = [&](int x, int y, int side, const T2& own) -> T2 {
@@ -773,6 +785,15 @@ __global__ void __launch_bounds__(NT) k_grid_march(const T* __restrict__ in, con
 }
 
 // --------------------------------------------------------------- drivers --
+// boundary conditions per direction: 1 periodic, 0 Dirichlet
+int check_bc(mgx_ctx* ctx, const mgx_grid_t& g)
+{
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, g.bc[i] == 0 || g.bc[i] == 1,
+            "boundary conditions are 0 (Dirichlet) or 1 (periodic); multipole values are not supported");
+    return MGX_OK;
+}
+
 int check_block(mgx_ctx* ctx, const mgx_block_t* b)
 {
     MGX_REQUIRE(ctx, b && b->data, "null block");
@@ -781,7 +802,7 @@ int check_block(mgx_ctx* ctx, const mgx_block_t* b)
     for (int i = 0; i < 3; i++)
         MGX_REQUIRE(ctx, b->grid.dim[i] > 0 && b->grid.dim[i] < 100000, "bad grid dim");
     MGX_REQUIRE(ctx, b->grid.ghosts >= 0, "bad ghost width");
-    return MGX_OK;
+    return check_bc(ctx, b->grid);
 }
 
 bool same_shape(const mgx_block_t* a, const mgx_block_t* b)
@@ -1217,7 +1238,7 @@ int precond_run(mgx_ctx* ctx, int type, int max_levels, double gamma, const mgx_
 // vel * sum of squares (or the plain sum) over the interior of one function:
 // per-block partials, then one block adds them in a fixed order
 template <bool SQ>
-__global__ void __launch_bounds__(256) k_int_sum(const double* u, GGeom g, double* part)
+__global__ void __launch_bounds__(256) k_int_sum(const double* u, GGeom g, int lo0, int lo1, int lo2, double* part)
 {
     __shared__ double lds[4];
     double acc = 0.;
@@ -1228,6 +1249,7 @@ __global__ void __launch_bounds__(256) k_int_sum(const double* u, GGeom g, doubl
         r -= x * g.d1 * g.d2;
         const int y  = r / g.d2;
         const int z  = r - y * g.d2;
+        if (x < lo0 || y < lo1 || z < lo2) continue;
         const double v = u[(int64_t)(x + g.ng) * g.incx + (int64_t)(y + g.ng) * g.incy + z + g.ng];
         acc += SQ ? v * v : v;
     }
@@ -1277,13 +1299,16 @@ struct PoissonMG
 
 constexpr int kPartBlocks = 1024;
 
-// one sum over the interior to dev_out (device); SQ: of squares
+// one sum over the interior to dev_out (device); SQ: of squares.  As in
+// GridFunc::gdot (src/pb/GridFunc.cc:2745-2800), the index-0 layer of a
+// Dirichlet direction (the boundary plane) is left out.
 template <bool SQ>
 int int_sum(mgx_ctx* ctx, PoissonMG& P, const double* u, const mgx_grid_t& gr, double* dev_out)
 {
     GGeom g = ggeom(gr);
     const int nb = (int)std::min<int64_t>(kPartBlocks, (g.size + 255) / 256);
-    hipLaunchKernelGGL(k_int_sum<SQ>, dim3(nb), dim3(256), 0, ctx->stream, u, g, P.part);
+    hipLaunchKernelGGL(k_int_sum<SQ>, dim3(nb), dim3(256), 0, ctx->stream, u, g, gr.bc[0] != 1, gr.bc[1] != 1,
+        gr.bc[2] != 1, P.part);
     hipLaunchKernelGGL(k_part_sum, dim3(1), dim3(256), 0, ctx->stream, (const double*)P.part, nb, dev_out);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
@@ -1301,7 +1326,8 @@ int int_norm(mgx_ctx* ctx, PoissonMG& P, const double* u, const mgx_grid_t& gr, 
     return MGX_OK;
 }
 
-// pb::Vcycle, src/pb/Vcycle.h:28-249 (periodic, one task, USE_LOWER_ORDER)
+// pb::Vcycle, src/pb/Vcycle.h:28-249 (one task, USE_LOWER_ORDER); every level
+// keeps the fine grid's boundary conditions
 int poisson_vcycle(mgx_ctx* ctx, PoissonMG& P, int l, double* x, const double* f, int nu1, int nu2)
 {
     const mgx_grid_t& g = P.grids[l];
@@ -1348,7 +1374,7 @@ int poisson_vcycle(mgx_ctx* ctx, PoissonMG& P, int l, double* x, const double* f
 }
 
 // pb::Mgm, src/pb/Mgm.h:12-111, as SolverLap::solve calls it
-// (src/pb/SolverLap.cc:49-63): periodic grids only
+// (src/pb/SolverLap.cc:49-63); Dirichlet directions hold zero boundary values
 int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho, double* vh, int nu1, int nu2,
     int max_sweeps, double tol, int max_nlevels, mgx_poisson_stats_t* st)
 {
@@ -1451,12 +1477,16 @@ int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho,
         if ((rc = int_norm(ctx, P, lhs, g0, &fin))) return rc;
         rel = fin * inv_rhs_norm;
     }
-    // GridFunc::average0 for the fully periodic grid, then fresh ghosts
-    GGeom a = ggeom(g0);
-    if ((rc = int_sum<false>(ctx, P, vh, g0, P.part + kPartBlocks))) return rc;
-    hipLaunchKernelGGL(k_add_const, dim3(nblocks(n0)), dim3(kBlock), 0, ctx->stream, vh, n0,
-        (const double*)(P.part + kPartBlocks), 1. / (double)a.size);
-    MGX_CHECK_LAUNCH(ctx);
+    // GridFunc::average0 on a fully periodic grid only (SolverLap::solve,
+    // src/pb/SolverLap.cc:60-62), then fresh ghosts
+    if (g0.bc[0] == 1 && g0.bc[1] == 1 && g0.bc[2] == 1)
+    {
+        GGeom a = ggeom(g0);
+        if ((rc = int_sum<false>(ctx, P, vh, g0, P.part + kPartBlocks))) return rc;
+        hipLaunchKernelGGL(k_add_const, dim3(nblocks(n0)), dim3(kBlock), 0, ctx->stream, vh, n0,
+            (const double*)(P.part + kPartBlocks), 1. / (double)a.size);
+        MGX_CHECK_LAUNCH(ctx);
+    }
     if ((rc = launch_halo(ctx, vh, 1, g0))) return rc;
     if (st)
     {
@@ -1740,7 +1770,8 @@ int mgx_poisson_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, mgx_block
     MGX_REQUIRE(ctx, rho->dtype == MGX_F64 && rho->nfunc == 1, "one FP64 function");
     MGX_REQUIRE(ctx, lap_type >= 0 && lap_type <= 2, "LapFactory::createLap() --- option invalid");
     for (int i = 0; i < 3; i++)
-        MGX_REQUIRE(ctx, vh->grid.bc[i] == 1, "the multigrid Poisson solver takes periodic grids");
+        MGX_REQUIRE(ctx, vh->grid.bc[i] == 0 || vh->grid.bc[i] == 1,
+            "boundary conditions are 0 (Dirichlet, zero) or 1 (periodic); multipole values are not supported");
     MGX_REQUIRE(ctx, vh->grid.ghosts >= (lap_type == MGX_LAP_4TH ? 2 : 1), "too few ghosts for the operator");
     MGX_REQUIRE(ctx, nu1 >= 0 && nu2 >= 0 && max_sweeps >= 0 && max_nlevels >= 0, "bad cycle parameters");
     rc = poisson_run(ctx, lap_type, vh->grid, (const double*)rho->data, (double*)vh->data, nu1, nu2, max_sweeps, tol,
@@ -1753,6 +1784,7 @@ int mgx_hloc_apply(mgx_ctx* ctx, int type, const mgx_grid_t* grid, int dtype, co
 {
     MGX_REQUIRE(ctx, grid && psi && vtot && hpsi && nfunc > 0, "bad hloc arguments");
     MGX_REQUIRE(ctx, dtype == MGX_F32 || dtype == MGX_F64, "bad dtype");
+    if (int rc = check_bc(ctx, *grid)) return rc;
     PGeom g;
     g.d0   = grid->dim[0];
     g.d1   = grid->dim[1];
@@ -1832,6 +1864,7 @@ int mgx_precond_mg(mgx_ctx* ctx, int type, int max_levels, double gamma, int pre
 {
     MGX_REQUIRE(ctx, grid && res && out && nfunc > 0, "bad precond arguments");
     MGX_REQUIRE(ctx, gamma > 0., "gamma must be positive");
+    if (int r = check_bc(ctx, *grid)) return r;
     const int64_t numpt = (int64_t)grid->dim[0] * grid->dim[1] * grid->dim[2];
     MGX_REQUIRE(ctx, ld >= numpt && ldo >= numpt, "leading dimension smaller than numpt");
     int rc = prec_dtype == MGX_F32

@@ -196,6 +196,11 @@ def interior(g, block):
     return block[..., ng:ng + g.dim[0], ng:ng + g.dim[1], ng:ng + g.dim[2]]
 
 
+# boundary code of a box cut out of the grid: zero ghosts, interior kept (the
+# zero-outside truncation of a localized orbital, LocGridOrbitals masks)
+BOX_OPEN = (3, 3, 3)
+
+
 def trade(g, u, bc=(1, 1, 1)):
     nf = u.shape[0]
     _chk(getattr(lib(), "or_g_trade_" + _suf(u))(C.byref(g), _p(u), nf, _bc(bc)), "trade")
@@ -291,10 +296,13 @@ def mg(g, lap_type, max_levels, v, f, bc=(1, 1, 1), masks=None):
     return v
 
 
-def gnorm(g, u):
+def gnorm(g, u, bc=(1, 1, 1)):
     """pb::norm (GridFunc::norm2, src/pb/GridFunc.cc:2856-2861): sqrt(vel *
-    sum of squares over the interior), one function."""
-    v = interior(g, u)
+    sum of squares over the interior), one function; GridFunc::gdot
+    (src/pb/GridFunc.cc:2745-2800) leaves out the index-0 layer (the boundary
+    plane) of every Dirichlet direction."""
+    v = interior(g, u)[0]
+    v = v[int(bc[0] != 1):, int(bc[1] != 1):, int(bc[2] != 1):]
     return float(np.sqrt(np.sum(v.astype(F64) ** 2) * g.h[0] * g.h[1] * g.h[2]))
 
 
@@ -306,68 +314,71 @@ def _lower_order(lap_type):
     return 1 if lap_type == 2 else lap_type
 
 
-def _vcycle(g, lap_type, x, rhs, level, cogr, nu1, nu2):
-    """pb::Vcycle (src/pb/Vcycle.h:28-249), periodic, one process: nu1
-    Jacobi sweeps; on a coarsenable level the fresh residual A x - rhs is
-    restricted, the coarse problem solved from zero by the same cycle, its
-    prolongation subtracted; nu2 Jacobi sweeps."""
+def _vcycle(g, lap_type, x, rhs, level, cogr, nu1, nu2, bc=(1, 1, 1)):
+    """pb::Vcycle (src/pb/Vcycle.h:28-249), one process: nu1 Jacobi sweeps;
+    on a coarsenable level the fresh residual A x - rhs is restricted, the
+    coarse problem solved from zero by the same cycle, its prolongation
+    subtracted; nu2 Jacobi sweeps.  bc per direction: 1 periodic, 0 zero
+    ghosts (the coarse levels keep the fine level's)."""
     res = np.zeros_like(x)
     jf = jacobi_factor(lap_type, list(g.h))
     for _ in range(nu1):
-        jacobi(g, lap_type, rhs, x, res, jf)
+        jacobi(g, lap_type, rhs, x, res, jf, bc=bc)
     coarsen = all(g.dim[i] % 2 == 0 and g.dim[i] >= 2 * g.ghosts for i in range(3))
     if level < cogr and coarsen:
-        trade(g, x)
+        trade(g, x, bc)
         res = lap(g, lap_type, x)
         res -= rhs
-        trade(g, res)
+        trade(g, res, bc)
         gc = Grid.make([d // 2 for d in g.dim], 1, [2 * v for v in g.h])
         rc = restrict(g, res, gc)
         uc = empty_block(gc, 1, x.dtype)
-        _vcycle(gc, _lower_order(lap_type), uc, rc, level + 1, cogr, nu1, nu2)
-        trade(gc, uc)
+        _vcycle(gc, _lower_order(lap_type), uc, rc, level + 1, cogr, nu1, nu2, bc)
+        trade(gc, uc, bc)
         p = extend(gc, uc, g)
         x -= p
     for _ in range(nu2):
-        jacobi(g, lap_type, rhs, x, res, jf)
+        jacobi(g, lap_type, rhs, x, res, jf, bc=bc)
 
 
-def poisson_mgm(g, lap_type, rho, vh, nu1=2, nu2=2, max_sweeps=10, tol=1e-16, max_nlevels=10):
+def poisson_mgm(g, lap_type, rho, vh, nu1=2, nu2=2, max_sweeps=10, tol=1e-16, max_nlevels=10, bc=(1, 1, 1)):
     """pb::Mgm (src/pb/Mgm.h:12-111) as SolverLap::solve calls it
-    (src/pb/SolverLap.cc:49-63), periodic: rhs = A.rhs(rho) (the Mehrstellen
-    B operator for Laph4M, src/pb/Laph4M.h:68-71; a copy otherwise,
-    src/pb/FDoper.h:68-73); up to max_sweeps times: res = rhs - A vh, stop when
-    |res| / |rhs| < tol, else vh += Vcycle(0, res); then vh -= mean(vh)
-    (GridFunc::average0).  rho, vh: ghosted (1, ...) blocks; vh is updated.
+    (src/pb/SolverLap.cc:49-63): rhs = A.rhs(rho) (the Mehrstellen B operator
+    for Laph4M, src/pb/Laph4M.h:68-71; a copy otherwise, src/pb/FDoper.h:68-73);
+    up to max_sweeps times: res = rhs - A vh, stop when |res| / |rhs| < tol,
+    else vh += Vcycle(0, res); then, on a fully periodic grid only,
+    vh -= mean(vh) (GridFunc::average0).  bc per direction: 1 periodic, 0 zero
+    boundary values.  rho, vh: ghosted (1, ...) blocks; vh is updated.
     Returns (converged, final_residual, final_relative_residual,
     residual_reduction, nb_sweeps)."""
     r = rho.copy()
-    trade(g, r)
+    trade(g, r, bc)
     rhs = rhs_mehr(g, r) if lap_type == 0 else r.copy()
-    inv_rhs_norm = 1.0 / gnorm(g, rhs)
+    inv_rhs_norm = 1.0 / gnorm(g, rhs, bc)
     converged, sweeps = False, 0
     init = final = rel = 1.0
     for i in range(max_sweeps):
-        trade(g, vh)
+        trade(g, vh, bc)
         lhs = lap(g, lap_type, vh)
         res = rhs - lhs
-        rn = gnorm(g, res)
+        rn = gnorm(g, res, bc)
         if i == 0:
             init = rn
         if rn * inv_rhs_norm < tol:
             final, rel, converged = rn, rn * inv_rhs_norm, True
             break
         w = np.zeros_like(vh)
-        _vcycle(g, lap_type, w, res, 0, max_nlevels, nu1, nu2)
+        _vcycle(g, lap_type, w, res, 0, max_nlevels, nu1, nu2, bc)
         sweeps += 1
         vh += w
     if not converged:
-        trade(g, vh)
+        trade(g, vh, bc)
         lhs = lap(g, lap_type, vh)
         lhs -= rhs
-        final = gnorm(g, lhs)
+        final = gnorm(g, lhs, bc)
         rel = final * inv_rhs_norm
-    vh -= np.mean(interior(g, vh))
+    if all(b == 1 for b in bc):
+        vh -= np.mean(interior(g, vh))
     return converged, final, rel, final / init, sweeps
 
 
@@ -699,7 +710,7 @@ def b_hloc_mehr(b, vglob, psi):
         e = int(b._edge[j])
         g = _box_grid(b, e)
         u = embed(g, psi[off[j]:off[j + 1]].reshape(1, e, e, e).astype(F64))
-        out[off[j]:off[j + 1]] = hloc(g, 0, u, _box_v(b, vglob, j, 0), bc=(0, 0, 0))[0]
+        out[off[j]:off[j + 1]] = hloc(g, 0, u, _box_v(b, vglob, j, 0), bc=BOX_OPEN)[0]
     return out
 
 
@@ -710,7 +721,7 @@ def b_bpsi_mehr(b, psi):
     for j in range(b.nbox):
         e = int(b._edge[j])
         g = _box_grid(b, e)
-        u = trade(g, embed(g, psi[off[j]:off[j + 1]].reshape(1, e, e, e).astype(F64)), bc=(0, 0, 0))
+        u = trade(g, embed(g, psi[off[j]:off[j + 1]].reshape(1, e, e, e).astype(F64)), bc=BOX_OPEN)
         out[off[j]:off[j + 1]] = rhs_mehr(g, u, 0).reshape(-1)
     return out
 
@@ -723,8 +734,8 @@ def _grown_ops(b, vglob, psi, j):
     full = np.zeros((1, e + 2, e + 2, e + 2))
     full[0, 1:e + 1, 1:e + 1, 1:e + 1] = psi[off[j]:off[j + 1]].reshape(e, e, e)
     u = embed(g, full)
-    hp = hloc(g, 0, u, _box_v(b, vglob, j, 1), bc=(0, 0, 0))[0].reshape(e + 2, e + 2, e + 2)
-    bp = rhs_mehr(g, trade(g, u.copy(), bc=(0, 0, 0)), 0)[0]
+    hp = hloc(g, 0, u, _box_v(b, vglob, j, 1), bc=BOX_OPEN)[0].reshape(e + 2, e + 2, e + 2)
+    bp = rhs_mehr(g, trade(g, u.copy(), bc=BOX_OPEN), 0)[0]
     return hp, bp
 
 

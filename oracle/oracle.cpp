@@ -61,10 +61,19 @@ inline int wrap(int i, int n)
 // ---------------------------------------------------------------------------
 // Halo fill.  GridFuncVector::trade_boundaries, single-task branches
 // (src/pb/GridFuncVector.cc:1544-1622, 585-604, 1052-1070, 1426-1440):
-// Dirichlet directions get zero ghosts (GridFunc::setBoundaryValues), periodic
-// directions copy Y then Z then X so that edges and corners hold the periodic
-// image.  The net value of every ghost point is the periodic image of the
-// interior point, or zero if a Dirichlet coordinate is out of range.
+// along a Dirichlet direction GridFunc::setBoundaryValues
+// (src/pb/GridFunc.cc:2189-2340) zeroes the ghosts on both sides AND the first
+// interior layer (index 0: the boundary plane of the vertex-centred grid, so
+// that the free points 1 .. n-1 are an odd number); periodic directions copy
+// Y then Z then X so that edges and corners hold the periodic image.  The net
+// value of every point is zero if a Dirichlet coordinate lies outside
+// 1 .. n-1, else (ghosts) the periodic image of the interior point.  The
+// reference's loops skip the index-0 layers of the other directions when a
+// grid mixes periodic and Dirichlet directions (a boundary row is left
+// unset); this restatement zeroes the whole layer, as for an all-Dirichlet
+// grid, where the two agree.  bc 3 (no counterpart in the reference's
+// GridFunc; used for a localized orbital's box cut out of the grid) gives
+// zero ghosts and leaves the interior as it is.
 template <typename T>
 int trade(const or_grid_t* gg, T* u, long nfunc, const int bc[3])
 {
@@ -84,6 +93,7 @@ int trade(const or_grid_t* gg, T* u, long nfunc, const int bc[3])
                     bool ghost = false, zero = false;
                     for (int k = 0; k < 3; k++)
                     {
+                        if (bc[k] == 0 && c[k] == 0) zero = true;
                         if (c[k] < 0 || c[k] >= g.d[k])
                         {
                             ghost = true;
@@ -93,7 +103,7 @@ int trade(const or_grid_t* gg, T* u, long nfunc, const int bc[3])
                                 zero = true;
                         }
                     }
-                    if (!ghost) continue;
+                    if (!ghost && !zero) continue;
                     long dst = (long)x * g.incx + (long)y * g.incy + z;
                     uf[dst]  = zero ? (T)0 : uf[g.idx(c[0], c[1], c[2])];
                 }

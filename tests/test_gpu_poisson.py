@@ -1,8 +1,9 @@
 """GPU parity of the Hartree multigrid (mgx_poisson_mg: pb::Mgm + pb::Vcycle)
-against the oracle's restatement on periodic grids, through the C ABI, for the
-Mehrstellen, 2nd- and 4th-order operators: fixed numbers of V-cycles (1e-12
-relative to max |vh|; FP64 sums in another order), a converged solve (the
-stopping rule and sweep count), and the argument checks."""
+against the oracle's restatement, through the C ABI, for the Mehrstellen,
+2nd- and 4th-order operators: fixed numbers of V-cycles on periodic and on
+Dirichlet / mixed grids (1e-12 relative to max |vh|; FP64 sums in another
+order), a converged solve (the stopping rule and sweep count), and the
+argument checks."""
 import numpy as np
 import pytest
 
@@ -19,10 +20,10 @@ def ctx():
     return shared()
 
 
-def _case(lap_type, dims, h=0.4, seed=0):
+def _case(lap_type, dims, h=0.4, seed=0, bc=(1, 1, 1)):
     ng = 2 if lap_type == 2 else 1
     og = O.Grid.make(dims, ng, [h] * 3)
-    mg_ = mgx.Grid.make(dims, ng, [h] * 3)
+    mg_ = mgx.Grid.make(dims, ng, [h] * 3, bc=bc)
     rng = np.random.default_rng(seed)
     rho = rng.standard_normal((1,) + tuple(dims))
     rho -= rho.mean()
@@ -57,6 +58,28 @@ def test_fixed_cycles_match_oracle(ctx, lap_type, dims):
     assert np.array_equal(g, got)
 
 
+@pytest.mark.parametrize("lap_type,dims,bc", [(0, (32, 32, 32), (0, 0, 0)), (1, (16, 24, 32), (1, 0, 1)),
+                                             (2, (32, 16, 16), (0, 1, 1)), (0, (16, 32, 24), (1, 1, 0))])
+def test_fixed_cycles_dirichlet_match_oracle(ctx, lap_type, dims, bc):
+    og, mg_, rho = _case(lap_type, dims, seed=5, bc=bc)
+    rng = np.random.default_rng(2)
+    vh0 = O.embed(og, 0.1 * rng.standard_normal((1,) + tuple(dims)))
+    got, st = _run(ctx, mg_, rho, vh0, lap_type, max_sweeps=3, tol=0.0)
+    ref = vh0.copy()
+    conv, fin, rel, red, sweeps = O.poisson_mgm(og, lap_type, rho, ref, max_sweeps=3, tol=0.0, bc=bc)
+    err = relmax(O.interior(og, got), O.interior(og, ref))
+    say("poisson type %d dims %s bc %s: rel err %.3e, residual %.3e vs %.3e" % (lap_type, dims, bc, err,
+                                                                                st.final_residual, fin))
+    assert err <= 1e-12
+    assert st.nb_sweeps == 3 and not st.converged
+    assert st.final_residual == pytest.approx(fin, rel=1e-9)
+    assert st.residual_reduction == pytest.approx(red, rel=1e-9)
+    # ghosts refreshed, the boundary planes zero, no mean removed
+    g = O.trade(og, got.copy(), bc)
+    assert np.array_equal(g, got)
+    assert abs(float(np.mean(O.interior(og, got)))) > 1e-6 * np.max(np.abs(got))
+
+
 def test_converged_solve(ctx):
     og, mg_, rho = _case(0, (32, 32, 32), seed=3)
     vh0 = O.empty_block(og, 1)
@@ -78,7 +101,7 @@ def test_rejects_bad_arguments(ctx):
     # 4th order needs 2 ghosts
     with pytest.raises(mgx.MgxError):
         ctx.call("mgx_poisson_mg", 2, mgx.block(r1, g1), mgx.block(r1, g1), 2, 2, 1, 0.0, 10, st)
-    # non-periodic grid
-    gd = mgx.Grid.make((16, 16, 16), 1, [0.4] * 3, bc=(0, 1, 1))
+    # multipole boundary values (bc 2) are not built
+    gd = mgx.Grid.make((16, 16, 16), 1, [0.4] * 3, bc=(2, 1, 1))
     with pytest.raises(mgx.MgxError):
         ctx.call("mgx_poisson_mg", 0, mgx.block(r1, gd), mgx.block(r1, gd), 2, 2, 1, 0.0, 10, st)

@@ -162,8 +162,12 @@ def test_trade_periodic_and_dirichlet():
     blk = O.trade(g, O.embed(g, u))
     full = np.pad(u, ((0, 0), (2, 2), (2, 2), (2, 2)), mode="wrap")
     assert np.array_equal(blk, full)
+    # Dirichlet y (GridFunc::setBoundaryValues, src/pb/GridFunc.cc:2189-2340):
+    # the ghosts and the boundary plane y = 0 hold zero
     blk2 = O.trade(g, O.embed(g, u), bc=(1, 0, 1))
-    exp = np.pad(u, ((0, 0), (2, 2), (0, 0), (2, 2)), mode="wrap")
+    uz = u.copy()
+    uz[:, :, 0, :] = 0.0
+    exp = np.pad(uz, ((0, 0), (2, 2), (0, 0), (2, 2)), mode="wrap")
     exp = np.pad(exp, ((0, 0), (0, 0), (2, 2), (0, 0)))
     assert np.array_equal(blk2, exp)
 
