@@ -307,6 +307,50 @@ __global__ void k_fill_odd(T* uf, GGeom gf, int64_t total)
     }
 }
 
+// The coarse correction of a V-cycle in one pass: x -= mask * P(uc) on the
+// fine interior, P the MGkernelExtend3D interpolation (src/pb/MGkernels.cc:
+// 85-195) formed from the coarse values directly (the fine even points are
+// the injected coarse points, so every term and its order is that of
+// k_inject + k_fill_odd), the subtraction as k_axpy with alpha = -1.  One
+// block row per fine (function, x) plane; uc must have fresh ghosts.
+template <typename T>
+__global__ void __launch_bounds__(256) k_prolong_sub(const T* __restrict__ uc, GGeom gc, T* __restrict__ xf,
+    GGeom gf, const uint8_t* __restrict__ mask)
+{
+    const int plane = blockIdx.y;  // f * d0 + x
+    const int f = plane / gf.d0, x = plane - (plane / gf.d0) * gf.d0;
+    const int np = gf.d1 * gf.d2;
+    const int p  = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int y = p / gf.d2, z = p - (p / gf.d2) * gf.d2;
+    const int64_t t = ((int64_t)f * gf.d0 + x) * np + p;
+    if (mask && !mask[t]) return;
+    const int ox = x & 1, oy = y & 1, oz = z & 1;
+    const T* c = uc + (int64_t)f * gc.sizeg + (int64_t)((x >> 1) + gc.ng) * gc.incx
+                 + (int64_t)((y >> 1) + gc.ng) * gc.incy + (z >> 1) + gc.ng;
+    // fine +-1 along an odd direction: coarse +1 / +0
+    const int64_t X = gc.incx, Y = gc.incy;
+    T v;
+    if (!ox && !oy && !oz)
+        v = c[0];
+    else if (!ox && !oy && oz)
+        v = (T)(0.5 * (T)(c[0] + c[1]));
+    else if (!ox && oy && !oz)
+        v = (T)(0.5 * (T)(c[Y] + c[0]));
+    else if (!ox && oy && oz)
+        v = (T)(0.25 * (T)(c[1 + Y] + c[1] + c[Y] + c[0]));
+    else if (ox && !oy && oz)
+        v = (T)(0.25 * (T)(c[X + 1] + c[X] + c[1] + c[0]));
+    else if (ox && !oy && !oz)
+        v = (T)(0.5 * (T)(c[X] + c[0]));
+    else if (ox && oy && oz)
+        v = (T)(0.125 * (T)(c[X + Y + 1] + c[X + Y] + c[X + 1] + c[X] + c[Y + 1] + c[Y] + c[1] + c[0]));
+    else
+        v = (T)(0.25 * (T)(c[X + Y] + c[X] + c[Y] + c[0]));
+    T* xp = xf + (int64_t)f * gf.sizeg + (int64_t)(x + gf.ng) * gf.incx + (int64_t)(y + gf.ng) * gf.incy + z + gf.ng;
+    *xp += static_cast<T>(-1. * static_cast<double>(v));
+}
+
 template <typename T>
 __global__ void k_mask(T* u, const uint8_t* m, GGeom g, int64_t total)
 {
@@ -1070,6 +1114,23 @@ int launch_extend(mgx_ctx* ctx, const T* uc, const mgx_grid_t& gc, T* uf, const 
     return MGX_OK;
 }
 
+// x -= mask * P(uc) on the fine interior (uc with fresh ghosts), in place of
+// launch_extend + launch_mask + launch_axpy
+template <typename T>
+int launch_prolong_sub(mgx_ctx* ctx, const T* uc, const mgx_grid_t& gc, T* xf, const mgx_grid_t& gf, int nfunc,
+    const uint8_t* mask)
+{
+    for (int i = 0; i < 3; i++)
+        MGX_REQUIRE(ctx, gf.dim[i] == 2 * gc.dim[i], "fine grid is not twice the coarse grid");
+    MGX_REQUIRE(ctx, gf.ghosts >= 1 && gc.ghosts >= 1, "prolongation needs ghosts");
+    const GGeom a = ggeom(gc), b = ggeom(gf);
+    const int np = gf.dim[1] * gf.dim[2];
+    const dim3 grid((unsigned)((np + 255) / 256), (unsigned)(nfunc * gf.dim[0]));
+    hipLaunchKernelGGL(k_prolong_sub<T>, grid, dim3(256), 0, ctx->stream, uc, a, xf, b, mask);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
 template <typename T>
 int launch_mask(mgx_ctx* ctx, T* u, const uint8_t* m, int nfunc, const mgx_grid_t& gr)
 {
@@ -1191,9 +1252,7 @@ int mg_rec(mgx_ctx* ctx, MGBufs<T>& B, int max_levels, T* v, const T* f, int typ
     if ((rc = launch_fill(ctx, nv, (int64_t)nfunc * ggeom(gc).sizeg, (T)0))) return rc;
     if ((rc = mg_rec(ctx, B, max_levels, nv, rcoarse, MGX_LAP_2ND, level + 1, nfunc, masks))) return rc;
     if ((rc = launch_halo(ctx, nv, nfunc, gc))) return rc;
-    if ((rc = launch_extend(ctx, nv, gc, w, g, nfunc))) return rc;
-    if ((rc = launch_mask(ctx, w, mk, nfunc, g))) return rc;
-    if ((rc = launch_axpy(ctx, (int64_t)nfunc * ggeom(g).sizeg, -1., (const T*)w, v))) return rc;
+    if ((rc = launch_prolong_sub(ctx, (const T*)nv, gc, v, g, nfunc, mk))) return rc;
     for (int it = 0; it < 2; it++)
         if ((rc = sweep(false))) return rc;
     if ((rc = settle())) return rc;
@@ -1367,8 +1426,8 @@ int poisson_vcycle(mgx_ctx* ctx, PoissonMG& P, int l, double* x, const double* f
         if ((rc = launch_fill(ctx, P.x[l + 1], ggeom(gc).sizeg, 0.))) return rc;
         if ((rc = poisson_vcycle(ctx, P, l + 1, P.x[l + 1], P.f[l + 1], nu1, nu2))) return rc;
         if ((rc = launch_halo(ctx, P.x[l + 1], 1, gc))) return rc;
-        if ((rc = launch_extend(ctx, (const double*)P.x[l + 1], gc, res, g, 1))) return rc;
-        if ((rc = launch_axpy(ctx, n, -1., (const double*)res, x))) return rc;
+        if ((rc = launch_prolong_sub(ctx, (const double*)P.x[l + 1], gc, x, g, 1, (const uint8_t*)nullptr)))
+            return rc;
     }
     return sweeps(nu2);
 }

@@ -36,11 +36,15 @@ from . import learned, mgx, synth
 
 
 def _timeit(fn, reps):
+    """median wall time of reps calls (each entry point waits for its own
+    kernels), after one warm-up call"""
     fn()
-    t0 = time.perf_counter()
+    ts = []
     for _ in range(reps):
+        t0 = time.perf_counter()
         fn()
-    return (time.perf_counter() - t0) / reps
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
 
 
 def _cpu_info():
@@ -83,11 +87,11 @@ def c1(ctx, cpu=True):
     vel = h ** 3
     gam = mgx.lib().mgx_gamma((C.c_double * 3)(h, h, h), 2, float(np.max(v)), -1.0)
     t_h = _timeit(lambda: ctx.check(L.mgx_hloc_apply(ctx.h, mgx.LAP_4TH, C.byref(grid), mgx.F64, mgx.ptr(dp), K, n,
-                                                     mgx.ptr(dv), mgx.ptr(dh), K), "hloc"), 10)
+                                                     mgx.ptr(dv), mgx.ptr(dh), K), "hloc"), 30)
     t_p = _timeit(lambda: ctx.check(L.mgx_precond_mg(ctx.h, mgx.LAP_4TH, 2, gam, mgx.F64, C.byref(grid), mgx.ptr(dh),
-                                                     K, n, None, mgx.ptr(dr), K), "precond"), 5)
+                                                     K, n, None, mgx.ptr(dr), K), "precond"), 15)
     t_s = _timeit(lambda: ctx.check(L.mgx_local_gram(ctx.h, mgx.PREC_F64, mgx.ptr(dp), K, K, n, 1, vel,
-                                                     mgx.ptr(dS)), "gram"), 10)
+                                                     mgx.ptr(dS)), "gram"), 30)
     bh = 16.0 * n * K + 8.0 * K
     bp = 18.5 * 8.0 * n * K
     fs = K * n * (n + 1.0)
