@@ -828,6 +828,15 @@ def _anderson_theta(mat, rhs):
     return th
 
 
+def _anderson_update(x, x0, f, f0, t, beta):
+    """AndersonMix::update with one history vector (src/AndersonMix.cc):
+    the extrapolated point xb = (1 - t) x + t x0 and direction
+    fb = (1 - t) f + t f0, the next iterate xb + beta fb."""
+    xb = (1.0 - t) * x + t * x0
+    fb = (1.0 - t) * f + t * f0
+    return xb + beta * fb
+
+
 class BSolver:
     """The orbital iteration with DFTsolver's energy bookkeeping and ABPG's
     optional Anderson extrapolation (m <= 1), stateful like the engine's
@@ -884,10 +893,9 @@ class BSolver:
                 df = f - self.f0
                 t = _anderson_theta(self._dot(df, df, Sinv), self._dot(df, f, Sinv))
                 beta = self.beta
-            xb = (1.0 - t) * psi + t * (self.x0 if self.mm > 0 else psi)
-            fb = (1.0 - t) * f + t * (self.f0 if self.mm > 0 else f)
+            nxt = _anderson_update(psi, self.x0 if self.mm > 0 else psi, f, self.f0 if self.mm > 0 else f, t, beta)
             self.x0, self.f0 = psi.copy(), f.copy()
-            psi = xb + beta * fb
+            psi = nxt
             self.mm = min(self.mm + 1, self.m)
         d = b_diag_dots(b, psi)
         off = b.offsets(0)

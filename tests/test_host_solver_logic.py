@@ -33,6 +33,35 @@ def test_anderson_single_history_is_secant_minimiser():
         assert th == pytest.approx(best, abs=2e-4)
 
 
+def test_anderson_update_is_a_step_from_the_extrapolated_point():
+    """For a linear residual map f(x) = alpha (b - A x) the combination of
+    AndersonMix::update is one preconditioned step taken from the
+    extrapolated point: xb + beta f(xb), with xb = (1 - t) x + t x0 (fb =
+    f(xb) by linearity); and the secant theta makes |f(xb)| no larger than
+    |f(x)| (t = 0 is a candidate of the minimisation)."""
+    rng = np.random.default_rng(3)
+    n = 40
+    M = rng.standard_normal((n, n))
+    A = M @ M.T / n + np.eye(n)
+    b = rng.standard_normal(n)
+    alpha = 0.2
+
+    def fmap(x):
+        return alpha * (b - A @ x)
+    x0 = rng.standard_normal(n)
+    x = x0 + fmap(x0)
+    f, f0 = fmap(x), fmap(x0)
+    df = f - f0
+    t = O._anderson_theta(float(df @ df), float(df @ f))
+    for beta in (1.0, 0.5):
+        nxt = O._anderson_update(x, x0, f, f0, t, beta)
+        xb = (1 - t) * x + t * x0
+        assert np.allclose(nxt, xb + beta * fmap(xb), rtol=1e-12, atol=1e-12)
+    assert np.linalg.norm(fmap((1 - t) * x + t * x0)) <= np.linalg.norm(f) * (1 + 1e-12)
+    # without history (x0 = x, f0 = f) the update is the plain step x + beta f
+    assert np.array_equal(O._anderson_update(x, x, f, f, 0.0, 1.0), x + f)
+
+
 def test_centres_min_separation():
     rng = np.random.default_rng(5)
     L = np.array([20.0, 20.0, 20.0])
