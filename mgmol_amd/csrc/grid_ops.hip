@@ -1348,6 +1348,119 @@ __global__ void k_add_const(double* u, int64_t n, const double* s, double scale)
         u[t] += v;
 }
 
+// ------------------------------------------- Poisson-Boltzmann (PBh4M) --
+// interior point t of a grid -> offset in the ghosted layout
+__device__ __forceinline__ int64_t gpoint(const GGeom& g, int64_t t, int* xo = nullptr, int* yo = nullptr,
+    int* zo = nullptr)
+{
+    int r       = (int)t;
+    const int x = r / (g.d1 * g.d2);
+    r -= x * g.d1 * g.d2;
+    const int y = r / g.d2;
+    const int z = r - y * g.d2;
+    if (xo)
+    {
+        *xo = x;
+        *yo = y;
+        *zo = z;
+    }
+    return (int64_t)(x + g.ng) * g.incx + (int64_t)(y + g.ng) * g.incy + z + g.ng;
+}
+
+// DielFunc::epsilon_rho (src/pb/DielFunc.cc:38-43) at every interior point,
+// then sqrt_a = sqrt(eps) and inv_sqrt_a = 1 / sqrt_a (GridFunc::sqrt_func,
+// GridFunc::inv)
+__global__ void k_pb_eps(const double* rhod, GGeom g, double e0, double rho0, double drho0, double* sa, double* isa)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i    = gpoint(g, t);
+        const double ratio = pow(rhod[i] / rho0, 2. * drho0);
+        const double eps   = 1. + 0.5 * (e0 - 1.) * ((1. - ratio) / (1. + ratio) + 1.);
+        const double s     = sqrt(eps);
+        sa[i]              = s;
+        isa[i]             = 1. / s;
+    }
+}
+
+// pp = (-1 * L) * inv_sqrt_a, L = -Lap_4 sqrt_a (PBh4M<T>::init)
+__global__ void k_pb_pp(const double* L, const double* isa, GGeom g, double* pp)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        pp[i]           = (-1. * L[i]) * isa[i];
+    }
+}
+
+// out += B(pp x) (PBh4M<T>::apply: the product on the 7 points, then
+// FDkernelRHS_4th_Mehr1's arithmetic as in k_rhs); x and pp with fresh ghosts
+__global__ void k_pb_bterm(const double* x, const double* pp, GGeom g, double* out)
+{
+    const int64_t X = g.incx, Y = g.incy;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        const double* a = x + i;
+        const double* q = pp + i;
+        const double b  = 0.5 * (q[0] * a[0])
+                         + kInv12 * ((((((q[-X] * a[-X]) + (q[X] * a[X])) + (q[-Y] * a[-Y])) + (q[Y] * a[Y]))
+                                        + (q[-1] * a[-1]))
+                                       + (q[1] * a[1]));
+        out[i] = out[i] + b;
+    }
+}
+
+// PBh4M<T>::jacobi update: x -= (0.75 / (c0 + pp)) (w - f), w = A x
+__global__ void k_pb_jacobi(double* x, const double* w, const double* f, const double* pp, GGeom g, double c0)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        x[i] -= (0.75 / (c0 + pp[i])) * (w[i] - f[i]);
+    }
+}
+
+// interior out = a * b (mode 0) or a / b (mode 1)
+__global__ void k_pb_scale(const double* a, const double* b, GGeom g, int mode, double* out)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        out[i]          = mode == 0 ? a[i] * b[i] : a[i] / b[i];
+    }
+}
+
+// PBh4M<T>::get_vepsilon (src/pb/PBh4M.cc:138-162): vks = depsilon(rhod)
+// |grad vh|^2 (-1/(8 pi)), the gradient by FDoper::del1_4th
+// (src/pb/FDoper.cc:119-178); vh with fresh ghosts (2)
+__global__ void k_pb_vks(const double* vh, const double* rhod, GGeom g, double h0, double h1, double h2, double e0,
+    double rho0, double drho0, double* out)
+{
+    const int64_t inc[3] = { g.incx, g.incy, 1 };
+    const double hh[3]   = { h0, h1, h2 };
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        double g2       = 0.;
+        for (int d = 0; d < 3; d++)
+        {
+            const double e1 = (8. / 12.) * (1. / hh[d]), e2 = (1. / 12.) * (1. / hh[d]);
+            const int64_t c = inc[d];
+            const double dv = e1 * (vh[i + c] - vh[i - c]) + e2 * (vh[i - 2 * c] - vh[i + 2 * c]);
+            g2 += dv * dv;
+        }
+        const double r = rhod[i];
+        double de      = 0.;
+        if (!(r < 0.001 * rho0))
+        {
+            const double inv = 1. / rho0, ratio = r * inv, pr = pow(ratio, 2. * drho0);
+            de = (2. * inv) * (1. - e0) * drho0 * pow(ratio, 2. * drho0 - 1.) / ((1. + pr) * (1. + pr));
+        }
+        out[i] = (de * g2) * (-1. / (8. * M_PI));
+    }
+}
+
 struct PoissonMG
 {
     std::vector<mgx_grid_t> grids;
@@ -1547,6 +1660,212 @@ int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho,
         MGX_CHECK_LAUNCH(ctx);
     }
     if ((rc = launch_halo(ctx, vh, 1, g0))) return rc;
+    if (st)
+    {
+        st->converged               = conv;
+        st->nb_sweeps               = sweeps;
+        st->final_residual          = fin;
+        st->final_relative_residual = rel;
+        st->residual_reduction      = fin / init;
+    }
+    return MGX_OK;
+}
+
+// SolverPB<PBh4M>::solve (src/pb/SolverPB.cc:56-84): the dielectric
+// operator's Mgm on the transformed potential, then get_vepsilon
+struct PBLevel
+{
+    mgx_grid_t g;
+    double *pp, *x, *f, *w, *t;
+};
+
+// the PBh4M operator at level l: out = -Lap_M x + B(pp x) (x gets its halo)
+int pb_apply(mgx_ctx* ctx, const PBLevel& L, double* x, double* out)
+{
+    int rc;
+    if ((rc = launch_halo(ctx, x, 1, L.g))) return rc;
+    if ((rc = launch_lap(ctx, MGX_LAP_MEHRSTELLEN, (const double*)x, out, 1, L.g))) return rc;
+    const GGeom a = ggeom(L.g);
+    hipLaunchKernelGGL(k_pb_bterm, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)x,
+        (const double*)L.pp, a, out);
+    MGX_CHECK_LAUNCH(ctx);
+    return MGX_OK;
+}
+
+int pb_vcycle(mgx_ctx* ctx, std::vector<PBLevel>& Lv, PoissonMG& P, int l, double* x, const double* f, int nu1,
+    int nu2)
+{
+    const PBLevel& L = Lv[l];
+    const GGeom a    = ggeom(L.g);
+    const double c0  = (30. / 12.) * (1. / (L.g.h[0] * L.g.h[0]) + 1. / (L.g.h[1] * L.g.h[1]) +
+                                       1. / (L.g.h[2] * L.g.h[2]));
+    int rc;
+    auto sweeps = [&](int k) -> int {
+        for (int i = 0; i < k; i++)
+        {
+            int r = pb_apply(ctx, L, x, L.w);
+            if (r) return r;
+            hipLaunchKernelGGL(k_pb_jacobi, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, x,
+                (const double*)L.w, f, (const double*)L.pp, a, c0);
+            MGX_CHECK_LAUNCH(ctx);
+        }
+        return MGX_OK;
+    };
+    if ((rc = sweeps(nu1))) return rc;
+    if (l + 1 < (int)Lv.size())
+    {
+        const PBLevel& C = Lv[l + 1];
+        if ((rc = pb_apply(ctx, L, x, L.w))) return rc;
+        hipLaunchKernelGGL(k_diff, dim3(nblocks(a.sizeg)), dim3(kBlock), 0, ctx->stream, (const double*)L.w, f, L.t,
+            a.sizeg);
+        MGX_CHECK_LAUNCH(ctx);
+        if ((rc = launch_halo(ctx, L.t, 1, L.g))) return rc;
+        if ((rc = launch_restrict(ctx, (const double*)L.t, L.g, C.f, C.g, 1))) return rc;
+        if ((rc = launch_fill(ctx, C.x, ggeom(C.g).sizeg, 0.))) return rc;
+        if ((rc = pb_vcycle(ctx, Lv, P, l + 1, C.x, C.f, nu1, nu2))) return rc;
+        if ((rc = launch_halo(ctx, C.x, 1, C.g))) return rc;
+        if ((rc = launch_prolong_sub(ctx, (const double*)C.x, C.g, x, L.g, 1, (const uint8_t*)nullptr))) return rc;
+    }
+    return sweeps(nu2);
+}
+
+int pb_run(mgx_ctx* ctx, const mgx_grid_t& g0, const double* rho, const double* rhod, double* vh, double* vks,
+    double e0, double rho0, double drho0, int nu1, int nu2, int max_sweeps, double tol, int max_nlevels,
+    mgx_poisson_stats_t* st)
+{
+    // levels: the fine grid, then while coarsenable (every dimension even and
+    // at least twice the ghost width) grids of half the points with one ghost
+    std::vector<PBLevel> Lv(1);
+    Lv[0].g = g0;
+    while ((int)Lv.size() - 1 < max_nlevels)
+    {
+        const mgx_grid_t& p = Lv.back().g;
+        bool ok = true;
+        for (int i = 0; i < 3; i++)
+            ok = ok && p.dim[i] % 2 == 0 && p.dim[i] >= 2 * p.ghosts;
+        if (!ok) break;
+        PBLevel c;
+        c.g        = coarse_of(p);
+        c.g.ghosts = 1;
+        Lv.push_back(c);
+    }
+    const int nl     = (int)Lv.size();
+    const int64_t n0 = ggeom(g0).sizeg;
+    // fine: sa, isa, rhs, r0, corr + the level arrays (pp, w, t; x, f coarse)
+    size_t total = 5 * (size_t)n0 + kPartBlocks + 8;
+    for (int l = 0; l < nl; l++)
+        total += (size_t)ggeom(Lv[l].g).sizeg * (l == 0 ? 3 : 5);
+    double* base = (double*)workspace(ctx, 0, total * sizeof(double));
+    if (!base) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    double* sa   = base;
+    double* isa  = sa + n0;
+    double* rhs  = isa + n0;
+    double* r0   = rhs + n0;
+    double* corr = r0 + n0;
+    double* q    = corr + n0;
+    PoissonMG P;
+    P.part = q;
+    q += kPartBlocks + 8;
+    for (int l = 0; l < nl; l++)
+    {
+        const int64_t m = ggeom(Lv[l].g).sizeg;
+        Lv[l].pp        = q;
+        q += m;
+        Lv[l].w = q;
+        q += m;
+        Lv[l].t = q;
+        q += m;
+        if (l == 0)
+        {
+            Lv[l].x = Lv[l].f = nullptr;
+            continue;
+        }
+        Lv[l].x = q;
+        q += m;
+        Lv[l].f = q;
+        q += m;
+    }
+    // the whole workspace starts at zero (ghosts the kernels do not write)
+    if (int rc = launch_fill(ctx, base, (int64_t)total, 0.)) return rc;
+    mgx_grid_t gp = g0;  // the dielectric functions are periodic
+    gp.bc[0] = gp.bc[1] = gp.bc[2] = 1;
+    const GGeom a = ggeom(g0);
+    int rc;
+    // PBh4M<T>::init: eps, sqrt_a, inv_sqrt_a, pp = (-(-Lap_4 sqrt_a)) inv_sqrt_a
+    hipLaunchKernelGGL(k_pb_eps, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rhod, a, e0, rho0, drho0, sa, isa);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = launch_halo(ctx, sa, 1, gp)) || (rc = launch_halo(ctx, isa, 1, gp))) return rc;
+    if ((rc = launch_lap(ctx, MGX_LAP_4TH, (const double*)sa, Lv[0].t, 1, gp))) return rc;
+    hipLaunchKernelGGL(k_pb_pp, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)Lv[0].t,
+        (const double*)isa, a, Lv[0].pp);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = launch_halo(ctx, Lv[0].pp, 1, gp))) return rc;
+    // the coarse operators' pp: full-weighting restriction of the finer pp
+    // (traded periodically first), with periodic ghosts
+    for (int l = 1; l < nl; l++)
+    {
+        mgx_grid_t cp = Lv[l].g;
+        cp.bc[0] = cp.bc[1] = cp.bc[2] = 1;
+        if ((rc = launch_restrict(ctx, (const double*)Lv[l - 1].pp, l == 1 ? gp : Lv[l - 1].g, Lv[l].pp, Lv[l].g, 1)))
+            return rc;
+        if ((rc = launch_halo(ctx, Lv[l].pp, 1, cp))) return rc;
+    }
+    // Mgm: vh /= inv_sqrt_a; rhs = B(rho inv_sqrt_a)
+    hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh,
+        (const double*)isa, a, 1, vh);
+    hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rho, (const double*)isa, a, 0,
+        r0);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = launch_halo(ctx, r0, 1, g0))) return rc;
+    hipLaunchKernelGGL(k_rhs<double>, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)r0, rhs, a,
+        a, a.size);
+    MGX_CHECK_LAUNCH(ctx);
+    double rhs_norm = 0.;
+    if ((rc = int_norm(ctx, P, rhs, g0, &rhs_norm))) return rc;
+    MGX_REQUIRE(ctx, rhs_norm > 0., "zero right-hand side");
+    const double inv_rhs_norm = 1. / rhs_norm;
+    double init = 1., fin = 1., rel = 1.;
+    int conv = 0, sweeps = 0;
+    double* lhs = Lv[0].t;
+    for (int i = 0; i < max_sweeps; i++)
+    {
+        if ((rc = pb_apply(ctx, Lv[0], vh, lhs))) return rc;
+        hipLaunchKernelGGL(k_diff, dim3(nblocks(n0)), dim3(kBlock), 0, ctx->stream, (const double*)rhs,
+            (const double*)lhs, r0, n0);
+        MGX_CHECK_LAUNCH(ctx);
+        double rn = 0.;
+        if ((rc = int_norm(ctx, P, r0, g0, &rn))) return rc;
+        if (i == 0) init = rn;
+        if (rn * inv_rhs_norm < tol)
+        {
+            fin  = rn;
+            rel  = rn * inv_rhs_norm;
+            conv = 1;
+            break;
+        }
+        if ((rc = launch_fill(ctx, corr, n0, 0.))) return rc;
+        if ((rc = pb_vcycle(ctx, Lv, P, 0, corr, r0, nu1, nu2))) return rc;
+        sweeps++;
+        if ((rc = launch_axpy(ctx, n0, 1., (const double*)corr, vh))) return rc;
+    }
+    if (!conv)
+    {
+        if ((rc = pb_apply(ctx, Lv[0], vh, lhs))) return rc;
+        if ((rc = launch_axpy(ctx, n0, -1., (const double*)rhs, lhs))) return rc;
+        if ((rc = int_norm(ctx, P, lhs, g0, &fin))) return rc;
+        rel = fin * inv_rhs_norm;
+    }
+    // vh *= inv_sqrt_a, fresh ghosts; get_vepsilon
+    hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh,
+        (const double*)isa, a, 0, vh);
+    MGX_CHECK_LAUNCH(ctx);
+    if ((rc = launch_halo(ctx, vh, 1, g0))) return rc;
+    if (vks)
+    {
+        hipLaunchKernelGGL(k_pb_vks, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh, rhod, a,
+            g0.h[0], g0.h[1], g0.h[2], e0, rho0, drho0, vks);
+        MGX_CHECK_LAUNCH(ctx);
+    }
     if (st)
     {
         st->converged               = conv;
@@ -1835,6 +2154,28 @@ int mgx_poisson_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, mgx_block
     MGX_REQUIRE(ctx, nu1 >= 0 && nu2 >= 0 && max_sweeps >= 0 && max_nlevels >= 0, "bad cycle parameters");
     rc = poisson_run(ctx, lap_type, vh->grid, (const double*)rho->data, (double*)vh->data, nu1, nu2, max_sweeps, tol,
         max_nlevels, stats);
+    return rc ? rc : sync(ctx);
+}
+
+int mgx_pb_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, const mgx_block_t* rhod, mgx_block_t* vh,
+    mgx_block_t* vks, double e0, double rho0, double drho0, int nu1, int nu2, int max_sweeps, double tol,
+    int max_nlevels, mgx_poisson_stats_t* stats)
+{
+    int rc = check_block(ctx, rho);
+    if (!rc) rc = check_block(ctx, rhod);
+    if (!rc) rc = check_block(ctx, vh);
+    if (!rc && vks) rc = check_block(ctx, vks);
+    if (rc) return rc;
+    MGX_REQUIRE(ctx, lap_type == MGX_LAP_MEHRSTELLEN,
+        "the dielectric operator is built for the Mehrstellen discretisation (PBh4M) only");
+    MGX_REQUIRE(ctx, same_shape(rho, vh) && same_shape(rhod, vh) && (!vks || same_shape(vks, vh)),
+        "rho/rhod/vh/vks blocks differ in shape");
+    MGX_REQUIRE(ctx, vh->dtype == MGX_F64 && vh->nfunc == 1, "one FP64 function");
+    MGX_REQUIRE(ctx, vh->grid.ghosts >= 2, "Not enough ghosts points in PBh4M (2 needed)");
+    MGX_REQUIRE(ctx, e0 >= 1. && rho0 > 0. && drho0 > 0., "bad dielectric parameters");
+    MGX_REQUIRE(ctx, nu1 >= 0 && nu2 >= 0 && max_sweeps >= 0 && max_nlevels >= 0, "bad cycle parameters");
+    rc = pb_run(ctx, vh->grid, (const double*)rho->data, (const double*)rhod->data, (double*)vh->data,
+        vks ? (double*)vks->data : nullptr, e0, rho0, drho0, nu1, nu2, max_sweeps, tol, max_nlevels, stats);
     return rc ? rc : sync(ctx);
 }
 

@@ -31,7 +31,7 @@ EXPORTS = [
     "mgx_halo", "mgx_lap_apply", "mgx_rhs_apply", "mgx_pointwise_product", "mgx_axpy", "mgx_stream_triad", "mgx_synth_separable",
     "mgx_jacobi",
     "mgx_restrict", "mgx_prolong", "mgx_mask", "mgx_mg",
-    "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg",
+    "mgx_hloc_apply", "mgx_precond_mg", "mgx_gamma", "mgx_poisson_mg", "mgx_pb_mg",
     "mgx_local_gram", "mgx_local_gram_noalloc", "mgx_local_product", "mgx_local_to_global", "mgx_chol_inv", "mgx_theta", "mgx_eigsum",
     "mgx_ritz", "mgx_residual", "mgx_residual_norm", "mgx_mehr_bmatrix", "mgx_mehr_projected", "mgx_mehr_residual",
     "mgx_boxset_create", "mgx_boxset_destroy", "mgx_boxset_autotune", "mgx_boxset_set_tune", "mgx_boxset_get_tune", "mgx_boxset_size", "mgx_boxset_offset", "mgx_boxset_npairs",
@@ -115,6 +115,7 @@ def lib():
             "mgx_jacobi": (P, I, D, BP, BP, BP), "mgx_restrict": (P, BP, BP), "mgx_prolong": (P, BP, BP),
             "mgx_mask": (P, BP, P), "mgx_mg": (P, I, I, BP, BP, PP),
             "mgx_poisson_mg": (P, I, BP, BP, I, I, I, D, I, C.POINTER(PoissonStats)),
+            "mgx_pb_mg": (P, I, BP, BP, BP, BP, D, D, D, I, I, I, D, I, C.POINTER(PoissonStats)),
             "mgx_hloc_apply": (P, I, GP, I, P, I64, I, P, P, I64),
             "mgx_precond_mg": (P, I, I, D, I, GP, P, I64, I, PP, P, I64),
             "mgx_local_gram": (P, I, P, I64, I64, I, I, D, P),

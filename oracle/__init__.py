@@ -502,6 +502,157 @@ def mehr_residual(g, psi, hpsi, th, bc=(1, 1, 1)):
     return residual(mehr_bpsi(g, psi, bc), hpsi, th)
 
 
+# ------------------------------------------------ Poisson-Boltzmann (dielectric)
+def pb_epsilon(rhod, e0, rho0, drho0):
+    """DielFunc::epsilon_rho (src/pb/DielFunc.cc:38-43): the dielectric
+    function of the electronic density, 1 + (e0 - 1)/2 ((1 - r)/(1 + r) + 1),
+    r = (rho/rho0)^(2 drho0)."""
+    ratio = np.power(np.asarray(rhod, dtype=F64) / rho0, 2.0 * drho0)
+    return 1.0 + 0.5 * (e0 - 1.0) * ((1.0 - ratio) / (1.0 + ratio) + 1.0)
+
+
+def pb_depsilon(rhod, e0, rho0, drho0):
+    """DielFunc::depsilon_rho (src/pb/DielFunc.cc:57-66): d epsilon / d rho,
+    0 below rho0 / 1000."""
+    rhod = np.asarray(rhod, dtype=F64)
+    inv = 1.0 / rho0
+    ratio = rhod * inv
+    pratio = np.power(ratio, 2.0 * drho0)
+    out = (2.0 * inv) * (1.0 - e0) * drho0 * np.power(ratio, 2.0 * drho0 - 1.0) / ((1.0 + pratio) * (1.0 + pratio))
+    return np.where(rhod < 0.001 * rho0, 0.0, out)
+
+
+def pb_init(g, rhod, e0, rho0, drho0):
+    """PBh4M<T>::init (src/pb/PBh4M.cc:164-184): epsilon from rhod (interior,
+    (n0, n1, n2)), sqrt_a = sqrt(epsilon), inv_sqrt_a = 1/sqrt_a, and
+    pp = (-(-Lap_4 sqrt_a)) inv_sqrt_a with the 4th-order operator; the
+    dielectric functions are periodic (bc 1) whatever the potential's
+    boundary conditions.  Returns ghosted (1, ...) blocks (sqrt_a, inv_sqrt_a,
+    pp) with fresh ghosts."""
+    assert g.ghosts >= 2, "PBh4M needs 2 ghosts"
+    sa = embed(g, np.sqrt(pb_epsilon(rhod, e0, rho0, drho0))[None])
+    trade(g, sa)
+    isa = embed(g, (1.0 / interior(g, sa)[0])[None])
+    trade(g, isa)
+    L = lap(g, 2, sa)
+    pp = embed(g, ((-1.0 * interior(g, L)[0]) * interior(g, isa)[0])[None])
+    trade(g, pp)
+    return sa, isa, pp
+
+
+def pb_apply(g, pp, x, bc):
+    """PBh4M<T>::apply (src/pb/PBh4M.h:83-89): -Lap_M x + B(pp x), x traded
+    first (boundary conditions bc), the product formed on every point, ghosts
+    included (pp's are periodic images)."""
+    trade(g, x, bc)
+    out = lap(g, 0, x)
+    w2 = rhs_mehr(g, pp * x)
+    return out + w2
+
+
+def pb_jacobi_factor(h):
+    """c0 of PBh4M<T>::jacobi (src/pb/PBh4M.cc:102-136): (30/12) sum 1/h^2."""
+    return (30.0 / 12.0) * sum(1.0 / (v * v) for v in h)
+
+
+def pb_jacobi(g, pp, x, f, bc):
+    """PBh4M<T>::jacobi: W = A x - f, x -= 0.75 / (c0 + pp) W (every point)."""
+    W = pb_apply(g, pp, x, bc) - f
+    c0 = pb_jacobi_factor(list(g.h))
+    x -= (0.75 / (c0 + pp)) * W
+    return W
+
+
+def _pb_vcycle(levels, lev, x, rhs, cogr, nu1, nu2, bc):
+    """pb::Vcycle (src/pb/Vcycle.h:28-249) with PBh4M on every level: the
+    coarse operator is PBh4M again on the full-weighting restriction of pp
+    (PBh4M<T>::coarseOp, src/pb/PBh4M.cc:73-90); coarse functions have one
+    ghost.  On the coarse levels this restatement keeps pp and the functions
+    on the same points with pp's ghosts periodic (the reference pairs pp, laid
+    out with the fine level's 2 ghosts, element by element with 1-ghost
+    coarse functions, past their end)."""
+    g, pp = levels[lev]
+    for _ in range(nu1):
+        pb_jacobi(g, pp, x, rhs, bc)
+    coarsen = all(g.dim[i] % 2 == 0 and g.dim[i] >= 2 * g.ghosts for i in range(3))
+    if lev < cogr and coarsen:
+        res = pb_apply(g, pp, x, bc) - rhs
+        trade(g, res, bc)
+        gc = Grid.make([d // 2 for d in g.dim], 1, [2 * v for v in g.h])
+        if len(levels) == lev + 1:
+            ppf = pp.copy()
+            trade(g, ppf)
+            ppc = restrict(g, ppf, gc)
+            trade(gc, ppc)
+            levels.append((gc, ppc))
+        rc = restrict(g, res, gc)
+        uc = empty_block(gc, 1)
+        _pb_vcycle(levels, lev + 1, uc, rc, cogr, nu1, nu2, bc)
+        trade(gc, uc, bc)
+        x -= extend(gc, uc, g)
+    for _ in range(nu2):
+        pb_jacobi(g, pp, x, rhs, bc)
+
+
+def pb_del1(g, u, d):
+    """FDoper<T>::del1_4th (src/pb/FDoper.cc:119-178) along direction d on the
+    interior of a traded block: (8/12)/h (u+1 - u-1) + (1/12)/h (u-2 - u+2)."""
+    ng = g.ghosts
+    e1 = (8.0 / 12.0) / g.h[d]
+    e2 = (1.0 / 12.0) / g.h[d]
+
+    def sh(k):
+        sl = [slice(ng, ng + g.dim[i]) for i in range(3)]
+        sl[d] = slice(ng + k, ng + k + g.dim[d])
+        return u[(0,) + tuple(sl)]
+    return e1 * (sh(1) - sh(-1)) + e2 * (sh(-2) - sh(2))
+
+
+def pb_mgm(g, rho, rhod, vh, e0, rho0, drho0, nu1=2, nu2=2, max_sweeps=10, tol=1e-16, max_nlevels=10,
+           bc=(1, 1, 1)):
+    """SolverPB<PBh4M>::solve (src/pb/SolverPB.cc:56-84) -> pb::Mgm
+    (src/pb/Mgm.h:12-111) with its transforms: vh *= sqrt_a, rhs =
+    B(rho inv_sqrt_a), the Mgm loop of poisson_mgm with the PBh4M operator, vh
+    *= inv_sqrt_a (no mean removal); then PBh4M<T>::get_vepsilon
+    (src/pb/PBh4M.cc:138-162): vks = depsilon(rhod) |grad vh|^2 (-1/(8 pi)),
+    the gradient by del1_4th.  rho, vh: ghosted (1, ...) blocks (vh updated);
+    rhod: interior (n0, n1, n2).  Returns (converged, final_residual,
+    final_relative_residual, residual_reduction, nb_sweeps, vks interior)."""
+    sa, isa, pp = pb_init(g, rhod, e0, rho0, drho0)
+    vh /= isa
+    res = rho * isa
+    trade(g, res, bc)
+    rhs = rhs_mehr(g, res)
+    inv_rhs_norm = 1.0 / gnorm(g, rhs, bc)
+    levels = [(g, pp)]
+    converged, sweeps = False, 0
+    init = final = rel = 1.0
+    for i in range(max_sweeps):
+        r = rhs - pb_apply(g, pp, vh, bc)
+        rn = gnorm(g, r, bc)
+        if i == 0:
+            init = rn
+        if rn * inv_rhs_norm < tol:
+            final, rel, converged = rn, rn * inv_rhs_norm, True
+            break
+        w = np.zeros_like(vh)
+        _pb_vcycle(levels, 0, w, r, max_nlevels, nu1, nu2, bc)
+        sweeps += 1
+        vh += w
+    if not converged:
+        lhs = pb_apply(g, pp, vh, bc) - rhs
+        final = gnorm(g, lhs, bc)
+        rel = final * inv_rhs_norm
+    vh *= isa
+    trade(g, vh, bc)
+    g2 = np.zeros(tuple(g.dim))
+    for d in range(3):
+        dv = pb_del1(g, vh, d)
+        g2 += dv * dv
+    vks = pb_depsilon(rhod, e0, rho0, drho0) * g2 * (-1.0 / (8.0 * np.pi))
+    return converged, final, rel, final / init, sweeps, vks
+
+
 # ------------------------------------------------------------------ box family
 def b_call(name, b, *args):
     return getattr(lib(), name)(C.byref(b), *args)

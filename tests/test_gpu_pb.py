@@ -1,0 +1,105 @@
+"""GPU parity of the Poisson-Boltzmann multigrid (mgx_pb_mg: SolverPB with the
+Mehrstellen dielectric operator PBh4M) against the oracle's restatement,
+through the C ABI: fixed numbers of V-cycles on periodic, Dirichlet and mixed
+grids (potential and the dielectric KS potential vks within 1e-12 of their
+largest values: FP64, sums in another order, the library's pow), a converged
+solve (same sweep count), and the argument checks."""
+import numpy as np
+import pytest
+
+import oracle as O
+from mgmol_amd import mgx
+from tests.gpu_util import relmax, say, to_dev, to_np
+
+pytestmark = pytest.mark.gpu
+
+E0, RHO0, DRHO0 = 78.36, 0.0004, 1.3
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tests.gpu_util import ctx as shared
+    return shared()
+
+
+def _density(dims, h, seed):
+    rng = np.random.default_rng(seed)
+    r = [np.arange(n) * hh for n, hh in zip(dims, h)]
+    X, Y, Z = np.meshgrid(*r, indexing="ij")
+    L = [n * hh for n, hh in zip(dims, h)]
+    out = np.zeros(dims)
+    for _ in range(3):
+        c = rng.uniform(0.2, 0.8, 3) * L
+        out += 0.01 * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / 1.2)
+    return out
+
+
+def _case(dims, bc, h=0.4, seed=0):
+    og = O.Grid.make(dims, 2, [h] * 3)
+    mg_ = mgx.Grid.make(dims, 2, [h] * 3, bc=bc)
+    rng = np.random.default_rng(seed)
+    rho = O.embed(og, rng.standard_normal((1,) + tuple(dims)))
+    rhod = _density(dims, [h] * 3, seed + 1)
+    return og, mg_, rho, rhod
+
+
+def _run(ctx, og, mg_, rho, rhod, vh0, **kw):
+    trho, trhod, tvh = to_dev(rho), to_dev(O.embed(og, rhod[None])), to_dev(vh0)
+    tvks = to_dev(O.empty_block(og, 1))
+    st = mgx.PoissonStats()
+    ctx.call("mgx_pb_mg", mgx.LAP_MEHRSTELLEN, mgx.block(trho, mg_), mgx.block(trhod, mg_), mgx.block(tvh, mg_),
+             mgx.block(tvks, mg_), E0, RHO0, DRHO0, 2, 2, kw.get("max_sweeps", 10), kw.get("tol", 1e-16),
+             kw.get("max_nlevels", 10), st)
+    return to_np(tvh), to_np(tvks), st
+
+
+@pytest.mark.parametrize("dims,bc", [((32, 32, 32), (1, 1, 1)), ((32, 32, 32), (0, 0, 0)),
+                                     ((16, 24, 32), (1, 0, 1)), ((24, 16, 16), (0, 1, 1))])
+def test_fixed_cycles_match_oracle(ctx, dims, bc):
+    og, mg_, rho, rhod = _case(dims, bc)
+    vh0 = O.embed(og, 0.05 * np.random.default_rng(7).standard_normal((1,) + tuple(dims)))
+    got, gvks, st = _run(ctx, og, mg_, rho, rhod, vh0, max_sweeps=3, tol=0.0)
+    ref = vh0.copy()
+    conv, fin, rel, red, sweeps, vks = O.pb_mgm(og, rho, rhod, ref, E0, RHO0, DRHO0, max_sweeps=3, tol=0.0, bc=bc)
+    err = relmax(O.interior(og, got), O.interior(og, ref))
+    ev = relmax(O.interior(og, gvks)[0], vks)
+    say("pb dims %s bc %s: vh rel err %.3e, vks rel err %.3e, residual %.4e vs %.4e" % (dims, bc, err, ev,
+                                                                                      st.final_residual, fin))
+    assert err <= 1e-12 and ev <= 1e-11
+    assert st.nb_sweeps == 3 and not st.converged
+    assert st.final_residual == pytest.approx(fin, rel=1e-9)
+    assert st.residual_reduction == pytest.approx(red, rel=1e-9)
+    assert np.max(np.abs(vks)) > 0.0
+
+
+def test_converged_solve(ctx):
+    dims, bc = (32, 32, 32), (0, 0, 0)
+    og, mg_, rho, rhod = _case(dims, bc, seed=3)
+    vh0 = O.empty_block(og, 1)
+    got, gvks, st = _run(ctx, og, mg_, rho, rhod, vh0, max_sweeps=80, tol=1e-11)
+    ref = vh0.copy()
+    conv, fin, rel, red, sweeps, vks = O.pb_mgm(og, rho, rhod, ref, E0, RHO0, DRHO0, max_sweeps=80, tol=1e-11, bc=bc)
+    say("pb converged: sweeps %d vs %d, rel residual %.3e" % (st.nb_sweeps, sweeps, st.final_relative_residual))
+    assert st.converged and conv and st.nb_sweeps == sweeps
+    assert relmax(O.interior(og, got), O.interior(og, ref)) <= 1e-10
+
+
+def test_rejects_bad_arguments(ctx):
+    og, mg_, rho, rhod = _case((16, 16, 16), (1, 1, 1))
+    t = to_dev(rho)
+    st = mgx.PoissonStats()
+    b = mgx.block(t, mg_)
+    # only the Mehrstellen dielectric operator is built
+    with pytest.raises(mgx.MgxError):
+        ctx.call("mgx_pb_mg", mgx.LAP_4TH, b, b, b, None, E0, RHO0, DRHO0, 2, 2, 1, 0.0, 10, st)
+    # PBh4M needs 2 ghosts
+    g1 = mgx.Grid.make((16, 16, 16), 1, [0.4] * 3)
+    t1 = to_dev(O.embed(O.Grid.make((16, 16, 16), 1, [0.4] * 3), np.zeros((1, 16, 16, 16))))
+    b1 = mgx.block(t1, g1)
+    with pytest.raises(mgx.MgxError):
+        ctx.call("mgx_pb_mg", mgx.LAP_MEHRSTELLEN, b1, b1, b1, None, E0, RHO0, DRHO0, 2, 2, 1, 0.0, 10, st)
+    # multipole boundary values are not built
+    g2 = mgx.Grid.make((16, 16, 16), 2, [0.4] * 3, bc=(2, 2, 2))
+    b2 = mgx.block(t, g2)
+    with pytest.raises(mgx.MgxError):
+        ctx.call("mgx_pb_mg", mgx.LAP_MEHRSTELLEN, b2, b2, b2, None, E0, RHO0, DRHO0, 2, 2, 1, 0.0, 10, st)
