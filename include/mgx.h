@@ -511,9 +511,17 @@ int mgx_solver_profile_work(mgx_solver* s, double* work);
 int mgx_solver_set_profile(mgx_solver* s, int on);
 int mgx_solver_get_psi(mgx_solver* s, double* psi_host);
 /* device pointers of the solver's psi / hpsi / S / H / residual (for inspection; the
- * residual is that of the last evaluated step when cfg.anderson_m = 0 and
- * precond_kind = MGX_PRECOND_VCYCLE, otherwise overwritten by the step) */
+ * residual R = psi theta - H psi of the last evaluated step is kept only with
+ * mgx_solver_set_keep_residual(s, 1): by default a step whose R only feeds the
+ * FP32/FP64 V-cycle from zero writes the V-cycle's start in its place) */
 int mgx_solver_buffers(mgx_solver* s, double** psi, double** hpsi, double** S, double** H, double** res);
+/* keep R in the residual buffer every step (on = 1).  Off (the default), the
+ * residual kernel of a step whose R is read by nothing but the V-cycle (FP32 or
+ * FP64 storage, zero start, no network trial around the step) writes the
+ * V-cycle's start f = R, x1 = jf f in its storage type instead of R, and the
+ * V-cycle skips its pointwise first sweep: one pass over the orbital points less
+ * per step, the same values.  No counterpart in the reference. */
+int mgx_solver_set_keep_residual(mgx_solver* s, int on);
 /* device pointer of the solver's S^-1 (n x n, column-major): the inverse of the
  * S of the last evaluated step (refreshed, or Cholesky) */
 int mgx_solver_inverse(mgx_solver* s, double** Sinv);

@@ -1877,10 +1877,75 @@ __global__ void k_pair_reduce(const double* part, int64_t nslot, const int* pslo
 // R_b(p) = sum_a psi_a(p) theta(a,b) - hpsi_b(p) for every box b covering
 // point p of the cell, as a 512 x k by k x k product on FP64 MFMA:
 // A = psi gathered per (point, box a), B = theta sub-block from LDS.
-template <bool PAIR>
+// where the residual goes: OM 0 writes R (FP64); OM 1 / 2 write instead the
+// V-cycle's zero start in FP32 / FP64, f = (T)R and x = 0 + (T)(jf f), the
+// values k_first_sweep forms from R (one pass over the points less)
+template <int OM>
+__device__ __forceinline__ void res_put(double* res, void* fo, void* xo, double jf, int64_t i, double rv)
+{
+    if constexpr (OM == 0)
+        res[i] = rv;
+    else if constexpr (OM == 1)
+    {
+        const float fv = (float)rv;
+        float xv       = 0.f;
+        xv += static_cast<float>(jf * static_cast<double>(fv));
+        static_cast<float*>(fo)[i] = fv;
+        static_cast<float*>(xo)[i] = xv;
+    }
+    else
+    {
+        double xv = 0.;
+        xv += jf * rv;
+        static_cast<double*>(fo)[i] = rv;
+        static_cast<double*>(xo)[i] = xv;
+    }
+}
+
+// two adjacent points (i even): 16-byte stores of R, 8-byte (FP32) or
+// 16-byte (FP64) stores of the start
+template <int OM>
+__device__ __forceinline__ void res_put2(double* res, void* fo, void* xo, double jf, int64_t i, double r0, double r1)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    if constexpr (OM == 0)
+    {
+        d2 v;
+        v[0]                                = r0;
+        v[1]                                = r1;
+        *reinterpret_cast<d2*>(res + i)     = v;
+    }
+    else if constexpr (OM == 1)
+    {
+        f2 fv, xv;
+        fv[0] = (float)r0;
+        fv[1] = (float)r1;
+        xv[0] = 0.f;
+        xv[1] = 0.f;
+        xv[0] += static_cast<float>(jf * static_cast<double>(fv[0]));
+        xv[1] += static_cast<float>(jf * static_cast<double>(fv[1]));
+        *reinterpret_cast<f2*>(static_cast<float*>(fo) + i) = fv;
+        *reinterpret_cast<f2*>(static_cast<float*>(xo) + i) = xv;
+    }
+    else
+    {
+        d2 fv, xv;
+        fv[0] = r0;
+        fv[1] = r1;
+        xv[0] = 0.;
+        xv[1] = 0.;
+        xv[0] += jf * r0;
+        xv[1] += jf * r1;
+        *reinterpret_cast<d2*>(static_cast<double*>(fo) + i) = fv;
+        *reinterpret_cast<d2*>(static_cast<double*>(xo) + i) = xv;
+    }
+}
+
+template <bool PAIR, int OM>
 __global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
     int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res, double* esq,
-    int ncell)
+    int ncell, void* fo, void* xo, double jf)
 {
     __shared__ CellEnt ce[kCellK];
     __shared__ double th[kCellK * kCellK];
@@ -1962,7 +2027,7 @@ __global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, con
                         d2 rv;
                         rv[0] = acc0[v] + (-1. * hv[u][v][0]);
                         rv[1] = acc1[v] + (-1. * hv[u][v][1]);
-                        *reinterpret_cast<d2*>(res + ib[u][v]) = rv;
+                        res_put2<OM>(res, fo, xo, jf, ib[u][v], rv[0], rv[1]);
                         sq[0][v] += rv[0] * rv[0];
                         sq[0][v] += rv[1] * rv[1];
                     }
@@ -2014,7 +2079,7 @@ __global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, con
                     if (ib[u][v] >= 0)
                     {
                         const double rv = acc[v] + (-1. * hv[u][v]);
-                        res[ib[u][v]]   = rv;
+                        res_put<OM>(res, fo, xo, jf, ib[u][v], rv);
                         sq[0][v] += rv * rv;
                     }
             }
@@ -2050,7 +2115,7 @@ __global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, con
             if (ib >= 0)
             {
                 const double rv = acc[v] + (-1. * hpsi[ib]);
-                res[ib]         = rv;
+                res_put<OM>(res, fo, xo, jf, ib, rv);
                 sq[job >> 5][v] += rv * rv;
             }
         }
@@ -2979,7 +3044,7 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
 template <typename T>
 int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, const double* res,
     double* out, double* psi, double alpha, const MarchDots* dots = nullptr, bool* dots_done = nullptr,
-    T* xdst = nullptr)
+    T* xdst = nullptr, bool start_given = false)
 {
     MGX_REQUIRE(ctx, type >= 0 && type <= 2, "LapFactory::createLap() --- option invalid");
     MGX_REQUIRE(ctx, gamma >= 0., "gamma must be non-negative");
@@ -2995,10 +3060,13 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     const bool zero = gamma == 0.;
     // the residual read (FP64), f and x written
     const int tk = prof_begin(ctx, MGX_PROF_FIRST, (8. + 2. * sizeof(T)) * (double)n);
-    if (zero)
+    // start_given: the residual kernel has written f and the first sweep's x
+    // already (box_residual_start)
+    MGX_REQUIRE(ctx, zero || !start_given, "a given start is the zero start");
+    if (zero && !start_given)
         hipLaunchKernelGGL(k_first_sweep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, B.tmp[0], n,
             jacobi_factor(type, bs->h));
-    else
+    else if (!zero)
         hipLaunchKernelGGL(k_prep<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, f, x, n, gamma);
     prof_end(ctx, tk);
     MGX_CHECK_LAUNCH(ctx);
@@ -3067,26 +3135,75 @@ int box_precond_h(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
 // into d (2 nbox) when its kernel can (*dots_done); dres: the residual's
 // per-box vel |R_j|^2 when at hand (FP16 scaling), else null
 int box_precond_x(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
-    const double* res, void* xdst, const void* kr0, double a, double* d, bool* dots_done, const double* dres)
+    const double* res, void* xdst, const void* kr0, double a, double* d, bool* dots_done, const double* dres,
+    bool start_given)
 {
     MarchDots md{ kr0, a, bs->h[0] * bs->h[1] * bs->h[2], d };
     const MarchDots* dots = kr0 ? &md : nullptr;
     if (dots_done) *dots_done = false;
     if (!xdst) return set_err(ctx, MGX_EINVAL, "no destination for the correction");
     // FP16 storage: the correction comes out in FP32
-    if (prec == MGX_F16) return box_precond_h(ctx, bs, type, max_levels, gamma, res, dres, (float*)xdst, dots, dots_done);
+    if (prec == MGX_F16)
+    {
+        MGX_REQUIRE(ctx, !start_given, "the FP16 V-cycle forms its own (scaled) start");
+        return box_precond_h(ctx, bs, type, max_levels, gamma, res, dres, (float*)xdst, dots, dots_done);
+    }
     return prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots,
-                                 dots_done, (float*)xdst)
+                                 dots_done, (float*)xdst, start_given)
                            : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, nullptr, nullptr, 0., dots,
-                                 dots_done, (double*)xdst);
+                                 dots_done, (double*)xdst, start_given);
 }
 
 int box_precond(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, double gamma, int prec,
-    const double* res, double* out, double* psi, double alpha)
+    const double* res, double* out, double* psi, double alpha, bool start_given)
 {
-    return prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, out, psi, alpha)
-                           : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, out, psi, alpha);
+    return prec == MGX_F32 ? box_precond_t<float>(ctx, bs, type, max_levels, gamma, res, out, psi, alpha, nullptr,
+                                 nullptr, nullptr, start_given)
+                           : box_precond_t<double>(ctx, bs, type, max_levels, gamma, res, out, psi, alpha, nullptr,
+                                 nullptr, nullptr, start_given);
 }
+
+int box_residual_om(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
+    double* res, double* d, int om, void* fo, void* xo, double jf);
+
+// the buffers box_precond_t keeps its zero start in: f, and the first
+// sweep's x (the V-cycle's first temporary of the fine level)
+template <typename T>
+int precond_start_bufs(mgx_ctx* ctx, const mgx_boxset* bs, int max_levels, T** f, T** x)
+{
+    const int64_t n = bs->off[0][bs->nbox];
+    T* xf           = (T*)workspace(ctx, 2, 2 * n * sizeof(T));
+    if (!xf) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
+    BoxMG<T> B;
+    int rc = box_mg_setup(ctx, bs, max_levels, B, 3);
+    if (rc) return rc;
+    *f = xf + n;
+    *x = B.tmp[0];
+    return MGX_OK;
+}
+
+// R = psi theta - hpsi as the V-cycle's zero start (FP32 or FP64, for a
+// box_precond_* call with start_given that follows), with the norms in d; R
+// itself is not written.  Needs the cell tables.
+int box_residual_start(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
+    const double* theta, double* d, int prec, int type, int max_levels)
+{
+    MGX_REQUIRE(ctx, bs->use_cells && (prec == MGX_F32 || prec == MGX_F64),
+        "the fused V-cycle start needs the cell tables and an FP32 or FP64 V-cycle");
+    const double jf = jacobi_factor(type, bs->h);
+    int rc;
+    if (prec == MGX_F32)
+    {
+        float *f, *x;
+        if ((rc = precond_start_bufs<float>(ctx, bs, max_levels, &f, &x))) return rc;
+        return box_residual_om(ctx, bs, psi, hpsi, theta, nullptr, d, 1, f, x, jf);
+    }
+    double *f, *x;
+    if ((rc = precond_start_bufs<double>(ctx, bs, max_levels, &f, &x))) return rc;
+    return box_residual_om(ctx, bs, psi, hpsi, theta, nullptr, d, 2, f, x, jf);
+}
+
+bool boxset_has_cells(const mgx_boxset* bs) { return bs->use_cells; }
 
 int box_hloc(mgx_ctx* ctx, const mgx_boxset* bs, int dtype, const void* psi, const double* vtot, void* hpsi)
 {
@@ -3320,25 +3437,42 @@ int box_autotune(mgx_ctx* ctx, mgx_boxset* bs, double* ms)
 
 int box_sumsq(mgx_ctx* ctx, const mgx_boxset* bs, const double* r, double* d);
 
+// the cell residual kernel for output mode OM (R, or the V-cycle's zero start)
+template <int OM>
+void cell_residual_launch(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
+    const double* theta, double* res, double* esq, void* fo, void* xo, double jf)
+{
+    auto kern = bs->pair_z ? k_cell_residual<true, OM> : k_cell_residual<false, OM>;
+    hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, hpsi, theta, bs->nbox,
+        (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
+        (const int*)bs->d_cell_ent, res, esq, bs->ncell, fo, xo, jf);
+}
+
 // R = psi theta - hpsi per box; with d: also d_j = vel |R_j|^2 (computed in
-// the cell kernel when the cell tables exist, else by k_box_sumsq)
-int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
-    double* res, double* d)
+// the cell kernel when the cell tables exist, else by k_box_sumsq).  om 1 / 2
+// (cell tables only): instead of R the kernel writes the FP32 / FP64 zero
+// start of the V-cycle, f = (T)R to fo and x = 0 + (T)(jf f) to xo
+int box_residual_om(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
+    double* res, double* d, int om, void* fo, void* xo, double jf)
 {
     if (bs->use_cells)
     {
         double* esq = nullptr;
         if (d)
         {
-            esq = (double*)workspace(ctx, 3, (size_t)std::max(bs->nent, 1) * sizeof(double));
+            // its own slot: the V-cycle's buffers (slot 3) may hold the start
+            esq = (double*)workspace(ctx, 9, (size_t)std::max(bs->nent, 1) * sizeof(double));
             if (!esq) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
         }
-        // psi, H psi read, R written (FP64)
-        const int tk = prof_begin(ctx, MGX_PROF_RESIDK, 24. * (double)bs->off[0][bs->nbox]);
-        hipLaunchKernelGGL(bs->pair_z ? k_cell_residual<true> : k_cell_residual<false>, dim3(cell_grid(bs->ncell)),
-            dim3(256), 0, ctx->stream, psi, hpsi, theta,
-            bs->nbox, (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-            (const int*)bs->d_cell_ent, res, esq, bs->ncell);
+        // psi, H psi read; R written (FP64), or f and x (2 x the V-cycle's type)
+        const double wb = om == 0 ? 8. : (om == 1 ? 8. : 16.);
+        const int tk    = prof_begin(ctx, MGX_PROF_RESIDK, (16. + wb) * (double)bs->off[0][bs->nbox]);
+        if (om == 1)
+            cell_residual_launch<1>(ctx, bs, psi, hpsi, theta, res, esq, fo, xo, jf);
+        else if (om == 2)
+            cell_residual_launch<2>(ctx, bs, psi, hpsi, theta, res, esq, fo, xo, jf);
+        else
+            cell_residual_launch<0>(ctx, bs, psi, hpsi, theta, res, esq, nullptr, nullptr, 0.);
         prof_end(ctx, tk);
         if (d)
         {
@@ -3349,11 +3483,18 @@ int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const do
         MGX_CHECK_LAUNCH(ctx);
         return MGX_OK;
     }
+    MGX_REQUIRE(ctx, om == 0, "the fused V-cycle start needs the cell tables");
     BoxView v = view(bs, 0);
     hipLaunchKernelGGL(k_box_residual, dim3(tile_grid(v.ntiles)), dim3(kTile), 0, ctx->stream, psi, hpsi, theta,
         bs->nbox, (const int*)bs->d_nbr_ptr, (const int*)bs->d_nbr, v, res);
     MGX_CHECK_LAUNCH(ctx);
     return d ? box_sumsq(ctx, bs, res, d) : MGX_OK;
+}
+
+int box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi, const double* theta,
+    double* res, double* d)
+{
+    return box_residual_om(ctx, bs, psi, hpsi, theta, res, d, 0, nullptr, nullptr, 0.);
 }
 
 // rho on the level-0 grid (x slowest, z fastest) from the boxes' psi and
@@ -4062,7 +4203,7 @@ int mgx_box_precond_mg(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_lev
         MGX_CHECK_LAUNCH(ctx);
         return sync(ctx);
     }
-    rc = box_precond(ctx, bs, type, max_levels, gamma, prec, res, out, nullptr, 0.);
+    rc = box_precond(ctx, bs, type, max_levels, gamma, prec, res, out, nullptr, 0., false);
     return rc ? rc : sync(ctx);
 }
 

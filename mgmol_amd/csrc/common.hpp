@@ -36,7 +36,7 @@ struct mgx_ctx
     std::string err;
     // grow-only device workspaces; a call owns them for its duration (the
     // context is not re-entrant), slot = independent buffer
-    static constexpr int kSlots = 9;
+    static constexpr int kSlots = 10;
     void* ws[kSlots]     = {};
     size_t ws_sz[kSlots] = {};
     uint64_t ws_gen      = 0;  // bumped whenever a workspace moves (captured graphs hold the old address)

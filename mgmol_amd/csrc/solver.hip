@@ -35,7 +35,10 @@ int box_norm(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*)
 int box_normalize(mgx_ctx*, const mgx_boxset*, double*, double*);
 int box_hloc_normalize(mgx_ctx*, const mgx_boxset*, double*, const double*, double*, const double*);
 int box_scale(mgx_ctx*, const mgx_boxset*, double*, const double*);
-int box_precond(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, double*, double*, double);
+int box_precond(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, double*, double*, double, bool);
+int box_residual_start(mgx_ctx*, const mgx_boxset*, const double*, const double*, const double*, double*, int, int,
+    int);
+bool boxset_has_cells(const mgx_boxset*);
 int64_t box_size(const mgx_boxset*, int);
 void boxset_geom(const mgx_boxset*, int gdim[3], double h[3], int* nbox);
 int chol_inv(mgx_ctx*, int, const double*, double*, double*);
@@ -50,7 +53,7 @@ int box_anderson_update_sq(mgx_ctx*, const mgx_boxset*, int, const double*, doub
     const void*, void*, const double*, double, double*);
 int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, double, double*);
 int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, void*, const void*, double,
-    double*, bool*, const double*);
+    double*, bool*, const double*, bool);
 int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
 int inverse_update_spec(mgx_ctx*, int, const double*, const PairTiles&, double*, double*, int, bool, double*);
 int ns_spec_check(const double*, int, double, bool, int*);
@@ -115,6 +118,10 @@ struct mgx_solver
     // the normalisation of the last update is pending: d holds vel*|psi_j|^2
     // of the unnormalised psi; the next Hpsi applies it (or flush_norm)
     bool pending_norm = false;
+    // the residual kernel writes the V-cycle's zero start instead of R this
+    // step (set per step); keep_res: always write R (a caller reads it)
+    bool start_fused = false;
+    bool keep_res    = false;
     // learned preconditioner: the state before the last CNN step, to undo it
     mgx_cnn* cnn       = nullptr;
     double *psi_prev = nullptr, *r_prev = nullptr;  // the saved state of a trial
@@ -187,7 +194,12 @@ int project(mgx_solver* s, int spec, bool bf16, int& tk)
     if ((r = diag_sum(ctx, n, s->th, s->scal))) return r;
     prof_end(ctx, tk);
     tk = prof_begin(ctx, MGX_PROF_RESID);
-    if ((r = box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d))) return r;
+    // the residual itself, or (start_fused) the V-cycle's zero start formed
+    // from it in the same kernel; the norms in d either way
+    if ((r = s->start_fused ? box_residual_start(ctx, s->bs, s->psi, s->hpsi, s->th, s->d, s->cfg.precond_dtype,
+                                  s->cfg.lap_type, s->cfg.mg_levels)
+                            : box_residual(ctx, s->bs, s->psi, s->hpsi, s->th, s->res, s->d)))
+        return r;
     if ((r = box_norm(ctx, s->bs, s->d, s->Sinv, s->scal + 1))) return r;
     prof_end(ctx, tk);
     MGX_CHECK_HIP(ctx, hipMemcpyAsync(s->host, s->scal, (8 + spec + (bf16 ? 1 : 0)) * sizeof(double),
@@ -250,7 +262,7 @@ int tail_body(mgx_solver* s, bool use_cnn, bool restored, bool plain_step, doubl
         // preconditioned step psi += alpha K(R) (ABPG::update_states, no
         // extrapolation), fused with the V-cycle's last kernel
         if ((rc = box_precond(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, nullptr, s->psi,
-                 alpha)))
+                 alpha, s->start_fused)))
             return rc;
     }
     else
@@ -272,7 +284,7 @@ int tail_body(mgx_solver* s, bool use_cnn, bool restored, bool plain_step, doubl
             // d holds the residual's per-box sums unless R was just restored
             if ((rc = box_precond_x(ctx, s->bs, s->cfg.lap_type, s->cfg.mg_levels, gamma, prec, s->res, s->kr1,
                      (s->accelerate && s->mm > 0) ? s->kr0 : nullptr, alpha, s->d, &dots_done,
-                     restored ? nullptr : s->d)))
+                     restored ? nullptr : s->d, s->start_fused && !restored)))
                 return rc;
             kr = s->kr1;
         }
@@ -440,6 +452,15 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     const bool chol    = mehr || !s->have_inv || s->cfg.inverse_mode == MGX_INV_CHOL;
     int spec           = chol ? 0 : std::max(1, std::min(s->ns_pred, s->ns_max_spec));
     const bool pending = s->pending_norm;
+    // the residual goes straight into the V-cycle's zero start unless R itself
+    // is read this step (a network trial, its evaluation, or the caller)
+    {
+        const int prec     = s->cfg.precond_dtype;
+        const bool learned = s->cfg.precond_kind == MGX_PRECOND_LEARNED;
+        const bool need_r  = s->keep_res || (learned && (s->trial != 0 || s->backoff == 0));
+        s->start_fused = !mehr && !need_r && boxset_has_cells(s->bs) && (prec == MGX_F32 || prec == MGX_F64)
+                         && s->cfg.precond_start == MGX_START_ZERO;
+    }
     if (mehr)
         rc = head_mehr(s, pending);  // Cholesky of S and of the B-matrix every step
     else if (chol)
@@ -447,7 +468,8 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     else
     {
         uint64_t key = 1;
-        for (uint64_t v : { (uint64_t)pending, (uint64_t)spec, (uint64_t)bf16, (uint64_t)(uintptr_t)s->psi })
+        for (uint64_t v : { (uint64_t)pending, (uint64_t)spec, (uint64_t)bf16, (uint64_t)(uintptr_t)s->psi,
+                 (uint64_t)s->start_fused })
             key = mix_key(key, v);
         rc = run_seq(s, key, nullptr, [&]() { return head_body(s, pending, false, spec, bf16); });
     }
@@ -605,7 +627,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
         std::memcpy(&gbits, &gamma, sizeof(gbits));
         for (uint64_t v : { (uint64_t)s->accelerate, (uint64_t)plain_step, (uint64_t)(s->mm > 0), gbits,
                  (uint64_t)(uintptr_t)s->psi, (uint64_t)(uintptr_t)s->x0, (uint64_t)(uintptr_t)s->kr0,
-                 (uint64_t)(uintptr_t)s->kr1 })
+                 (uint64_t)(uintptr_t)s->kr1, (uint64_t)s->start_fused })
             key = mix_key(key, v);
         rc = run_seq(s, key, &o, [&]() { return tail_body(s, false, false, plain_step, gamma, o); });
     }
@@ -891,6 +913,13 @@ int mgx_solver_set_psi(mgx_solver* s, const double* host)
     s->pending_norm = false;
     s->mm           = 0;
     s->trial        = 0;
+    return MGX_OK;
+}
+
+int mgx_solver_set_keep_residual(mgx_solver* s, int on)
+{
+    if (!s) return MGX_EINVAL;
+    s->keep_res = on != 0;
     return MGX_OK;
 }
 

@@ -161,6 +161,7 @@ def build_dataset(ctx, seeds, box_kw, steps, capture_every, precond_dtype=None, 
         cfg = synth.box_config(seed=int(seed), **box_kw)
         bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, cfg.levels)
         sol = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, mg_levels=cfg.levels - 1, precond_dtype=precond_dtype)
+        sol.set_keep_residual(True)  # the residuals are read between steps
         kr = mgx.DeviceArray(ctx, shape=(cfg.psi.size,), dtype=np.float64, zero=False)
         pres = C.c_void_p()
         for it in range(steps):

@@ -40,7 +40,7 @@ EXPORTS = [
     "mgx_box_hloc_mehr", "mgx_box_gram_hb_mehr", "mgx_box_residual_mehr",
     "mgx_solver_create", "mgx_solver_destroy", "mgx_solver_step", "mgx_solver_run", "mgx_solver_reset",
     "mgx_solver_profile", "mgx_solver_profile_work", "mgx_solver_set_profile",
-    "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_inverse", "mgx_solver_set_psi", "mgx_solver_set_cnn", "mgx_solver_density",
+    "mgx_solver_get_psi", "mgx_solver_buffers", "mgx_solver_set_keep_residual", "mgx_solver_inverse", "mgx_solver_set_psi", "mgx_solver_set_cnn", "mgx_solver_density",
     "mgx_solver_set_graph", "mgx_solver_graph_stats",
     "mgx_cnn_nparams", "mgx_cnn_create", "mgx_cnn_destroy", "mgx_cnn_set_params", "mgx_cnn_get_params",
     "mgx_cnn_forward", "mgx_cnn_grad", "mgx_cnn_train_step", "mgx_cnn_apply", "mgx_cnn_make_pairs",
@@ -148,6 +148,7 @@ def lib():
             "mgx_solver_reset": (P,), "mgx_solver_profile": (P, C.POINTER(D), C.POINTER(I64), I),
             "mgx_solver_profile_work": (P, C.POINTER(D)), "mgx_solver_set_profile": (P, I),
             "mgx_solver_buffers": (P, PP, PP, PP, PP, PP), "mgx_solver_inverse": (P, PP),
+            "mgx_solver_set_keep_residual": (P, I),
             "mgx_solver_set_psi": (P, P), "mgx_solver_set_cnn": (P, P),
             "mgx_solver_set_graph": (P, I),
             "mgx_solver_graph_stats": (P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)),
@@ -426,6 +427,10 @@ class Solver:
 
     def set_profile(self, on):
         self.ctx.check(lib().mgx_solver_set_profile(self.h, 1 if on else 0), "mgx_solver_set_profile")
+
+    def set_keep_residual(self, on):
+        """keep R in the residual buffer every step (mgx_solver_set_keep_residual)"""
+        self.ctx.check(lib().mgx_solver_set_keep_residual(self.h, 1 if on else 0), "mgx_solver_set_keep_residual")
 
     def set_graph(self, on):
         """replay each step's launch sequences as HIP graphs (mgx_solver_set_graph)"""

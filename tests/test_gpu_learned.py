@@ -47,6 +47,7 @@ def test_trained_network_approximates_the_vcycle(ctx):
     cfg = _cfg()
     bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, cfg.levels)
     sol = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, mg_levels=cfg.levels - 1, precond_dtype=mgx.F32)
+    sol.set_keep_residual(True)  # the residuals are read between steps
     trained = learned.Cnn(ctx, meta["edge"], 64, params)
     untrained = learned.Cnn(ctx, meta["edge"], 64, learned.init_params(seed=0))
     kr = mgx.DeviceArray(ctx, shape=(cfg.psi.size,), dtype=np.float64)
