@@ -1532,12 +1532,16 @@ __device__ __forceinline__ void cell_gram_tile(const double* psi, const double* 
 // G > 1 packs G row groups into the 16 box columns (16/G boxes, k <= 16/G):
 // lane column c = g (16/G) + a loads box a's row of group g, so one step
 // covers G rows; the product's diagonal blocks (same g) are the partials.
-template <int PREC, bool SAME, int NW, int G>
+// The cell's 64 rows of 8 points are summed in two fixed halves (rows 0-31,
+// 32-63: NS steps of G rows each); one wave per cell computes both halves,
+// two waves one each, and the halves are added in the same order either way,
+// so the sums do not depend on the number of waves.
+template <int PREC, bool SAME, int G>
 __device__ __forceinline__ void cell_gram_tile2(const double* psi, const double* hpsi, const CellEnt& A,
-    const CellEnt& B, int wid, int r, int grp, double (&accG)[4], double (&accP)[4])
+    const CellEnt& B, int half, int r, int grp, double (&accG)[4], double (&accP)[4])
 {
-    // NW waves share the cell's 64 rows of 8 points: NS steps of G rows each
-    constexpr int NS = 64 / (NW * G);
+    constexpr int NS = 32 / G;
+    const int wid    = half;
     typedef double d2 __attribute__((ext_vector_type(2)));
     const d2 z2 = { 0., 0. };
     auto ld2 = [](const double* p, int64_t i) -> d2 { return *reinterpret_cast<const d2*>(p + i); };
@@ -1616,33 +1620,38 @@ __device__ __forceinline__ void cell_gram_packed(const double* psi, const double
     int wid, int lane, double (*red)[64][4], double* part, int64_t nslot)
 {
     constexpr int kp = 16 / G;
+    constexpr int HW = 2 / NW;  // halves per wave
     const int c = lane & 15, r = lane >> 4;
     const int a = c % kp, g = c / kp;
     CellEnt none{};
     const CellEnt A = a < k ? ce[a] : none;
-    double accG[4], accP[4];
-    cell_gram_tile2<PREC, true, NW, G>(psi, hpsi, A, A, wid, r, g, accG, accP);
-    for (int pass = 0; pass < 2; pass++)
+    // red[pass * 2 + half]: each half's partials go to LDS as soon as formed
+    for (int q = 0; q < HW; q++)
     {
-        const double* acc = pass == 0 ? accG : accP;
+        double accG[4], accP[4];
+        const int h = wid * HW + q;
+        cell_gram_tile2<PREC, true, G>(psi, hpsi, A, A, h, r, g, accG, accP);
         for (int v = 0; v < 4; v++)
-            red[wid][lane][v] = acc[v];
-        __syncthreads();
+        {
+            red[h][lane][v]     = accG[v];
+            red[2 + h][lane][v] = accP[v];
+        }
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; pass++)
         for (int o = threadIdx.x; o < k * k; o += 64 * NW)
         {
             const int aa = o / k, bb = o - (o / k) * k;
             double sum = 0.;
-            for (int w = 0; w < NW; w++)
+            for (int w = 0; w < 2; w++)
                 for (int gg = 0; gg < G; gg++)
                 {
                     const int i = gg * kp + aa, j = gg * kp + bb;
                     const int ln = j + 16 * (PREC == MGX_PREC_F64 ? (i & 3) : (i >> 2));
-                    sum += red[w][ln][PREC == MGX_PREC_F64 ? (i >> 2) : (i & 3)];
+                    sum += red[2 * pass + w][ln][PREC == MGX_PREC_F64 ? (i >> 2) : (i & 3)];
                 }
             part[pass * nslot + o] = sum;
         }
-        __syncthreads();
-    }
 }
 
 // S / H partials per 8^3 cell over all box pairs covering it (k <= 32):
@@ -1658,8 +1667,11 @@ __global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, con
     int ncell)
 {
     static_assert(PAIR || NW == 4, "the unpaired tiles take four waves");
+    static_assert(!PAIR || NW == 1 || NW == 2, "the paired tiles take one or two waves");
+    // paired tiles: one slot per half of the cell's rows; unpaired: per wave
+    constexpr int NR = PAIR ? 2 : NW;
     __shared__ CellEnt ce[kCellK];
-    __shared__ double red[NW][64][4];
+    __shared__ double red[2 * NR][64][4];  // [pass * NR + part]
     // consecutive cells (z-neighbours share cache lines) on one XCD
     const int64_t cl = xcd_block(blockIdx.x, gridDim.x);
     if (cl >= ncell) return;
@@ -1672,14 +1684,17 @@ __global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, con
     const int nt = (k + 15) / 16;
     const int64_t slot = cell_slot[cell];
     CellEnt none{};
-    if (PAIR && k <= 8)
+    if constexpr (PAIR)
     {
-        // k <= 4: four row groups per step, k <= 8: two
-        if (k <= 4)
-            cell_gram_packed<PREC, NW, 4>(psi, hpsi, ce, k, wid, lane, red, part + slot, nslot);
-        else
-            cell_gram_packed<PREC, NW, 2>(psi, hpsi, ce, k, wid, lane, red, part + slot, nslot);
-        return;
+        if (k <= 8)
+        {
+            // k <= 4: four row groups per step, k <= 8: two
+            if (k <= 4)
+                cell_gram_packed<PREC, NW, 4>(psi, hpsi, ce, k, wid, lane, red, part + slot, nslot);
+            else
+                cell_gram_packed<PREC, NW, 2>(psi, hpsi, ce, k, wid, lane, red, part + slot, nslot);
+            return;
+        }
     }
     for (int ti = 0; ti < nt; ti++)
         for (int tj = 0; tj < nt; tj++)
@@ -1687,28 +1702,38 @@ __global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, con
             const int a = ti * 16 + c, b = tj * 16 + c;
             const CellEnt A = a < k ? ce[a] : none;
             const CellEnt B = b < k ? ce[b] : none;
-            double accG[4], accP[4];
-            if (PAIR && ti == tj)
-                cell_gram_tile2<PREC, true, NW, 1>(psi, hpsi, A, B, wid, r, 0, accG, accP);
-            else if (PAIR)
-                cell_gram_tile2<PREC, false, NW, 1>(psi, hpsi, A, B, wid, r, 0, accG, accP);
-            else if (ti == tj)
-                cell_gram_tile<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
-            else
-                cell_gram_tile<PREC, false>(psi, hpsi, A, B, wid, r, accG, accP);
+            constexpr int HW = PAIR ? (NW == 1 ? 2 : 1) : 1;  // row halves per wave (paired), else one part
+            for (int q = 0; q < HW; q++)
+            {
+                double accG[4], accP[4];
+                const int part_i = wid * HW + q;
+                if constexpr (PAIR)
+                {
+                    if (ti == tj)
+                        cell_gram_tile2<PREC, true, 1>(psi, hpsi, A, B, part_i, r, 0, accG, accP);
+                    else
+                        cell_gram_tile2<PREC, false, 1>(psi, hpsi, A, B, part_i, r, 0, accG, accP);
+                }
+                else if (ti == tj)
+                    cell_gram_tile<PREC, true>(psi, hpsi, A, B, wid, r, accG, accP);
+                else
+                    cell_gram_tile<PREC, false>(psi, hpsi, A, B, wid, r, accG, accP);
+                for (int v = 0; v < 4; v++)
+                {
+                    red[part_i][lane][v]      = accG[v];
+                    red[NR + part_i][lane][v] = accP[v];
+                }
+            }
+            __syncthreads();
             for (int pass = 0; pass < 2; pass++)
             {
-                const double* acc = pass == 0 ? accG : accP;
-                for (int v = 0; v < 4; v++)
-                    red[wid][lane][v] = acc[v];
-                __syncthreads();
                 if (wid == 0)
                 {
                     for (int v = 0; v < 4; v++)
                     {
-                        double sum = red[0][lane][v];
-                        for (int w = 1; w < NW; w++)
-                            sum += red[w][lane][v];
+                        double sum = red[pass * NR][lane][v];
+                        for (int w = 1; w < NR; w++)
+                            sum += red[pass * NR + w][lane][v];
                         const int row = ti * 16 + ((PREC == MGX_PREC_F64) ? r + 4 * v : 4 * r + v);
                         const int col = tj * 16 + c;
                         if (row < k && col < k) part[pass * nslot + slot + row * k + col] = sum;
@@ -2517,6 +2542,43 @@ __global__ void __launch_bounds__(256) k_box_and_update_sq(const double* x, doub
     if (threadIdx.x == 0) d[j] = acc[0] * vel;
 }
 
+// the Anderson combination of k_box_and_update_nrm for PT points per thread:
+// nv = xb + beta fb, xb = (1 - th) x + th x0, fb = (1 - th) f + th f0 (f = a kr)
+// with a history (HIST), nv = x + beta f without; acc += nv^2
+template <typename T, int PT, bool HIST>
+__device__ __forceinline__ void and_combine(const double* xj, const double* xoj, const T* kj, const T* k0j, T* k0o,
+    double a, double th, double beta, int n, double* nv, double& acc)
+{
+#pragma unroll
+    for (int c = 0; c < PT; c += 8)
+    {
+        // eight points in flight at a time (the scheduler may not hoist the
+        // next group's loads: registers hold the PT values already formed)
+#pragma unroll
+        for (int i = c; i < c + 8; i++)
+        {
+            const int t = threadIdx.x + 1024 * i;
+            nv[i]       = 0.;
+            if (t < n)
+            {
+                const T k       = kj[t];
+                const double xv = xj[t], fv = a * (double)k;
+                double xb = xv, fb = fv;
+                if constexpr (HIST)
+                {
+                    const double x0v = xoj[t], f0v = a * (double)k0j[t];
+                    xb = (1. - th) * xv + th * x0v;
+                    fb = (1. - th) * fv + th * f0v;
+                }
+                if (k0o) k0o[t] = k;
+                nv[i] = xb + beta * fb;
+                acc += nv[i] * nv[i];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // k_box_and_update_sq with the normalisation done in the same pass: one
 // block of 1024 threads per box keeps the box's new values in registers
 // (PT per thread, boxes of at most 1024 PT points), forms the box's norm
@@ -2539,31 +2601,12 @@ __global__ void __launch_bounds__(1024) k_box_and_update_nrm(const double* x, do
     T* k0o           = kr0_out ? kr0_out + q0 : nullptr;
     double nv[PT];
     double acc[1] = { 0. };
-#pragma unroll
-    for (int c = 0; c < PT; c += 8)
-    {
-        // eight points in flight at a time (the scheduler may not hoist the
-        // next group's loads: registers hold the PT values already formed)
-#pragma unroll
-        for (int i = c; i < c + 8; i++)
-        {
-            const int t = threadIdx.x + 1024 * i;
-            nv[i]       = 0.;
-            if (t < n)
-            {
-                // without a history (th = 0) the history buffers are not
-                // read: they may not hold values yet
-                const T k       = kj[t];
-                const double xv = xj[t], fv = a * (double)k;
-                const double xb = th != 0. ? (1. - th) * xv + th * xoj[t] : xv;
-                const double fb = th != 0. ? (1. - th) * fv + th * (a * (double)k0j[t]) : fv;
-                if (k0o) k0o[t] = k;
-                nv[i] = xb + beta * fb;
-                acc[0] += nv[i] * nv[i];
-            }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
+    // without a history (th = 0) the history buffers are not read: they may
+    // not hold values yet (th is the same for the whole block)
+    if (th != 0.)
+        and_combine<T, PT, true>(xj, xoj, kj, k0j, k0o, a, th, beta, n, nv, acc[0]);
+    else
+        and_combine<T, PT, false>(xj, xoj, kj, k0j, k0o, a, th, beta, n, nv, acc[0]);
     block_sum<1>(acc, lds);
     if (threadIdx.x == 0) lds[16] = acc[0] * vel;
     __syncthreads();
