@@ -425,15 +425,20 @@ __global__ void __launch_bounds__(256) k_diag_dot(const double* d, const double*
 // kSpTile output columns of tile t for 256 consecutive rows c, walks the
 // union of the tile's rows k once (X[c, k] = X[k, c] read down column k,
 // coalesced over c) and applies the tile's coefficients from LDS.
+// One block per (column tile, 256-row chunk), mapped XCD-aware so that one
+// XCD takes all the tiles of a row chunk: the union columns of the tiles it
+// runs together then share that XCD's L2 (a row chunk of all of X is n x 2 KB)
 __global__ void __launch_bounds__(256) k_spmm_tiles(const double* __restrict__ X, int n,
     const double* __restrict__ val, const int* tptr, const int* tcol, const int* tk, const int* te,
-    double* __restrict__ Y)
+    double* __restrict__ Y, int ntile, int64_t nreal)
 {
     constexpr int U = 64;
     __shared__ double cf[U][kSpTile];
     __shared__ int kk[U];
-    const int t = blockIdx.x;
-    const int c = blockIdx.y * 256 + threadIdx.x;
+    const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    if (L >= nreal) return;
+    const int t = (int)(L % ntile);
+    const int c = (int)(L / ntile) * 256 + threadIdx.x;
     double acc[kSpTile];
     for (int q = 0; q < kSpTile; q++)
         acc[q] = 0.;
@@ -489,8 +494,9 @@ int spmm_pairs(mgx_ctx* ctx, int n, const double* X, const double* val, const Pa
     if (pt.ntile <= 0) return set_err(ctx, MGX_EINVAL, "no column tiles");
     // X read and Y written once: 16 n^2 bytes
     const int tk = prof_begin(ctx, MGX_PROF_SPMM, 16. * (double)n * n);
-    hipLaunchKernelGGL(k_spmm_tiles, dim3(pt.ntile, (n + 255) / 256), dim3(256), 0, ctx->stream, X, n, val, pt.ptr,
-        pt.col, pt.k, pt.e, Y);
+    const int64_t nreal = (int64_t)pt.ntile * ((n + 255) / 256);
+    hipLaunchKernelGGL(k_spmm_tiles, dim3((unsigned)((nreal + 7) / 8 * 8)), dim3(256), 0, ctx->stream, X, n, val,
+        pt.ptr, pt.col, pt.k, pt.e, Y, pt.ntile, nreal);
     prof_end(ctx, tk);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
