@@ -1752,11 +1752,16 @@ __global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, con
 //   out[12*q + s] = sum_{X in slab s, X in box i} psi_i(X) hv(X)
 // for its entry q in j's interaction list, reading box i's two planes
 // together (adjacent values for z faces).
+// Blocks run the boxes in Morton order of their centres (order), mapped
+// XCD-aware: the faces of nearby boxes, which read rows of the same
+// neighbours, run on one XCD and share its L2.
 __global__ void __launch_bounds__(256) k_slab_skin(const double* psi, const int64_t* off, const int* edge,
-    const int* ilist_ptr, const int* ilist, Coef4 c4, int use_lds, double* out)
+    const int* ilist_ptr, const int* ilist, Coef4 c4, int use_lds, double* out, const int* order, int nbox)
 {
     extern __shared__ double hv[];  // 2 planes of ej^2
-    const int j = blockIdx.x / 6, f = blockIdx.x - (blockIdx.x / 6) * 6;
+    const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    if (L >= 6 * (int64_t)nbox) return;
+    const int j = order[L / 6], f = (int)(L - (L / 6) * 6);
     const int d = f >> 1, side = f & 1;
     const int ej = edge[j];
     const double* pj = psi + off[j];
@@ -3292,9 +3297,9 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
             const int use_lds = lds <= 64 * 1024;
             if (!use_lds) lds = 0;
             const int tk = prof_begin(ctx, MGX_PROF_SKIN);
-            hipLaunchKernelGGL(k_slab_skin, dim3(6 * bs->nbox), dim3(256), lds, ctx->stream, psi,
+            hipLaunchKernelGGL(k_slab_skin, dim3((6 * bs->nbox + 7) / 8 * 8), dim3(256), lds, ctx->stream, psi,
                 (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_ilist_ptr,
-                (const int*)bs->d_ilist, coef4(bs->h), use_lds, skin);
+                (const int*)bs->d_ilist, coef4(bs->h), use_lds, skin, (const int*)bs->d_sp_col, bs->nbox);
             prof_end(ctx, tk);
             MGX_CHECK_LAUNCH(ctx);
         }
