@@ -292,7 +292,19 @@ struct MarchArgs
     // sweep): x_out = x * oinv[j], undoing the box's scale of the residual;
     // dkr0 then has the output type
     const double* oinv = nullptr;
+    // the boxes in Morton order of their centres (nbox of them), blocks
+    // mapped XCD-aware: nearby boxes, whose H psi reads overlapping parts of
+    // V, run on one XCD
+    const int* order = nullptr;
+    int nbox         = 0;
 };
+
+// the box of this block (grids padded to a multiple of 8); -1: no box
+__device__ __forceinline__ int march_box(const MarchArgs& a)
+{
+    const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    return L < a.nbox ? a.order[L] : -1;
+}
 
 struct MarchDots
 {
@@ -312,7 +324,8 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
 {
     extern __shared__ unsigned char lds_raw[];
     T* pl0      = reinterpret_cast<T*>(lds_raw);
-    const int j = blockIdx.x;
+    const int j = march_box(a);
+    if (j < 0) return;
     const int e = a.edge[j] >> a.level;
     const int E2 = e * e;
     const int W  = a.lds_w;
@@ -453,7 +466,8 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
     constexpr int ZR = V > 2 ? V : 2;
     extern __shared__ unsigned char lds_raw[];
     T* pl0      = reinterpret_cast<T*>(lds_raw);
-    const int j = blockIdx.x;
+    const int j = march_box(a);
+    if (j < 0) return;
     const int e = a.edge[j] >> a.level;
     const int E2 = e * e;
     const int W  = a.lds_w;  // max edge + 2 ZR
@@ -667,7 +681,7 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
                 s1 += dred[1][w];
             }
             a.dout[j]             = s0 * a.dvel;
-            a.dout[gridDim.x + j] = s1 * a.dvel;
+            a.dout[a.nbox + j]    = s1 * a.dvel;
         }
     }
 }
@@ -2680,6 +2694,9 @@ int box_tiles_launch_check(const mgx_boxset* bs, int level)
     return (level >= 0 && level < bs->levels) ? MGX_OK : MGX_EINVAL;
 }
 
+// blocks of the per-box marching kernels: one per box, padded to a multiple of 8
+inline unsigned march_grid_dim(const mgx_boxset* bs) { return (unsigned)((bs->nbox + 7) / 8 * 8); }
+
 template <typename T, int TYPE, int MODE, int NT, int P>
 void march_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, const T* in, const T* f, const double* v,
     T* out, T* w, const Coefs& c)
@@ -2690,10 +2707,10 @@ void march_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, const 
     const size_t plane = (size_t)a.lds_w * a.lds_w * sizeof(T);
     constexpr int PD   = (P * sizeof(T) <= 16) ? 2 : 1;
     if (2 * plane * 8 <= 160 * 1024)
-        hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P, 2, PD>), dim3(bs->nbox), dim3(NT), 2 * plane,
+        hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P, 2, PD>), dim3(march_grid_dim(bs)), dim3(NT), 2 * plane,
             ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
     else
-        hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P, 1, PD>), dim3(bs->nbox), dim3(NT), plane, ctx->stream,
+        hipLaunchKernelGGL((k_box_march<T, TYPE, MODE, NT, P, 1, PD>), dim3(march_grid_dim(bs)), dim3(NT), plane, ctx->stream,
             in, f, v, out, w, a, c.c4, c.c2);
 }
 
@@ -2713,10 +2730,10 @@ void march4_launch_d(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, siz
     // two plane buffers when the blocks that fill a CU (32 waves) still fit
     // its 160 KB of LDS
     if (2 * plane * (2048 / NT) <= 160 * 1024)
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V, DOTS, TO>), dim3(bs->nbox), dim3(NT), 2 * plane,
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V, DOTS, TO>), dim3(march_grid_dim(bs)), dim3(NT), 2 * plane,
             ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
     else
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V, DOTS, TO>), dim3(bs->nbox), dim3(NT), plane,
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V, DOTS, TO>), dim3(march_grid_dim(bs)), dim3(NT), plane,
             ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
 }
 
@@ -2751,6 +2768,8 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
     a.g2     = bs->gdim[2] >> level;
     a.lds_w  = emax + 4;
     a.mjf    = mjf;
+    a.order  = bs->d_sp_col;
+    a.nbox   = bs->nbox;
     if (nrm)
     {
         a.nrm   = nrm;
