@@ -53,8 +53,9 @@ C5_POINTS = 2048 * 32 ** 3
 # passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, both calibrated on
 # streaming kernels of known size), copied from profiles/pmc_traffic_c5.md;
 # keyed by (group, precision of the path's kernel)
-TRAFFIC_C5 = {("jacobi0", "f32"): 895234355, ("jacobi0", "f16"): 492663945, ("cellgram", "split"): 1540153344,
-              ("residk", "f64"): 1774300000, ("learned", "bf16"): 17287778496}
+TRAFFIC_C5 = {("jacobi0", "f32"): 893785190, ("jacobi0", "f16"): 403009536, ("cellgram", "split", 1): 1417057075,
+              ("cellgram", "split", 2): 1300885811, ("residk", "f32"): 1957872640,
+              ("learned", "bf16"): 17287778496}
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c5.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
 
 PATHS = {
@@ -80,7 +81,8 @@ GROUPS = {
     "hloc": ("k_box_march4<double, 2, 1, 512, ...> H psi (FP64, V gathered per box)", "hbm"),
     "cellgram": ("k_cell_gram<PREC, true, 1> S/H partials per 8^3 cell (MFMA)", "hbm"),
     "skin": ("k_slab_skin Galerkin H outside each box", "gather"),
-    "residk": ("k_cell_residual<true> R = psi theta - H psi per 8^3 cell (FP64 MFMA)", "hbm"),
+    "residk": ("k_cell_residual<true, OM> R = psi theta - H psi per 8^3 cell (FP64 MFMA), written as the "
+               "V-cycle's zero start f, x (or as R on network-trial steps)", "hbm"),
     "update": ("k_box_and_update_nrm<T, 32> Anderson update + normalisation", "hbm"),
     "jacobi0": ("k_box_march4<T, 2, 2, ...> fine-level Jacobi sweep", "hbm"),
     "jacobi0d": ("k_box_march4<T, 2, 2, ..., DOTS=true> last fine sweep with the Anderson dots", "hbm"),
@@ -165,9 +167,11 @@ def run_mode(ctx, bs, cfg, mode, tol, maxit, steps, warmup, cnn, graph=True, nog
                 seconds_without_graphs=float(np.mean(nograph)) if nograph else None)
 
 
-def group_rates(r, mode, npts):
+def group_rates(r, mode, npts, gram_waves=None):
     """Per kernel group of one profiled solve: ms per solve, launches, average
-    launch, algorithmic work per launch and the rate against its peak."""
+    launch, algorithmic work per launch and the rate against its peak; the
+    PMC traffic of the same kernel where a pass measured it (the cell Gram's
+    for the waves per cell the autotuner chose)."""
     out = {}
     prec = {"plain": "f64", "mixed": "f32", "learned": "f32", "ext16": "f16"}[mode]
     gprec = "split" if PATHS[mode]["gram_prec"] == mgx.PREC_SPLIT else "f64"
@@ -182,7 +186,9 @@ def group_rates(r, mode, npts):
         if work > 0 and bound == "hbm":
             per = work / launches
             ach = per / avg_s / 1e9
-            key = (g, {"cellgram": gprec, "residk": "f64", "learned": "bf16"}.get(g, prec))
+            key = (g, {"cellgram": gprec, "learned": "bf16"}.get(g, prec))
+            if g == "cellgram":
+                key = (g, gprec, gram_waves)
             e.update(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                      frac=round(ach / HBM_PEAK_GBS, 4), bytes_per_launch=int(per),
                      traffic=TRAFFIC_C5.get(key) if npts == C5_POINTS else None)
@@ -312,7 +318,8 @@ def main():
     npts = cfg.psi.size
 
     stream = stream_reference(ctx)
-    kernels = {m: group_rates(r, m, npts) for m, r in res.items()}
+    kernels = {m: group_rates(r, m, npts, tune["gram_split" if PATHS[m]["gram_prec"] == mgx.PREC_SPLIT
+                                                else "gram_f64"][0]) for m, r in res.items()}
     for ks in kernels.values():
         for e in ks.values():
             if e.get("unit") == "GB/s":

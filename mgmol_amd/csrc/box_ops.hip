@@ -3125,8 +3125,8 @@ int box_precond_t(mgx_ctx* ctx, const mgx_boxset* bs, int type, int max_levels, 
     if (rc) return rc;
     // from x0 = 0 the first fine sweep is pointwise and fused with the cast
     const bool zero = gamma == 0.;
-    // the residual read (FP64), f and x written
-    const int tk = prof_begin(ctx, MGX_PROF_FIRST, (8. + 2. * sizeof(T)) * (double)n);
+    // the residual read (FP64), f and x written (no launch with a given start)
+    const int tk = start_given ? -1 : prof_begin(ctx, MGX_PROF_FIRST, (8. + 2. * sizeof(T)) * (double)n);
     // start_given: the residual kernel has written f and the first sweep's x
     // already (box_residual_start)
     MGX_REQUIRE(ctx, zero || !start_given, "a given start is the zero start");
