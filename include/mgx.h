@@ -337,7 +337,7 @@ int mgx_box_precond_mg(mgx_ctx* ctx, const mgx_boxset* bs, int lap_type, int max
  * outside a box psi is zero. */
 int mgx_box_gram_h(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double* psi,
     const double* hpsi, double* S, double* H);
-/* res_j = sum_k psi_k theta_kj - hpsi_j on box j */
+/* res_j = sum_k psi_k theta_kj - hpsi_j on box j; res must not overlap psi or hpsi */
 int mgx_box_residual(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const double* hpsi,
     const double* theta, double* res);
 /* d_j = vel*|r_j|^2 (device n); norm = 0.5*sqrt(sum d_j Sinv_jj) host out */

@@ -45,6 +45,7 @@ struct mgx_boxset
     int ncell = 0, kmax = 0;
     int* d_cell_ptr  = nullptr;
     int* d_cell_ent  = nullptr;
+    void* d_cell_ce  = nullptr;  // the entries as the kernels read them (mgx::CellEnt)
     int64_t* d_cell_slot = nullptr;  // base of the cell's k*k partial block
     int64_t nslot    = 0;
     // per pair: the partial slots (ab, ba) of every cell both boxes cover
@@ -1422,33 +1423,23 @@ constexpr int kCellK = 32;   // max boxes per cell handled by the cell kernels
 typedef double d4c __attribute__((ext_vector_type(4)));
 typedef float f4c __attribute__((ext_vector_type(4)));
 
+// one box covering a cell, as the cell kernels read it: built once with the
+// cell tables (base = level-0 offset of the box's data, e = its edge, b = the
+// cell origin in box-local coordinates, [lo, hi) = the covered cell-local
+// range), so that a block reads its entries in one dependent load
 struct CellEnt
 {
     int64_t base;  // offset of the box's data
     int e;
     int b0, b1, b2;
     int lo0, lo1, lo2, hi0, hi1, hi2;
+    int box;
 };
 
-__device__ __forceinline__ void load_cell(const int* ent, const int64_t* off, const int* edge, int k, CellEnt* ce)
+__device__ __forceinline__ void load_cell(const CellEnt* cce, int k, CellEnt* ce)
 {
     for (int a = threadIdx.x; a < k; a += blockDim.x)
-    {
-        const int* E = ent + 10 * a;
-        CellEnt c;
-        c.base = off[E[0]];
-        c.e    = edge[E[0]];
-        c.b0 = E[1];
-        c.b1 = E[2];
-        c.b2 = E[3];
-        c.lo0 = E[4];
-        c.lo1 = E[5];
-        c.lo2 = E[6];
-        c.hi0 = E[7];
-        c.hi1 = E[8];
-        c.hi2 = E[9];
-        ce[a] = c;
-    }
+        ce[a] = cce[a];
 }
 
 // index of cell point p (x = p>>6, y = (p>>3)&7, z = p&7) in the box, or -1
@@ -1676,9 +1667,8 @@ __device__ __forceinline__ void cell_gram_packed(const double* psi, const double
 // the loads of a step fill the wave (a cell is covered by 5-6 boxes on
 // average when 32^3 boxes cover the grid 4 times).
 template <int PREC, bool PAIR, int NW>
-__global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, const double* hpsi, const int64_t* off,
-    const int* edge, const int* cell_ptr, const int* cell_ent, const int64_t* cell_slot, int64_t nslot, double* part,
-    int ncell)
+__global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, const double* hpsi, const int* cell_ptr,
+    const CellEnt* cce, const int64_t* cell_slot, int64_t nslot, double* part, int ncell)
 {
     static_assert(PAIR || NW == 4, "the unpaired tiles take four waves");
     static_assert(!PAIR || NW == 1 || NW == 2, "the paired tiles take one or two waves");
@@ -1691,7 +1681,7 @@ __global__ void __launch_bounds__(64 * NW, 5) k_cell_gram(const double* psi, con
     if (cl >= ncell) return;
     const int cell = (int)cl;
     const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
-    load_cell(cell_ent + 10 * e0, off, edge, k, ce);
+    load_cell(cce + e0, k, ce);
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int c = lane & 15, r = lane >> 4;
@@ -1986,30 +1976,36 @@ __device__ __forceinline__ void res_put2(double* res, void* fo, void* xo, double
     }
 }
 
+// The outputs never alias the inputs.  A block reads its cell's entries (one
+// dependent load after the cell pointer); with paired loads and k <= 16 each
+// lane then loads its theta operands itself, at the same time as the first
+// psi / H psi batch (no LDS stage and no second barrier); otherwise theta goes
+// through LDS.
 template <bool PAIR, int OM>
-__global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, const double* hpsi, const double* theta,
-    int n, const int64_t* off, const int* edge, const int* cell_ptr, const int* cell_ent, double* res, double* esq,
-    int ncell, void* fo, void* xo, double jf)
+__global__ void __launch_bounds__(256, 4) k_cell_residual(const double* __restrict__ psi,
+    const double* __restrict__ hpsi, const double* __restrict__ theta, int n, const int* cell_ptr, const CellEnt* cce,
+    double* __restrict__ res, double* __restrict__ esq, int ncell, void* __restrict__ fo, void* __restrict__ xo,
+    double jf)
 {
     __shared__ CellEnt ce[kCellK];
     __shared__ double th[kCellK * kCellK];
-    __shared__ int jid[kCellK];
     __shared__ double wsq[4][kCellK];
     // consecutive cells (z-neighbours share cache lines) on one XCD
     const int64_t cl = xcd_block(blockIdx.x, gridDim.x);
     if (cl >= ncell) return;
     const int cell = (int)cl;
     const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
-    load_cell(cell_ent + 10 * e0, off, edge, k, ce);
-    for (int t = threadIdx.x; t < k; t += blockDim.x)
-        jid[t] = cell_ent[10 * (e0 + t)];
+    load_cell(cce + e0, k, ce);
     __syncthreads();
-    for (int t = threadIdx.x; t < kCellK * kCellK; t += blockDim.x)
+    if (!PAIR || k > 16)
     {
-        const int aa = t / kCellK, bb = t - (t / kCellK) * kCellK;
-        th[t] = (aa < k && bb < k) ? theta[(int64_t)jid[bb] * n + jid[aa]] : 0.;
+        for (int t = threadIdx.x; t < kCellK * kCellK; t += blockDim.x)
+        {
+            const int aa = t / kCellK, bb = t - (t / kCellK) * kCellK;
+            th[t] = (aa < k && bb < k) ? theta[(int64_t)ce[bb].box * n + ce[aa].box] : 0.;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int c = lane & 15, r = lane >> 4;
     const int nct = (k + 15) / 16;
@@ -2020,61 +2016,75 @@ __global__ void __launch_bounds__(256, 6) k_cell_residual(const double* psi, con
     {
         // even box origins and edges: each lane takes two adjacent z-points
         // (16-byte loads and stores), which feed two MFMAs with the same
-        // theta operand; 32-point tiles, wave w takes tiles w + 4 jj, RJP
-        // per load batch (the pairs hold twice the registers)
+        // theta operand; 32-point tiles, wave w takes tiles w + 4 jb.  The
+        // loads of tile jb + 1 are issued before the products of tile jb
+        // (two register sets), and those of the first tile before theta's.
         typedef double d2 __attribute__((ext_vector_type(2)));
         const d2 z2 = { 0., 0. };
-#pragma unroll 1
-        for (int jb = 0; jb < 4; jb += RJP)
-        {
-            d2 xa[RJP][4], hv[RJP][4];
-            int64_t ib[RJP][4];
+        // psi of boxes 4 st + r and H psi of boxes r + 4 v at the lane's pair
+        auto issue = [&](int jb, d2(&xa)[4], d2(&hv)[4]) {
+            const int pp = (wid + 4 * jb) * 32 + 2 * c;
 #pragma unroll
-            for (int u = 0; u < RJP; u++)
+            for (int st = 0; st < 4; st++)
             {
-                const int pp = (wid + 4 * (jb + u)) * 32 + 2 * c;
-#pragma unroll
-                for (int st = 0; st < 4; st++)
+                const int aa = 4 * st + r;
+                xa[st]       = z2;
+                if (aa < k)
                 {
-                    const int aa = 4 * st + r;
-                    xa[u][st]    = z2;
-                    if (aa < k)
-                    {
-                        const int64_t ia = cell_idx(ce[aa], pp);
-                        if (ia >= 0) xa[u][st] = *reinterpret_cast<const d2*>(psi + ia);
-                    }
-                }
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                {
-                    const int bb = r + 4 * v;
-                    ib[u][v]     = bb < k ? cell_idx(ce[bb], pp) : -1;
-                    hv[u][v]     = ib[u][v] >= 0 ? *reinterpret_cast<const d2*>(hpsi + ib[u][v]) : z2;
+                    const int64_t ia = cell_idx(ce[aa], pp);
+                    if (ia >= 0) xa[st] = *reinterpret_cast<const d2*>(psi + ia);
                 }
             }
 #pragma unroll
-            for (int u = 0; u < RJP; u++)
+            for (int v = 0; v < 4; v++)
             {
-                d4c acc0 = { 0., 0., 0., 0. }, acc1 = acc0;
+                const int bb = r + 4 * v;
+                hv[v]        = z2;
+                if (bb < k)
+                {
+                    const int64_t ib = cell_idx(ce[bb], pp);
+                    if (ib >= 0) hv[v] = *reinterpret_cast<const d2*>(hpsi + ib);
+                }
+            }
+        };
+        d2 xa[2][4], hv[2][4];
+        issue(0, xa[0], hv[0]);
+        for (int t = threadIdx.x; t < kCellK * kCellK; t += blockDim.x)
+        {
+            const int aa = t / kCellK, bb = t - (t / kCellK) * kCellK;
+            th[t] = (aa < k && bb < k) ? theta[(int64_t)ce[bb].box * n + ce[aa].box] : 0.;
+        }
+        __syncthreads();
 #pragma unroll
-                for (int st = 0; st < 4; st++)
-                    if (st < ksteps)
-                    {
-                        const double t = th[(4 * st + r) * kCellK + c];
-                        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[u][st][0], acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[u][st][1], acc1, 0, 0, 0);
-                    }
+        for (int jb = 0; jb < 4; jb++)
+        {
+            const int cur = jb & 1;
+            if (jb + 1 < 4) issue(jb + 1, xa[cur ^ 1], hv[cur ^ 1]);
+            const int pp = (wid + 4 * jb) * 32 + 2 * c;
+            d4c acc0 = { 0., 0., 0., 0. }, acc1 = acc0;
 #pragma unroll
-                for (int v = 0; v < 4; v++)
-                    if (ib[u][v] >= 0)
-                    {
-                        d2 rv;
-                        rv[0] = acc0[v] + (-1. * hv[u][v][0]);
-                        rv[1] = acc1[v] + (-1. * hv[u][v][1]);
-                        res_put2<OM>(res, fo, xo, jf, ib[u][v], rv[0], rv[1]);
-                        sq[0][v] += rv[0] * rv[0];
-                        sq[0][v] += rv[1] * rv[1];
-                    }
+            for (int st = 0; st < 4; st++)
+                if (st < ksteps)
+                {
+                    const double t = th[(4 * st + r) * kCellK + c];
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[cur][st][0], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(t, xa[cur][st][1], acc1, 0, 0, 0);
+                }
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+            {
+                const int bb = r + 4 * v;
+                if (bb >= k) continue;
+                const int64_t ib = cell_idx(ce[bb], pp);
+                if (ib >= 0)
+                {
+                    d2 rv;
+                    rv[0] = acc0[v] + (-1. * hv[cur][v][0]);
+                    rv[1] = acc1[v] + (-1. * hv[cur][v][1]);
+                    res_put2<OM>(res, fo, xo, jf, ib, rv[0], rv[1]);
+                    sq[0][v] += rv[0] * rv[0];
+                    sq[0][v] += rv[1] * rv[1];
+                }
             }
         }
     }
@@ -2204,9 +2214,8 @@ __global__ void __launch_bounds__(64) k_box_from_entries(const double* esq, cons
 // one cell, so rho is written once per point (points no box covers stay as
 // the caller set them); csum[cell] = the cell's sum of rho, in a fixed order.
 template <bool PAIR>
-__global__ void __launch_bounds__(256) k_cell_density(const double* psi, const double* X, int n, const int64_t* off,
-    const int* edge, const int* origin, int g0, int g1, int g2, const int* cell_ptr, const int* cell_ent, double* rho,
-    double* csum, int ncell)
+__global__ void __launch_bounds__(256) k_cell_density(const double* psi, const double* X, int n, const int* origin, int g0,
+    int g1, int g2, const int* cell_ptr, const CellEnt* cce, double* rho, double* csum, int ncell)
 {
     __shared__ CellEnt ce[kCellK];
     __shared__ double xs[kCellK * kCellK];
@@ -2216,9 +2225,9 @@ __global__ void __launch_bounds__(256) k_cell_density(const double* psi, const d
     if (cl >= ncell) return;
     const int cell = (int)cl;
     const int e0 = cell_ptr[cell], k = cell_ptr[cell + 1] - e0;
-    load_cell(cell_ent + 10 * e0, off, edge, k, ce);
+    load_cell(cce + e0, k, ce);
     for (int t = threadIdx.x; t < k; t += blockDim.x)
-        jid[t] = cell_ent[10 * (e0 + t)];
+        jid[t] = cce[e0 + t].box;
     __syncthreads();
     for (int t = threadIdx.x; t < kCellK * kCellK; t += blockDim.x)
     {
@@ -3303,8 +3312,8 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
         // psi and H psi read once (FP64)
         const int tk   = prof_begin(ctx, MGX_PROF_CELLGRAM, 16. * (double)bs->off[0][bs->nbox]);
         hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(nthr), 0, ctx->stream, psi, hpsi,
-            (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-            (const int*)bs->d_cell_ent, (const int64_t*)bs->d_cell_slot, bs->nslot, part, bs->ncell);
+            (const int*)bs->d_cell_ptr, (const CellEnt*)bs->d_cell_ce, (const int64_t*)bs->d_cell_slot, bs->nslot,
+            part, bs->ncell);
         prof_end(ctx, tk);
     }
     MGX_CHECK_LAUNCH(ctx);
@@ -3511,8 +3520,7 @@ void cell_residual_launch(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi,
 {
     auto kern = bs->pair_z ? k_cell_residual<true, OM> : k_cell_residual<false, OM>;
     hipLaunchKernelGGL(kern, dim3(cell_grid(bs->ncell)), dim3(256), 0, ctx->stream, psi, hpsi, theta, bs->nbox,
-        (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_cell_ptr,
-        (const int*)bs->d_cell_ent, res, esq, bs->ncell, fo, xo, jf);
+        (const int*)bs->d_cell_ptr, (const CellEnt*)bs->d_cell_ce, res, esq, bs->ncell, fo, xo, jf);
 }
 
 // R = psi theta - hpsi per box; with d: also d_j = vel |R_j|^2 (computed in
@@ -3582,9 +3590,8 @@ int box_density(mgx_ctx* ctx, const mgx_boxset* bs, const double* psi, const dou
     if (!cs) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     hipLaunchKernelGGL(k_fill_b<double>, dim3(nblocks(ng)), dim3(kBlock), 0, ctx->stream, rho, ng, 0.);
     hipLaunchKernelGGL(bs->pair_z ? k_cell_density<true> : k_cell_density<false>, dim3(cell_grid(bs->ncell)),
-        dim3(256), 0, ctx->stream, psi, X, n,
-        (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_origin, bs->gdim[0], bs->gdim[1],
-        bs->gdim[2], (const int*)bs->d_cell_ptr, (const int*)bs->d_cell_ent, rho, cs, bs->ncell);
+        dim3(256), 0, ctx->stream, psi, X, n, (const int*)bs->d_origin, bs->gdim[0], bs->gdim[1], bs->gdim[2],
+        (const int*)bs->d_cell_ptr, (const CellEnt*)bs->d_cell_ce, rho, cs, bs->ncell);
     const double vel = bs->h[0] * bs->h[1] * bs->h[2];
     hipLaunchKernelGGL(k_sum_scaled, dim3(1), dim3(256), 0, ctx->stream, (const double*)cs, bs->ncell, vel,
         cs + bs->ncell);
@@ -4084,6 +4091,32 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset
             if (!ok) return fail("hipMalloc cell tables");
             hipMemcpy(s->d_cell_ptr, cptr.data(), sizeof(int) * cptr.size(), hipMemcpyHostToDevice);
             if (!cent.empty()) hipMemcpy(s->d_cell_ent, cent.data(), sizeof(int) * cent.size(), hipMemcpyHostToDevice);
+            {
+                // the entries with the box offsets and edges resolved
+                std::vector<CellEnt> ce(cent.size() / 10);
+                for (size_t q = 0; q < ce.size(); q++)
+                {
+                    const int* E = &cent[10 * q];
+                    CellEnt c;
+                    c.base = s->off[0][E[0]];
+                    c.e    = s->edge[E[0]];
+                    c.b0   = E[1];
+                    c.b1   = E[2];
+                    c.b2   = E[3];
+                    c.lo0  = E[4];
+                    c.lo1  = E[5];
+                    c.lo2  = E[6];
+                    c.hi0  = E[7];
+                    c.hi1  = E[8];
+                    c.hi2  = E[9];
+                    c.box  = E[0];
+                    ce[q]  = c;
+                }
+                if (hipMalloc(&s->d_cell_ce, sizeof(CellEnt) * std::max<size_t>(ce.size(), 1)) != hipSuccess)
+                    return fail("hipMalloc cell entries");
+                if (!ce.empty())
+                    hipMemcpy(s->d_cell_ce, ce.data(), sizeof(CellEnt) * ce.size(), hipMemcpyHostToDevice);
+            }
             if (!cslot.empty())
                 hipMemcpy(s->d_cell_slot, cslot.data(), sizeof(int64_t) * cslot.size(), hipMemcpyHostToDevice);
             hipMemcpy(s->d_pslot_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice);
@@ -4165,6 +4198,7 @@ int mgx_boxset_destroy(mgx_boxset* s)
     if (s->d_nbr) hipFree(s->d_nbr);
     if (s->d_cell_ptr) hipFree(s->d_cell_ptr);
     if (s->d_cell_ent) hipFree(s->d_cell_ent);
+    if (s->d_cell_ce) hipFree(s->d_cell_ce);
     if (s->d_cell_slot) hipFree(s->d_cell_slot);
     if (s->d_pslot_ptr) hipFree(s->d_pslot_ptr);
     if (s->d_pslot) hipFree(s->d_pslot);
