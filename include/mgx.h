@@ -177,8 +177,10 @@ typedef struct
 } mgx_poisson_stats_t;
 int mgx_poisson_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, mgx_block_t* vh,
     int nu1, int nu2, int max_sweeps, double tol, int max_nlevels, mgx_poisson_stats_t* stats);
-/* SolverPB<PBh4M<double>, double>::solve(phi, rhs, rhod, vks), src/pb/SolverPB.cc:56-84
- * (PBdiel::solve, src/PBdiel.cc:30-68): the Poisson-Boltzmann potential in a
+/* SolverPB<PBh4M / PBh4 / PBh2<double>, double>::solve(phi, rhs, rhod, vks),
+ * src/pb/SolverPB.cc:56-84 (PBdiel::solve, src/PBdiel.cc:30-68, with the operator
+ * PoissonSolverFactory::createDiel picks, src/PoissonSolverFactory.h:151-190):
+ * the Poisson-Boltzmann potential in a
  * dielectric continuum eps(rhod) (DielFunc::epsilon_rho, src/pb/DielFunc.cc:38-43:
  * 1 + (e0-1)/2 ((1-r)/(1+r) + 1), r = (rhod/rho0)^(2 drho0)), as pb::Mgm solves it
  * with PBh4M (src/pb/PBh4M.h, src/pb/PBh4M.cc): vh *= sqrt(eps), rhs = B(rho /
@@ -186,12 +188,18 @@ int mgx_poisson_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, mgx_block
  * the PBh4M Jacobi smoother (x -= 0.75 / (c0 + pp) (A x - f), c0 = 30/12 sum 1/h^2)
  * and PBh4M on every coarse level (pp restricted), vh /= sqrt(eps) at the end (no
  * mean removal); then, if vks is not null, PBh4M::get_vepsilon: vks = d eps/d rho
- * |grad vh|^2 (-1/(8 pi)), the gradient of 4th order.  lap_type must be
- * MGX_LAP_MEHRSTELLEN (PBh2/PBh4 are not built); rho, rhod, vh, vks: one FP64
- * function each, same grid, 2 ghosts at least, bc per direction 1 or 0; the
- * dielectric functions are periodic whatever bc says.  The coarse levels keep pp
- * and the functions on the same points (the reference pairs pp laid out with 2
- * ghosts with 1-ghost coarse functions element by element). */
+ * |grad vh|^2 (-1/(8 pi)), the gradient of 4th order.  That is lap_type
+ * MGX_LAP_MEHRSTELLEN.  MGX_LAP_4TH (PBh4, src/pb/PBh4.cc) and MGX_LAP_2ND (PBh2,
+ * src/pb/PBh2.cc) take the gradient form on the untransformed potential: A x =
+ * eps (-Lap x) - grad x . grad eps (first differences del1_4th / del1_2nd),
+ * rhs = rho, the Jacobi smoother x -= (A x - f) / (c0 eps) with c0 = 1.5 (30/12)
+ * sum 1/h^2, PBh2 on every coarse level (USE_LOWER_ORDER) on the restricted eps,
+ * and get_vepsilon's gradient of the operator's order.  rho, rhod, vh, vks: one
+ * FP64 function each, same grid, 2 ghosts at least (1 for MGX_LAP_2ND), bc per
+ * direction 1 or 0; the dielectric functions are periodic whatever bc says.  With
+ * PBh4M the coarse levels keep pp and the functions on the same points (the
+ * reference pairs pp laid out with 2 ghosts with 1-ghost coarse functions element
+ * by element). */
 int mgx_pb_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, const mgx_block_t* rhod,
     mgx_block_t* vh, mgx_block_t* vks, double e0, double rho0, double drho0, int nu1, int nu2,
     int max_sweeps, double tol, int max_nlevels, mgx_poisson_stats_t* stats);

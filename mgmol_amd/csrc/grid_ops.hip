@@ -1370,16 +1370,76 @@ __device__ __forceinline__ int64_t gpoint(const GGeom& g, int64_t t, int* xo = n
 // DielFunc::epsilon_rho (src/pb/DielFunc.cc:38-43) at every interior point,
 // then sqrt_a = sqrt(eps) and inv_sqrt_a = 1 / sqrt_a (GridFunc::sqrt_func,
 // GridFunc::inv)
+__device__ __forceinline__ double pb_eps_of(double r, double e0, double rho0, double drho0)
+{
+    const double ratio = pow(r / rho0, 2. * drho0);
+    return 1. + 0.5 * (e0 - 1.) * ((1. - ratio) / (1. + ratio) + 1.);
+}
+
 __global__ void k_pb_eps(const double* rhod, GGeom g, double e0, double rho0, double drho0, double* sa, double* isa)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
     {
-        const int64_t i    = gpoint(g, t);
-        const double ratio = pow(rhod[i] / rho0, 2. * drho0);
-        const double eps   = 1. + 0.5 * (e0 - 1.) * ((1. - ratio) / (1. + ratio) + 1.);
-        const double s     = sqrt(eps);
-        sa[i]              = s;
-        isa[i]             = 1. / s;
+        const int64_t i = gpoint(g, t);
+        const double s  = sqrt(pb_eps_of(rhod[i], e0, rho0, drho0));
+        sa[i]           = s;
+        isa[i]          = 1. / s;
+    }
+}
+
+// PBh4<T>::init / PBh2 (DielFunc::Gepsilon_rho): eps at every interior point
+__global__ void k_pbg_eps(const double* rhod, GGeom g, double e0, double rho0, double drho0, double* eps)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        eps[i]          = pb_eps_of(rhod[i], e0, rho0, drho0);
+    }
+}
+
+// the first difference of FDoper::del1_4th (ORDER 4, src/pb/FDoper.cc:119-178)
+// or del1_2nd (ORDER 2, :72-115) along the axis of stride c, ih = 1 / h
+template <int ORDER>
+__device__ __forceinline__ double pb_d1(const double* u, int64_t c, double ih)
+{
+    if constexpr (ORDER == 4)
+    {
+        const double e1 = (8. / 12.) * ih, e2 = (1. / 12.) * ih;
+        return e1 * (u[c] - u[-c]) + e2 * (u[-2 * c] - u[2 * c]);
+    }
+    else
+        return (0.5 * ih) * (u[c] - u[-c]);
+}
+
+// the gradient form of -div(eps grad x), PBh4<T>::pb_4th (src/pb/PBh4.cc:13-28)
+// and PBh2<T>::pb_2nd (src/pb/PBh2.cc:13-30): out holds -Lap x on entry;
+// out = (((0 - dx0 de0) - dx1 de1) - dx2 de2) + (-Lap x) eps, x and eps with
+// fresh ghosts
+template <int ORDER>
+__global__ void k_pbg_grad(const double* x, const double* eps, GGeom g, double ih0, double ih1, double ih2,
+    double* out)
+{
+    const int64_t inc[3] = { g.incx, g.incy, 1 };
+    const double ih[3]   = { ih0, ih1, ih2 };
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        double acc      = 0.;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            acc = acc - pb_d1<ORDER>(x + i, inc[d], ih[d]) * pb_d1<ORDER>(eps + i, inc[d], ih[d]);
+        out[i] = acc + out[i] * eps[i];
+    }
+}
+
+// PBh4<T>::jacobi / PBh2<T>::jacobi (GridFunc<T>::jacobi,
+// src/pb/GridFunc.cc:3264-3280): x -= (1 / (c0 eps)) (w - f), w = A x
+__global__ void k_pbg_jacobi(double* x, const double* w, const double* f, const double* eps, GGeom g, double c0)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t i = gpoint(g, t);
+        x[i] -= (1. / (c0 * eps[i])) * (w[i] - f[i]);
     }
 }
 
@@ -1421,19 +1481,20 @@ __global__ void k_pb_jacobi(double* x, const double* w, const double* f, const d
     }
 }
 
-// interior out = a * b (mode 0) or a / b (mode 1)
+// interior out = a * b (mode 0), a / b (mode 1) or a (mode 2)
 __global__ void k_pb_scale(const double* a, const double* b, GGeom g, int mode, double* out)
 {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < g.size; t += (int64_t)gridDim.x * blockDim.x)
     {
         const int64_t i = gpoint(g, t);
-        out[i]          = mode == 0 ? a[i] * b[i] : a[i] / b[i];
+        out[i]          = mode == 0 ? a[i] * b[i] : (mode == 1 ? a[i] / b[i] : a[i]);
     }
 }
 
-// PBh4M<T>::get_vepsilon (src/pb/PBh4M.cc:138-162): vks = depsilon(rhod)
-// |grad vh|^2 (-1/(8 pi)), the gradient by FDoper::del1_4th
-// (src/pb/FDoper.cc:119-178); vh with fresh ghosts (2)
+// get_vepsilon of PBh4M / PBh4 (src/pb/PBh4M.cc:138-162, src/pb/PBh4.cc:49-63)
+// and PBh2 (src/pb/PBh2.cc:57-72): vks = depsilon(rhod) |grad vh|^2 (-1/(8 pi)),
+// the gradient by del1_4th (ORDER 4) or del1_2nd (ORDER 2); vh with fresh ghosts
+template <int ORDER>
 __global__ void k_pb_vks(const double* vh, const double* rhod, GGeom g, double h0, double h1, double h2, double e0,
     double rho0, double drho0, double* out)
 {
@@ -1445,9 +1506,7 @@ __global__ void k_pb_vks(const double* vh, const double* rhod, GGeom g, double h
         double g2       = 0.;
         for (int d = 0; d < 3; d++)
         {
-            const double e1 = (8. / 12.) * (1. / hh[d]), e2 = (1. / 12.) * (1. / hh[d]);
-            const int64_t c = inc[d];
-            const double dv = e1 * (vh[i + c] - vh[i - c]) + e2 * (vh[i - 2 * c] - vh[i + 2 * c]);
+            const double dv = pb_d1<ORDER>(vh + i, inc[d], 1. / hh[d]);
             g2 += dv * dv;
         }
         const double r = rhod[i];
@@ -1671,23 +1730,31 @@ int poisson_run(mgx_ctx* ctx, int type, const mgx_grid_t& g0, const double* rho,
     return MGX_OK;
 }
 
-// SolverPB<PBh4M>::solve (src/pb/SolverPB.cc:56-84): the dielectric
-// operator's Mgm on the transformed potential, then get_vepsilon
+// SolverPB<PBh4M / PBh4 / PBh2>::solve (src/pb/SolverPB.cc:56-84): the
+// dielectric operator's Mgm (on the transformed potential for PBh4M), then
+// get_vepsilon.  op per level: MGX_LAP_MEHRSTELLEN (PBh4M, coef = pp),
+// MGX_LAP_4TH (PBh4, coef = eps) or MGX_LAP_2ND (PBh2, coef = eps)
 struct PBLevel
 {
     mgx_grid_t g;
+    int op;
     double *pp, *x, *f, *w, *t;
 };
 
-// the PBh4M operator at level l: out = -Lap_M x + B(pp x) (x gets its halo)
+// the dielectric operator at level l (x gets its halo): PBh4M, out =
+// -Lap_M x + B(pp x); PBh4 / PBh2, out = eps (-Lap x) - grad x . grad eps
 int pb_apply(mgx_ctx* ctx, const PBLevel& L, double* x, double* out)
 {
     int rc;
     if ((rc = launch_halo(ctx, x, 1, L.g))) return rc;
-    if ((rc = launch_lap(ctx, MGX_LAP_MEHRSTELLEN, (const double*)x, out, 1, L.g))) return rc;
+    if ((rc = launch_lap(ctx, L.op, (const double*)x, out, 1, L.g))) return rc;
     const GGeom a = ggeom(L.g);
-    hipLaunchKernelGGL(k_pb_bterm, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)x,
-        (const double*)L.pp, a, out);
+    if (L.op == MGX_LAP_MEHRSTELLEN)
+        hipLaunchKernelGGL(k_pb_bterm, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)x,
+            (const double*)L.pp, a, out);
+    else
+        hipLaunchKernelGGL(L.op == MGX_LAP_4TH ? k_pbg_grad<4> : k_pbg_grad<2>, dim3(nblocks(a.size)), dim3(kBlock), 0,
+            ctx->stream, (const double*)x, (const double*)L.pp, a, 1. / L.g.h[0], 1. / L.g.h[1], 1. / L.g.h[2], out);
     MGX_CHECK_LAUNCH(ctx);
     return MGX_OK;
 }
@@ -1695,18 +1762,20 @@ int pb_apply(mgx_ctx* ctx, const PBLevel& L, double* x, double* out)
 int pb_vcycle(mgx_ctx* ctx, std::vector<PBLevel>& Lv, PoissonMG& P, int l, double* x, const double* f, int nu1,
     int nu2)
 {
-    const PBLevel& L = Lv[l];
-    const GGeom a    = ggeom(L.g);
-    const double c0  = (30. / 12.) * (1. / (L.g.h[0] * L.g.h[0]) + 1. / (L.g.h[1] * L.g.h[1]) +
-                                       1. / (L.g.h[2] * L.g.h[2]));
+    const PBLevel& L  = Lv[l];
+    const GGeom a     = ggeom(L.g);
+    const double ih2  = 1. / (L.g.h[0] * L.g.h[0]) + 1. / (L.g.h[1] * L.g.h[1]) + 1. / (L.g.h[2] * L.g.h[2]);
+    const bool mehr   = L.op == MGX_LAP_MEHRSTELLEN;
+    // PBh4M: (30/12) sum 1/h^2 (+ pp in the kernel); PBh4, PBh2: 1.5 (30/12) sum 1/h^2
+    const double c0   = mehr ? (30. / 12.) * ih2 : 1.5 * (30. / 12.) * ih2;
     int rc;
     auto sweeps = [&](int k) -> int {
         for (int i = 0; i < k; i++)
         {
             int r = pb_apply(ctx, L, x, L.w);
             if (r) return r;
-            hipLaunchKernelGGL(k_pb_jacobi, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, x,
-                (const double*)L.w, f, (const double*)L.pp, a, c0);
+            hipLaunchKernelGGL(mehr ? k_pb_jacobi : k_pbg_jacobi, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream,
+                x, (const double*)L.w, f, (const double*)L.pp, a, c0);
             MGX_CHECK_LAUNCH(ctx);
         }
         return MGX_OK;
@@ -1729,14 +1798,18 @@ int pb_vcycle(mgx_ctx* ctx, std::vector<PBLevel>& Lv, PoissonMG& P, int l, doubl
     return sweeps(nu2);
 }
 
-int pb_run(mgx_ctx* ctx, const mgx_grid_t& g0, const double* rho, const double* rhod, double* vh, double* vks,
-    double e0, double rho0, double drho0, int nu1, int nu2, int max_sweeps, double tol, int max_nlevels,
+int pb_run(mgx_ctx* ctx, int op, const mgx_grid_t& g0, const double* rho, const double* rhod, double* vh,
+    double* vks, double e0, double rho0, double drho0, int nu1, int nu2, int max_sweeps, double tol, int max_nlevels,
     mgx_poisson_stats_t* st)
 {
+    const bool mehr = op == MGX_LAP_MEHRSTELLEN;
     // levels: the fine grid, then while coarsenable (every dimension even and
     // at least twice the ghost width) grids of half the points with one ghost
+    // (PBh4 and PBh2 take PBh2 below the fine level: USE_LOWER_ORDER,
+    // PBh4<T>::getLowerOrderOp, src/pb/PBh4.h:72-80)
     std::vector<PBLevel> Lv(1);
-    Lv[0].g = g0;
+    Lv[0].g  = g0;
+    Lv[0].op = op;
     while ((int)Lv.size() - 1 < max_nlevels)
     {
         const mgx_grid_t& p = Lv.back().g;
@@ -1747,6 +1820,7 @@ int pb_run(mgx_ctx* ctx, const mgx_grid_t& g0, const double* rho, const double* 
         PBLevel c;
         c.g        = coarse_of(p);
         c.g.ghosts = 1;
+        c.op       = mehr ? MGX_LAP_MEHRSTELLEN : MGX_LAP_2ND;
         Lv.push_back(c);
     }
     const int nl     = (int)Lv.size();
@@ -1791,17 +1865,24 @@ int pb_run(mgx_ctx* ctx, const mgx_grid_t& g0, const double* rho, const double* 
     gp.bc[0] = gp.bc[1] = gp.bc[2] = 1;
     const GGeom a = ggeom(g0);
     int rc;
-    // PBh4M<T>::init: eps, sqrt_a, inv_sqrt_a, pp = (-(-Lap_4 sqrt_a)) inv_sqrt_a
-    hipLaunchKernelGGL(k_pb_eps, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rhod, a, e0, rho0, drho0, sa, isa);
-    MGX_CHECK_LAUNCH(ctx);
-    if ((rc = launch_halo(ctx, sa, 1, gp)) || (rc = launch_halo(ctx, isa, 1, gp))) return rc;
-    if ((rc = launch_lap(ctx, MGX_LAP_4TH, (const double*)sa, Lv[0].t, 1, gp))) return rc;
-    hipLaunchKernelGGL(k_pb_pp, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)Lv[0].t,
-        (const double*)isa, a, Lv[0].pp);
+    if (mehr)
+    {
+        // PBh4M<T>::init: eps, sqrt_a, inv_sqrt_a, pp = (-(-Lap_4 sqrt_a)) inv_sqrt_a
+        hipLaunchKernelGGL(k_pb_eps, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rhod, a, e0, rho0, drho0, sa,
+            isa);
+        MGX_CHECK_LAUNCH(ctx);
+        if ((rc = launch_halo(ctx, sa, 1, gp)) || (rc = launch_halo(ctx, isa, 1, gp))) return rc;
+        if ((rc = launch_lap(ctx, MGX_LAP_4TH, (const double*)sa, Lv[0].t, 1, gp))) return rc;
+        hipLaunchKernelGGL(k_pb_pp, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)Lv[0].t,
+            (const double*)isa, a, Lv[0].pp);
+    }
+    else
+        hipLaunchKernelGGL(k_pbg_eps, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rhod, a, e0, rho0, drho0,
+            Lv[0].pp);
     MGX_CHECK_LAUNCH(ctx);
     if ((rc = launch_halo(ctx, Lv[0].pp, 1, gp))) return rc;
-    // the coarse operators' pp: full-weighting restriction of the finer pp
-    // (traded periodically first), with periodic ghosts
+    // the coarse operators' pp (eps): full-weighting restriction of the
+    // finer level's (traded periodically first), with periodic ghosts
     for (int l = 1; l < nl; l++)
     {
         mgx_grid_t cp = Lv[l].g;
@@ -1810,15 +1891,21 @@ int pb_run(mgx_ctx* ctx, const mgx_grid_t& g0, const double* rho, const double* 
             return rc;
         if ((rc = launch_halo(ctx, Lv[l].pp, 1, cp))) return rc;
     }
-    // Mgm: vh /= inv_sqrt_a; rhs = B(rho inv_sqrt_a)
-    hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh,
-        (const double*)isa, a, 1, vh);
-    hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rho, (const double*)isa, a, 0,
-        r0);
-    MGX_CHECK_LAUNCH(ctx);
-    if ((rc = launch_halo(ctx, r0, 1, g0))) return rc;
-    hipLaunchKernelGGL(k_rhs<double>, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)r0, rhs, a,
-        a, a.size);
+    // Mgm: for PBh4M vh /= inv_sqrt_a, rhs = B(rho inv_sqrt_a); PBh4 and PBh2
+    // keep FDoper's identity transforms (src/pb/FDoper.h:66-79): rhs = rho
+    if (mehr)
+    {
+        hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh,
+            (const double*)isa, a, 1, vh);
+        hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rho, (const double*)isa, a,
+            0, r0);
+        MGX_CHECK_LAUNCH(ctx);
+        if ((rc = launch_halo(ctx, r0, 1, g0))) return rc;
+        hipLaunchKernelGGL(k_rhs<double>, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)r0, rhs,
+            a, a, a.size);
+    }
+    else
+        hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, rho, rho, a, 2, rhs);
     MGX_CHECK_LAUNCH(ctx);
     double rhs_norm = 0.;
     if ((rc = int_norm(ctx, P, rhs, g0, &rhs_norm))) return rc;
@@ -1855,15 +1942,18 @@ int pb_run(mgx_ctx* ctx, const mgx_grid_t& g0, const double* rho, const double* 
         if ((rc = int_norm(ctx, P, lhs, g0, &fin))) return rc;
         rel = fin * inv_rhs_norm;
     }
-    // vh *= inv_sqrt_a, fresh ghosts; get_vepsilon
-    hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh,
-        (const double*)isa, a, 0, vh);
-    MGX_CHECK_LAUNCH(ctx);
+    // PBh4M: vh *= inv_sqrt_a; fresh ghosts; get_vepsilon
+    if (mehr)
+    {
+        hipLaunchKernelGGL(k_pb_scale, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh,
+            (const double*)isa, a, 0, vh);
+        MGX_CHECK_LAUNCH(ctx);
+    }
     if ((rc = launch_halo(ctx, vh, 1, g0))) return rc;
     if (vks)
     {
-        hipLaunchKernelGGL(k_pb_vks, dim3(nblocks(a.size)), dim3(kBlock), 0, ctx->stream, (const double*)vh, rhod, a,
-            g0.h[0], g0.h[1], g0.h[2], e0, rho0, drho0, vks);
+        hipLaunchKernelGGL(op == MGX_LAP_2ND ? k_pb_vks<2> : k_pb_vks<4>, dim3(nblocks(a.size)), dim3(kBlock), 0,
+            ctx->stream, (const double*)vh, rhod, a, g0.h[0], g0.h[1], g0.h[2], e0, rho0, drho0, vks);
         MGX_CHECK_LAUNCH(ctx);
     }
     if (st)
@@ -2166,15 +2256,16 @@ int mgx_pb_mg(mgx_ctx* ctx, int lap_type, const mgx_block_t* rho, const mgx_bloc
     if (!rc) rc = check_block(ctx, vh);
     if (!rc && vks) rc = check_block(ctx, vks);
     if (rc) return rc;
-    MGX_REQUIRE(ctx, lap_type == MGX_LAP_MEHRSTELLEN,
-        "the dielectric operator is built for the Mehrstellen discretisation (PBh4M) only");
+    MGX_REQUIRE(ctx, lap_type == MGX_LAP_MEHRSTELLEN || lap_type == MGX_LAP_4TH || lap_type == MGX_LAP_2ND,
+        "Electrostatic, Undefined option (the dielectric operators are PBh4M, PBh4 and PBh2)");
     MGX_REQUIRE(ctx, same_shape(rho, vh) && same_shape(rhod, vh) && (!vks || same_shape(vks, vh)),
         "rho/rhod/vh/vks blocks differ in shape");
     MGX_REQUIRE(ctx, vh->dtype == MGX_F64 && vh->nfunc == 1, "one FP64 function");
-    MGX_REQUIRE(ctx, vh->grid.ghosts >= 2, "Not enough ghosts points in PBh4M (2 needed)");
+    MGX_REQUIRE(ctx, vh->grid.ghosts >= (lap_type == MGX_LAP_2ND ? 1 : 2),
+        "Not enough ghosts points in the dielectric operator (2 for PBh4M and PBh4, 1 for PBh2)");
     MGX_REQUIRE(ctx, e0 >= 1. && rho0 > 0. && drho0 > 0., "bad dielectric parameters");
     MGX_REQUIRE(ctx, nu1 >= 0 && nu2 >= 0 && max_sweeps >= 0 && max_nlevels >= 0, "bad cycle parameters");
-    rc = pb_run(ctx, vh->grid, (const double*)rho->data, (const double*)rhod->data, (double*)vh->data,
+    rc = pb_run(ctx, lap_type, vh->grid, (const double*)rho->data, (const double*)rhod->data, (double*)vh->data,
         vks ? (double*)vks->data : nullptr, e0, rho0, drho0, nu1, nu2, max_sweeps, tol, max_nlevels, stats);
     return rc ? rc : sync(ctx);
 }

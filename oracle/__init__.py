@@ -653,6 +653,126 @@ def pb_mgm(g, rho, rhod, vh, e0, rho0, drho0, nu1=2, nu2=2, max_sweeps=10, tol=1
     return converged, final, rel, final / init, sweeps, vks
 
 
+def pb_del1_2nd(g, u, d):
+    """FDoper<T>::del1_2nd (src/pb/FDoper.cc:72-115) along direction d on the
+    interior of a traded block: 0.5/h (u+1 - u-1)."""
+    ng = g.ghosts
+    e1 = 0.5 * (1.0 / g.h[d])
+
+    def sh(k):
+        sl = [slice(ng, ng + g.dim[i]) for i in range(3)]
+        sl[d] = slice(ng + k, ng + k + g.dim[d])
+        return u[(0,) + tuple(sl)]
+    return e1 * (sh(1) - sh(-1))
+
+
+def pbg_apply(g, eps, x, bc, order):
+    """The gradient form of -div(eps grad x): PBh4<T>::pb_4th
+    (src/pb/PBh4.cc:13-28, order 4) and PBh2<T>::pb_2nd (src/pb/PBh2.cc:13-30,
+    order 2): out = 0; out -= d_i x * d_i eps for i = 0, 1, 2 (del1_4th /
+    del1_2nd); out += (-Lap x) * eps (del2_4th / del2_2nd); x traded with bc
+    first, eps with periodic ghosts.  Returns a ghosted block (interior set)."""
+    trade(g, x, bc)
+    D = pb_del1 if order == 4 else pb_del1_2nd
+    L = interior(g, lap(g, 2 if order == 4 else 1, x))[0]
+    acc = np.zeros(tuple(g.dim))
+    for d in range(3):
+        acc = acc - D(g, x, d) * D(g, eps, d)
+    acc = acc + L * interior(g, eps)[0]
+    return embed(g, acc[None])
+
+
+def pbg_jacobi_factor(h):
+    """c0 of PBh4<T>::jacobi / PBh2<T>::jacobi (src/pb/PBh4.cc:29-48,
+    src/pb/PBh2.cc:32-55): 1.5 (30/12) sum 1/h^2."""
+    return 1.5 * (30.0 / 12.0) * sum(1.0 / (v * v) for v in h)
+
+
+def pbg_jacobi(g, eps, x, f, bc, order):
+    """PBh4 / PBh2 jacobi: W = A x - f, x -= (1 / (c0 eps)) W
+    (GridFunc<T>::jacobi, src/pb/GridFunc.cc:3264-3280), interior points."""
+    W = pbg_apply(g, eps, x, bc, order) - f
+    c0 = pbg_jacobi_factor(list(g.h))
+    xi, ei, wi = interior(g, x), interior(g, eps), interior(g, W)
+    xi -= (1.0 / (c0 * ei)) * wi
+    return W
+
+
+def _pbg_vcycle(levels, lev, x, rhs, cogr, nu1, nu2, bc):
+    """pb::Vcycle (src/pb/Vcycle.h:28-249, USE_LOWER_ORDER) with the gradient
+    form: levels[lev] = (grid, eps, order); the coarse operator of either
+    order is PBh2 (PBh4<T>::getLowerOrderOp, src/pb/PBh4.h:72-80) on the
+    full-weighting restriction of the finer eps (PBh4<T>::coarseOp,
+    PBh2<T>::coarseOp, src/pb/PBh2.cc:80-90), traded periodically first;
+    coarse functions have one ghost."""
+    g, eps, order = levels[lev]
+    for _ in range(nu1):
+        pbg_jacobi(g, eps, x, rhs, bc, order)
+    coarsen = all(g.dim[i] % 2 == 0 and g.dim[i] >= 2 * g.ghosts for i in range(3))
+    if lev < cogr and coarsen:
+        res = pbg_apply(g, eps, x, bc, order) - rhs
+        trade(g, res, bc)
+        gc = Grid.make([d // 2 for d in g.dim], 1, [2 * v for v in g.h])
+        if len(levels) == lev + 1:
+            ef = eps.copy()
+            trade(g, ef)
+            ec = restrict(g, ef, gc)
+            trade(gc, ec)
+            levels.append((gc, ec, 2))
+        rc = restrict(g, res, gc)
+        uc = empty_block(gc, 1)
+        _pbg_vcycle(levels, lev + 1, uc, rc, cogr, nu1, nu2, bc)
+        trade(gc, uc, bc)
+        x -= extend(gc, uc, g)
+    for _ in range(nu2):
+        pbg_jacobi(g, eps, x, rhs, bc, order)
+
+
+def pbg_mgm(g, rho, rhod, vh, e0, rho0, drho0, order=4, nu1=2, nu2=2, max_sweeps=10, tol=1e-16, max_nlevels=10,
+            bc=(1, 1, 1)):
+    """SolverPB<PBh4> / SolverPB<PBh2>::solve (src/pb/SolverPB.cc:56-84):
+    PBh4<T>::init / PBh2 (eps from rhod, periodic ghosts), pb::Mgm
+    (src/pb/Mgm.h:12-111) with no transform (FDoper's identity transform and
+    rhs, src/pb/FDoper.h:66-79: rhs = rho), the fine operator of the given
+    order and PBh2 below it; then get_vepsilon (src/pb/PBh4.cc:49-63,
+    src/pb/PBh2.cc:57-72): vks = depsilon(rhod) |grad vh|^2 (-1/(8 pi)), the
+    gradient by del1_4th / del1_2nd.  rho, vh: ghosted (1, ...) blocks (vh
+    updated); rhod: interior.  Returns (converged, final_residual,
+    final_relative_residual, residual_reduction, nb_sweeps, vks interior)."""
+    assert g.ghosts >= (2 if order == 4 else 1)
+    eps = embed(g, pb_epsilon(rhod, e0, rho0, drho0)[None])
+    trade(g, eps)
+    rhs = rho.copy()
+    inv_rhs_norm = 1.0 / gnorm(g, rhs, bc)
+    levels = [(g, eps, order)]
+    converged, sweeps = False, 0
+    init = final = rel = 1.0
+    for i in range(max_sweeps):
+        r = rhs - pbg_apply(g, eps, vh, bc, order)
+        rn = gnorm(g, r, bc)
+        if i == 0:
+            init = rn
+        if rn * inv_rhs_norm < tol:
+            final, rel, converged = rn, rn * inv_rhs_norm, True
+            break
+        w = np.zeros_like(vh)
+        _pbg_vcycle(levels, 0, w, r, max_nlevels, nu1, nu2, bc)
+        sweeps += 1
+        vh += w
+    if not converged:
+        lhs = pbg_apply(g, eps, vh, bc, order) - rhs
+        final = gnorm(g, lhs, bc)
+        rel = final * inv_rhs_norm
+    trade(g, vh, bc)
+    D = pb_del1 if order == 4 else pb_del1_2nd
+    g2 = np.zeros(tuple(g.dim))
+    for d in range(3):
+        dv = D(g, vh, d)
+        g2 += dv * dv
+    vks = pb_depsilon(rhod, e0, rho0, drho0) * g2 * (-1.0 / (8.0 * np.pi))
+    return converged, final, rel, final / init, sweeps, vks
+
+
 # ------------------------------------------------------------------ box family
 def b_call(name, b, *args):
     return getattr(lib(), name)(C.byref(b), *args)

@@ -1,9 +1,10 @@
 """GPU parity of the Poisson-Boltzmann multigrid (mgx_pb_mg: SolverPB with the
-Mehrstellen dielectric operator PBh4M) against the oracle's restatement,
-through the C ABI: fixed numbers of V-cycles on periodic, Dirichlet and mixed
-grids (potential and the dielectric KS potential vks within 1e-12 of their
-largest values: FP64, sums in another order, the library's pow), a converged
-solve (same sweep count), and the argument checks."""
+Mehrstellen dielectric operator PBh4M, and with the gradient forms PBh4 and
+PBh2) against the oracle's restatements, through the C ABI: fixed numbers of
+V-cycles on periodic, Dirichlet and mixed grids (potential and the dielectric
+KS potential vks within 1e-12 of their largest values: FP64, sums in another
+order, the library's pow), converged solves (same sweep count), and the
+argument checks."""
 import numpy as np
 import pytest
 
@@ -34,21 +35,21 @@ def _density(dims, h, seed):
     return out
 
 
-def _case(dims, bc, h=0.4, seed=0):
-    og = O.Grid.make(dims, 2, [h] * 3)
-    mg_ = mgx.Grid.make(dims, 2, [h] * 3, bc=bc)
+def _case(dims, bc, h=0.4, seed=0, ghosts=2):
+    og = O.Grid.make(dims, ghosts, [h] * 3)
+    mg_ = mgx.Grid.make(dims, ghosts, [h] * 3, bc=bc)
     rng = np.random.default_rng(seed)
     rho = O.embed(og, rng.standard_normal((1,) + tuple(dims)))
     rhod = _density(dims, [h] * 3, seed + 1)
     return og, mg_, rho, rhod
 
 
-def _run(ctx, og, mg_, rho, rhod, vh0, **kw):
+def _run(ctx, og, mg_, rho, rhod, vh0, lap=mgx.LAP_MEHRSTELLEN, e0=E0, **kw):
     trho, trhod, tvh = to_dev(rho), to_dev(O.embed(og, rhod[None])), to_dev(vh0)
     tvks = to_dev(O.empty_block(og, 1))
     st = mgx.PoissonStats()
-    ctx.call("mgx_pb_mg", mgx.LAP_MEHRSTELLEN, mgx.block(trho, mg_), mgx.block(trhod, mg_), mgx.block(tvh, mg_),
-             mgx.block(tvks, mg_), E0, RHO0, DRHO0, 2, 2, kw.get("max_sweeps", 10), kw.get("tol", 1e-16),
+    ctx.call("mgx_pb_mg", lap, mgx.block(trho, mg_), mgx.block(trhod, mg_), mgx.block(tvh, mg_),
+             mgx.block(tvks, mg_), e0, RHO0, DRHO0, 2, 2, kw.get("max_sweeps", 10), kw.get("tol", 1e-16),
              kw.get("max_nlevels", 10), st)
     return to_np(tvh), to_np(tvks), st
 
@@ -84,20 +85,61 @@ def test_converged_solve(ctx):
     assert relmax(O.interior(og, got), O.interior(og, ref)) <= 1e-10
 
 
+GRAD = [(mgx.LAP_4TH, 4, 2), (mgx.LAP_2ND, 2, 1)]  # (lap type, order, ghosts)
+
+
+@pytest.mark.parametrize("lap,order,ng", GRAD)
+@pytest.mark.parametrize("dims,bc", [((32, 32, 32), (1, 1, 1)), ((32, 32, 32), (0, 0, 0)),
+                                     ((16, 24, 32), (1, 0, 1))])
+def test_gradient_form_fixed_cycles_match_oracle(ctx, lap, order, ng, dims, bc):
+    og, mg_, rho, rhod = _case(dims, bc, ghosts=ng)
+    vh0 = O.embed(og, 0.05 * np.random.default_rng(7).standard_normal((1,) + tuple(dims)))
+    got, gvks, st = _run(ctx, og, mg_, rho, rhod, vh0, lap=lap, max_sweeps=3, tol=0.0)
+    ref = vh0.copy()
+    conv, fin, rel, red, sweeps, vks = O.pbg_mgm(og, rho, rhod, ref, E0, RHO0, DRHO0, order=order, max_sweeps=3,
+                                                 tol=0.0, bc=bc)
+    err = relmax(O.interior(og, got), O.interior(og, ref))
+    ev = relmax(O.interior(og, gvks)[0], vks)
+    say("pb%d dims %s bc %s: vh rel err %.3e, vks rel err %.3e, residual %.4e vs %.4e" % (order, dims, bc, err, ev,
+                                                                                        st.final_residual, fin))
+    assert err <= 1e-12 and ev <= 1e-11
+    assert st.nb_sweeps == 3 and not st.converged
+    assert st.final_residual == pytest.approx(fin, rel=1e-9)
+    assert np.max(np.abs(vks)) > 0.0
+
+
+@pytest.mark.parametrize("lap,order,ng", GRAD)
+def test_gradient_form_converged_solve(ctx, lap, order, ng):
+    # eps from 1 to 10: the gradient forms converge on this grid (with
+    # e0 = 78.36 the 4th-order form's cycles stall)
+    dims, bc, e0 = (32, 32, 32), (0, 0, 0), 10.0
+    og, mg_, rho, rhod = _case(dims, bc, seed=3, ghosts=ng)
+    vh0 = O.empty_block(og, 1)
+    got, gvks, st = _run(ctx, og, mg_, rho, rhod, vh0, lap=lap, e0=e0, max_sweeps=120, tol=1e-11)
+    ref = vh0.copy()
+    conv, fin, rel, red, sweeps, vks = O.pbg_mgm(og, rho, rhod, ref, e0, RHO0, DRHO0, order=order, max_sweeps=120,
+                                                 tol=1e-11, bc=bc)
+    say("pb%d converged: sweeps %d vs %d, rel residual %.3e" % (order, st.nb_sweeps, sweeps,
+                                                               st.final_relative_residual))
+    assert st.converged and conv and st.nb_sweeps == sweeps
+    assert relmax(O.interior(og, got), O.interior(og, ref)) <= 1e-10
+
+
 def test_rejects_bad_arguments(ctx):
     og, mg_, rho, rhod = _case((16, 16, 16), (1, 1, 1))
     t = to_dev(rho)
     st = mgx.PoissonStats()
     b = mgx.block(t, mg_)
-    # only the Mehrstellen dielectric operator is built
+    # PBh4M, PBh4 and PBh2 are the dielectric operators (lap types 0, 2, 1)
     with pytest.raises(mgx.MgxError):
-        ctx.call("mgx_pb_mg", mgx.LAP_4TH, b, b, b, None, E0, RHO0, DRHO0, 2, 2, 1, 0.0, 10, st)
-    # PBh4M needs 2 ghosts
+        ctx.call("mgx_pb_mg", 3, b, b, b, None, E0, RHO0, DRHO0, 2, 2, 1, 0.0, 10, st)
+    # PBh4M and PBh4 need 2 ghosts (PBh2 one)
     g1 = mgx.Grid.make((16, 16, 16), 1, [0.4] * 3)
     t1 = to_dev(O.embed(O.Grid.make((16, 16, 16), 1, [0.4] * 3), np.zeros((1, 16, 16, 16))))
     b1 = mgx.block(t1, g1)
-    with pytest.raises(mgx.MgxError):
-        ctx.call("mgx_pb_mg", mgx.LAP_MEHRSTELLEN, b1, b1, b1, None, E0, RHO0, DRHO0, 2, 2, 1, 0.0, 10, st)
+    for lap in (mgx.LAP_MEHRSTELLEN, mgx.LAP_4TH):
+        with pytest.raises(mgx.MgxError):
+            ctx.call("mgx_pb_mg", lap, b1, b1, b1, None, E0, RHO0, DRHO0, 2, 2, 1, 0.0, 10, st)
     # multipole boundary values are not built
     g2 = mgx.Grid.make((16, 16, 16), 2, [0.4] * 3, bc=(2, 2, 2))
     b2 = mgx.block(t, g2)
