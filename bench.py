@@ -53,8 +53,8 @@ C5_POINTS = 2048 * 32 ** 3
 # passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, both calibrated on
 # streaming kernels of known size), copied from profiles/pmc_traffic_c5.md;
 # keyed by (group, precision of the path's kernel)
-TRAFFIC_C5 = {("jacobi0", "f32"): 893785190, ("jacobi0", "f16"): 403009536, ("cellgram", "split", 1): 1417057075,
-              ("cellgram", "split", 2): 1300885811, ("residk", "f32"): 1957872640,
+TRAFFIC_C5 = {("jacobi0", "f32"): 893786214, ("jacobi0", "f16"): 403009536, ("cellgram", "split", 1): 1423095296,
+              ("cellgram", "split", 2): 1309666713, ("residk", "f32"): 1737051648,
               ("learned", "bf16"): 17287778496}
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c5.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
 

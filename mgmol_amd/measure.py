@@ -342,12 +342,64 @@ def poisson(ctx):
     return out
 
 
+def pb(ctx):
+    """Poisson-Boltzmann solves on 256^3 (h = 0.2, periodic, 2 ghosts): the
+    dielectric function of a density of 64 Gaussian blobs (values across
+    rho0), a zero-mean charge; up to 10 V-cycles to 1e-10 with PBh4M, PBh4
+    and PBh2, the dielectric KS potential included."""
+    n, h, ng = 256, 0.2, 2
+    e0, rho0, drho0 = 78.36, 0.0004, 1.3
+    g = mgx.Grid.make((n, n, n), ng, [h] * 3)
+    rng = np.random.default_rng(11)
+    x = (np.arange(n) * h).astype(np.float64)
+    L = n * h
+    rhod_i = np.zeros((n, n, n))
+    for c in rng.uniform(0, L, (64, 3)):
+        f = [np.exp(-np.minimum(np.abs(x - c[k]), L - np.abs(x - c[k])) ** 2 / 1.2) for k in range(3)]
+        rhod_i += 0.01 * (f[0][:, None, None] * f[1][None, :, None]) * f[2][None, None, :]
+    sh = (n + 2 * ng,) * 3
+    inner = (slice(ng, ng + n),) * 3
+    rhod = np.zeros(sh)
+    rhod[inner] = rhod_i
+    rho = np.zeros(sh)
+    q = rng.standard_normal((n, n, n))
+    rho[inner] = 4 * np.pi * (q - q.mean())
+    del rhod_i, q
+    trho, trhod = mgx.DeviceArray(ctx, rho), mgx.DeviceArray(ctx, rhod)
+    tvh = mgx.DeviceArray(ctx, shape=sh, dtype=np.float64)
+    tvks = mgx.DeviceArray(ctx, shape=sh, dtype=np.float64)
+    zero = np.zeros(sh)
+    out = {"config": "pb 256^3 periodic, e0 = 78.36, 64 density blobs, up to 10 V-cycles to 1e-10"}
+    for name, lap in (("PBh4M", 0), ("PBh4", 2), ("PBh2", 1)):
+        st = mgx.PoissonStats()
+
+        def run():
+            ctx.check(mgx.lib().mgx_memcpy_htod(ctx.h, mgx.ptr(tvh), zero.ctypes.data_as(C.c_void_p), tvh.nbytes),
+                      "mgx_memcpy_htod")
+            ctx.check(mgx.lib().mgx_pb_mg(ctx.h, lap, mgx.block(trho, g, 1), mgx.block(trhod, g, 1),
+                                          mgx.block(tvh, g, 1), mgx.block(tvks, g, 1), e0, rho0, drho0, 2, 2, 10,
+                                          1e-10, 10, C.byref(st)), "mgx_pb_mg")
+
+        run()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            run()
+        t = (time.perf_counter() - t0) / reps
+        out[name] = {"ms_per_solve_incl_zeroing": round(1e3 * t, 2), "v_cycles": st.nb_sweeps,
+                     "converged": bool(st.converged), "final_relative_residual": st.final_relative_residual,
+                     "ms_per_v_cycle": round(1e3 * t / max(st.nb_sweeps, 1), 3)}
+    for a in (trho, trhod, tvh, tvks):
+        a.free()
+    return out
+
+
 def main(argv=None):
     argv = sys.argv[1:] if argv is None else argv
     ctx = mgx.Ctx(0)
-    for name in argv or ["c1", "c2", "c3", "c4", "density", "poisson"]:
+    for name in argv or ["c1", "c2", "c3", "c4", "density", "poisson", "pb"]:
         res = {"c1": c1, "c1_gpu": lambda c: c1(c, cpu=False), "c2": c2, "c3": c3, "c4": c4, "density": density,
-               "poisson": poisson}[name](ctx)
+               "poisson": poisson, "pb": pb}[name](ctx)
         print(json.dumps(res), flush=True)
     ctx.close()
 
