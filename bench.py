@@ -55,7 +55,8 @@ C5_POINTS = 2048 * 32 ** 3
 # keyed by (group, precision of the path's kernel)
 TRAFFIC_C5 = {("jacobi0", "f32"): 893786214, ("jacobi0", "f16"): 403009536, ("cellgram", "split", 1): 1423095296,
               ("cellgram", "split", 2): 1309666713, ("residk", "f32"): 1737051648,
-              ("learned", "bf16"): 17287778496}
+              ("learned", "bf16"): 17287778496, ("skin", "f32"): 1679117516, ("skin", "f64"): 1679117516,
+              ("skin", "f16"): 1679117516}
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c5.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
 
 PATHS = {
@@ -80,7 +81,7 @@ DTYPES = {"plain": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64", "ext16"
 GROUPS = {
     "hloc": ("k_box_march4<double, 2, 1, 512, ...> H psi (FP64, V gathered per box)", "hbm"),
     "cellgram": ("k_cell_gram<PREC, true, 1> S/H partials per 8^3 cell (MFMA)", "hbm"),
-    "skin": ("k_slab_skin Galerkin H outside each box", "gather"),
+    "skin": ("k_slab_skin Galerkin H outside each box (the reach through each face of box j into boxes i < j)", "hbm"),
     "residk": ("k_cell_residual<true, OM> R = psi theta - H psi per 8^3 cell (FP64 MFMA), written as the "
                "V-cycle's zero start f, x (or as R on network-trial steps)", "hbm"),
     "update": ("k_box_and_update_nrm<T, 32> Anderson update + normalisation", "hbm"),

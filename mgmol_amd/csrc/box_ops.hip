@@ -64,6 +64,11 @@ struct mgx_boxset
     int* d_ilist     = nullptr;
     int64_t nilist   = 0;
     int* d_pair_ient = nullptr;  // 2 ints per pair (-1 for i == j)
+    // the skin's algorithmic bytes per call (k_slab_skin): for every face of
+    // box j with a box i < j across it, the two planes of psi_j next to the
+    // face, and per such i the psi_i values on the two layers beyond the
+    // face inside box i
+    int64_t skin_bytes = 0;
     // column tiles of the pair-sparse S / H for sparse x dense products
     int ntile        = 0;
     int* d_sp_ptr    = nullptr;
@@ -3324,7 +3329,7 @@ int box_gram_h_cells(mgx_ctx* ctx, const mgx_boxset* bs, int prec, const double*
             size_t lds        = 2 * (size_t)bs->maxedge * bs->maxedge * sizeof(double);
             const int use_lds = lds <= 64 * 1024;
             if (!use_lds) lds = 0;
-            const int tk = prof_begin(ctx, MGX_PROF_SKIN);
+            const int tk = prof_begin(ctx, MGX_PROF_SKIN, (double)bs->skin_bytes);
             hipLaunchKernelGGL(k_slab_skin, dim3((6 * bs->nbox + 7) / 8 * 8), dim3(256), lds, ctx->stream, psi,
                 (const int64_t*)bs->d_off[0], (const int*)bs->d_edge, (const int*)bs->d_ilist_ptr,
                 (const int*)bs->d_ilist, coef4(bs->h), use_lds, skin, (const int*)bs->d_sp_col, bs->nbox);
@@ -3892,6 +3897,33 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* b, int levels, mgx_boxset
             iflat.insert(iflat.end(), il[j].begin(), il[j].end());
         }
         iptr[s->nbox] = (int)(iflat.size() / 4);
+        // the skin's algorithmic bytes, by k_slab_skin's own tests
+        for (int j = 0; j < s->nbox; j++)
+        {
+            const int ej = s->edge[j];
+            for (int f = 0; f < 6; f++)
+            {
+                const int d = f >> 1, side = f & 1, u = (d + 1) % 3, w = (d + 2) % 3;
+                const int Xd1 = side == 0 ? -1 : ej, Xd2 = side == 0 ? -2 : ej + 1;
+                bool any = false;
+                for (size_t q = 0; q < il[j].size(); q += 4)
+                {
+                    const int i = il[j][q];
+                    if (i >= j) continue;
+                    const int ei = s->edge[i];
+                    const int sh[3] = { il[j][q + 1], il[j][q + 2], il[j][q + 3] };
+                    const int nu  = std::min(ej, ei - sh[u]) - std::max(0, -sh[u]);
+                    const int nww = std::min(ej, ei - sh[w]) - std::max(0, -sh[w]);
+                    if (nu <= 0 || nww <= 0) continue;
+                    const int xd1 = Xd1 + sh[d], xd2 = Xd2 + sh[d];
+                    const int layers = (xd1 >= 0 && xd1 < ei ? 1 : 0) + (xd2 >= 0 && xd2 < ei ? 1 : 0);
+                    if (!layers) continue;
+                    any = true;
+                    s->skin_bytes += 8 * (int64_t)nu * nww * layers;
+                }
+                if (any) s->skin_bytes += 2 * 8 * (int64_t)ej * ej;
+            }
+        }
         // pair entries are local to each box's list: make them global
         for (int64_t q = 0; q < s->npairs; q++)
         {

@@ -76,9 +76,42 @@ def test_solver_profile_groups(ctx):
     # pointwise and fused with the FP64 -> FP32 cast
     assert prof["jacobi0"][1] == 3 * 3
     assert prof["hloc"][1] == 3 and prof["hloc"][0] > 0
+    # the skin's algorithmic bytes: counted by brute force on the grid
+    assert prof["skin"][1] == 3
+    assert prof["skin"][2] == 3 * _skin_bytes(cfg)
     assert s.profile()["hloc"][1] == 0
     s.close()
     bs.close()
+
+
+def _skin_bytes(cfg):
+    """Bytes the skin needs, from the grid points themselves: for every face
+    of box j, the two layers beyond it over j's extent (periodic); for each
+    box i < j, the points of those layers inside box i (8 bytes each), and,
+    when some i meets them, the two planes of psi_j next to the face."""
+    g = tuple(int(x) for x in cfg.gdim)
+    nbox = len(cfg.edge)
+
+    def box_mask(lo, hi):
+        ix = [np.arange(lo[d], hi[d]) % g[d] for d in range(3)]
+        m = np.zeros(g, dtype=bool)
+        m[np.ix_(*ix)] = True
+        return m
+
+    boxes = [box_mask(cfg.origin[i], cfg.origin[i] + cfg.edge[i]) for i in range(nbox)]
+    total = 0
+    for j in range(nbox):
+        o, e = cfg.origin[j], int(cfg.edge[j])
+        for d in range(3):
+            for side in (0, 1):
+                lo, hi = o.copy(), o + e
+                lo[d] = o[d] - 2 if side == 0 else o[d] + e
+                hi[d] = lo[d] + 2
+                slab = box_mask(lo, hi)
+                counts = [int(np.count_nonzero(slab & boxes[i])) for i in range(j)]
+                if any(counts):
+                    total += 8 * sum(counts) + 2 * 8 * e * e
+    return total
 
 
 def test_solver_anderson_matches_oracle(ctx):
