@@ -974,9 +974,30 @@ int inverse_update_pairs(mgx_ctx* ctx, int n, const double* S, const double* Sv,
 // the delta of the final X is measured too, into deltas[nsteps].  The caller
 // reads the deltas later with its other scalars and checks them
 // (ns_spec_check).
-int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& pt, double* X, double* work,
-    int nsteps, bool bf16, double* deltas)
+// the start of a refresh: X <- 2 X - P (on) and P <- the X it was given: the
+// linear extrapolation of the last two inverses, S changing little from one
+// step to the next (the steps then start an order closer)
+__global__ void k_extrap(double* X, double* P, int64_t n, int on)
 {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    {
+        const double t = X[i];
+        if (on) X[i] = 2. * t - P[i];
+        P[i] = t;
+    }
+}
+
+// xprev (may be null): the inverse of the step before, n x n; with extrap the
+// steps start from 2 X - xprev, and xprev receives the X they were given
+int inverse_update_spec(mgx_ctx* ctx, int n, const double* Sv, const PairTiles& pt, double* X, double* work,
+    int nsteps, bool bf16, double* deltas, double* xprev, bool extrap)
+{
+    if (xprev)
+    {
+        hipLaunchKernelGGL(k_extrap, dim3(nblocks((int64_t)n * n)), dim3(kBlock), 0, ctx->stream, X, xprev,
+            (int64_t)n * n, extrap ? 1 : 0);
+        MGX_CHECK_LAUNCH(ctx);
+    }
     const double mone = -1., two = 2.;
     const size_t nn = (size_t)n * n;
     double* T   = work;

@@ -55,7 +55,8 @@ int box_psi_step_sq(mgx_ctx*, const mgx_boxset*, int, double*, const void*, doub
 int box_precond_x(mgx_ctx*, const mgx_boxset*, int, int, double, int, const double*, void*, const void*, double,
     double*, bool*, const double*, bool);
 int box_anderson_dots_total(mgx_ctx*, const mgx_boxset*, const double*, const double*, double*);
-int inverse_update_spec(mgx_ctx*, int, const double*, const PairTiles&, double*, double*, int, bool, double*);
+int inverse_update_spec(mgx_ctx*, int, const double*, const PairTiles&, double*, double*, int, bool, double*, double*,
+    bool);
 int ns_spec_check(const double*, int, double, bool, int*);
 int cnn_apply_boxes(mgx_cnn*, int, int, const double*, void*, int);
 bool boxset_uniform_edge(const mgx_boxset*, int*);
@@ -133,6 +134,10 @@ struct mgx_solver
     int learned_steps = 0, fallbacks = 0;
     mgx::PairTiles tiles;
     bool have_inv = false;  // Sinv holds the inverse of a recent S
+    // the inverse of the step before the last refresh (FP64 refresh): the
+    // next refresh starts from 2 Sinv - xprev when have_xprev
+    double* xprev   = nullptr;
+    bool have_xprev = false;
     int64_t ns_steps = 0, chol_calls = 0, dots_sweeps = 0;
     int ns_pred = 2;  // Newton-Schulz steps taken without host checks (the last step's need)
     int ns_max_spec = 4;
@@ -165,7 +170,7 @@ void free_solver(mgx_solver* s)
 {
     double* bufs[] = { s->psi, s->psi0, s->hpsi, s->res, s->vtot, s->S, s->H, s->L, s->Sinv, s->th, s->eig, s->d,
         s->scal, s->nswork, s->Sv, s->Hv, s->x0, s->psi_prev, s->r_prev, s->psi_cnn, s->r_cnn, s->bpsi, s->Bm,
-        s->invB };
+        s->invB, s->xprev };
     for (double* b : bufs)
         if (b) hipFree(b);
     if (s->kr0) (void)hipFree(s->kr0);
@@ -237,7 +242,8 @@ int head_body(mgx_solver* s, bool pending, bool chol, int spec, bool bf16)
     {
         if ((rc = chol_inv(ctx, s->n, s->S, s->L, s->Sinv))) return rc;
     }
-    else if ((rc = inverse_update_spec(ctx, s->n, s->Sv, s->tiles, s->Sinv, s->nswork, spec, bf16, s->scal + 8)))
+    else if ((rc = inverse_update_spec(ctx, s->n, s->Sv, s->tiles, s->Sinv, s->nswork, spec, bf16, s->scal + 8,
+                  bf16 ? nullptr : s->xprev, s->have_xprev)))
         return rc;
     return project(s, spec, bf16, tk);
 }
@@ -469,7 +475,7 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     {
         uint64_t key = 1;
         for (uint64_t v : { (uint64_t)pending, (uint64_t)spec, (uint64_t)bf16, (uint64_t)(uintptr_t)s->psi,
-                 (uint64_t)s->start_fused })
+                 (uint64_t)s->start_fused, (uint64_t)s->have_xprev })
             key = mix_key(key, v);
         rc = run_seq(s, key, nullptr, [&]() { return head_body(s, pending, false, spec, bf16); });
     }
@@ -479,8 +485,11 @@ int step_body(mgx_solver* s, double tol, mgx_solver_stats_t* st, bool* done)
     if (chol)
     {
         s->chol_calls++;
-        s->have_inv = true;
+        s->have_inv   = true;
+        s->have_xprev = false;
     }
+    else if (!bf16)
+        s->have_xprev = true;  // xprev holds this step's starting inverse
     if (spec > 0)
     {
         int needed     = 0;
@@ -754,6 +763,7 @@ int mgx_solver_create(mgx_ctx* ctx, const mgx_boxset* bs, const mgx_solver_cfg_t
                       && hipMalloc(&s->kcnn, s->npts * tsize) == hipSuccess))
               && (cfg->lap_type != MGX_LAP_MEHRSTELLEN
                   || (alloc(&s->bpsi, s->npts) && alloc(&s->Bm, nn) && alloc(&s->invB, nn)))
+              && (cfg->lap_type == MGX_LAP_MEHRSTELLEN || alloc(&s->xprev, nn))
               && hipHostMalloc((void**)&s->host, 32 * sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok)
     {
@@ -832,6 +842,7 @@ int mgx_solver_reset(mgx_solver* s)
     s->iter       = 0;
     s->lowest     = 0.;
     s->have_inv   = false;
+    s->have_xprev = false;
     s->pending_norm = false;
     s->e_prev     = 0.;
     s->deig       = 1e9;
@@ -913,6 +924,7 @@ int mgx_solver_set_psi(mgx_solver* s, const double* host)
     s->pending_norm = false;
     s->mm           = 0;
     s->trial        = 0;
+    s->have_xprev   = false;  // the orbitals jumped: no extrapolation from the last two inverses
     return MGX_OK;
 }
 
@@ -964,6 +976,7 @@ int mgx_solver_density(mgx_solver* s, double nel, double* rho, double* charge)
     if ((rc = box_gram_h(ctx, s->bs, MGX_PREC_F64, s->psi, s->psi, s->S, s->H, s->Sv, s->Hv))) return rc;
     if ((rc = chol_inv(ctx, s->n, s->S, s->L, s->Sinv))) return rc;
     s->have_inv      = true;
+    s->have_xprev    = false;
     const int64_t nn = (int64_t)s->n * s->n;
     hipLaunchKernelGGL(k_scaled_copy, dim3(nblocks(nn)), dim3(kBlock), 0, ctx->stream, (const double*)s->Sinv, nn, 2.,
         s->th);
