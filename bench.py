@@ -56,7 +56,7 @@ C5_POINTS = 2048 * 32 ** 3
 TRAFFIC_C5 = {("jacobi0", "f32"): 893786214, ("jacobi0", "f16"): 403009536, ("cellgram", "split", 1): 1423095296,
               ("cellgram", "split", 2): 1309666713, ("residk", "f32"): 1737051648,
               ("learned", "bf16"): 17287778496, ("skin", "f32"): 1679117516, ("skin", "f64"): 1679117516,
-              ("skin", "f16"): 1679117516}
+              ("skin", "f16"): 1679117516, ("nsgemm", "f64"): 913193984}
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c5.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
 
 PATHS = {
