@@ -372,8 +372,9 @@ def main():
                                for m, r in res.items()},
         "stream": stream,
         "autotune": {k: {"choice": c, "ms_per_launch": [round(t1, 4), round(t2, 4)],
-                         "candidates": ("lane groups per thread 1, 2" if not k.startswith("gram")
-                                        else "waves per cell 1, 2")}
+                         "candidates": ("waves per cell 1, 2" if k.startswith("gram")
+                                        else "restriction inside the FP32 fine pre-sweep / separate kernel"
+                                        if k == "restrict" else "lane groups per thread 1, 2")}
                      for k, (c, t1, t2) in tune.items()},
         "cpu_baseline": cpu,
     }

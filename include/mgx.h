@@ -295,14 +295,19 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* boxes, int levels, mgx_bo
  *     lane groups (16 bytes of z-points, 8 for FP16) per thread, 1 or 2; the
  *     block then has (e^2 / group size) / choice threads (64..1024);
  *   MGX_TUNE_GRAM_F64 / _SPLIT: waves per 8^3 cell of the S / H cell kernel
- *     with paired z-loads (FP64 / hi-lo split MFMA), 1 or 2.
+ *     with paired z-loads (FP64 / hi-lo split MFMA), 1 or 2;
+ *   MGX_TUNE_RESTRICT: the V-cycle's restriction of the stale residual
+ *     (MGkernelRestrict3D) inside the last pre-sweep of the marching kernel
+ *     (1; w is then never written) or as its own kernel after it (2); timed
+ *     on the FP32 fine sweep, applied at every level and precision.
  * mgx_boxset_autotune times every candidate on the box set's own layout
  * (scratch buffers of its size, 1 warm-up + 3 timed launches each, hipEvents)
  * and keeps the fastest; ms (may be NULL) receives MGX_TUNE_N x 2 times (ms
  * per launch, candidate 1 then 2; -1 where the kernel does not apply).  Any
  * choice gives the same results (the per-point arithmetic does not depend
  * on the lane mapping; only the Anderson dot sums of the last sweep are
- * ordered differently).  A solver tunes its box set at creation unless it
+ * ordered differently; the fused restriction sums its 27 FP64 terms in
+ * another order: the same FP32 / FP16 values, FP64 ones to the last bit).  A solver tunes its box set at creation unless it
  * was tuned or set before.  mgx_boxset_set_tune forces a choice; get reads it. */
 #define MGX_TUNE_MARCH_F16 0
 #define MGX_TUNE_MARCH_F32 1
@@ -310,7 +315,8 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* boxes, int levels, mgx_bo
 #define MGX_TUNE_HLOC_F64 3
 #define MGX_TUNE_GRAM_F64 4
 #define MGX_TUNE_GRAM_SPLIT 5
-#define MGX_TUNE_N 6
+#define MGX_TUNE_RESTRICT 6
+#define MGX_TUNE_N 7
 int mgx_boxset_autotune(mgx_ctx* ctx, mgx_boxset* bs, int* choice, double* ms);
 int mgx_boxset_set_tune(mgx_boxset* bs, const int* choice);
 int mgx_boxset_get_tune(const mgx_boxset* bs, int* choice, int* tuned);

@@ -77,7 +77,7 @@ struct mgx_boxset
     int* d_sp_e      = nullptr;
     // autotuned launch shapes (MGX_TUNE_*) and whether the tuner ran or a
     // caller set them
-    int tune[MGX_TUNE_N] = { 1, 1, 1, 1, 1, 1 };
+    int tune[MGX_TUNE_N] = { 1, 1, 1, 1, 1, 1, 1 };
     bool tuned           = false;
 };
 
@@ -303,6 +303,12 @@ struct MarchArgs
     // V, run on one XCD
     const int* order = nullptr;
     int nbox         = 0;
+    // JACOBI (k_box_march4, no dots, output type T): the stale residual w
+    // restricted to the next level inside the sweep (MGkernelRestrict3D,
+    // zero outside the box) and written to rc (type T) at the coarse level's
+    // offsets offc, instead of w itself
+    void* rc            = nullptr;
+    const int64_t* offc = nullptr;
 };
 
 // the box of this block (grids padded to a multiple of 8); -1: no box
@@ -461,8 +467,8 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
 // z-neighbours read from LDS as whole V-wide pieces.  LDS rows have a ring of
 // ZR = max(V, 2) in z (so the pieces stay aligned) and 2 in y; the arithmetic
 // per point is that of k_box_march.
-template <typename T, int TYPE, int MODE, int NT, int P, int NB, int V, bool DOTS, typename TO = T>
-__global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __restrict__ in, const T* __restrict__ f,
+template <typename T, int TYPE, int MODE, int NT, int P, int NB, int V, int XT, typename TO = T>
+__global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restrict__ in, const T* __restrict__ f,
     const double* __restrict__ vtot, TO* __restrict__ out, T* __restrict__ wout, MarchArgs a, Coef4 c4, Coef2 c2)
 {
     typedef T V4 __attribute__((ext_vector_type(V)));
@@ -512,10 +518,51 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
         zero4[i] = (T)0;
     V4 qv[P][5];
     V4 nx[P];
-    constexpr bool dots = DOTS && MODE == MARCH_JACOBI;
+    // XT: 0 the sweep alone, 1 with the Anderson dots, 2 with the restriction
+    constexpr bool dots = XT == 1 && MODE == MARCH_JACOBI;
     const TO* dkr0  = reinterpret_cast<const TO*>(a.dkr0);
     const double oj = mixed ? a.oinv[j] : 1.;
     double dacc0 = 0., dacc1 = 0.;
+    // the fused restriction: each plane of w goes to a ring of two e x e
+    // planes in LDS behind the stencil planes; after the next barrier every
+    // thread forms the 3 x 3 (y, z) sums with weights (1 2 1) x (1 2 1) of
+    // its coarse points (ec^2 <= 2 NT: the host gives NT P V >= e^2 and
+    // V P <= 8) and adds them to the coarse plane in registers with the x
+    // weights 1 2 1, so that coarse = (s(2xc-1) + 2 s(2xc) + s(2xc+1)) / 64:
+    // MGkernelRestrict3D's 27 terms and weights, summed in FP64 in another
+    // order (the same FP32 / FP16 results, FP64 ones to the last bit)
+    constexpr bool rst = XT == 2 && MODE == MARCH_JACOBI && !mixed;
+    static_assert(!rst || V * P <= 8, "the restriction holds two coarse points per thread");
+    T* rw          = pl0 + NB * HW;
+    const int ec   = e >> 1;
+    double racc[2] = { 0., 0. };
+    auto rplane    = [&](int X) {
+        const T* R = rw + (X & 1) * E2;
+        T* dst     = reinterpret_cast<T*>(a.rc) + a.offc[j] + (int64_t)(X >> 1) * ec * ec;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+        {
+            const int cp = threadIdx.x + u * NT;
+            if (cp >= ec * ec) break;
+            const int yc = cp / ec, zc = cp - (cp / ec) * ec;
+            const T* r0  = R + 2 * yc * e + 2 * zc;
+            auto row     = [&](const T* rr) -> double {
+                double t = 2. * (double)rr[0] + (double)rr[1];
+                if (zc > 0) t += (double)rr[-1];
+                return t;
+            };
+            double sp = 2. * row(r0) + row(r0 + e);
+            if (yc > 0) sp += row(r0 - e);
+            if (X & 1)
+            {
+                racc[u] += sp;
+                dst[cp]  = (T)(kInv64 * racc[u]);
+                racc[u]  = sp;
+            }
+            else
+                racc[u] += 2. * sp;
+        }
+    };
 #pragma unroll
     for (int k = 0; k < P; k++)
     {
@@ -558,6 +605,7 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
         for (int k = 0; k < P; k++)
             if (act[k]) *reinterpret_cast<V4*>(pl + (qy[k] + 2) * W + qz[k] + ZR) = qv[k][2];
         __syncthreads();
+        if (rst && x > 0) rplane(x - 1);
 #pragma unroll
         for (int k = 0; k < P; k++)
         {
@@ -627,6 +675,7 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
                     ov[i] = (T)r;
             }
             if (MODE == MARCH_JACOBI && wout) *reinterpret_cast<V4*>(wout + gi) = wv4;
+            if (rst) *reinterpret_cast<V4*>(rw + (x & 1) * E2 + q) = wv4;
             VO oo;
             if constexpr (mixed)
             {
@@ -662,6 +711,11 @@ __global__ void __launch_bounds__(NT, DOTS ? 6 : 8) k_box_march4(const T* __rest
             qv[k][3] = qv[k][4];
             qv[k][4] = nx[k];
         }
+    }
+    if (rst)
+    {
+        __syncthreads();
+        rplane(e - 1);
     }
     if (dots)
     {
@@ -2736,40 +2790,59 @@ constexpr int march_v()
     return sizeof(T) == 2 ? 4 : 16 / (int)sizeof(T);
 }
 
-template <typename T, int TYPE, int MODE, int NT, int P, bool DOTS, typename TO>
+template <typename T, int TYPE, int MODE, int NT, int P, int XT, typename TO>
 void march4_launch_d(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
     const double* v, TO* out, T* w, const Coefs& c)
 {
-    constexpr int V = march_v<T>();
+    constexpr int V  = march_v<T>();
+    constexpr int ZR = V > 2 ? V : 2;
+    // the fused restriction's two planes of w behind the stencil planes
+    const int em    = a.lds_w - 2 * ZR;
+    const size_t rb = a.rc ? 2 * (size_t)em * em * sizeof(T) : 0;
     // two plane buffers when the blocks that fill a CU (32 waves) still fit
     // its 160 KB of LDS
     if (2 * plane * (2048 / NT) <= 160 * 1024)
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V, DOTS, TO>), dim3(march_grid_dim(bs)), dim3(NT), 2 * plane,
-            ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 2, V, XT, TO>), dim3(march_grid_dim(bs)), dim3(NT),
+            2 * plane + rb, ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
     else
-        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V, DOTS, TO>), dim3(march_grid_dim(bs)), dim3(NT), plane,
-            ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
+        hipLaunchKernelGGL((k_box_march4<T, TYPE, MODE, NT, P, 1, V, XT, TO>), dim3(march_grid_dim(bs)), dim3(NT),
+            plane + rb, ctx->stream, in, f, v, out, w, a, c.c4, c.c2);
 }
 
-// the Anderson dots exist for the Jacobi sweep only (a.dout set)
+// the Anderson dots (a.dout set) and the fused restriction (a.rc set, output
+// type T) exist for the Jacobi sweep only, each in a kernel of its own
 template <typename T, int TYPE, int MODE, int NT, int P, typename TO>
 void march4_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_t plane, const T* in, const T* f,
     const double* v, TO* out, T* w, const Coefs& c)
 {
-    if (MODE == MARCH_JACOBI && a.dout)
-        march4_launch_d<T, TYPE, MODE, NT, P, MODE == MARCH_JACOBI, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
-    else
-        march4_launch_d<T, TYPE, MODE, NT, P, false, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
+    constexpr int J = MODE == MARCH_JACOBI ? 1 : 0;
+    if (J && a.dout)
+    {
+        march4_launch_d<T, TYPE, MODE, NT, P, J, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
+        return;
+    }
+    if constexpr (J && std::is_same<T, TO>::value)
+        if (a.rc)
+        {
+            march4_launch_d<T, TYPE, MODE, NT, P, 2, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
+            return;
+        }
+    march4_launch_d<T, TYPE, MODE, NT, P, 0, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
 }
 
 // the marching sweep for 2nd/4th-order operators; returns false if the box
 // sizes at this level need another kernel
+// rc (JACOBI, output type T, no dots): the stale residual restricted to
+// level + 1 inside the sweep, in place of w; false if the box sizes need
+// the separate restriction
 template <typename T, int TYPE, int MODE, typename TO = T>
 bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* in, const T* f, const double* v,
-    TO* out, T* w, const double* nrm = nullptr, const MarchDots* dots = nullptr, const double* oinv = nullptr)
+    TO* out, T* w, const double* nrm = nullptr, const MarchDots* dots = nullptr, const double* oinv = nullptr,
+    T* rc = nullptr)
 {
     constexpr bool mixed = !std::is_same<T, TO>::value;
     static_assert(!mixed || MODE == MARCH_JACOBI, "another output type for the Jacobi sweep only");
+    if (rc && (MODE != MARCH_JACOBI || mixed || dots || w || level + 1 >= bs->levels)) return false;
     const int emax = bs->maxedge >> level;
     const int E2   = emax * emax;
     MarchArgs a;
@@ -2813,6 +2886,12 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
         a.lds_w            = emax + 2 * ZR;
         const size_t plane = (size_t)a.lds_w * (emax + 4) * sizeof(T);
         const int groups   = E2 / V;
+        if (rc)
+        {
+            if (2 * plane + 2 * (size_t)E2 * sizeof(T) > 64 * 1024 || groups > 1024) return false;
+            a.rc   = rc;
+            a.offc = bs->d_off[level + 1];
+        }
         if (plane <= 60 * 1024 && groups <= 1024)
         {
             if constexpr (TYPE == MGX_LAP_4TH && (MODE == MARCH_JACOBI || MODE == MARCH_HLOC))
@@ -2848,6 +2927,7 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
         if (dots) return false;
         a.lds_w = emax + 4;
     }
+    if (rc) return false;
     if constexpr (mixed)
         return false;
     else
@@ -2915,11 +2995,26 @@ int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vt
     return MGX_OK;
 }
 
+// rc (with rc_done): the stale residual restricted to level + 1 inside the
+// sweep when the marching kernel applies (*rc_done = true; w is then not
+// written), else w as usual
 template <typename T>
 int box_jacobi_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, double jf, const T* x, const T* f, T* xo,
-    T* w, const MarchDots* dots = nullptr, bool* dots_done = nullptr)
+    T* w, const MarchDots* dots = nullptr, bool* dots_done = nullptr, T* rc = nullptr, bool* rc_done = nullptr)
 {
     const double mjf = -1. * jf;
+    if (rc && !dots
+        && ((type == MGX_LAP_4TH
+                && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, nullptr, nullptr,
+                    nullptr, nullptr, rc))
+            || (type == MGX_LAP_2ND
+                && march<T, MGX_LAP_2ND, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, nullptr, nullptr,
+                    nullptr, nullptr, rc))))
+    {
+        MGX_CHECK_LAUNCH(ctx);
+        *rc_done = true;
+        return MGX_OK;
+    }
     if (dots && type == MGX_LAP_4TH
         && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, w, nullptr, dots))
     {
@@ -3068,20 +3163,32 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
     // (last_out) and read the previous correction for the Anderson dots
     const double np0 = (double)bs->off[0][bs->nbox];
     const double st  = sizeof(T);
+    // the last pre-sweep leaves the stale residual w for the restriction, or
+    // restricts it itself (MGX_TUNE_RESTRICT choice 1, where it applies)
+    T* rcoarse    = level < max_levels ? B.rc[level] : nullptr;
+    bool restricted = false;
     for (int it = first_done ? 1 : 0; it < ncycl; it++)
     {
         T* w          = (it == ncycl - 1 && level < max_levels) ? B.w[level] : nullptr;
+        T* rfuse      = (w && bs->tune[MGX_TUNE_RESTRICT] == 1) ? rcoarse : nullptr;
         const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0, (w ? 4. : 3.) * st * np0) : -1;
-        if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, w))) return rc;
+        if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, w, nullptr, nullptr, rfuse,
+                 &restricted)))
+            return rc;
         prof_end(ctx, tok);
+        // fused: x and f read, x written, the coarse residual written
+        if (restricted) prof_retag(ctx, tok, MGX_PROF_JACOBI0, (3. + 1. / 8.) * st * np0);
         std::swap(a, b);
     }
     if (level == max_levels) return MGX_OK;
-    T* rcoarse = B.rc[level];
-    T* nv      = B.nv[level];
-    int tok    = level == 0 ? prof_begin(ctx, MGX_PROF_TRANSFER, st * np0 * 9. / 8.) : -1;
-    if ((rc = box_restrict_t(ctx, bs, level, (const T*)B.w[level], rcoarse))) return rc;
-    prof_end(ctx, tok);
+    T* nv   = B.nv[level];
+    int tok = -1;
+    if (!restricted)
+    {
+        tok = level == 0 ? prof_begin(ctx, MGX_PROF_TRANSFER, st * np0 * 9. / 8.) : -1;
+        if ((rc = box_restrict_t(ctx, bs, level, (const T*)B.w[level], rcoarse))) return rc;
+        prof_end(ctx, tok);
+    }
     const int64_t nc = bs->off[level + 1][bs->nbox];
     // the coarse level starts from 0: its first sweep is pointwise
     double hc[3] = { 2. * h[0], 2. * h[1], 2. * h[2] };
@@ -3502,6 +3609,16 @@ int box_autotune(mgx_ctx* ctx, mgx_boxset* bs, double* ms)
     });
     tune1(MGX_TUNE_HLOC_F64, 2, [&]() {
         return march<double, MGX_LAP_4TH, MARCH_HLOC>(ctx, bs, 0, 0., a, nullptr, v, c, nullptr);
+    });
+    // the FP32 pre-sweep with the restriction fused, or followed by it
+    tune1(MGX_TUNE_RESTRICT, 2, [&]() {
+        if (bs->levels < 2) return false;
+        float *x = (float*)a, *w = x + np, *fv = (float*)b, *rcv = fv + np, *xo = (float*)c;
+        if (bs->tune[MGX_TUNE_RESTRICT] == 1)
+            return march<float, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, x, fv, nullptr, xo, nullptr, nullptr,
+                nullptr, nullptr, rcv);
+        return march<float, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, x, fv, nullptr, xo, w)
+               && box_restrict_t<float>(ctx, bs, 0, w, rcv) == MGX_OK;
     });
     for (int prec : { MGX_PREC_F64, MGX_PREC_SPLIT })
         tune1(prec == MGX_PREC_F64 ? MGX_TUNE_GRAM_F64 : MGX_TUNE_GRAM_SPLIT, 2, [&]() {

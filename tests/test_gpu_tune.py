@@ -9,8 +9,13 @@ can choose.
   mapping); S / H with one or two waves per cell within the oracle's
   tolerances and bit-identical to each other (the cell's rows are summed in
   two fixed halves whatever the number of waves).
+* The V-cycle's restriction inside the last pre-sweep against the separate
+  restriction kernel: FP32 and FP16 storage bit-identical, FP64 to the last
+  bits (the 27 FP64 terms are summed in another order), both within the
+  oracle's tolerances, on levels where the fused form applies and where it
+  does not.
 * A solver on a box set that was not tuned tunes it at creation, and its
-  steps do not depend on the choices (the same energies bit for bit).
+  steps do not depend on the lane mappings (the same energies bit for bit).
 """
 import numpy as np
 import pytest
@@ -103,6 +108,39 @@ def test_f16_vcycle_shapes_identical(ctx):
         bs.close()
 
 
+@pytest.mark.parametrize("edges,gdim", [((32,), 72), ((8, 12, 16, 20, 24), 56)])
+@pytest.mark.parametrize("start", ["zero", "gamma"])
+def test_fused_restriction_matches_separate(ctx, edges, gdim, start):
+    # (8 .. 24): at level 1 the edges 6, 10 are not multiples of 4, so that
+    # level keeps the separate restriction under either choice
+    cfg = synth.box_config(nbox=12, gdim=gdim, h=0.3, edges=edges, nwells=4, seed=23, levels=3)
+    ob = O.Boxes.make(cfg.origin, cfg.edge, cfg.gdim, cfg.h)
+    bs = mgx.BoxSet(ctx, cfg.origin, cfg.edge, cfg.gdim, cfg.h, 3)
+    try:
+        r = np.random.default_rng(6).standard_normal(ob.size(0))
+        gam = 0.0 if start == "zero" else O.gamma(list(ob.h), 2, 0.0, -1.2)
+        # the FP16 V-cycle needs edges that are multiples of 8 at every level
+        kinds = ((mgx.F16, "f16"),) if edges == (32,) else ()
+        for code, name in kinds + ((mgx.F32, "f32"), (mgx.F64, "f64")):
+            outs = []
+            for ch in (1, 2):
+                bs.set_tune({"restrict": ch})
+                tr, to = to_dev(r), to_dev(np.zeros_like(r))
+                ctx.call("mgx_box_precond_mg", bs.h, 2, 2, gam, code, mgx.ptr(tr), mgx.ptr(to))
+                outs.append(to_np(to))
+            d = relmax(outs[0], outs[1])
+            say("restriction fused / separate", edges, start, name, d)
+            if name == "f64":
+                assert d <= 1e-14
+            else:
+                assert np.array_equal(outs[0], outs[1])
+            if name != "f16":
+                ref = O.b_precond(ob, 2, 2, gam, r, dtype=np.float64 if name == "f64" else np.float32)
+                assert relmax(outs[0], ref) <= (1e-12 if name == "f64" else 1e-5)
+    finally:
+        bs.close()
+
+
 @pytest.mark.parametrize("prec", [mgx.PREC_F64, mgx.PREC_SPLIT])
 def test_gram_waves_match_oracle(ctx, prec):
     cfg = _cfg()
@@ -139,8 +177,9 @@ def test_solver_tunes_its_box_set(ctx):
         assert tuned
         e = [s.step().band_energy for _ in range(6)]
         s.close()
-        # the other shapes give the same steps to rounding
-        other = {k: 3 - v for k, v in choice.items()}
+        # the other lane mappings give the same steps bit for bit (the
+        # restriction's choice, which moves FP64 sums by rounding, is kept)
+        other = {k: (v if k == "restrict" else 3 - v) for k, v in choice.items()}
         bs.set_tune(other)
         s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, mg_levels=2, anderson_m=1)
         assert bs.get_tune()[0] == other
