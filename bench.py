@@ -374,7 +374,9 @@ def main():
         "autotune": {k: {"choice": c, "ms_per_launch": [round(t1, 4), round(t2, 4)],
                          "candidates": ("waves per cell 1, 2" if k.startswith("gram")
                                         else "restriction inside the FP32 fine pre-sweep / separate kernel"
-                                        if k == "restrict" else "lane groups per thread 1, 2")}
+                                        if k == "restrict"
+                                        else "prolongation inside the FP32 fine post-sweep / separate kernel"
+                                        if k == "prolong" else "lane groups per thread 1, 2")}
                      for k, (c, t1, t2) in tune.items()},
         "cpu_baseline": cpu,
     }

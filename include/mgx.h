@@ -298,8 +298,12 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* boxes, int levels, mgx_bo
  *     with paired z-loads (FP64 / hi-lo split MFMA), 1 or 2;
  *   MGX_TUNE_RESTRICT: the V-cycle's restriction of the stale residual
  *     (MGkernelRestrict3D) inside the last pre-sweep of the marching kernel
- *     (1; w is then never written) or as its own kernel after it (2); timed
- *     on the FP32 fine sweep, applied at every level and precision.
+ *     (1; w is then never written) or as its own kernel after it (2);
+ *   MGX_TUNE_PROLONG: the coarse correction's prolongation
+ *     (MGkernelExtend3D, v -= P(c)) inside the first post-sweep (1; v is
+ *     then never written with it) or as its own kernel before it (2); both
+ *     transfers timed on the FP32 fine sweeps, applied at every level and
+ *     precision where the marching kernel runs the level.
  * mgx_boxset_autotune times every candidate on the box set's own layout
  * (scratch buffers of its size, 1 warm-up + 3 timed launches each, hipEvents)
  * and keeps the fastest; ms (may be NULL) receives MGX_TUNE_N x 2 times (ms
@@ -307,7 +311,8 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* boxes, int levels, mgx_bo
  * choice gives the same results (the per-point arithmetic does not depend
  * on the lane mapping; only the Anderson dot sums of the last sweep are
  * ordered differently; the fused restriction sums its 27 FP64 terms in
- * another order: the same FP32 / FP16 values, FP64 ones to the last bit).  A solver tunes its box set at creation unless it
+ * another order: the same FP32 / FP16 values, FP64 ones to the last bit;
+ * the fused prolongation is bit-identical).  A solver tunes its box set at creation unless it
  * was tuned or set before.  mgx_boxset_set_tune forces a choice; get reads it. */
 #define MGX_TUNE_MARCH_F16 0
 #define MGX_TUNE_MARCH_F32 1
@@ -316,7 +321,8 @@ int mgx_boxset_create(mgx_ctx* ctx, const mgx_boxes_t* boxes, int levels, mgx_bo
 #define MGX_TUNE_GRAM_F64 4
 #define MGX_TUNE_GRAM_SPLIT 5
 #define MGX_TUNE_RESTRICT 6
-#define MGX_TUNE_N 7
+#define MGX_TUNE_PROLONG 7
+#define MGX_TUNE_N 8
 int mgx_boxset_autotune(mgx_ctx* ctx, mgx_boxset* bs, int* choice, double* ms);
 int mgx_boxset_set_tune(mgx_boxset* bs, const int* choice);
 int mgx_boxset_get_tune(const mgx_boxset* bs, int* choice, int* tuned);

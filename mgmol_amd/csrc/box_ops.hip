@@ -77,7 +77,7 @@ struct mgx_boxset
     int* d_sp_e      = nullptr;
     // autotuned launch shapes (MGX_TUNE_*) and whether the tuner ran or a
     // caller set them
-    int tune[MGX_TUNE_N] = { 1, 1, 1, 1, 1, 1, 1 };
+    int tune[MGX_TUNE_N] = { 1, 1, 1, 1, 1, 1, 1, 1 };
     bool tuned           = false;
 };
 
@@ -309,6 +309,13 @@ struct MarchArgs
     // offsets offc, instead of w itself
     void* rc            = nullptr;
     const int64_t* offc = nullptr;
+    // JACOBI (k_box_march4, no dots, output type T), first sweep after the
+    // coarse correction: the input is in - P(pc) (MGkernelExtend3D of the
+    // coarse correction pc, type T, at offsets offc), formed as each plane
+    // is loaded, instead of being written back by a separate prolongation;
+    // cpl = elements per coarse plane in LDS, (max coarse edge + 1)^2
+    const void* pc = nullptr;
+    int cpl        = 0;
 };
 
 // the box of this block (grids padded to a multiple of 8); -1: no box
@@ -461,6 +468,9 @@ __global__ void __launch_bounds__(NT) k_box_march(const T* __restrict__ in, cons
     }
 }
 
+template <typename T>
+__device__ __forceinline__ T prolong8_val(const T (&c)[2][2][2], int ox, int oy, int oz);
+
 // The same sweeps with V consecutive z-points per thread (V = 4 for FP32,
 // 2 for FP64: 16 bytes; every box edge at this level a multiple of 4): one
 // 16-byte load or store per array and plane instead of V, and the y- and
@@ -485,7 +495,10 @@ __global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restri
     const int W  = a.lds_w;  // max edge + 2 ZR
     const int HW = W * (W - 2 * ZR + 4);
     const int64_t base = a.off[j];
-    for (int t = threadIdx.x; t < NB * HW; t += NT)
+    // XT 3: three coarse planes after the stencil planes, zero in their last
+    // row and column (coarse index ec, outside the box)
+    constexpr bool pro = XT == 3 && MODE == MARCH_JACOBI && std::is_same<T, TO>::value;
+    for (int t = threadIdx.x; t < NB * HW + (pro ? 3 * a.cpl : 0); t += NT)
         pl0[t] = (T)0;
     int qy[P], qz[P];
     bool act[P];
@@ -563,6 +576,64 @@ __global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restri
                 racc[u] += 2. * sp;
         }
     };
+    // the fused prolongation: coarse plane k (zero for k >= ec) is loaded to
+    // registers and stored to ring slot k % 3 (row stride ec + 1); plane
+    // m + 3 is staged in sweep iteration 2m, two iterations before the first
+    // fine plane that reads it is corrected (plane 2m + 5 at the end of
+    // iteration 2m + 2), and after the last read of plane m (fine plane
+    // 2m + 1, iteration 2m - 2)
+    static_assert(!pro || V * P <= 8, "the prolongation stages two coarse points per thread");
+    const T* pcs = reinterpret_cast<const T*>(a.pc);
+    T* cring     = pl0 + NB * HW;
+    auto cload   = [&](int k, T (&cv)[2]) {
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+        {
+            const int cp = threadIdx.x + u * NT;
+            cv[u]        = (k < ec && cp < ec * ec) ? pcs[a.offc[j] + (int64_t)k * ec * ec + cp] : (T)0;
+        }
+    };
+    auto cstore = [&](int k, const T (&cv)[2]) {
+        T* sl = cring + (k % 3) * a.cpl;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+        {
+            const int cp = threadIdx.x + u * NT;
+            if (cp < ec * ec) sl[(cp / ec) * (ec + 1) + cp - (cp / ec) * ec] = cv[u];
+        }
+    };
+    // lane group k of fine plane pn: val -= P(coarse), prolong8_val on the
+    // coarse values (xc + a, yc + b, zc + d) of each point, as k_box_prolong8
+    auto pcorr = [&](int pn, V4& val, int k) {
+        const int xc = pn >> 1, yc = qy[k] >> 1, zb = qz[k] >> 1;
+        const T* s0  = cring + (xc % 3) * a.cpl + yc * (ec + 1) + zb;
+        const T* s1  = cring + ((xc + 1) % 3) * a.cpl + yc * (ec + 1) + zb;
+        T cz[2][2][V / 2 + 1];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int d = 0; d <= V / 2; d++)
+            {
+                cz[0][b][d] = s0[b * (ec + 1) + d];
+                cz[1][b][d] = s1[b * (ec + 1) + d];
+            }
+#pragma unroll
+        for (int i = 0; i < V; i++)
+        {
+            T c[2][2][2];
+#pragma unroll
+            for (int aa = 0; aa < 2; aa++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int d = 0; d < 2; d++)
+                        c[aa][b][d] = cz[aa][b][(i >> 1) + d];
+            const T pv = prolong8_val(c, pn & 1, qy[k] & 1, i & 1);
+            T v        = val[i];
+            v += static_cast<T>(-1. * static_cast<double>(pv));
+            val[i] = v;
+        }
+    };
 #pragma unroll
     for (int k = 0; k < P; k++)
     {
@@ -573,11 +644,34 @@ __global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restri
         qv[k][3]    = (act[k] && e > 1) ? ld(base + E2 + q) : zero4;
         qv[k][4]    = (act[k] && e > 2) ? ld(base + 2 * E2 + q) : zero4;
     }
+    if constexpr (pro)
+    {
+        T cv[2];
+        __syncthreads();  // the ring's zero fill before its first planes
+        for (int kc = 0; kc < 3; kc++)
+        {
+            cload(kc, cv);
+            cstore(kc, cv);
+        }
+    }
     __syncthreads();
+    if constexpr (pro)
+    {
+#pragma unroll
+        for (int k = 0; k < P; k++)
+            if (act[k])
+            {
+                pcorr(0, qv[k][2], k);
+                pcorr(1, qv[k][3], k);
+                pcorr(2, qv[k][4], k);
+            }
+    }
     for (int x = 0; x < e; x++)
     {
         T* pl = (NB == 2 && (x & 1)) ? pl0 + HW : pl0;
         double pot[P][V];
+        T cv[2];
+        if (pro && !(x & 1)) cload((x >> 1) + 3, cv);
 #pragma unroll
         for (int k = 0; k < P; k++)
         {
@@ -606,6 +700,7 @@ __global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restri
             if (act[k]) *reinterpret_cast<V4*>(pl + (qy[k] + 2) * W + qz[k] + ZR) = qv[k][2];
         __syncthreads();
         if (rst && x > 0) rplane(x - 1);
+        if (pro && !(x & 1)) cstore((x >> 1) + 3, cv);
 #pragma unroll
         for (int k = 0; k < P; k++)
         {
@@ -705,6 +800,7 @@ __global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restri
 #pragma unroll
         for (int k = 0; k < P; k++)
         {
+            if (pro && act[k] && x + 3 < e) pcorr(x + 3, nx[k], k);
             qv[k][0] = qv[k][1];
             qv[k][1] = qv[k][2];
             qv[k][2] = qv[k][3];
@@ -2796,9 +2892,10 @@ void march4_launch_d(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, siz
 {
     constexpr int V  = march_v<T>();
     constexpr int ZR = V > 2 ? V : 2;
-    // the fused restriction's two planes of w behind the stencil planes
+    // the fused restriction's two planes of w, or the fused prolongation's
+    // three coarse planes, behind the stencil planes
     const int em    = a.lds_w - 2 * ZR;
-    const size_t rb = a.rc ? 2 * (size_t)em * em * sizeof(T) : 0;
+    const size_t rb = a.rc ? 2 * (size_t)em * em * sizeof(T) : a.pc ? 3 * (size_t)a.cpl * sizeof(T) : 0;
     // two plane buffers when the blocks that fill a CU (32 waves) still fit
     // its 160 KB of LDS
     if (2 * plane * (2048 / NT) <= 160 * 1024)
@@ -2822,27 +2919,62 @@ void march4_launch(mgx_ctx* ctx, const mgx_boxset* bs, const MarchArgs& a, size_
         return;
     }
     if constexpr (J && std::is_same<T, TO>::value)
+    {
         if (a.rc)
         {
             march4_launch_d<T, TYPE, MODE, NT, P, 2, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
             return;
         }
+        if (a.pc)
+        {
+            march4_launch_d<T, TYPE, MODE, NT, P, 3, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
+            return;
+        }
+    }
     march4_launch_d<T, TYPE, MODE, NT, P, 0, TO>(ctx, bs, a, plane, in, f, v, out, w, c);
 }
 
 // the marching sweep for 2nd/4th-order operators; returns false if the box
 // sizes at this level need another kernel
+// whether k_box_march4 runs the sweeps of this level with `extra` bytes of
+// LDS per block beyond its two stencil planes (the fused transfers)
+template <typename T>
+bool march4_applies(const mgx_boxset* bs, int level, size_t extra)
+{
+    constexpr int V  = march_v<T>();
+    constexpr int VM = V > 4 ? V : 4;
+    constexpr int ZR = V > 2 ? V : 2;
+    const int emax   = bs->maxedge >> level;
+    if (emax % VM) return false;
+    for (int jb = 0; jb < bs->nbox; jb++)
+        if ((bs->edge[jb] >> level) % VM) return false;
+    const size_t plane = (size_t)(emax + 2 * ZR) * (emax + 4) * sizeof(T);
+    return plane <= 60 * 1024 && (emax * emax) / V <= 1024 && 2 * plane + extra <= 64 * 1024;
+}
+
+// LDS bytes of the fused prolongation (three coarse planes) at this level
+template <typename T>
+size_t march_pro_lds(const mgx_boxset* bs, int level)
+{
+    const size_t c = (size_t)(bs->maxedge >> (level + 1)) + 1;
+    return 3 * c * c * sizeof(T);
+}
+
 // rc (JACOBI, output type T, no dots): the stale residual restricted to
-// level + 1 inside the sweep, in place of w; false if the box sizes need
-// the separate restriction
+// level + 1 inside the sweep, in place of w; pc (the same sweeps): the input
+// is in - P(pc), pc the correction at level + 1 (in is not written); false
+// if the box sizes need the separate transfer kernels
 template <typename T, int TYPE, int MODE, typename TO = T>
 bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* in, const T* f, const double* v,
     TO* out, T* w, const double* nrm = nullptr, const MarchDots* dots = nullptr, const double* oinv = nullptr,
-    T* rc = nullptr)
+    T* rc = nullptr, const T* pc = nullptr)
 {
     constexpr bool mixed = !std::is_same<T, TO>::value;
     static_assert(!mixed || MODE == MARCH_JACOBI, "another output type for the Jacobi sweep only");
-    if (rc && (MODE != MARCH_JACOBI || mixed || dots || w || level + 1 >= bs->levels)) return false;
+    if ((rc || pc) && (MODE != MARCH_JACOBI || mixed || dots || level + 1 >= bs->levels || (rc && pc)))
+        return false;
+    if (rc && w) return false;
+    if (pc && !march4_applies<T>(bs, level, march_pro_lds<T>(bs, level))) return false;
     const int emax = bs->maxedge >> level;
     const int E2   = emax * emax;
     MarchArgs a;
@@ -2892,6 +3024,13 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
             a.rc   = rc;
             a.offc = bs->d_off[level + 1];
         }
+        if (pc)
+        {
+            const int ecm = (emax >> 1) + 1;
+            a.pc          = pc;
+            a.cpl         = ecm * ecm;
+            a.offc        = bs->d_off[level + 1];
+        }
         if (plane <= 60 * 1024 && groups <= 1024)
         {
             if constexpr (TYPE == MGX_LAP_4TH && (MODE == MARCH_JACOBI || MODE == MARCH_HLOC))
@@ -2927,7 +3066,7 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
         if (dots) return false;
         a.lds_w = emax + 4;
     }
-    if (rc) return false;
+    if (rc || pc) return false;
     if constexpr (mixed)
         return false;
     else
@@ -2997,12 +3136,27 @@ int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vt
 
 // rc (with rc_done): the stale residual restricted to level + 1 inside the
 // sweep when the marching kernel applies (*rc_done = true; w is then not
-// written), else w as usual
+// written), else w as usual.  pc: the sweep of x - P(pc) (pc at level + 1),
+// only where march4_applies says the fused form runs (else an error)
 template <typename T>
 int box_jacobi_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, double jf, const T* x, const T* f, T* xo,
-    T* w, const MarchDots* dots = nullptr, bool* dots_done = nullptr, T* rc = nullptr, bool* rc_done = nullptr)
+    T* w, const MarchDots* dots = nullptr, bool* dots_done = nullptr, T* rc = nullptr, bool* rc_done = nullptr,
+    const T* pc = nullptr)
 {
     const double mjf = -1. * jf;
+    if (pc)
+    {
+        const bool ok = !dots && !w
+                        && ((type == MGX_LAP_4TH
+                                && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, nullptr,
+                                    nullptr, nullptr, nullptr, nullptr, pc))
+                            || (type == MGX_LAP_2ND
+                                && march<T, MGX_LAP_2ND, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo,
+                                    nullptr, nullptr, nullptr, nullptr, nullptr, pc)));
+        if (!ok) return set_err(ctx, MGX_EINVAL, "fused prolongation needs the marching kernel");
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
     if (rc && !dots
         && ((type == MGX_LAP_4TH
                 && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, nullptr, nullptr,
@@ -3196,20 +3350,33 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
         rcoarse, B.tmp[level + 1], nc, jacobi_factor(MGX_LAP_2ND, hc));
     MGX_CHECK_LAUNCH(ctx);
     if ((rc = box_mg_rec(ctx, bs, B, max_levels, nv, (const T*)rcoarse, MGX_LAP_2ND, level + 1, true))) return rc;
-    tok = level == 0 ? prof_begin(ctx, MGX_PROF_TRANSFER, st * np0 * 17. / 8.) : -1;
-    if ((rc = box_prolong_t<T, true>(ctx, bs, level, (const T*)nv, v))) return rc;
-    prof_end(ctx, tok);
+    // the coarse correction prolonged inside the first post-sweep
+    // (MGX_TUNE_PROLONG choice 1, where the marching kernel runs this level),
+    // else by its own kernel into v
+    const bool pfuse = bs->tune[MGX_TUNE_PROLONG] == 1 && (type == MGX_LAP_4TH || type == MGX_LAP_2ND)
+                       && march4_applies<T>(bs, level, march_pro_lds<T>(bs, level));
+    if (!pfuse)
+    {
+        tok = level == 0 ? prof_begin(ctx, MGX_PROF_TRANSFER, st * np0 * 17. / 8.) : -1;
+        if ((rc = box_prolong_t<T, true>(ctx, bs, level, (const T*)nv, v))) return rc;
+        prof_end(ctx, tok);
+    }
     a = v;
     b = B.tmp[level];
     for (int it = 0; it < 2; it++)
     {
-        const int tk = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0, 3. * st * np0) : -1;
+        // fused: x, f and the coarse correction read, x written
+        const double pw = (it == 0 && pfuse) ? 1. / 8. : 0.;
+        const int tk    = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0, (3. + pw) * st * np0) : -1;
         // the last fine sweep (it writes v) may also form the Anderson dots;
         // with last_out it writes FP32 there instead, times last_inv per box
         const MarchDots* dd = (level == 0 && it == 1) ? dots : nullptr;
         bool dd_done        = false;
         if (level == 0 && it == 1 && last_out)
             rc = box_jacobi_out(ctx, bs, type, jf, (const T*)a, f, last_out, last_inv, dd, &dd_done);
+        else if (it == 0 && pfuse)
+            rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr, nullptr, nullptr,
+                (T*)nullptr, nullptr, (const T*)nv);
         else
             rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, (T*)nullptr, dd, &dd_done);
         if (rc) return rc;
@@ -3619,6 +3786,16 @@ int box_autotune(mgx_ctx* ctx, mgx_boxset* bs, double* ms)
                 nullptr, nullptr, rcv);
         return march<float, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, x, fv, nullptr, xo, w)
                && box_restrict_t<float>(ctx, bs, 0, w, rcv) == MGX_OK;
+    });
+    // the FP32 post-sweep with the prolongation fused, or after it
+    tune1(MGX_TUNE_PROLONG, 2, [&]() {
+        if (bs->levels < 2) return false;
+        float *x = (float*)a, *fv = (float*)b, *pcv = fv + np, *xo = (float*)c;
+        if (bs->tune[MGX_TUNE_PROLONG] == 1)
+            return march<float, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, x, fv, nullptr, xo, nullptr, nullptr,
+                nullptr, nullptr, nullptr, pcv);
+        return box_prolong_t<float, true>(ctx, bs, 0, pcv, x) == MGX_OK
+               && march<float, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, 0, mjf, x, fv, nullptr, xo, nullptr);
     });
     for (int prec : { MGX_PREC_F64, MGX_PREC_SPLIT })
         tune1(prec == MGX_PREC_F64 ? MGX_TUNE_GRAM_F64 : MGX_TUNE_GRAM_SPLIT, 2, [&]() {

@@ -277,7 +277,7 @@ def block(t, grid, nfunc=None):
 
 
 # the autotuner's tunables, in MGX_TUNE_* order
-TUNE_NAMES = ["march_f16", "march_f32", "march_f64", "hloc_f64", "gram_f64", "gram_split", "restrict"]
+TUNE_NAMES = ["march_f16", "march_f32", "march_f64", "hloc_f64", "gram_f64", "gram_split", "restrict", "prolong"]
 
 
 class BoxSet:

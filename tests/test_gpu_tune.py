@@ -9,11 +9,12 @@ can choose.
   mapping); S / H with one or two waves per cell within the oracle's
   tolerances and bit-identical to each other (the cell's rows are summed in
   two fixed halves whatever the number of waves).
-* The V-cycle's restriction inside the last pre-sweep against the separate
-  restriction kernel: FP32 and FP16 storage bit-identical, FP64 to the last
-  bits (the 27 FP64 terms are summed in another order), both within the
-  oracle's tolerances, on levels where the fused form applies and where it
-  does not.
+* The V-cycle's transfers inside the sweeps against the separate kernels:
+  the restriction inside the last pre-sweep (FP32 and FP16 storage
+  bit-identical, FP64 to the last bits: the 27 FP64 terms are summed in
+  another order) and the prolongation inside the first post-sweep
+  (bit-identical in every precision), all within the oracle's tolerances,
+  on levels where the fused forms apply and where they do not.
 * A solver on a box set that was not tuned tunes it at creation, and its
   steps do not depend on the lane mappings (the same energies bit for bit).
 """
@@ -110,7 +111,7 @@ def test_f16_vcycle_shapes_identical(ctx):
 
 @pytest.mark.parametrize("edges,gdim", [((32,), 72), ((8, 12, 16, 20, 24), 56)])
 @pytest.mark.parametrize("start", ["zero", "gamma"])
-def test_fused_restriction_matches_separate(ctx, edges, gdim, start):
+def test_fused_transfers_match_separate(ctx, edges, gdim, start):
     # (8 .. 24): at level 1 the edges 6, 10 are not multiples of 4, so that
     # level keeps the separate restriction under either choice
     cfg = synth.box_config(nbox=12, gdim=gdim, h=0.3, edges=edges, nwells=4, seed=23, levels=3)
@@ -122,21 +123,27 @@ def test_fused_restriction_matches_separate(ctx, edges, gdim, start):
         # the FP16 V-cycle needs edges that are multiples of 8 at every level
         kinds = ((mgx.F16, "f16"),) if edges == (32,) else ()
         for code, name in kinds + ((mgx.F32, "f32"), (mgx.F64, "f64")):
-            outs = []
-            for ch in (1, 2):
-                bs.set_tune({"restrict": ch})
-                tr, to = to_dev(r), to_dev(np.zeros_like(r))
-                ctx.call("mgx_box_precond_mg", bs.h, 2, 2, gam, code, mgx.ptr(tr), mgx.ptr(to))
-                outs.append(to_np(to))
-            d = relmax(outs[0], outs[1])
-            say("restriction fused / separate", edges, start, name, d)
+            outs = {}
+            for rs in (1, 2):
+                for pr in (1, 2):
+                    bs.set_tune({"restrict": rs, "prolong": pr})
+                    tr, to = to_dev(r), to_dev(np.zeros_like(r))
+                    ctx.call("mgx_box_precond_mg", bs.h, 2, 2, gam, code, mgx.ptr(tr), mgx.ptr(to))
+                    outs[rs, pr] = to_np(to)
+            sep = outs[2, 2]
+            for rs in (1, 2):
+                # the prolongation's choice changes nothing
+                assert np.array_equal(outs[rs, 1], outs[rs, 2])
+            d = relmax(outs[1, 2], sep)
+            say("transfers fused / separate", edges, start, name, d)
             if name == "f64":
                 assert d <= 1e-14
             else:
-                assert np.array_equal(outs[0], outs[1])
+                assert np.array_equal(outs[1, 2], sep)
             if name != "f16":
                 ref = O.b_precond(ob, 2, 2, gam, r, dtype=np.float64 if name == "f64" else np.float32)
-                assert relmax(outs[0], ref) <= (1e-12 if name == "f64" else 1e-5)
+                for o in outs.values():
+                    assert relmax(o, ref) <= (1e-12 if name == "f64" else 1e-5)
     finally:
         bs.close()
 
@@ -177,8 +184,9 @@ def test_solver_tunes_its_box_set(ctx):
         assert tuned
         e = [s.step().band_energy for _ in range(6)]
         s.close()
-        # the other lane mappings give the same steps bit for bit (the
-        # restriction's choice, which moves FP64 sums by rounding, is kept)
+        # the other lane mappings and prolongations give the same steps bit
+        # for bit (the restriction's choice, which moves FP64 sums by
+        # rounding, is kept)
         other = {k: (v if k == "restrict" else 3 - v) for k, v in choice.items()}
         bs.set_tune(other)
         s = mgx.Solver(ctx, bs, cfg.vtot, cfg.psi, mg_levels=2, anderson_m=1)
