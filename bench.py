@@ -53,10 +53,10 @@ C5_POINTS = 2048 * 32 ** 3
 # passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, both calibrated on
 # streaming kernels of known size), copied from profiles/pmc_traffic_c5.md;
 # keyed by (group, precision of the path's kernel)
-TRAFFIC_C5 = {("jacobi0", "f32"): 893786214, ("jacobi0", "f16"): 403009536, ("cellgram", "split", 1): 1423095296,
-              ("cellgram", "split", 2): 1309666713, ("residk", "f32"): 1737051648,
-              ("learned", "bf16"): 17287778496, ("skin", "f32"): 1679117516, ("skin", "f64"): 1679117516,
-              ("skin", "f16"): 1679117516, ("nsgemm", "f64"): 913193984}
+TRAFFIC_C5 = {("jacobi0", "f32"): 839914598, ("jacobi0", "f16"): 420104704, ("cellgram", "split", 1): 1423482880,
+              ("cellgram", "split", 2): 1308473754, ("residk", "f32"): 1738115277,
+              ("learned", "bf16"): 17287778496, ("skin", "f32"): 1679062938, ("skin", "f64"): 1679062938,
+              ("skin", "f16"): 1679062938, ("nsgemm", "f64"): 912851968}
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c5.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; calibrated)"
 
 PATHS = {
@@ -85,10 +85,12 @@ GROUPS = {
     "residk": ("k_cell_residual<true, OM> R = psi theta - H psi per 8^3 cell (FP64 MFMA), written as the "
                "V-cycle's zero start f, x (or as R on network-trial steps)", "hbm"),
     "update": ("k_box_and_update_nrm<T, 32> Anderson update + normalisation", "hbm"),
-    "jacobi0": ("k_box_march4<T, 2, 2, ...> fine-level Jacobi sweep", "hbm"),
-    "jacobi0d": ("k_box_march4<T, 2, 2, ..., DOTS=true> last fine sweep with the Anderson dots", "hbm"),
+    "jacobi0": ("k_box_march4<T, 2, 2, ..., XT> fine-level Jacobi sweep: the pre-sweep with the restriction "
+                "inside (XT=2), the first post-sweep with the prolongation inside (XT=3)", "hbm"),
+    "jacobi0d": ("k_box_march4<T, 2, 2, ..., XT=1> last fine sweep with the Anderson dots", "hbm"),
     "first": ("k_first_sweep<T> zero-start first fine sweep (pointwise)", "hbm"),
-    "transfer": ("k_box_restrict2p<T> + k_box_prolong8<T, true> fine-level transfers (2 kernels)", "hbm"),
+    "transfer": ("k_box_restrict2p<T> + k_box_prolong8<T, true> fine-level transfers where the sweeps do not "
+                 "hold them (2 kernels)", "hbm"),
     "spmm": ("k_spmm_tiles pair-sparse X S and S^-1 H", "hbm"),
     "nsgemm": ("rocBLAS dgemm (Cijk_..._DB_MT64x64x16) Newton-Schulz X <- 2X - (XS)X", "mfma:f64"),
     "nsgemm16": ("rocBLAS gemm_ex BF16 (Cijk_..._BSS_BH_...) Newton-Schulz correction, K = 3n", "mfma:bf16"),
