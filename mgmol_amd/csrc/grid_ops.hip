@@ -943,7 +943,13 @@ int march_grid(mgx_ctx* ctx, int type, int mode, const T* in, const MView& vi, c
     g.oy = vo.iy;
     g.o0 = vo.i0;
     g.ostr = vo.str;
-    g.ty = std::min(std::min(16, 1024 / G), d1);
+    // tile rows: at most 16, and few enough z-pairs per block that each of
+    // the 256 threads holds one lane group (H psi) or two (the sweeps and
+    // -Lap): three or four (182-221 VGPRs in FP64) leave 2 waves per SIMD;
+    // on 96^3 x 64 functions H psi takes 0.28 / 0.37 / 0.46 ms with 5 / 10 /
+    // 16 rows and the FP64 V-cycle 4.0 / 3.8 / 4.2 ms
+    const int maxitems = mode == MG_HLOC ? 256 : 512;
+    g.ty = std::min(std::min(16, std::max(1, maxitems / G)), d1);
     const size_t lds = 2 * (size_t)(g.ty + 2 * R) * (d2 + 4) * sizeof(T);
     if (lds > 64 * 1024) return 0;
     g.nty = (d1 + g.ty - 1) / g.ty;
