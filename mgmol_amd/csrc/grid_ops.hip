@@ -65,6 +65,61 @@ __global__ void k_halo(T* u, GGeom g, int64_t total, int bc0, int bc1, int bc2)
     }
 }
 
+// k_halo for grids periodic in every direction: only the ghost points, each
+// copied from its wrapped interior image (the same values).  Index space:
+// first the rows that are ghosts as a whole (the 2 ng ghost x-planes, then
+// the 2 ng ghost y-rows of each interior x-plane), na points, then the 2 ng
+// z-ghosts of each interior row, nb points.
+template <typename T>
+__global__ void k_halo_per(T* u, GGeom g, int64_t na, int64_t nb)
+{
+    const int ng = g.ng;
+    const int ey = g.d1 + 2 * ng, ez = (int)g.incy, w = 2 * ng;
+    const int64_t rows = (int64_t)w * ey + (int64_t)g.d0 * w;  // whole ghost rows per function
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < na + nb;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        int64_t f;
+        int x, y, z;
+        if (t < na)
+        {
+            const int64_t q = t / ez;
+            z               = (int)(t - q * ez);
+            f               = q / rows;
+            int64_t r       = q - f * rows;
+            if (r < (int64_t)w * ey)
+            {
+                const int p = (int)(r / ey);
+                y           = (int)(r - (int64_t)p * ey);
+                x           = p < ng ? p : g.d0 + p;
+            }
+            else
+            {
+                r -= (int64_t)w * ey;
+                const int xi = (int)(r / w), k = (int)(r - (int64_t)xi * w);
+                x = ng + xi;
+                y = k < ng ? k : g.d1 + k;
+            }
+        }
+        else
+        {
+            int64_t s   = t - na;
+            const int k = (int)(s % w);
+            s /= w;
+            const int yi = (int)(s % g.d1);
+            s /= g.d1;
+            const int xi = (int)(s % g.d0);
+            f            = s / g.d0;
+            x = ng + xi;
+            y = ng + yi;
+            z = k < ng ? k : g.d2 + k;
+        }
+        T* uf             = u + f * g.sizeg;
+        const int cx      = wrapi(x - ng, g.d0) + ng, cy = wrapi(y - ng, g.d1) + ng, cz = wrapi(z - ng, g.d2) + ng;
+        uf[(int64_t)x * g.incx + (int64_t)y * g.incy + z] = uf[(int64_t)cx * g.incx + (int64_t)cy * g.incy + cz];
+    }
+}
+
 // ---------------------------------------------------------------- stencils --
 template <typename T>
 __device__ __forceinline__ double lap4_at(const T* a, int64_t i, int64_t X, int64_t Y, const Coef4& c,
@@ -410,6 +465,31 @@ __global__ void k_packed_to_ghosted(const double* src, int64_t ld, T* dst, GGeom
         if (x >= 0 && y >= 0 && z >= 0 && x < g.d0 && y < g.d1 && z < g.d2)
             v = (T)src[f * ld + ((int64_t)x * g.d1 + y) * g.d2 + z];
         dst[t] = v;
+    }
+}
+
+// k_packed_to_ghosted into f with the preconditioner's start x = 0 + gamma f
+// (the fill and k_axpy it replaces, the same arithmetic) in the same pass
+template <typename T>
+__global__ void k_packed_to_ghosted_start(const double* src, int64_t ld, T* dst, T* xs, double gamma, GGeom g,
+    int64_t total)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t f = t / g.sizeg;
+        int64_t r       = t - f * g.sizeg;
+        const int x     = (int)(r / g.incx) - g.ng;
+        r -= (int64_t)(x + g.ng) * g.incx;
+        const int y = (int)(r / g.incy) - g.ng;
+        const int z = (int)(r - (int64_t)(y + g.ng) * g.incy) - g.ng;
+        T v         = (T)0;
+        if (x >= 0 && y >= 0 && z >= 0 && x < g.d0 && y < g.d1 && z < g.d2)
+            v = (T)src[f * ld + ((int64_t)x * g.d1 + y) * g.d2 + z];
+        dst[t] = v;
+        T xv   = (T)0;
+        xv += static_cast<T>(gamma * static_cast<double>(v));
+        xs[t] = xv;
     }
 }
 
@@ -862,6 +942,16 @@ int launch_halo(mgx_ctx* ctx, T* u, int nfunc, const mgx_grid_t& gr)
     GGeom g = ggeom(gr);
     for (int i = 0; i < 3; i++)
         MGX_REQUIRE(ctx, gr.ghosts <= gr.dim[i], "ghost width exceeds dimension");
+    if (gr.bc[0] == 1 && gr.bc[1] == 1 && gr.bc[2] == 1)
+    {
+        if (g.ng == 0) return MGX_OK;
+        const int w        = 2 * g.ng;
+        const int64_t rows = (int64_t)w * (g.d1 + w) + (int64_t)g.d0 * w;
+        const int64_t na = (int64_t)nfunc * rows * g.incy, nb = (int64_t)nfunc * g.d0 * g.d1 * w;
+        hipLaunchKernelGGL(k_halo_per<T>, dim3(nblocks(na + nb)), dim3(kBlock), 0, ctx->stream, u, g, na, nb);
+        MGX_CHECK_LAUNCH(ctx);
+        return MGX_OK;
+    }
     const int64_t total = (int64_t)nfunc * g.sizeg;
     hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, u, g, total, gr.bc[0],
         gr.bc[1], gr.bc[2]);
@@ -1287,12 +1377,12 @@ int precond_run(mgx_ctx* ctx, int type, int max_levels, double gamma, const mgx_
     if (!buf) return set_err(ctx, MGX_ENOMEM, "workspace allocation failed");
     T* x = buf;
     T* f = buf + n;
-    hipLaunchKernelGGL(k_packed_to_ghosted<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, ld, f, g, n);
+    // f from the packed residual and the start x = gamma f in one pass
+    hipLaunchKernelGGL(k_packed_to_ghosted_start<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, ld, f, x,
+        gamma, g, n);
     MGX_CHECK_LAUNCH(ctx);
-    int rc = launch_fill(ctx, x, n, (T)0);
+    int rc = mg_run(ctx, type, max_levels, x, (const T*)f, nfunc, g0, masks, 1);
     if (rc) return rc;
-    if ((rc = launch_axpy(ctx, n, gamma, (const T*)f, x))) return rc;
-    if ((rc = mg_run(ctx, type, max_levels, x, (const T*)f, nfunc, g0, masks, 1))) return rc;
     const int64_t m = (int64_t)nfunc * g.size;
     hipLaunchKernelGGL(k_ghosted_to_packed<T>, dim3(nblocks(m)), dim3(kBlock), 0, ctx->stream, x, g, out, ldo, m);
     MGX_CHECK_LAUNCH(ctx);
