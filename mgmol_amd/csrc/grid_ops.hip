@@ -1307,8 +1307,11 @@ int mg_setup(mgx_ctx* ctx, MGBufs<T>& B, const mgx_grid_t& g0, int max_levels, i
 
 // Preconditioning<T>::mg, src/Preconditioning.cc:155-216
 template <typename T>
+// pout (FP64 V-cycles, no masks, periodic grids): the last sweep of the call
+// writes the packed result there (*pout_done = true; v then keeps the
+// previous iterate) when the marching kernel applies
 int mg_rec(mgx_ctx* ctx, MGBufs<T>& B, int max_levels, T* v, const T* f, int type, int level, int nfunc,
-    const uint8_t* const* masks)
+    const uint8_t* const* masks, double* pout = nullptr, int64_t ldo = 0, bool* pout_done = nullptr)
 {
     const mgx_grid_t& g = B.grids[level];
     const int ncycl     = (level == max_levels) ? 4 : 2;
@@ -1350,7 +1353,21 @@ int mg_rec(mgx_ctx* ctx, MGBufs<T>& B, int max_levels, T* v, const T* f, int typ
     if ((rc = launch_halo(ctx, nv, nfunc, gc))) return rc;
     if ((rc = launch_prolong_sub(ctx, (const T*)nv, gc, v, g, nfunc, mk))) return rc;
     for (int it = 0; it < 2; it++)
+    {
+        if constexpr (std::is_same<T, double>::value)
+            if (it == 1 && pout && !mk && g.bc[0] == 1 && g.bc[1] == 1 && g.bc[2] == 1)
+            {
+                const int m = march_grid(ctx, type, MG_JACOBI, (const T*)cur, mview_ghosted(g), f, nullptr, pout,
+                    mview_packed(g, ldo), (T*)nullptr, nfunc, g, -1. * jf);
+                if (m < 0) return m;
+                if (m)
+                {
+                    *pout_done = true;
+                    return MGX_OK;
+                }
+            }
         if ((rc = sweep(false))) return rc;
+    }
     if ((rc = settle())) return rc;
     if (g.bc[0] != 1 || g.bc[1] != 1 || g.bc[2] != 1) return launch_halo(ctx, v, nfunc, g);
     return MGX_OK;
@@ -1358,13 +1375,13 @@ int mg_rec(mgx_ctx* ctx, MGBufs<T>& B, int max_levels, T* v, const T* f, int typ
 
 template <typename T>
 int mg_run(mgx_ctx* ctx, int type, int max_levels, T* v, const T* f, int nfunc, const mgx_grid_t& g0,
-    const uint8_t* const* masks, int slot)
+    const uint8_t* const* masks, int slot, double* pout = nullptr, int64_t ldo = 0, bool* pout_done = nullptr)
 {
     MGX_REQUIRE(ctx, type >= 0 && type <= 2, "LapFactory::createLap() --- option invalid");
     MGBufs<T> B;
     int rc = mg_setup(ctx, B, g0, max_levels, nfunc, slot);
     if (rc) return rc;
-    return mg_rec(ctx, B, max_levels, v, f, type, 0, nfunc, masks);
+    return mg_rec(ctx, B, max_levels, v, f, type, 0, nfunc, masks, pout, ldo, pout_done);
 }
 
 template <typename T>
@@ -1381,8 +1398,10 @@ int precond_run(mgx_ctx* ctx, int type, int max_levels, double gamma, const mgx_
     hipLaunchKernelGGL(k_packed_to_ghosted_start<T>, dim3(nblocks(n)), dim3(kBlock), 0, ctx->stream, res, ld, f, x,
         gamma, g, n);
     MGX_CHECK_LAUNCH(ctx);
-    int rc = mg_run(ctx, type, max_levels, x, (const T*)f, nfunc, g0, masks, 1);
-    if (rc) return rc;
+    // the last fine sweep may write the packed output itself
+    bool done = false;
+    int rc    = mg_run(ctx, type, max_levels, x, (const T*)f, nfunc, g0, masks, 1, out, ldo, &done);
+    if (rc || done) return rc;
     const int64_t m = (int64_t)nfunc * g.size;
     hipLaunchKernelGGL(k_ghosted_to_packed<T>, dim3(nblocks(m)), dim3(kBlock), 0, ctx->stream, x, g, out, ldo, m);
     MGX_CHECK_LAUNCH(ctx);

@@ -214,20 +214,28 @@ def test_hloc_packed_f32(ctx):
     assert relmax(to_np(th), ref) <= 2e-6
 
 
+# output leading dimensions: numpt and numpt + 8 (the FP64 cycle's last
+# sweep writes the packed output itself), numpt + 3 (odd: the conversion
+# kernel writes it); the padding keeps what it held
+@pytest.mark.parametrize("pad", [0, 8, 3])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_precond_mg(ctx, prec):
+def test_precond_mg(ctx, prec, pad):
     og, mg_ = _grids(dims=(16, 16, 16), ll=4.8)
     rng = np.random.default_rng(10)
     nf = 2
     res = rng.standard_normal((nf, og.size))
     gam = O.gamma(list(og.h), 2, 0.0, -1.0)
-    tr, to = to_dev(res), to_dev(np.zeros_like(res))
+    ldo = og.size + pad
+    out = np.full(nf * ldo, 7.25)
+    tr, to = to_dev(res), to_dev(out)
     pd = mgx.F32 if prec == "f32" else mgx.F64
-    ctx.call("mgx_precond_mg", 2, 2, gam, pd, mg_, mgx.ptr(tr), og.size, nf, None, mgx.ptr(to), og.size)
+    ctx.call("mgx_precond_mg", 2, 2, gam, pd, mg_, mgx.ptr(tr), og.size, nf, None, mgx.ptr(to), ldo)
+    got = to_np(to).reshape(nf, ldo)
     ref = O.precond(og, 2, 2, gam, res, dtype=DT[prec])
-    err = relmax(to_np(to), ref)
-    say("precond", prec, err)
+    err = relmax(got[:, :og.size], ref)
+    say("precond", prec, "pad", pad, err)
     assert err <= (1e-12 if prec == "f64" else 1e-5)
+    assert np.all(got[:, og.size:] == 7.25)
 
 
 def test_invalid_lap_type_rejected(ctx):
