@@ -309,6 +309,10 @@ struct MarchArgs
     // offsets offc, instead of w itself
     void* rc            = nullptr;
     const int64_t* offc = nullptr;
+    // with rc: the coarse level's zero-start first sweep x = 0 + rjf f
+    // (k_first_sweep's arithmetic) written to rx beside it
+    void* rx   = nullptr;
+    double rjf = 0.;
     // JACOBI (k_box_march4, no dots, output type T), first sweep after the
     // coarse correction: the input is in - P(pc) (MGkernelExtend3D of the
     // coarse correction pc, type T, at offsets offc), formed as each plane
@@ -551,7 +555,8 @@ __global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restri
     double racc[2] = { 0., 0. };
     auto rplane    = [&](int X) {
         const T* R = rw + (X & 1) * E2;
-        T* dst     = reinterpret_cast<T*>(a.rc) + a.offc[j] + (int64_t)(X >> 1) * ec * ec;
+        const int64_t co = a.offc[j] + (int64_t)(X >> 1) * ec * ec;
+        T* dst           = reinterpret_cast<T*>(a.rc) + co;
 #pragma unroll
         for (int u = 0; u < 2; u++)
         {
@@ -569,8 +574,15 @@ __global__ void __launch_bounds__(NT, XT ? 6 : 8) k_box_march4(const T* __restri
             if (X & 1)
             {
                 racc[u] += sp;
-                dst[cp]  = (T)(kInv64 * racc[u]);
-                racc[u]  = sp;
+                const T cv = (T)(kInv64 * racc[u]);
+                dst[cp]    = cv;
+                if (a.rx)
+                {
+                    T xv = (T)0;
+                    xv += static_cast<T>(a.rjf * static_cast<double>(cv));
+                    reinterpret_cast<T*>(a.rx)[co + cp] = xv;
+                }
+                racc[u] = sp;
             }
             else
                 racc[u] += 2. * sp;
@@ -2967,7 +2979,7 @@ size_t march_pro_lds(const mgx_boxset* bs, int level)
 template <typename T, int TYPE, int MODE, typename TO = T>
 bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* in, const T* f, const double* v,
     TO* out, T* w, const double* nrm = nullptr, const MarchDots* dots = nullptr, const double* oinv = nullptr,
-    T* rc = nullptr, const T* pc = nullptr)
+    T* rc = nullptr, const T* pc = nullptr, T* rx = nullptr, double rjf = 0.)
 {
     constexpr bool mixed = !std::is_same<T, TO>::value;
     static_assert(!mixed || MODE == MARCH_JACOBI, "another output type for the Jacobi sweep only");
@@ -3023,6 +3035,8 @@ bool march(mgx_ctx* ctx, const mgx_boxset* bs, int level, double mjf, const T* i
             if (2 * plane + 2 * (size_t)E2 * sizeof(T) > 64 * 1024 || groups > 1024) return false;
             a.rc   = rc;
             a.offc = bs->d_off[level + 1];
+            a.rx   = rx;
+            a.rjf  = rjf;
         }
         if (pc)
         {
@@ -3136,12 +3150,14 @@ int box_hloc_t(mgx_ctx* ctx, const mgx_boxset* bs, const T* in, const double* vt
 
 // rc (with rc_done): the stale residual restricted to level + 1 inside the
 // sweep when the marching kernel applies (*rc_done = true; w is then not
-// written), else w as usual.  pc: the sweep of x - P(pc) (pc at level + 1),
-// only where march4_applies says the fused form runs (else an error)
+// written, and rx, if given, receives the coarse level's first sweep
+// x = 0 + rjf rc), else w as usual.  pc: the sweep of x - P(pc) (pc at
+// level + 1), only where march4_applies says the fused form runs (else an
+// error)
 template <typename T>
 int box_jacobi_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, double jf, const T* x, const T* f, T* xo,
     T* w, const MarchDots* dots = nullptr, bool* dots_done = nullptr, T* rc = nullptr, bool* rc_done = nullptr,
-    const T* pc = nullptr)
+    const T* pc = nullptr, T* rx = nullptr, double rjf = 0.)
 {
     const double mjf = -1. * jf;
     if (pc)
@@ -3160,10 +3176,10 @@ int box_jacobi_t(mgx_ctx* ctx, const mgx_boxset* bs, int level, int type, double
     if (rc && !dots
         && ((type == MGX_LAP_4TH
                 && march<T, MGX_LAP_4TH, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, nullptr, nullptr,
-                    nullptr, nullptr, rc))
+                    nullptr, nullptr, rc, nullptr, rx, rjf))
             || (type == MGX_LAP_2ND
                 && march<T, MGX_LAP_2ND, MARCH_JACOBI>(ctx, bs, level, mjf, x, f, nullptr, xo, nullptr, nullptr,
-                    nullptr, nullptr, rc))))
+                    nullptr, nullptr, rc, nullptr, rx, rjf))))
     {
         MGX_CHECK_LAUNCH(ctx);
         *rc_done = true;
@@ -3317,6 +3333,7 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
     // (last_out) and read the previous correction for the Anderson dots
     const double np0 = (double)bs->off[0][bs->nbox];
     const double st  = sizeof(T);
+    const double hc[3] = { 2. * h[0], 2. * h[1], 2. * h[2] };
     // the last pre-sweep leaves the stale residual w for the restriction, or
     // restricts it itself (MGX_TUNE_RESTRICT choice 1, where it applies)
     T* rcoarse    = level < max_levels ? B.rc[level] : nullptr;
@@ -3326,12 +3343,14 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
         T* w          = (it == ncycl - 1 && level < max_levels) ? B.w[level] : nullptr;
         T* rfuse      = (w && bs->tune[MGX_TUNE_RESTRICT] == 1) ? rcoarse : nullptr;
         const int tok = level == 0 ? prof_begin(ctx, MGX_PROF_JACOBI0, (w ? 4. : 3.) * st * np0) : -1;
+        // fused, the coarse level's zero-start first sweep is written too
         if ((rc = box_jacobi_t(ctx, bs, level, type, jf, (const T*)a, f, b, w, nullptr, nullptr, rfuse,
-                 &restricted)))
+                 &restricted, (const T*)nullptr, rfuse ? B.tmp[level + 1] : (T*)nullptr,
+                 jacobi_factor(MGX_LAP_2ND, hc))))
             return rc;
         prof_end(ctx, tok);
-        // fused: x and f read, x written, the coarse residual written
-        if (restricted) prof_retag(ctx, tok, MGX_PROF_JACOBI0, (3. + 1. / 8.) * st * np0);
+        // fused: x and f read, x written, the coarse residual and start written
+        if (restricted) prof_retag(ctx, tok, MGX_PROF_JACOBI0, (3. + 2. / 8.) * st * np0);
         std::swap(a, b);
     }
     if (level == max_levels) return MGX_OK;
@@ -3343,12 +3362,15 @@ int box_mg_rec(mgx_ctx* ctx, const mgx_boxset* bs, BoxMG<T>& B, int max_levels, 
         if ((rc = box_restrict_t(ctx, bs, level, (const T*)B.w[level], rcoarse))) return rc;
         prof_end(ctx, tok);
     }
-    const int64_t nc = bs->off[level + 1][bs->nbox];
-    // the coarse level starts from 0: its first sweep is pointwise
-    double hc[3] = { 2. * h[0], 2. * h[1], 2. * h[2] };
-    hipLaunchKernelGGL(k_first_sweep<T>, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream, (const double*)nullptr,
-        rcoarse, B.tmp[level + 1], nc, jacobi_factor(MGX_LAP_2ND, hc));
-    MGX_CHECK_LAUNCH(ctx);
+    // the coarse level starts from 0: its first sweep is pointwise (written
+    // by the fused restriction, else here)
+    if (!restricted)
+    {
+        const int64_t nc = bs->off[level + 1][bs->nbox];
+        hipLaunchKernelGGL(k_first_sweep<T>, dim3(nblocks(nc)), dim3(kBlock), 0, ctx->stream,
+            (const double*)nullptr, rcoarse, B.tmp[level + 1], nc, jacobi_factor(MGX_LAP_2ND, hc));
+        MGX_CHECK_LAUNCH(ctx);
+    }
     if ((rc = box_mg_rec(ctx, bs, B, max_levels, nv, (const T*)rcoarse, MGX_LAP_2ND, level + 1, true))) return rc;
     // the coarse correction prolonged inside the first post-sweep
     // (MGX_TUNE_PROLONG choice 1, where the marching kernel runs this level),
