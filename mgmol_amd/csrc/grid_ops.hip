@@ -184,6 +184,33 @@ __global__ void k_lap(const T* in, T* out, GGeom g, int64_t total, AllCoef c)
     }
 }
 
+// one Mehrstellen Jacobi sweep out of place on the interior: w = -Lap_M x - f
+// (written if w is given), xo = x + mjf w, with the arithmetic of k_lap and
+// k_jacobi_update in order (the same interior values); x's ghosts are read as
+// stored, xo's and w's are not written
+template <typename T>
+__global__ void k_jacobi_mehr(const T* __restrict__ x, const T* __restrict__ f, T* __restrict__ xo,
+    T* __restrict__ w, GGeom g, int64_t total, AllCoef c, double mjf)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x)
+    {
+        const int64_t fn = t / g.size;
+        int r            = (int)(t - fn * g.size);
+        const int px     = r / (g.d1 * g.d2);
+        r -= px * g.d1 * g.d2;
+        const int py    = r / g.d2;
+        const int pz    = r - py * g.d2;
+        const int64_t i = fn * g.sizeg + (int64_t)(px + g.ng) * g.incx + (int64_t)(py + g.ng) * g.incy + pz + g.ng;
+        T wv            = (T)lapM_at(x, i, g.incx, g.incy, c.cm);
+        wv += static_cast<T>(-1. * static_cast<double>(f[i]));
+        if (w) w[i] = wv;
+        T xv = x[i];
+        xv += static_cast<T>(mjf * static_cast<double>(wv));
+        xo[i] = xv;
+    }
+}
+
 // FDkernelRHS_4th_Mehr1, src/pb/FDkernels.cc:523-584
 template <typename T>
 __global__ void k_rhs(const T* v, T* rhs, GGeom g, GGeom go, int64_t total)
@@ -1141,6 +1168,24 @@ int jacobi_sweep(mgx_ctx* ctx, int type, double jf, const T* f, T* x, T* xo, T* 
         if (m < 0) return m;
         if (m)
         {
+            *moved = true;
+            return MGX_OK;
+        }
+        if (type == MGX_LAP_MEHRSTELLEN && gr.ghosts >= 1)
+        {
+            // the Mehrstellen stencil and the update in one pass (after the
+            // halo exchange its ghosts need)
+            int rc = launch_halo(ctx, x, nfunc, gr);
+            if (rc) return rc;
+            const GGeom g = ggeom(gr);
+            AllCoef c;
+            c.c4 = coef4(gr.h);
+            c.c2 = coef2(gr.h);
+            c.cm = coefm(gr.h);
+            const int64_t total = (int64_t)nfunc * g.size;
+            hipLaunchKernelGGL(k_jacobi_mehr<T>, dim3(nblocks(total)), dim3(kBlock), 0, ctx->stream, (const T*)x, f,
+                xo, need_w ? w : (T*)nullptr, g, total, c, -1. * jf);
+            MGX_CHECK_LAUNCH(ctx);
             *moved = true;
             return MGX_OK;
         }
