@@ -53,7 +53,7 @@ C5_POINTS = 2048 * 32 ** 3
 # passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000, both calibrated on
 # streaming kernels of known size), copied from profiles/pmc_traffic_c5.md;
 # keyed by (group, precision of the path's kernel)
-TRAFFIC_C5 = {("jacobi0", "f32"): 839914598, ("jacobi0", "f16"): 420104704, ("cellgram", "split", 1): 1423482880,
+TRAFFIC_C5 = {("jacobi0", "f32"): 855957812, ("jacobi0", "f16"): 428524544, ("cellgram", "split", 1): 1423482880,
               ("cellgram", "split", 2): 1308473754, ("residk", "f32"): 1738115277,
               ("learned", "bf16"): 17287778496, ("skin", "f32"): 1679062938, ("skin", "f64"): 1679062938,
               ("skin", "f16"): 1679062938, ("nsgemm", "f64"): 912851968}
